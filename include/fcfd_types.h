@@ -1,0 +1,114 @@
+/*
+ * fcfd_types.h — plain-C types shared by the MI355X library (libfcfd) and the
+ * CPU oracle: grid descriptor, physical parameters, field identifiers, status
+ * codes and diagnostics.
+ *
+ * Data layout contract (identical to the Fortran reference so host arrays map
+ * 1:1, `Two phase flows/newOpenMP/2D/lib/data/branches.f90:64-71`):
+ *   every field is a real(8) array over (1-g:nx+g, 1-g:ny+g[, 1-g:nz+g]),
+ *   column-major (i fastest), g = ghc = 3.  In 2D nz = 1 and there is no z
+ *   ghost layer, so a 2D field holds (nx+2g)*(ny+2g) values.
+ *   MAC staggering: u(i) is the east face of cell i, v(j) the north face,
+ *   w(k) the top face; phi, p, rho, mu are cell centred
+ *   (`Two phase flows/newOpenMP/2D/src/ns_solver/ns_solvers.f90:125`).
+ */
+#ifndef FCFD_TYPES_H
+#define FCFD_TYPES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes returned by every entry point. */
+enum fcfd_status {
+    FCFD_OK = 0,
+    FCFD_ERR_ARG = 1,            /* bad argument / size / null pointer        */
+    FCFD_ERR_HIP = 2,            /* HIP runtime error                          */
+    FCFD_ERR_NOT_CONVERGED = 3,  /* iteration cap reached before tolerance     */
+    FCFD_ERR_NAN = 4,            /* NaN or Inf met in a reduction              */
+    FCFD_ERR_FIELD = 5,          /* field id not held by this context          */
+    FCFD_ERR_SINGULAR = 6        /* zero pivot / zero denominator              */
+};
+
+/* Grid descriptor (the reference's p%glb node_x.., xstart.., ghc). */
+typedef struct fcfd_grid {
+    int32_t dim;                 /* 2 or 3                                     */
+    int32_t nx, ny, nz;          /* interior cells; nz = 1 in 2D               */
+    int32_t ghc;                 /* ghost layers; must be 3                    */
+    int32_t pad_;
+    double xstart, xend, ystart, yend, zstart, zend;
+} fcfd_grid;
+
+/*
+ * Physical and numerical parameters after the reference's derivation step
+ * (`.../3D+IBM/lib/data/tree.f90:165-201`): dt, rdt and ls_wid are already
+ * multiplied by dx; tolerances are absolute values, not exponents.
+ */
+typedef struct fcfd_params {
+    double dt;                   /* time step                                  */
+    double rdt;                  /* reinitialisation pseudo time step          */
+    double ls_wid;               /* interface half width (1.5 dx)              */
+    double rho_12, mu_12;        /* density / viscosity ratio fluid2/fluid1    */
+    double re, we, fr;           /* Reynolds, Weber, Froude numbers            */
+    double gx, gy, gz;           /* unit gravity direction                     */
+    double btn_sf, btn_g;        /* surface tension / gravity switches (0|1)   */
+    double imass;                /* reference mass for MPLS (initial mass)     */
+    double ppe_tol;              /* PPE relative residual target (1e-10)       */
+    double p_w1;                 /* reference relaxation weight (oracle only)  */
+    double p_tol;                /* reference max-update tolerance (oracle)    */
+    int32_t ubc[2], vbc[2], wbc[2];  /* 1 = no-slip, 2 = slip (bcs.f90)        */
+    int32_t redis_interval;      /* reinitialise when iter % interval == 0     */
+    int32_t picard_iters;        /* outer Picard iterations per step (4)       */
+    int32_t ppe_precision;       /* FCFD_PPE_FP64 or FCFD_PPE_MIXED            */
+    int32_t ppe_max_cycles;      /* cap on V-cycles per PPE solve              */
+    int32_t pad_;
+} fcfd_params;
+
+enum fcfd_ppe_precision {
+    FCFD_PPE_FP64 = 0,           /* FP64 smoothing/transfer, FP64 everywhere   */
+    FCFD_PPE_MIXED = 1           /* FP32 V-cycles inside FP64 defect correction*/
+};
+
+/*
+ * Field identifiers.  Names follow the reference's p%loc members
+ * (`.../3D+IBM/lib/data/branches.f90:40-66`): <member>_<time level>.
+ */
+enum fcfd_field {
+    FCFD_PHI = 0,        /* phi%now                                            */
+    FCFD_PHI_OLD,        /* phi%old  (phi at the start of the step)            */
+    FCFD_U, FCFD_V, FCFD_W,                 /* vel%{x,y,z}%now (MAC faces)     */
+    FCFD_U_OLD, FCFD_V_OLD, FCFD_W_OLD,     /* vel%{x,y,z}%old                 */
+    FCFD_NU, FCFD_NV, FCFD_NW,              /* nvel%{x,y,z} (cell centred)     */
+    FCFD_SRC_U, FCFD_SRC_V, FCFD_SRC_W,     /* velsrc%{x,y,z}%now              */
+    FCFD_SRC_U_OLD, FCFD_SRC_V_OLD, FCFD_SRC_W_OLD, /* velsrc%{x,y,z}%old      */
+    FCFD_P,              /* p%now                                              */
+    FCFD_HEAVY, FCFD_DELTA, FCFD_SIGN,      /* ls_funs outputs                 */
+    FCFD_RHO, FCFD_MU,   /* rho_mu outputs                                     */
+    FCFD_GRAD,           /* grad%now                                           */
+    FCFD_NX, FCFD_NY, FCFD_NZ,              /* normals%{x,y,z}%now             */
+    FCFD_NXX, FCFD_NYY, FCFD_NZZ,           /* normals%{xx,yy,zz}%now          */
+    FCFD_NXY, FCFD_NXZ, FCFD_NYZ,           /* normals%{xy,xz,yz}%now          */
+    FCFD_CURV,           /* normals%curv%now                                   */
+    FCFD_PPE_SRC,        /* coe%src (div u* / dt)                              */
+    FCFD_NFIELDS
+};
+
+/* Diagnostics (the reference's p%glb scalars printed by output.f90). */
+typedef struct fcfd_diag {
+    double mass, vol;            /* ls_mv sums                                 */
+    double imass;                /* MPLS target                                */
+    double ppe_rel_resid;        /* last PPE relative residual ||r||/||f||     */
+    double vel_div, vel_sdiv;    /* max / mean |div u|                         */
+    double mpls_lambda;          /* last MPLS Newton multiplier                */
+    int32_t ppe_cycles;          /* V-cycles (or outer corrections) last solve */
+    int32_t ppe_cycles_step;     /* summed over the Picard loop of last step   */
+    int32_t redis_iters;         /* pseudo steps of the last reinitialisation  */
+    int32_t iter;                /* time step counter (p%glb%iter)             */
+} fcfd_diag;
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FCFD_TYPES_H */

@@ -1,0 +1,42 @@
+#!/usr/bin/env bash
+# Build the compiled Fortran reference harness into oracle/_ref/ (test
+# infrastructure only).  The reference sources are compiled where they lie
+# under the reference checkout; nothing is copied.  Needs AMD flang
+# (/opt/rocm/lib/llvm/bin/flang).  Outputs: oracle/_ref/ref2d, oracle/_ref/ref3d.
+#
+# usage: oracle/build_ref.sh [REFERENCE_ROOT]   (default /root/reference)
+set -euo pipefail
+here="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+ref_root="${1:-/root/reference}"
+FC="${FC:-/opt/rocm/lib/llvm/bin/flang}"
+out="$here/_ref"
+if [ ! -d "$ref_root/Two phase flows/newOpenMP" ]; then
+    echo "build_ref: reference checkout not found at $ref_root; skipping" >&2
+    exit 0
+fi
+if [ ! -x "$FC" ]; then
+    echo "build_ref: flang not found at $FC; skipping" >&2
+    exit 0
+fi
+mkdir -p "$out"
+# reference build flags (N2/makefile:1-6) translated to flang; -O2 without
+# -march so no FMA contraction happens on x86-64.
+FFLAGS="-O2 -fopenmp"
+for d in 2D 3D; do
+    if [ "$d" = 2D ]; then src="$ref_root/Two phase flows/newOpenMP/2D"; def=""; exe=ref2d;
+    else src="$ref_root/Two phase flows/newOpenMP/3D+IBM"; def="-DHARNESS_3D"; exe=ref3d; fi
+    b="$out/build_$d"
+    mkdir -p "$b/mod"
+    "$FC" $FFLAGS -c -module-dir "$b/mod" -I"$src/lib" "$src/lib/modules.f90" -o "$b/modules.o"
+    objs=("$b/modules.o")
+    # the vendored VTK writer is output-only and slow to compile: reuse its
+    # objects when they are already built
+    for f in ir_precision lib_pack_data lib_base64 lib_vtk_io; do
+        if [ ! -f "$b/$f.o" ] || [ ! -f "$b/mod/$f.mod" ]; then
+            "$FC" -cpp -O1 -c -module-dir "$b/mod" "$src/lib/plot/$f.f90" -o "$b/$f.o"
+        fi
+        objs+=("$b/$f.o")
+    done
+    "$FC" $FFLAGS -cpp $def -I"$b/mod" -I"$src/src" "$here/ref_harness.F90" "${objs[@]}" -o "$out/$exe"
+    echo "build_ref: built $out/$exe"
+done
