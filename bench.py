@@ -1,0 +1,240 @@
+"""Benchmark of the two-phase step on one MI355X.
+
+Workload (default): 3D rising bubble on [0,1]^2 x [0,2] with 256^3 cells
+(dx = dy = 1/256, dz = 2/256), sphere r = 0.2 at (0.5, 0.5, 0.5), density
+ratio 1000, viscosity ratio 100, CSF surface tension, gravity -z, no-slip
+walls, dt = 0.1 dx, reinitialisation every 5 steps, 4 Picard iterations per
+step, each with a multigrid pressure solve to ||r||/||f|| <= 1e-10 (FP32
+V-cycles inside FP64 defect correction).  Synthetic initial condition (signed
+distance, fluid at rest); W untimed warm-up steps, then K timed steps.
+
+Prints one JSON line: Mcell-steps/s, the roofline of the dominant kernel
+(algorithmic bytes per launch / mean launch time from HIP events recorded on
+the solver's stream), the FP64-vs-mixed multigrid time to 1e-10 on the last
+step's pressure system, and a CPU baseline (the oracle, on a bounded sample).
+
+`--poisson N` instead times the standalone variable-coefficient Poisson solve
+on N^3 (rho = 1000 outside / 1 inside a sphere r = 0.3, smoothed over 1.5
+cells; seeded U[-1,1] rhs with zero mean; Neumann walls).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, GB/s
+
+
+def rising_case(n: int, precision: int, tol: float):
+    from fortrancfd_amd.types import Case
+    return Case(dim=3, nx=n, ny=n, nz=n, xend=1.0, yend=1.0, zend=2.0, gvec=(0.0, 0.0, -1.0),
+                cfl=0.1, cfl_r=0.1, ppe_tol=tol, ppe_precision=precision, redis_interval=5,
+                picard_iters=4, ppe_max_cycles=100, name="rising_bubble_3d")
+
+
+def region_bytes(case, mixed: bool) -> dict:
+    """Algorithmic bytes per launch of each kernel region: every distinct
+    array a kernel reads or writes counted once per interior cell (ghosts,
+    halo re-reads and reduction partials excluded), 8 B per FP64 value,
+    4 B per FP32 value."""
+    N = case.nx * case.ny * case.nz
+    b = 4 if mixed else 8
+    nchild0 = 4 if case.dz > 1.5 * case.dx else 8   # first coarsening: x,y only when dz = 2 dx
+    return {
+        "ls_adv_stage": 8 * N * 20 / 3,        # R{phi,nu,nv,nw,l1..} W{phi',l} averaged over 3 stages
+        "redis_pass_a": 8 * N * 5,             # R{phi,S0} W{grad,s2,s3}
+        "redis_pass_b": 8 * N * 8,             # R{phi,S0,grad,s2,s3,l1,l2,phi0} W{phi,l} averaged over stages
+        "ccd_lines": 8 * N * 3 / 3,            # per direction launch: R{f} W{fx,fxx} on its lines
+        "curv": 8 * N * 10,
+        "diff_source": 8 * N * 11,
+        "predictor": 8 * N * 18,
+        "ppe_src": 8 * N * 4,
+        "projection": 8 * N * 8,
+        "node_vel": 8 * N * 6,
+        "mpls_update": 8 * N * 3,
+        "mg_smooth_l0": b * N * (1 + 0.5 + 1 + 0.5),     # R{e, f/2, rho} W{e/2} per colour
+        "mg_restrict_l0": b * N * (3 + 2.0 / nchild0),   # R{e,f,rho} W{fc,ec}
+        "mg_prolong_l0": b * N * (2 + 1.0 / nchild0),    # R{e,ec} W{e}
+        "mg_resid_l0": N * (3 * 8 + (2 * 4 if mixed else 0)),  # R{p,f,rho} (W{r32,e32})
+    }
+
+
+def cpu_baseline(steps: int = 5, n: int = 96, threads: int = 0) -> dict:
+    """The oracle (CPU restatement of the reference step, OpenMP) on a bounded
+    sample: the same rising-bubble case on n^3 cells, init + `steps` steps.
+    Its pressure solve is a converged Jacobi-PCG to the same 1e-10 relative
+    residual, because the reference's relaxation cannot reach 1e-10."""
+    from oracle.oracle import Oracle
+    from fortrancfd_amd.types import bc_copy, bubble_phi
+    th = threads or min(16, os.cpu_count() or 1)
+    used = Oracle.threads(th)
+    case = rising_case(n, 0, 1e-10)
+    o = Oracle(case)
+    o.set("phi", bc_copy(case, bubble_phi(case, 0.2, (0.5, 0.5, 0.5))))
+    o.call("problem_init", 0)
+    o.call("step", 1)   # first step from p = 0: untimed
+    t = time.perf_counter()
+    for _ in range(steps):
+        o.call("step", 1)
+    dt = time.perf_counter() - t
+    return {"value": round(case.ncells * steps / dt / 1e6, 4), "unit": "Mcell-steps/s", "cores": used,
+            "kind": "port",
+            "sample": f"oracle step (reference operation order, Jacobi-PCG pressure solve to 1e-10) on the "
+                      f"same rising-bubble case at {n}x{n}x{n}, {steps} timed steps after 1 untimed step, "
+                      f"{used} OpenMP threads, {dt:.1f} s"}
+
+
+def mg_compare(s, case) -> dict:
+    """FP64 vs mixed multigrid to 1e-10 on the last step's pressure system
+    (operator of rho now, rhs = last Picard source), both from p = 0."""
+    from fortrancfd_amd.types import PPE_FP64, PPE_MIXED
+    out = {}
+    zeros = np.zeros(case.size)
+    sols = {}
+    for name, prec in (("fp64", PPE_FP64), ("mixed", PPE_MIXED)):
+        best = None
+        for rep in range(2):
+            s.push("p", zeros)
+            s.sync()
+            t = time.perf_counter()
+            cyc, rel = s.ppe_solve_rhs(1e-10, prec)
+            s.sync()
+            ms = (time.perf_counter() - t) * 1e3
+            best = ms if best is None else min(best, ms)
+        sols[name] = case.interior(s.pull("p")).copy()
+        out[name] = {"ms": round(best, 3), "cycles": cyc, "rel_resid": rel}
+    a, b = sols["fp64"], sols["mixed"]
+    out["speedup"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    out["p_rel_diff"] = float(np.abs(a - b).max() / max(np.abs(a).max(), 1e-300))
+    return out
+
+
+def poisson_bench(n: int) -> dict:
+    from fortrancfd_amd import Solver
+    from fortrancfd_amd.types import PPE_FP64, PPE_MIXED, Case
+    case = Case(dim=3, nx=n, ny=n, nz=n, xend=1.0, yend=1.0, zend=1.0, ppe_max_cycles=200,
+                name="poisson")
+    s = Solver(case, allow_unconverged=True)
+    X, Y, Z = case.coords()
+    r = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2)
+    eps = 1.5 * case.dx
+    x = (r - 0.3) / eps
+    H = np.where(x > 1, 1.0, np.where(x < -1, 0.0, 0.5 * (1 + x + np.sin(np.pi * x) / np.pi)))
+    rho = 1.0 + 999.0 * H
+    del X, Y, Z, r, x, H
+    s.push("rho", rho)
+    del rho
+    rng = np.random.default_rng(0x5EEDF00D)
+    f = rng.uniform(-1.0, 1.0, size=case.shape)
+    fi = case.interior(f)
+    fi -= fi.mean()
+    s.push("ppe_src", f)
+    del f
+    s.ppe_sor_init()
+    res = mg_compare(s, case)
+    return {"metric": f"MG ms to 1e-10, {n}^3 variable-coefficient Poisson", "value": res["mixed"]["ms"],
+            "unit": "ms", "higher_is_better": False, "n_gpus": 1, "dtype": "f64 (FP32 V-cycles in FP64 "
+            "defect correction)", "data": "synthetic", "config": {"workload": f"poisson_{n}^3"}, "mg": res}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mg", action="store_true")
+    ap.add_argument("--poisson", type=int, default=0)
+    ap.add_argument("--regions-out", default="")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU decomposition "
+              "is a design note, not a measured path.", file=sys.stderr)
+        return 2
+    if args.poisson:
+        print(json.dumps(poisson_bench(args.poisson)))
+        return 0
+
+    from fortrancfd_amd import Solver
+    from fortrancfd_amd.types import PPE_FP64, PPE_MIXED, bc_copy, bubble_phi
+    mixed = args.precision == "mixed"
+    case = rising_case(args.n, PPE_MIXED if mixed else PPE_FP64, 1e-10)
+    s = Solver(case, allow_unconverged=True)
+    s.push("phi", bc_copy(case, bubble_phi(case, 0.2, (0.5, 0.5, 0.5))))
+    s.problem_init(False)
+    for w in range(args.warmup):
+        s.step()
+        print(f"warmup step {w + 1}/{args.warmup} {s.diag()}", file=sys.stderr, flush=True)
+    s.sync()
+    s.prof_enable(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s.step()
+    s.sync()
+    t1 = time.perf_counter()
+    prof = s.prof_read()
+    s.prof_enable(False)
+    d = s.diag()
+    print(f"timed {args.steps} steps in {t1 - t0:.3f} s; last diag {d}", file=sys.stderr, flush=True)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = case.ncells * args.steps / (t1 - t0) / 1e6
+
+    rb = region_bytes(case, mixed)
+    regions = {}
+    tot_ms, tot_bytes = 0.0, 0.0
+    for name, (ms, cnt) in prof.items():
+        if cnt == 0:
+            continue
+        regions[name] = {"ms_total": round(ms, 3), "launches": cnt, "ms_per_launch": round(ms / cnt, 5)}
+        tot_ms += ms
+        if name in rb:
+            gbs = rb[name] / (ms / cnt * 1e-3) / 1e9
+            regions[name]["GB/s"] = round(gbs, 1)
+            tot_bytes += rb[name] * cnt
+    dom = max(regions, key=lambda k: regions[k]["ms_total"])
+    dom_r = regions[dom]
+    achieved = dom_r.get("GB/s")
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
+                "bytes_per_launch": rb.get(dom), "ms_per_launch": dom_r["ms_per_launch"],
+                "share_of_step": round(dom_r["ms_total"] / (ms_step * args.steps), 4)}
+    out = {
+        "metric": "Mcell-steps/s @256^3 two-phase, %HBM peak; MG ms to 1e-10 mixed vs FP64",
+        "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"rising_bubble_3d_{case.nx}x{case.ny}x{case.nz}", "grid": [case.nx, case.ny, case.nz],
+                   "domain": [1.0, 1.0, 2.0], "ppe": "fp32 V(2,2) in fp64 defect correction" if mixed else "fp64 V(2,2)",
+                   "ppe_tol": 1e-10, "picard": 4, "redis_interval": 5},
+        "roofline": roofline,
+        "step_hbm_frac_measured_kernels": round(tot_bytes / (tot_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms_per_step": round(tot_ms / args.steps, 3),
+        "ppe_cycles_last_step": d["ppe_cycles_step"],
+    }
+    if not args.no_mg:
+        out["mg"] = mg_compare(s, case)
+    if args.regions_out:
+        with open(args.regions_out, "w") as fh:
+            json.dump(regions, fh, indent=1)
+    s.close()
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # the oracle is optional at run time
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

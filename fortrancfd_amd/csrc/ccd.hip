@@ -1,0 +1,238 @@
+// ccd.hip — combined compact difference (CCD) derivatives along grid lines
+// and the interface curvature.
+//
+// Reference: `Two phase flows/newOpenMP/3D+IBM/lib/solver/ccd_solvers.f90`
+// (init_ccd :132-159, boundary closures :76-130, sources :287-306, driver
+// :379-399) and `.../lib/solver/matrix_solver.f90` (TWIN_DEC :6-48,
+// TWIN_BKS :50-78); surface_norms / curv in
+// `.../src/interface_solver/others_normals.f90:1-83,133-167`.
+//
+// The 2x2 block-tridiagonal CCD matrix depends only on the line length and
+// spacing, so it is factorised once per direction at context creation
+// (the reference refactorises it for every line of every call).  A solve is
+// then one forward and one backward sweep per line; one GPU thread owns one
+// line.  For y and z lines consecutive threads hold consecutive i, so every
+// sweep step is a coalesced row access; x lines walk the contiguous axis.
+#include <cmath>
+#include <vector>
+
+#include "fcfd_internal.h"
+
+using namespace fcfd;
+
+namespace fcfd {
+
+// coefficient rows stored per direction: [r][N]
+enum { CA1, CB1, CAA1, CBB1, CA2, CB2, CAA2, CBB2, CA3, CB3, CAA3, CBB3, CDEN, NCOEF };
+
+// TWIN_DEC of init_ccd + bc on a line of N points, spacing dx (host)
+static void ccd_factor(int N, double dx, std::vector<double> &out) {
+    std::vector<double> A(3 * N), B(3 * N), AA(3 * N), BB(3 * N);
+    auto M = [&](std::vector<double> &X, int r, int i) -> double & { return X[i * 3 + r - 1]; };
+    const int is = 0, ie = N - 1;
+    for (int i = is + 1; i <= ie - 1; ++i) {
+        M(A, 1, i) = 7.0 / 16.0; M(A, 2, i) = 1.0; M(A, 3, i) = 7.0 / 16.0;
+        M(B, 1, i) = dx / 16.0; M(B, 2, i) = 0.0; M(B, 3, i) = -dx / 16.0;
+        M(AA, 1, i) = -9.0 / 8.0 / dx; M(AA, 2, i) = 0.0; M(AA, 3, i) = 9.0 / 8.0 / dx;
+        M(BB, 1, i) = -1.0 / 8.0; M(BB, 2, i) = 1.0; M(BB, 3, i) = -1.0 / 8.0;
+    }
+    M(A, 2, is) = 1.0; M(B, 2, is) = 0.0; M(AA, 2, is) = 0.0; M(BB, 2, is) = 1.0;
+    M(A, 3, is) = 2.0; M(B, 3, is) = -dx; M(AA, 3, is) = -2.5 / dx; M(BB, 3, is) = 8.5;
+    M(A, 2, ie) = 1.0; M(B, 2, ie) = 0.0; M(AA, 2, ie) = 0.0; M(BB, 2, ie) = 1.0;
+    M(A, 1, ie) = 2.0; M(B, 1, ie) = dx; M(AA, 1, ie) = 2.5 / dx; M(BB, 1, ie) = 8.5;
+    M(A, 1, is) = 0.0; M(AA, 1, is) = 0.0; M(A, 3, ie) = 0.0; M(AA, 3, ie) = 0.0;
+    M(B, 1, is) = 0.0; M(BB, 1, is) = 0.0; M(B, 3, ie) = 0.0; M(BB, 3, ie) = 0.0;
+    for (int i = is + 1; i <= ie; ++i) {
+        double Den = M(A, 2, i - 1) * M(BB, 2, i - 1) - M(AA, 2, i - 1) * M(B, 2, i - 1);
+        double SC1 = -(M(AA, 2, i - 1) * M(B, 1, i) - M(A, 1, i) * M(BB, 2, i - 1)) / Den;
+        double SC2 = (M(B, 1, i) * M(A, 2, i - 1) - M(B, 2, i - 1) * M(A, 1, i)) / Den;
+        M(A, 1, i) = SC1;
+        M(A, 2, i) = M(A, 2, i) - (SC1 * M(A, 3, i - 1) + SC2 * M(AA, 3, i - 1));
+        M(B, 1, i) = SC2;
+        M(B, 2, i) = M(B, 2, i) - (SC1 * M(B, 3, i - 1) + SC2 * M(BB, 3, i - 1));
+        SC1 = -(M(AA, 2, i - 1) * M(BB, 1, i) - M(AA, 1, i) * M(BB, 2, i - 1)) / Den;
+        SC2 = (M(BB, 1, i) * M(A, 2, i - 1) - M(B, 2, i - 1) * M(AA, 1, i)) / Den;
+        M(AA, 1, i) = SC1;
+        M(AA, 2, i) = M(AA, 2, i) - (SC1 * M(A, 3, i - 1) + SC2 * M(AA, 3, i - 1));
+        M(BB, 1, i) = SC2;
+        M(BB, 2, i) = M(BB, 2, i) - (SC1 * M(B, 3, i - 1) + SC2 * M(BB, 3, i - 1));
+    }
+    out.assign((size_t)NCOEF * N, 0.0);
+    for (int i = 0; i < N; ++i) {
+        out[CA1 * N + i] = M(A, 1, i); out[CB1 * N + i] = M(B, 1, i);
+        out[CAA1 * N + i] = M(AA, 1, i); out[CBB1 * N + i] = M(BB, 1, i);
+        out[CA2 * N + i] = M(A, 2, i); out[CB2 * N + i] = M(B, 2, i);
+        out[CAA2 * N + i] = M(AA, 2, i); out[CBB2 * N + i] = M(BB, 2, i);
+        out[CA3 * N + i] = M(A, 3, i); out[CB3 * N + i] = M(B, 3, i);
+        out[CAA3 * N + i] = M(AA, 3, i); out[CBB3 * N + i] = M(BB, 3, i);
+        out[CDEN * N + i] = M(A, 2, i) * M(BB, 2, i) - M(AA, 2, i) * M(B, 2, i);
+    }
+}
+
+int ccd_setup(fcfd_ctx *c) {
+    const Geom &G = c->G;
+    const int n[3] = {G.nx, G.ny, G.nz};
+    const double h[3] = {G.dx, G.dy, G.dz};
+    for (int d = 0; d < 3; ++d) {
+        c->ccd_coef[d] = nullptr;
+        if (d >= G.dim) continue;
+        int N = n[d] + 2 * G.g;
+        std::vector<double> co;
+        ccd_factor(N, h[d], co);
+        FCFD_CHECK(hipMalloc((void **)&c->ccd_coef[d], sizeof(double) * co.size()));
+        FCFD_CHECK(hipMemcpy(c->ccd_coef[d], co.data(), sizeof(double) * co.size(), hipMemcpyHostToDevice));
+    }
+    return FCFD_OK;
+}
+
+// asgn_src_ccd + TWIN_BKS along one line; ghost line ends included
+__global__ void __launch_bounds__(256) k_ccd_lines(const double *__restrict__ f, double *__restrict__ fx,
+                                                   double *__restrict__ fxx, const double *__restrict__ co,
+                                                   Geom G, int dir) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int a, b, N;
+    long base, st;
+    double dx;
+    if (dir == 0) {
+        if (t >= (long)G.ny * G.nz) return;
+        a = (int)(t % G.ny); b = (int)(t / G.ny);
+        base = G.id(-G.g, a, b); st = 1; N = G.nx + 2 * G.g; dx = G.dx;
+    } else if (dir == 1) {
+        if (t >= (long)G.nx * G.nz) return;
+        a = (int)(t % G.nx); b = (int)(t / G.nx);
+        base = G.id(a, -G.g, b); st = G.sx; N = G.ny + 2 * G.g; dx = G.dy;
+    } else {
+        if (t >= (long)G.nx * G.ny) return;
+        a = (int)(t % G.nx); b = (int)(t / G.nx);
+        base = G.id(a, b, -G.g); st = G.sxy; N = G.nz + 2 * G.g; dx = G.dz;
+    }
+    const double *F = f + base;
+    double *X = fx + base, *XX = fxx + base;
+    const double dx2 = dx * dx;
+    const double *A1 = co + CA1 * N, *B1 = co + CB1 * N, *AA1 = co + CAA1 * N, *BB1 = co + CBB1 * N;
+    const double *A2 = co + CA2 * N, *B2 = co + CB2 * N, *AA2 = co + CAA2 * N, *BB2 = co + CBB2 * N;
+    const double *A3 = co + CA3 * N, *B3 = co + CB3 * N, *AA3 = co + CAA3 * N, *BB3 = co + CBB3 * N;
+    const double *DEN = co + CDEN * N;
+    // forward elimination with the source of asgn_src_ccd
+    double f0 = F[0], f1 = F[st], f2 = F[2 * st], f3 = F[3 * st];
+    double S = (-3.5 * f0 + 4.0 * f1 - 0.5 * f2) / dx;
+    double SS = (34.0 / 3.0 * f0 - 83.0 / 4.0 * f1 + 10.0 * f2 - 7.0 / 12.0 * f3) / dx2;
+    X[0] = S;
+    XX[0] = SS;
+    double fm = f0, fc = f1;  // f(i-1), f(i)
+    for (int i = 1; i < N - 1; ++i) {
+        double fp = F[(long)(i + 1) * st];
+        double s = 15.0 / 16.0 * (-fm + fp) / dx;
+        double ss = (3.0 * fm - 6.0 * fc + 3.0 * fp) / dx2;
+        double Sn = s - (A1[i] * S + B1[i] * SS);
+        double SSn = ss - (AA1[i] * S + BB1[i] * SS);
+        S = Sn; SS = SSn;
+        X[(long)i * st] = S;
+        XX[(long)i * st] = SS;
+        fm = fc; fc = fp;
+    }
+    {
+        const int ie = N - 1;
+        double e0 = F[(long)ie * st], e1 = F[(long)(ie - 1) * st], e2 = F[(long)(ie - 2) * st], e3 = F[(long)(ie - 3) * st];
+        double s = -(-3.5 * e0 + 4.0 * e1 - 0.5 * e2) / dx;
+        double ss = (34.0 / 3.0 * e0 - 83.0 / 4.0 * e1 + 10.0 * e2 - 7.0 / 12.0 * e3) / dx2;
+        double Sn = s - (A1[ie] * S + B1[ie] * SS);
+        double SSn = ss - (AA1[ie] * S + BB1[ie] * SS);
+        double Den = DEN[ie];
+        double SolS = -(B2[ie] * SSn - BB2[ie] * Sn) / Den;
+        double SolSS = (A2[ie] * SSn - AA2[ie] * Sn) / Den;
+        X[(long)ie * st] = SolS;
+        XX[(long)ie * st] = SolSS;
+        // back substitution
+        for (int i = ie - 1; i >= 0; --i) {
+            double Si = X[(long)i * st] - (A3[i] * SolS + B3[i] * SolSS);
+            double SSi = XX[(long)i * st] - (AA3[i] * SolS + BB3[i] * SolSS);
+            double D = DEN[i];
+            SolS = -(B2[i] * SSi - BB2[i] * Si) / D;
+            SolSS = (A2[i] * SSi - AA2[i] * Si) / D;
+            X[(long)i * st] = SolS;
+            XX[(long)i * st] = SolSS;
+        }
+    }
+}
+
+int ccd_dir(fcfd_ctx *c, int dir, const double *f, double *fx, double *fxx) {
+    const Geom &G = c->G;
+    long nl = dir == 0 ? (long)G.ny * G.nz : (dir == 1 ? (long)G.nx * G.nz : (long)G.nx * G.ny);
+    int blk = 64;
+    ProfScope ps_(c, R_CCD);
+    hipLaunchKernelGGL(k_ccd_lines, dim3((unsigned)((nl + blk - 1) / blk)), dim3(blk), 0, c->stream, f, fx, fxx,
+                       c->ccd_coef[dir], G, dir);
+    FCFD_CHECK(hipGetLastError());
+    return FCFD_OK;
+}
+
+// surface_norms: the reference solves every line of the ghosted box and then
+// ghost-copies the results.  Lines through ghost cells of phi repeat interior
+// lines (phi's ghosts are copies), so only interior lines are solved here;
+// the first derivatives are ghost-copied before the cross passes read them.
+int surface_norms(fcfd_ctx *c, const double *phi, int full) {
+    const Geom &G = c->G;
+    double **F = c->f;
+    FCFD_TRY(ccd_dir(c, 0, phi, F[FCFD_NX], F[FCFD_NXX]));
+    FCFD_TRY(ccd_dir(c, 1, phi, F[FCFD_NY], F[FCFD_NYY]));
+    if (G.dim == 3) FCFD_TRY(ccd_dir(c, 2, phi, F[FCFD_NZ], F[FCFD_NZZ]));
+    FCFD_TRY(bc_scalar(c, F[FCFD_NX]));
+    FCFD_TRY(bc_scalar(c, F[FCFD_NY]));
+    if (G.dim == 3) FCFD_TRY(bc_scalar(c, F[FCFD_NZ]));
+    if (!full) return FCFD_OK;
+    FCFD_TRY(ccd_dir(c, 1, F[FCFD_NX], F[FCFD_NXY], c->cx));
+    if (G.dim == 3) {
+        FCFD_TRY(ccd_dir(c, 2, F[FCFD_NX], F[FCFD_NXZ], c->cx));
+        FCFD_TRY(ccd_dir(c, 2, F[FCFD_NY], F[FCFD_NYZ], c->cx));
+    }
+    FCFD_TRY(bc_scalar(c, F[FCFD_NXX]));
+    FCFD_TRY(bc_scalar(c, F[FCFD_NYY]));
+    FCFD_TRY(bc_scalar(c, F[FCFD_NXY]));
+    if (G.dim == 3) {
+        FCFD_TRY(bc_scalar(c, F[FCFD_NZZ]));
+        FCFD_TRY(bc_scalar(c, F[FCFD_NXZ]));
+        FCFD_TRY(bc_scalar(c, F[FCFD_NYZ]));
+    }
+    return FCFD_OK;
+}
+
+// curvature from the CCD derivatives (others_normals.f90:145-165 in 3D,
+// 2D :244-260)
+template <int DIM>
+__global__ void k_curv(const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
+                       const double *__restrict__ XX, const double *__restrict__ YY, const double *__restrict__ ZZ,
+                       const double *__restrict__ XY, const double *__restrict__ XZ, const double *__restrict__ YZ,
+                       double *__restrict__ C, Geom G) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= G.nx || j >= G.ny) return;
+    long id = G.id(i, j, k);
+    double fx = X[id], fy = Y[id], fxx = XX[id], fyy = YY[id], fxy = XY[id];
+    if (DIM == 3) {
+        double fz = Z[id], fzz = ZZ[id], fxz = XZ[id], fyz = YZ[id];
+        C[id] = ((fyy + fzz) * (fx * fx) + (fxx + fzz) * (fy * fy) + (fxx + fyy) * (fz * fz) -
+                 2.0 * (fxy * fx * fy + fxz * fx * fz + fyz * fy * fz)) /
+                pow(fx * fx + fy * fy + fz * fz + 1.0e-12, 1.5);
+    } else {
+        C[id] = (fyy * (fx * fx) + fxx * (fy * fy) - 2.0 * fxy * fx * fy) / pow(fx * fx + fy * fy + 1.0e-12, 1.5);
+    }
+}
+
+int curv_kernel(fcfd_ctx *c) {
+    const Geom &G = c->G;
+    double **F = c->f;
+    dim3 blk(64, 4), grd = grid_interior(G, blk);
+    {
+        ProfScope ps_(c, R_CURV);
+        if (G.dim == 3)
+            hipLaunchKernelGGL(k_curv<3>, grd, blk, 0, c->stream, F[FCFD_NX], F[FCFD_NY], F[FCFD_NZ], F[FCFD_NXX],
+                               F[FCFD_NYY], F[FCFD_NZZ], F[FCFD_NXY], F[FCFD_NXZ], F[FCFD_NYZ], F[FCFD_CURV], G);
+        else
+            hipLaunchKernelGGL(k_curv<2>, grd, blk, 0, c->stream, F[FCFD_NX], F[FCFD_NY], nullptr, F[FCFD_NXX],
+                               F[FCFD_NYY], nullptr, F[FCFD_NXY], nullptr, nullptr, F[FCFD_CURV], G);
+        FCFD_CHECK(hipGetLastError());
+    }
+    return bc_scalar(c, F[FCFD_CURV]);
+}
+
+}  // namespace fcfd

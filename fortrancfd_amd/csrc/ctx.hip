@@ -1,0 +1,345 @@
+// ctx.hip — context lifetime, field transfer, ghost-cell boundary conditions
+// and deterministic reductions.
+//
+// Boundary rules follow `Two phase flows/newOpenMP/3D+IBM/src/others/bcs.f90`
+// (2D: `.../2D/src/others/bcs.f90`).
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+
+#include "fcfd_internal.h"
+
+using namespace fcfd;
+
+namespace fcfd {
+
+// ---------------------------------------------------------------- bc(dat)
+// Zero-gradient copy of all ghost layers, x then y then z (bcs.f90:113-154).
+// Because every pass copies along one axis over the full extent of the other
+// axes, the final value of a ghost cell is the value of the cell whose
+// indices are clamped to the interior on every axis.  One thread per ghost
+// cell; the ghost shell is enumerated as z slabs, then y slabs, then x slabs.
+template <int DIM>
+__global__ void k_bc_scalar(double *__restrict__ d, Geom G, long nzslab, long nyslab, long total) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int g = G.g, X = G.nx + 2 * g, Y = G.ny + 2 * g;
+    int i, j, k;
+    if (DIM == 3 && t < nzslab) {                    // k in ghost layers, all i, j
+        long per = (long)X * Y;
+        int layer = (int)(t / per);
+        long r = t % per;
+        j = (int)(r / X) - g;
+        i = (int)(r % X) - g;
+        k = layer < G.gz ? layer - G.gz : G.nz + (layer - G.gz);
+    } else if (t < nzslab + nyslab) {                // j in ghost layers, interior k, all i
+        long u = t - nzslab;
+        long per = (long)X * 2 * g;
+        k = (int)(u / per);
+        long r = u % per;
+        int layer = (int)(r / X);
+        i = (int)(r % X) - g;
+        j = layer < g ? layer - g : G.ny + (layer - g);
+    } else {                                         // i in ghost layers, interior j, k
+        long u = t - nzslab - nyslab;
+        long per = (long)G.ny * 2 * g;
+        k = (int)(u / per);
+        long r = u % per;
+        j = (int)(r / (2 * g));
+        int layer = (int)(r % (2 * g));
+        i = layer < g ? layer - g : G.nx + (layer - g);
+    }
+    int ci = clampi(i, 0, G.nx - 1), cj = clampi(j, 0, G.ny - 1), ck = DIM == 3 ? clampi(k, 0, G.nz - 1) : 0;
+    d[G.id(i, j, k)] = d[G.id(ci, cj, ck)];
+}
+
+int bc_scalar(fcfd_ctx *c, double *d) {
+    const Geom &G = c->G;
+    const int g = G.g;
+    long X = G.nx + 2 * g, Y = G.ny + 2 * g;
+    long nzslab = G.dim == 3 ? X * Y * 2 * G.gz : 0;
+    long nyslab = X * 2 * g * G.nz;
+    long nxslab = (long)2 * g * G.ny * G.nz;
+    long total = nzslab + nyslab + nxslab;
+    int blk = 256;
+    long nb = (total + blk - 1) / blk;
+    ProfScope ps_(c, R_BC);
+    if (G.dim == 3)
+        hipLaunchKernelGGL(k_bc_scalar<3>, dim3((unsigned)nb), dim3(blk), 0, c->stream, d, G, nzslab, nyslab, total);
+    else
+        hipLaunchKernelGGL(k_bc_scalar<2>, dim3((unsigned)nb), dim3(blk), 0, c->stream, d, G, nzslab, nyslab, total);
+    FCFD_CHECK(hipGetLastError());
+    return FCFD_OK;
+}
+
+// ------------------------------------------------------------ velbc/nvelbc
+// MAC no-slip (1) / slip (2) rules of velbc (bcs.f90:156-369) and the
+// cell-centred rules of nvelbc (bcs.f90:371-455).  Three launches, x then y
+// then z, each over the full extent of the two other axes like the
+// reference loops, so corner ghosts end with the same values.
+struct VelBC {
+    int lo, hi;      // bc type on the low / high wall of this axis (1|2, else none)
+    int nodal;
+};
+
+template <int AX, int DIM>
+__global__ void k_velbc(double *__restrict__ u, double *__restrict__ v, double *__restrict__ w, Geom G, VelBC b) {
+    // thread over the 2D set of the two other axes (full ghosted extents)
+    const int g = G.g;
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    int bb = blockIdx.y * blockDim.y + threadIdx.y;
+    int na, nb;
+    if (AX == 0) { na = G.ny + 2 * g; nb = G.nz + 2 * G.gz; }
+    else if (AX == 1) { na = G.nx + 2 * g; nb = G.nz + 2 * G.gz; }
+    else { na = G.nx + 2 * g; nb = G.ny + 2 * g; }
+    if (a >= na || bb >= nb) return;
+    double *comp[3] = {u, v, w};
+    const int n = AX == 0 ? G.nx : (AX == 1 ? G.ny : G.nz);
+    for (int side = 0; side < 2; ++side) {
+        int bt = side == 0 ? b.lo : b.hi;
+        if (bt != 1 && bt != 2) continue;
+        double sg = bt == 1 ? -1.0 : 1.0;
+        for (int cpt = 0; cpt < DIM; ++cpt) {
+            double *f = comp[cpt];
+            bool normal = (cpt == AX) && !b.nodal;   // face-normal component
+            for (int ii = 1; ii <= g; ++ii) {
+                int dst = side == 0 ? -ii : n - 1 + ii;
+                int src;
+                if (side == 0) src = normal ? ii - 2 : ii - 1;
+                else src = normal ? n - 1 - ii : n - ii;
+                long ld, ls;
+                if (AX == 0) { ld = G.id(dst, a - g, bb - G.gz); ls = G.id(src, a - g, bb - G.gz); }
+                else if (AX == 1) { ld = G.id(a - g, dst, bb - G.gz); ls = G.id(a - g, src, bb - G.gz); }
+                else { ld = G.id(a - g, bb - g, dst); ls = G.id(a - g, bb - g, src); }
+                f[ld] = sg * f[ls];
+            }
+            if (normal) {
+                int wall = side == 0 ? -1 : n - 1;
+                long lw = AX == 0 ? G.id(wall, a - g, bb - G.gz)
+                                  : (AX == 1 ? G.id(a - g, wall, bb - G.gz) : G.id(a - g, bb - g, wall));
+                f[lw] = 0.0;
+            }
+        }
+    }
+}
+
+int velbc(fcfd_ctx *c, double *u, double *v, double *w, int nodal) {
+    const Geom &G = c->G;
+    const int g = G.g;
+    dim3 blk(64, 4);
+    ProfScope ps_(c, R_BC);
+    for (int ax = 0; ax < G.dim; ++ax) {
+        VelBC b;
+        const int32_t *t = ax == 0 ? c->pr.ubc : (ax == 1 ? c->pr.vbc : c->pr.wbc);
+        b.lo = t[0]; b.hi = t[1]; b.nodal = nodal;
+        int na = ax == 0 ? G.ny + 2 * g : G.nx + 2 * g;
+        int nb = ax == 2 ? G.ny + 2 * g : G.nz + 2 * G.gz;
+        dim3 grd((na + 63) / 64, (nb + 3) / 4);
+        if (G.dim == 3) {
+            if (ax == 0) hipLaunchKernelGGL((k_velbc<0, 3>), grd, blk, 0, c->stream, u, v, w, G, b);
+            if (ax == 1) hipLaunchKernelGGL((k_velbc<1, 3>), grd, blk, 0, c->stream, u, v, w, G, b);
+            if (ax == 2) hipLaunchKernelGGL((k_velbc<2, 3>), grd, blk, 0, c->stream, u, v, w, G, b);
+        } else {
+            if (ax == 0) hipLaunchKernelGGL((k_velbc<0, 2>), grd, blk, 0, c->stream, u, v, w, G, b);
+            if (ax == 1) hipLaunchKernelGGL((k_velbc<1, 2>), grd, blk, 0, c->stream, u, v, w, G, b);
+        }
+        FCFD_CHECK(hipGetLastError());
+    }
+    return FCFD_OK;
+}
+
+int copy_field(fcfd_ctx *c, double *dst, const double *src) {
+    FCFD_CHECK(hipMemcpyAsync(dst, src, sizeof(double) * c->G.ntot, hipMemcpyDeviceToDevice, c->stream));
+    return FCFD_OK;
+}
+
+}  // namespace fcfd
+
+// ==================================================================== C ABI
+
+static int alloc_field(double **p, long n) {
+    if (hipMalloc((void **)p, sizeof(double) * n) != hipSuccess) return FCFD_ERR_HIP;
+    if (hipMemset(*p, 0, sizeof(double) * n) != hipSuccess) return FCFD_ERR_HIP;
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_create(const fcfd_grid *grid, const fcfd_params *params, fcfd_ctx **out) {
+    if (!grid || !params || !out) return FCFD_ERR_ARG;
+    if (grid->dim != 2 && grid->dim != 3) return FCFD_ERR_ARG;
+    if (grid->ghc != 3) return FCFD_ERR_ARG;
+    if (grid->nx < 4 || grid->ny < 4) return FCFD_ERR_ARG;
+    if (grid->dim == 3 && grid->nz < 4) return FCFD_ERR_ARG;
+    if (grid->dim == 2 && grid->nz != 1) return FCFD_ERR_ARG;
+    fcfd_ctx *c = new fcfd_ctx();
+    memset(c->f, 0, sizeof(c->f));
+    c->gd = *grid;
+    c->pr = *params;
+    Geom &G = c->G;
+    G.dim = grid->dim;
+    G.nx = grid->nx; G.ny = grid->ny; G.nz = grid->nz;
+    G.g = grid->ghc; G.gz = grid->dim == 3 ? grid->ghc : 0;
+    G.sx = G.nx + 2 * G.g;
+    G.sxy = G.sx * (G.ny + 2 * G.g);
+    G.ntot = G.sxy * (G.nz + 2 * G.gz);
+    G.dx = (grid->xend - grid->xstart) / G.nx;
+    G.dy = (grid->yend - grid->ystart) / G.ny;
+    G.dz = grid->dim == 3 ? (grid->zend - grid->zstart) / G.nz : 1.0;
+    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return FCFD_ERR_HIP; }
+    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return FCFD_ERR_HIP; }
+    const long n = G.ntot;
+    int rc = FCFD_OK;
+    for (int f = 0; f < FCFD_NFIELDS && rc == FCFD_OK; ++f) {
+        bool zf = f == FCFD_W || f == FCFD_W_OLD || f == FCFD_NW || f == FCFD_SRC_W || f == FCFD_SRC_W_OLD ||
+                  f == FCFD_NZ || f == FCFD_NZZ || f == FCFD_NXZ || f == FCFD_NYZ;
+        if (zf && G.dim == 2) continue;
+        rc = alloc_field(&c->f[f], n);
+    }
+    double **scr[] = {&c->phiB, &c->phiS, &c->l1, &c->l2, &c->s2, &c->s3, &c->s0, &c->cx, &c->cy,
+                      &c->cz, &c->dsu, &c->dsv, &c->dsw, &c->un, &c->vn, &c->wn};
+    for (auto p : scr) {
+        if (rc != FCFD_OK) break;
+        rc = alloc_field(p, n);
+    }
+    // reduction partials: 8 slots x 1024 blocks, or two slots per block of an
+    // interior launch with 64x4 blocks, whichever is larger
+    {
+        long nbi = (long)((G.nx + 63) / 64) * ((G.ny + 3) / 4) * G.nz;
+        long need = std::max(8L * 1024, 2 * nbi + 16);
+        c->red.nblocks = (int)((need + 7) / 8);
+    }
+    if (rc == FCFD_OK) rc = alloc_field(&c->red.partial, 8L * c->red.nblocks);
+    if (rc == FCFD_OK) rc = alloc_field(&c->red.result, 16);
+    if (rc == FCFD_OK && hipHostMalloc((void **)&c->red.host, 16 * sizeof(double)) != hipSuccess) rc = FCFD_ERR_HIP;
+    if (rc == FCFD_OK && hipMalloc((void **)&c->flag, 16 * sizeof(unsigned long long)) != hipSuccess) rc = FCFD_ERR_HIP;
+    if (rc == FCFD_OK && hipHostMalloc((void **)&c->flag_host, 16 * sizeof(unsigned long long)) != hipSuccess)
+        rc = FCFD_ERR_HIP;
+    if (rc == FCFD_OK) rc = ccd_setup(c);
+    c->rho32 = nullptr;
+    c->mg_scalars = nullptr;
+    c->mg_ready = 0;
+    c->time = 0.0;
+    c->iter = 0;
+    c->diag_pending = 0;
+    memset(&c->diag, 0, sizeof(c->diag));
+    c->diag.imass = params->imass;
+    if (rc == FCFD_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = FCFD_ERR_HIP;
+    if (rc != FCFD_OK) {
+        fcfd_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_destroy(fcfd_ctx *c) {
+    if (!c) return FCFD_OK;
+    hipStreamSynchronize(c->stream);
+    for (int f = 0; f < FCFD_NFIELDS; ++f)
+        if (c->f[f]) hipFree(c->f[f]);
+    double *scr[] = {c->phiB, c->phiS, c->l1, c->l2, c->s2, c->s3, c->s0, c->cx, c->cy,
+                     c->cz, c->dsu, c->dsv, c->dsw, c->un, c->vn, c->wn,
+                     c->ccd_coef[0], c->ccd_coef[1], c->ccd_coef[2], c->red.partial, c->red.result};
+    for (auto p : scr)
+        if (p) hipFree(p);
+    if (c->red.host) hipHostFree(c->red.host);
+    if (c->flag) hipFree(c->flag);
+    if (c->flag_host) hipHostFree(c->flag_host);
+    for (hipEvent_t e : c->prof.pool) hipEventDestroy(e);
+    mg_free(c);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_set_params(fcfd_ctx *c, const fcfd_params *p) {
+    if (!c || !p) return FCFD_ERR_ARG;
+    c->pr = *p;
+    c->diag.imass = p->imass;
+    return FCFD_OK;
+}
+
+extern "C" int64_t fcfd_field_size(const fcfd_ctx *c) { return c ? (int64_t)c->G.ntot : 0; }
+
+static double *field_or_null(fcfd_ctx *c, int f) {
+    if (!c || f < 0 || f >= FCFD_NFIELDS) return nullptr;
+    return c->f[f];
+}
+
+extern "C" int fcfd_push(fcfd_ctx *c, int f, const double *host) {
+    double *d = field_or_null(c, f);
+    if (!d) return FCFD_ERR_FIELD;
+    if (!host) return FCFD_ERR_ARG;
+    FCFD_CHECK(hipMemcpyAsync(d, host, sizeof(double) * c->G.ntot, hipMemcpyHostToDevice, c->stream));
+    FCFD_CHECK(hipStreamSynchronize(c->stream));
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_pull(fcfd_ctx *c, int f, double *host) {
+    double *d = field_or_null(c, f);
+    if (!d) return FCFD_ERR_FIELD;
+    if (!host) return FCFD_ERR_ARG;
+    FCFD_CHECK(hipMemcpyAsync(host, d, sizeof(double) * c->G.ntot, hipMemcpyDeviceToHost, c->stream));
+    FCFD_CHECK(hipStreamSynchronize(c->stream));
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_push_device(fcfd_ctx *c, int f, const double *dev) {
+    double *d = field_or_null(c, f);
+    if (!d) return FCFD_ERR_FIELD;
+    if (!dev) return FCFD_ERR_ARG;
+    FCFD_CHECK(hipMemcpyAsync(d, dev, sizeof(double) * c->G.ntot, hipMemcpyDeviceToDevice, c->stream));
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_pull_device(fcfd_ctx *c, int f, double *dev) {
+    double *d = field_or_null(c, f);
+    if (!d) return FCFD_ERR_FIELD;
+    if (!dev) return FCFD_ERR_ARG;
+    FCFD_CHECK(hipMemcpyAsync(dev, d, sizeof(double) * c->G.ntot, hipMemcpyDeviceToDevice, c->stream));
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_sync(fcfd_ctx *c) {
+    if (!c) return FCFD_ERR_ARG;
+    FCFD_CHECK(hipStreamSynchronize(c->stream));
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_get_diag(fcfd_ctx *c, fcfd_diag *out) {
+    if (!c || !out) return FCFD_ERR_ARG;
+    FCFD_CHECK(hipStreamSynchronize(c->stream));
+    if (c->diag_pending) {
+        c->diag.mass = c->red.host[8];
+        c->diag.vol = c->red.host[9];
+        c->diag_pending = 0;
+    }
+    *out = c->diag;
+    out->iter = c->iter;
+    out->imass = c->pr.imass;
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_set_time(fcfd_ctx *c, double time, int iter) {
+    if (!c) return FCFD_ERR_ARG;
+    c->time = time;
+    c->iter = iter;
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_set_imass(fcfd_ctx *c, double imass) {
+    if (!c) return FCFD_ERR_ARG;
+    c->pr.imass = imass;
+    c->diag.imass = imass;
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_bc(fcfd_ctx *c, int f) {
+    double *d = field_or_null(c, f);
+    if (!d) return FCFD_ERR_FIELD;
+    return bc_scalar(c, d);
+}
+
+extern "C" int fcfd_velbc(fcfd_ctx *c) {
+    if (!c) return FCFD_ERR_ARG;
+    return velbc(c, c->f[FCFD_U], c->f[FCFD_V], c->f[FCFD_W], 0);
+}

@@ -1,0 +1,161 @@
+// fcfd_internal.h — device context and helpers shared by the HIP sources of
+// libfcfd.  Everything here is MI355X (gfx950, wave64) code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/fcfd.h"
+
+#define FCFD_CHECK(call)                                                                \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            fprintf(stderr, "libfcfd: HIP error %s at %s:%d\n", hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                \
+            return FCFD_ERR_HIP;                                                        \
+        }                                                                               \
+    } while (0)
+
+#define FCFD_TRY(expr)              \
+    do {                            \
+        int rc_ = (expr);           \
+        if (rc_ != FCFD_OK) return rc_; \
+    } while (0)
+
+namespace fcfd {
+
+constexpr int WAVE = 64;
+
+// Geometry of one ghosted field: interior (nx, ny, nz), ghost width g (gz = 0
+// in 2D), strides of the column-major reference layout.
+struct Geom {
+    int dim, nx, ny, nz, g, gz;
+    long sx, sxy, ntot;
+    double dx, dy, dz;
+    __host__ __device__ long id(int i, int j, int k) const {
+        return (long)(i + g) + (long)(j + g) * sx + (long)(k + gz) * sxy;
+    }
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// One level of the pressure multigrid hierarchy (see mg.hip).
+struct MgLevel {
+    int nx, ny, nz;          // cells
+    int cx, cy, cz;          // coarsening factor (1|2) to the next level
+    double hx, hy, hz;       // spacing
+    long sx, sxy, ntot;      // layout with one ghost layer (finest uses Geom)
+    // FP64 arrays (level >= 1): solution/error, rhs, residual, face coeffs
+    double *e64, *f64, *r64, *bx64, *by64, *bz64;
+    // FP32 mirrors for the mixed-precision inner cycle
+    float *e32, *f32, *r32, *bx32, *by32, *bz32;
+};
+
+// Kernel regions timed by the built-in HIP-event profiler (fcfd_prof_*).
+enum Region {
+    R_LS_ADV, R_REDIS_A, R_REDIS_B, R_MASS, R_MPLS_UPD, R_CCD, R_CURV, R_DIFF, R_PRED, R_PPE_SRC,
+    R_PROJ, R_NODE_VEL, R_BC, R_SMOOTH0, R_SMOOTHC, R_RESTRICT0, R_RESTRICTC, R_PROLONG0, R_PROLONGC,
+    R_RESID0, R_COARSE, R_MISC, NREG
+};
+
+struct ProfRec {
+    int reg;
+    int ev;   // index of the start event in the pool (end = ev + 1)
+};
+
+struct Prof {
+    int on = 0;
+    std::vector<hipEvent_t> pool;
+    std::vector<ProfRec> recs;
+    size_t used = 0;
+};
+
+struct Reduce {
+    double *partial;         // per-block partial results (nslots * nblocks)
+    double *result;          // reduced values (device)
+    double *host;            // pinned host mirror
+    int nblocks;
+};
+
+}  // namespace fcfd
+
+struct fcfd_ctx {
+    fcfd_grid gd;
+    fcfd_params pr;
+    fcfd::Geom G;
+    hipStream_t stream;
+    int device;
+    // ---- state (device, full ghosted arrays) ----
+    double *f[FCFD_NFIELDS];   // indexed by enum fcfd_field; NULL when not held
+    // ---- scratch ----
+    double *phiB, *phiS;       // ping-pong phi buffers / stage start copy
+    double *l1, *l2;           // stored RK stage sources
+    double *s2, *s3, *s0;      // reinit work: lambda numerator/denominator, frozen sign
+    double *cx, *cy, *cz;      // CCD second-derivative scratch / cross-derivative temp
+    double *dsu, *dsv, *dsw;   // viscous + body force source (sec_part1..3), once per step
+    double *un, *vn, *wn;      // predictor output buffer (ping-pong with u, v, w)
+    double *ccd_coef[3];       // factorised CCD line matrices per direction
+    // ---- reductions ----
+    fcfd::Reduce red;
+    unsigned long long *flag;  // device flags (reinit error max bits, done, count)
+    unsigned long long *flag_host;
+    // ---- multigrid ----
+    std::vector<fcfd::MgLevel> mg;
+    float *rho32;              // FP32 copy of rho for the mixed inner cycle
+    double *mg_scalars;        // device scalars of the solve (norms)
+    int mg_ready;
+    // ---- bookkeeping ----
+    double time;
+    int iter;
+    int diag_pending;          // step mass/volume waiting in red.host[8..9]
+    fcfd_diag diag;
+    fcfd::Prof prof;
+};
+
+namespace fcfd {
+// Records a start/end HIP event pair on the context stream around the launches
+// of its scope when profiling is on.
+struct ProfScope {
+    fcfd_ctx *c;
+    int reg, ev;
+    ProfScope(fcfd_ctx *c_, int reg_);
+    ~ProfScope();
+};
+}  // namespace fcfd
+
+namespace fcfd {
+
+// launch helpers --------------------------------------------------------
+inline dim3 grid_interior(const Geom &G, dim3 blk) {
+    return dim3((G.nx + blk.x - 1) / blk.x, (G.ny + blk.y - 1) / blk.y, G.nz);
+}
+
+// kernels implemented in the .hip files (host wrappers)
+int bc_scalar(fcfd_ctx *c, double *d);
+int velbc(fcfd_ctx *c, double *u, double *v, double *w, int nodal);
+int ls_props(fcfd_ctx *c, const double *phi, double *H, double *D, double *S, double *rho, double *mu);
+int reduce_sums(fcfd_ctx *c, int kind, const double *a, const double *b, const double *cc, const double *d);
+int ccd_dir(fcfd_ctx *c, int dir, const double *f, double *fx, double *fxx);
+int ccd_setup(fcfd_ctx *c);
+int surface_norms(fcfd_ctx *c, const double *phi, int full);
+int curv_kernel(fcfd_ctx *c);
+int ls_advect(fcfd_ctx *c);
+int redis(fcfd_ctx *c, int btn, int *iters);
+int mpls(fcfd_ctx *c);
+int ls_mv(fcfd_ctx *c, double *mass, double *vol);
+int ns_diff_source(fcfd_ctx *c);
+int ns_predictor(fcfd_ctx *c, const double *ut, const double *vt, const double *wt);
+int ppe_src(fcfd_ctx *c);
+int projection(fcfd_ctx *c);
+int node_vel(fcfd_ctx *c);
+int div_check(fcfd_ctx *c);
+int mg_setup(fcfd_ctx *c);
+int mg_free(fcfd_ctx *c);
+int ppe_solve(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel);
+int copy_field(fcfd_ctx *c, double *dst, const double *src);
+
+}  // namespace fcfd

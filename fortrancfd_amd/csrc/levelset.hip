@@ -1,0 +1,560 @@
+// levelset.hip — level-set kernels of the two-phase step on MI355X:
+//   * ls_funs / rho_mu / ls_mv        (`.../interface_solver/others.f90:56-176`)
+//   * WENO5 flux-split advection with SSP-RK3
+//                                     (`.../interface_solver/level_set_rk3_solver.f90:1-219`)
+//   * PDE reinitialisation with the mass-constrained lambda
+//                                     (`.../interface_solver/level_set_rk3_redis.f90:1-410`)
+//   * mass-preserving correction      (`.../interface_solver/mass_preserving_level_set.f90:1-54`)
+// Paths are under `Two phase flows/newOpenMP/3D+IBM/` (2D twins under `2D/`).
+//
+// Layout: reference column-major ghosted arrays.  One thread per interior cell,
+// 64 threads along i (the contiguous axis) per wavefront, so every stencil
+// load of a wavefront is a contiguous 512-byte run.  Intermediate arrays that
+// the reference fills and then ghost-copies with bc() (the flux split s1/s2,
+// the backward differences, the lambda numerator/denominator) are recomputed
+// or read with clamped indices instead of being stored with ghosts.
+#include "fcfd_internal.h"
+
+using namespace fcfd;
+
+namespace fcfd {
+
+constexpr double kPi = 3.141592653589793;  // dacos(-1.0_8)
+
+// ls_funs for one value (others.f90:62-87)
+__device__ __forceinline__ void lsfun(double phi, double w, double &H, double &D) {
+    const double eps = 1.0e-12;
+    double x = phi / w;
+    if (x > 1.0 - eps) {
+        H = 1.0; D = 0.0;
+    } else if (x < -1.0 + eps) {
+        H = 0.0; D = 0.0;
+    } else {
+        double s, c;
+        sincos(kPi * x, &s, &c);
+        H = 0.5 * (1.0 + x + s / kPi);
+        D = 0.5 * (1.0 + c) / w;
+    }
+    H = fmax(fmin(H, 1.0), 0.0);
+}
+
+// WENO5-JS left-biased face value from g(i-2..i+2) (level_set_rk3_solver.f90:170-191)
+__device__ __forceinline__ double weno_left(double g0, double g1, double g2, double g3, double g4, double EPS) {
+    double t1 = g0 - 2.0 * g1 + g2, t2 = g0 - 4.0 * g1 + 3.0 * g2;
+    double b1 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
+    t1 = g1 - 2.0 * g2 + g3; t2 = g1 - g3;
+    double b2 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
+    t1 = g2 - 2.0 * g3 + g4; t2 = 3.0 * g2 - 4.0 * g3 + g4;
+    double b3 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
+    double a1 = 1.0 / ((EPS + b1) * (EPS + b1));
+    double a2 = 6.0 / ((EPS + b2) * (EPS + b2));
+    double a3 = 3.0 / ((EPS + b3) * (EPS + b3));
+    double w1 = a1 / (a1 + a2 + a3), w2 = a2 / (a1 + a2 + a3), w3 = a3 / (a1 + a2 + a3);
+    return w1 / 3.0 * g0 - (7.0 * w1 + w2) / 6.0 * g1 + (11.0 * w1 + 5.0 * w2 + 2.0 * w3) / 6.0 * g2 +
+           (2.0 * w2 + 5.0 * w3) / 6.0 * g3 - w3 / 6.0 * g4;
+}
+
+// WENO5-JS right-biased face value from f(i-1..i+3) (level_set_rk3_solver.f90:193-216)
+__device__ __forceinline__ double weno_right(double f0, double f1, double f2, double f3, double f4, double EPS) {
+    double t1 = f0 - 2.0 * f1 + f2, t2 = f0 - 4.0 * f1 + 3.0 * f2;
+    double b3 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
+    t1 = f1 - 2.0 * f2 + f3; t2 = f1 - f3;
+    double b2 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
+    t1 = f2 - 2.0 * f3 + f4; t2 = 3.0 * f2 - 4.0 * f3 + f4;
+    double b1 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
+    double a1 = 1.0 / ((EPS + b1) * (EPS + b1));
+    double a2 = 6.0 / ((EPS + b2) * (EPS + b2));
+    double a3 = 3.0 / ((EPS + b3) * (EPS + b3));
+    double w1 = a1 / (a1 + a2 + a3), w2 = a2 / (a1 + a2 + a3), w3 = a3 / (a1 + a2 + a3);
+    return w3 * (-f0 + 5.0 * f1 + 2.0 * f2) / 6.0 + w2 * (2.0 * f1 + 5.0 * f2 - f3) / 6.0 +
+           w1 * (11.0 * f2 - 7.0 * f3 + 2.0 * f4) / 6.0;
+}
+
+// ------------------------------------------------------------- properties
+// ls_funs + rho_mu over the full ghosted box (others.f90:56-120)
+__global__ void k_ls_props(const double *__restrict__ phi, double *__restrict__ H, double *__restrict__ D,
+                           double *__restrict__ S, double *__restrict__ R, double *__restrict__ M, long n,
+                           double w, double r12, double m12) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double h, d;
+    lsfun(phi[t], w, h, d);
+    if (H) H[t] = h;
+    if (D) D[t] = d;
+    if (S) S[t] = 2.0 * h - 1.0;
+    if (R) R[t] = h + r12 * (1.0 - h);
+    if (M) M[t] = h + m12 * (1.0 - h);
+}
+
+int ls_props(fcfd_ctx *c, const double *phi, double *H, double *D, double *S, double *rho, double *mu) {
+    long n = c->G.ntot;
+    ProfScope ps_(c, R_MISC);
+    hipLaunchKernelGGL(k_ls_props, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, phi, H, D, S, rho, mu,
+                       n, c->pr.ls_wid, c->pr.rho_12, c->pr.mu_12);
+    FCFD_CHECK(hipGetLastError());
+    return FCFD_OK;
+}
+
+// ------------------------------------------------------------- reductions
+// Deterministic sums over the interior: a fixed grid of RBLK blocks, each
+// thread accumulates a fixed strided subset in order, then fixed-shape tree
+// reductions in LDS; a second single-block pass sums the block partials.
+constexpr int RBLK = 1024;
+constexpr int RTHR = 256;
+
+__device__ __forceinline__ void cell_of(long t, const Geom &G, int &i, int &j, int &k) {
+    i = (int)(t % G.nx);
+    long r = t / G.nx;
+    j = (int)(r % G.ny);
+    k = (int)(r / G.ny);
+}
+
+template <int NS>
+__device__ __forceinline__ void block_sum_store(double (&acc)[NS], double *partial, int nblocks) {
+    __shared__ double sh[NS][RTHR];
+    for (int s = 0; s < NS; ++s) sh[s][threadIdx.x] = acc[s];
+    __syncthreads();
+    for (int w = RTHR / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int s = 0; s < NS; ++s) sh[s][threadIdx.x] += sh[s][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int s = 0; s < NS; ++s) partial[s * nblocks + blockIdx.x] = sh[s][0];
+}
+
+// second pass: result[s] = sum_b partial[s*nb + b]  (one block of RTHR threads)
+__global__ void k_reduce_final(const double *__restrict__ partial, double *__restrict__ result, int ns, int nb) {
+    __shared__ double sh[RTHR];
+    for (int s = 0; s < ns; ++s) {
+        double a = 0.0;
+        for (int b = threadIdx.x; b < nb; b += RTHR) a += partial[s * nb + b];
+        sh[threadIdx.x] = a;
+        __syncthreads();
+        for (int w = RTHR / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) result[s] = sh[0];
+        __syncthreads();
+    }
+}
+
+// mass / volume (ls_mv, others.f90:129-152) and, with nrm != null, the MPLS
+// lambda denominator sum delta^2 |grad phi| (2(1-rho12)H + rho12) dV with
+// |grad phi| from the CCD normals, written to grad
+// (mass_preserving_level_set.f90:17-34).
+template <int DIM>
+__global__ void k_mass_sums(const double *__restrict__ phi, const double *__restrict__ nx_,
+                            const double *__restrict__ ny_, const double *__restrict__ nz_,
+                            double *__restrict__ grad, double *__restrict__ partial, Geom G, double w,
+                            double r12, int with_lam) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    const long n = (long)G.nx * G.ny * G.nz;
+    const double dv = DIM == 3 ? G.dx * G.dy * G.dz : G.dx * G.dy;
+    for (long t = (long)blockIdx.x * RTHR + threadIdx.x; t < n; t += (long)RBLK * RTHR) {
+        int i, j, k;
+        cell_of(t, G, i, j, k);
+        long id = G.id(i, j, k);
+        double h, d;
+        lsfun(phi[id], w, h, d);
+        double rho = h + r12 * (1.0 - h);
+        acc[0] += rho * h * dv;
+        acc[1] += h * dv;
+        if (with_lam) {
+            double fx = nx_[id], fy = ny_[id], g;
+            if (DIM == 3) {
+                double fz = nz_[id];
+                g = sqrt(fx * fx + fy * fy + fz * fz);
+            } else {
+                g = sqrt(fx * fx + fy * fy);
+            }
+            grad[id] = g;
+            double plam = d * d * g;
+            plam = DIM == 3 ? plam * (2.0 * (1.0 - r12) * h + r12) * G.dx * G.dy * G.dz
+                            : plam * (2.0 * (1.0 - r12) * h + r12) * G.dx * G.dy;
+            acc[2] += plam;
+        }
+    }
+    block_sum_store<3>(acc, partial, RBLK);
+}
+
+// phi += lambda delta |grad phi|, lambda = (imass - mass) / denominator
+// computed here from the reduced sums (mass_preserving_level_set.f90:36-48)
+template <int DIM>
+__global__ void k_mpls_update(double *__restrict__ phi, const double *__restrict__ grad,
+                              const double *__restrict__ sums, double *__restrict__ lam_out, Geom G, double w,
+                              double imass) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    double lam = (imass - sums[0]) / sums[2];
+    if (i == 0 && j == 0 && k == 0) *lam_out = lam;
+    if (i >= G.nx || j >= G.ny) return;
+    long id = G.id(i, j, k);
+    double h, d;
+    lsfun(phi[id], w, h, d);
+    phi[id] = phi[id] + lam * d * grad[id];
+}
+
+int mass_sums(fcfd_ctx *c, const double *phi, int with_lam) {
+    const Geom &G = c->G;
+    ProfScope ps_(c, R_MASS);
+    if (G.dim == 3)
+        hipLaunchKernelGGL(k_mass_sums<3>, dim3(RBLK), dim3(RTHR), 0, c->stream, phi, c->f[FCFD_NX], c->f[FCFD_NY],
+                           c->f[FCFD_NZ], c->f[FCFD_GRAD], c->red.partial, G, c->pr.ls_wid, c->pr.rho_12, with_lam);
+    else
+        hipLaunchKernelGGL(k_mass_sums<2>, dim3(RBLK), dim3(RTHR), 0, c->stream, phi, c->f[FCFD_NX], c->f[FCFD_NY],
+                           nullptr, c->f[FCFD_GRAD], c->red.partial, G, c->pr.ls_wid, c->pr.rho_12, with_lam);
+    FCFD_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RTHR), 0, c->stream, c->red.partial, c->red.result, 3, RBLK);
+    FCFD_CHECK(hipGetLastError());
+    return FCFD_OK;
+}
+
+int fetch_result(fcfd_ctx *c, int n) {
+    FCFD_CHECK(hipMemcpyAsync(c->red.host, c->red.result, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    FCFD_CHECK(hipStreamSynchronize(c->stream));
+    return FCFD_OK;
+}
+
+int ls_mv(fcfd_ctx *c, double *mass, double *vol) {
+    FCFD_TRY(ls_props(c, c->f[FCFD_PHI], c->f[FCFD_HEAVY], c->f[FCFD_DELTA], c->f[FCFD_SIGN], c->f[FCFD_RHO],
+                      c->f[FCFD_MU]));
+    FCFD_TRY(mass_sums(c, c->f[FCFD_PHI], 0));
+    FCFD_TRY(fetch_result(c, 2));
+    c->diag.mass = c->red.host[0];
+    c->diag.vol = c->red.host[1];
+    if (mass) *mass = c->red.host[0];
+    if (vol) *vol = c->red.host[1];
+    return FCFD_OK;
+}
+
+int mpls(fcfd_ctx *c) {
+    const Geom &G = c->G;
+    double *phi = c->f[FCFD_PHI];
+    dim3 blk(64, 4), grd = grid_interior(G, blk);
+    double *lam_dev = c->red.result + 8;
+    for (int it = 0; it < 2; ++it) {
+        // surface_norms: only the first derivatives enter the correction
+        FCFD_TRY(ccd_dir(c, 0, phi, c->f[FCFD_NX], c->cx));
+        FCFD_TRY(ccd_dir(c, 1, phi, c->f[FCFD_NY], c->cy));
+        if (G.dim == 3) FCFD_TRY(ccd_dir(c, 2, phi, c->f[FCFD_NZ], c->cz));
+        FCFD_TRY(mass_sums(c, phi, 1));
+        {
+            ProfScope ps_(c, R_MPLS_UPD);
+            if (G.dim == 3)
+                hipLaunchKernelGGL(k_mpls_update<3>, grd, blk, 0, c->stream, phi, c->f[FCFD_GRAD], c->red.result,
+                                   lam_dev, G, c->pr.ls_wid, c->pr.imass);
+            else
+                hipLaunchKernelGGL(k_mpls_update<2>, grd, blk, 0, c->stream, phi, c->f[FCFD_GRAD], c->red.result,
+                                   lam_dev, G, c->pr.ls_wid, c->pr.imass);
+            FCFD_CHECK(hipGetLastError());
+        }
+        FCFD_TRY(bc_scalar(c, phi));
+    }
+    return FCFD_OK;
+}
+
+// ------------------------------------------------------------- advection
+// One SSP-RK3 stage of level_set_rk3_solver, fused: velocity flux split
+// (source_setup :67-96, with the bc() copy of s1/s2 realised by clamped
+// indices), WENO5-JS faces (:162-219), flux divergence (:145-158) and the
+// increment-form stage update (:13-60).
+template <int DIM>
+__global__ void __launch_bounds__(256) k_ls_adv_stage(const double *__restrict__ phi, double *__restrict__ phin,
+                                                      const double *__restrict__ nu, const double *__restrict__ nv,
+                                                      const double *__restrict__ nw, double *__restrict__ l1,
+                                                      double *__restrict__ l2, Geom G, double dt, int stage) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= G.nx || j >= G.ny) return;
+    const long id = G.id(i, j, k);
+    const double *vel[3] = {nu, nv, nw};
+    const int n[3] = {G.nx, G.ny, G.nz};
+    const int pos[3] = {i, j, k};
+    const long st[3] = {1, G.sx, G.sxy};
+    const double h[3] = {G.dx, G.dy, G.dz};
+    double L = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        double s1[7], s2[7];  // positions pos-3 .. pos+3
+#pragma unroll
+        for (int o = -3; o <= 3; ++o) {
+            int q = clampi(pos[d] + o, 0, n[d] - 1);
+            long a = id + (long)(q - pos[d]) * st[d];
+            double u = vel[d][a], p = phi[a];
+            s1[o + 3] = 0.5 * (u + fabs(u)) * p;
+            s2[o + 3] = 0.5 * (u - fabs(u)) * p;
+        }
+        const double EPS = 1.0e-13;
+        // face i+1/2: gm from s1(i-2..i+2), fp from s2(i-1..i+3)
+        double gmR = weno_left(s1[1], s1[2], s1[3], s1[4], s1[5], EPS);
+        double fpR = weno_right(s2[2], s2[3], s2[4], s2[5], s2[6], EPS);
+        // face i-1/2
+        double gmL = weno_left(s1[0], s1[1], s1[2], s1[3], s1[4], EPS);
+        double fpL = weno_right(s2[1], s2[2], s2[3], s2[4], s2[5], EPS);
+        double t = -(gmR + fpR - gmL - fpL) / h[d];
+        L = d == 0 ? t : L + t;
+    }
+    double src;
+    if (stage == 0) {
+        l1[id] = L;
+        src = L;
+    } else if (stage == 1) {
+        l2[id] = L;
+        src = (-3.0 * l1[id] + L) / 4.0;
+    } else {
+        src = (-l1[id] - l2[id] + 8.0 * L) / 12.0;
+    }
+    phin[id] = phi[id] + src * dt;
+}
+
+int ls_advect(fcfd_ctx *c) {
+    const Geom &G = c->G;
+    dim3 blk(64, 4), grd = grid_interior(G, blk);
+    double *a = c->f[FCFD_PHI], *b = c->phiB;
+    for (int s = 0; s < 3; ++s) {
+        {
+            ProfScope ps_(c, R_LS_ADV);
+            if (G.dim == 3)
+                hipLaunchKernelGGL(k_ls_adv_stage<3>, grd, blk, 0, c->stream, a, b, c->f[FCFD_NU], c->f[FCFD_NV],
+                                   c->f[FCFD_NW], c->l1, c->l2, G, c->pr.dt, s);
+            else
+                hipLaunchKernelGGL(k_ls_adv_stage<2>, grd, blk, 0, c->stream, a, b, c->f[FCFD_NU], c->f[FCFD_NV],
+                                   nullptr, c->l1, c->l2, G, c->pr.dt, s);
+            FCFD_CHECK(hipGetLastError());
+        }
+        FCFD_TRY(bc_scalar(c, b));
+        double *t = a; a = b; b = t;
+    }
+    // three stages end in the buffer that was phiB: swap roles
+    c->phiB = c->f[FCFD_PHI];
+    c->f[FCFD_PHI] = a;
+    return FCFD_OK;
+}
+
+// ------------------------------------------------------------- reinit
+// Pass A of one reinitialisation stage: backward differences of phi with
+// the bc() ghost copy as clamped indices (redis_gradient :219-233), one-sided
+// WENO5 values, EPS = 1e-10 (wenojs_flux :295-339), Godunov |grad phi| by the
+// frozen sign (:262-291), then the lambda numerator/denominator terms
+// (redis_lambda :366-380; 2D: lambda = delta, 3D: density weighted).
+template <int DIM>
+__global__ void __launch_bounds__(256) k_redis_a(const double *__restrict__ phi, const double *__restrict__ S0,
+                                                 double *__restrict__ grad, double *__restrict__ s2,
+                                                 double *__restrict__ s3, const unsigned long long *__restrict__ done,
+                                                 Geom G, double w, double r12, int btn) {
+    if (done && done[1]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= G.nx || j >= G.ny) return;
+    const long id = G.id(i, j, k);
+    const int n[3] = {G.nx, G.ny, G.nz};
+    const int pos[3] = {i, j, k};
+    const long st[3] = {1, G.sx, G.sxy};
+    const double h[3] = {G.dx, G.dy, G.dz};
+    double sel[3];
+    const bool pos_sign = S0[id] >= 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        double nr[7];  // D-phi at pos-3 .. pos+3 (ghost entries copy the boundary one)
+#pragma unroll
+        for (int o = -3; o <= 3; ++o) {
+            int q = clampi(pos[d] + o, 0, n[d] - 1);
+            long a = id + (long)(q - pos[d]) * st[d];
+            nr[o + 3] = (phi[a] - phi[a - st[d]]) / h[d];
+        }
+        const double EPS = 1.0e-10;
+        double fm = weno_left(nr[1], nr[2], nr[3], nr[4], nr[5], EPS);   // tdata%s2
+        double fp = weno_right(nr[2], nr[3], nr[4], nr[5], nr[6], EPS);  // tdata%s1
+        if (pos_sign) sel[d] = fmax(-fmin(fp, 0.0), fmax(fm, 0.0));
+        else sel[d] = fmax(fmax(fp, 0.0), -fmin(fm, 0.0));
+    }
+    double g = DIM == 3 ? sqrt(sel[0] * sel[0] + sel[1] * sel[1] + sel[2] * sel[2])
+                        : sqrt(sel[0] * sel[0] + sel[1] * sel[1]);
+    grad[id] = g;
+    if (btn != 0) {
+        double hh, dd;
+        lsfun(phi[id], w, hh, dd);
+        double lam = DIM == 3 ? (2.0 * (1.0 - r12) * hh + r12) * dd : dd;
+        s2[id] = lam * S0[id] * (g - 1.0);
+        s3[id] = g * dd * lam;
+    }
+}
+
+// Pass B: 3^d weighted sums (centre weight 16 in 2D, 51 in 3D, bc() copy as
+// clamped indices), correction s1 (redis_lambda :385-408), stage source and
+// in-place increment-form SSP-RK3 update (redis_solver :150-208).  The last
+// stage also folds max|phi - phi_start| into flags[0] (reinit stop test :42-54).
+template <int DIM>
+__global__ void __launch_bounds__(256) k_redis_b(double *__restrict__ phi, const double *__restrict__ S0,
+                                                 const double *__restrict__ grad, const double *__restrict__ s2,
+                                                 const double *__restrict__ s3, double *__restrict__ l1,
+                                                 double *__restrict__ l2, const double *__restrict__ phis,
+                                                 unsigned long long *__restrict__ flags, Geom G, double w,
+                                                 double rdt, int btn, int stage) {
+    if (flags[1]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    bool in = i < G.nx && j < G.ny;
+    double err = 0.0;
+    if (in) {
+        const long id = G.id(i, j, k);
+        double hh, dd;
+        double p0 = phi[id];
+        lsfun(p0, w, hh, dd);
+        double s1 = 0.0;
+        if (btn != 0) {
+            const double wc = DIM == 3 ? 51.0 : 16.0;
+            double a = wc * s2[id], b = wc * s3[id];
+            const int kl = DIM == 3 ? -1 : 0, kh = DIM == 3 ? 1 : 0;
+            for (int kk = kl; kk <= kh; ++kk) {
+                int kq = DIM == 3 ? clampi(k + kk, 0, G.nz - 1) : 0;
+                for (int jj = -1; jj <= 1; ++jj) {
+                    int jq = clampi(j + jj, 0, G.ny - 1);
+                    for (int ii = -1; ii <= 1; ++ii) {
+                        int iq = clampi(i + ii, 0, G.nx - 1);
+                        long o = G.id(iq, jq, kq);
+                        a = a + s2[o];
+                        b = b + s3[o];
+                    }
+                }
+            }
+            if (fabs(b) > 1.0e-12) s1 = a / b * dd;
+        }
+        double S = S0[id];
+        double l = (S - s1) * grad[id] - S, src;
+        if (stage == 0) { l1[id] = l; src = l; }
+        else if (stage == 1) { l2[id] = l; src = (-3.0 * l1[id] + l) / 4.0; }
+        else { src = (-l1[id] - l2[id] + 8.0 * l) / 12.0; }
+        double pn = p0 - rdt * src;
+        phi[id] = pn;
+        if (stage == 2) err = fabs(phis[id] - pn);
+    }
+    if (stage == 2) {
+        // wavefront max then one atomic per wavefront; max of non-negative
+        // doubles is order independent, compared through their bit patterns
+        for (int off = 32; off > 0; off >>= 1) err = fmax(err, __shfl_down(err, off, 64));
+        if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0)
+            atomicMax(&flags[0], (unsigned long long)__double_as_longlong(err));
+    }
+}
+
+// end of one pseudo step: count it, stop when error <= tol, clear the max
+__global__ void k_redis_check(unsigned long long *flags, double tol) {
+    if (flags[1]) return;
+    double err = __longlong_as_double((long long)flags[0]);
+    flags[2] += 1;
+    if (err <= tol) flags[1] = 1;
+    flags[0] = 0ull;
+}
+
+__global__ void k_copy_interior(double *__restrict__ dst, const double *__restrict__ src,
+                                const unsigned long long *__restrict__ done, Geom G) {
+    if (done && done[1]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= G.nx || j >= G.ny) return;
+    long id = G.id(i, j, k);
+    dst[id] = src[id];
+}
+
+__global__ void k_sign_init(const double *__restrict__ phi, double *__restrict__ S0, Geom G, double w, int btn) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= G.nx || j >= G.ny) return;
+    long id = G.id(i, j, k);
+    if (btn == 0) {
+        S0[id] = phi[id];
+    } else {
+        double h, d;
+        lsfun(phi[id], w, h, d);
+        S0[id] = 2.0 * h - 1.0;
+    }
+}
+
+// max of grad over the interior into flags[3] (level_set_redis_stable :113-122)
+__global__ void k_max_interior(const double *__restrict__ a, unsigned long long *flags, Geom G) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    double v = 0.0;
+    if (i < G.nx && j < G.ny) v = a[G.id(i, j, k)];
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0)
+        atomicMax(&flags[3], (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void k_scale_by_flag(double *__restrict__ phi, const unsigned long long *flags, Geom G) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= G.nx || j >= G.ny) return;
+    double gmax = __longlong_as_double((long long)flags[3]);
+    long id = G.id(i, j, k);
+    phi[id] = phi[id] / gmax;
+}
+
+int redis(fcfd_ctx *c, int btn, int *iters) {
+    const Geom &G = c->G;
+    dim3 blk(64, 4), grd = grid_interior(G, blk);
+    double *phi = c->f[FCFD_PHI], *grad = c->f[FCFD_GRAD];
+    const double w = c->pr.ls_wid, r12 = c->pr.rho_12, rdt = c->pr.rdt;
+    FCFD_CHECK(hipMemsetAsync(c->flag, 0, 8 * sizeof(unsigned long long), c->stream));
+    // level_set_redis_init / stable (:67-136)
+    hipLaunchKernelGGL(k_sign_init, grd, blk, 0, c->stream, phi, c->s0, G, w, btn);
+    if (btn == 0) {
+        if (G.dim == 3)
+            hipLaunchKernelGGL(k_redis_a<3>, grd, blk, 0, c->stream, phi, c->s0, grad, c->s2, c->s3, nullptr, G, w,
+                               r12, 0);
+        else
+            hipLaunchKernelGGL(k_redis_a<2>, grd, blk, 0, c->stream, phi, c->s0, grad, c->s2, c->s3, nullptr, G, w,
+                               r12, 0);
+        hipLaunchKernelGGL(k_max_interior, grd, blk, 0, c->stream, grad, c->flag, G);
+        hipLaunchKernelGGL(k_scale_by_flag, grd, blk, 0, c->stream, phi, c->flag, G);
+        FCFD_TRY(bc_scalar(c, phi));
+    }
+    // pseudo-time budget: the reference stops once the accumulated time exceeds
+    // timestop (:18-28,54); replay that accumulation to fix the launch count
+    double timestop;
+    if (btn == 0) {
+        double m = c->gd.xend - c->gd.xstart;
+        m = fmax(m, c->gd.yend - c->gd.ystart);
+        if (G.dim == 3) m = fmax(m, c->gd.zend - c->gd.zstart);
+        timestop = 1.5 * m;
+    } else {
+        timestop = 3.0 * G.dx;
+    }
+    int nmax = 0;
+    for (double t = 0.0;;) {
+        nmax += 1;
+        t = t + rdt;
+        if (t > timestop) break;
+        if (nmax > 10000000) return FCFD_ERR_ARG;
+    }
+    const double tol = G.dim == 3 ? 1.0e-5 : 1.0e-8;
+    for (int it = 0; it < nmax; ++it) {
+        hipLaunchKernelGGL(k_copy_interior, grd, blk, 0, c->stream, c->phiS, phi, c->flag, G);
+        for (int s = 0; s < 3; ++s) {
+            {
+                ProfScope ps_(c, R_REDIS_A);
+                if (G.dim == 3)
+                    hipLaunchKernelGGL(k_redis_a<3>, grd, blk, 0, c->stream, phi, c->s0, grad, c->s2, c->s3, c->flag,
+                                       G, w, r12, btn);
+                else
+                    hipLaunchKernelGGL(k_redis_a<2>, grd, blk, 0, c->stream, phi, c->s0, grad, c->s2, c->s3, c->flag,
+                                       G, w, r12, btn);
+            }
+            {
+                ProfScope ps_(c, R_REDIS_B);
+                if (G.dim == 3)
+                    hipLaunchKernelGGL(k_redis_b<3>, grd, blk, 0, c->stream, phi, c->s0, grad, c->s2, c->s3, c->l1,
+                                       c->l2, c->phiS, c->flag, G, w, rdt, btn, s);
+                else
+                    hipLaunchKernelGGL(k_redis_b<2>, grd, blk, 0, c->stream, phi, c->s0, grad, c->s2, c->s3, c->l1,
+                                       c->l2, c->phiS, c->flag, G, w, rdt, btn, s);
+            }
+            FCFD_CHECK(hipGetLastError());
+            FCFD_TRY(bc_scalar(c, phi));
+        }
+        hipLaunchKernelGGL(k_redis_check, dim3(1), dim3(1), 0, c->stream, c->flag, tol);
+    }
+    FCFD_CHECK(hipMemcpyAsync(c->flag_host, c->flag, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              c->stream));
+    FCFD_CHECK(hipStreamSynchronize(c->stream));
+    c->diag.redis_iters = (int)c->flag_host[2];
+    if (iters) *iters = (int)c->flag_host[2];
+    return FCFD_OK;
+}
+
+}  // namespace fcfd

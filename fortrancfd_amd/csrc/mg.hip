@@ -1,0 +1,771 @@
+// mg.hip — multigrid solve of the variable-coefficient pressure Poisson
+// equation on MI355X.
+//
+// Operator: the reference's ppe_sor_init
+// (`Two phase flows/newOpenMP/3D+IBM/src/ns_solver/ns_ppe_solver.f90:1-65`):
+//   sum_nb c_nb (p_nb - p_P) = f,  c_nb = 2/(rho_P + rho_nb)/h^2 on interior
+//   faces and 0 on wall faces (the reference folds the wall coefficient into
+//   c_P, which is the same homogeneous Neumann operator),
+// with f = div(u*)/dt and the reference's normalisation mean(p) = 0
+// (ns_ppe_solver.f90:316-339).  The reference relaxes this system with an
+// in-place point iteration; here it is solved by V-cycles to a relative
+// residual ||f - A p|| / ||f||.
+//
+// Multigrid: cell-centred, red-black Gauss-Seidel V(nu1,nu2) cycles.  Level 0
+// is the ghosted field layout and takes its face coefficients straight from
+// rho (one read of rho instead of six coefficient arrays).  Coarse levels
+// store face coefficients beta = 1/rho_face averaged over the fine faces that
+// make up a coarse face, restriction averages the children (the reference's
+// 8-cell mean, `DomainDecomposition/3D/src/others/multigrid_solver.f90:68-72`),
+// prolongation is piecewise constant.  An axis is coarsened only while its
+// spacing is within 1.5x of the finest spacing, so anisotropic cells
+// (dz = 2 dx) are first semi-coarsened to isotropy.  The coarsest level is
+// solved in FP64 by Jacobi-preconditioned CG inside one workgroup.
+//
+// Precision modes: FCFD_PPE_FP64 runs the V-cycles on p in FP64.
+// FCFD_PPE_MIXED keeps p, f and the residual in FP64 and runs each V-cycle in
+// FP32 on the error equation A e = r (iterative refinement / defect
+// correction); only the outer residual, the update, the coarsest solve and
+// the norms are FP64.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+
+#include "fcfd_internal.h"
+
+using namespace fcfd;
+
+namespace fcfd {
+
+// layout of one level: interior (nx,ny,nz), ghost width g (gz = 0 in 2D)
+struct Lay {
+    int nx, ny, nz, g, gz;
+    long sx, sxy;
+    __host__ __device__ long id(int i, int j, int k) const {
+        return (long)(i + g) + (long)(j + g) * sx + (long)(k + gz) * sxy;
+    }
+};
+
+static Lay lay_of(const Geom &G) {
+    Lay L;
+    L.nx = G.nx; L.ny = G.ny; L.nz = G.nz; L.g = G.g; L.gz = G.gz; L.sx = G.sx; L.sxy = G.sxy;
+    return L;
+}
+
+static Lay lay_of(const MgLevel &M, int dim) {
+    Lay L;
+    L.nx = M.nx; L.ny = M.ny; L.nz = M.nz; L.g = 1; L.gz = dim == 3 ? 1 : 0; L.sx = M.sx; L.sxy = M.sxy;
+    return L;
+}
+
+struct Inv2 {
+    double x, y, z;  // 1/h^2 per axis
+};
+
+// face coefficient of level 0 between cells with densities a and b
+template <typename T>
+__device__ __forceinline__ T beta_rho(T a, T b) {
+    return (T)2 / (a + b);
+}
+
+// Face coefficients of cell (i,j,k): sum_c = sum c_nb and
+// sum_ce = sum c_nb (e_nb - e_P).  FINE: from rho; otherwise from stored beta
+// arrays whose wall and ghost entries are zero.
+template <typename T, int DIM, bool FINE>
+__device__ __forceinline__ void stencil(const T *__restrict__ e, const T *__restrict__ rho,
+                                        const T *__restrict__ bx, const T *__restrict__ by,
+                                        const T *__restrict__ bz, const Lay &L, Inv2 ih, int i, int j, int k,
+                                        long id, T &sum_c, T &sum_ce) {
+    T cw, ce_, cs, cn, cb = 0, ct = 0;
+    if (FINE) {
+        T r = rho[id];
+        ce_ = i < L.nx - 1 ? beta_rho<T>(r, rho[id + 1]) * (T)ih.x : (T)0;
+        cw = i > 0 ? beta_rho<T>(r, rho[id - 1]) * (T)ih.x : (T)0;
+        cn = j < L.ny - 1 ? beta_rho<T>(r, rho[id + L.sx]) * (T)ih.y : (T)0;
+        cs = j > 0 ? beta_rho<T>(r, rho[id - L.sx]) * (T)ih.y : (T)0;
+        if (DIM == 3) {
+            ct = k < L.nz - 1 ? beta_rho<T>(r, rho[id + L.sxy]) * (T)ih.z : (T)0;
+            cb = k > 0 ? beta_rho<T>(r, rho[id - L.sxy]) * (T)ih.z : (T)0;
+        }
+    } else {
+        ce_ = bx[id] * (T)ih.x;
+        cw = bx[id - 1] * (T)ih.x;
+        cn = by[id] * (T)ih.y;
+        cs = by[id - L.sx] * (T)ih.y;
+        if (DIM == 3) {
+            ct = bz[id] * (T)ih.z;
+            cb = bz[id - L.sxy] * (T)ih.z;
+        }
+    }
+    const T ep = e[id];
+    sum_c = ce_ + cw + cn + cs;
+    sum_ce = ce_ * (e[id + 1] - ep) + cw * (e[id - 1] - ep) + cn * (e[id + L.sx] - ep) + cs * (e[id - L.sx] - ep);
+    if (DIM == 3) {
+        sum_c += ct + cb;
+        sum_ce += ct * (e[id + L.sxy] - ep) + cb * (e[id - L.sxy] - ep);
+    }
+}
+
+// (A e)_P = sum_nb c_nb (e_nb - e_P), formed from the differences: in the
+// gas (rho ~ 1e-3) c_nb is ~1000x larger than in the liquid, and the form
+// sum c e_nb - (sum c) e_P would cancel terms of size c |e| and put the
+// round-off floor of the residual above 1e-10 relative on fine grids.
+template <typename T, int DIM, bool FINE>
+__device__ __forceinline__ T apply_A(const T *__restrict__ e, const T *__restrict__ rho, const T *__restrict__ bx,
+                                     const T *__restrict__ by, const T *__restrict__ bz, const Lay &L, Inv2 ih,
+                                     int i, int j, int k, long id) {
+    const T ep = e[id];
+    T acc;
+    if (FINE) {
+        T r = rho[id];
+        acc = i < L.nx - 1 ? beta_rho<T>(r, rho[id + 1]) * (T)ih.x * (e[id + 1] - ep) : (T)0;
+        if (i > 0) acc += beta_rho<T>(r, rho[id - 1]) * (T)ih.x * (e[id - 1] - ep);
+        if (j < L.ny - 1) acc += beta_rho<T>(r, rho[id + L.sx]) * (T)ih.y * (e[id + L.sx] - ep);
+        if (j > 0) acc += beta_rho<T>(r, rho[id - L.sx]) * (T)ih.y * (e[id - L.sx] - ep);
+        if (DIM == 3) {
+            if (k < L.nz - 1) acc += beta_rho<T>(r, rho[id + L.sxy]) * (T)ih.z * (e[id + L.sxy] - ep);
+            if (k > 0) acc += beta_rho<T>(r, rho[id - L.sxy]) * (T)ih.z * (e[id - L.sxy] - ep);
+        }
+    } else {
+        acc = bx[id] * (T)ih.x * (e[id + 1] - ep) + bx[id - 1] * (T)ih.x * (e[id - 1] - ep) +
+              by[id] * (T)ih.y * (e[id + L.sx] - ep) + by[id - L.sx] * (T)ih.y * (e[id - L.sx] - ep);
+        if (DIM == 3)
+            acc += bz[id] * (T)ih.z * (e[id + L.sxy] - ep) + bz[id - L.sxy] * (T)ih.z * (e[id - L.sxy] - ep);
+    }
+    return acc;
+}
+
+// one colour of a red-black Gauss-Seidel sweep: e_P = (sum c e_nb - f) / sum c.
+// Thread x covers two cells of a row; it updates the one whose colour matches.
+template <typename T, int DIM, bool FINE>
+__global__ void __launch_bounds__(256) k_smooth(T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ rho,
+                                                const T *__restrict__ bx, const T *__restrict__ by,
+                                                const T *__restrict__ bz, Lay L, Inv2 ih, int color) {
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x) + ((j + k + color) & 1);
+    if (i >= L.nx || j >= L.ny) return;
+    const long id = L.id(i, j, k);
+    T sc, sce;
+    stencil<T, DIM, FINE>(e, rho, bx, by, bz, L, ih, i, j, k, id, sc, sce);
+    // correction form e_P += (sum c (e_nb - e_P) - f) / sum c: the Gauss-Seidel
+    // update without cancelling terms of size c |e|
+    if (sc > (T)0) e[id] = e[id] + (sce - f[id]) / sc;
+}
+
+// residual of the level, restricted to the coarse right-hand side by the
+// mean over the children; the coarse error is zeroed in the same pass
+template <typename T, int DIM, bool FINE>
+__global__ void __launch_bounds__(256) k_resid_restrict(const T *__restrict__ e, const T *__restrict__ f,
+                                                        const T *__restrict__ rho, const T *__restrict__ bx,
+                                                        const T *__restrict__ by, const T *__restrict__ bz, Lay L,
+                                                        Inv2 ih, T *__restrict__ fc, T *__restrict__ ec, Lay C,
+                                                        int cx, int cy, int cz) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
+    if (I >= C.nx || J >= C.ny) return;
+    T acc = 0;
+    for (int kk = 0; kk < cz; ++kk)
+        for (int jj = 0; jj < cy; ++jj)
+            for (int ii = 0; ii < cx; ++ii) {
+                int i = I * cx + ii, j = J * cy + jj, k = K * cz + kk;
+                long id = L.id(i, j, k);
+                acc += f[id] - apply_A<T, DIM, FINE>(e, rho, bx, by, bz, L, ih, i, j, k, id);
+            }
+    const long cid = C.id(I, J, K);
+    fc[cid] = acc / (T)(cx * cy * cz);
+    ec[cid] = (T)0;
+}
+
+// e_fine += e_coarse(parent)
+template <typename T>
+__global__ void k_prolong(T *__restrict__ e, const T *__restrict__ ec, Lay L, Lay C, int cx, int cy, int cz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    e[L.id(i, j, k)] += ec[C.id(i / cx, j / cy, k / cz)];
+}
+
+// coarse face coefficients: mean of the fine faces forming each coarse face
+template <int DIM, bool FINE>
+__global__ void k_coef_restrict(const double *__restrict__ rho, const double *__restrict__ bxf,
+                                const double *__restrict__ byf, const double *__restrict__ bzf, Lay L,
+                                double *__restrict__ bxc, double *__restrict__ byc, double *__restrict__ bzc, Lay C,
+                                int cx, int cy, int cz) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
+    if (I >= C.nx || J >= C.ny) return;
+    const long cid = C.id(I, J, K);
+    // x faces at fine i = I*cx + cx - 1
+    {
+        double s = 0.0;
+        int i = I * cx + cx - 1;
+        for (int kk = 0; kk < cz; ++kk)
+            for (int jj = 0; jj < cy; ++jj) {
+                int j = J * cy + jj, k = K * cz + kk;
+                long id = L.id(i, j, k);
+                double b;
+                if (FINE) b = i < L.nx - 1 ? 2.0 / (rho[id] + rho[id + 1]) : 0.0;
+                else b = bxf[id];
+                s += b;
+            }
+        bxc[cid] = I < C.nx - 1 ? s / (cy * cz) : 0.0;
+    }
+    {
+        double s = 0.0;
+        int j = J * cy + cy - 1;
+        for (int kk = 0; kk < cz; ++kk)
+            for (int ii = 0; ii < cx; ++ii) {
+                int i = I * cx + ii, k = K * cz + kk;
+                long id = L.id(i, j, k);
+                double b;
+                if (FINE) b = j < L.ny - 1 ? 2.0 / (rho[id] + rho[id + L.sx]) : 0.0;
+                else b = byf[id];
+                s += b;
+            }
+        byc[cid] = J < C.ny - 1 ? s / (cx * cz) : 0.0;
+    }
+    if (DIM == 3) {
+        double s = 0.0;
+        int k = K * cz + cz - 1;
+        for (int jj = 0; jj < cy; ++jj)
+            for (int ii = 0; ii < cx; ++ii) {
+                int i = I * cx + ii, j = J * cy + jj;
+                long id = L.id(i, j, k);
+                double b;
+                if (FINE) b = k < L.nz - 1 ? 2.0 / (rho[id] + rho[id + L.sxy]) : 0.0;
+                else b = bzf[id];
+                s += b;
+            }
+        bzc[cid] = K < C.nz - 1 ? s / (cx * cy) : 0.0;
+    }
+}
+
+__global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, long n) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = (float)a[t];
+}
+
+// FP64 residual of level 0 with its squared norm folded into block partials;
+// with r32 != null the residual is also stored in FP32 (mixed mode rhs) and
+// the FP32 error is zeroed.  Also accumulates sum f^2 when fnorm is set.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_resid_norm(const double *__restrict__ p, const double *__restrict__ f,
+                                                    const double *__restrict__ rho, Lay L, Inv2 ih,
+                                                    float *__restrict__ r32, float *__restrict__ e32,
+                                                    double *__restrict__ partial, int with_f) {
+    __shared__ double s0[256], s1[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    double rr = 0.0, ff = 0.0;
+    if (i < L.nx && j < L.ny) {
+        long id = L.id(i, j, k);
+        double r = f[id] - apply_A<double, DIM, true>(p, rho, nullptr, nullptr, nullptr, L, ih, i, j, k, id);
+        rr = r * r;
+        if (with_f) ff = f[id] * f[id];
+        if (r32) {
+            r32[id] = (float)r;
+            e32[id] = 0.0f;
+        }
+    }
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    s0[t] = rr;
+    s1[t] = ff;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s0[t] += s0[t + w];
+            s1[t] += s1[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+        long nb = (long)gridDim.x * gridDim.y * gridDim.z;
+        partial[b] = s0[0];
+        partial[nb + b] = s1[0];
+    }
+}
+
+// sum the block partials of k_resid_norm / k_sum (fixed order)
+__global__ void k_sum_partials(const double *__restrict__ partial, long nb, int ns, double *__restrict__ out) {
+    __shared__ double sh[256];
+    for (int s = 0; s < ns; ++s) {
+        double a = 0.0;
+        for (long b = threadIdx.x; b < nb; b += 256) a += partial[s * nb + b];
+        sh[threadIdx.x] = a;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[s] = sh[0];
+        __syncthreads();
+    }
+}
+
+// sum of an interior field into block partials
+__global__ void k_sum(const double *__restrict__ a, Lay L, double *__restrict__ partial) {
+    __shared__ double s0[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    double v = 0.0;
+    if (i < L.nx && j < L.ny) v = a[L.id(i, j, k)];
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    s0[t] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s0[t] += s0[t + w];
+        __syncthreads();
+    }
+    if (t == 0) partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = s0[0];
+}
+
+// a -= scale * s[0]  (interior)
+__global__ void k_sub_scalar(double *__restrict__ a, Lay L, const double *__restrict__ s, double scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    a[L.id(i, j, k)] -= s[0] * scale;
+}
+
+// copy f into the working rhs minus its mean (compatibility of the Neumann
+// problem): out = f - s[0]*scale
+__global__ void k_center_copy(const double *__restrict__ f, double *__restrict__ out, Lay L,
+                              const double *__restrict__ s, double scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long id = L.id(i, j, k);
+    out[id] = f[id] - s[0] * scale;
+}
+
+// p += e (FP32 correction into the FP64 solution)
+__global__ void k_add_f32(double *__restrict__ p, const float *__restrict__ e, Lay L) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long id = L.id(i, j, k);
+    p[id] += (double)e[id];
+}
+
+// Coarsest level: Jacobi-preconditioned CG in FP64 inside one workgroup.
+// The rhs is projected to zero mean; the solution is returned with zero mean.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(1024) k_coarse_cg(T *__restrict__ eout, const T *__restrict__ fin,
+                                                    const T *__restrict__ bxT, const T *__restrict__ byT,
+                                                    const T *__restrict__ bzT, const double *__restrict__ bx,
+                                                    const double *__restrict__ by, const double *__restrict__ bz,
+                                                    double *__restrict__ ws, Lay L, Inv2 ih, int maxit, double rtol) {
+    (void)bxT; (void)byT; (void)bzT;
+    __shared__ double red[1024];
+    const int nt = blockDim.x, t = threadIdx.x;
+    const long n = (long)L.nx * L.ny * L.nz;
+    double *x = ws, *r = ws + n, *z = ws + 2 * n, *q = ws + 3 * n, *Aq = ws + 4 * n, *dg = ws + 5 * n;
+    auto block_sum = [&](double v) -> double {
+        red[t] = v;
+        __syncthreads();
+        for (int w = nt / 2; w > 0; w >>= 1) {
+            if (t < w) red[t] += red[t + w];
+            __syncthreads();
+        }
+        double s = red[0];
+        __syncthreads();
+        return s;
+    };
+    auto coords = [&](long c, int &i, int &j, int &k) {
+        i = (int)(c % L.nx);
+        long rr = c / L.nx;
+        j = (int)(rr % L.ny);
+        k = (int)(rr / L.ny);
+    };
+    // Apply -A (positive semi-definite) to v at cell c
+    auto negA = [&](const double *v, long c) -> double {
+        int i, j, k;
+        coords(c, i, j, k);
+        long id = L.id(i, j, k);
+        double ce_ = bx[id] * ih.x, cw = bx[id - 1] * ih.x, cn = by[id] * ih.y, cs = by[id - L.sx] * ih.y;
+        double sc = ce_ + cw + cn + cs;
+        double acc = sc * v[c];
+        if (i < L.nx - 1) acc -= ce_ * v[c + 1];
+        if (i > 0) acc -= cw * v[c - 1];
+        if (j < L.ny - 1) acc -= cn * v[c + L.nx];
+        if (j > 0) acc -= cs * v[c - L.nx];
+        if (DIM == 3) {
+            double ct = bz[id] * ih.z, cb = bz[id - L.sxy] * ih.z;
+            long pz = (long)L.nx * L.ny;
+            if (k < L.nz - 1) acc -= ct * v[c + pz];
+            if (k > 0) acc -= cb * v[c - pz];
+            sc += ct + cb;
+            acc += (ct + cb) * v[c];
+        }
+        return acc;
+    };
+    // rhs mean
+    double s = 0.0;
+    for (long c = t; c < n; c += nt) {
+        int i, j, k;
+        coords(c, i, j, k);
+        s += (double)fin[L.id(i, j, k)];
+    }
+    const double mean = block_sum(s) / (double)n;
+    double bb = 0.0;
+    for (long c = t; c < n; c += nt) {
+        int i, j, k;
+        coords(c, i, j, k);
+        long id = L.id(i, j, k);
+        double b = -((double)fin[id] - mean);   // solve -A x = -f
+        x[c] = 0.0;
+        r[c] = b;
+        double d = bx[id] * ih.x + bx[id - 1] * ih.x + by[id] * ih.y + by[id - L.sx] * ih.y;
+        if (DIM == 3) d += bz[id] * ih.z + bz[id - L.sxy] * ih.z;
+        dg[c] = d > 0.0 ? 1.0 / d : 0.0;
+        bb += b * b;
+    }
+    bb = block_sum(bb);
+    double rz = 0.0;
+    for (long c = t; c < n; c += nt) {
+        z[c] = r[c] * dg[c];
+        q[c] = z[c];
+        rz += r[c] * z[c];
+    }
+    rz = block_sum(rz);
+    const double stop = rtol * rtol * bb;
+    double rr = bb;
+    for (int it = 0; it < maxit && rr > stop && bb > 0.0; ++it) {
+        double qAq = 0.0;
+        for (long c = t; c < n; c += nt) {
+            Aq[c] = negA(q, c);
+            qAq += q[c] * Aq[c];
+        }
+        qAq = block_sum(qAq);
+        if (!(qAq > 0.0)) break;
+        const double alpha = rz / qAq;
+        double rn = 0.0, rz2 = 0.0;
+        for (long c = t; c < n; c += nt) {
+            x[c] += alpha * q[c];
+            r[c] -= alpha * Aq[c];
+            z[c] = r[c] * dg[c];
+            rn += r[c] * r[c];
+            rz2 += r[c] * z[c];
+        }
+        rn = block_sum(rn);
+        rz2 = block_sum(rz2);
+        const double beta = rz2 / rz;
+        rz = rz2;
+        rr = rn;
+        for (long c = t; c < n; c += nt) q[c] = z[c] + beta * q[c];
+        __syncthreads();
+    }
+    double xm = 0.0;
+    for (long c = t; c < n; c += nt) xm += x[c];
+    xm = block_sum(xm) / (double)n;
+    for (long c = t; c < n; c += nt) {
+        int i, j, k;
+        coords(c, i, j, k);
+        eout[L.id(i, j, k)] = (T)(x[c] - xm);
+    }
+}
+
+// ------------------------------------------------------------ host side
+
+static dim3 grd3(int nx, int ny, int nz, dim3 blk) {
+    return dim3((nx + blk.x - 1) / blk.x, (ny + blk.y - 1) / blk.y, nz);
+}
+
+static Inv2 inv2(double hx, double hy, double hz) {
+    Inv2 r;
+    r.x = 1.0 / (hx * hx);
+    r.y = 1.0 / (hy * hy);
+    r.z = 1.0 / (hz * hz);
+    return r;
+}
+
+int mg_free(fcfd_ctx *c) {
+    for (auto &M : c->mg) {
+        void *ptrs[] = {M.e64, M.f64, M.r64, M.bx64, M.by64, M.bz64, M.e32, M.f32, M.r32, M.bx32, M.by32, M.bz32};
+        for (void *p : ptrs)
+            if (p) hipFree(p);
+    }
+    c->mg.clear();
+    if (c->rho32) hipFree(c->rho32);
+    if (c->mg_scalars) hipFree(c->mg_scalars);
+    c->rho32 = nullptr;
+    c->mg_scalars = nullptr;
+    c->mg_ready = 0;
+    return FCFD_OK;
+}
+
+template <typename T>
+static int alloc0(T **p, long n) {
+    if (hipMalloc((void **)p, sizeof(T) * n) != hipSuccess) return FCFD_ERR_HIP;
+    if (hipMemset(*p, 0, sizeof(T) * n) != hipSuccess) return FCFD_ERR_HIP;
+    return FCFD_OK;
+}
+
+// Build the level hierarchy once per grid (sizes only).  Level 0 describes
+// the field layout; its arrays are the context's fields (FP64) and, for the
+// mixed mode, e32/f32/rho32 in the same ghosted layout.
+static int mg_build(fcfd_ctx *c) {
+    const Geom &G = c->G;
+    mg_free(c);
+    MgLevel L0 = {};
+    L0.nx = G.nx; L0.ny = G.ny; L0.nz = G.nz;
+    L0.hx = G.dx; L0.hy = G.dy; L0.hz = G.dim == 3 ? G.dz : 1.0;
+    L0.sx = G.sx; L0.sxy = G.sxy; L0.ntot = G.ntot;
+    c->mg.push_back(L0);
+    const double hmin0 = G.dim == 3 ? std::min({G.dx, G.dy, G.dz}) : std::min(G.dx, G.dy);
+    (void)hmin0;
+    for (;;) {
+        MgLevel &F = c->mg.back();
+        long ncell = (long)F.nx * F.ny * F.nz;
+        if (ncell <= 512) break;
+        double hmin = G.dim == 3 ? std::min({F.hx, F.hy, F.hz}) : std::min(F.hx, F.hy);
+        int cf[3] = {1, 1, 1};
+        const int n[3] = {F.nx, F.ny, F.nz};
+        const double h[3] = {F.hx, F.hy, F.hz};
+        bool any = false;
+        for (int d = 0; d < G.dim; ++d)
+            if (n[d] % 2 == 0 && n[d] >= 4 && h[d] <= 1.5 * hmin) { cf[d] = 2; any = true; }
+        if (!any) break;
+        F.cx = cf[0]; F.cy = cf[1]; F.cz = cf[2];
+        MgLevel C = {};
+        C.nx = F.nx / cf[0]; C.ny = F.ny / cf[1]; C.nz = G.dim == 3 ? F.nz / cf[2] : 1;
+        C.hx = F.hx * cf[0]; C.hy = F.hy * cf[1]; C.hz = F.hz * cf[2];
+        C.sx = C.nx + 2;
+        C.sxy = C.sx * (C.ny + 2);
+        C.ntot = C.sxy * (C.nz + (G.dim == 3 ? 2 : 0));
+        c->mg.push_back(C);
+    }
+    c->mg.back().cx = c->mg.back().cy = c->mg.back().cz = 1;
+    for (size_t l = 1; l < c->mg.size(); ++l) {
+        MgLevel &M = c->mg[l];
+        long n = M.ntot;
+        FCFD_TRY(alloc0(&M.e64, n)); FCFD_TRY(alloc0(&M.f64, n));
+        FCFD_TRY(alloc0(&M.bx64, n)); FCFD_TRY(alloc0(&M.by64, n));
+        if (G.dim == 3) FCFD_TRY(alloc0(&M.bz64, n));
+        FCFD_TRY(alloc0(&M.e32, n)); FCFD_TRY(alloc0(&M.f32, n));
+        FCFD_TRY(alloc0(&M.bx32, n)); FCFD_TRY(alloc0(&M.by32, n));
+        if (G.dim == 3) FCFD_TRY(alloc0(&M.bz32, n));
+    }
+    MgLevel &M0 = c->mg[0];
+    FCFD_TRY(alloc0(&M0.e32, G.ntot));
+    FCFD_TRY(alloc0(&M0.f32, G.ntot));
+    FCFD_TRY(alloc0(&c->rho32, G.ntot));
+    FCFD_TRY(alloc0(&M0.f64, G.ntot));   // zero-mean working rhs
+    // coarse CG workspace (6 vectors) + scalars
+    long ncoarse = (long)c->mg.back().nx * c->mg.back().ny * c->mg.back().nz;
+    FCFD_TRY(alloc0(&c->mg_scalars, 64 + 6 * ncoarse));
+    return FCFD_OK;
+}
+
+// ppe_sor_init: coefficients of every level from rho%now
+int mg_setup(fcfd_ctx *c) {
+    const Geom &G = c->G;
+    if (c->mg.empty()) FCFD_TRY(mg_build(c));
+    dim3 blk(64, 4);
+    for (size_t l = 1; l < c->mg.size(); ++l) {
+        MgLevel &F = c->mg[l - 1], &C = c->mg[l];
+        Lay LF = l == 1 ? lay_of(G) : lay_of(F, G.dim), LC = lay_of(C, G.dim);
+        dim3 g = grd3(C.nx, C.ny, C.nz, blk);
+        if (l == 1) {
+            if (G.dim == 3)
+                hipLaunchKernelGGL((k_coef_restrict<3, true>), g, blk, 0, c->stream, c->f[FCFD_RHO], nullptr, nullptr,
+                                   nullptr, LF, C.bx64, C.by64, C.bz64, LC, F.cx, F.cy, F.cz);
+            else
+                hipLaunchKernelGGL((k_coef_restrict<2, true>), g, blk, 0, c->stream, c->f[FCFD_RHO], nullptr, nullptr,
+                                   nullptr, LF, C.bx64, C.by64, nullptr, LC, F.cx, F.cy, 1);
+        } else {
+            if (G.dim == 3)
+                hipLaunchKernelGGL((k_coef_restrict<3, false>), g, blk, 0, c->stream, nullptr, F.bx64, F.by64, F.bz64,
+                                   LF, C.bx64, C.by64, C.bz64, LC, F.cx, F.cy, F.cz);
+            else
+                hipLaunchKernelGGL((k_coef_restrict<2, false>), g, blk, 0, c->stream, nullptr, F.bx64, F.by64,
+                                   nullptr, LF, C.bx64, C.by64, nullptr, LC, F.cx, F.cy, 1);
+        }
+        FCFD_CHECK(hipGetLastError());
+        long n = C.ntot;
+        unsigned nb = (unsigned)((n + 255) / 256);
+        hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, c->stream, C.bx64, C.bx32, n);
+        hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, c->stream, C.by64, C.by32, n);
+        if (G.dim == 3) hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, c->stream, C.bz64, C.bz32, n);
+    }
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((G.ntot + 255) / 256)), dim3(256), 0, c->stream, c->f[FCFD_RHO],
+                       c->rho32, G.ntot);
+    FCFD_CHECK(hipGetLastError());
+    c->mg_ready = 1;
+    return FCFD_OK;
+}
+
+// One V-cycle from level l with error e_l (level 0: the solution itself)
+// and rhs f_l of precision T.
+template <typename T, int DIM>
+static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1, int nu2) {
+    const Geom &G = c->G;
+    dim3 blk(64, 4);
+    const int nl = (int)c->mg.size();
+    auto arrays = [&](int lv, T *&e, const T *&f, const T *&bx, const T *&by, const T *&bz) {
+        MgLevel &M = c->mg[lv];
+        if (lv == 0) { e = e0; f = f0; bx = by = bz = nullptr; return; }
+        if constexpr (sizeof(T) == 8) {
+            e = (T *)M.e64; f = (const T *)M.f64; bx = (const T *)M.bx64; by = (const T *)M.by64; bz = (const T *)M.bz64;
+        } else {
+            e = (T *)M.e32; f = (const T *)M.f32; bx = (const T *)M.bx32; by = (const T *)M.by32; bz = (const T *)M.bz32;
+        }
+    };
+    auto smooth = [&](int lv, int sweeps, bool reverse) -> int {
+        MgLevel &M = c->mg[lv];
+        Lay L = lv == 0 ? lay_of(G) : lay_of(M, G.dim);
+        Inv2 ih = inv2(M.hx, M.hy, M.hz);
+        T *e; const T *f, *bx, *by, *bz;
+        arrays(lv, e, f, bx, by, bz);
+        dim3 g((unsigned)(((M.nx + 1) / 2 + blk.x - 1) / blk.x), (M.ny + blk.y - 1) / blk.y, M.nz);
+        for (int s = 0; s < sweeps; ++s)
+            for (int h = 0; h < 2; ++h) {
+                ProfScope ps_(c, lv == 0 ? R_SMOOTH0 : R_SMOOTHC);
+                int color = reverse ? 1 - h : h;
+                if (lv == 0)
+                    hipLaunchKernelGGL((k_smooth<T, DIM, true>), g, blk, 0, c->stream, e, f, rho0, bx, by, bz, L, ih,
+                                       color);
+                else
+                    hipLaunchKernelGGL((k_smooth<T, DIM, false>), g, blk, 0, c->stream, e, f, (const T *)nullptr, bx,
+                                       by, bz, L, ih, color);
+            }
+        FCFD_CHECK(hipGetLastError());
+        return FCFD_OK;
+    };
+    // down
+    for (int lv = l; lv < nl - 1; ++lv) {
+        FCFD_TRY(smooth(lv, nu1, false));
+        MgLevel &M = c->mg[lv], &C = c->mg[lv + 1];
+        Lay L = lv == 0 ? lay_of(G) : lay_of(M, G.dim), LC = lay_of(C, G.dim);
+        Inv2 ih = inv2(M.hx, M.hy, M.hz);
+        T *e; const T *f, *bx, *by, *bz;
+        arrays(lv, e, f, bx, by, bz);
+        T *ec; const T *fc_, *a, *b, *cc;
+        arrays(lv + 1, ec, fc_, a, b, cc);
+        dim3 g = grd3(C.nx, C.ny, C.nz, blk);
+        ProfScope ps_(c, lv == 0 ? R_RESTRICT0 : R_RESTRICTC);
+        if (lv == 0)
+            hipLaunchKernelGGL((k_resid_restrict<T, DIM, true>), g, blk, 0, c->stream, e, f, rho0, bx, by, bz, L, ih,
+                               (T *)fc_, ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
+        else
+            hipLaunchKernelGGL((k_resid_restrict<T, DIM, false>), g, blk, 0, c->stream, e, f, (const T *)nullptr, bx,
+                               by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
+        FCFD_CHECK(hipGetLastError());
+    }
+    // coarsest: FP64 CG
+    {
+        MgLevel &M = c->mg[nl - 1];
+        Lay L = lay_of(M, G.dim);
+        Inv2 ih = inv2(M.hx, M.hy, M.hz);
+        T *e; const T *f, *bx, *by, *bz;
+        arrays(nl - 1, e, f, bx, by, bz);
+        long n = (long)M.nx * M.ny * M.nz;
+        ProfScope ps_(c, R_COARSE);
+        hipLaunchKernelGGL((k_coarse_cg<T, DIM>), dim3(1), dim3(n >= 1024 ? 1024 : 256), 0, c->stream, e, f, bx, by,
+                           bz, M.bx64, M.by64, M.bz64, c->mg_scalars + 64, L, ih, (int)std::min<long>(4 * n, 4000),
+                           1e-13);
+        FCFD_CHECK(hipGetLastError());
+    }
+    // up
+    for (int lv = nl - 2; lv >= l; --lv) {
+        MgLevel &M = c->mg[lv], &C = c->mg[lv + 1];
+        Lay L = lv == 0 ? lay_of(G) : lay_of(M, G.dim), LC = lay_of(C, G.dim);
+        T *e; const T *f, *bx, *by, *bz;
+        arrays(lv, e, f, bx, by, bz);
+        T *ec; const T *fc_, *a, *b, *cc;
+        arrays(lv + 1, ec, fc_, a, b, cc);
+        {
+            ProfScope ps_(c, lv == 0 ? R_PROLONG0 : R_PROLONGC);
+            hipLaunchKernelGGL((k_prolong<T>), grd3(M.nx, M.ny, M.nz, blk), blk, 0, c->stream, e, (const T *)ec, L,
+                               LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
+            FCFD_CHECK(hipGetLastError());
+        }
+        FCFD_TRY(smooth(lv, nu2, true));
+    }
+    return FCFD_OK;
+}
+
+// ||f - A p||^2 (and ||f||^2) of level 0 in FP64 -> host
+template <int DIM>
+static int resid_norm(fcfd_ctx *c, const double *p, const double *f, float *r32, float *e32, int with_f,
+                      double &rr, double &ff) {
+    const Geom &G = c->G;
+    dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
+    long nb = (long)g.x * g.y * g.z;   // the partial buffer holds 2*nb (sized at creation)
+    Inv2 ih = inv2(G.dx, G.dy, G.dz);
+    {
+        ProfScope ps_(c, R_RESID0);
+        hipLaunchKernelGGL(k_resid_norm<DIM>, g, blk, 0, c->stream, p, f, c->f[FCFD_RHO], lay_of(G), ih, r32, e32,
+                           c->red.partial, with_f);
+    }
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->red.partial, nb, 2, c->mg_scalars);
+    FCFD_CHECK(hipGetLastError());
+    FCFD_CHECK(hipMemcpyAsync(c->red.host, c->mg_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FCFD_CHECK(hipStreamSynchronize(c->stream));
+    rr = c->red.host[0];
+    ff = c->red.host[1];
+    return FCFD_OK;
+}
+
+// mean of an interior FP64 field into mg_scalars[4] (device)
+static int field_mean(fcfd_ctx *c, const double *a) {
+    const Geom &G = c->G;
+    dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
+    long nb = (long)g.x * g.y * g.z;
+    hipLaunchKernelGGL(k_sum, g, blk, 0, c->stream, a, lay_of(G), c->red.partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->red.partial, nb, 1, c->mg_scalars + 4);
+    FCFD_CHECK(hipGetLastError());
+    return FCFD_OK;
+}
+
+template <int DIM>
+static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel) {
+    const Geom &G = c->G;
+    dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
+    MgLevel &M0 = c->mg[0];
+    const double ncell = (double)G.nx * (double)G.ny * (double)G.nz;
+    double *p = c->f[FCFD_P];
+    double *f = M0.f64;
+    // compatibility: f = src - mean(src)
+    FCFD_TRY(field_mean(c, c->f[FCFD_PPE_SRC]));
+    hipLaunchKernelGGL(k_center_copy, g, blk, 0, c->stream, c->f[FCFD_PPE_SRC], f, lay_of(G), c->mg_scalars + 4,
+                       1.0 / ncell);
+    FCFD_CHECK(hipGetLastError());
+    double rr, ff;
+    const int maxc = c->pr.ppe_max_cycles > 0 ? c->pr.ppe_max_cycles : 200;
+    int k = 0, rc = FCFD_OK;
+    const bool mixed = precision == FCFD_PPE_MIXED;
+    FCFD_TRY(resid_norm<DIM>(c, p, f, mixed ? M0.f32 : nullptr, M0.e32, 1, rr, ff));
+    const double fn = ff > 0.0 ? sqrt(ff) : 1.0;
+    double r = sqrt(rr) / fn;
+    if (!std::isfinite(r)) return FCFD_ERR_NAN;
+    static const bool verbose = getenv("FCFD_MG_VERBOSE") != nullptr;
+    if (verbose) fprintf(stderr, "fcfd mg[%s]: cycle 0 rel %.3e\n", mixed ? "mixed" : "fp64", r);
+    while (r > rel_tol && k < maxc) {
+        if (mixed) {
+            FCFD_TRY((vcycle<float, DIM>(c, 0, M0.e32, M0.f32, c->rho32, 2, 2)));
+            hipLaunchKernelGGL(k_add_f32, g, blk, 0, c->stream, p, M0.e32, lay_of(G));
+            FCFD_CHECK(hipGetLastError());
+            FCFD_TRY(bc_scalar(c, p));
+        } else {
+            FCFD_TRY((vcycle<double, DIM>(c, 0, p, f, c->f[FCFD_RHO], 2, 2)));
+        }
+        ++k;
+        FCFD_TRY(resid_norm<DIM>(c, p, f, mixed ? M0.f32 : nullptr, M0.e32, 0, rr, ff));
+        r = sqrt(rr) / fn;
+        if (verbose) fprintf(stderr, "fcfd mg[%s]: cycle %d rel %.3e\n", mixed ? "mixed" : "fp64", k, r);
+        if (!std::isfinite(r)) return FCFD_ERR_NAN;
+    }
+    if (r > rel_tol) rc = FCFD_ERR_NOT_CONVERGED;
+    // mean(p) = 0 as the reference's iteration enforces, then ghost copy
+    FCFD_TRY(field_mean(c, p));
+    hipLaunchKernelGGL(k_sub_scalar, g, blk, 0, c->stream, p, lay_of(G), c->mg_scalars + 4, 1.0 / ncell);
+    FCFD_CHECK(hipGetLastError());
+    FCFD_TRY(bc_scalar(c, p));
+    if (cycles) *cycles = k;
+    if (rel) *rel = r;
+    c->diag.ppe_cycles = k;
+    c->diag.ppe_rel_resid = r;
+    return rc;
+}
+
+int ppe_solve(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel) {
+    if (!c->mg_ready) FCFD_TRY(mg_setup(c));
+    if (c->G.dim == 3) return ppe_solve_t<3>(c, rel_tol, precision, cycles, rel);
+    return ppe_solve_t<2>(c, rel_tol, precision, cycles, rel);
+}
+
+}  // namespace fcfd

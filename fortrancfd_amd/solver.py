@@ -1,0 +1,242 @@
+"""Host-side mirror of the reference's `type(manager) :: p` over libfcfd.
+
+`Solver` owns one libfcfd context (device-resident fields in the reference
+layout) and exposes the reference's hot-path subroutines under their own
+names: level_set_rk3_solver, level_set_rk3_redis(btn),
+mass_preserving_level_set, ls_funs, rho_mu, ls_mv, surface_norms, curv,
+ppe_sor_init, ppe_sor_solver, ns_momentum_correction, ns_ab_predictor,
+node_vel, ns_solver, switch, interface_solver, plus step() for one pass of
+the main loop (`Two phase flows/newOpenMP/3D+IBM/src/main.f90:15-27`).
+
+Errors: the reference `stop`s on a diverging pressure solve
+(`.../ns_solver/ns_ppe_solver.f90:352-355`); here every non-zero status of
+the C ABI raises FcfdError (NOT_CONVERGED included unless allow_unconverged).
+There is no CPU fallback: without the built library or a HIP device the
+constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .types import FIELD_ID, PPE_FP64, STATUS, Case, FcfdDiag
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfcfd.so")
+
+_lib = None
+
+
+class FcfdError(RuntimeError):
+    def __init__(self, fn: str, code: int):
+        super().__init__(f"{fn} failed: status {code} ({STATUS.get(code, '?')})")
+        self.code = code
+
+
+def lib():
+    """Load libfcfd.so (built by fortrancfd_amd/build.py); raises when absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, dbl = C.c_void_p, C.c_int, C.c_double
+    sig = {
+        "fcfd_create": [vp, vp, C.POINTER(vp)],
+        "fcfd_destroy": [vp],
+        "fcfd_set_params": [vp, vp],
+        "fcfd_push": [vp, i32, vp], "fcfd_pull": [vp, i32, vp],
+        "fcfd_push_device": [vp, i32, vp], "fcfd_pull_device": [vp, i32, vp],
+        "fcfd_sync": [vp], "fcfd_get_diag": [vp, vp],
+        "fcfd_set_time": [vp, dbl, i32], "fcfd_set_imass": [vp, dbl],
+        "fcfd_bc": [vp, i32], "fcfd_velbc": [vp],
+        "fcfd_ls_funs": [vp], "fcfd_rho_mu": [vp], "fcfd_ls_mv": [vp, vp, vp],
+        "fcfd_level_set_rk3_solver": [vp], "fcfd_level_set_rk3_redis": [vp, i32, vp],
+        "fcfd_mass_preserving_level_set": [vp], "fcfd_surface_norms": [vp], "fcfd_curv": [vp],
+        "fcfd_ppe_sor_init": [vp], "fcfd_ppe_sor_solver": [vp, dbl, i32, vp, vp],
+        "fcfd_ppe_solve_rhs": [vp, dbl, i32, vp, vp],
+        "fcfd_ns_momentum_correction": [vp], "fcfd_ns_ab_predictor": [vp],
+        "fcfd_node_vel": [vp], "fcfd_ns_check_convergence_div": [vp], "fcfd_ns_solver": [vp],
+        "fcfd_switch": [vp], "fcfd_interface_solver": [vp], "fcfd_step": [vp],
+        "fcfd_problem_init": [vp, i32],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.fcfd_field_size.argtypes = [vp]
+    L.fcfd_field_size.restype = C.c_int64
+    L.fcfd_prof_nregions.argtypes = []
+    L.fcfd_prof_nregions.restype = C.c_int
+    L.fcfd_prof_region_name.argtypes = [C.c_int]
+    L.fcfd_prof_region_name.restype = C.c_char_p
+    L.fcfd_prof_enable.argtypes = [vp, i32]
+    L.fcfd_prof_enable.restype = C.c_int
+    L.fcfd_prof_read.argtypes = [vp, vp, vp, i32]
+    L.fcfd_prof_read.restype = C.c_int
+    _lib = L
+    return L
+
+
+class Solver:
+    """One grid on the current HIP device (the reference's `p`)."""
+
+    def __init__(self, case: Case, allow_unconverged: bool = False):
+        self.case = case
+        self.allow_unconverged = allow_unconverged
+        self._grid = case.grid_struct()
+        self._params = case.param_struct()
+        h = C.c_void_p()
+        self._check("fcfd_create", lib().fcfd_create(C.byref(self._grid), C.byref(self._params), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fcfd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, fn: str, rc: int):
+        if rc == 3 and getattr(self, "allow_unconverged", False):
+            return rc
+        if rc != 0:
+            raise FcfdError(fn, rc)
+        return rc
+
+    def _call(self, name: str, *args):
+        return self._check(name, getattr(lib(), name)(self.h, *args))
+
+    # ---- parameters / fields -------------------------------------------
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.case, k, v)
+        self._params = self.case.param_struct()
+        self._call("fcfd_set_params", C.byref(self._params))
+
+    def push(self, name: str, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        if a.size != self.case.size:
+            raise ValueError(f"{name}: {a.size} values, field holds {self.case.size}")
+        self._call("fcfd_push", FIELD_ID[name], a.ctypes.data)
+
+    def pull(self, name: str) -> np.ndarray:
+        a = np.empty(self.case.size, dtype=np.float64)
+        self._call("fcfd_pull", FIELD_ID[name], a.ctypes.data)
+        return a.reshape(self.case.shape)
+
+    def push_device(self, name: str, ptr: int) -> None:
+        self._call("fcfd_push_device", FIELD_ID[name], C.c_void_p(ptr))
+
+    def pull_device(self, name: str, ptr: int) -> None:
+        self._call("fcfd_pull_device", FIELD_ID[name], C.c_void_p(ptr))
+
+    def sync(self) -> None:
+        self._call("fcfd_sync")
+
+    def diag(self) -> dict:
+        d = FcfdDiag()
+        self._call("fcfd_get_diag", C.byref(d))
+        return d.as_dict()
+
+    # ---- built-in kernel timing ----------------------------------------
+    def prof_enable(self, on: bool = True) -> None:
+        self._call("fcfd_prof_enable", 1 if on else 0)
+
+    def prof_read(self) -> dict:
+        """{region: (milliseconds, launches)} since prof_enable."""
+        n = lib().fcfd_prof_nregions()
+        ms = np.zeros(n, dtype=np.float64)
+        cnt = np.zeros(n, dtype=np.int64)
+        self._call("fcfd_prof_read", ms.ctypes.data, cnt.ctypes.data, n)
+        return {lib().fcfd_prof_region_name(r).decode(): (float(ms[r]), int(cnt[r])) for r in range(n)}
+
+    def set_time(self, time: float, it: int) -> None:
+        self._call("fcfd_set_time", C.c_double(time), it)
+
+    def set_imass(self, m: float) -> None:
+        self.case.imass = m
+        self._call("fcfd_set_imass", C.c_double(m))
+
+    # ---- reference subroutines -------------------------------------------
+    def bc(self, name: str) -> None:
+        self._call("fcfd_bc", FIELD_ID[name])
+
+    def velbc(self) -> None:
+        self._call("fcfd_velbc")
+
+    def ls_funs(self) -> None:
+        self._call("fcfd_ls_funs")
+
+    def rho_mu(self) -> None:
+        self._call("fcfd_rho_mu")
+
+    def ls_mv(self):
+        m, v = C.c_double(), C.c_double()
+        self._call("fcfd_ls_mv", C.byref(m), C.byref(v))
+        return m.value, v.value
+
+    def level_set_rk3_solver(self) -> None:
+        self._call("fcfd_level_set_rk3_solver")
+
+    def level_set_rk3_redis(self, btn: int) -> int:
+        it = C.c_int()
+        self._call("fcfd_level_set_rk3_redis", btn, C.byref(it))
+        return it.value
+
+    def mass_preserving_level_set(self) -> None:
+        self._call("fcfd_mass_preserving_level_set")
+
+    def surface_norms(self) -> None:
+        self._call("fcfd_surface_norms")
+
+    def curv(self) -> None:
+        self._call("fcfd_curv")
+
+    def ppe_sor_init(self) -> None:
+        self._call("fcfd_ppe_sor_init")
+
+    def ppe_sor_solver(self, rel_tol: float = 1e-10, precision: int = PPE_FP64):
+        cyc, rel = C.c_int(), C.c_double()
+        self._call("fcfd_ppe_sor_solver", C.c_double(rel_tol), precision, C.byref(cyc), C.byref(rel))
+        return cyc.value, rel.value
+
+    def ppe_solve_rhs(self, rel_tol: float = 1e-10, precision: int = PPE_FP64):
+        """A p = PPE_SRC with the current operator, from the current P."""
+        cyc, rel = C.c_int(), C.c_double()
+        self._call("fcfd_ppe_solve_rhs", C.c_double(rel_tol), precision, C.byref(cyc), C.byref(rel))
+        return cyc.value, rel.value
+
+    def ns_momentum_correction(self) -> None:
+        self._call("fcfd_ns_momentum_correction")
+
+    def ns_ab_predictor(self) -> None:
+        self._call("fcfd_ns_ab_predictor")
+
+    def node_vel(self) -> None:
+        self._call("fcfd_node_vel")
+
+    def ns_check_convergence_div(self) -> None:
+        self._call("fcfd_ns_check_convergence_div")
+
+    def ns_solver(self) -> None:
+        self._call("fcfd_ns_solver")
+
+    def switch(self) -> None:
+        self._call("fcfd_switch")
+
+    def interface_solver(self) -> None:
+        self._call("fcfd_interface_solver")
+
+    def step(self) -> None:
+        self._call("fcfd_step")
+
+    def problem_init(self, redis0: bool = True) -> None:
+        self._call("fcfd_problem_init", 1 if redis0 else 0)

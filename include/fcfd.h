@@ -1,0 +1,149 @@
+/*
+ * fcfd.h — C ABI of libfcfd, the MI355X (gfx950) implementation of the
+ * two-phase incompressible Navier-Stokes step of the reference
+ * (`Two phase flows/newOpenMP/{2D,3D+IBM}`): WENO5 level-set advection with
+ * SSP-RK3, mass-preserving level-set reinitialisation and volume correction,
+ * the variable-coefficient (1/rho) pressure Poisson solve and the velocity
+ * projection, plus the step glue that closes the time step.
+ *
+ * The reference has no FFI layer: its hot path is a set of argument-less
+ * internal subroutines that mutate the global `type(manager) :: p`
+ * (`.../src/main.f90:1-6`).  Here a context object plays the role of `p`:
+ * it owns device-resident fields in the reference layout (fcfd_types.h) and
+ * every entry point named fcfd_<subroutine> replaces the reference subroutine
+ * of that name.  Host arrays are only borrowed by fcfd_push / fcfd_pull.
+ * All calls are stream ordered on the context's HIP stream and must come from
+ * one host thread per context (the reference calls them from serial code).
+ * Every function returns an fcfd_status code; 0 is success.
+ */
+#ifndef FCFD_H
+#define FCFD_H
+
+#include "fcfd_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fcfd_ctx fcfd_ctx;
+
+/* Create a context for one grid on the current HIP device (`manager_init` +
+ * `job_init`, `.../3D+IBM/lib/data/tree.f90:108-227`, `branches.f90:70-140`).
+ * Fields start at zero. */
+int fcfd_create(const fcfd_grid *grid, const fcfd_params *params, fcfd_ctx **out);
+int fcfd_destroy(fcfd_ctx *ctx);
+/* Replace the parameters (dt, tolerances ...) of an existing context. */
+int fcfd_set_params(fcfd_ctx *ctx, const fcfd_params *params);
+/* Number of float64 values of one ghosted field. */
+int64_t fcfd_field_size(const fcfd_ctx *ctx);
+
+/* Host <-> device copies of one full ghosted field (borrowed host arrays). */
+int fcfd_push(fcfd_ctx *ctx, int field, const double *host);
+int fcfd_pull(fcfd_ctx *ctx, int field, double *host);
+/* Device-to-device copies (e.g. from a torch tensor's storage). */
+int fcfd_push_device(fcfd_ctx *ctx, int field, const double *dev);
+int fcfd_pull_device(fcfd_ctx *ctx, int field, double *dev);
+/* Wait for all work queued on the context's stream. */
+int fcfd_sync(fcfd_ctx *ctx);
+
+int fcfd_get_diag(fcfd_ctx *ctx, fcfd_diag *out);
+int fcfd_set_time(fcfd_ctx *ctx, double time, int iter);
+int fcfd_set_imass(fcfd_ctx *ctx, double imass);
+
+/* ---- boundary conditions (`.../3D+IBM/src/others/bcs.f90`) ---- */
+/* bc(dat): zero-gradient ghost copy of one field (bcs.f90:113-154). */
+int fcfd_bc(fcfd_ctx *ctx, int field);
+/* velbc(u,v,w): no-slip / slip ghost rules of vel%now (bcs.f90:156-369). */
+int fcfd_velbc(fcfd_ctx *ctx);
+
+/* ---- level-set properties (`.../interface_solver/others.f90`) ---- */
+/* ls_funs: H, delta, sign of phi over the ghosted box (others.f90:56-94). */
+int fcfd_ls_funs(fcfd_ctx *ctx);
+/* rho_mu: ls_funs + rho, mu (others.f90:96-120). */
+int fcfd_rho_mu(fcfd_ctx *ctx);
+/* ls_mv: rho_mu + mass and volume sums (others.f90:122-176). */
+int fcfd_ls_mv(fcfd_ctx *ctx, double *mass, double *vol);
+
+/* ---- level-set advection (`.../interface_solver/level_set_rk3_solver.f90:1-219`) ---- */
+/* level_set_rk3_solver: WENO5-JS flux split + SSP-RK3 of phi with the cell
+ * velocity nvel (fields NU/NV/NW hold nvel%old of the reference). */
+int fcfd_level_set_rk3_solver(fcfd_ctx *ctx);
+
+/* ---- reinitialisation (`.../interface_solver/level_set_rk3_redis.f90:1-410`) ---- */
+/* level_set_rk3_redis(btn): btn = 1 maintains the distance property with the
+ * mass-constrained lambda; btn = 0 is the initial redistancing.  *iters gets
+ * the pseudo-step count (may be NULL). */
+int fcfd_level_set_rk3_redis(fcfd_ctx *ctx, int btn, int *iters);
+
+/* ---- mass correction (`.../interface_solver/mass_preserving_level_set.f90:1-54`) ---- */
+int fcfd_mass_preserving_level_set(fcfd_ctx *ctx);
+
+/* ---- CCD derivatives and curvature (`.../interface_solver/others_normals.f90`) ---- */
+/* surface_norms: CCD first/second and cross derivatives of phi (others_normals.f90:1-83). */
+int fcfd_surface_norms(fcfd_ctx *ctx);
+/* curv: surface_norms + curvature (others_normals.f90:133-167). */
+int fcfd_curv(fcfd_ctx *ctx);
+
+/* ---- pressure Poisson + projection (`.../ns_solver/ns_ppe_solver.f90`) ---- */
+/* ppe_sor_init: build the 1/rho operator hierarchy from rho (ns_ppe_solver.f90:1-65). */
+int fcfd_ppe_sor_init(fcfd_ctx *ctx);
+/* ppe_sor_solver: src = div(u*)/dt, multigrid solve of the ppe_sor_init
+ * operator to ||r||/||f|| <= rel_tol (mean(p) = 0), then
+ * ns_momentum_correction, as the reference's ppe_sor_solver ends with it
+ * (ns_ppe_solver.f90:67-172).  precision: FCFD_PPE_FP64 or FCFD_PPE_MIXED.
+ * Returns FCFD_ERR_NOT_CONVERGED after params.ppe_max_cycles. */
+int fcfd_ppe_sor_solver(fcfd_ctx *ctx, double rel_tol, int precision, int *cycles, double *rel_resid);
+/* The solve alone: A p = PPE_SRC (minus its mean) with the operator of the
+ * last fcfd_ppe_sor_init, starting from P, to ||r||/||f|| <= rel_tol, then
+ * mean(p) = 0.  No source evaluation and no projection: the standalone
+ * variable-coefficient Poisson problem.  (The reference has no standalone
+ * solver; this is the loop body of ppe_sor_solver, ns_ppe_solver.f90:289-361.) */
+int fcfd_ppe_solve_rhs(fcfd_ctx *ctx, double rel_tol, int precision, int *cycles, double *rel_resid);
+/* ns_momentum_correction: u -= dt grad p / rho_face, then velbc (ns_ppe_solver.f90:174-205). */
+int fcfd_ns_momentum_correction(fcfd_ctx *ctx);
+
+/* ---- momentum predictor glue (`.../ns_solver/`) ---- */
+/* ns_linearize + ns_ab_setup + ns_ab_predictor (ns_solvers.f90:33-57,
+ * admas_bashforth_solver.f90:230-311, ns_ab_adv.f90:1-73, ns_ab_diff.f90:1-198):
+ * the viscous/body-force part uses the start-of-step fields (PHI_OLD,
+ * U_OLD.., and the curvature/normals held in CURV/NX/NY/NZ of the previous
+ * ns_init); SRC_* gets adv + diff, U/V/W the AB2 predictor. */
+int fcfd_ns_ab_predictor(fcfd_ctx *ctx);
+/* node_vel: MAC -> cell velocity + nvelbc (ns_solvers.f90:128-147). */
+int fcfd_node_vel(fcfd_ctx *ctx);
+/* ns_check_convergence_div: max / mean |div u| into the diagnostics. */
+int fcfd_ns_check_convergence_div(fcfd_ctx *ctx);
+/* ns_solver: ns_init, picard_iters x (predictor, PPE, projection), node_vel
+ * (ns_solvers.f90:1-31, admas_bashforth_solver.f90:254-281). */
+int fcfd_ns_solver(fcfd_ctx *ctx);
+
+/* ---- whole step ---- */
+/* manager_switch (tree.f90:229-252): old <- now for the time-levelled fields. */
+int fcfd_switch(fcfd_ctx *ctx);
+/* mpls(): advection, reinitialisation every redis_interval steps, mass
+ * correction (interface_solver.f90:25-34). */
+int fcfd_interface_solver(fcfd_ctx *ctx);
+/* one pass of the main loop body (main.f90:15-27): time += dt, iter += 1,
+ * switch, interface_solver, ns_solver, ls_mv. */
+int fcfd_step(fcfd_ctx *ctx);
+/* problem_init after the initial condition is pushed (problem_init.f90:109-133
+ * without IBM and plotting): bc, velbc, [redis(0)], node_vel, ns_init,
+ * p%loc%init, ns_ab_setup, ls_mv, imass = mass. */
+int fcfd_problem_init(fcfd_ctx *ctx, int redis0);
+
+/* ---- built-in kernel timing (no reference counterpart; replaces the
+ * reference's system_clock buckets ls_adv / ls_red / ppe / ns,
+ * `.../lib/data/branches.f90:36`, printed by print_CPU_info) ---- */
+int fcfd_prof_nregions(void);
+const char *fcfd_prof_region_name(int region);
+/* on != 0: record a HIP event pair around every kernel region from now on
+ * (clears earlier records); on == 0: stop recording. */
+int fcfd_prof_enable(fcfd_ctx *ctx, int on);
+/* summed milliseconds and launch counts per region since fcfd_prof_enable;
+ * n must be >= fcfd_prof_nregions(). */
+int fcfd_prof_read(fcfd_ctx *ctx, double *ms, int64_t *count, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FCFD_H */

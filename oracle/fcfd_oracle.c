@@ -217,6 +217,10 @@ void orc_get_diag(const orc_state *s, fcfd_diag *d) {
 #define KLO(s) 0
 #define KHI(s) ((s)->nz)
 #define FOR_INT(s) for (int k = 0; k < (s)->nz; ++k) for (int j = 0; j < (s)->ny; ++j) for (int i = 0; i < (s)->nx; ++i)
+#define ORC_PRAGMA(x) _Pragma(#x)
+/* OpenMP versions over the interior (used by the converged pressure solve) */
+#define PFOR_INT(s) ORC_PRAGMA(omp parallel for collapse(2)) FOR_INT(s)
+#define PFOR_SUM(s, v) ORC_PRAGMA(omp parallel for collapse(2) reduction(+ : v)) FOR_INT(s)
 #define FOR_ALL(s)                                           \
     for (int k = -(s)->gz; k < (s)->nz + (s)->gz; ++k)       \
         for (int j = -(s)->g; j < (s)->ny + (s)->g; ++j)     \
@@ -1071,14 +1075,14 @@ int orc_ppe_exact_solver(orc_state *s, double rel_tol) {
     double *r = falloc(n), *z = falloc(n), *q = falloc(n), *Aq = falloc(n), *f = falloc(n);
     const double ncell = (double)s->nx * (double)s->ny * (double)s->nz;
     double mean = 0.0;
-    FOR_INT(s) mean += s->coe_src[ID(s, i, j, k)];
+    PFOR_SUM(s, mean) mean += s->coe_src[ID(s, i, j, k)];
     mean /= ncell;
     double fn = 0.0;
-    FOR_INT(s) { long id = ID(s, i, j, k); f[id] = -(s->coe_src[id] - mean); fn += f[id] * f[id]; }
+    PFOR_SUM(s, fn) { long id = ID(s, i, j, k); f[id] = -(s->coe_src[id] - mean); fn += f[id] * f[id]; }
     fn = sqrt(fn);
     orc_ppe_apply(s, P, Aq);
     double rn = 0.0, rz = 0.0;
-    FOR_INT(s) {
+    PFOR_SUM(s, rn) {
         long id = ID(s, i, j, k);
         r[id] = f[id] + Aq[id];
         rn += r[id] * r[id];
@@ -1086,14 +1090,14 @@ int orc_ppe_exact_solver(orc_state *s, double rel_tol) {
     int it = 0, rc = FCFD_OK;
     if (fn == 0.0) fn = 1.0;
     if (sqrt(rn) / fn > rel_tol) {
-        FOR_INT(s) { long id = ID(s, i, j, k); z[id] = r[id] / (-s->coe_c[id]); q[id] = z[id]; rz += r[id] * z[id]; }
+        PFOR_SUM(s, rz) { long id = ID(s, i, j, k); z[id] = r[id] / (-s->coe_c[id]); q[id] = z[id]; rz += r[id] * z[id]; }
         for (it = 1; it <= 200000; ++it) {
             orc_ppe_apply(s, q, Aq);
             double qAq = 0.0;
-            FOR_INT(s) { long id = ID(s, i, j, k); Aq[id] = -Aq[id]; qAq += q[id] * Aq[id]; }
+            PFOR_SUM(s, qAq) { long id = ID(s, i, j, k); Aq[id] = -Aq[id]; qAq += q[id] * Aq[id]; }
             double alpha = rz / qAq;
             rn = 0.0;
-            FOR_INT(s) {
+            PFOR_SUM(s, rn) {
                 long id = ID(s, i, j, k);
                 P[id] += alpha * q[id];
                 r[id] -= alpha * Aq[id];
@@ -1101,17 +1105,17 @@ int orc_ppe_exact_solver(orc_state *s, double rel_tol) {
             }
             if (sqrt(rn) / fn <= rel_tol) break;
             double rz2 = 0.0;
-            FOR_INT(s) { long id = ID(s, i, j, k); z[id] = r[id] / (-s->coe_c[id]); rz2 += r[id] * z[id]; }
+            PFOR_SUM(s, rz2) { long id = ID(s, i, j, k); z[id] = r[id] / (-s->coe_c[id]); rz2 += r[id] * z[id]; }
             double beta = rz2 / rz;
             rz = rz2;
-            FOR_INT(s) { long id = ID(s, i, j, k); q[id] = z[id] + beta * q[id]; }
+            PFOR_INT(s) { long id = ID(s, i, j, k); q[id] = z[id] + beta * q[id]; }
         }
         if (it > 200000) rc = FCFD_ERR_NOT_CONVERGED;
     }
     double pm = 0.0;
-    FOR_INT(s) pm += P[ID(s, i, j, k)];
+    PFOR_SUM(s, pm) pm += P[ID(s, i, j, k)];
     pm /= ncell;
-    FOR_INT(s) P[ID(s, i, j, k)] -= pm;
+    PFOR_INT(s) P[ID(s, i, j, k)] -= pm;
     bc(s, P);
     s->ppe_rel = sqrt(rn) / fn;
     s->piter += it;
