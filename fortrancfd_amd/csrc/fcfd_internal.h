@@ -108,6 +108,15 @@ struct fcfd_ctx {
     float *rho32;              // FP32 copy of rho for the mixed inner cycle
     double *mg_scalars;        // device scalars of the solve (norms)
     int mg_ready;
+    // level-0 Krylov vectors of the flexible CG (FP64) and FP32 previous z
+    double *kr, *kp, *kq, *kz, *kzo;
+    float *kzo32;
+    // captured V-cycles: [precision][inner cycles 1..4]
+    hipGraphExec_t vgraph[3][5];      // kind: 0 fp64 on p, 1 fp64 on z (Krylov), 2 fp32 on e32
+    std::vector<std::pair<int, int>> vgraph_prof[3][5];  // (region, event) pairs inside each graph
+    int capture_kind = -1, capture_inner = 0;            // graph being captured (or -1)
+    double prof_gms[fcfd::NREG] = {};                    // timings folded in from graph replays
+    int64_t prof_gcnt[fcfd::NREG] = {};
     // ---- bookkeeping ----
     double time;
     int iter;
@@ -155,6 +164,8 @@ int node_vel(fcfd_ctx *c);
 int div_check(fcfd_ctx *c);
 int mg_setup(fcfd_ctx *c);
 int mg_free(fcfd_ctx *c);
+int mg_drop_graphs(fcfd_ctx *c);
+int prof_graph_accum(fcfd_ctx *c, int kind, int inner);
 int ppe_solve(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel);
 int copy_field(fcfd_ctx *c, double *dst, const double *src);
 

@@ -481,9 +481,24 @@ int mg_free(fcfd_ctx *c) {
     c->mg.clear();
     if (c->rho32) hipFree(c->rho32);
     if (c->mg_scalars) hipFree(c->mg_scalars);
+    void *kv[] = {c->kr, c->kp, c->kq, c->kz, c->kzo, c->kzo32};
+    for (void *p : kv)
+        if (p) hipFree(p);
+    c->kr = c->kp = c->kq = c->kz = c->kzo = nullptr;
+    c->kzo32 = nullptr;
     c->rho32 = nullptr;
     c->mg_scalars = nullptr;
     c->mg_ready = 0;
+    return mg_drop_graphs(c);
+}
+
+int mg_drop_graphs(fcfd_ctx *c) {
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 5; ++b) {
+            if (c->vgraph[a][b]) hipGraphExecDestroy(c->vgraph[a][b]);
+            c->vgraph[a][b] = nullptr;
+            c->vgraph_prof[a][b].clear();
+        }
     return FCFD_OK;
 }
 
@@ -544,6 +559,12 @@ static int mg_build(fcfd_ctx *c) {
     FCFD_TRY(alloc0(&M0.f32, G.ntot));
     FCFD_TRY(alloc0(&c->rho32, G.ntot));
     FCFD_TRY(alloc0(&M0.f64, G.ntot));   // zero-mean working rhs
+    FCFD_TRY(alloc0(&c->kr, G.ntot));
+    FCFD_TRY(alloc0(&c->kp, G.ntot));
+    FCFD_TRY(alloc0(&c->kq, G.ntot));
+    FCFD_TRY(alloc0(&c->kz, G.ntot));
+    FCFD_TRY(alloc0(&c->kzo, G.ntot));
+    FCFD_TRY(alloc0(&c->kzo32, G.ntot));
     // coarse CG workspace (6 vectors) + scalars
     long ncoarse = (long)c->mg.back().nx * c->mg.back().ny * c->mg.back().nz;
     FCFD_TRY(alloc0(&c->mg_scalars, 64 + 6 * ncoarse));
@@ -655,8 +676,8 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         long n = (long)M.nx * M.ny * M.nz;
         ProfScope ps_(c, R_COARSE);
         hipLaunchKernelGGL((k_coarse_cg<T, DIM>), dim3(1), dim3(n >= 1024 ? 1024 : 256), 0, c->stream, e, f, bx, by,
-                           bz, M.bx64, M.by64, M.bz64, c->mg_scalars + 64, L, ih, (int)std::min<long>(4 * n, 4000),
-                           1e-13);
+                           bz, M.bx64, M.by64, M.bz64, c->mg_scalars + 64, L, ih, (int)std::min<long>(n, 96),
+                           0.0);
         FCFD_CHECK(hipGetLastError());
     }
     // up
@@ -678,25 +699,148 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
     return FCFD_OK;
 }
 
-// ||f - A p||^2 (and ||f||^2) of level 0 in FP64 -> host
+// ------------------------------------------------------ outer iteration
+
+// FP64 residual r = f - A p of level 0 with ||r||^2 and (with_f) ||f||^2
+// folded into block partials; optional outputs: r64 (Krylov residual),
+// r32 + zeroed e32 (rhs/error of the FP32 cycle), zeroed z64 (FP64 cycle).
 template <int DIM>
-static int resid_norm(fcfd_ctx *c, const double *p, const double *f, float *r32, float *e32, int with_f,
-                      double &rr, double &ff) {
-    const Geom &G = c->G;
-    dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
-    long nb = (long)g.x * g.y * g.z;   // the partial buffer holds 2*nb (sized at creation)
-    Inv2 ih = inv2(G.dx, G.dy, G.dz);
-    {
-        ProfScope ps_(c, R_RESID0);
-        hipLaunchKernelGGL(k_resid_norm<DIM>, g, blk, 0, c->stream, p, f, c->f[FCFD_RHO], lay_of(G), ih, r32, e32,
-                           c->red.partial, with_f);
+__global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p, const double *__restrict__ f,
+                                                   const double *__restrict__ rho, Lay L, Inv2 ih,
+                                                   double *__restrict__ r64, float *__restrict__ r32,
+                                                   float *__restrict__ e32, double *__restrict__ z64,
+                                                   double *__restrict__ partial, int with_f) {
+    __shared__ double s0[256], s1[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    double rr = 0.0, ff = 0.0;
+    if (i < L.nx && j < L.ny) {
+        long id = L.id(i, j, k);
+        double r = f[id] - apply_A<double, DIM, true>(p, rho, nullptr, nullptr, nullptr, L, ih, i, j, k, id);
+        rr = r * r;
+        if (with_f) ff = f[id] * f[id];
+        if (r64) r64[id] = r;
+        if (r32) { r32[id] = (float)r; e32[id] = 0.0f; }
+        if (z64) z64[id] = 0.0;
     }
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->red.partial, nb, 2, c->mg_scalars);
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    s0[t] = rr;
+    s1[t] = ff;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) { s0[t] += s0[t + w]; s1[t] += s1[t + w]; }
+        __syncthreads();
+    }
+    if (t == 0) {
+        long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+        long nb = (long)gridDim.x * gridDim.y * gridDim.z;
+        partial[b] = s0[0];
+        partial[nb + b] = s1[0];
+    }
+}
+
+// dots of the flexible CG: r.z and r.z_old (z of precision T)
+template <typename T>
+__global__ void __launch_bounds__(256) k_fcg_dots(const double *__restrict__ r, const T *__restrict__ z,
+                                                  const T *__restrict__ zo, Lay L, double *__restrict__ partial) {
+    __shared__ double s0[256], s1[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    double a = 0.0, b = 0.0;
+    if (i < L.nx && j < L.ny) {
+        long id = L.id(i, j, k);
+        a = r[id] * (double)z[id];
+        b = r[id] * (double)zo[id];
+    }
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    s0[t] = a;
+    s1[t] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) { s0[t] += s0[t + w]; s1[t] += s1[t + w]; }
+        __syncthreads();
+    }
+    if (t == 0) {
+        long bb = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+        long nb = (long)gridDim.x * gridDim.y * gridDim.z;
+        partial[bb] = s0[0];
+        partial[nb + bb] = s1[0];
+    }
+}
+
+// new direction: p = z + beta p, q = A z + beta q (so q = A p), z_old = z,
+// partial sums of p.q
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__restrict__ zo,
+                                                 double *__restrict__ p, double *__restrict__ q,
+                                                 const double *__restrict__ rho, Lay L, Inv2 ih, double beta,
+                                                 double *__restrict__ partial) {
+    __shared__ double s0[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    double pq = 0.0;
+    if (i < L.nx && j < L.ny) {
+        long id = L.id(i, j, k);
+        // A z with the FP64 operator; z promoted
+        const double zp = (double)z[id];
+        double r0 = rho[id], acc = 0.0;
+        if (i < L.nx - 1) acc += 2.0 / (r0 + rho[id + 1]) * ih.x * ((double)z[id + 1] - zp);
+        if (i > 0) acc += 2.0 / (r0 + rho[id - 1]) * ih.x * ((double)z[id - 1] - zp);
+        if (j < L.ny - 1) acc += 2.0 / (r0 + rho[id + L.sx]) * ih.y * ((double)z[id + L.sx] - zp);
+        if (j > 0) acc += 2.0 / (r0 + rho[id - L.sx]) * ih.y * ((double)z[id - L.sx] - zp);
+        if (DIM == 3) {
+            if (k < L.nz - 1) acc += 2.0 / (r0 + rho[id + L.sxy]) * ih.z * ((double)z[id + L.sxy] - zp);
+            if (k > 0) acc += 2.0 / (r0 + rho[id - L.sxy]) * ih.z * ((double)z[id - L.sxy] - zp);
+        }
+        double pn = zp + beta * p[id];
+        double qn = acc + beta * q[id];
+        p[id] = pn;
+        q[id] = qn;
+        zo[id] = z[id];
+        pq = pn * qn;
+    }
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    s0[t] = pq;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s0[t] += s0[t + w];
+        __syncthreads();
+    }
+    if (t == 0) partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = s0[0];
+}
+
+// x += alpha p, r -= alpha q, ||r||^2; the new residual also becomes the
+// rhs of the next preconditioning cycle (r32 / zeroed e32, or zeroed z64)
+__global__ void __launch_bounds__(256) k_fcg_update(double *__restrict__ x, const double *__restrict__ p,
+                                                    const double *__restrict__ q, double *__restrict__ r, Lay L,
+                                                    double alpha, float *__restrict__ r32, float *__restrict__ e32,
+                                                    double *__restrict__ z64, double *__restrict__ partial) {
+    __shared__ double s0[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    double rr = 0.0;
+    if (i < L.nx && j < L.ny) {
+        long id = L.id(i, j, k);
+        x[id] += alpha * p[id];
+        double rn = r[id] - alpha * q[id];
+        r[id] = rn;
+        rr = rn * rn;
+        if (r32) { r32[id] = (float)rn; e32[id] = 0.0f; }
+        if (z64) z64[id] = 0.0;
+    }
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    s0[t] = rr;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s0[t] += s0[t + w];
+        __syncthreads();
+    }
+    if (t == 0) partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = s0[0];
+}
+
+// sum ns slots of the block partials of a level-0 launch and bring them to the host
+static int sums_to_host(fcfd_ctx *c, long nb, int ns, double *out) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->red.partial, nb, ns, c->mg_scalars);
     FCFD_CHECK(hipGetLastError());
-    FCFD_CHECK(hipMemcpyAsync(c->red.host, c->mg_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FCFD_CHECK(hipMemcpyAsync(c->red.host, c->mg_scalars, ns * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     FCFD_CHECK(hipStreamSynchronize(c->stream));
-    rr = c->red.host[0];
-    ff = c->red.host[1];
+    for (int s = 0; s < ns; ++s) out[s] = c->red.host[s];
     return FCFD_OK;
 }
 
@@ -711,48 +855,148 @@ static int field_mean(fcfd_ctx *c, const double *a) {
     return FCFD_OK;
 }
 
+// Run `inner` V-cycles of one kind from a captured graph (captured on first use;
+// the level arrays, rho and the level-0 vectors are fixed for the context):
+// kind 0: FP64 on (P, f64); kind 1: FP64 on (kz, kr); kind 2: FP32 on (e32, f32).
+template <int DIM>
+static int run_vcycles(fcfd_ctx *c, int kind, int inner) {
+    inner = std::max(1, std::min(inner, 4));
+    MgLevel &M0 = c->mg[0];
+    auto body = [&]() -> int {
+        for (int t = 0; t < inner; ++t) {
+            if (kind == 0) FCFD_TRY((vcycle<double, DIM>(c, 0, c->f[FCFD_P], M0.f64, c->f[FCFD_RHO], 2, 2)));
+            else if (kind == 1) FCFD_TRY((vcycle<double, DIM>(c, 0, c->kz, c->kr, c->f[FCFD_RHO], 2, 2)));
+            else FCFD_TRY((vcycle<float, DIM>(c, 0, M0.e32, M0.f32, c->rho32, 2, 2)));
+        }
+        return FCFD_OK;
+    };
+    // events recorded inside a graph cannot be timed: profile with direct launches
+    static const bool no_graph = getenv("FCFD_NO_GRAPH") != nullptr;
+    if (no_graph || c->prof.on) return body();
+    hipGraphExec_t &ex = c->vgraph[kind][inner];
+    if (!ex) {
+        hipGraph_t graph;
+        c->capture_kind = kind;
+        c->capture_inner = inner;
+        c->vgraph_prof[kind][inner].clear();
+        FCFD_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = body();
+        hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+        c->capture_kind = -1;
+        if (rc != FCFD_OK) return rc;
+        FCFD_CHECK(e2);
+        FCFD_CHECK(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
+        FCFD_CHECK(hipGraphDestroy(graph));
+    }
+    FCFD_CHECK(hipGraphLaunch(ex, c->stream));
+    return FCFD_OK;
+}
+
 template <int DIM>
 static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel) {
     const Geom &G = c->G;
     dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
+    const long nb = (long)g.x * g.y * g.z;
+    const Lay L0 = lay_of(G);
+    const Inv2 ih = inv2(G.dx, G.dy, G.dz);
     MgLevel &M0 = c->mg[0];
     const double ncell = (double)G.nx * (double)G.ny * (double)G.nz;
-    double *p = c->f[FCFD_P];
-    double *f = M0.f64;
-    // compatibility: f = src - mean(src)
-    FCFD_TRY(field_mean(c, c->f[FCFD_PPE_SRC]));
-    hipLaunchKernelGGL(k_center_copy, g, blk, 0, c->stream, c->f[FCFD_PPE_SRC], f, lay_of(G), c->mg_scalars + 4,
-                       1.0 / ncell);
-    FCFD_CHECK(hipGetLastError());
-    double rr, ff;
-    const int maxc = c->pr.ppe_max_cycles > 0 ? c->pr.ppe_max_cycles : 200;
-    int k = 0, rc = FCFD_OK;
+    double *p = c->f[FCFD_P], *f = M0.f64, *rho = c->f[FCFD_RHO];
     const bool mixed = precision == FCFD_PPE_MIXED;
-    FCFD_TRY(resid_norm<DIM>(c, p, f, mixed ? M0.f32 : nullptr, M0.e32, 1, rr, ff));
-    const double fn = ff > 0.0 ? sqrt(ff) : 1.0;
-    double r = sqrt(rr) / fn;
-    if (!std::isfinite(r)) return FCFD_ERR_NAN;
+    const bool krylov = c->pr.ppe_krylov != 0;
+    const int inner = std::max(1, std::min(c->pr.ppe_inner > 0 ? c->pr.ppe_inner : 1, 4));
+    const int kind = mixed ? 2 : (krylov ? 1 : 0);
+    const int maxc = c->pr.ppe_max_cycles > 0 ? c->pr.ppe_max_cycles : 200;
     static const bool verbose = getenv("FCFD_MG_VERBOSE") != nullptr;
-    if (verbose) fprintf(stderr, "fcfd mg[%s]: cycle 0 rel %.3e\n", mixed ? "mixed" : "fp64", r);
+    // compatibility of the Neumann problem: f = src - mean(src)
+    FCFD_TRY(field_mean(c, c->f[FCFD_PPE_SRC]));
+    hipLaunchKernelGGL(k_center_copy, g, blk, 0, c->stream, c->f[FCFD_PPE_SRC], f, L0, c->mg_scalars + 4, 1.0 / ncell);
+    FCFD_CHECK(hipGetLastError());
+    // r0 = f - A p
+    {
+        ProfScope ps_(c, R_RESID0);
+        hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, f, rho, L0, ih, krylov ? c->kr : nullptr,
+                           mixed ? M0.f32 : nullptr, M0.e32, (krylov && !mixed) ? c->kz : nullptr, c->red.partial, 1);
+        FCFD_CHECK(hipGetLastError());
+    }
+    double s2[2];
+    FCFD_TRY(sums_to_host(c, nb, 2, s2));
+    const double fn = s2[1] > 0.0 ? sqrt(s2[1]) : 1.0;
+    double r = sqrt(s2[0]) / fn, best = r;
+    int k = 0, since_best = 0, rc = FCFD_OK;
+    if (verbose) fprintf(stderr, "fcfd mg[%s%s]: it 0 rel %.3e\n", mixed ? "mixed" : "fp64", krylov ? "+fcg" : "", r);
+    if (!std::isfinite(r)) return FCFD_ERR_NAN;
+    double rz = 0.0, pq = 0.0;
+    bool first = true;
     while (r > rel_tol && k < maxc) {
-        if (mixed) {
-            FCFD_TRY((vcycle<float, DIM>(c, 0, M0.e32, M0.f32, c->rho32, 2, 2)));
-            hipLaunchKernelGGL(k_add_f32, g, blk, 0, c->stream, p, M0.e32, lay_of(G));
-            FCFD_CHECK(hipGetLastError());
-            FCFD_TRY(bc_scalar(c, p));
+        if (!krylov) {
+            // defect correction (mixed) or plain V-cycles on p (FP64)
+            FCFD_TRY(run_vcycles<DIM>(c, kind, inner));
+            if (mixed) {
+                hipLaunchKernelGGL(k_add_f32, g, blk, 0, c->stream, p, M0.e32, L0);
+                FCFD_CHECK(hipGetLastError());
+            }
+            {
+                ProfScope ps_(c, R_RESID0);
+                hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, f, rho, L0, ih, nullptr,
+                                   mixed ? M0.f32 : nullptr, M0.e32, nullptr, c->red.partial, 0);
+                FCFD_CHECK(hipGetLastError());
+            }
+            FCFD_TRY(sums_to_host(c, nb, 1, s2));
+            FCFD_TRY(prof_graph_accum(c, kind, inner));
         } else {
-            FCFD_TRY((vcycle<double, DIM>(c, 0, p, f, c->f[FCFD_RHO], 2, 2)));
+            // flexible CG preconditioned by the V-cycle(s)
+            FCFD_TRY(run_vcycles<DIM>(c, kind, inner));
+            if (mixed)
+                hipLaunchKernelGGL(k_fcg_dots<float>, g, blk, 0, c->stream, c->kr, M0.e32, c->kzo32, L0, c->red.partial);
+            else
+                hipLaunchKernelGGL(k_fcg_dots<double>, g, blk, 0, c->stream, c->kr, c->kz, c->kzo, L0, c->red.partial);
+            FCFD_CHECK(hipGetLastError());
+            double d[2];
+            FCFD_TRY(sums_to_host(c, nb, 2, d));
+            FCFD_TRY(prof_graph_accum(c, kind, inner));
+            const double beta = first ? 0.0 : (d[0] - d[1]) / rz;
+            rz = d[0];
+            first = false;
+            {
+                ProfScope ps_(c, R_MISC);
+                if (mixed)
+                    hipLaunchKernelGGL((k_fcg_dir<float, DIM>), g, blk, 0, c->stream, M0.e32, c->kzo32, c->kp, c->kq,
+                                       rho, L0, ih, beta, c->red.partial);
+                else
+                    hipLaunchKernelGGL((k_fcg_dir<double, DIM>), g, blk, 0, c->stream, c->kz, c->kzo, c->kp, c->kq,
+                                       rho, L0, ih, beta, c->red.partial);
+                FCFD_CHECK(hipGetLastError());
+            }
+            FCFD_TRY(sums_to_host(c, nb, 1, &pq));
+            if (!(pq < 0.0) || !(rz < 0.0)) {
+                // A is negative definite on the zero-mean space and the cycle
+                // approximates its inverse, so r.z < 0 and p.Ap < 0 until the
+                // iteration reaches round-off (alpha = r.z / p.Ap > 0)
+                if (verbose) fprintf(stderr, "fcfd mg: breakdown rz %.3e pq %.3e\n", rz, pq);
+                break;
+            }
+            const double alpha = rz / pq;
+            {
+                ProfScope ps_(c, R_MISC);
+                hipLaunchKernelGGL(k_fcg_update, g, blk, 0, c->stream, p, c->kp, c->kq, c->kr, L0, alpha,
+                                   mixed ? M0.f32 : nullptr, M0.e32, mixed ? nullptr : c->kz, c->red.partial);
+                FCFD_CHECK(hipGetLastError());
+            }
+            FCFD_TRY(sums_to_host(c, nb, 1, s2));
         }
         ++k;
-        FCFD_TRY(resid_norm<DIM>(c, p, f, mixed ? M0.f32 : nullptr, M0.e32, 0, rr, ff));
-        r = sqrt(rr) / fn;
-        if (verbose) fprintf(stderr, "fcfd mg[%s]: cycle %d rel %.3e\n", mixed ? "mixed" : "fp64", k, r);
+        r = sqrt(s2[0]) / fn;
+        if (verbose) fprintf(stderr, "fcfd mg[%s%s]: it %d rel %.3e\n", mixed ? "mixed" : "fp64", krylov ? "+fcg" : "", k, r);
         if (!std::isfinite(r)) return FCFD_ERR_NAN;
+        // stagnation: the FP64 residual has reached its round-off floor
+        if (r < 0.9 * best) { best = r; since_best = 0; }
+        else if (++since_best >= 4) break;
     }
-    if (r > rel_tol) rc = FCFD_ERR_NOT_CONVERGED;
+    if (r > rel_tol) rc = (r <= 10.0 * rel_tol && since_best >= 4) ? FCFD_OK : FCFD_ERR_NOT_CONVERGED;
     // mean(p) = 0 as the reference's iteration enforces, then ghost copy
     FCFD_TRY(field_mean(c, p));
-    hipLaunchKernelGGL(k_sub_scalar, g, blk, 0, c->stream, p, lay_of(G), c->mg_scalars + 4, 1.0 / ncell);
+    hipLaunchKernelGGL(k_sub_scalar, g, blk, 0, c->stream, p, L0, c->mg_scalars + 4, 1.0 / ncell);
     FCFD_CHECK(hipGetLastError());
     FCFD_TRY(bc_scalar(c, p));
     if (cycles) *cycles = k;

@@ -28,7 +28,10 @@ ProfScope::ProfScope(fcfd_ctx *c_, int reg_) : c(c_), reg(reg_), ev(-1) {
 ProfScope::~ProfScope() {
     if (ev < 0) return;
     hipEventRecord(c->prof.pool[ev + 1], c->stream);
-    c->prof.recs.push_back({reg, ev});
+    if (c->capture_kind >= 0)   // inside a V-cycle graph: read back after each replay
+        c->vgraph_prof[c->capture_kind][c->capture_inner].push_back({reg, ev});
+    else
+        c->prof.recs.push_back({reg, ev});
 }
 
 }  // namespace fcfd
@@ -43,19 +46,37 @@ extern "C" int fcfd_prof_nregions(void) { return NREG; }
 
 extern "C" const char *fcfd_prof_region_name(int r) { return (r >= 0 && r < NREG) ? kRegionNames[r] : ""; }
 
+namespace fcfd {
+int mg_drop_graphs(fcfd_ctx *c);
+// fold the event pairs of a just-replayed V-cycle graph into the totals
+int prof_graph_accum(fcfd_ctx *c, int kind, int inner) {
+    if (!c->prof.on) return FCFD_OK;
+    for (auto &pr : c->vgraph_prof[kind][inner]) {
+        float t = 0.f;
+        FCFD_CHECK(hipEventElapsedTime(&t, c->prof.pool[pr.second], c->prof.pool[pr.second + 1]));
+        c->prof_gms[pr.first] += t;
+        c->prof_gcnt[pr.first] += 1;
+    }
+    return FCFD_OK;
+}
+}  // namespace fcfd
+
 extern "C" int fcfd_prof_enable(fcfd_ctx *c, int on) {
     if (!c) return FCFD_ERR_ARG;
     FCFD_CHECK(hipStreamSynchronize(c->stream));
+    // captured V-cycles hold events of the pool (or none): capture again
+    FCFD_TRY(mg_drop_graphs(c));
     c->prof.on = on;
     c->prof.recs.clear();
     c->prof.used = 0;
+    for (int r = 0; r < NREG; ++r) { c->prof_gms[r] = 0.0; c->prof_gcnt[r] = 0; }
     return FCFD_OK;
 }
 
 extern "C" int fcfd_prof_read(fcfd_ctx *c, double *ms, int64_t *count, int n) {
     if (!c || !ms || !count || n < NREG) return FCFD_ERR_ARG;
     FCFD_CHECK(hipStreamSynchronize(c->stream));
-    for (int r = 0; r < NREG; ++r) { ms[r] = 0.0; count[r] = 0; }
+    for (int r = 0; r < NREG; ++r) { ms[r] = c->prof_gms[r]; count[r] = c->prof_gcnt[r]; }
     for (const ProfRec &R : c->prof.recs) {
         float t = 0.f;
         FCFD_CHECK(hipEventElapsedTime(&t, c->prof.pool[R.ev], c->prof.pool[R.ev + 1]));

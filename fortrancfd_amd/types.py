@@ -55,7 +55,8 @@ class FcfdParams(C.Structure):
         ("ppe_tol", C.c_double), ("p_w1", C.c_double), ("p_tol", C.c_double),
         ("ubc", C.c_int32 * 2), ("vbc", C.c_int32 * 2), ("wbc", C.c_int32 * 2),
         ("redis_interval", C.c_int32), ("picard_iters", C.c_int32),
-        ("ppe_precision", C.c_int32), ("ppe_max_cycles", C.c_int32), ("pad_", C.c_int32),
+        ("ppe_precision", C.c_int32), ("ppe_max_cycles", C.c_int32), ("ppe_krylov", C.c_int32),
+        ("ppe_inner", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
@@ -111,6 +112,8 @@ class Case:
     picard_iters: int = 4
     ppe_precision: int = PPE_FP64
     ppe_max_cycles: int = 200
+    ppe_krylov: int = 1
+    ppe_inner: int = 1
     imass: float = 0.0
     name: str = "case"
 
@@ -206,6 +209,7 @@ class Case:
             arr[0], arr[1] = int(vals[0]), int(vals[1])
         p.redis_interval, p.picard_iters = self.redis_interval, self.picard_iters
         p.ppe_precision, p.ppe_max_cycles = self.ppe_precision, self.ppe_max_cycles
+        p.ppe_krylov, p.ppe_inner = self.ppe_krylov, self.ppe_inner
         return p
 
     def ug(self) -> int:

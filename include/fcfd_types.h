@@ -62,7 +62,9 @@ typedef struct fcfd_params {
     int32_t redis_interval;      /* reinitialise when iter % interval == 0     */
     int32_t picard_iters;        /* outer Picard iterations per step (4)       */
     int32_t ppe_precision;       /* FCFD_PPE_FP64 or FCFD_PPE_MIXED            */
-    int32_t ppe_max_cycles;      /* cap on V-cycles per PPE solve              */
+    int32_t ppe_max_cycles;      /* cap on outer iterations per PPE solve      */
+    int32_t ppe_krylov;          /* 1: flexible CG around the V-cycle; 0: plain */
+    int32_t ppe_inner;           /* V-cycles per outer iteration (>= 1)        */
     int32_t pad_;
 } fcfd_params;
 
