@@ -12,7 +12,8 @@
 // (the reference refactorises it for every line of every call).  A solve is
 // then one forward and one backward sweep per line; one GPU thread owns one
 // line.  For y and z lines consecutive threads hold consecutive i, so every
-// sweep step is a coalesced row access; x lines walk the contiguous axis.
+// sweep step is a coalesced row access; x lines walk the contiguous axis, a
+// chunk of 16 points being 128 contiguous bytes per thread.
 #include <cmath>
 #include <vector>
 
@@ -69,6 +70,12 @@ static void ccd_factor(int N, double dx, std::vector<double> &out) {
     }
 }
 
+// Per-point coefficients of the line solve, packed for the two sweeps:
+// forward [A1 B1 AA1 BB1], backward [A3 B3 AA3 BB3 A2/D AA2/D B2/D BB2/D]
+// with D the determinant of the diagonal block (the reference divides by D;
+// the reciprocal products are folded in here once).
+enum { PF_A1, PF_B1, PF_AA1, PF_BB1, PB_A3, PB_B3, PB_AA3, PB_BB3, PB_A2D, PB_AA2D, PB_B2D, PB_BB2D, NPACK };
+
 int ccd_setup(fcfd_ctx *c) {
     const Geom &G = c->G;
     const int n[3] = {G.nx, G.ny, G.nz};
@@ -77,27 +84,41 @@ int ccd_setup(fcfd_ctx *c) {
         c->ccd_coef[d] = nullptr;
         if (d >= G.dim) continue;
         int N = n[d] + 2 * G.g;
-        std::vector<double> co;
+        std::vector<double> co, pk((size_t)NPACK * N);
         ccd_factor(N, h[d], co);
-        FCFD_CHECK(hipMalloc((void **)&c->ccd_coef[d], sizeof(double) * co.size()));
-        FCFD_CHECK(hipMemcpy(c->ccd_coef[d], co.data(), sizeof(double) * co.size(), hipMemcpyHostToDevice));
+        for (int i = 0; i < N; ++i) {
+            double *q = &pk[(size_t)i * NPACK];
+            const double D = co[CDEN * N + i];
+            q[PF_A1] = co[CA1 * N + i]; q[PF_B1] = co[CB1 * N + i];
+            q[PF_AA1] = co[CAA1 * N + i]; q[PF_BB1] = co[CBB1 * N + i];
+            q[PB_A3] = co[CA3 * N + i]; q[PB_B3] = co[CB3 * N + i];
+            q[PB_AA3] = co[CAA3 * N + i]; q[PB_BB3] = co[CBB3 * N + i];
+            q[PB_A2D] = co[CA2 * N + i] / D; q[PB_AA2D] = co[CAA2 * N + i] / D;
+            q[PB_B2D] = co[CB2 * N + i] / D; q[PB_BB2D] = co[CBB2 * N + i] / D;
+        }
+        FCFD_CHECK(hipMalloc((void **)&c->ccd_coef[d], sizeof(double) * pk.size()));
+        FCFD_CHECK(hipMemcpy(c->ccd_coef[d], pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
     }
     return FCFD_OK;
 }
 
-// asgn_src_ccd + TWIN_BKS along one line; ghost line ends included
-__global__ void __launch_bounds__(256) k_ccd_lines(const double *__restrict__ f, double *__restrict__ fx,
-                                                   double *__restrict__ fxx, const double *__restrict__ co,
-                                                   Geom G, int dir) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// asgn_src_ccd + TWIN_BKS along one line; ghost line ends included.  One
+// thread per line; each sweep moves through the line in chunks of CH points
+// whose loads are all issued before the chunk's dependent recurrence runs,
+// so a line pays one memory latency per chunk instead of one per point.
+template <int DIR, int CH>
+__global__ void __launch_bounds__(64) k_ccd_lines(const double *__restrict__ f, double *__restrict__ fx,
+                                                  double *__restrict__ fxx, const double *__restrict__ co,
+                                                  Geom G) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int a, b, N;
     long base, st;
     double dx;
-    if (dir == 0) {
+    if (DIR == 0) {
         if (t >= (long)G.ny * G.nz) return;
         a = (int)(t % G.ny); b = (int)(t / G.ny);
         base = G.id(-G.g, a, b); st = 1; N = G.nx + 2 * G.g; dx = G.dx;
-    } else if (dir == 1) {
+    } else if (DIR == 1) {
         if (t >= (long)G.nx * G.nz) return;
         a = (int)(t % G.nx); b = (int)(t / G.nx);
         base = G.id(a, -G.g, b); st = G.sx; N = G.ny + 2 * G.g; dx = G.dy;
@@ -108,50 +129,70 @@ __global__ void __launch_bounds__(256) k_ccd_lines(const double *__restrict__ f,
     }
     const double *F = f + base;
     double *X = fx + base, *XX = fxx + base;
-    const double dx2 = dx * dx;
-    const double *A1 = co + CA1 * N, *B1 = co + CB1 * N, *AA1 = co + CAA1 * N, *BB1 = co + CBB1 * N;
-    const double *A2 = co + CA2 * N, *B2 = co + CB2 * N, *AA2 = co + CAA2 * N, *BB2 = co + CBB2 * N;
-    const double *A3 = co + CA3 * N, *B3 = co + CB3 * N, *AA3 = co + CAA3 * N, *BB3 = co + CBB3 * N;
-    const double *DEN = co + CDEN * N;
+    const double idx = 1.0 / dx, idx2 = idx * idx;
     // forward elimination with the source of asgn_src_ccd
-    double f0 = F[0], f1 = F[st], f2 = F[2 * st], f3 = F[3 * st];
-    double S = (-3.5 * f0 + 4.0 * f1 - 0.5 * f2) / dx;
-    double SS = (34.0 / 3.0 * f0 - 83.0 / 4.0 * f1 + 10.0 * f2 - 7.0 / 12.0 * f3) / dx2;
+    const double f0 = F[0], f1 = F[st], f2 = F[2 * st], f3 = F[3 * st];
+    double S = (-3.5 * f0 + 4.0 * f1 - 0.5 * f2) * idx;
+    double SS = (34.0 / 3.0 * f0 - 83.0 / 4.0 * f1 + 10.0 * f2 - 7.0 / 12.0 * f3) * idx2;
     X[0] = S;
     XX[0] = SS;
     double fm = f0, fc = f1;  // f(i-1), f(i)
-    for (int i = 1; i < N - 1; ++i) {
-        double fp = F[(long)(i + 1) * st];
-        double s = 15.0 / 16.0 * (-fm + fp) / dx;
-        double ss = (3.0 * fm - 6.0 * fc + 3.0 * fp) / dx2;
-        double Sn = s - (A1[i] * S + B1[i] * SS);
-        double SSn = ss - (AA1[i] * S + BB1[i] * SS);
-        S = Sn; SS = SSn;
-        X[(long)i * st] = S;
-        XX[(long)i * st] = SS;
-        fm = fc; fc = fp;
+    for (int i0 = 1; i0 < N - 1; i0 += CH) {
+        double fp[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+            if (i0 + u < N - 1) fp[u] = F[(long)(i0 + u + 1) * st];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            const int i = i0 + u;
+            if (i < N - 1) {
+                const double *q = co + (long)i * NPACK;
+                const double s = 15.0 / 16.0 * (fp[u] - fm) * idx;
+                const double ss = (3.0 * fm - 6.0 * fc + 3.0 * fp[u]) * idx2;
+                const double Sn = s - (q[PF_A1] * S + q[PF_B1] * SS);
+                const double SSn = ss - (q[PF_AA1] * S + q[PF_BB1] * SS);
+                S = Sn; SS = SSn;
+                X[(long)i * st] = S;
+                XX[(long)i * st] = SS;
+                fm = fc; fc = fp[u];
+            }
+        }
     }
+    const int ie = N - 1;
+    const double e0 = F[(long)ie * st], e1 = F[(long)(ie - 1) * st], e2 = F[(long)(ie - 2) * st],
+                 e3 = F[(long)(ie - 3) * st];
     {
-        const int ie = N - 1;
-        double e0 = F[(long)ie * st], e1 = F[(long)(ie - 1) * st], e2 = F[(long)(ie - 2) * st], e3 = F[(long)(ie - 3) * st];
-        double s = -(-3.5 * e0 + 4.0 * e1 - 0.5 * e2) / dx;
-        double ss = (34.0 / 3.0 * e0 - 83.0 / 4.0 * e1 + 10.0 * e2 - 7.0 / 12.0 * e3) / dx2;
-        double Sn = s - (A1[ie] * S + B1[ie] * SS);
-        double SSn = ss - (AA1[ie] * S + BB1[ie] * SS);
-        double Den = DEN[ie];
-        double SolS = -(B2[ie] * SSn - BB2[ie] * Sn) / Den;
-        double SolSS = (A2[ie] * SSn - AA2[ie] * Sn) / Den;
-        X[(long)ie * st] = SolS;
-        XX[(long)ie * st] = SolSS;
-        // back substitution
-        for (int i = ie - 1; i >= 0; --i) {
-            double Si = X[(long)i * st] - (A3[i] * SolS + B3[i] * SolSS);
-            double SSi = XX[(long)i * st] - (AA3[i] * SolS + BB3[i] * SolSS);
-            double D = DEN[i];
-            SolS = -(B2[i] * SSi - BB2[i] * Si) / D;
-            SolSS = (A2[i] * SSi - AA2[i] * Si) / D;
-            X[(long)i * st] = SolS;
-            XX[(long)i * st] = SolSS;
+        const double *q = co + (long)ie * NPACK;
+        const double s = -(-3.5 * e0 + 4.0 * e1 - 0.5 * e2) * idx;
+        const double ss = (34.0 / 3.0 * e0 - 83.0 / 4.0 * e1 + 10.0 * e2 - 7.0 / 12.0 * e3) * idx2;
+        const double Sn = s - (q[PF_A1] * S + q[PF_B1] * SS);
+        const double SSn = ss - (q[PF_AA1] * S + q[PF_BB1] * SS);
+        S = q[PB_BB2D] * Sn - q[PB_B2D] * SSn;
+        SS = q[PB_A2D] * SSn - q[PB_AA2D] * Sn;
+        X[(long)ie * st] = S;
+        XX[(long)ie * st] = SS;
+    }
+    // back substitution
+    for (int i1 = ie - 1; i1 >= 0; i1 -= CH) {
+        double xv[CH], xxv[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+            if (i1 - u >= 0) {
+                xv[u] = X[(long)(i1 - u) * st];
+                xxv[u] = XX[(long)(i1 - u) * st];
+            }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            const int i = i1 - u;
+            if (i >= 0) {
+                const double *q = co + (long)i * NPACK;
+                const double Si = xv[u] - (q[PB_A3] * S + q[PB_B3] * SS);
+                const double SSi = xxv[u] - (q[PB_AA3] * S + q[PB_BB3] * SS);
+                S = q[PB_BB2D] * Si - q[PB_B2D] * SSi;
+                SS = q[PB_A2D] * SSi - q[PB_AA2D] * Si;
+                X[(long)i * st] = S;
+                XX[(long)i * st] = SS;
+            }
         }
     }
 }
@@ -159,10 +200,15 @@ __global__ void __launch_bounds__(256) k_ccd_lines(const double *__restrict__ f,
 int ccd_dir(fcfd_ctx *c, int dir, const double *f, double *fx, double *fxx) {
     const Geom &G = c->G;
     long nl = dir == 0 ? (long)G.ny * G.nz : (dir == 1 ? (long)G.nx * G.nz : (long)G.nx * G.ny);
-    int blk = 64;
+    constexpr int blk = 64, CH = 16;
+    dim3 grd((unsigned)((nl + blk - 1) / blk));
     ProfScope ps_(c, R_CCD);
-    hipLaunchKernelGGL(k_ccd_lines, dim3((unsigned)((nl + blk - 1) / blk)), dim3(blk), 0, c->stream, f, fx, fxx,
-                       c->ccd_coef[dir], G, dir);
+    if (dir == 0)
+        hipLaunchKernelGGL((k_ccd_lines<0, CH>), grd, dim3(blk), 0, c->stream, f, fx, fxx, c->ccd_coef[0], G);
+    else if (dir == 1)
+        hipLaunchKernelGGL((k_ccd_lines<1, CH>), grd, dim3(blk), 0, c->stream, f, fx, fxx, c->ccd_coef[1], G);
+    else
+        hipLaunchKernelGGL((k_ccd_lines<2, CH>), grd, dim3(blk), 0, c->stream, f, fx, fxx, c->ccd_coef[2], G);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }

@@ -207,7 +207,9 @@ extern "C" int fcfd_create(const fcfd_grid *grid, const fcfd_params *params, fcf
         long need = std::max(8L * 1024, 2 * nbi + 16);
         c->red.nblocks = (int)((need + 7) / 8);
     }
-    if (rc == FCFD_OK) rc = alloc_field(&c->red.partial, 8L * c->red.nblocks);
+    // + 4 slots x 256 second-stage partials of the level-0 reductions
+    if (rc == FCFD_OK) rc = alloc_field(&c->red.partial, 8L * c->red.nblocks + 4 * 256);
+    if (rc == FCFD_OK) c->red.partial2 = c->red.partial + 8L * c->red.nblocks;
     if (rc == FCFD_OK) rc = alloc_field(&c->red.result, 16);
     if (rc == FCFD_OK && hipHostMalloc((void **)&c->red.host, 16 * sizeof(double)) != hipSuccess) rc = FCFD_ERR_HIP;
     if (rc == FCFD_OK && hipMalloc((void **)&c->flag, 16 * sizeof(unsigned long long)) != hipSuccess) rc = FCFD_ERR_HIP;

@@ -76,6 +76,7 @@ struct Prof {
 
 struct Reduce {
     double *partial;         // per-block partial results (nslots * nblocks)
+    double *partial2;        // second-stage partials (4 slots x 256), inside `partial`
     double *result;          // reduced values (device)
     double *host;            // pinned host mirror
     int nblocks;
@@ -137,6 +138,20 @@ struct ProfScope {
 }  // namespace fcfd
 
 namespace fcfd {
+
+// Sum over the threads of a 256-thread block in a fixed order: a butterfly
+// inside each wave (lane 0's value is used), then the four wave sums in
+// order.  Every thread gets the result; every thread of the block must call
+// it; sh4 is LDS scratch of 4 doubles that may be reused after the call.
+__device__ __forceinline__ double block_sum256(double v, double *sh4) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((t & (WAVE - 1)) == 0) sh4[t / WAVE] = v;
+    __syncthreads();
+    const double s = (sh4[0] + sh4[1]) + (sh4[2] + sh4[3]);
+    __syncthreads();
+    return s;
+}
 
 // launch helpers --------------------------------------------------------
 inline dim3 grid_interior(const Geom &G, dim3 blk) {

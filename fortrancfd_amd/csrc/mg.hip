@@ -242,77 +242,38 @@ __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, lo
     if (t < n) b[t] = (float)a[t];
 }
 
-// FP64 residual of level 0 with its squared norm folded into block partials;
-// with r32 != null the residual is also stored in FP32 (mixed mode rhs) and
-// the FP32 error is zeroed.  Also accumulates sum f^2 when fnorm is set.
-template <int DIM>
-__global__ void __launch_bounds__(256) k_resid_norm(const double *__restrict__ p, const double *__restrict__ f,
-                                                    const double *__restrict__ rho, Lay L, Inv2 ih,
-                                                    float *__restrict__ r32, float *__restrict__ e32,
-                                                    double *__restrict__ partial, int with_f) {
-    __shared__ double s0[256], s1[256];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
-    double rr = 0.0, ff = 0.0;
-    if (i < L.nx && j < L.ny) {
-        long id = L.id(i, j, k);
-        double r = f[id] - apply_A<double, DIM, true>(p, rho, nullptr, nullptr, nullptr, L, ih, i, j, k, id);
-        rr = r * r;
-        if (with_f) ff = f[id] * f[id];
-        if (r32) {
-            r32[id] = (float)r;
-            e32[id] = 0.0f;
-        }
-    }
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    s0[t] = rr;
-    s1[t] = ff;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) {
-            s0[t] += s0[t + w];
-            s1[t] += s1[t + w];
-        }
-        __syncthreads();
-    }
-    if (t == 0) {
-        long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-        long nb = (long)gridDim.x * gridDim.y * gridDim.z;
-        partial[b] = s0[0];
-        partial[nb + b] = s1[0];
-    }
+// Fixed-order two-stage sum of the block partials of a level-0 launch:
+// stage 1, block (b, s) sums the contiguous chunk b of slot s into
+// partial2[s*256 + b]; stage 2 (one block) sums the 256 chunk sums per slot.
+__global__ void __launch_bounds__(256) k_sum_stage1(const double *__restrict__ partial, long nb, long chunk,
+                                                    double *__restrict__ partial2) {
+    __shared__ double sh[4];
+    const int s = blockIdx.y;
+    const long b0 = (long)blockIdx.x * chunk, b1 = b0 + chunk < nb ? b0 + chunk : nb;
+    double a = 0.0;
+    for (long b = b0 + threadIdx.x; b < b1; b += 256) a += partial[s * nb + b];
+    a = block_sum256(a, sh);
+    if (threadIdx.x == 0) partial2[s * 256 + blockIdx.x] = a;
 }
 
-// sum the block partials of k_resid_norm / k_sum (fixed order)
-__global__ void k_sum_partials(const double *__restrict__ partial, long nb, int ns, double *__restrict__ out) {
-    __shared__ double sh[256];
+__global__ void __launch_bounds__(256) k_sum_stage2(const double *__restrict__ partial2, int ns,
+                                                    double *__restrict__ out) {
+    __shared__ double sh[4];
     for (int s = 0; s < ns; ++s) {
-        double a = 0.0;
-        for (long b = threadIdx.x; b < nb; b += 256) a += partial[s * nb + b];
-        sh[threadIdx.x] = a;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[s] = sh[0];
-        __syncthreads();
+        double a = block_sum256(partial2[s * 256 + threadIdx.x], sh);
+        if (threadIdx.x == 0) out[s] = a;
     }
 }
 
 // sum of an interior field into block partials
-__global__ void k_sum(const double *__restrict__ a, Lay L, double *__restrict__ partial) {
-    __shared__ double s0[256];
+__global__ void __launch_bounds__(256) k_sum(const double *__restrict__ a, Lay L, double *__restrict__ partial) {
+    __shared__ double sh[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
     double v = 0.0;
     if (i < L.nx && j < L.ny) v = a[L.id(i, j, k)];
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    s0[t] = v;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) s0[t] += s0[t + w];
-        __syncthreads();
-    }
-    if (t == 0) partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = s0[0];
+    v = block_sum256(v, sh);
+    if (threadIdx.x + threadIdx.y == 0)
+        partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = v;
 }
 
 // a -= scale * s[0]  (interior)
@@ -458,6 +419,105 @@ __global__ void __launch_bounds__(1024) k_coarse_cg(T *__restrict__ eout, const 
     }
 }
 
+// Coarsest levels of at most CDENSE cells are solved exactly with a dense
+// inverse built once per coefficient setup.  The matrix is B = -A (symmetric,
+// positive semi-definite, null space = constants) shifted to
+// M = B + sigma 1 1^T, which is positive definite; for a zero-mean rhs b the
+// solution of M x = b has zero mean and solves B x = b.  sigma = mean(diag B)/n
+// keeps the shift at the scale of the operator.
+constexpr int CDENSE = 64;
+
+__device__ __forceinline__ void cell_coords(const Lay &L, int c, int &i, int &j, int &k) {
+    i = c % L.nx;
+    const int r = c / L.nx;
+    j = r % L.ny;
+    k = r / L.ny;
+}
+
+// In-place Gauss-Jordan inversion in LDS (no pivoting: M is SPD); one block
+// of 256 threads; writes M^-1 (symmetric) to minv[n*n].
+template <int DIM>
+__global__ void __launch_bounds__(256) k_coarse_inv(const double *__restrict__ bx, const double *__restrict__ by,
+                                                    const double *__restrict__ bz, Lay L, Inv2 ih,
+                                                    double *__restrict__ minv) {
+    __shared__ double M[CDENSE * CDENSE];
+    __shared__ double col[CDENSE];
+    __shared__ double sh[4];
+    const int t = threadIdx.x, n = L.nx * L.ny * L.nz;
+    double dsum = 0.0;
+    for (int c = t; c < n; c += 256) {
+        int i, j, k;
+        cell_coords(L, c, i, j, k);
+        const long id = L.id(i, j, k);
+        double d = (bx[id] + bx[id - 1]) * ih.x + (by[id] + by[id - L.sx]) * ih.y;
+        if (DIM == 3) d += (bz[id] + bz[id - L.sxy]) * ih.z;
+        dsum += d;
+    }
+    const double sigma = block_sum256(dsum, sh) / ((double)n * (double)n);
+    for (int e = t; e < n * n; e += 256) M[e] = sigma;
+    __syncthreads();
+    for (int c = t; c < n; c += 256) {   // row c of B, added to the shift
+        int i, j, k;
+        cell_coords(L, c, i, j, k);
+        const long id = L.id(i, j, k);
+        const double ce_ = bx[id] * ih.x, cw = bx[id - 1] * ih.x, cn = by[id] * ih.y, cs = by[id - L.sx] * ih.y;
+        double d = ce_ + cw + cn + cs;
+        if (i < L.nx - 1) M[c * n + c + 1] -= ce_;
+        if (i > 0) M[c * n + c - 1] -= cw;
+        if (j < L.ny - 1) M[c * n + c + L.nx] -= cn;
+        if (j > 0) M[c * n + c - L.nx] -= cs;
+        if (DIM == 3) {
+            const double ct = bz[id] * ih.z, cb = bz[id - L.sxy] * ih.z;
+            const int pz = L.nx * L.ny;
+            d += ct + cb;
+            if (k < L.nz - 1) M[c * n + c + pz] -= ct;
+            if (k > 0) M[c * n + c - pz] -= cb;
+        }
+        M[c * n + c] += d;
+    }
+    __syncthreads();
+    for (int kk = 0; kk < n; ++kk) {
+        if (t < n) col[t] = M[t * n + kk];
+        __syncthreads();
+        const double piv = col[kk];
+        if (t < n) M[kk * n + t] = (t == kk ? 1.0 : M[kk * n + t]) / piv;
+        __syncthreads();
+        for (int e = t; e < n * n; e += 256) {
+            const int r = e / n, cc = e - r * n;
+            if (r != kk) M[e] = (cc == kk ? 0.0 : M[e]) - col[r] * M[kk * n + cc];
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < n * n; e += 256) minv[e] = M[e];
+}
+
+// e = M^-1 b with b = -(f - mean f), then the zero-mean projection of e;
+// one wave (n <= 64 = WAVE cells, one cell per lane)
+template <typename T>
+__global__ void __launch_bounds__(WAVE) k_coarse_dense(T *__restrict__ eout, const T *__restrict__ fin,
+                                                       const double *__restrict__ minv, Lay L) {
+    __shared__ double b[CDENSE];
+    const int t = threadIdx.x, n = L.nx * L.ny * L.nz;
+    int i = 0, j = 0, k = 0;
+    double f = 0.0;
+    if (t < n) {
+        cell_coords(L, t, i, j, k);
+        f = (double)fin[L.id(i, j, k)];
+    }
+    auto wave_sum = [](double v) {
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+        return __shfl(v, 0, WAVE);   // lane 0's order for every lane
+    };
+    const double fm = wave_sum(f) / (double)n;
+    if (t < n) b[t] = fm - f;
+    __syncthreads();
+    double x = 0.0;
+    if (t < n)
+        for (int c = 0; c < n; ++c) x += minv[c * n + t] * b[c];
+    const double xm = wave_sum(x) / (double)n;
+    if (t < n) eout[L.id(i, j, k)] = (T)(x - xm);
+}
+
 // ------------------------------------------------------------ host side
 
 static dim3 grd3(int nx, int ny, int nz, dim3 blk) {
@@ -520,12 +580,10 @@ static int mg_build(fcfd_ctx *c) {
     L0.hx = G.dx; L0.hy = G.dy; L0.hz = G.dim == 3 ? G.dz : 1.0;
     L0.sx = G.sx; L0.sxy = G.sxy; L0.ntot = G.ntot;
     c->mg.push_back(L0);
-    const double hmin0 = G.dim == 3 ? std::min({G.dx, G.dy, G.dz}) : std::min(G.dx, G.dy);
-    (void)hmin0;
     for (;;) {
         MgLevel &F = c->mg.back();
         long ncell = (long)F.nx * F.ny * F.nz;
-        if (ncell <= 512) break;
+        if (ncell <= CDENSE && c->mg.size() > 1) break;
         double hmin = G.dim == 3 ? std::min({F.hx, F.hy, F.hz}) : std::min(F.hx, F.hy);
         int cf[3] = {1, 1, 1};
         const int n[3] = {F.nx, F.ny, F.nz};
@@ -533,7 +591,13 @@ static int mg_build(fcfd_ctx *c) {
         bool any = false;
         for (int d = 0; d < G.dim; ++d)
             if (n[d] % 2 == 0 && n[d] >= 4 && h[d] <= 1.5 * hmin) { cf[d] = 2; any = true; }
-        if (!any) break;
+        if (!any) {
+            if (c->mg.size() == 1) {
+                fprintf(stderr, "libfcfd: PPE multigrid needs an even cell count (>= 4) on some axis\n");
+                return FCFD_ERR_ARG;
+            }
+            break;
+        }
         F.cx = cf[0]; F.cy = cf[1]; F.cz = cf[2];
         MgLevel C = {};
         C.nx = F.nx / cf[0]; C.ny = F.ny / cf[1]; C.nz = G.dim == 3 ? F.nz / cf[2] : 1;
@@ -565,9 +629,9 @@ static int mg_build(fcfd_ctx *c) {
     FCFD_TRY(alloc0(&c->kz, G.ntot));
     FCFD_TRY(alloc0(&c->kzo, G.ntot));
     FCFD_TRY(alloc0(&c->kzo32, G.ntot));
-    // coarse CG workspace (6 vectors) + scalars
+    // scalars + coarse CG workspace (6 vectors) or the dense coarse inverse
     long ncoarse = (long)c->mg.back().nx * c->mg.back().ny * c->mg.back().nz;
-    FCFD_TRY(alloc0(&c->mg_scalars, 64 + 6 * ncoarse));
+    FCFD_TRY(alloc0(&c->mg_scalars, 64 + std::max(6 * ncoarse, ncoarse * ncoarse)));
     return FCFD_OK;
 }
 
@@ -605,6 +669,21 @@ int mg_setup(fcfd_ctx *c) {
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((G.ntot + 255) / 256)), dim3(256), 0, c->stream, c->f[FCFD_RHO],
                        c->rho32, G.ntot);
     FCFD_CHECK(hipGetLastError());
+    {
+        MgLevel &M = c->mg.back();
+        const long n = (long)M.nx * M.ny * M.nz;
+        if (c->mg.size() > 1 && n <= CDENSE) {
+            Lay L = lay_of(M, G.dim);
+            Inv2 ih = inv2(M.hx, M.hy, M.hz);
+            if (G.dim == 3)
+                hipLaunchKernelGGL(k_coarse_inv<3>, dim3(1), dim3(256), 0, c->stream, M.bx64, M.by64, M.bz64, L, ih,
+                                   c->mg_scalars + 64);
+            else
+                hipLaunchKernelGGL(k_coarse_inv<2>, dim3(1), dim3(256), 0, c->stream, M.bx64, M.by64, nullptr, L, ih,
+                                   c->mg_scalars + 64);
+            FCFD_CHECK(hipGetLastError());
+        }
+    }
     c->mg_ready = 1;
     return FCFD_OK;
 }
@@ -675,9 +754,13 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         arrays(nl - 1, e, f, bx, by, bz);
         long n = (long)M.nx * M.ny * M.nz;
         ProfScope ps_(c, R_COARSE);
-        hipLaunchKernelGGL((k_coarse_cg<T, DIM>), dim3(1), dim3(n >= 1024 ? 1024 : 256), 0, c->stream, e, f, bx, by,
-                           bz, M.bx64, M.by64, M.bz64, c->mg_scalars + 64, L, ih, (int)std::min<long>(n, 96),
-                           0.0);
+        if (nl > 1 && n <= CDENSE)
+            hipLaunchKernelGGL((k_coarse_dense<T>), dim3(1), dim3(WAVE), 0, c->stream, e, f,
+                               (const double *)(c->mg_scalars + 64), L);
+        else
+            hipLaunchKernelGGL((k_coarse_cg<T, DIM>), dim3(1), dim3(n >= 1024 ? 1024 : 256), 0, c->stream, e, f, bx,
+                               by, bz, M.bx64, M.by64, M.bz64, c->mg_scalars + 64, L, ih, (int)std::min<long>(n, 96),
+                               0.0);
         FCFD_CHECK(hipGetLastError());
     }
     // up
@@ -710,7 +793,7 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
                                                    double *__restrict__ r64, float *__restrict__ r32,
                                                    float *__restrict__ e32, double *__restrict__ z64,
                                                    double *__restrict__ partial, int with_f) {
-    __shared__ double s0[256], s1[256];
+    __shared__ double sh[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
     double rr = 0.0, ff = 0.0;
     if (i < L.nx && j < L.ny) {
@@ -722,19 +805,13 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
         if (r32) { r32[id] = (float)r; e32[id] = 0.0f; }
         if (z64) z64[id] = 0.0;
     }
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    s0[t] = rr;
-    s1[t] = ff;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) { s0[t] += s0[t + w]; s1[t] += s1[t + w]; }
-        __syncthreads();
-    }
-    if (t == 0) {
+    rr = block_sum256(rr, sh);
+    ff = block_sum256(ff, sh);
+    if (threadIdx.x + threadIdx.y == 0) {
         long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
         long nb = (long)gridDim.x * gridDim.y * gridDim.z;
-        partial[b] = s0[0];
-        partial[nb + b] = s1[0];
+        partial[b] = rr;
+        partial[nb + b] = ff;
     }
 }
 
@@ -742,7 +819,7 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
 template <typename T>
 __global__ void __launch_bounds__(256) k_fcg_dots(const double *__restrict__ r, const T *__restrict__ z,
                                                   const T *__restrict__ zo, Lay L, double *__restrict__ partial) {
-    __shared__ double s0[256], s1[256];
+    __shared__ double sh[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
     double a = 0.0, b = 0.0;
     if (i < L.nx && j < L.ny) {
@@ -750,19 +827,13 @@ __global__ void __launch_bounds__(256) k_fcg_dots(const double *__restrict__ r, 
         a = r[id] * (double)z[id];
         b = r[id] * (double)zo[id];
     }
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    s0[t] = a;
-    s1[t] = b;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) { s0[t] += s0[t + w]; s1[t] += s1[t + w]; }
-        __syncthreads();
-    }
-    if (t == 0) {
+    a = block_sum256(a, sh);
+    b = block_sum256(b, sh);
+    if (threadIdx.x + threadIdx.y == 0) {
         long bb = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
         long nb = (long)gridDim.x * gridDim.y * gridDim.z;
-        partial[bb] = s0[0];
-        partial[nb + bb] = s1[0];
+        partial[bb] = a;
+        partial[nb + bb] = b;
     }
 }
 
@@ -773,7 +844,7 @@ __global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__r
                                                  double *__restrict__ p, double *__restrict__ q,
                                                  const double *__restrict__ rho, Lay L, Inv2 ih, double beta,
                                                  double *__restrict__ partial) {
-    __shared__ double s0[256];
+    __shared__ double sh[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
     double pq = 0.0;
     if (i < L.nx && j < L.ny) {
@@ -796,14 +867,9 @@ __global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__r
         zo[id] = z[id];
         pq = pn * qn;
     }
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    s0[t] = pq;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) s0[t] += s0[t + w];
-        __syncthreads();
-    }
-    if (t == 0) partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = s0[0];
+    pq = block_sum256(pq, sh);
+    if (threadIdx.x + threadIdx.y == 0)
+        partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = pq;
 }
 
 // x += alpha p, r -= alpha q, ||r||^2; the new residual also becomes the
@@ -812,7 +878,7 @@ __global__ void __launch_bounds__(256) k_fcg_update(double *__restrict__ x, cons
                                                     const double *__restrict__ q, double *__restrict__ r, Lay L,
                                                     double alpha, float *__restrict__ r32, float *__restrict__ e32,
                                                     double *__restrict__ z64, double *__restrict__ partial) {
-    __shared__ double s0[256];
+    __shared__ double sh[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
     double rr = 0.0;
     if (i < L.nx && j < L.ny) {
@@ -824,20 +890,24 @@ __global__ void __launch_bounds__(256) k_fcg_update(double *__restrict__ x, cons
         if (r32) { r32[id] = (float)rn; e32[id] = 0.0f; }
         if (z64) z64[id] = 0.0;
     }
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    s0[t] = rr;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) s0[t] += s0[t + w];
-        __syncthreads();
-    }
-    if (t == 0) partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = s0[0];
+    rr = block_sum256(rr, sh);
+    if (threadIdx.x + threadIdx.y == 0)
+        partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = rr;
+}
+
+// out[0..ns) = sums of the ns slots of the block partials of a level-0 launch
+static int sum_partials(fcfd_ctx *c, long nb, int ns, double *out) {
+    const long chunk = (nb + 255) / 256;
+    hipLaunchKernelGGL(k_sum_stage1, dim3(256, ns), dim3(256), 0, c->stream, c->red.partial, nb, chunk,
+                       c->red.partial2);
+    hipLaunchKernelGGL(k_sum_stage2, dim3(1), dim3(256), 0, c->stream, c->red.partial2, ns, out);
+    FCFD_CHECK(hipGetLastError());
+    return FCFD_OK;
 }
 
 // sum ns slots of the block partials of a level-0 launch and bring them to the host
 static int sums_to_host(fcfd_ctx *c, long nb, int ns, double *out) {
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->red.partial, nb, ns, c->mg_scalars);
-    FCFD_CHECK(hipGetLastError());
+    FCFD_TRY(sum_partials(c, nb, ns, c->mg_scalars));
     FCFD_CHECK(hipMemcpyAsync(c->red.host, c->mg_scalars, ns * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     FCFD_CHECK(hipStreamSynchronize(c->stream));
     for (int s = 0; s < ns; ++s) out[s] = c->red.host[s];
@@ -850,8 +920,7 @@ static int field_mean(fcfd_ctx *c, const double *a) {
     dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
     long nb = (long)g.x * g.y * g.z;
     hipLaunchKernelGGL(k_sum, g, blk, 0, c->stream, a, lay_of(G), c->red.partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->red.partial, nb, 1, c->mg_scalars + 4);
-    FCFD_CHECK(hipGetLastError());
+    FCFD_TRY(sum_partials(c, nb, 1, c->mg_scalars + 4));
     return FCFD_OK;
 }
 
