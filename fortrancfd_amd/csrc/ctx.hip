@@ -212,7 +212,8 @@ extern "C" int fcfd_create(const fcfd_grid *grid, const fcfd_params *params, fcf
     if (rc == FCFD_OK) c->red.partial2 = c->red.partial + 8L * c->red.nblocks;
     if (rc == FCFD_OK) rc = alloc_field(&c->red.result, 16);
     if (rc == FCFD_OK && hipHostMalloc((void **)&c->red.host, 16 * sizeof(double)) != hipSuccess) rc = FCFD_ERR_HIP;
-    if (rc == FCFD_OK && hipMalloc((void **)&c->flag, 16 * sizeof(unsigned long long)) != hipSuccess) rc = FCFD_ERR_HIP;
+    if (rc == FCFD_OK && hipMalloc((void **)&c->flag, (FLAG_SHARD0 + NSHARD) * sizeof(unsigned long long)) != hipSuccess)
+        rc = FCFD_ERR_HIP;
     if (rc == FCFD_OK && hipHostMalloc((void **)&c->flag_host, 16 * sizeof(unsigned long long)) != hipSuccess)
         rc = FCFD_ERR_HIP;
     if (rc == FCFD_OK) rc = ccd_setup(c);

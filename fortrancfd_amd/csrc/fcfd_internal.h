@@ -153,6 +153,39 @@ __device__ __forceinline__ double block_sum256(double v, double *sh4) {
     return s;
 }
 
+// Workgroups of a launch are dealt to the 8 XCDs round-robin by linear id
+// (x fastest, then y, then z).  Renumbering them so that XCD x owns the
+// contiguous range of logical blocks [x*nb/8, (x+1)*nb/8) keeps the k-1 and
+// k+1 planes a stencil re-reads in the L2 of the XCD that read them first.
+// Returns the logical (bx, by, bz) of the calling workgroup.
+constexpr unsigned NXCD = 8;
+__device__ __forceinline__ void xcd_block(int &bx, int &by, int &bz) {
+    const unsigned nbx = gridDim.x, nby = gridDim.y, nb = nbx * nby * gridDim.z;
+    const unsigned w = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    const unsigned q = nb / NXCD, rem = nb % NXCD, x = w % NXCD;
+    const unsigned l = x * q + (x < rem ? x : rem) + w / NXCD;
+    bx = (int)(l % nbx);
+    const unsigned r = l / nbx;
+    by = (int)(r % nby);
+    bz = (int)(r / nby);
+}
+
+// Max over the threads of a 256-thread block (values >= 0 or any order-free
+// max); every thread gets the result; every thread of the block must call it.
+__device__ __forceinline__ double block_max256(double v, double *sh4) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((t & (WAVE - 1)) == 0) sh4[t / WAVE] = v;
+    __syncthreads();
+    const double m = fmax(fmax(sh4[0], sh4[1]), fmax(sh4[2], sh4[3]));
+    __syncthreads();
+    return m;
+}
+
+// device flag words: [0] unused, [1] reinit done, [2] reinit pseudo-step
+// count, [3] max |grad phi| bits; [FLAG_SHARD0, +NSHARD) per-block max slots
+constexpr int FLAG_SHARD0 = 16, NSHARD = 256;
+
 // launch helpers --------------------------------------------------------
 inline dim3 grid_interior(const Geom &G, dim3 blk) {
     return dim3((G.nx + blk.x - 1) / blk.x, (G.ny + blk.y - 1) / blk.y, G.nz);

@@ -38,36 +38,50 @@ __device__ __forceinline__ void lsfun(double phi, double w, double &H, double &D
     H = fmax(fmin(H, 1.0), 0.0);
 }
 
+// WENO5-JS nonlinear weights w_k = a_k / sum a, a_k = d_k / (EPS + beta_k)^2,
+// d = (1, 6, 3) (level_set_rk3_solver.f90:170-216).  Multiplying every a_k
+// by q1 q2 q3 (q_k = (EPS + beta_k)^2) leaves the weights unchanged and needs
+// one division instead of six: w_k = (d_k prod_{m != k} q_m) / sum_l (...).
+// With |phi| and |u| of O(1) the products stay far inside the FP64 range
+// (q >= EPS^2 = 1e-26, so a product of two q is >= 1e-52).
+__device__ __forceinline__ void weno_weights(double b1, double b2, double b3, double EPS, double &w1, double &w2,
+                                             double &w3) {
+    const double q1 = (EPS + b1) * (EPS + b1), q2 = (EPS + b2) * (EPS + b2), q3 = (EPS + b3) * (EPS + b3);
+    const double a1 = q2 * q3, a2 = 6.0 * (q1 * q3), a3 = 3.0 * (q1 * q2);
+    const double r = 1.0 / (a1 + a2 + a3);
+    w1 = a1 * r;
+    w2 = a2 * r;
+    w3 = a3 * r;
+}
+
 // WENO5-JS left-biased face value from g(i-2..i+2) (level_set_rk3_solver.f90:170-191)
 __device__ __forceinline__ double weno_left(double g0, double g1, double g2, double g3, double g4, double EPS) {
     double t1 = g0 - 2.0 * g1 + g2, t2 = g0 - 4.0 * g1 + 3.0 * g2;
-    double b1 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
+    const double b1 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
     t1 = g1 - 2.0 * g2 + g3; t2 = g1 - g3;
-    double b2 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
+    const double b2 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
     t1 = g2 - 2.0 * g3 + g4; t2 = 3.0 * g2 - 4.0 * g3 + g4;
-    double b3 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
-    double a1 = 1.0 / ((EPS + b1) * (EPS + b1));
-    double a2 = 6.0 / ((EPS + b2) * (EPS + b2));
-    double a3 = 3.0 / ((EPS + b3) * (EPS + b3));
-    double w1 = a1 / (a1 + a2 + a3), w2 = a2 / (a1 + a2 + a3), w3 = a3 / (a1 + a2 + a3);
-    return w1 / 3.0 * g0 - (7.0 * w1 + w2) / 6.0 * g1 + (11.0 * w1 + 5.0 * w2 + 2.0 * w3) / 6.0 * g2 +
-           (2.0 * w2 + 5.0 * w3) / 6.0 * g3 - w3 / 6.0 * g4;
+    const double b3 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
+    double w1, w2, w3;
+    weno_weights(b1, b2, b3, EPS, w1, w2, w3);
+    constexpr double r6 = 1.0 / 6.0;
+    return w1 * (1.0 / 3.0) * g0 - (7.0 * w1 + w2) * r6 * g1 + (11.0 * w1 + 5.0 * w2 + 2.0 * w3) * r6 * g2 +
+           (2.0 * w2 + 5.0 * w3) * r6 * g3 - w3 * r6 * g4;
 }
 
 // WENO5-JS right-biased face value from f(i-1..i+3) (level_set_rk3_solver.f90:193-216)
 __device__ __forceinline__ double weno_right(double f0, double f1, double f2, double f3, double f4, double EPS) {
     double t1 = f0 - 2.0 * f1 + f2, t2 = f0 - 4.0 * f1 + 3.0 * f2;
-    double b3 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
+    const double b3 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
     t1 = f1 - 2.0 * f2 + f3; t2 = f1 - f3;
-    double b2 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
+    const double b2 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
     t1 = f2 - 2.0 * f3 + f4; t2 = 3.0 * f2 - 4.0 * f3 + f4;
-    double b1 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
-    double a1 = 1.0 / ((EPS + b1) * (EPS + b1));
-    double a2 = 6.0 / ((EPS + b2) * (EPS + b2));
-    double a3 = 3.0 / ((EPS + b3) * (EPS + b3));
-    double w1 = a1 / (a1 + a2 + a3), w2 = a2 / (a1 + a2 + a3), w3 = a3 / (a1 + a2 + a3);
-    return w3 * (-f0 + 5.0 * f1 + 2.0 * f2) / 6.0 + w2 * (2.0 * f1 + 5.0 * f2 - f3) / 6.0 +
-           w1 * (11.0 * f2 - 7.0 * f3 + 2.0 * f4) / 6.0;
+    const double b1 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
+    double w1, w2, w3;
+    weno_weights(b1, b2, b3, EPS, w1, w2, w3);
+    constexpr double r6 = 1.0 / 6.0;
+    return w3 * (-f0 + 5.0 * f1 + 2.0 * f2) * r6 + w2 * (2.0 * f1 + 5.0 * f2 - f3) * r6 +
+           w1 * (11.0 * f2 - 7.0 * f3 + 2.0 * f4) * r6;
 }
 
 // ------------------------------------------------------------- properties
@@ -185,7 +199,9 @@ template <int DIM>
 __global__ void k_mpls_update(double *__restrict__ phi, const double *__restrict__ grad,
                               const double *__restrict__ sums, double *__restrict__ lam_out, Geom G, double w,
                               double imass) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     double lam = (imass - sums[0]) / sums[2];
     if (i == 0 && j == 0 && k == 0) *lam_out = lam;
     if (i >= G.nx || j >= G.ny) return;
@@ -264,7 +280,9 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage(const double *__restrict__
                                                       const double *__restrict__ nu, const double *__restrict__ nv,
                                                       const double *__restrict__ nw, double *__restrict__ l1,
                                                       double *__restrict__ l2, Geom G, double dt, int stage) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
     const double *vel[3] = {nu, nv, nw};
@@ -343,13 +361,15 @@ __global__ void __launch_bounds__(256) k_redis_a(const double *__restrict__ phi,
                                                  double *__restrict__ s3, const unsigned long long *__restrict__ done,
                                                  Geom G, double w, double r12, int btn) {
     if (done && done[1]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
     const int n[3] = {G.nx, G.ny, G.nz};
     const int pos[3] = {i, j, k};
     const long st[3] = {1, G.sx, G.sxy};
-    const double h[3] = {G.dx, G.dy, G.dz};
+    const double ih[3] = {1.0 / G.dx, 1.0 / G.dy, 1.0 / G.dz};
     double sel[3];
     const bool pos_sign = S0[id] >= 0.0;
 #pragma unroll
@@ -359,7 +379,7 @@ __global__ void __launch_bounds__(256) k_redis_a(const double *__restrict__ phi,
         for (int o = -3; o <= 3; ++o) {
             int q = clampi(pos[d] + o, 0, n[d] - 1);
             long a = id + (long)(q - pos[d]) * st[d];
-            nr[o + 3] = (phi[a] - phi[a - st[d]]) / h[d];
+            nr[o + 3] = (phi[a] - phi[a - st[d]]) * ih[d];
         }
         const double EPS = 1.0e-10;
         double fm = weno_left(nr[1], nr[2], nr[3], nr[4], nr[5], EPS);   // tdata%s2
@@ -382,7 +402,8 @@ __global__ void __launch_bounds__(256) k_redis_a(const double *__restrict__ phi,
 // Pass B: 3^d weighted sums (centre weight 16 in 2D, 51 in 3D, bc() copy as
 // clamped indices), correction s1 (redis_lambda :385-408), stage source and
 // in-place increment-form SSP-RK3 update (redis_solver :150-208).  The last
-// stage also folds max|phi - phi_start| into flags[0] (reinit stop test :42-54).
+// stage also folds max|phi - phi_start| into the flag shards (reinit stop
+// test :42-54).
 template <int DIM>
 __global__ void __launch_bounds__(256) k_redis_b(double *__restrict__ phi, const double *__restrict__ S0,
                                                  const double *__restrict__ grad, const double *__restrict__ s2,
@@ -391,7 +412,9 @@ __global__ void __launch_bounds__(256) k_redis_b(double *__restrict__ phi, const
                                                  unsigned long long *__restrict__ flags, Geom G, double w,
                                                  double rdt, int btn, int stage) {
     if (flags[1]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     bool in = i < G.nx && j < G.ny;
     double err = 0.0;
     if (in) {
@@ -428,34 +451,48 @@ __global__ void __launch_bounds__(256) k_redis_b(double *__restrict__ phi, const
         if (stage == 2) err = fabs(phis[id] - pn);
     }
     if (stage == 2) {
-        // wavefront max then one atomic per wavefront; max of non-negative
-        // doubles is order independent, compared through their bit patterns
-        for (int off = 32; off > 0; off >>= 1) err = fmax(err, __shfl_down(err, off, 64));
-        if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0)
-            atomicMax(&flags[0], (unsigned long long)__double_as_longlong(err));
+        // block max, then one atomic per block into one of NSHARD slots (a
+        // single slot serialises tens of thousands of atomics); max of
+        // non-negative doubles is order independent, compared as bit patterns
+        __shared__ double sh[4];
+        const double m = block_max256(err, sh);
+        if (threadIdx.x + threadIdx.y == 0) {
+            const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+            atomicMax(&flags[FLAG_SHARD0 + b % NSHARD], (unsigned long long)__double_as_longlong(m));
+        }
     }
 }
 
-// end of one pseudo step: count it, stop when error <= tol, clear the max
-__global__ void k_redis_check(unsigned long long *flags, double tol) {
+// end of one pseudo step: count it, stop when the max over the shards is
+// <= tol, clear the shards (one block of NSHARD threads)
+__global__ void __launch_bounds__(NSHARD) k_redis_check(unsigned long long *flags, double tol) {
+    __shared__ double sh[4];
     if (flags[1]) return;
-    double err = __longlong_as_double((long long)flags[0]);
-    flags[2] += 1;
-    if (err <= tol) flags[1] = 1;
-    flags[0] = 0ull;
+    const int t = threadIdx.x;
+    const double e = __longlong_as_double((long long)flags[FLAG_SHARD0 + t]);
+    const double err = block_max256(e, sh);
+    flags[FLAG_SHARD0 + t] = 0ull;
+    if (t == 0) {
+        flags[2] += 1;
+        if (err <= tol) flags[1] = 1;
+    }
 }
 
 __global__ void k_copy_interior(double *__restrict__ dst, const double *__restrict__ src,
                                 const unsigned long long *__restrict__ done, Geom G) {
     if (done && done[1]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     long id = G.id(i, j, k);
     dst[id] = src[id];
 }
 
 __global__ void k_sign_init(const double *__restrict__ phi, double *__restrict__ S0, Geom G, double w, int btn) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     long id = G.id(i, j, k);
     if (btn == 0) {
@@ -469,16 +506,20 @@ __global__ void k_sign_init(const double *__restrict__ phi, double *__restrict__
 
 // max of grad over the interior into flags[3] (level_set_redis_stable :113-122)
 __global__ void k_max_interior(const double *__restrict__ a, unsigned long long *flags, Geom G) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
+    __shared__ double sh[4];
     double v = 0.0;
     if (i < G.nx && j < G.ny) v = a[G.id(i, j, k)];
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
-    if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0)
-        atomicMax(&flags[3], (unsigned long long)__double_as_longlong(v));
+    v = block_max256(v, sh);
+    if (threadIdx.x + threadIdx.y == 0) atomicMax(&flags[3], (unsigned long long)__double_as_longlong(v));
 }
 
 __global__ void k_scale_by_flag(double *__restrict__ phi, const unsigned long long *flags, Geom G) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     double gmax = __longlong_as_double((long long)flags[3]);
     long id = G.id(i, j, k);
@@ -490,7 +531,7 @@ int redis(fcfd_ctx *c, int btn, int *iters) {
     dim3 blk(64, 4), grd = grid_interior(G, blk);
     double *phi = c->f[FCFD_PHI], *grad = c->f[FCFD_GRAD];
     const double w = c->pr.ls_wid, r12 = c->pr.rho_12, rdt = c->pr.rdt;
-    FCFD_CHECK(hipMemsetAsync(c->flag, 0, 8 * sizeof(unsigned long long), c->stream));
+    FCFD_CHECK(hipMemsetAsync(c->flag, 0, (FLAG_SHARD0 + NSHARD) * sizeof(unsigned long long), c->stream));
     // level_set_redis_init / stable (:67-136)
     hipLaunchKernelGGL(k_sign_init, grd, blk, 0, c->stream, phi, c->s0, G, w, btn);
     if (btn == 0) {
@@ -547,7 +588,7 @@ int redis(fcfd_ctx *c, int btn, int *iters) {
             FCFD_CHECK(hipGetLastError());
             FCFD_TRY(bc_scalar(c, phi));
         }
-        hipLaunchKernelGGL(k_redis_check, dim3(1), dim3(1), 0, c->stream, c->flag, tol);
+        hipLaunchKernelGGL(k_redis_check, dim3(1), dim3(NSHARD), 0, c->stream, c->flag, tol);
     }
     FCFD_CHECK(hipMemcpyAsync(c->flag_host, c->flag, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                               c->stream));

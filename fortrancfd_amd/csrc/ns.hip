@@ -49,7 +49,9 @@ __global__ void __launch_bounds__(256) k_diff_source(const double *__restrict__ 
                                                      const double *__restrict__ NYo, const double *__restrict__ NZo,
                                                      double *__restrict__ Tx, double *__restrict__ Ty,
                                                      double *__restrict__ Tz, Geom G, Phys P) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
     const long sx = G.sx, sxy = G.sxy;
@@ -147,7 +149,9 @@ __global__ void __launch_bounds__(256) k_predictor(
     double *__restrict__ Sx, double *__restrict__ Sy, double *__restrict__ Sz, double *__restrict__ Un,
     double *__restrict__ Vn, double *__restrict__ Wn, Geom G, double dt, int zero_e, int zero_n, int zero_t,
     int setup_only) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
     const long sx = G.sx, sxy = G.sxy;
@@ -203,7 +207,9 @@ __global__ void __launch_bounds__(256) k_predictor(
 template <int DIM>
 __global__ void k_ppe_src(const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
                           double *__restrict__ f, Geom G, double dt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
     double d = (U[id] - U[id - 1]) / G.dx + (V[id] - V[id - G.sx]) / G.dy;
@@ -215,7 +221,9 @@ __global__ void k_ppe_src(const double *__restrict__ U, const double *__restrict
 template <int DIM>
 __global__ void k_projection(double *__restrict__ U, double *__restrict__ V, double *__restrict__ W,
                              const double *__restrict__ P, const double *__restrict__ R, Geom G, double dt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
     double rho = (R[id] + R[id + 1]) / 2.0, px = (P[id + 1] - P[id]) / G.dx;
@@ -234,7 +242,9 @@ __global__ void k_projection(double *__restrict__ U, double *__restrict__ V, dou
 template <int DIM>
 __global__ void k_node_vel(const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
                            double *__restrict__ NU, double *__restrict__ NV, double *__restrict__ NW, Geom G) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
     NU[id] = 0.5 * (U[id - 1] + U[id]);
