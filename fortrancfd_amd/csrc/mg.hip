@@ -62,10 +62,37 @@ struct Inv2 {
     double x, y, z;  // 1/h^2 per axis
 };
 
-// face coefficient of level 0 between cells with densities a and b
+// face coefficient of level 0 between cells with densities a and b.  The
+// FP32 operator only preconditions (the FP64 outer iteration owns the
+// accuracy), so it uses the hardware reciprocal (about 1 ulp) instead of an
+// IEEE division sequence; FP64 stays correctly rounded.
 template <typename T>
 __device__ __forceinline__ T beta_rho(T a, T b) {
     return (T)2 / (a + b);
+}
+template <>
+__device__ __forceinline__ float beta_rho<float>(float a, float b) {
+    return 2.0f * __builtin_amdgcn_rcpf(a + b);
+}
+
+// Gauss-Seidel correction (sum c (e_nb - e_P) - f) / sum c, same precision
+// rule as beta_rho
+template <typename T>
+__device__ __forceinline__ T gs_step(T num, T sc) {
+    return num / sc;
+}
+template <>
+__device__ __forceinline__ float gs_step<float>(float num, float sc) {
+    return num * __builtin_amdgcn_rcpf(sc);
+}
+
+// Wall-masked face coefficient c = beta_rho(r, rn) * h, or 0 across a wall.
+// The neighbour density is always loaded (ghost cells exist on every level)
+// and the wall test selects without a branch.
+template <typename T>
+__device__ __forceinline__ T face(bool open, T r, T rn, T h) {
+    const T c = beta_rho<T>(r, rn) * h;
+    return open ? c : (T)0;
 }
 
 // Face coefficients of cell (i,j,k): sum_c = sum c_nb and
@@ -79,13 +106,13 @@ __device__ __forceinline__ void stencil(const T *__restrict__ e, const T *__rest
     T cw, ce_, cs, cn, cb = 0, ct = 0;
     if (FINE) {
         T r = rho[id];
-        ce_ = i < L.nx - 1 ? beta_rho<T>(r, rho[id + 1]) * (T)ih.x : (T)0;
-        cw = i > 0 ? beta_rho<T>(r, rho[id - 1]) * (T)ih.x : (T)0;
-        cn = j < L.ny - 1 ? beta_rho<T>(r, rho[id + L.sx]) * (T)ih.y : (T)0;
-        cs = j > 0 ? beta_rho<T>(r, rho[id - L.sx]) * (T)ih.y : (T)0;
+        ce_ = face<T>(i < L.nx - 1, r, rho[id + 1], (T)ih.x);
+        cw = face<T>(i > 0, r, rho[id - 1], (T)ih.x);
+        cn = face<T>(j < L.ny - 1, r, rho[id + L.sx], (T)ih.y);
+        cs = face<T>(j > 0, r, rho[id - L.sx], (T)ih.y);
         if (DIM == 3) {
-            ct = k < L.nz - 1 ? beta_rho<T>(r, rho[id + L.sxy]) * (T)ih.z : (T)0;
-            cb = k > 0 ? beta_rho<T>(r, rho[id - L.sxy]) * (T)ih.z : (T)0;
+            ct = face<T>(k < L.nz - 1, r, rho[id + L.sxy], (T)ih.z);
+            cb = face<T>(k > 0, r, rho[id - L.sxy], (T)ih.z);
         }
     } else {
         ce_ = bx[id] * (T)ih.x;
@@ -118,13 +145,13 @@ __device__ __forceinline__ T apply_A(const T *__restrict__ e, const T *__restric
     T acc;
     if (FINE) {
         T r = rho[id];
-        acc = i < L.nx - 1 ? beta_rho<T>(r, rho[id + 1]) * (T)ih.x * (e[id + 1] - ep) : (T)0;
-        if (i > 0) acc += beta_rho<T>(r, rho[id - 1]) * (T)ih.x * (e[id - 1] - ep);
-        if (j < L.ny - 1) acc += beta_rho<T>(r, rho[id + L.sx]) * (T)ih.y * (e[id + L.sx] - ep);
-        if (j > 0) acc += beta_rho<T>(r, rho[id - L.sx]) * (T)ih.y * (e[id - L.sx] - ep);
+        acc = face<T>(i < L.nx - 1, r, rho[id + 1], (T)ih.x) * (e[id + 1] - ep);
+        acc += face<T>(i > 0, r, rho[id - 1], (T)ih.x) * (e[id - 1] - ep);
+        acc += face<T>(j < L.ny - 1, r, rho[id + L.sx], (T)ih.y) * (e[id + L.sx] - ep);
+        acc += face<T>(j > 0, r, rho[id - L.sx], (T)ih.y) * (e[id - L.sx] - ep);
         if (DIM == 3) {
-            if (k < L.nz - 1) acc += beta_rho<T>(r, rho[id + L.sxy]) * (T)ih.z * (e[id + L.sxy] - ep);
-            if (k > 0) acc += beta_rho<T>(r, rho[id - L.sxy]) * (T)ih.z * (e[id - L.sxy] - ep);
+            acc += face<T>(k < L.nz - 1, r, rho[id + L.sxy], (T)ih.z) * (e[id + L.sxy] - ep);
+            acc += face<T>(k > 0, r, rho[id - L.sxy], (T)ih.z) * (e[id - L.sxy] - ep);
         }
     } else {
         acc = bx[id] * (T)ih.x * (e[id + 1] - ep) + bx[id - 1] * (T)ih.x * (e[id - 1] - ep) +
@@ -151,7 +178,7 @@ __global__ void __launch_bounds__(256) k_smooth(T *__restrict__ e, const T *__re
     stencil<T, DIM, FINE>(e, rho, bx, by, bz, L, ih, i, j, k, id, sc, sce);
     // correction form e_P += (sum c (e_nb - e_P) - f) / sum c: the Gauss-Seidel
     // update without cancelling terms of size c |e|
-    if (sc > (T)0) e[id] = e[id] + (sce - f[id]) / sc;
+    if (sc > (T)0) e[id] = e[id] + gs_step<T>(sce - f[id], sc);
 }
 
 // residual of the level, restricted to the coarse right-hand side by the
@@ -877,13 +904,13 @@ __global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__r
         // A z with the FP64 operator; z promoted
         const double zp = (double)z[id];
         double r0 = rho[id], acc = 0.0;
-        if (i < L.nx - 1) acc += 2.0 / (r0 + rho[id + 1]) * ih.x * ((double)z[id + 1] - zp);
-        if (i > 0) acc += 2.0 / (r0 + rho[id - 1]) * ih.x * ((double)z[id - 1] - zp);
-        if (j < L.ny - 1) acc += 2.0 / (r0 + rho[id + L.sx]) * ih.y * ((double)z[id + L.sx] - zp);
-        if (j > 0) acc += 2.0 / (r0 + rho[id - L.sx]) * ih.y * ((double)z[id - L.sx] - zp);
+        acc += face<double>(i < L.nx - 1, r0, rho[id + 1], ih.x) * ((double)z[id + 1] - zp);
+        acc += face<double>(i > 0, r0, rho[id - 1], ih.x) * ((double)z[id - 1] - zp);
+        acc += face<double>(j < L.ny - 1, r0, rho[id + L.sx], ih.y) * ((double)z[id + L.sx] - zp);
+        acc += face<double>(j > 0, r0, rho[id - L.sx], ih.y) * ((double)z[id - L.sx] - zp);
         if (DIM == 3) {
-            if (k < L.nz - 1) acc += 2.0 / (r0 + rho[id + L.sxy]) * ih.z * ((double)z[id + L.sxy] - zp);
-            if (k > 0) acc += 2.0 / (r0 + rho[id - L.sxy]) * ih.z * ((double)z[id - L.sxy] - zp);
+            acc += face<double>(k < L.nz - 1, r0, rho[id + L.sxy], ih.z) * ((double)z[id + L.sxy] - zp);
+            acc += face<double>(k > 0, r0, rho[id - L.sxy], ih.z) * ((double)z[id - L.sxy] - zp);
         }
         double pn = zp + beta * p[id];
         double qn = acc + beta * q[id];
