@@ -112,9 +112,10 @@ struct fcfd_ctx {
     // level-0 Krylov vectors of the flexible CG (FP64) and FP32 previous z
     double *kr, *kp, *kq, *kz, *kzo;
     float *kzo32;
-    // captured V-cycles: [precision][inner cycles 1..4]
-    hipGraphExec_t vgraph[3][5];      // kind: 0 fp64 on p, 1 fp64 on z (Krylov), 2 fp32 on e32
-    std::vector<std::pair<int, int>> vgraph_prof[3][5];  // (region, event) pairs inside each graph
+    // captured V-cycles: [kind][inner cycles 1..4]; kind 0 fp64 on p, 1 fp64 on
+    // z with the CG dots, 2 fp32 on e32, 3 fp32 on e32 with the CG dots
+    hipGraphExec_t vgraph[4][5];
+    std::vector<std::pair<int, int>> vgraph_prof[4][5];  // (region, event) pairs inside each graph
     int capture_kind = -1, capture_inner = 0;            // graph being captured (or -1)
     double prof_gms[fcfd::NREG] = {};                    // timings folded in from graph replays
     int64_t prof_gcnt[fcfd::NREG] = {};

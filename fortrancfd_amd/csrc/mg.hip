@@ -95,14 +95,29 @@ __device__ __forceinline__ T face(bool open, T r, T rn, T h) {
     return open ? c : (T)0;
 }
 
+// Coarse-grid correction folded into the first post-smoothing sweep: the
+// value of a fine cell is e + ec(parent) until that cell has been updated.
+template <typename T>
+struct Prol {
+    const T *ec;
+    Lay C;
+    int cx, cy, cz;
+    // parent of fine (i, j, k); i = -1 or nx maps to a coarse interior or
+    // ghost cell, always inside the coarse array (its coefficient is 0)
+    __device__ __forceinline__ T at(int i, int j, int k) const { return ec[C.id(i / cx, j / cy, k / cz)]; }
+};
+
 // Face coefficients of cell (i,j,k): sum_c = sum c_nb and
-// sum_ce = sum c_nb (e_nb - e_P).  FINE: from rho; otherwise from stored beta
-// arrays whose wall and ghost entries are zero.
-template <typename T, int DIM, bool FINE>
+// sum_ce = sum c_nb (e_nb - e_P), and the centre value ep.  FINE: from rho;
+// otherwise from stored beta arrays whose wall and ghost entries are zero.
+// PROL 0: plain values; 1: centre and neighbours get the coarse correction
+// (first colour after prolongation); 2: only the centre gets it (second
+// colour: its neighbours are first-colour cells, already corrected).
+template <typename T, int DIM, bool FINE, int PROL>
 __device__ __forceinline__ void stencil(const T *__restrict__ e, const T *__restrict__ rho,
                                         const T *__restrict__ bx, const T *__restrict__ by,
                                         const T *__restrict__ bz, const Lay &L, Inv2 ih, int i, int j, int k,
-                                        long id, T &sum_c, T &sum_ce) {
+                                        long id, const Prol<T> &P, T &sum_c, T &sum_ce, T &ep) {
     T cw, ce_, cs, cn, cb = 0, ct = 0;
     if (FINE) {
         T r = rho[id];
@@ -124,12 +139,19 @@ __device__ __forceinline__ void stencil(const T *__restrict__ e, const T *__rest
             cb = bz[id - L.sxy] * (T)ih.z;
         }
     }
-    const T ep = e[id];
+    auto nb = [&](long o, int ii, int jj, int kk) -> T {
+        T v = e[id + o];
+        if (PROL == 1) v += P.at(ii, jj, kk);
+        return v;
+    };
+    ep = e[id];
+    if (PROL != 0) ep += P.at(i, j, k);
     sum_c = ce_ + cw + cn + cs;
-    sum_ce = ce_ * (e[id + 1] - ep) + cw * (e[id - 1] - ep) + cn * (e[id + L.sx] - ep) + cs * (e[id - L.sx] - ep);
+    sum_ce = ce_ * (nb(1, i + 1, j, k) - ep) + cw * (nb(-1, i - 1, j, k) - ep) + cn * (nb(L.sx, i, j + 1, k) - ep) +
+             cs * (nb(-L.sx, i, j - 1, k) - ep);
     if (DIM == 3) {
         sum_c += ct + cb;
-        sum_ce += ct * (e[id + L.sxy] - ep) + cb * (e[id - L.sxy] - ep);
+        sum_ce += ct * (nb(L.sxy, i, j, k + 1) - ep) + cb * (nb(-L.sxy, i, j, k - 1) - ep);
     }
 }
 
@@ -162,23 +184,62 @@ __device__ __forceinline__ T apply_A(const T *__restrict__ e, const T *__restric
     return acc;
 }
 
+// Dot products of the flexible CG folded into the last half-sweep of a
+// V-cycle: r.z and r.zold over both cells of every thread's pair (z = the
+// cycle's output), block sums into partial[b] and partial[nb + b].
+template <typename T>
+struct Dots {
+    const double *r;
+    const T *zo;
+    double *partial;
+};
+
 // one colour of a red-black Gauss-Seidel sweep: e_P = (sum c e_nb - f) / sum c.
 // Thread x covers two cells of a row; it updates the one whose colour matches.
-template <typename T, int DIM, bool FINE>
+template <typename T, int DIM, bool FINE, int PROL, bool DOTS>
 __global__ void __launch_bounds__(256) k_smooth(T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ rho,
                                                 const T *__restrict__ bx, const T *__restrict__ by,
-                                                const T *__restrict__ bz, Lay L, Inv2 ih, int color) {
+                                                const T *__restrict__ bz, Lay L, Inv2 ih, int color, Prol<T> P,
+                                                Dots<T> D) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int j = by_ * blockDim.y + threadIdx.y, k = bz_;
-    const int i = 2 * (bx_ * blockDim.x + threadIdx.x) + ((j + k + color) & 1);
-    if (i >= L.nx || j >= L.ny) return;
-    const long id = L.id(i, j, k);
-    T sc, sce;
-    stencil<T, DIM, FINE>(e, rho, bx, by, bz, L, ih, i, j, k, id, sc, sce);
-    // correction form e_P += (sum c (e_nb - e_P) - f) / sum c: the Gauss-Seidel
-    // update without cancelling terms of size c |e|
-    if (sc > (T)0) e[id] = e[id] + gs_step<T>(sce - f[id], sc);
+    const int i0 = 2 * (bx_ * blockDim.x + threadIdx.x), par_ = (j + k + color) & 1;
+    const int i = i0 + par_;
+    const bool upd = i < L.nx && j < L.ny;
+    T enew = (T)0;
+    if (upd) {
+        const long id = L.id(i, j, k);
+        T sc, sce, ep;
+        stencil<T, DIM, FINE, PROL>(e, rho, bx, by, bz, L, ih, i, j, k, id, P, sc, sce, ep);
+        // correction form e_P += (sum c (e_nb - e_P) - f) / sum c: the Gauss-Seidel
+        // update without cancelling terms of size c |e|
+        enew = sc > (T)0 ? ep + gs_step<T>(sce - f[id], sc) : ep;
+        e[id] = enew;
+    }
+    if (DOTS) {
+        __shared__ double sh[4];
+        double rz = 0.0, rzo = 0.0;
+        if (upd) {
+            const long id = L.id(i, j, k);
+            rz = D.r[id] * (double)enew;
+            rzo = D.r[id] * (double)D.zo[id];
+        }
+        const int ip = i0 + 1 - par_;   // the pair's other cell: final from the previous half-sweep
+        if (ip < L.nx && j < L.ny) {
+            const long idp = L.id(ip, j, k);
+            rz += D.r[idp] * (double)e[idp];
+            rzo += D.r[idp] * (double)D.zo[idp];
+        }
+        rz = block_sum256(rz, sh);
+        rzo = block_sum256(rzo, sh);
+        if (threadIdx.x + threadIdx.y == 0) {
+            const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+            const long nb = (long)gridDim.x * gridDim.y * gridDim.z;
+            D.partial[b] = rz;
+            D.partial[nb + b] = rzo;
+        }
+    }
 }
 
 // residual of the level, restricted to the coarse right-hand side by the
@@ -471,7 +532,7 @@ __global__ void __launch_bounds__(1024) k_coarse_cg(T *__restrict__ eout, const 
 // M = B + sigma 1 1^T, which is positive definite; for a zero-mean rhs b the
 // solution of M x = b has zero mean and solves B x = b.  sigma = mean(diag B)/n
 // keeps the shift at the scale of the operator.
-constexpr int CDENSE = 64;
+constexpr int CDENSE = 128;
 
 __device__ __forceinline__ void cell_coords(const Lay &L, int c, int &i, int &j, int &k) {
     i = c % L.nx;
@@ -538,11 +599,13 @@ __global__ void __launch_bounds__(256) k_coarse_inv(const double *__restrict__ b
 }
 
 // e = M^-1 b with b = -(f - mean f), then the zero-mean projection of e;
-// one wave (n <= 64 = WAVE cells, one cell per lane)
+// one block of CDENSE threads, one cell per thread
 template <typename T>
-__global__ void __launch_bounds__(WAVE) k_coarse_dense(T *__restrict__ eout, const T *__restrict__ fin,
-                                                       const double *__restrict__ minv, Lay L) {
+__global__ void __launch_bounds__(CDENSE) k_coarse_dense(T *__restrict__ eout, const T *__restrict__ fin,
+                                                         const double *__restrict__ minv, Lay L) {
+    constexpr int NW = CDENSE / WAVE;
     __shared__ double b[CDENSE];
+    __shared__ double ws[NW];
     const int t = threadIdx.x, n = L.nx * L.ny * L.nz;
     int i = 0, j = 0, k = 0;
     double f = 0.0;
@@ -550,17 +613,23 @@ __global__ void __launch_bounds__(WAVE) k_coarse_dense(T *__restrict__ eout, con
         cell_coords(L, t, i, j, k);
         f = (double)fin[L.id(i, j, k)];
     }
-    auto wave_sum = [](double v) {
+    // fixed-order block sum: butterfly per wave (lane 0's order), then waves in order
+    auto block_sum = [&](double v) {
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-        return __shfl(v, 0, WAVE);   // lane 0's order for every lane
+        if ((t & (WAVE - 1)) == 0) ws[t / WAVE] = v;
+        __syncthreads();
+        double sum = 0.0;
+        for (int w = 0; w < NW; ++w) sum += ws[w];
+        __syncthreads();
+        return sum;
     };
-    const double fm = wave_sum(f) / (double)n;
+    const double fm = block_sum(f) / (double)n;
     if (t < n) b[t] = fm - f;
     __syncthreads();
     double x = 0.0;
     if (t < n)
         for (int c = 0; c < n; ++c) x += minv[c * n + t] * b[c];
-    const double xm = wave_sum(x) / (double)n;
+    const double xm = block_sum(x) / (double)n;
     if (t < n) eout[L.id(i, j, k)] = (T)(x - xm);
 }
 
@@ -599,7 +668,7 @@ int mg_free(fcfd_ctx *c) {
 }
 
 int mg_drop_graphs(fcfd_ctx *c) {
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 4; ++a)
         for (int b = 0; b < 5; ++b) {
             if (c->vgraph[a][b]) hipGraphExecDestroy(c->vgraph[a][b]);
             c->vgraph[a][b] = nullptr;
@@ -737,10 +806,12 @@ int mg_setup(fcfd_ctx *c) {
 // One V-cycle from level l with error e_l (level 0: the solution itself)
 // and rhs f_l of precision T.
 template <typename T, int DIM>
-static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1, int nu2) {
+static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1, int nu2,
+                  const Dots<T> *dots = nullptr) {
     const Geom &G = c->G;
     dim3 blk(64, 4);
     const int nl = (int)c->mg.size();
+    if (dots && nu2 < 1) return FCFD_ERR_ARG;   // the dots come from the last post-smoothing sweep
     auto arrays = [&](int lv, T *&e, const T *&f, const T *&bx, const T *&by, const T *&bz) {
         MgLevel &M = c->mg[lv];
         if (lv == 0) { e = e0; f = f0; bx = by = bz = nullptr; return; }
@@ -750,30 +821,46 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
             e = (T *)M.e32; f = (const T *)M.f32; bx = (const T *)M.bx32; by = (const T *)M.by32; bz = (const T *)M.bz32;
         }
     };
-    auto smooth = [&](int lv, int sweeps, bool reverse) -> int {
+    // sweeps of RB-GS; with prol the coarse correction is added by the first
+    // sweep, with dots the last half-sweep of level 0 forms the CG dots
+    auto smooth = [&](int lv, int sweeps, bool reverse, const Prol<T> *prol, const Dots<T> *dt) -> int {
         MgLevel &M = c->mg[lv];
         Lay L = lv == 0 ? lay_of(G) : lay_of(M, G.dim);
         Inv2 ih = inv2(M.hx, M.hy, M.hz);
         T *e; const T *f, *bx, *by, *bz;
         arrays(lv, e, f, bx, by, bz);
         dim3 g((unsigned)(((M.nx + 1) / 2 + blk.x - 1) / blk.x), (M.ny + blk.y - 1) / blk.y, M.nz);
+        const Prol<T> P0 = prol ? *prol : Prol<T>{nullptr, Lay{}, 1, 1, 1};
+        const Dots<T> D0 = dt ? *dt : Dots<T>{nullptr, nullptr, nullptr};
         for (int s = 0; s < sweeps; ++s)
             for (int h = 0; h < 2; ++h) {
                 ProfScope ps_(c, lv == 0 ? R_SMOOTH0 : R_SMOOTHC);
-                int color = reverse ? 1 - h : h;
-                if (lv == 0)
-                    hipLaunchKernelGGL((k_smooth<T, DIM, true>), g, blk, 0, c->stream, e, f, rho0, bx, by, bz, L, ih,
-                                       color);
-                else
-                    hipLaunchKernelGGL((k_smooth<T, DIM, false>), g, blk, 0, c->stream, e, f, (const T *)nullptr, bx,
-                                       by, bz, L, ih, color);
+                const int color = reverse ? 1 - h : h;
+                const int pm = (prol && s == 0) ? 1 + h : 0;
+                const bool dm = dt && lv == 0 && s == sweeps - 1 && h == 1;
+#define FCFD_SMOOTH(FINE, PM, DM)                                                                              \
+    hipLaunchKernelGGL((k_smooth<T, DIM, FINE, PM, DM>), g, blk, 0, c->stream, e, f, FINE ? rho0 : (const T *)nullptr, \
+                       bx, by, bz, L, ih, color, P0, D0)
+                if (lv == 0) {
+                    if (dm) {
+                        if (pm == 2) FCFD_SMOOTH(true, 2, true);
+                        else FCFD_SMOOTH(true, 0, true);
+                    } else if (pm == 1) FCFD_SMOOTH(true, 1, false);
+                    else if (pm == 2) FCFD_SMOOTH(true, 2, false);
+                    else FCFD_SMOOTH(true, 0, false);
+                } else {
+                    if (pm == 1) FCFD_SMOOTH(false, 1, false);
+                    else if (pm == 2) FCFD_SMOOTH(false, 2, false);
+                    else FCFD_SMOOTH(false, 0, false);
+                }
+#undef FCFD_SMOOTH
             }
         FCFD_CHECK(hipGetLastError());
         return FCFD_OK;
     };
     // down
     for (int lv = l; lv < nl - 1; ++lv) {
-        FCFD_TRY(smooth(lv, nu1, false));
+        FCFD_TRY(smooth(lv, nu1, false, nullptr, nullptr));
         MgLevel &M = c->mg[lv], &C = c->mg[lv + 1];
         Lay L = lv == 0 ? lay_of(G) : lay_of(M, G.dim), LC = lay_of(C, G.dim);
         Inv2 ih = inv2(M.hx, M.hy, M.hz);
@@ -801,7 +888,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         long n = (long)M.nx * M.ny * M.nz;
         ProfScope ps_(c, R_COARSE);
         if (nl > 1 && n <= CDENSE)
-            hipLaunchKernelGGL((k_coarse_dense<T>), dim3(1), dim3(WAVE), 0, c->stream, e, f,
+            hipLaunchKernelGGL((k_coarse_dense<T>), dim3(1), dim3(CDENSE), 0, c->stream, e, f,
                                (const double *)(c->mg_scalars + 64), L);
         else
             hipLaunchKernelGGL((k_coarse_cg<T, DIM>), dim3(1), dim3(n >= 1024 ? 1024 : 256), 0, c->stream, e, f, bx,
@@ -817,13 +904,16 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         arrays(lv, e, f, bx, by, bz);
         T *ec; const T *fc_, *a, *b, *cc;
         arrays(lv + 1, ec, fc_, a, b, cc);
-        {
+        if (nu2 > 0) {
+            // prolongation folded into the first post-smoothing sweep
+            const Prol<T> P{(const T *)ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1};
+            FCFD_TRY(smooth(lv, nu2, true, &P, lv == 0 ? dots : nullptr));
+        } else {
             ProfScope ps_(c, lv == 0 ? R_PROLONG0 : R_PROLONGC);
             hipLaunchKernelGGL((k_prolong<T>), grd3(M.nx, M.ny, M.nz, blk), blk, 0, c->stream, e, (const T *)ec, L,
                                LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
             FCFD_CHECK(hipGetLastError());
         }
-        FCFD_TRY(smooth(lv, nu2, true));
     }
     return FCFD_OK;
 }
@@ -860,30 +950,6 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
         long nb = (long)gridDim.x * gridDim.y * gridDim.z;
         partial[b] = rr;
         partial[nb + b] = ff;
-    }
-}
-
-// dots of the flexible CG: r.z and r.z_old (z of precision T)
-template <typename T>
-__global__ void __launch_bounds__(256) k_fcg_dots(const double *__restrict__ r, const T *__restrict__ z,
-                                                  const T *__restrict__ zo, Lay L, double *__restrict__ partial) {
-    __shared__ double sh[4];
-    int bx_, by_, bz_;
-    xcd_block(bx_, by_, bz_);
-    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
-    double a = 0.0, b = 0.0;
-    if (i < L.nx && j < L.ny) {
-        long id = L.id(i, j, k);
-        a = r[id] * (double)z[id];
-        b = r[id] * (double)zo[id];
-    }
-    a = block_sum256(a, sh);
-    b = block_sum256(b, sh);
-    if (threadIdx.x + threadIdx.y == 0) {
-        long bb = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-        long nb = (long)gridDim.x * gridDim.y * gridDim.z;
-        partial[bb] = a;
-        partial[nb + bb] = b;
     }
 }
 
@@ -980,16 +1046,24 @@ static int field_mean(fcfd_ctx *c, const double *a) {
 
 // Run `inner` V-cycles of one kind from a captured graph (captured on first use;
 // the level arrays, rho and the level-0 vectors are fixed for the context):
-// kind 0: FP64 on (P, f64); kind 1: FP64 on (kz, kr); kind 2: FP32 on (e32, f32).
+// kind 0: FP64 on (P, f64); kind 1: FP64 on (kz, kr) with the CG dots r.z,
+// r.zold; kind 2: FP32 on (e32, f32); kind 3: as 2 with the CG dots.  The
+// dots are left as block partials of the level-0 smoother grid.
 template <int DIM>
 static int run_vcycles(fcfd_ctx *c, int kind, int inner) {
     inner = std::max(1, std::min(inner, 4));
     MgLevel &M0 = c->mg[0];
+    const Dots<double> d64{c->kr, c->kzo, c->red.partial};
+    const Dots<float> d32{c->kr, c->kzo32, c->red.partial};
     auto body = [&]() -> int {
         for (int t = 0; t < inner; ++t) {
+            const bool last = t == inner - 1;
             if (kind == 0) FCFD_TRY((vcycle<double, DIM>(c, 0, c->f[FCFD_P], M0.f64, c->f[FCFD_RHO], 2, 2)));
-            else if (kind == 1) FCFD_TRY((vcycle<double, DIM>(c, 0, c->kz, c->kr, c->f[FCFD_RHO], 2, 2)));
-            else FCFD_TRY((vcycle<float, DIM>(c, 0, M0.e32, M0.f32, c->rho32, 2, 2)));
+            else if (kind == 1)
+                FCFD_TRY((vcycle<double, DIM>(c, 0, c->kz, c->kr, c->f[FCFD_RHO], 2, 2, last ? &d64 : nullptr)));
+            else
+                FCFD_TRY((vcycle<float, DIM>(c, 0, M0.e32, M0.f32, c->rho32, 2, 2,
+                                             (kind == 3 && last) ? &d32 : nullptr)));
         }
         return FCFD_OK;
     };
@@ -1028,7 +1102,9 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     const bool mixed = precision == FCFD_PPE_MIXED;
     const bool krylov = c->pr.ppe_krylov != 0;
     const int inner = std::max(1, std::min(c->pr.ppe_inner > 0 ? c->pr.ppe_inner : 1, 4));
-    const int kind = mixed ? 2 : (krylov ? 1 : 0);
+    const int kind = mixed ? (krylov ? 3 : 2) : (krylov ? 1 : 0);
+    // blocks of the level-0 smoother grid (the CG dots are its block partials)
+    const long nb_s = (long)(((G.nx + 1) / 2 + blk.x - 1) / blk.x) * ((G.ny + blk.y - 1) / blk.y) * G.nz;
     const int maxc = c->pr.ppe_max_cycles > 0 ? c->pr.ppe_max_cycles : 200;
     static const bool verbose = getenv("FCFD_MG_VERBOSE") != nullptr;
     // compatibility of the Neumann problem: f = src - mean(src)
@@ -1070,13 +1146,8 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
         } else {
             // flexible CG preconditioned by the V-cycle(s)
             FCFD_TRY(run_vcycles<DIM>(c, kind, inner));
-            if (mixed)
-                hipLaunchKernelGGL(k_fcg_dots<float>, g, blk, 0, c->stream, c->kr, M0.e32, c->kzo32, L0, c->red.partial);
-            else
-                hipLaunchKernelGGL(k_fcg_dots<double>, g, blk, 0, c->stream, c->kr, c->kz, c->kzo, L0, c->red.partial);
-            FCFD_CHECK(hipGetLastError());
             double d[2];
-            FCFD_TRY(sums_to_host(c, nb, 2, d));
+            FCFD_TRY(sums_to_host(c, nb_s, 2, d));
             FCFD_TRY(prof_graph_accum(c, kind, inner));
             const double beta = first ? 0.0 : (d[0] - d[1]) / rz;
             rz = d[0];
