@@ -60,11 +60,34 @@ def region_bytes(case, mixed: bool) -> dict:
         "projection": 8 * N * 8,
         "node_vel": 8 * N * 6,
         "mpls_update": 8 * N * 3,
-        "mg_smooth_l0": b * N * (1 + 0.5 + 1 + 0.5),     # R{e, f/2, rho} W{e/2} per colour
+        # per colour: R{e, f/2, rho} W{e/2}; averaged over the 8 launches of a
+        # V(2,2) cycle: + R{ec} (N/nchild0 coarse values) in the first post
+        # sweep (prolongation folded in) and + R{r (FP64), z_old} in the last
+        # half-sweep (CG dots folded in)
+        "mg_smooth_l0": N * (8 * b * 3 + 2 * b / nchild0 + (8 + b)) / 8,
         "mg_restrict_l0": b * N * (3 + 2.0 / nchild0),   # R{e,f,rho} W{fc,ec}
-        "mg_prolong_l0": b * N * (2 + 1.0 / nchild0),    # R{e,ec} W{e}
         "mg_resid_l0": N * (3 * 8 + (2 * 4 if mixed else 0)),  # R{p,f,rho} (W{r32,e32})
     }
+
+
+def profiled_traffic(region: str):
+    """HBM bytes per launch of a kernel region as measured by the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/*.txt, line
+    `traffic <region> ... bytes_per_launch=B`, counters scaled by a
+    device-to-device copy of known size); None when no profile holds it."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None, None
+    for fn in sorted(os.listdir(pdir)):
+        if not fn.endswith(".txt"):
+            continue
+        with open(os.path.join(pdir, fn)) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) > 2 and parts[0] == "traffic" and parts[1] == region:
+                    kv = dict(p.split("=", 1) for p in parts[2:] if "=" in p)
+                    return float(kv["bytes_per_launch"]), fn
+    return None, None
 
 
 def cpu_baseline(steps: int = 5, n: int = 96, threads: int = 0) -> dict:
@@ -231,10 +254,14 @@ def main() -> int:
     dom = max(regions, key=lambda k: regions[k]["ms_total"])
     dom_r = regions[dom]
     achieved = dom_r.get("GB/s")
+    traffic, tsrc = profiled_traffic(dom) if case.nx == 256 else (None, None)
     roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                "traffic": round(traffic) if traffic else None,
                 "bytes_per_launch": rb.get(dom), "ms_per_launch": dom_r["ms_per_launch"],
                 "share_of_step": round(dom_r["ms_total"] / ((tp1 - tp0) * 1e3), 4)}
+    if tsrc:
+        roofline["traffic_source"] = "profiles/" + tsrc
     out = {
         "metric": "Mcell-steps/s @256^3 two-phase, %HBM peak; MG ms to 1e-10 mixed vs FP64",
         "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,

@@ -105,9 +105,11 @@ int fcfd_ns_momentum_correction(fcfd_ctx *ctx);
 /* ---- momentum predictor glue (`.../ns_solver/`) ---- */
 /* ns_linearize + ns_ab_setup + ns_ab_predictor (ns_solvers.f90:33-57,
  * admas_bashforth_solver.f90:230-311, ns_ab_adv.f90:1-73, ns_ab_diff.f90:1-198):
- * the viscous/body-force part uses the start-of-step fields (PHI_OLD,
- * U_OLD.., and the curvature/normals held in CURV/NX/NY/NZ of the previous
- * ns_init); SRC_* gets adv + diff, U/V/W the AB2 predictor. */
+ * the viscous / body-force / surface-tension part (sec_part1..3) depends only
+ * on start-of-step fields, so it is evaluated once per step by fcfd_switch
+ * (and fcfd_problem_init) from PHI_OLD, U_OLD.. and the curvature/normals of
+ * the previous ns_init; this call adds the upwind advection of the current
+ * iterate: SRC_* gets adv + diff, U/V/W the AB2 predictor. */
 int fcfd_ns_ab_predictor(fcfd_ctx *ctx);
 /* node_vel: MAC -> cell velocity + nvelbc (ns_solvers.f90:128-147). */
 int fcfd_node_vel(fcfd_ctx *ctx);
@@ -118,7 +120,9 @@ int fcfd_ns_check_convergence_div(fcfd_ctx *ctx);
 int fcfd_ns_solver(fcfd_ctx *ctx);
 
 /* ---- whole step ---- */
-/* manager_switch (tree.f90:229-252): old <- now for the time-levelled fields. */
+/* manager_switch (tree.f90:229-252): old <- now for the time-levelled fields,
+ * then the step's viscous / body-force / surface-tension source of the
+ * momentum predictor (ns_ab_diff.f90:1-198, see fcfd_ns_ab_predictor). */
 int fcfd_switch(fcfd_ctx *ctx);
 /* mpls(): advection, reinitialisation every redis_interval steps, mass
  * correction (interface_solver.f90:25-34). */

@@ -37,6 +37,6 @@ for d in 2D 3D; do
         fi
         objs+=("$b/$f.o")
     done
-    "$FC" $FFLAGS -cpp $def -I"$b/mod" -I"$src/src" "$here/ref_harness.F90" "${objs[@]}" -o "$out/$exe"
+    "$FC" $FFLAGS -cpp $def -module-dir "$b/mod" -I"$b/mod" -I"$src/src" "$here/ref_harness.F90" "${objs[@]}" -o "$out/$exe"
     echo "build_ref: built $out/$exe"
 done
