@@ -52,7 +52,7 @@ def region_bytes(case, mixed: bool) -> dict:
         "ls_adv_stage": 8 * N * 20 / 3,        # R{phi,nu,nv,nw,l1..} W{phi',l} averaged over 3 stages
         "redis_pass_a": 8 * N * 5,             # R{phi,S0} W{grad,s2,s3}
         "redis_pass_b": 8 * N * 8,             # R{phi,S0,grad,s2,s3,l1,l2,phi0} W{phi,l} averaged over stages
-        "ccd_lines": 8 * N * 3 / 3,            # per direction launch: R{f} W{fx,fxx} on its lines
+        "ccd_lines": 8 * N * 3,                # one direction per launch: R{f} W{fx,fxx}
         "curv": 8 * N * 10,
         "diff_source": 8 * N * 11,
         "predictor": 8 * N * 18,
@@ -67,6 +67,9 @@ def region_bytes(case, mixed: bool) -> dict:
         "mg_smooth_l0": N * (8 * b * 3 + 2 * b / nchild0 + (8 + b)) / 8,
         "mg_restrict_l0": b * N * (3 + 2.0 / nchild0),   # R{e,f,rho} W{fc,ec}
         "mg_resid_l0": N * (3 * 8 + (2 * 4 if mixed else 0)),  # R{p,f,rho} (W{r32,e32})
+        # flexible CG, averaged over its two launches per iteration:
+        # dir R{z, p, q, rho} W{p, q, z_old}; update R{x, p, q, r} W{x, r, r32, e32}
+        "fcg_vectors": N * ((8 * 5 + 2 * b) + (8 * 6 + (2 * 4 if mixed else 8))) / 2,
     }
 
 

@@ -59,7 +59,7 @@ struct MgLevel {
 enum Region {
     R_LS_ADV, R_REDIS_A, R_REDIS_B, R_MASS, R_MPLS_UPD, R_CCD, R_CURV, R_DIFF, R_PRED, R_PPE_SRC,
     R_PROJ, R_NODE_VEL, R_BC, R_SMOOTH0, R_SMOOTHC, R_RESTRICT0, R_RESTRICTC, R_PROLONG0, R_PROLONGC,
-    R_RESID0, R_COARSE, R_MISC, NREG
+    R_RESID0, R_COARSE, R_FCG, R_MISC, NREG
 };
 
 struct ProfRec {

@@ -1153,7 +1153,7 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
             rz = d[0];
             first = false;
             {
-                ProfScope ps_(c, R_MISC);
+                ProfScope ps_(c, R_FCG);
                 if (mixed)
                     hipLaunchKernelGGL((k_fcg_dir<float, DIM>), g, blk, 0, c->stream, M0.e32, c->kzo32, c->kp, c->kq,
                                        rho, L0, ih, beta, c->red.partial);
@@ -1172,7 +1172,7 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
             }
             const double alpha = rz / pq;
             {
-                ProfScope ps_(c, R_MISC);
+                ProfScope ps_(c, R_FCG);
                 hipLaunchKernelGGL(k_fcg_update, g, blk, 0, c->stream, p, c->kp, c->kq, c->kr, L0, alpha,
                                    mixed ? M0.f32 : nullptr, M0.e32, mixed ? nullptr : c->kz, c->red.partial);
                 FCFD_CHECK(hipGetLastError());

@@ -40,7 +40,7 @@ static const char *kRegionNames[NREG] = {
     "ls_adv_stage", "redis_pass_a", "redis_pass_b", "mass_sums", "mpls_update", "ccd_lines", "curv",
     "diff_source", "predictor", "ppe_src", "projection", "node_vel", "bc_ghost", "mg_smooth_l0",
     "mg_smooth_coarse", "mg_restrict_l0", "mg_restrict_coarse", "mg_prolong_l0", "mg_prolong_coarse",
-    "mg_resid_l0", "mg_coarse_cg", "misc"};
+    "mg_resid_l0", "mg_coarse_cg", "fcg_vectors", "misc"};
 
 extern "C" int fcfd_prof_nregions(void) { return NREG; }
 
