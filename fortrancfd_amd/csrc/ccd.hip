@@ -25,17 +25,34 @@ using namespace fcfd;
 
 namespace fcfd {
 
-// coefficient rows stored per direction: [r][N]
-enum { CA1, CB1, CAA1, CBB1, CA2, CB2, CAA2, CBB2, CA3, CB3, CAA3, CBB3, CDEN, NCOEF };
+// Per-point coefficients of the line solve, packed for the two sweeps:
+// forward [A1 B1 AA1 BB1], backward [A3 B3 AA3 BB3 A2/D AA2/D B2/D BB2/D]
+// with D the determinant of the diagonal block (the reference divides by D;
+// the reciprocal products are folded in here once).
+enum { PF_A1, PF_B1, PF_AA1, PF_BB1, PB_A3, PB_B3, PB_AA3, PB_BB3, PB_A2D, PB_AA2D, PB_B2D, PB_BB2D, NPACK };
 
-// TWIN_DEC of init_ccd + bc on a line of N points, spacing dx (host)
-static void ccd_factor(int N, double dx, std::vector<double> &out) {
+// line matrices of the CCD family (working_matrix rows 1..3 = sub, diag, super)
+enum { MAT_CCD = 0, MAT_UCCD_UP = 1, MAT_UCCD_DOWN = 2, NMAT = 3 };
+
+// Matrix of one kind on a line of N points, spacing dx: interior rows of
+// init_ccd / init_uccd, the closure rows of working_matrix_bc, then TWIN_DEC
+// (`.../lib/solver/ccd_solvers.f90:76-210`, `matrix_solver.f90:6-48`), packed.
+static void ccd_factor(int kind, int N, double dx, std::vector<double> &pk) {
     std::vector<double> A(3 * N), B(3 * N), AA(3 * N), BB(3 * N);
     auto M = [&](std::vector<double> &X, int r, int i) -> double & { return X[i * 3 + r - 1]; };
     const int is = 0, ie = N - 1;
+    const double A1 = 0.875, B1 = 0.1251282341599089, B2 = -0.2487176584009104, B3 = 0.0001282341599089;
     for (int i = is + 1; i <= ie - 1; ++i) {
-        M(A, 1, i) = 7.0 / 16.0; M(A, 2, i) = 1.0; M(A, 3, i) = 7.0 / 16.0;
-        M(B, 1, i) = dx / 16.0; M(B, 2, i) = 0.0; M(B, 3, i) = -dx / 16.0;
+        if (kind == MAT_CCD) {
+            M(A, 1, i) = 7.0 / 16.0; M(A, 2, i) = 1.0; M(A, 3, i) = 7.0 / 16.0;
+            M(B, 1, i) = dx / 16.0; M(B, 2, i) = 0.0; M(B, 3, i) = -dx / 16.0;
+        } else if (kind == MAT_UCCD_UP) {
+            M(A, 1, i) = A1; M(A, 2, i) = 1.0; M(A, 3, i) = 0.0;
+            M(B, 1, i) = B1 * dx; M(B, 2, i) = B2 * dx; M(B, 3, i) = B3 * dx;
+        } else {
+            M(A, 1, i) = 0.0; M(A, 2, i) = 1.0; M(A, 3, i) = A1;
+            M(B, 1, i) = -B3 * dx; M(B, 2, i) = -B2 * dx; M(B, 3, i) = -B1 * dx;
+        }
         M(AA, 1, i) = -9.0 / 8.0 / dx; M(AA, 2, i) = 0.0; M(AA, 3, i) = 9.0 / 8.0 / dx;
         M(BB, 1, i) = -1.0 / 8.0; M(BB, 2, i) = 1.0; M(BB, 3, i) = -1.0 / 8.0;
     }
@@ -60,47 +77,31 @@ static void ccd_factor(int N, double dx, std::vector<double> &out) {
         M(BB, 1, i) = SC2;
         M(BB, 2, i) = M(BB, 2, i) - (SC1 * M(B, 3, i - 1) + SC2 * M(BB, 3, i - 1));
     }
-    out.assign((size_t)NCOEF * N, 0.0);
+    pk.assign((size_t)NPACK * N, 0.0);
     for (int i = 0; i < N; ++i) {
-        out[CA1 * N + i] = M(A, 1, i); out[CB1 * N + i] = M(B, 1, i);
-        out[CAA1 * N + i] = M(AA, 1, i); out[CBB1 * N + i] = M(BB, 1, i);
-        out[CA2 * N + i] = M(A, 2, i); out[CB2 * N + i] = M(B, 2, i);
-        out[CAA2 * N + i] = M(AA, 2, i); out[CBB2 * N + i] = M(BB, 2, i);
-        out[CA3 * N + i] = M(A, 3, i); out[CB3 * N + i] = M(B, 3, i);
-        out[CAA3 * N + i] = M(AA, 3, i); out[CBB3 * N + i] = M(BB, 3, i);
-        out[CDEN * N + i] = M(A, 2, i) * M(BB, 2, i) - M(AA, 2, i) * M(B, 2, i);
+        double *q = &pk[(size_t)i * NPACK];
+        const double D = M(A, 2, i) * M(BB, 2, i) - M(AA, 2, i) * M(B, 2, i);
+        q[PF_A1] = M(A, 1, i); q[PF_B1] = M(B, 1, i); q[PF_AA1] = M(AA, 1, i); q[PF_BB1] = M(BB, 1, i);
+        q[PB_A3] = M(A, 3, i); q[PB_B3] = M(B, 3, i); q[PB_AA3] = M(AA, 3, i); q[PB_BB3] = M(BB, 3, i);
+        q[PB_A2D] = M(A, 2, i) / D; q[PB_AA2D] = M(AA, 2, i) / D;
+        q[PB_B2D] = M(B, 2, i) / D; q[PB_BB2D] = M(BB, 2, i) / D;
     }
 }
-
-// Per-point coefficients of the line solve, packed for the two sweeps:
-// forward [A1 B1 AA1 BB1], backward [A3 B3 AA3 BB3 A2/D AA2/D B2/D BB2/D]
-// with D the determinant of the diagonal block (the reference divides by D;
-// the reciprocal products are folded in here once).
-enum { PF_A1, PF_B1, PF_AA1, PF_BB1, PB_A3, PB_B3, PB_AA3, PB_BB3, PB_A2D, PB_AA2D, PB_B2D, PB_BB2D, NPACK };
 
 int ccd_setup(fcfd_ctx *c) {
     const Geom &G = c->G;
     const int n[3] = {G.nx, G.ny, G.nz};
     const double h[3] = {G.dx, G.dy, G.dz};
-    for (int d = 0; d < 3; ++d) {
-        c->ccd_coef[d] = nullptr;
-        if (d >= G.dim) continue;
-        int N = n[d] + 2 * G.g;
-        std::vector<double> co, pk((size_t)NPACK * N);
-        ccd_factor(N, h[d], co);
-        for (int i = 0; i < N; ++i) {
-            double *q = &pk[(size_t)i * NPACK];
-            const double D = co[CDEN * N + i];
-            q[PF_A1] = co[CA1 * N + i]; q[PF_B1] = co[CB1 * N + i];
-            q[PF_AA1] = co[CAA1 * N + i]; q[PF_BB1] = co[CBB1 * N + i];
-            q[PB_A3] = co[CA3 * N + i]; q[PB_B3] = co[CB3 * N + i];
-            q[PB_AA3] = co[CAA3 * N + i]; q[PB_BB3] = co[CBB3 * N + i];
-            q[PB_A2D] = co[CA2 * N + i] / D; q[PB_AA2D] = co[CAA2 * N + i] / D;
-            q[PB_B2D] = co[CB2 * N + i] / D; q[PB_BB2D] = co[CBB2 * N + i] / D;
+    for (int kind = 0; kind < NMAT; ++kind)
+        for (int d = 0; d < 3; ++d) {
+            double *&dst = c->ccd_coef[kind * 3 + d];
+            dst = nullptr;
+            if (d >= G.dim) continue;
+            std::vector<double> pk;
+            ccd_factor(kind, n[d] + 2 * G.g, h[d], pk);
+            FCFD_CHECK(hipMalloc((void **)&dst, sizeof(double) * pk.size()));
+            FCFD_CHECK(hipMemcpy(dst, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
         }
-        FCFD_CHECK(hipMalloc((void **)&c->ccd_coef[d], sizeof(double) * pk.size()));
-        FCFD_CHECK(hipMemcpy(c->ccd_coef[d], pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
-    }
     return FCFD_OK;
 }
 
@@ -244,15 +245,165 @@ int ccd_dir(fcfd_ctx *c, int dir, const double *f, double *fx, double *fxx) {
     constexpr int CH = 8;
     dim3 grd((unsigned)((nl + CCD_THREADS - 1) / CCD_THREADS));
     ProfScope ps_(c, R_CCD);
+    const double *co = c->ccd_coef[MAT_CCD * 3 + dir];
     if (dir == 0)
-        hipLaunchKernelGGL((k_ccd_lines<0, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, c->ccd_coef[0],
-                           G);
+        hipLaunchKernelGGL((k_ccd_lines<0, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, co, G);
     else if (dir == 1)
-        hipLaunchKernelGGL((k_ccd_lines<1, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, c->ccd_coef[1],
-                           G);
+        hipLaunchKernelGGL((k_ccd_lines<1, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, co, G);
     else
-        hipLaunchKernelGGL((k_ccd_lines<2, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, c->ccd_coef[2],
-                           G);
+        hipLaunchKernelGGL((k_ccd_lines<2, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, co, G);
+    FCFD_CHECK(hipGetLastError());
+    return FCFD_OK;
+}
+
+// root_solve("uccd") along one line (`.../lib/solver/ccd_solvers.f90:
+// 308-341,400-412`): the up- and down-wind systems share the second-
+// derivative source and differ in the first-derivative source and matrix;
+// both are swept together, and the result keeps fx = SU where u > 0, else SD,
+// fxx = (SSU + SSD)/2 (fxx optional).  Forward results park in fx (SU) and
+// three scratch arrays (SSU, SD, SSD).  One thread per line, CH-point chunks
+// with the next chunk's loads issued before the current chunk's recurrence.
+template <int DIR, int CH>
+__global__ void __launch_bounds__(CCD_THREADS) k_uccd_lines(const double *__restrict__ f, const double *__restrict__ u,
+                                                            double *__restrict__ fx, double *__restrict__ fxx,
+                                                            double *__restrict__ t1, double *__restrict__ t2,
+                                                            double *__restrict__ t3, const double *__restrict__ cog_u,
+                                                            const double *__restrict__ cog_d, Geom G, int use_lds) {
+    extern __shared__ double lds[];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long nlines, st;
+    int N;
+    double dx;
+    if (DIR == 0) { nlines = (long)G.ny * G.nz; N = G.nx + 2 * G.g; st = 1; dx = G.dx; }
+    else if (DIR == 1) { nlines = (long)G.nx * G.nz; N = G.ny + 2 * G.g; st = G.sx; dx = G.dy; }
+    else { nlines = (long)G.nx * G.ny; N = G.nz + 2 * G.g; st = G.sxy; dx = G.dz; }
+    const double *cu = cog_u, *cd = cog_d;
+    if (use_lds) {
+        for (int e = threadIdx.x; e < N * NPACK; e += CCD_THREADS) {
+            lds[e] = cog_u[e];
+            lds[N * NPACK + e] = cog_d[e];
+        }
+        __syncthreads();
+        cu = lds;
+        cd = lds + N * NPACK;
+    }
+    if (t >= nlines) return;
+    long base;
+    if (DIR == 0) base = G.id(-G.g, (int)(t % G.ny), (int)(t / G.ny));
+    else if (DIR == 1) base = G.id((int)(t % G.nx), -G.g, (int)(t / G.nx));
+    else base = G.id((int)(t % G.nx), (int)(t / G.nx), -G.g);
+    const double *F = f + base, *U = u + base;
+    double *X = fx + base, *XX = fxx ? fxx + base : nullptr, *T1 = t1 + base, *T2 = t2 + base, *T3 = t3 + base;
+    const double idx = 1.0 / dx, idx2 = idx * idx;
+    const double c3 = -0.06096119008109, c2 = 1.99692238016218, c1 = -1.93596119008109;
+    // boundary rows (shared by both systems)
+    const double f0 = F[0], f1 = F[st], f2 = F[2 * st], f3 = F[3 * st];
+    double Su = (-3.5 * f0 + 4.0 * f1 - 0.5 * f2) * idx;
+    double SSu = (34.0 / 3.0 * f0 - 83.0 / 4.0 * f1 + 10.0 * f2 - 7.0 / 12.0 * f3) * idx2;
+    double Sd = Su, SSd = SSu;
+    X[0] = Su; T1[0] = SSu; T2[0] = Sd; T3[0] = SSd;
+    double fm = f0, fc = f1;
+    double fp[CH], fq[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q)
+        if (1 + q < N - 1) fp[q] = F[(long)(2 + q) * st];
+    for (int i0 = 1; i0 < N - 1; i0 += CH) {
+        const int n0 = i0 + CH;
+#pragma unroll
+        for (int q = 0; q < CH; ++q)
+            if (n0 + q < N - 1) fq[q] = F[(long)(n0 + q + 1) * st];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int i = i0 + q;
+            if (i < N - 1) {
+                const double *a = cu + i * NPACK, *b = cd + i * NPACK;
+                const double ss = (3.0 * fm - 6.0 * fc + 3.0 * fp[q]) * idx2;
+                const double su = (c1 * fm + c2 * fc + c3 * fp[q]) * idx;
+                const double sd = -(c3 * fm + c2 * fc + c1 * fp[q]) * idx;
+                const double Sun = su - (a[PF_A1] * Su + a[PF_B1] * SSu);
+                const double SSun = ss - (a[PF_AA1] * Su + a[PF_BB1] * SSu);
+                const double Sdn = sd - (b[PF_A1] * Sd + b[PF_B1] * SSd);
+                const double SSdn = ss - (b[PF_AA1] * Sd + b[PF_BB1] * SSd);
+                Su = Sun; SSu = SSun; Sd = Sdn; SSd = SSdn;
+                const long o = (long)i * st;
+                X[o] = Su; T1[o] = SSu; T2[o] = Sd; T3[o] = SSd;
+                fm = fc; fc = fp[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) fp[q] = fq[q];
+    }
+    const int ie = N - 1;
+    {
+        const double e0 = F[(long)ie * st], e1 = F[(long)(ie - 1) * st], e2 = F[(long)(ie - 2) * st],
+                     e3 = F[(long)(ie - 3) * st];
+        const double s = -(-3.5 * e0 + 4.0 * e1 - 0.5 * e2) * idx;
+        const double ss = (34.0 / 3.0 * e0 - 83.0 / 4.0 * e1 + 10.0 * e2 - 7.0 / 12.0 * e3) * idx2;
+        const double *a = cu + ie * NPACK, *b = cd + ie * NPACK;
+        const double Sun = s - (a[PF_A1] * Su + a[PF_B1] * SSu);
+        const double SSun = ss - (a[PF_AA1] * Su + a[PF_BB1] * SSu);
+        const double Sdn = s - (b[PF_A1] * Sd + b[PF_B1] * SSd);
+        const double SSdn = ss - (b[PF_AA1] * Sd + b[PF_BB1] * SSd);
+        Su = a[PB_BB2D] * Sun - a[PB_B2D] * SSun;
+        SSu = a[PB_A2D] * SSun - a[PB_AA2D] * Sun;
+        Sd = b[PB_BB2D] * Sdn - b[PB_B2D] * SSdn;
+        SSd = b[PB_A2D] * SSdn - b[PB_AA2D] * Sdn;
+        const long o = (long)ie * st;
+        X[o] = U[o] > 0.0 ? Su : Sd;
+        if (XX) XX[o] = 0.5 * (SSu + SSd);
+    }
+    // back substitution of both systems
+    for (int i1 = ie - 1; i1 >= 0; i1 -= CH) {
+        double xu[CH], xxu[CH], xd[CH], xxd[CH], uv[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q)
+            if (i1 - q >= 0) {
+                const long o = (long)(i1 - q) * st;
+                xu[q] = X[o]; xxu[q] = T1[o]; xd[q] = T2[o]; xxd[q] = T3[o]; uv[q] = U[o];
+            }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int i = i1 - q;
+            if (i >= 0) {
+                const double *a = cu + i * NPACK, *b = cd + i * NPACK;
+                const double Siu = xu[q] - (a[PB_A3] * Su + a[PB_B3] * SSu);
+                const double SSiu = xxu[q] - (a[PB_AA3] * Su + a[PB_BB3] * SSu);
+                const double Sid = xd[q] - (b[PB_A3] * Sd + b[PB_B3] * SSd);
+                const double SSid = xxd[q] - (b[PB_AA3] * Sd + b[PB_BB3] * SSd);
+                Su = a[PB_BB2D] * Siu - a[PB_B2D] * SSiu;
+                SSu = a[PB_A2D] * SSiu - a[PB_AA2D] * Siu;
+                Sd = b[PB_BB2D] * Sid - b[PB_B2D] * SSid;
+                SSd = b[PB_A2D] * SSid - b[PB_AA2D] * Sid;
+                const long o = (long)i * st;
+                X[o] = uv[q] > 0.0 ? Su : Sd;
+                if (XX) XX[o] = 0.5 * (SSu + SSd);
+            }
+        }
+    }
+}
+
+// UCCD of f along direction dir on every interior line (ghost ends
+// included), upwind by u; scratch: the context's cx, cy, cz
+int uccd_dir(fcfd_ctx *c, int dir, const double *f, const double *u, double *fx, double *fxx) {
+    const Geom &G = c->G;
+    const long nl = dir == 0 ? (long)G.ny * G.nz : (dir == 1 ? (long)G.nx * G.nz : (long)G.nx * G.ny);
+    const int N = (dir == 0 ? G.nx : (dir == 1 ? G.ny : G.nz)) + 2 * G.g;
+    const size_t need = 2 * sizeof(double) * NPACK * N;
+    const int use_lds = (long)need <= CCD_LDS_MAX;
+    const size_t lds = use_lds ? need : 0;
+    constexpr int CH = 8;
+    dim3 grd((unsigned)((nl + CCD_THREADS - 1) / CCD_THREADS));
+    const double *cu = c->ccd_coef[MAT_UCCD_UP * 3 + dir], *cd = c->ccd_coef[MAT_UCCD_DOWN * 3 + dir];
+    ProfScope ps_(c, R_CCD);
+    if (dir == 0)
+        hipLaunchKernelGGL((k_uccd_lines<0, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, u, fx, fxx, c->cx, c->cy,
+                           c->cz, cu, cd, G, use_lds);
+    else if (dir == 1)
+        hipLaunchKernelGGL((k_uccd_lines<1, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, u, fx, fxx, c->cx, c->cy,
+                           c->cz, cu, cd, G, use_lds);
+    else
+        hipLaunchKernelGGL((k_uccd_lines<2, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, u, fx, fxx, c->cx, c->cy,
+                           c->cz, cu, cd, G, use_lds);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }

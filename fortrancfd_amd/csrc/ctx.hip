@@ -241,8 +241,10 @@ extern "C" int fcfd_destroy(fcfd_ctx *c) {
         if (c->f[f]) hipFree(c->f[f]);
     double *scr[] = {c->phiB, c->phiS, c->l1, c->l2, c->s2, c->s3, c->s0, c->cx, c->cy,
                      c->cz, c->dsu, c->dsv, c->dsw, c->un, c->vn, c->wn,
-                     c->ccd_coef[0], c->ccd_coef[1], c->ccd_coef[2], c->red.partial, c->red.result};
+                     c->red.partial, c->red.result};
     for (auto p : scr)
+        if (p) hipFree(p);
+    for (auto p : c->ccd_coef)
         if (p) hipFree(p);
     if (c->red.host) hipHostFree(c->red.host);
     if (c->flag) hipFree(c->flag);

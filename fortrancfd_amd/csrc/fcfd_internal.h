@@ -99,7 +99,7 @@ struct fcfd_ctx {
     double *cx, *cy, *cz;      // CCD second-derivative scratch / cross-derivative temp
     double *dsu, *dsv, *dsw;   // viscous + body force source (sec_part1..3), once per step
     double *un, *vn, *wn;      // predictor output buffer (ping-pong with u, v, w)
-    double *ccd_coef[3];       // factorised CCD line matrices per direction
+    double *ccd_coef[9];       // factorised line matrices [kind * 3 + direction]: CCD, UCCD up, UCCD down
     // ---- reductions ----
     fcfd::Reduce red;
     unsigned long long *flag;  // device flags (reinit error max bits, done, count)
@@ -198,6 +198,7 @@ int velbc(fcfd_ctx *c, double *u, double *v, double *w, int nodal);
 int ls_props(fcfd_ctx *c, const double *phi, double *H, double *D, double *S, double *rho, double *mu);
 int reduce_sums(fcfd_ctx *c, int kind, const double *a, const double *b, const double *cc, const double *d);
 int ccd_dir(fcfd_ctx *c, int dir, const double *f, double *fx, double *fxx);
+int uccd_dir(fcfd_ctx *c, int dir, const double *f, const double *u, double *fx, double *fxx);
 int ccd_setup(fcfd_ctx *c);
 int surface_norms(fcfd_ctx *c, const double *phi, int full);
 int curv_kernel(fcfd_ctx *c);

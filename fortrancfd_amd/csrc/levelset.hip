@@ -325,10 +325,64 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage(const double *__restrict__
     phin[id] = phi[id] + src * dt;
 }
 
+// One SSP-RK3 stage with the UCCD source L = -(u phi_x + v phi_y + w phi_z)
+// (the reference's commented alternative source,
+// `.../interface_solver/level_set_rk3_solver.f90:155-157`), from the upwind
+// CCD derivatives fx, fy, fz of the stage input.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_ls_uccd_stage(const double *__restrict__ phi, double *__restrict__ phin,
+                                                       const double *__restrict__ nu, const double *__restrict__ nv,
+                                                       const double *__restrict__ nw, const double *__restrict__ fx,
+                                                       const double *__restrict__ fy, const double *__restrict__ fz,
+                                                       double *__restrict__ l1, double *__restrict__ l2, Geom G,
+                                                       double dt, int stage) {
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
+    if (i >= G.nx || j >= G.ny) return;
+    const long id = G.id(i, j, k);
+    double L = -nu[id] * fx[id] - nv[id] * fy[id];
+    if (DIM == 3) L = L - nw[id] * fz[id];
+    double src;
+    if (stage == 0) {
+        l1[id] = L;
+        src = L;
+    } else if (stage == 1) {
+        l2[id] = L;
+        src = (-3.0 * l1[id] + L) / 4.0;
+    } else {
+        src = (-l1[id] - l2[id] + 8.0 * L) / 12.0;
+    }
+    phin[id] = phi[id] + src * dt;
+}
+
 int ls_advect(fcfd_ctx *c) {
     const Geom &G = c->G;
     dim3 blk(64, 4), grd = grid_interior(G, blk);
     double *a = c->f[FCFD_PHI], *b = c->phiB;
+    if (c->pr.ls_scheme == FCFD_LS_UCCD) {
+        // upwind-CCD derivatives park in the predictor's spare velocity buffers
+        double *d[3] = {c->un, c->vn, c->wn};
+        const double *vel[3] = {c->f[FCFD_NU], c->f[FCFD_NV], c->f[FCFD_NW]};
+        for (int s = 0; s < 3; ++s) {
+            for (int dir = 0; dir < G.dim; ++dir) FCFD_TRY(uccd_dir(c, dir, a, vel[dir], d[dir], nullptr));
+            {
+                ProfScope ps_(c, R_LS_ADV);
+                if (G.dim == 3)
+                    hipLaunchKernelGGL(k_ls_uccd_stage<3>, grd, blk, 0, c->stream, a, b, vel[0], vel[1], vel[2], d[0],
+                                       d[1], d[2], c->l1, c->l2, G, c->pr.dt, s);
+                else
+                    hipLaunchKernelGGL(k_ls_uccd_stage<2>, grd, blk, 0, c->stream, a, b, vel[0], vel[1], nullptr, d[0],
+                                       d[1], nullptr, c->l1, c->l2, G, c->pr.dt, s);
+                FCFD_CHECK(hipGetLastError());
+            }
+            FCFD_TRY(bc_scalar(c, b));
+            double *t = a; a = b; b = t;
+        }
+        c->phiB = c->f[FCFD_PHI];
+        c->f[FCFD_PHI] = a;
+        return FCFD_OK;
+    }
     for (int s = 0; s < 3; ++s) {
         {
             ProfScope ps_(c, R_LS_ADV);

@@ -94,6 +94,15 @@ extern "C" int fcfd_ns_ab_predictor(fcfd_ctx *c) {
     return ns_predictor(c, c->f[FCFD_U], c->f[FCFD_V], c->f[FCFD_W]);
 }
 
+extern "C" int fcfd_uccd(fcfd_ctx *c) {
+    CTX_OK(c);
+    double **F = c->f;
+    const int fx[3] = {FCFD_NX, FCFD_NY, FCFD_NZ}, fxx[3] = {FCFD_NXX, FCFD_NYY, FCFD_NZZ};
+    const int vel[3] = {FCFD_NU, FCFD_NV, FCFD_NW};
+    for (int d = 0; d < c->G.dim; ++d) FCFD_TRY(uccd_dir(c, d, F[FCFD_PHI], F[vel[d]], F[fx[d]], F[fxx[d]]));
+    return FCFD_OK;
+}
+
 extern "C" int fcfd_node_vel(fcfd_ctx *c) {
     CTX_OK(c);
     return node_vel(c);

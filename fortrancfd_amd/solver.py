@@ -55,7 +55,7 @@ def lib():
         "fcfd_bc": [vp, i32], "fcfd_velbc": [vp],
         "fcfd_ls_funs": [vp], "fcfd_rho_mu": [vp], "fcfd_ls_mv": [vp, vp, vp],
         "fcfd_level_set_rk3_solver": [vp], "fcfd_level_set_rk3_redis": [vp, i32, vp],
-        "fcfd_mass_preserving_level_set": [vp], "fcfd_surface_norms": [vp], "fcfd_curv": [vp],
+        "fcfd_mass_preserving_level_set": [vp], "fcfd_surface_norms": [vp], "fcfd_curv": [vp], "fcfd_uccd": [vp],
         "fcfd_ppe_sor_init": [vp], "fcfd_ppe_sor_solver": [vp, dbl, i32, vp, vp],
         "fcfd_ppe_solve_rhs": [vp, dbl, i32, vp, vp],
         "fcfd_ns_momentum_correction": [vp], "fcfd_ns_ab_predictor": [vp],
@@ -196,6 +196,10 @@ class Solver:
 
     def surface_norms(self) -> None:
         self._call("fcfd_surface_norms")
+
+    def uccd(self) -> None:
+        """root_solve("uccd") of phi with nvel%old: phi_x -> nx/ny/nz, fxx -> nxx/nyy/nzz."""
+        self._call("fcfd_uccd")
 
     def curv(self) -> None:
         self._call("fcfd_curv")

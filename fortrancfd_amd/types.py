@@ -56,7 +56,7 @@ class FcfdParams(C.Structure):
         ("ubc", C.c_int32 * 2), ("vbc", C.c_int32 * 2), ("wbc", C.c_int32 * 2),
         ("redis_interval", C.c_int32), ("picard_iters", C.c_int32),
         ("ppe_precision", C.c_int32), ("ppe_max_cycles", C.c_int32), ("ppe_krylov", C.c_int32),
-        ("ppe_inner", C.c_int32), ("pad_", C.c_int32),
+        ("ppe_inner", C.c_int32), ("ls_scheme", C.c_int32),
     ]
 
 
@@ -114,6 +114,7 @@ class Case:
     ppe_max_cycles: int = 200
     ppe_krylov: int = 1
     ppe_inner: int = 1
+    ls_scheme: int = 0          # 0: WENO5 flux split, 1: upwind CCD (FCFD_LS_*)
     imass: float = 0.0
     name: str = "case"
 
@@ -210,6 +211,7 @@ class Case:
         p.redis_interval, p.picard_iters = self.redis_interval, self.picard_iters
         p.ppe_precision, p.ppe_max_cycles = self.ppe_precision, self.ppe_max_cycles
         p.ppe_krylov, p.ppe_inner = self.ppe_krylov, self.ppe_inner
+        p.ls_scheme = self.ls_scheme
         return p
 
     def ug(self) -> int:

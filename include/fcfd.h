@@ -65,9 +65,18 @@ int fcfd_rho_mu(fcfd_ctx *ctx);
 int fcfd_ls_mv(fcfd_ctx *ctx, double *mass, double *vol);
 
 /* ---- level-set advection (`.../interface_solver/level_set_rk3_solver.f90:1-219`) ---- */
-/* level_set_rk3_solver: WENO5-JS flux split + SSP-RK3 of phi with the cell
- * velocity nvel (fields NU/NV/NW hold nvel%old of the reference). */
+/* level_set_rk3_solver: SSP-RK3 of phi with the cell velocity nvel (fields
+ * NU/NV/NW hold nvel%old of the reference); the source is the WENO5-JS flux
+ * split, or with params.ls_scheme = FCFD_LS_UCCD the non-conservative
+ * upwind-CCD form -(u phi_x + v phi_y + w phi_z) (the commented alternative
+ * of level_set_rk3_solver.f90:109,123,137,155-157). */
 int fcfd_level_set_rk3_solver(fcfd_ctx *ctx);
+
+/* root_solve("uccd") (`.../lib/solver/ccd_solvers.f90:161-210,308-341,400-412`)
+ * of phi along every interior line, upwind by NU/NV/NW (nvel%old): phi_x ->
+ * NX/NY/NZ, the averaged second derivative -> NXX/NYY/NZZ.  The derivative
+ * kernel of the UCCD advection source selected by params.ls_scheme. */
+int fcfd_uccd(fcfd_ctx *ctx);
 
 /* ---- reinitialisation (`.../interface_solver/level_set_rk3_redis.f90:1-410`) ---- */
 /* level_set_rk3_redis(btn): btn = 1 maintains the distance property with the

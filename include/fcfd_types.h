@@ -65,8 +65,17 @@ typedef struct fcfd_params {
     int32_t ppe_max_cycles;      /* cap on outer iterations per PPE solve      */
     int32_t ppe_krylov;          /* 1: flexible CG around the V-cycle; 0: plain */
     int32_t ppe_inner;           /* V-cycles per outer iteration (>= 1)        */
-    int32_t pad_;
+    int32_t ls_scheme;           /* FCFD_LS_WENO5 or FCFD_LS_UCCD              */
 } fcfd_params;
+
+/* Level-set advection source of level_set_rk3_solver: the reference's WENO5
+ * flux split, or its upwind-CCD alternative s = -(u phi_x + v phi_y + w phi_z)
+ * (`Two phase flows/newOpenMP/3D+IBM/src/interface_solver/
+ * level_set_rk3_solver.f90:109,123,137,155-157`). */
+enum fcfd_ls_scheme {
+    FCFD_LS_WENO5 = 0,
+    FCFD_LS_UCCD = 1
+};
 
 enum fcfd_ppe_precision {
     FCFD_PPE_FP64 = 0,           /* FP64 smoothing/transfer, FP64 everywhere   */

@@ -507,14 +507,22 @@ static void ls_source(orc_state *s, double *out) {
             }
 }
 
-/* level_set_rk3_solver: SSP-RK3 in increment form (3D :1-65, 2D :1-59) */
+static void ls_source_uccd(orc_state *s, double *out);
+
+/* level_set_rk3_solver: SSP-RK3 in increment form (3D :1-65, 2D :1-59); the
+ * source is the WENO5 flux split or, with ls_scheme = FCFD_LS_UCCD, the
+ * non-conservative UCCD form */
 int orc_level_set_rk3_solver(orc_state *s) {
     double *phi = s->phi.lv[L_NOW];
     const double dt = s->pr.dt;
     double *L[3] = {s->t_l1, s->t_l2, s->t_l3};
     for (int st = 0; st < 3; ++st) {
-        ls_source_setup(s);
-        ls_source(s, L[st]);
+        if (s->pr.ls_scheme == FCFD_LS_UCCD) {
+            ls_source_uccd(s, L[st]);
+        } else {
+            ls_source_setup(s);
+            ls_source(s, L[st]);
+        }
         double *l1 = s->t_l1, *l2 = s->t_l2, *l3 = s->t_l3;
 #pragma omp parallel for collapse(2)
         for (int k = 0; k < s->nz; ++k)
@@ -743,27 +751,33 @@ int orc_level_set_rk3_redis(orc_state *s, int btn) {
 
 /* ----------------------------------------------------------- CCD lines */
 
-/* working_matrix_init_ccd + bc + TWIN_DEC, then asgn_src_ccd + TWIN_BKS
- * (3D+IBM/lib/solver/ccd_solvers.f90:76-159,287-306,379-399;
- *  3D+IBM/lib/solver/matrix_solver.f90:6-78).  Line of N points, stride st. */
-static void ccd_line(const double *f, double *fx, double *fxx, int N, long st, double dx) {
-    double *A = (double *)malloc(sizeof(double) * 3 * N), *B = (double *)malloc(sizeof(double) * 3 * N);
-    double *AA = (double *)malloc(sizeof(double) * 3 * N), *BB = (double *)malloc(sizeof(double) * 3 * N);
-    double *S = (double *)malloc(sizeof(double) * N), *SS = (double *)malloc(sizeof(double) * N);
+/* 2x2 block-tridiagonal line system of the CCD family: rows (1,2,3) =
+ * (sub, diag, super) per point, as working_matrix's A, B, AA, BB
+ * (3D+IBM/lib/solver/ccd_solvers.f90:5-11). */
+typedef struct { double *A, *B, *AA, *BB; } twin_mat;
 #define M(X, r, i) X[(i) * 3 + (r) - 1]
-#define FF(i) f[(long)(i) * st]
+
+static twin_mat twin_alloc(int N) {
+    twin_mat m;
+    m.A = (double *)calloc(3 * (size_t)N, sizeof(double)); m.B = (double *)calloc(3 * (size_t)N, sizeof(double));
+    m.AA = (double *)calloc(3 * (size_t)N, sizeof(double)); m.BB = (double *)calloc(3 * (size_t)N, sizeof(double));
+    return m;
+}
+static void twin_free(twin_mat *m) { free(m->A); free(m->B); free(m->AA); free(m->BB); }
+
+/* working_matrix_bc: one-sided closure rows of both ends (ccd_solvers.f90:76-130) */
+static void twin_bc(twin_mat *m, int N, double dx) {
     const int is = 0, ie = N - 1;
-    for (int i = is + 1; i <= ie - 1; ++i) {
-        M(A, 1, i) = 7.0 / 16.0; M(A, 2, i) = 1.0; M(A, 3, i) = 7.0 / 16.0;
-        M(B, 1, i) = dx / 16.0; M(B, 2, i) = 0.0; M(B, 3, i) = -dx / 16.0;
-        M(AA, 1, i) = -9.0 / 8.0 / dx; M(AA, 2, i) = 0.0; M(AA, 3, i) = 9.0 / 8.0 / dx;
-        M(BB, 1, i) = -1.0 / 8.0; M(BB, 2, i) = 1.0; M(BB, 3, i) = -1.0 / 8.0;
-    }
-    M(A, 2, is) = 1.0; M(B, 2, is) = 0.0; M(AA, 2, is) = 0.0; M(BB, 2, is) = 1.0;
-    M(A, 3, is) = 2.0; M(B, 3, is) = -dx; M(AA, 3, is) = -2.5 / dx; M(BB, 3, is) = 8.5;
-    M(A, 2, ie) = 1.0; M(B, 2, ie) = 0.0; M(AA, 2, ie) = 0.0; M(BB, 2, ie) = 1.0;
-    M(A, 1, ie) = 2.0; M(B, 1, ie) = dx; M(AA, 1, ie) = 2.5 / dx; M(BB, 1, ie) = 8.5;
-    /* TWIN_DEC */
+    M(m->A, 2, is) = 1.0; M(m->B, 2, is) = 0.0; M(m->AA, 2, is) = 0.0; M(m->BB, 2, is) = 1.0;
+    M(m->A, 3, is) = 2.0; M(m->B, 3, is) = -dx; M(m->AA, 3, is) = -2.5 / dx; M(m->BB, 3, is) = 8.5;
+    M(m->A, 2, ie) = 1.0; M(m->B, 2, ie) = 0.0; M(m->AA, 2, ie) = 0.0; M(m->BB, 2, ie) = 1.0;
+    M(m->A, 1, ie) = 2.0; M(m->B, 1, ie) = dx; M(m->AA, 1, ie) = 2.5 / dx; M(m->BB, 1, ie) = 8.5;
+}
+
+/* TWIN_DEC (3D+IBM/lib/solver/matrix_solver.f90:6-48) */
+static void twin_dec(twin_mat *m, int N) {
+    double *A = m->A, *B = m->B, *AA = m->AA, *BB = m->BB;
+    const int is = 0, ie = N - 1;
     M(A, 1, is) = 0.0; M(AA, 1, is) = 0.0; M(A, 3, ie) = 0.0; M(AA, 3, ie) = 0.0;
     M(B, 1, is) = 0.0; M(BB, 1, is) = 0.0; M(B, 3, ie) = 0.0; M(BB, 3, ie) = 0.0;
     for (int i = is + 1; i <= ie; ++i) {
@@ -781,17 +795,12 @@ static void ccd_line(const double *f, double *fx, double *fxx, int N, long st, d
         M(BB, 1, i) = SC2;
         M(BB, 2, i) = M(BB, 2, i) - (SC1 * M(B, 3, i - 1) + SC2 * M(BB, 3, i - 1));
     }
-    /* asgn_src_ccd */
-    const double dx2 = dx * dx;
-    S[is] = (-3.5 * FF(is) + 4.0 * FF(is + 1) - 0.5 * FF(is + 2)) / dx;
-    SS[is] = (34.0 / 3.0 * FF(is) - 83.0 / 4.0 * FF(is + 1) + 10.0 * FF(is + 2) - 7.0 / 12.0 * FF(is + 3)) / dx2;
-    for (int i = is + 1; i <= ie - 1; ++i) {
-        S[i] = 15.0 / 16.0 * (-FF(i - 1) + FF(i + 1)) / dx;
-        SS[i] = (3.0 * FF(i - 1) - 6.0 * FF(i) + 3.0 * FF(i + 1)) / dx2;
-    }
-    S[ie] = -(-3.5 * FF(ie) + 4.0 * FF(ie - 1) - 0.5 * FF(ie - 2)) / dx;
-    SS[ie] = (34.0 / 3.0 * FF(ie) - 83.0 / 4.0 * FF(ie - 1) + 10.0 * FF(ie - 2) - 7.0 / 12.0 * FF(ie - 3)) / dx2;
-    /* TWIN_BKS */
+}
+
+/* TWIN_BKS (matrix_solver.f90:50-78): S, SS rhs in, solution out */
+static void twin_bks(const twin_mat *m, double *S, double *SS, int N) {
+    const double *A = m->A, *B = m->B, *AA = m->AA, *BB = m->BB;
+    const int is = 0, ie = N - 1;
     for (int i = is + 1; i <= ie; ++i) {
         S[i] = S[i] - (M(A, 1, i) * S[i - 1] + M(B, 1, i) * SS[i - 1]);
         SS[i] = SS[i] - (M(AA, 1, i) * S[i - 1] + M(BB, 1, i) * SS[i - 1]);
@@ -808,11 +817,92 @@ static void ccd_line(const double *f, double *fx, double *fxx, int N, long st, d
         SolSS = (M(A, 2, i) * SS[i] - M(AA, 2, i) * S[i]) / Den;
         S[i] = SolS; SS[i] = SolSS;
     }
-    for (int i = is; i <= ie; ++i) { fx[(long)i * st] = S[i]; fxx[(long)i * st] = SS[i]; }
-#undef M
-#undef FF
-    free(A); free(B); free(AA); free(BB); free(S); free(SS);
 }
+
+/* one-sided closure sources of both ends (asgn_src_ccd / _uccd,
+ * ccd_solvers.f90:287-306,308-341) */
+#define FF(i) f[(long)(i) * st]
+static void src_ends(const double *f, long st, int N, double dx, double *S, double *SS) {
+    const int is = 0, ie = N - 1;
+    const double dx2 = dx * dx;
+    S[is] = (-3.5 * FF(is) + 4.0 * FF(is + 1) - 0.5 * FF(is + 2)) / dx;
+    SS[is] = (34.0 / 3.0 * FF(is) - 83.0 / 4.0 * FF(is + 1) + 10.0 * FF(is + 2) - 7.0 / 12.0 * FF(is + 3)) / dx2;
+    S[ie] = -(-3.5 * FF(ie) + 4.0 * FF(ie - 1) - 0.5 * FF(ie - 2)) / dx;
+    SS[ie] = (34.0 / 3.0 * FF(ie) - 83.0 / 4.0 * FF(ie - 1) + 10.0 * FF(ie - 2) - 7.0 / 12.0 * FF(ie - 3)) / dx2;
+}
+
+/* working_matrix_init_ccd + bc + TWIN_DEC, then asgn_src_ccd + TWIN_BKS
+ * (3D+IBM/lib/solver/ccd_solvers.f90:76-159,287-306,379-399;
+ *  3D+IBM/lib/solver/matrix_solver.f90:6-78).  Line of N points, stride st. */
+static void ccd_line(const double *f, double *fx, double *fxx, int N, long st, double dx) {
+    twin_mat m = twin_alloc(N);
+    double *S = (double *)malloc(sizeof(double) * N), *SS = (double *)malloc(sizeof(double) * N);
+    const int is = 0, ie = N - 1;
+    for (int i = is + 1; i <= ie - 1; ++i) {
+        M(m.A, 1, i) = 7.0 / 16.0; M(m.A, 2, i) = 1.0; M(m.A, 3, i) = 7.0 / 16.0;
+        M(m.B, 1, i) = dx / 16.0; M(m.B, 2, i) = 0.0; M(m.B, 3, i) = -dx / 16.0;
+        M(m.AA, 1, i) = -9.0 / 8.0 / dx; M(m.AA, 2, i) = 0.0; M(m.AA, 3, i) = 9.0 / 8.0 / dx;
+        M(m.BB, 1, i) = -1.0 / 8.0; M(m.BB, 2, i) = 1.0; M(m.BB, 3, i) = -1.0 / 8.0;
+    }
+    twin_bc(&m, N, dx);
+    twin_dec(&m, N);
+    const double dx2 = dx * dx;
+    src_ends(f, st, N, dx, S, SS);
+    for (int i = is + 1; i <= ie - 1; ++i) {
+        S[i] = 15.0 / 16.0 * (-FF(i - 1) + FF(i + 1)) / dx;
+        SS[i] = (3.0 * FF(i - 1) - 6.0 * FF(i) + 3.0 * FF(i + 1)) / dx2;
+    }
+    twin_bks(&m, S, SS, N);
+    for (int i = is; i <= ie; ++i) { fx[(long)i * st] = S[i]; fxx[(long)i * st] = SS[i]; }
+    twin_free(&m);
+    free(S); free(SS);
+}
+
+/* root_solve("uccd"): working_matrix_init_uccd + bc, TWIN_DEC of the up- and
+ * down-wind systems, asgn_src_uccd, TWIN_BKS of both, then fx = SU where
+ * u > 0 else SD, fxx = (SSU + SSD)/2 (3D+IBM/lib/solver/ccd_solvers.f90:
+ * 76-130,161-210,308-341,400-412).  Line of N points, stride st. */
+static void uccd_line(const double *f, const double *u, double *fx, double *fxx, int N, long st, double dx) {
+    twin_mat U = twin_alloc(N), D = twin_alloc(N);
+    double *SU = (double *)malloc(sizeof(double) * N), *SSU = (double *)malloc(sizeof(double) * N);
+    double *SD = (double *)malloc(sizeof(double) * N), *SSD = (double *)malloc(sizeof(double) * N);
+    const int is = 0, ie = N - 1;
+    const double A1 = 0.875, B1 = 0.1251282341599089, B2 = -0.2487176584009104, B3 = 0.0001282341599089;
+    twin_bc(&U, N, dx);
+    twin_bc(&D, N, dx);
+    for (int i = is + 1; i <= ie - 1; ++i) {
+        M(U.A, 1, i) = A1; M(U.A, 2, i) = 1.0; M(U.A, 3, i) = 0.0;
+        M(U.B, 1, i) = B1 * dx; M(U.B, 2, i) = B2 * dx; M(U.B, 3, i) = B3 * dx;
+        M(U.AA, 1, i) = -9.0 / 8.0 / dx; M(U.AA, 2, i) = 0.0; M(U.AA, 3, i) = 9.0 / 8.0 / dx;
+        M(U.BB, 1, i) = -1.0 / 8.0; M(U.BB, 2, i) = 1.0; M(U.BB, 3, i) = -1.0 / 8.0;
+        M(D.A, 1, i) = 0.0; M(D.A, 2, i) = 1.0; M(D.A, 3, i) = A1;
+        M(D.B, 1, i) = -B3 * dx; M(D.B, 2, i) = -B2 * dx; M(D.B, 3, i) = -B1 * dx;
+        M(D.AA, 1, i) = -9.0 / 8.0 / dx; M(D.AA, 2, i) = 0.0; M(D.AA, 3, i) = 9.0 / 8.0 / dx;
+        M(D.BB, 1, i) = -1.0 / 8.0; M(D.BB, 2, i) = 1.0; M(D.BB, 3, i) = -1.0 / 8.0;
+    }
+    twin_dec(&U, N);
+    twin_dec(&D, N);
+    const double dx2 = dx * dx;
+    src_ends(f, st, N, dx, SU, SSU);
+    src_ends(f, st, N, dx, SD, SSD);
+    const double c3 = -0.06096119008109, c2 = 1.99692238016218, c1 = -1.93596119008109;
+    for (int i = is + 1; i <= ie - 1; ++i) {
+        SU[i] = (c1 * FF(i - 1) + c2 * FF(i) + c3 * FF(i + 1)) / dx;
+        SSU[i] = (3.0 * FF(i - 1) - 6.0 * FF(i) + 3.0 * FF(i + 1)) / dx2;
+        SD[i] = -(c3 * FF(i - 1) + c2 * FF(i) + c1 * FF(i + 1)) / dx;
+        SSD[i] = (3.0 * FF(i - 1) - 6.0 * FF(i) + 3.0 * FF(i + 1)) / dx2;
+    }
+    twin_bks(&U, SU, SSU, N);
+    twin_bks(&D, SD, SSD, N);
+    for (int i = is; i <= ie; ++i) {
+        fx[(long)i * st] = u[(long)i * st] > 0.0 ? SU[i] : SD[i];
+        fxx[(long)i * st] = 0.5 * (SSU[i] + SSD[i]);
+    }
+    twin_free(&U); twin_free(&D);
+    free(SU); free(SSU); free(SD); free(SSD);
+}
+#undef FF
+#undef M
 
 /* ccd along direction d of every line of the full ghosted box */
 static void ccd_dir(orc_state *s, int d, const double *f, double *fx, double *fxx) {
@@ -839,6 +929,81 @@ static void ccd_dir(orc_state *s, int d, const double *f, double *fx, double *fx
                 ccd_line(f + o, fx + o, fxx + o, s->nz + 2 * g, s->sxy, s->dz);
             }
     }
+}
+
+/* root_solve("uccd") along direction d on every interior line (the lines
+ * the reference's UCCD level-set source walks, with their ghost ends:
+ * DomainDecomposition/3D/src/interface_solver/level_set_rk3_solver.f90:206-226;
+ * the commented alternative of 3D+IBM/.../level_set_rk3_solver.f90:109,123,137) */
+static void uccd_dir(orc_state *s, int d, const double *f, const double *u, double *fx, double *fxx) {
+    const int g = s->g, gz = s->gz;
+    (void)gz;
+    if (d == 0) {
+#pragma omp parallel for collapse(2)
+        for (int k = 0; k < s->nz; ++k)
+            for (int j = 0; j < s->ny; ++j) {
+                long o = ID(s, -g, j, k);
+                uccd_line(f + o, u + o, fx + o, fxx + o, s->nx + 2 * g, 1, s->dx);
+            }
+    } else if (d == 1) {
+#pragma omp parallel for collapse(2)
+        for (int k = 0; k < s->nz; ++k)
+            for (int i = 0; i < s->nx; ++i) {
+                long o = ID(s, i, -g, k);
+                uccd_line(f + o, u + o, fx + o, fxx + o, s->ny + 2 * g, s->sx, s->dy);
+            }
+    } else {
+#pragma omp parallel for collapse(2)
+        for (int j = 0; j < s->ny; ++j)
+            for (int i = 0; i < s->nx; ++i) {
+                long o = ID(s, i, j, -g);
+                uccd_line(f + o, u + o, fx + o, fxx + o, s->nz + 2 * g, s->sxy, s->dz);
+            }
+    }
+}
+
+/* UCCD derivatives of phi with the cell velocity nvel%old as upwind
+ * selector: fx -> normals x/y/z, fxx -> normals xx/yy/zz (test entry) */
+int orc_uccd(orc_state *s) {
+    double *phi = s->phi.lv[L_NOW];
+    double *vel[3] = {s->nu.lv[L_OLD], s->nv.lv[L_OLD], s->nw.lv[L_OLD]};
+    double *fx[3] = {s->nx_.lv[L_NOW], s->ny_.lv[L_NOW], s->nz_.lv[L_NOW]};
+    double *fxx[3] = {s->nxx.lv[L_NOW], s->nyy.lv[L_NOW], s->nzz.lv[L_NOW]};
+    for (int d = 0; d < s->dim; ++d) uccd_dir(s, d, phi, vel[d], fx[d], fxx[d]);
+    return FCFD_OK;
+}
+
+/* UCCD level-set source s = -(u phi_x + v phi_y + w phi_z), phi_x from
+ * upwind CCD (DomainDecomposition/3D/.../level_set_rk3_solver.f90:228-236;
+ * the commented form of 3D+IBM/.../level_set_rk3_solver.f90:155-157) */
+static void ls_source_uccd(orc_state *s, double *out) {
+    double *phi = s->phi.lv[L_NOW];
+    double *vel[3] = {s->nu.lv[L_OLD], s->nv.lv[L_OLD], s->nw.lv[L_OLD]};
+    for (int d = 0; d < s->dim; ++d) uccd_dir(s, d, phi, vel[d], s->t_s1[d], s->t_s2[d]);
+    double *ax = s->t_s1[0], *ay = s->t_s1[1], *az = s->t_s1[2];
+    double *u = vel[0], *v = vel[1], *w = vel[2];
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < s->nz; ++k)
+        for (int j = 0; j < s->ny; ++j)
+            for (int i = 0; i < s->nx; ++i) {
+                long id = ID(s, i, j, k);
+                double val = -u[id] * ax[id] - v[id] * ay[id];
+                if (s->dim == 3) val = val - w[id] * az[id];
+                out[id] = val;
+            }
+}
+
+/* the level-set source of the configured scheme for the current phi and
+ * nvel%old, written to the grad field (test entry: no RK stage, no bc) */
+int orc_level_set_source(orc_state *s) {
+    double *out = s->grad.lv[L_NOW];
+    if (s->pr.ls_scheme == FCFD_LS_UCCD) {
+        ls_source_uccd(s, out);
+    } else {
+        ls_source_setup(s);
+        ls_source(s, out);
+    }
+    return FCFD_OK;
 }
 
 /* surface_norms (3D+IBM/.../others_normals.f90:1-83; 2D :1-40) */

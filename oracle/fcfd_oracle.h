@@ -46,6 +46,8 @@ int orc_ls_mv(orc_state *s);
 int orc_level_set_rk3_solver(orc_state *s);
 int orc_level_set_rk3_redis(orc_state *s, int btn);   /* returns iterations */
 int orc_mass_preserving_level_set(orc_state *s);
+int orc_uccd(orc_state *s);                  /* UCCD phi derivatives -> normals (test entry) */
+int orc_level_set_source(orc_state *s);      /* source of the configured scheme -> grad (test entry) */
 int orc_surface_norms(orc_state *s);
 int orc_curv(orc_state *s);
 int orc_ppe_sor_init(orc_state *s);

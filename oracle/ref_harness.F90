@@ -69,6 +69,9 @@ case ('ls_mv')
 case ('ls_adv')
     call p%loc%nvel%switch
     call level_set_rk3_solver
+case ('uccd')
+    call p%loc%nvel%switch
+    call harness_uccd
 case ('redis0')
     call level_set_rk3_redis(0)
 case ('redis1')
@@ -118,6 +121,44 @@ call dump_all()
 contains
 
 include 'included.f90'
+
+! root_solve("uccd") of phi on every interior line with nvel%old as the
+! upwind selector (the call of the reference's commented UCCD level-set
+! source, `3D+IBM/src/interface_solver/level_set_rk3_solver.f90:109,123,137`,
+! in the current ccd_root interface); fx -> normals x/y/z, fxx -> xx/yy/zz.
+subroutine harness_uccd()
+    integer :: i, j
+#ifdef HARNESS_3D
+    integer :: k
+    do k = p%loc%ks, p%loc%ke
+    do j = p%loc%js, p%loc%je
+        call p%loc%ccdsolvers%x%solve("uccd", p%loc%phi%now(:,j,k), p%loc%normals%x%now(:,j,k), &
+            & p%loc%normals%xx%now(:,j,k), p%loc%nvel%x%old(:,j,k))
+    end do
+    end do
+    do k = p%loc%ks, p%loc%ke
+    do i = p%loc%is, p%loc%ie
+        call p%loc%ccdsolvers%y%solve("uccd", p%loc%phi%now(i,:,k), p%loc%normals%y%now(i,:,k), &
+            & p%loc%normals%yy%now(i,:,k), p%loc%nvel%y%old(i,:,k))
+    end do
+    end do
+    do j = p%loc%js, p%loc%je
+    do i = p%loc%is, p%loc%ie
+        call p%loc%ccdsolvers%z%solve("uccd", p%loc%phi%now(i,j,:), p%loc%normals%z%now(i,j,:), &
+            & p%loc%normals%zz%now(i,j,:), p%loc%nvel%z%old(i,j,:))
+    end do
+    end do
+#else
+    do j = p%loc%js, p%loc%je
+        call p%loc%ccdsolvers%x%solve("uccd", p%loc%phi%now(:,j), p%loc%normals%x%now(:,j), &
+            & p%loc%normals%xx%now(:,j), p%loc%nvel%x%old(:,j))
+    end do
+    do i = p%loc%is, p%loc%ie
+        call p%loc%ccdsolvers%y%solve("uccd", p%loc%phi%now(i,:), p%loc%normals%y%now(i,:), &
+            & p%loc%normals%yy%now(i,:), p%loc%nvel%y%old(i,:))
+    end do
+#endif
+end subroutine
 
 ! ns_solver without the 3D IBM gate (`3D+IBM/src/ns_solver/ns_solvers.f90:1-20`
 ! minus ibm_bc, which is out of scope); the 2D reference has no ibm_bc.

@@ -35,6 +35,8 @@ def ops(case):
     yield "redis0", {"phi": phi}, {}, [("level_set_rk3_redis", 0)], ["phi"]
     yield "mpls", {"phi": phi}, {"imass": 0.5}, [("mass_preserving_level_set",)], ["phi", "_mass"]
     yield "surface_norms", {"phi": phi}, {}, [("surface_norms",)], nf
+    nd = ["nx", "ny", "nxx", "nyy"] + (["nz", "nzz"] if case.dim == 3 else [])
+    yield "uccd", {"phi": phi, **NV}, {}, [("switch",), ("uccd",)], nd
     yield "curv", {"phi": phi}, {}, [("curv",)], nf + ["curv"]
     yield "ppe_init", {"rho": rho}, {}, [("ppe_sor_init",)], cf
     yield "projection", {"rho": rho, "p": seeded(case, 9), **V}, {}, [("ns_momentum_correction",)], vn

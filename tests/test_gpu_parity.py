@@ -83,6 +83,29 @@ def test_level_set_advection(tag, mk):
 
 
 @pytest.mark.parametrize("tag,mk", CASES)
+def test_uccd_derivatives(tag, mk):
+    """root_solve("uccd") of phi, upwind by the cell velocity (both signs)."""
+    case = mk()
+    f = {"phi": bubble(case, 0.3, 1), **node_velocities(case, 3)}
+    s, o = _pair(case, f)
+    s.uccd()
+    o.call("switch"); o.call("uccd")
+    names = ["nx", "ny", "nxx", "nyy"] + (["nz", "nzz"] if case.dim == 3 else [])
+    _cmp(s, o, names, 1e-12)
+
+
+@pytest.mark.parametrize("tag,mk", CASES)
+def test_level_set_advection_uccd(tag, mk):
+    """SSP-RK3 with the upwind-CCD source -(u phi_x + v phi_y + w phi_z)."""
+    case = mk(ls_scheme=1)
+    f = {"phi": bubble(case, 0.3, 1), **node_velocities(case, 3)}
+    s, o = _pair(case, f)
+    s.level_set_rk3_solver()
+    o.call("switch"); o.call("level_set_rk3_solver")
+    _cmp(s, o, ["phi"], 1e-12)
+
+
+@pytest.mark.parametrize("tag,mk", CASES)
 @pytest.mark.parametrize("btn", [1, 0])
 def test_reinitialisation(tag, mk, btn):
     case = mk()
