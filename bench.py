@@ -118,9 +118,11 @@ def cpu_baseline(steps: int = 5, n: int = 96, threads: int = 0) -> dict:
                       f"{used} OpenMP threads, {dt:.1f} s"}
 
 
-MG_VARIANTS = [  # (name, precision, krylov, V-cycles per outer iteration)
-    ("fp64_plain", 0, 0, 1), ("fp64_fcg", 0, 1, 1), ("fp64_fcg2", 0, 1, 2),
-    ("mixed_plain2", 1, 0, 2), ("mixed_fcg", 1, 1, 1), ("mixed_fcg2", 1, 1, 2),
+MG_VARIANTS = [  # (name, precision, krylov, V-cycles per outer iteration, pre/post sweeps)
+    ("fp64_plain", 0, 0, 1, 2, 2), ("fp64_fcg", 0, 1, 1, 2, 2), ("fp64_fcg_v11", 0, 1, 1, 1, 1),
+    ("fp64_fcg2", 0, 1, 2, 2, 2),
+    ("mixed_plain2", 1, 0, 2, 2, 2), ("mixed_fcg", 1, 1, 1, 2, 2), ("mixed_fcg_v11", 1, 1, 1, 1, 1),
+    ("mixed_fcg_v21", 1, 1, 1, 2, 1), ("mixed_fcg2", 1, 1, 2, 2, 2),
 ]
 
 
@@ -129,12 +131,12 @@ def mg_compare(s, case, variants=MG_VARIANTS) -> dict:
     of rho now and the rhs in PPE_SRC), every variant from p = 0; the best
     FP64 and the best mixed variant are compared.  Restores the case's own
     solver settings afterwards."""
-    keep = (case.ppe_krylov, case.ppe_inner)
+    keep = (case.ppe_krylov, case.ppe_inner, case.ppe_nu1, case.ppe_nu2)
     out = {"variants": {}}
     zeros = np.zeros(case.size)
     sols = {}
-    for name, prec, kry, inner in variants:
-        s.set_params(ppe_krylov=kry, ppe_inner=inner)
+    for name, prec, kry, inner, nu1, nu2 in variants:
+        s.set_params(ppe_krylov=kry, ppe_inner=inner, ppe_nu1=nu1, ppe_nu2=nu2)
         best = None
         for rep in range(2):
             s.push("p", zeros)
@@ -146,7 +148,7 @@ def mg_compare(s, case, variants=MG_VARIANTS) -> dict:
             best = ms if best is None else min(best, ms)
         sols[name] = case.interior(s.pull("p")).copy()
         out["variants"][name] = {"ms": round(best, 3), "outer_iters": cyc, "rel_resid": rel}
-    s.set_params(ppe_krylov=keep[0], ppe_inner=keep[1])
+    s.set_params(ppe_krylov=keep[0], ppe_inner=keep[1], ppe_nu1=keep[2], ppe_nu2=keep[3])
     conv = {k: v for k, v in out["variants"].items() if v["rel_resid"] <= 1e-10}
     f64 = min((k for k in conv if k.startswith("fp64")), key=lambda k: conv[k]["ms"], default=None)
     mix = min((k for k in conv if k.startswith("mixed")), key=lambda k: conv[k]["ms"], default=None)

@@ -260,7 +260,8 @@ extern "C" int fcfd_set_params(fcfd_ctx *c, const fcfd_params *p) {
     if (!c || !p) return FCFD_ERR_ARG;
     c->pr = *p;
     c->diag.imass = p->imass;
-    return FCFD_OK;
+    // captured V-cycles bake in the sweep counts: capture again on next use
+    return mg_drop_graphs(c);
 }
 
 extern "C" int64_t fcfd_field_size(const fcfd_ctx *c) { return c ? (int64_t)c->G.ntot : 0; }

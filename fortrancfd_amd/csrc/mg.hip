@@ -355,13 +355,48 @@ __global__ void __launch_bounds__(256) k_sum_stage1(const double *__restrict__ p
     if (threadIdx.x == 0) partial2[s * 256 + blockIdx.x] = a;
 }
 
+// Device scalars of a solve (mg_scalars[0..63]).  The flexible-CG
+// coefficients are formed on the device so an iteration needs one host
+// read (the residual norm and the breakdown flag) and replays as one graph.
+enum {
+    SC_RZ = 0, SC_RZO = 1, SC_PQ = 2, SC_RR = 3, SC_MEAN = 4, SC_RZ_OLD = 5, SC_BETA = 6, SC_ALPHA = 7,
+    SC_FIRST = 8, SC_BREAK = 9, SC_NHOST = 10
+};
+enum { SUM_STORE = 0, SUM_CG_BETA = 1, SUM_CG_ALPHA = 2 };
+
+// op SUM_CG_BETA: out = {r.z, r.zold} -> beta = (r.z - r.zold) / rz_old
+// (0 on the first iteration), rz_old = r.z (Polak-Ribiere);
+// op SUM_CG_ALPHA: out = {p.Ap} -> alpha = rz_old / p.Ap.  A is negative
+// definite on the zero-mean space and the cycle approximates its inverse, so
+// r.z < 0 and p.Ap < 0 until round-off; otherwise alpha = 0 and the
+// breakdown flag stops the iteration.
 __global__ void __launch_bounds__(256) k_sum_stage2(const double *__restrict__ partial2, int ns,
-                                                    double *__restrict__ out) {
+                                                    double *__restrict__ out, int op, double *__restrict__ sc) {
     __shared__ double sh[4];
     for (int s = 0; s < ns; ++s) {
         double a = block_sum256(partial2[s * 256 + threadIdx.x], sh);
         if (threadIdx.x == 0) out[s] = a;
     }
+    if (threadIdx.x == 0 && op == SUM_CG_BETA) {
+        const double rz = out[0], rzo = out[1];
+        sc[SC_BETA] = sc[SC_FIRST] != 0.0 ? 0.0 : (rz - rzo) / sc[SC_RZ_OLD];
+        sc[SC_RZ_OLD] = rz;
+        sc[SC_FIRST] = 0.0;
+    } else if (threadIdx.x == 0 && op == SUM_CG_ALPHA) {
+        const double pq = out[0], rz = sc[SC_RZ_OLD];
+        const bool ok = pq < 0.0 && rz < 0.0;
+        sc[SC_ALPHA] = ok ? rz / pq : 0.0;
+        if (!ok) sc[SC_BREAK] = 1.0;
+    }
+}
+
+// start of a flexible-CG solve: first iteration, no breakdown
+__global__ void k_cg_init(double *sc) {
+    sc[SC_FIRST] = 1.0;
+    sc[SC_BREAK] = 0.0;
+    sc[SC_RZ_OLD] = 0.0;
+    sc[SC_BETA] = 0.0;
+    sc[SC_ALPHA] = 0.0;
 }
 
 // sum of an interior field into block partials
@@ -958,8 +993,9 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
 template <typename T, int DIM>
 __global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__restrict__ zo,
                                                  double *__restrict__ p, double *__restrict__ q,
-                                                 const double *__restrict__ rho, Lay L, Inv2 ih, double beta,
-                                                 double *__restrict__ partial) {
+                                                 const double *__restrict__ rho, Lay L, Inv2 ih,
+                                                 const double *__restrict__ sc, double *__restrict__ partial) {
+    const double beta = sc[SC_BETA];
     __shared__ double sh[4];
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
@@ -994,8 +1030,10 @@ __global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__r
 // rhs of the next preconditioning cycle (r32 / zeroed e32, or zeroed z64)
 __global__ void __launch_bounds__(256) k_fcg_update(double *__restrict__ x, const double *__restrict__ p,
                                                     const double *__restrict__ q, double *__restrict__ r, Lay L,
-                                                    double alpha, float *__restrict__ r32, float *__restrict__ e32,
-                                                    double *__restrict__ z64, double *__restrict__ partial) {
+                                                    const double *__restrict__ sc, float *__restrict__ r32,
+                                                    float *__restrict__ e32, double *__restrict__ z64,
+                                                    double *__restrict__ partial) {
+    const double alpha = sc[SC_ALPHA];
     __shared__ double sh[4];
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
@@ -1015,12 +1053,14 @@ __global__ void __launch_bounds__(256) k_fcg_update(double *__restrict__ x, cons
         partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = rr;
 }
 
-// out[0..ns) = sums of the ns slots of the block partials of a level-0 launch
-static int sum_partials(fcfd_ctx *c, long nb, int ns, double *out) {
+// out[0..ns) = sums of the ns slots of the block partials of a level-0
+// launch, then the scalar step `op` (SUM_*)
+static int sum_partials(fcfd_ctx *c, long nb, int ns, double *out, int op = SUM_STORE) {
     const long chunk = (nb + 255) / 256;
     hipLaunchKernelGGL(k_sum_stage1, dim3(256, ns), dim3(256), 0, c->stream, c->red.partial, nb, chunk,
                        c->red.partial2);
-    hipLaunchKernelGGL(k_sum_stage2, dim3(1), dim3(256), 0, c->stream, c->red.partial2, ns, out);
+    hipLaunchKernelGGL(k_sum_stage2, dim3(1), dim3(256), 0, c->stream, c->red.partial2, ns, out, op,
+                       c->mg_scalars);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
@@ -1040,34 +1080,16 @@ static int field_mean(fcfd_ctx *c, const double *a) {
     dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
     long nb = (long)g.x * g.y * g.z;
     hipLaunchKernelGGL(k_sum, g, blk, 0, c->stream, a, lay_of(G), c->red.partial);
-    FCFD_TRY(sum_partials(c, nb, 1, c->mg_scalars + 4));
+    FCFD_TRY(sum_partials(c, nb, 1, c->mg_scalars + SC_MEAN));
     return FCFD_OK;
 }
 
-// Run `inner` V-cycles of one kind from a captured graph (captured on first use;
-// the level arrays, rho and the level-0 vectors are fixed for the context):
-// kind 0: FP64 on (P, f64); kind 1: FP64 on (kz, kr) with the CG dots r.z,
-// r.zold; kind 2: FP32 on (e32, f32); kind 3: as 2 with the CG dots.  The
-// dots are left as block partials of the level-0 smoother grid.
-template <int DIM>
-static int run_vcycles(fcfd_ctx *c, int kind, int inner) {
-    inner = std::max(1, std::min(inner, 4));
-    MgLevel &M0 = c->mg[0];
-    const Dots<double> d64{c->kr, c->kzo, c->red.partial};
-    const Dots<float> d32{c->kr, c->kzo32, c->red.partial};
-    auto body = [&]() -> int {
-        for (int t = 0; t < inner; ++t) {
-            const bool last = t == inner - 1;
-            if (kind == 0) FCFD_TRY((vcycle<double, DIM>(c, 0, c->f[FCFD_P], M0.f64, c->f[FCFD_RHO], 2, 2)));
-            else if (kind == 1)
-                FCFD_TRY((vcycle<double, DIM>(c, 0, c->kz, c->kr, c->f[FCFD_RHO], 2, 2, last ? &d64 : nullptr)));
-            else
-                FCFD_TRY((vcycle<float, DIM>(c, 0, M0.e32, M0.f32, c->rho32, 2, 2,
-                                             (kind == 3 && last) ? &d32 : nullptr)));
-        }
-        return FCFD_OK;
-    };
-    // events recorded inside a graph cannot be timed: profile with direct launches
+// Replay `body` from a graph captured on its first use for (kind, inner):
+// the level arrays, rho and the level-0 vectors are fixed for the context and
+// every scalar the body needs lives in device memory.  With the profiler on,
+// the body is launched directly (events inside a graph cannot be timed).
+template <typename Body>
+static int run_graph(fcfd_ctx *c, int kind, int inner, Body body) {
     static const bool no_graph = getenv("FCFD_NO_GRAPH") != nullptr;
     if (no_graph || c->prof.on) return body();
     hipGraphExec_t &ex = c->vgraph[kind][inner];
@@ -1089,6 +1111,66 @@ static int run_vcycles(fcfd_ctx *c, int kind, int inner) {
     return FCFD_OK;
 }
 
+// `inner` V-cycles of one kind: 0 FP64 on (P, f64); 2 FP32 on (e32, f32)
+template <int DIM>
+static int vcycles(fcfd_ctx *c, int kind, int inner) {
+    MgLevel &M0 = c->mg[0];
+    const int nu1 = c->pr.ppe_nu1 > 0 ? c->pr.ppe_nu1 : 2, nu2 = c->pr.ppe_nu2 > 0 ? c->pr.ppe_nu2 : 2;
+    for (int t = 0; t < inner; ++t) {
+        if (kind == 0) FCFD_TRY((vcycle<double, DIM>(c, 0, c->f[FCFD_P], M0.f64, c->f[FCFD_RHO], nu1, nu2)));
+        else FCFD_TRY((vcycle<float, DIM>(c, 0, M0.e32, M0.f32, c->rho32, nu1, nu2)));
+    }
+    return FCFD_OK;
+}
+
+// One flexible-CG iteration, all on the device: z = V-cycle(s)(r) (kind 1:
+// FP64 z, kind 3: FP32 z) with r.z and r.zold folded into the last sweep,
+// beta; p = z + beta p, q = A z + beta q, z_old = z, p.q; alpha; x += alpha p,
+// r -= alpha q, the next cycle's rhs; ||r||^2; then the scalars to the host.
+template <int DIM>
+static int fcg_iteration(fcfd_ctx *c, int kind, int inner) {
+    const Geom &G = c->G;
+    dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
+    const long nb = (long)g.x * g.y * g.z;
+    const long nb_s = (long)(((G.nx + 1) / 2 + blk.x - 1) / blk.x) * ((G.ny + blk.y - 1) / blk.y) * G.nz;
+    const Lay L0 = lay_of(G);
+    const Inv2 ih = inv2(G.dx, G.dy, G.dz);
+    MgLevel &M0 = c->mg[0];
+    double *sc = c->mg_scalars, *rho = c->f[FCFD_RHO];
+    const bool mixed = kind == 3;
+    const int nu1 = c->pr.ppe_nu1 > 0 ? c->pr.ppe_nu1 : 2, nu2 = c->pr.ppe_nu2 > 0 ? c->pr.ppe_nu2 : 2;
+    const Dots<double> d64{c->kr, c->kzo, c->red.partial};
+    const Dots<float> d32{c->kr, c->kzo32, c->red.partial};
+    for (int t = 0; t < inner; ++t) {
+        const bool last = t == inner - 1;
+        if (mixed)
+            FCFD_TRY((vcycle<float, DIM>(c, 0, M0.e32, M0.f32, c->rho32, nu1, nu2, last ? &d32 : nullptr)));
+        else
+            FCFD_TRY((vcycle<double, DIM>(c, 0, c->kz, c->kr, rho, nu1, nu2, last ? &d64 : nullptr)));
+    }
+    FCFD_TRY(sum_partials(c, nb_s, 2, sc + SC_RZ, SUM_CG_BETA));
+    {
+        ProfScope ps_(c, R_FCG);
+        if (mixed)
+            hipLaunchKernelGGL((k_fcg_dir<float, DIM>), g, blk, 0, c->stream, M0.e32, c->kzo32, c->kp, c->kq, rho, L0,
+                               ih, sc, c->red.partial);
+        else
+            hipLaunchKernelGGL((k_fcg_dir<double, DIM>), g, blk, 0, c->stream, c->kz, c->kzo, c->kp, c->kq, rho, L0,
+                               ih, sc, c->red.partial);
+        FCFD_CHECK(hipGetLastError());
+    }
+    FCFD_TRY(sum_partials(c, nb, 1, sc + SC_PQ, SUM_CG_ALPHA));
+    {
+        ProfScope ps_(c, R_FCG);
+        hipLaunchKernelGGL(k_fcg_update, g, blk, 0, c->stream, c->f[FCFD_P], c->kp, c->kq, c->kr, L0, sc,
+                           mixed ? M0.f32 : nullptr, M0.e32, mixed ? nullptr : c->kz, c->red.partial);
+        FCFD_CHECK(hipGetLastError());
+    }
+    FCFD_TRY(sum_partials(c, nb, 1, sc + SC_RR, SUM_STORE));
+    FCFD_CHECK(hipMemcpyAsync(c->red.host, sc, SC_NHOST * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return FCFD_OK;
+}
+
 template <int DIM>
 static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel) {
     const Geom &G = c->G;
@@ -1103,13 +1185,12 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     const bool krylov = c->pr.ppe_krylov != 0;
     const int inner = std::max(1, std::min(c->pr.ppe_inner > 0 ? c->pr.ppe_inner : 1, 4));
     const int kind = mixed ? (krylov ? 3 : 2) : (krylov ? 1 : 0);
-    // blocks of the level-0 smoother grid (the CG dots are its block partials)
-    const long nb_s = (long)(((G.nx + 1) / 2 + blk.x - 1) / blk.x) * ((G.ny + blk.y - 1) / blk.y) * G.nz;
     const int maxc = c->pr.ppe_max_cycles > 0 ? c->pr.ppe_max_cycles : 200;
     static const bool verbose = getenv("FCFD_MG_VERBOSE") != nullptr;
     // compatibility of the Neumann problem: f = src - mean(src)
     FCFD_TRY(field_mean(c, c->f[FCFD_PPE_SRC]));
-    hipLaunchKernelGGL(k_center_copy, g, blk, 0, c->stream, c->f[FCFD_PPE_SRC], f, L0, c->mg_scalars + 4, 1.0 / ncell);
+    hipLaunchKernelGGL(k_center_copy, g, blk, 0, c->stream, c->f[FCFD_PPE_SRC], f, L0, c->mg_scalars + SC_MEAN,
+                       1.0 / ncell);
     FCFD_CHECK(hipGetLastError());
     // r0 = f - A p
     {
@@ -1125,12 +1206,11 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     int k = 0, since_best = 0, rc = FCFD_OK;
     if (verbose) fprintf(stderr, "fcfd mg[%s%s]: it 0 rel %.3e\n", mixed ? "mixed" : "fp64", krylov ? "+fcg" : "", r);
     if (!std::isfinite(r)) return FCFD_ERR_NAN;
-    double rz = 0.0, pq = 0.0;
-    bool first = true;
+    if (krylov) hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, c->stream, c->mg_scalars);
     while (r > rel_tol && k < maxc) {
         if (!krylov) {
             // defect correction (mixed) or plain V-cycles on p (FP64)
-            FCFD_TRY(run_vcycles<DIM>(c, kind, inner));
+            FCFD_TRY(run_graph(c, kind, inner, [&]() { return vcycles<DIM>(c, kind, inner); }));
             if (mixed) {
                 hipLaunchKernelGGL(k_add_f32, g, blk, 0, c->stream, p, M0.e32, L0);
                 FCFD_CHECK(hipGetLastError());
@@ -1144,40 +1224,16 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
             FCFD_TRY(sums_to_host(c, nb, 1, s2));
             FCFD_TRY(prof_graph_accum(c, kind, inner));
         } else {
-            // flexible CG preconditioned by the V-cycle(s)
-            FCFD_TRY(run_vcycles<DIM>(c, kind, inner));
-            double d[2];
-            FCFD_TRY(sums_to_host(c, nb_s, 2, d));
+            // flexible CG preconditioned by the V-cycle(s), one graph per iteration
+            FCFD_TRY(run_graph(c, kind, inner, [&]() { return fcg_iteration<DIM>(c, kind, inner); }));
+            FCFD_CHECK(hipStreamSynchronize(c->stream));
             FCFD_TRY(prof_graph_accum(c, kind, inner));
-            const double beta = first ? 0.0 : (d[0] - d[1]) / rz;
-            rz = d[0];
-            first = false;
-            {
-                ProfScope ps_(c, R_FCG);
-                if (mixed)
-                    hipLaunchKernelGGL((k_fcg_dir<float, DIM>), g, blk, 0, c->stream, M0.e32, c->kzo32, c->kp, c->kq,
-                                       rho, L0, ih, beta, c->red.partial);
-                else
-                    hipLaunchKernelGGL((k_fcg_dir<double, DIM>), g, blk, 0, c->stream, c->kz, c->kzo, c->kp, c->kq,
-                                       rho, L0, ih, beta, c->red.partial);
-                FCFD_CHECK(hipGetLastError());
-            }
-            FCFD_TRY(sums_to_host(c, nb, 1, &pq));
-            if (!(pq < 0.0) || !(rz < 0.0)) {
-                // A is negative definite on the zero-mean space and the cycle
-                // approximates its inverse, so r.z < 0 and p.Ap < 0 until the
-                // iteration reaches round-off (alpha = r.z / p.Ap > 0)
-                if (verbose) fprintf(stderr, "fcfd mg: breakdown rz %.3e pq %.3e\n", rz, pq);
+            if (c->red.host[SC_BREAK] != 0.0) {
+                if (verbose) fprintf(stderr, "fcfd mg: breakdown rz %.3e pq %.3e\n", c->red.host[SC_RZ_OLD],
+                                     c->red.host[SC_PQ]);
                 break;
             }
-            const double alpha = rz / pq;
-            {
-                ProfScope ps_(c, R_FCG);
-                hipLaunchKernelGGL(k_fcg_update, g, blk, 0, c->stream, p, c->kp, c->kq, c->kr, L0, alpha,
-                                   mixed ? M0.f32 : nullptr, M0.e32, mixed ? nullptr : c->kz, c->red.partial);
-                FCFD_CHECK(hipGetLastError());
-            }
-            FCFD_TRY(sums_to_host(c, nb, 1, s2));
+            s2[0] = c->red.host[SC_RR];
         }
         ++k;
         r = sqrt(s2[0]) / fn;
@@ -1190,7 +1246,7 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     if (r > rel_tol) rc = (r <= 10.0 * rel_tol && since_best >= 4) ? FCFD_OK : FCFD_ERR_NOT_CONVERGED;
     // mean(p) = 0 as the reference's iteration enforces, then ghost copy
     FCFD_TRY(field_mean(c, p));
-    hipLaunchKernelGGL(k_sub_scalar, g, blk, 0, c->stream, p, L0, c->mg_scalars + 4, 1.0 / ncell);
+    hipLaunchKernelGGL(k_sub_scalar, g, blk, 0, c->stream, p, L0, c->mg_scalars + SC_MEAN, 1.0 / ncell);
     FCFD_CHECK(hipGetLastError());
     FCFD_TRY(bc_scalar(c, p));
     if (cycles) *cycles = k;

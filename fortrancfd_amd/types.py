@@ -56,7 +56,7 @@ class FcfdParams(C.Structure):
         ("ubc", C.c_int32 * 2), ("vbc", C.c_int32 * 2), ("wbc", C.c_int32 * 2),
         ("redis_interval", C.c_int32), ("picard_iters", C.c_int32),
         ("ppe_precision", C.c_int32), ("ppe_max_cycles", C.c_int32), ("ppe_krylov", C.c_int32),
-        ("ppe_inner", C.c_int32), ("ls_scheme", C.c_int32),
+        ("ppe_inner", C.c_int32), ("ls_scheme", C.c_int32), ("ppe_nu1", C.c_int32), ("ppe_nu2", C.c_int32),
     ]
 
 
@@ -115,6 +115,8 @@ class Case:
     ppe_krylov: int = 1
     ppe_inner: int = 1
     ls_scheme: int = 0          # 0: WENO5 flux split, 1: upwind CCD (FCFD_LS_*)
+    ppe_nu1: int = 2            # RB-GS sweeps before / after the coarse correction
+    ppe_nu2: int = 2
     imass: float = 0.0
     name: str = "case"
 
@@ -212,6 +214,7 @@ class Case:
         p.ppe_precision, p.ppe_max_cycles = self.ppe_precision, self.ppe_max_cycles
         p.ppe_krylov, p.ppe_inner = self.ppe_krylov, self.ppe_inner
         p.ls_scheme = self.ls_scheme
+        p.ppe_nu1, p.ppe_nu2 = self.ppe_nu1, self.ppe_nu2
         return p
 
     def ug(self) -> int:

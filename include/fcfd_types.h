@@ -66,6 +66,7 @@ typedef struct fcfd_params {
     int32_t ppe_krylov;          /* 1: flexible CG around the V-cycle; 0: plain */
     int32_t ppe_inner;           /* V-cycles per outer iteration (>= 1)        */
     int32_t ls_scheme;           /* FCFD_LS_WENO5 or FCFD_LS_UCCD              */
+    int32_t ppe_nu1, ppe_nu2;    /* pre/post RB-GS sweeps per level (0: 2, 2)  */
 } fcfd_params;
 
 /* Level-set advection source of level_set_rk3_solver: the reference's WENO5
