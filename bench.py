@@ -119,10 +119,9 @@ def cpu_baseline(steps: int = 5, n: int = 96, threads: int = 0) -> dict:
 
 
 MG_VARIANTS = [  # (name, precision, krylov, V-cycles per outer iteration, pre/post sweeps)
-    ("fp64_plain", 0, 0, 1, 2, 2), ("fp64_fcg", 0, 1, 1, 2, 2), ("fp64_fcg_v11", 0, 1, 1, 1, 1),
-    ("fp64_fcg2", 0, 1, 2, 2, 2),
-    ("mixed_plain2", 1, 0, 2, 2, 2), ("mixed_fcg", 1, 1, 1, 2, 2), ("mixed_fcg_v11", 1, 1, 1, 1, 1),
-    ("mixed_fcg_v21", 1, 1, 1, 2, 1), ("mixed_fcg2", 1, 1, 2, 2, 2),
+    ("fp64_plain", 0, 0, 1, 2, 2), ("fp64_fcg", 0, 1, 1, 2, 2), ("fp64_fcg2", 0, 1, 2, 2, 2),
+    ("mixed_plain", 1, 0, 1, 2, 2), ("mixed_plain2", 1, 0, 2, 2, 2), ("mixed_fcg", 1, 1, 1, 2, 2),
+    ("mixed_fcg2", 1, 1, 2, 2, 2),
 ]
 
 
@@ -256,7 +255,12 @@ def main() -> int:
             gbs = rb[name] / (ms / cnt * 1e-3) / 1e9
             regions[name]["GB/s"] = round(gbs, 1)
             tot_bytes += rb[name] * cnt
-    dom = max(regions, key=lambda k: regions[k]["ms_total"])
+    # the roofline is quoted for the costliest region whose algorithmic bytes
+    # are defined: the small coarse multigrid levels (mg_smooth_coarse etc.)
+    # are launch-bound and L2-resident, and the event pairs around their
+    # ~4 us launches inflate their profiled share
+    dom_any = max(regions, key=lambda k: regions[k]["ms_total"])
+    dom = max((k for k in regions if k in rb), key=lambda k: regions[k]["ms_total"])
     dom_r = regions[dom]
     achieved = dom_r.get("GB/s")
     traffic, tsrc = profiled_traffic(dom) if case.nx == 256 else (None, None)
@@ -267,6 +271,8 @@ def main() -> int:
                 "share_of_step": round(dom_r["ms_total"] / ((tp1 - tp0) * 1e3), 4)}
     if tsrc:
         roofline["traffic_source"] = "profiles/" + tsrc
+    if dom_any != dom:
+        roofline["costliest_region_profiled"] = dom_any
     out = {
         "metric": "Mcell-steps/s @256^3 two-phase, %HBM peak; MG ms to 1e-10 mixed vs FP64",
         "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,

@@ -113,6 +113,9 @@ struct Prol {
 // PROL 0: plain values; 1: centre and neighbours get the coarse correction
 // (first colour after prolongation); 2: only the centre gets it (second
 // colour: its neighbours are first-colour cells, already corrected).
+// 3 and 4: the same two half-sweeps for a zero initial guess: the values are
+// taken as 0 instead of read (3: centre and neighbours, 4: centre only), so
+// the error array needs no clearing between cycles.
 template <typename T, int DIM, bool FINE, int PROL>
 __device__ __forceinline__ void stencil(const T *__restrict__ e, const T *__restrict__ rho,
                                         const T *__restrict__ bx, const T *__restrict__ by,
@@ -140,12 +143,16 @@ __device__ __forceinline__ void stencil(const T *__restrict__ e, const T *__rest
         }
     }
     auto nb = [&](long o, int ii, int jj, int kk) -> T {
+        if (PROL == 3) return (T)0;
         T v = e[id + o];
         if (PROL == 1) v += P.at(ii, jj, kk);
         return v;
     };
-    ep = e[id];
-    if (PROL != 0) ep += P.at(i, j, k);
+    if (PROL >= 3) ep = (T)0;
+    else {
+        ep = e[id];
+        if (PROL != 0) ep += P.at(i, j, k);
+    }
     sum_c = ce_ + cw + cn + cs;
     sum_ce = ce_ * (nb(1, i + 1, j, k) - ep) + cw * (nb(-1, i - 1, j, k) - ep) + cn * (nb(L.sx, i, j + 1, k) - ep) +
              cs * (nb(-L.sx, i, j - 1, k) - ep);
@@ -842,11 +849,12 @@ int mg_setup(fcfd_ctx *c) {
 // and rhs f_l of precision T.
 template <typename T, int DIM>
 static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1, int nu2,
-                  const Dots<T> *dots = nullptr) {
+                  const Dots<T> *dots = nullptr, bool zero_init = false) {
     const Geom &G = c->G;
     dim3 blk(64, 4);
     const int nl = (int)c->mg.size();
-    if (dots && nu2 < 1) return FCFD_ERR_ARG;   // the dots come from the last post-smoothing sweep
+    if (dots && nu2 < 1) return FCFD_ERR_ARG;        // the dots come from the last post-smoothing sweep
+    if (zero_init && nu1 < 1) return FCFD_ERR_ARG;   // the first pre-sweep stands in for clearing e
     auto arrays = [&](int lv, T *&e, const T *&f, const T *&bx, const T *&by, const T *&bz) {
         MgLevel &M = c->mg[lv];
         if (lv == 0) { e = e0; f = f0; bx = by = bz = nullptr; return; }
@@ -858,7 +866,8 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
     };
     // sweeps of RB-GS; with prol the coarse correction is added by the first
     // sweep, with dots the last half-sweep of level 0 forms the CG dots
-    auto smooth = [&](int lv, int sweeps, bool reverse, const Prol<T> *prol, const Dots<T> *dt) -> int {
+    auto smooth = [&](int lv, int sweeps, bool reverse, const Prol<T> *prol, const Dots<T> *dt,
+                      bool zinit) -> int {
         MgLevel &M = c->mg[lv];
         Lay L = lv == 0 ? lay_of(G) : lay_of(M, G.dim);
         Inv2 ih = inv2(M.hx, M.hy, M.hz);
@@ -871,7 +880,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
             for (int h = 0; h < 2; ++h) {
                 ProfScope ps_(c, lv == 0 ? R_SMOOTH0 : R_SMOOTHC);
                 const int color = reverse ? 1 - h : h;
-                const int pm = (prol && s == 0) ? 1 + h : 0;
+                const int pm = (zinit && s == 0) ? 3 + h : ((prol && s == 0) ? 1 + h : 0);
                 const bool dm = dt && lv == 0 && s == sweeps - 1 && h == 1;
 #define FCFD_SMOOTH(FINE, PM, DM)                                                                              \
     hipLaunchKernelGGL((k_smooth<T, DIM, FINE, PM, DM>), g, blk, 0, c->stream, e, f, FINE ? rho0 : (const T *)nullptr, \
@@ -882,6 +891,8 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
                         else FCFD_SMOOTH(true, 0, true);
                     } else if (pm == 1) FCFD_SMOOTH(true, 1, false);
                     else if (pm == 2) FCFD_SMOOTH(true, 2, false);
+                    else if (pm == 3) FCFD_SMOOTH(true, 3, false);
+                    else if (pm == 4) FCFD_SMOOTH(true, 4, false);
                     else FCFD_SMOOTH(true, 0, false);
                 } else {
                     if (pm == 1) FCFD_SMOOTH(false, 1, false);
@@ -893,9 +904,19 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         FCFD_CHECK(hipGetLastError());
         return FCFD_OK;
     };
+    // The coarse-grid correction of this aggregation hierarchy is its weak
+    // part: levels of <= 1/32 of the finest cells get more sweeps (measured at
+    // 256^3: 6 sweeps there cut flexible-CG iterations from 12 to 10 and
+    // plain V-cycles from 30 to 19; the launch-bound small levels cost little).
+    const long n0 = (long)c->mg[0].nx * c->mg[0].ny * c->mg[0].nz;
+    const int nuc = c->pr.ppe_nu_coarse > 0 ? c->pr.ppe_nu_coarse : 6;
+    auto sweeps = [&](int lv, int nu) {
+        const MgLevel &M = c->mg[lv];
+        return 32L * M.nx * M.ny * M.nz <= n0 ? nuc : nu;
+    };
     // down
     for (int lv = l; lv < nl - 1; ++lv) {
-        FCFD_TRY(smooth(lv, nu1, false, nullptr, nullptr));
+        FCFD_TRY(smooth(lv, sweeps(lv, nu1), false, nullptr, nullptr, zero_init && lv == 0));
         MgLevel &M = c->mg[lv], &C = c->mg[lv + 1];
         Lay L = lv == 0 ? lay_of(G) : lay_of(M, G.dim), LC = lay_of(C, G.dim);
         Inv2 ih = inv2(M.hx, M.hy, M.hz);
@@ -942,7 +963,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         if (nu2 > 0) {
             // prolongation folded into the first post-smoothing sweep
             const Prol<T> P{(const T *)ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1};
-            FCFD_TRY(smooth(lv, nu2, true, &P, lv == 0 ? dots : nullptr));
+            FCFD_TRY(smooth(lv, sweeps(lv, nu2), true, &P, lv == 0 ? dots : nullptr, false));
         } else {
             ProfScope ps_(c, lv == 0 ? R_PROLONG0 : R_PROLONGC);
             hipLaunchKernelGGL((k_prolong<T>), grd3(M.nx, M.ny, M.nz, blk), blk, 0, c->stream, e, (const T *)ec, L,
@@ -956,13 +977,13 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
 // ------------------------------------------------------ outer iteration
 
 // FP64 residual r = f - A p of level 0 with ||r||^2 and (with_f) ||f||^2
-// folded into block partials; optional outputs: r64 (Krylov residual),
-// r32 + zeroed e32 (rhs/error of the FP32 cycle), zeroed z64 (FP64 cycle).
+// folded into block partials; optional outputs: r64 (Krylov residual), r32
+// (rhs of the FP32 cycle).  The cycles' error arrays need no clearing: the
+// first pre-smoothing half-sweeps take a zero initial guess.
 template <int DIM>
 __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p, const double *__restrict__ f,
                                                    const double *__restrict__ rho, Lay L, Inv2 ih,
                                                    double *__restrict__ r64, float *__restrict__ r32,
-                                                   float *__restrict__ e32, double *__restrict__ z64,
                                                    double *__restrict__ partial, int with_f) {
     __shared__ double sh[4];
     int bx_, by_, bz_;
@@ -975,8 +996,7 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
         rr = r * r;
         if (with_f) ff = f[id] * f[id];
         if (r64) r64[id] = r;
-        if (r32) { r32[id] = (float)r; e32[id] = 0.0f; }
-        if (z64) z64[id] = 0.0;
+        if (r32) r32[id] = (float)r;
     }
     rr = block_sum256(rr, sh);
     ff = block_sum256(ff, sh);
@@ -989,13 +1009,16 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
 }
 
 // new direction: p = z + beta p, q = A z + beta q (so q = A p), z_old = z,
-// partial sums of p.q
+// partial sums of p.q.  The previous iteration's x += alpha p is applied
+// here, where the old p is read anyway (alpha = 0 on the first iteration;
+// the last one is applied by k_fcg_finish).
 template <typename T, int DIM>
 __global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__restrict__ zo,
                                                  double *__restrict__ p, double *__restrict__ q,
-                                                 const double *__restrict__ rho, Lay L, Inv2 ih,
-                                                 const double *__restrict__ sc, double *__restrict__ partial) {
-    const double beta = sc[SC_BETA];
+                                                 double *__restrict__ x, const double *__restrict__ rho, Lay L,
+                                                 Inv2 ih, const double *__restrict__ sc,
+                                                 double *__restrict__ partial) {
+    const double beta = sc[SC_BETA], alpha_prev = sc[SC_ALPHA];
     __shared__ double sh[4];
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
@@ -1014,7 +1037,9 @@ __global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__r
             acc += face<double>(k < L.nz - 1, r0, rho[id + L.sxy], ih.z) * ((double)z[id + L.sxy] - zp);
             acc += face<double>(k > 0, r0, rho[id - L.sxy], ih.z) * ((double)z[id - L.sxy] - zp);
         }
-        double pn = zp + beta * p[id];
+        const double po = p[id];
+        x[id] += alpha_prev * po;
+        double pn = zp + beta * po;
         double qn = acc + beta * q[id];
         p[id] = pn;
         q[id] = qn;
@@ -1026,12 +1051,10 @@ __global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__r
         partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = pq;
 }
 
-// x += alpha p, r -= alpha q, ||r||^2; the new residual also becomes the
-// rhs of the next preconditioning cycle (r32 / zeroed e32, or zeroed z64)
-__global__ void __launch_bounds__(256) k_fcg_update(double *__restrict__ x, const double *__restrict__ p,
-                                                    const double *__restrict__ q, double *__restrict__ r, Lay L,
+// r -= alpha q, ||r||^2; the new residual also becomes the rhs of the next
+// preconditioning cycle (r32 for the FP32 cycle; the FP64 cycle reads r)
+__global__ void __launch_bounds__(256) k_fcg_update(const double *__restrict__ q, double *__restrict__ r, Lay L,
                                                     const double *__restrict__ sc, float *__restrict__ r32,
-                                                    float *__restrict__ e32, double *__restrict__ z64,
                                                     double *__restrict__ partial) {
     const double alpha = sc[SC_ALPHA];
     __shared__ double sh[4];
@@ -1041,16 +1064,25 @@ __global__ void __launch_bounds__(256) k_fcg_update(double *__restrict__ x, cons
     double rr = 0.0;
     if (i < L.nx && j < L.ny) {
         long id = L.id(i, j, k);
-        x[id] += alpha * p[id];
         double rn = r[id] - alpha * q[id];
         r[id] = rn;
         rr = rn * rn;
-        if (r32) { r32[id] = (float)rn; e32[id] = 0.0f; }
-        if (z64) z64[id] = 0.0;
+        if (r32) r32[id] = (float)rn;
     }
     rr = block_sum256(rr, sh);
     if (threadIdx.x + threadIdx.y == 0)
         partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = rr;
+}
+
+// the last iteration's x += alpha p
+__global__ void __launch_bounds__(256) k_fcg_finish(double *__restrict__ x, const double *__restrict__ p, Lay L,
+                                                    const double *__restrict__ sc) {
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
+    if (i >= L.nx || j >= L.ny) return;
+    const long id = L.id(i, j, k);
+    x[id] += sc[SC_ALPHA] * p[id];
 }
 
 // out[0..ns) = sums of the ns slots of the block partials of a level-0
@@ -1118,7 +1150,7 @@ static int vcycles(fcfd_ctx *c, int kind, int inner) {
     const int nu1 = c->pr.ppe_nu1 > 0 ? c->pr.ppe_nu1 : 2, nu2 = c->pr.ppe_nu2 > 0 ? c->pr.ppe_nu2 : 2;
     for (int t = 0; t < inner; ++t) {
         if (kind == 0) FCFD_TRY((vcycle<double, DIM>(c, 0, c->f[FCFD_P], M0.f64, c->f[FCFD_RHO], nu1, nu2)));
-        else FCFD_TRY((vcycle<float, DIM>(c, 0, M0.e32, M0.f32, c->rho32, nu1, nu2)));
+        else FCFD_TRY((vcycle<float, DIM>(c, 0, M0.e32, M0.f32, c->rho32, nu1, nu2, nullptr, t == 0)));
     }
     return FCFD_OK;
 }
@@ -1144,26 +1176,26 @@ static int fcg_iteration(fcfd_ctx *c, int kind, int inner) {
     for (int t = 0; t < inner; ++t) {
         const bool last = t == inner - 1;
         if (mixed)
-            FCFD_TRY((vcycle<float, DIM>(c, 0, M0.e32, M0.f32, c->rho32, nu1, nu2, last ? &d32 : nullptr)));
+            FCFD_TRY((vcycle<float, DIM>(c, 0, M0.e32, M0.f32, c->rho32, nu1, nu2, last ? &d32 : nullptr, t == 0)));
         else
-            FCFD_TRY((vcycle<double, DIM>(c, 0, c->kz, c->kr, rho, nu1, nu2, last ? &d64 : nullptr)));
+            FCFD_TRY((vcycle<double, DIM>(c, 0, c->kz, c->kr, rho, nu1, nu2, last ? &d64 : nullptr, t == 0)));
     }
     FCFD_TRY(sum_partials(c, nb_s, 2, sc + SC_RZ, SUM_CG_BETA));
     {
         ProfScope ps_(c, R_FCG);
         if (mixed)
-            hipLaunchKernelGGL((k_fcg_dir<float, DIM>), g, blk, 0, c->stream, M0.e32, c->kzo32, c->kp, c->kq, rho, L0,
-                               ih, sc, c->red.partial);
+            hipLaunchKernelGGL((k_fcg_dir<float, DIM>), g, blk, 0, c->stream, M0.e32, c->kzo32, c->kp, c->kq,
+                               c->f[FCFD_P], rho, L0, ih, sc, c->red.partial);
         else
-            hipLaunchKernelGGL((k_fcg_dir<double, DIM>), g, blk, 0, c->stream, c->kz, c->kzo, c->kp, c->kq, rho, L0,
-                               ih, sc, c->red.partial);
+            hipLaunchKernelGGL((k_fcg_dir<double, DIM>), g, blk, 0, c->stream, c->kz, c->kzo, c->kp, c->kq,
+                               c->f[FCFD_P], rho, L0, ih, sc, c->red.partial);
         FCFD_CHECK(hipGetLastError());
     }
     FCFD_TRY(sum_partials(c, nb, 1, sc + SC_PQ, SUM_CG_ALPHA));
     {
         ProfScope ps_(c, R_FCG);
-        hipLaunchKernelGGL(k_fcg_update, g, blk, 0, c->stream, c->f[FCFD_P], c->kp, c->kq, c->kr, L0, sc,
-                           mixed ? M0.f32 : nullptr, M0.e32, mixed ? nullptr : c->kz, c->red.partial);
+        hipLaunchKernelGGL(k_fcg_update, g, blk, 0, c->stream, c->kq, c->kr, L0, sc, mixed ? M0.f32 : nullptr,
+                           c->red.partial);
         FCFD_CHECK(hipGetLastError());
     }
     FCFD_TRY(sum_partials(c, nb, 1, sc + SC_RR, SUM_STORE));
@@ -1196,7 +1228,7 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     {
         ProfScope ps_(c, R_RESID0);
         hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, f, rho, L0, ih, krylov ? c->kr : nullptr,
-                           mixed ? M0.f32 : nullptr, M0.e32, (krylov && !mixed) ? c->kz : nullptr, c->red.partial, 1);
+                           mixed ? M0.f32 : nullptr, c->red.partial, 1);
         FCFD_CHECK(hipGetLastError());
     }
     double s2[2];
@@ -1218,7 +1250,7 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
             {
                 ProfScope ps_(c, R_RESID0);
                 hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, f, rho, L0, ih, nullptr,
-                                   mixed ? M0.f32 : nullptr, M0.e32, nullptr, c->red.partial, 0);
+                                   mixed ? M0.f32 : nullptr, c->red.partial, 0);
                 FCFD_CHECK(hipGetLastError());
             }
             FCFD_TRY(sums_to_host(c, nb, 1, s2));
@@ -1242,6 +1274,10 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
         // stagnation: the FP64 residual has reached its round-off floor
         if (r < 0.9 * best) { best = r; since_best = 0; }
         else if (++since_best >= 4) break;
+    }
+    if (krylov && k > 0) {
+        hipLaunchKernelGGL(k_fcg_finish, g, blk, 0, c->stream, p, c->kp, L0, c->mg_scalars);
+        FCFD_CHECK(hipGetLastError());
     }
     if (r > rel_tol) rc = (r <= 10.0 * rel_tol && since_best >= 4) ? FCFD_OK : FCFD_ERR_NOT_CONVERGED;
     // mean(p) = 0 as the reference's iteration enforces, then ghost copy

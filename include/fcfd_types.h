@@ -67,6 +67,9 @@ typedef struct fcfd_params {
     int32_t ppe_inner;           /* V-cycles per outer iteration (>= 1)        */
     int32_t ls_scheme;           /* FCFD_LS_WENO5 or FCFD_LS_UCCD              */
     int32_t ppe_nu1, ppe_nu2;    /* pre/post RB-GS sweeps per level (0: 2, 2)  */
+    int32_t ppe_nu_coarse;       /* pre/post sweeps on levels of <= 1/32 of the
+                                    finest cells (0: 6)                         */
+    int32_t pad_;
 } fcfd_params;
 
 /* Level-set advection source of level_set_rk3_solver: the reference's WENO5
