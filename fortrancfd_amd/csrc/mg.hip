@@ -675,6 +675,123 @@ __global__ void __launch_bounds__(CDENSE) k_coarse_dense(T *__restrict__ eout, c
     if (t < n) eout[L.id(i, j, k)] = (T)(x - xm);
 }
 
+// Small levels in one workgroup: every level of at most TAIL_MAX cells, down
+// to the dense coarsest solve and back up, runs inside a single launch with
+// barriers between the phases.  On MI355X a dependent launch costs ~4 us
+// however small the grid, and these levels take up to 12 sweeps each; a phase
+// with one cell per thread costs ~1-2 us.  (With 8 cells per thread, i.e. an
+// 8192-cell level inside the workgroup, the dependent L2 loads made the tail
+// slower than separate launches; measured at 256^3.)
+constexpr int TAIL_MAX = 1024, TAIL_THREADS = 1024, TAIL_LEVELS = 8;
+
+template <typename T>
+struct TailLevel {
+    T *e;
+    T *f;
+    const T *bx, *by, *bz;
+    Lay L;
+    Inv2 ih;
+    int cx, cy, cz;   // coarsening factors to the next level
+    int nu1, nu2;
+};
+
+template <typename T>
+struct TailArgs {
+    TailLevel<T> lv[TAIL_LEVELS];
+    int n;                 // levels in the tail; the last is solved densely
+    const double *minv;    // dense inverse of the last level (k_coarse_inv)
+};
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
+    __shared__ double red[TAIL_THREADS / WAVE];
+    __shared__ double bvec[CDENSE];
+    const int t = threadIdx.x;
+    auto ncells = [](const Lay &L) { return L.nx * L.ny * L.nz; };
+    auto gs = [&](const TailLevel<T> &V, int color) {
+        const Lay &L = V.L;
+        const int n = ncells(L);
+        const Prol<T> P0{nullptr, Lay{}, 1, 1, 1};
+        for (int c = t; c < n; c += TAIL_THREADS) {
+            int i, j, k;
+            cell_coords(L, c, i, j, k);
+            if (((i + j + k) & 1) != color) continue;
+            const long id = L.id(i, j, k);
+            T sc, sce, ep;
+            stencil<T, DIM, false, 0>(V.e, nullptr, V.bx, V.by, V.bz, L, V.ih, i, j, k, id, P0, sc, sce, ep);
+            V.e[id] = sc > (T)0 ? ep + gs_step<T>(sce - V.f[id], sc) : ep;
+        }
+        __syncthreads();
+    };
+    // fixed-order sum over the block: wave butterflies, then the waves in order
+    auto block_sum = [&](double v) {
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+        if ((t & (WAVE - 1)) == 0) red[t / WAVE] = v;
+        __syncthreads();
+        double s = 0.0;
+        for (int w = 0; w < TAIL_THREADS / WAVE; ++w) s += red[w];
+        __syncthreads();
+        return s;
+    };
+    const int nl = A.n;
+    // down: smooth, restrict the residual, clear the coarse error
+    for (int a = 0; a < nl - 1; ++a) {
+        const TailLevel<T> &F = A.lv[a];
+        const TailLevel<T> &C = A.lv[a + 1];
+        for (int s = 0; s < F.nu1; ++s) { gs(F, 0); gs(F, 1); }
+        const int nc = ncells(C.L), nch = F.cx * F.cy * F.cz;
+        for (int cc = t; cc < nc; cc += TAIL_THREADS) {
+            int I, J, K;
+            cell_coords(C.L, cc, I, J, K);
+            T acc = 0;
+            for (int kk = 0; kk < F.cz; ++kk)
+                for (int jj = 0; jj < F.cy; ++jj)
+                    for (int ii = 0; ii < F.cx; ++ii) {
+                        const int i = I * F.cx + ii, j = J * F.cy + jj, k = K * F.cz + kk;
+                        const long id = F.L.id(i, j, k);
+                        acc += F.f[id] - apply_A<T, DIM, false>(F.e, nullptr, F.bx, F.by, F.bz, F.L, F.ih, i, j, k, id);
+                    }
+            const long cid = C.L.id(I, J, K);
+            C.f[cid] = acc / (T)nch;
+            C.e[cid] = (T)0;
+        }
+        __syncthreads();
+    }
+    // coarsest: e = M^-1 b, b = -(f - mean f), zero-mean result
+    {
+        const TailLevel<T> &V = A.lv[nl - 1];
+        const int n = ncells(V.L);
+        int i = 0, j = 0, k = 0;
+        double fv = 0.0;
+        if (t < n) {
+            cell_coords(V.L, t, i, j, k);
+            fv = (double)V.f[V.L.id(i, j, k)];
+        }
+        const double fm = block_sum(fv) / (double)n;
+        if (t < n) bvec[t] = fm - fv;
+        __syncthreads();
+        double x = 0.0;
+        if (t < n)
+            for (int cc = 0; cc < n; ++cc) x += A.minv[cc * n + t] * bvec[cc];
+        const double xm = block_sum(t < n ? x : 0.0) / (double)n;
+        if (t < n) V.e[V.L.id(i, j, k)] = (T)(x - xm);
+        __syncthreads();
+    }
+    // up: prolong (piecewise constant), smooth
+    for (int a = nl - 2; a >= 0; --a) {
+        const TailLevel<T> &F = A.lv[a];
+        const TailLevel<T> &C = A.lv[a + 1];
+        const int n = ncells(F.L);
+        for (int c = t; c < n; c += TAIL_THREADS) {
+            int i, j, k;
+            cell_coords(F.L, c, i, j, k);
+            F.e[F.L.id(i, j, k)] += C.e[C.L.id(i / F.cx, j / F.cy, k / F.cz)];
+        }
+        __syncthreads();
+        for (int s = 0; s < F.nu2; ++s) { gs(F, 1); gs(F, 0); }
+    }
+}
+
 // ------------------------------------------------------------ host side
 
 static dim3 grd3(int nx, int ny, int nz, dim3 blk) {
@@ -914,8 +1031,17 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         const MgLevel &M = c->mg[lv];
         return 32L * M.nx * M.ny * M.nz <= n0 ? nuc : nu;
     };
+    // levels [lt, nl) run inside one single-workgroup launch (k_mg_tail) when
+    // they are all small and the coarsest has a dense inverse; else lt = nl
+    int lt = nl;
+    if (nl > 1 && (long)c->mg[nl - 1].nx * c->mg[nl - 1].ny * c->mg[nl - 1].nz <= CDENSE) {
+        for (int lv = std::max(l, 1); lv < nl; ++lv)
+            if ((long)c->mg[lv].nx * c->mg[lv].ny * c->mg[lv].nz <= TAIL_MAX) { lt = lv; break; }
+        if (nl - lt > TAIL_LEVELS) lt = nl;
+    }
+    const int ldown = lt < nl ? lt : nl - 1;   // levels smoothed by separate launches: [l, ldown)
     // down
-    for (int lv = l; lv < nl - 1; ++lv) {
+    for (int lv = l; lv < ldown; ++lv) {
         FCFD_TRY(smooth(lv, sweeps(lv, nu1), false, nullptr, nullptr, zero_init && lv == 0));
         MgLevel &M = c->mg[lv], &C = c->mg[lv + 1];
         Lay L = lv == 0 ? lay_of(G) : lay_of(M, G.dim), LC = lay_of(C, G.dim);
@@ -934,8 +1060,28 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
                                by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
         FCFD_CHECK(hipGetLastError());
     }
-    // coarsest: FP64 CG
-    {
+    if (lt < nl) {
+        TailArgs<T> A{};
+        A.n = nl - lt;
+        A.minv = c->mg_scalars + 64;
+        for (int q = 0; q < A.n; ++q) {
+            const int lv = lt + q;
+            MgLevel &M = c->mg[lv];
+            TailLevel<T> &V = A.lv[q];
+            T *e; const T *f, *bx, *by, *bz;
+            arrays(lv, e, f, bx, by, bz);
+            V.e = e; V.f = (T *)f; V.bx = bx; V.by = by; V.bz = bz;
+            V.L = lay_of(M, G.dim);
+            V.ih = inv2(M.hx, M.hy, M.hz);
+            V.cx = M.cx; V.cy = M.cy; V.cz = DIM == 3 ? M.cz : 1;
+            V.nu1 = sweeps(lv, nu1);
+            V.nu2 = sweeps(lv, nu2);
+        }
+        ProfScope ps_(c, R_COARSE);
+        hipLaunchKernelGGL((k_mg_tail<T, DIM>), dim3(1), dim3(TAIL_THREADS), 0, c->stream, A);
+        FCFD_CHECK(hipGetLastError());
+    } else {
+        // coarsest: dense inverse or FP64 CG in one workgroup
         MgLevel &M = c->mg[nl - 1];
         Lay L = lay_of(M, G.dim);
         Inv2 ih = inv2(M.hx, M.hy, M.hz);
@@ -953,7 +1099,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         FCFD_CHECK(hipGetLastError());
     }
     // up
-    for (int lv = nl - 2; lv >= l; --lv) {
+    for (int lv = ldown - 1; lv >= l; --lv) {
         MgLevel &M = c->mg[lv], &C = c->mg[lv + 1];
         Lay L = lv == 0 ? lay_of(G) : lay_of(M, G.dim), LC = lay_of(C, G.dim);
         T *e; const T *f, *bx, *by, *bz;
