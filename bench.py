@@ -60,16 +60,18 @@ def region_bytes(case, mixed: bool) -> dict:
         "projection": 8 * N * 8,
         "node_vel": 8 * N * 6,
         "mpls_update": 8 * N * 3,
-        # per colour: R{e, f/2, rho} W{e/2}; averaged over the 8 launches of a
-        # V(2,2) cycle: + R{ec} (N/nchild0 coarse values) in the first post
-        # sweep (prolongation folded in) and + R{r (FP64), z_old} in the last
+        # per colour (colour-split FP32 arrays in the mixed cycle): R{e of the
+        # other colour (N/2), f of this colour (N/2), rho (N)} W{e of this
+        # colour (N/2)}; averaged over the 8 launches of a V(2,2) cycle:
+        # + R{ec} (N/nchild0 coarse values) in the first post-sweep half
+        # (prolongation folded in) and + R{r (FP64), z_old} in the last
         # half-sweep (CG dots folded in)
-        "mg_smooth_l0": N * (8 * b * 3 + 2 * b / nchild0 + (8 + b)) / 8,
+        "mg_smooth_l0": N * (8 * b * 2.5 + b / nchild0 + (8 + b)) / 8,
         "mg_restrict_l0": b * N * (3 + 2.0 / nchild0),   # R{e,f,rho} W{fc,ec}
         "mg_resid_l0": N * (3 * 8 + (2 * 4 if mixed else 0)),  # R{p,f,rho} (W{r32,e32})
         # flexible CG, averaged over its two launches per iteration:
-        # dir R{z, p, q, rho} W{p, q, z_old}; update R{x, p, q, r} W{x, r, r32, e32}
-        "fcg_vectors": N * ((8 * 5 + 2 * b) + (8 * 6 + (2 * 4 if mixed else 8))) / 2,
+        # dir R{z, p, q, x, rho} W{p, q, x, z_old}; update R{q, r} W{r, r32}
+        "fcg_vectors": N * ((8 * 7 + 2 * b) + (8 * 3 + (4 if mixed else 0))) / 2,
     }
 
 

@@ -58,6 +58,31 @@ static Lay lay_of(const MgLevel &M, int dim) {
     return L;
 }
 
+// Colour-split layout of the FP32 level-0 arrays of the mixed cycle.  Cell
+// (i,j,k) has colour c = (i+j+k) & 1 and lives in colour array c (the first
+// or second `half` elements) at slot m = i >> 1 of row (j,k); each colour
+// array has mh = (nx+1)/2 slots per row plus one ghost slot on every side.
+// All six neighbours of a cell have the other colour, so a red-black
+// half-sweep streams whole cache lines of exactly the arrays it needs.
+// Slots that map to no cell (ghosts, and i = nx for odd nx) stay zero.
+struct Split {
+    int nx, ny, nz, kz;   // cells; kz = 1 in 3D (ghost plane below k = 0), 0 in 2D
+    long sx, sxy, half;   // row and plane pitch of one colour array; its size
+    __host__ __device__ long row(int j, int k) const { return (long)(j + 1) * sx + (long)(k + kz) * sxy; }
+    __host__ __device__ long at(int i, int j, int k) const {
+        return ((i + j + k) & 1) * half + (long)((i >> 1) + 1) + row(j, k);
+    }
+};
+
+static Split split_of(const Geom &G) {
+    Split S;
+    S.nx = G.nx; S.ny = G.ny; S.nz = G.nz; S.kz = G.dim == 3 ? 1 : 0;
+    S.sx = (G.nx + 1) / 2 + 2;
+    S.sxy = S.sx * (G.ny + 2);
+    S.half = S.sxy * (G.dim == 3 ? G.nz + 2 : 1);
+    return S;
+}
+
 struct Inv2 {
     double x, y, z;  // 1/h^2 per axis
 };
@@ -101,10 +126,10 @@ template <typename T>
 struct Prol {
     const T *ec;
     Lay C;
-    int cx, cy, cz;
-    // parent of fine (i, j, k); i = -1 or nx maps to a coarse interior or
-    // ghost cell, always inside the coarse array (its coefficient is 0)
-    __device__ __forceinline__ T at(int i, int j, int k) const { return ec[C.id(i / cx, j / cy, k / cz)]; }
+    int lx, ly, lz;   // log2 of the coarsening factor (1 or 2) per axis
+    // parent of fine (i, j, k) by arithmetic shifts; i = -1 or nx maps to a
+    // coarse ghost cell (zero, and its face coefficient is 0)
+    __device__ __forceinline__ T at(int i, int j, int k) const { return ec[C.id(i >> lx, j >> ly, k >> lz)]; }
 };
 
 // Face coefficients of cell (i,j,k): sum_c = sum c_nb and
@@ -247,6 +272,138 @@ __global__ void __launch_bounds__(256) k_smooth(T *__restrict__ e, const T *__re
             D.partial[nb + b] = rzo;
         }
     }
+}
+
+// One colour of the red-black Gauss-Seidel sweep on the colour-split FP32
+// level 0 (the mixed cycle): e_P = (sum c e_nb - f) / sum c.  A half-sweep
+// reads the other colour's e, this colour's f and both colours' rho and
+// writes this colour's e: 2.5 values per cell, where an interleaved array
+// moves whole lines of e, f, rho and e again.  The centre's previous value
+// does not enter the update, so MODE 0: plain; 1: the neighbours carry the
+// coarse-grid correction e + ec(parent) (first half-sweep after
+// prolongation; the second colour reads first-colour cells already
+// updated); 2: zero initial guess, the neighbours are 0 (first pre-sweep
+// half, so the error array needs no clearing).  DOTS: the CG dots of the
+// thread's pair (2m, 2m+1) as in k_smooth.
+template <int DIM, int MODE, bool DOTS>
+__global__ void __launch_bounds__(256) k_smooth_split(float *__restrict__ e, const float *__restrict__ f,
+                                                      const float *__restrict__ rho, Split S, Inv2 ih, int color,
+                                                      Prol<float> P, Dots<float> D, Lay L) {
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int m = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
+    const int s = (j + k + color) & 1, i = 2 * m + s;
+    const long id = (long)(m + 1) + S.row(j, k);
+    const long cc = color ? S.half : 0, oc = S.half - cc;   // this colour, the other colour
+    const bool upd = i < S.nx && j < S.ny;
+    float enew = 0.f;
+    if (upd) {
+        const float *ro = rho + oc, *eo = e + oc;
+        const float r = rho[cc + id];
+        // the x neighbours i-1 and i+1 are slots m-1+s and m+s of the other colour
+        const long xw = id - 1 + s, xe = id + s;
+        const float cE = face<float>(i < S.nx - 1, r, ro[xe], (float)ih.x);
+        const float cW = face<float>(i > 0, r, ro[xw], (float)ih.x);
+        const float cN = face<float>(j < S.ny - 1, r, ro[id + S.sx], (float)ih.y);
+        const float cS = face<float>(j > 0, r, ro[id - S.sx], (float)ih.y);
+        float cT = 0.f, cB = 0.f;
+        if (DIM == 3) {
+            cT = face<float>(k < S.nz - 1, r, ro[id + S.sxy], (float)ih.z);
+            cB = face<float>(k > 0, r, ro[id - S.sxy], (float)ih.z);
+        }
+        auto nb = [&](long o, int ii, int jj, int kk) -> float {
+            float v = eo[o];
+            if (MODE == 1) v += P.at(ii, jj, kk);
+            return v;
+        };
+        float sc = cE + cW + cN + cS, sce = 0.f;
+        if (MODE != 2)
+            sce = cE * nb(xe, i + 1, j, k) + cW * nb(xw, i - 1, j, k) + cN * nb(id + S.sx, i, j + 1, k) +
+                  cS * nb(id - S.sx, i, j - 1, k);
+        if (DIM == 3) {
+            sc += cT + cB;
+            if (MODE != 2) sce += cT * nb(id + S.sxy, i, j, k + 1) + cB * nb(id - S.sxy, i, j, k - 1);
+        }
+        enew = sc > 0.f ? (sce - f[cc + id]) * __builtin_amdgcn_rcpf(sc) : 0.f;
+        e[cc + id] = enew;
+    }
+    if (DOTS) {
+        __shared__ double sh[4];
+        double rz = 0.0, rzo = 0.0;
+        if (upd) {
+            const double rv = D.r[L.id(i, j, k)];
+            rz = rv * (double)enew;
+            rzo = rv * (double)D.zo[cc + id];
+        }
+        const int ip = 2 * m + 1 - s;   // the pair's other cell: final from the previous half-sweep
+        if (ip < S.nx && j < S.ny) {
+            const double rv = D.r[L.id(ip, j, k)];
+            rz += rv * (double)e[oc + id];
+            rzo += rv * (double)D.zo[oc + id];
+        }
+        rz = block_sum256(rz, sh);
+        rzo = block_sum256(rzo, sh);
+        if (threadIdx.x + threadIdx.y == 0) {
+            const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+            const long nb = (long)gridDim.x * gridDim.y * gridDim.z;
+            D.partial[b] = rz;
+            D.partial[nb + b] = rzo;
+        }
+    }
+}
+
+// (A e)_P of the colour-split FP32 level 0, difference form as apply_A
+template <int DIM>
+__device__ __forceinline__ float apply_A_split(const float *__restrict__ e, const float *__restrict__ rho,
+                                               const Split &S, Inv2 ih, int i, int j, int k) {
+    const long a = S.at(i, j, k);
+    const float ep = e[a], r = rho[a];
+    long n = S.at(i + 1, j, k);
+    float acc = face<float>(i < S.nx - 1, r, rho[n], (float)ih.x) * (e[n] - ep);
+    n = S.at(i - 1, j, k);
+    acc += face<float>(i > 0, r, rho[n], (float)ih.x) * (e[n] - ep);
+    n = S.at(i, j + 1, k);
+    acc += face<float>(j < S.ny - 1, r, rho[n], (float)ih.y) * (e[n] - ep);
+    n = S.at(i, j - 1, k);
+    acc += face<float>(j > 0, r, rho[n], (float)ih.y) * (e[n] - ep);
+    if (DIM == 3) {
+        n = S.at(i, j, k + 1);
+        acc += face<float>(k < S.nz - 1, r, rho[n], (float)ih.z) * (e[n] - ep);
+        n = S.at(i, j, k - 1);
+        acc += face<float>(k > 0, r, rho[n], (float)ih.z) * (e[n] - ep);
+    }
+    return acc;
+}
+
+// k_resid_restrict for the colour-split FP32 level 0
+template <int DIM>
+__global__ void __launch_bounds__(256) k_resid_restrict_split(const float *__restrict__ e, const float *__restrict__ f,
+                                                              const float *__restrict__ rho, Split S, Inv2 ih,
+                                                              float *__restrict__ fc, float *__restrict__ ec, Lay C,
+                                                              int cx, int cy, int cz) {
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int I = bx_ * blockDim.x + threadIdx.x, J = by_ * blockDim.y + threadIdx.y, K = bz_;
+    if (I >= C.nx || J >= C.ny) return;
+    float acc = 0.f;
+    for (int kk = 0; kk < cz; ++kk)
+        for (int jj = 0; jj < cy; ++jj)
+            for (int ii = 0; ii < cx; ++ii) {
+                const int i = I * cx + ii, j = J * cy + jj, k = K * cz + kk;
+                acc += f[S.at(i, j, k)] - apply_A_split<DIM>(e, rho, S, ih, i, j, k);
+            }
+    const long cid = C.id(I, J, K);
+    fc[cid] = acc / (float)(cx * cy * cz);
+    ec[cid] = 0.f;
+}
+
+// FP32 density of level 0 in the colour-split layout
+__global__ void k_rho_split(const double *__restrict__ rho, Lay L, float *__restrict__ out, Split S) {
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
+    if (i >= L.nx || j >= L.ny) return;
+    out[S.at(i, j, k)] = (float)rho[L.id(i, j, k)];
 }
 
 // residual of the level, restricted to the coarse right-hand side by the
@@ -440,14 +597,13 @@ __global__ void k_center_copy(const double *__restrict__ f, double *__restrict__
     out[id] = f[id] - s[0] * scale;
 }
 
-// p += e (FP32 correction into the FP64 solution)
-__global__ void k_add_f32(double *__restrict__ p, const float *__restrict__ e, Lay L) {
+// p += e (FP32 colour-split correction into the FP64 solution)
+__global__ void k_add_f32(double *__restrict__ p, const float *__restrict__ e, Lay L, Split S) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= L.nx || j >= L.ny) return;
-    long id = L.id(i, j, k);
-    p[id] += (double)e[id];
+    p[L.id(i, j, k)] += (double)e[S.at(i, j, k)];
 }
 
 // Coarsest level: Jacobi-preconditioned CG in FP64 inside one workgroup.
@@ -583,6 +739,16 @@ __device__ __forceinline__ void cell_coords(const Lay &L, int c, int &i, int &j,
     k = r / L.ny;
 }
 
+// interior (i, j, k) of a ghosted index of a small one-ghost level layout
+// (32-bit arithmetic: a 64-bit division is a long software sequence)
+__device__ __forceinline__ void cell_coords_g(const Lay &L, int id, int &i, int &j, int &k) {
+    const int sxy = (int)L.sxy, sx = (int)L.sx;
+    const int kq = id / sxy, r = id - kq * sxy, jq = r / sx;
+    k = kq - L.gz;
+    j = jq - 1;
+    i = r - jq * sx - 1;
+}
+
 // In-place Gauss-Jordan inversion in LDS (no pivoting: M is SPD); one block
 // of 256 threads; writes M^-1 (symmetric) to minv[n*n].
 template <int DIM>
@@ -679,10 +845,18 @@ __global__ void __launch_bounds__(CDENSE) k_coarse_dense(T *__restrict__ eout, c
 // to the dense coarsest solve and back up, runs inside a single launch with
 // barriers between the phases.  On MI355X a dependent launch costs ~4 us
 // however small the grid, and these levels take up to 12 sweeps each; a phase
-// with one cell per thread costs ~1-2 us.  (With 8 cells per thread, i.e. an
-// 8192-cell level inside the workgroup, the dependent L2 loads made the tail
-// slower than separate launches; measured at 256^3.)
+// with one cell per thread costs ~1-2 us.  (With 8 cells per thread read from
+// global memory, an 8192-cell level inside the workgroup was slower than
+// separate launches: the sweeps wait on dependent L2 loads.)
+//
+// FP32 cycles may in addition take one fully coarsened "top" level of up to
+// TAIL_TOP cells (8192 = 8 per thread) whose error (TAIL_LDS floats with
+// padding) and rhs live in LDS and whose six face coefficients per cell live
+// in the registers of the thread that owns the cell; a thread owns whole
+// coarse cells with all their children, so the restriction needs no
+// exchange.  FP64 would need twice the registers (spills) and keeps TAIL_MAX.
 constexpr int TAIL_MAX = 1024, TAIL_THREADS = 1024, TAIL_LEVELS = 8;
+constexpr int TAIL_TOP = 8192, TAIL_LDS = 12288, TAIL_SLOTS = TAIL_TOP / TAIL_THREADS;
 
 template <typename T>
 struct TailLevel {
@@ -699,6 +873,7 @@ template <typename T>
 struct TailArgs {
     TailLevel<T> lv[TAIL_LEVELS];
     int n;                 // levels in the tail; the last is solved densely
+    int top;               // lv[0] is an LDS/register top level (FP32 only)
     const double *minv;    // dense inverse of the last level (k_coarse_inv)
 };
 
@@ -711,7 +886,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
     auto gs = [&](const TailLevel<T> &V, int color) {
         const Lay &L = V.L;
         const int n = ncells(L);
-        const Prol<T> P0{nullptr, Lay{}, 1, 1, 1};
+        const Prol<T> P0{nullptr, Lay{}, 0, 0, 0};
         for (int c = t; c < n; c += TAIL_THREADS) {
             int i, j, k;
             cell_coords(L, c, i, j, k);
@@ -734,8 +909,118 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
         return s;
     };
     const int nl = A.n;
+    // ---- top level in LDS + registers (FP32 only; A.top is never set for FP64).
+    // Fully coarsened (2x2x2, or 2x2 in 2D): thread t owns coarse cell(s)
+    // cc = t + TAIL_THREADS * g and slot sl = g * NCH + ch holds child ch =
+    // ii + 2 jj + 4 kk.  The error is stored child-major, es[ch * stride +
+    // pad + cc], so a wave's accesses to one child are consecutive words
+    // (no bank conflicts) and every neighbour sits at a uniform offset;
+    // `pad` zero words (one coarse plane) before and after each child block
+    // stand in for the missing neighbours below the first and above the last
+    // coarse plane (their face coefficients are 0).
+    constexpr int NCH = DIM == 3 ? 8 : 4, NG = TAIL_SLOTS / NCH;
+    float *es = nullptr, *fs = nullptr;   // LDS error (child-major) and rhs (slot-major)
+    float tc[TAIL_SLOTS][6];              // face coefficients E W N S T B of each slot
+    int a0 = 0, cnx = 0, plane = 0, stride = 0, nct = 0;
+    // index of slot sl's cell in es
+    auto top_li = [&](int sl) { return (sl % NCH) * stride + plane + t + TAIL_THREADS * (sl / NCH); };
+    // one colour of Gauss-Seidel on the top level; same operation order as stencil()
+    auto top_gs = [&](int color) {
+        // opaque copies of the offsets: keeps the compiler from holding six
+        // neighbour addresses per slot in registers across the sweeps
+        int st = stride, cx_ = cnx, pl = plane;
+        asm volatile("" : "+s"(st), "+s"(cx_), "+s"(pl));
+#pragma unroll
+        for (int sl = 0; sl < TAIL_SLOTS; ++sl) {
+            const int ch = sl % NCH, ii = ch & 1, jj = (ch >> 1) & 1, kk = ch >> 2;
+            if (((ii + jj + kk) & 1) != color || t + TAIL_THREADS * (sl / NCH) >= nct) continue;
+            const int li = (sl % NCH) * st + pl + t + TAIL_THREADS * (sl / NCH);
+            const int oxp = ii ? 1 - st : st, oxm = ii ? -st : st - 1;
+            const int oyp = jj ? cx_ - 2 * st : 2 * st, oym = jj ? -2 * st : 2 * st - cx_;
+            const float ep = es[li];
+            float sc = tc[sl][0] + tc[sl][1] + tc[sl][2] + tc[sl][3];
+            float sce = tc[sl][0] * (es[li + oxp] - ep) + tc[sl][1] * (es[li + oxm] - ep) +
+                        tc[sl][2] * (es[li + oyp] - ep) + tc[sl][3] * (es[li + oym] - ep);
+            if (DIM == 3) {
+                const int ozp = kk ? pl - 4 * st : 4 * st, ozm = kk ? -4 * st : 4 * st - pl;
+                sc += tc[sl][4] + tc[sl][5];
+                sce += tc[sl][4] * (es[li + ozp] - ep) + tc[sl][5] * (es[li + ozm] - ep);
+            }
+            es[li] = sc > 0.f ? ep + gs_step<float>(sce - fs[sl * TAIL_THREADS + t], sc) : ep;
+        }
+        __syncthreads();
+    };
+    // global (ghosted) index of slot sl's cell
+    auto top_gid = [&](const TailLevel<T> &F, const TailLevel<T> &C, int sl) -> long {
+        const int ch = sl % NCH;
+        int I, J, K;
+        cell_coords(C.L, t + TAIL_THREADS * (sl / NCH), I, J, K);
+        return F.L.id(2 * I + (ch & 1), 2 * J + ((ch >> 1) & 1), DIM == 3 ? 2 * K + (ch >> 2) : 0);
+    };
+    if constexpr (sizeof(T) == 4) {
+        __shared__ float es_[TAIL_LDS], fs_[TAIL_TOP];
+        es = es_;
+        fs = fs_;
+        if (A.top) {
+            a0 = 1;
+            const TailLevel<T> &F = A.lv[0];
+            const TailLevel<T> &C = A.lv[1];
+            const Lay &L = F.L;
+            cnx = C.L.nx;
+            plane = DIM == 3 ? C.L.nx * C.L.ny : C.L.nx;
+            nct = ncells(C.L);
+            stride = nct + 2 * plane;
+            for (int q = t; q < NCH * stride; q += TAIL_THREADS) es[q] = 0.f;   // zero error and pads
+#pragma unroll
+            for (int sl = 0; sl < TAIL_SLOTS; ++sl) {
+                if (t + TAIL_THREADS * (sl / NCH) < nct) {
+                    const long id = top_gid(F, C, sl);
+                    tc[sl][0] = F.bx[id] * (float)F.ih.x;
+                    tc[sl][1] = F.bx[id - 1] * (float)F.ih.x;
+                    tc[sl][2] = F.by[id] * (float)F.ih.y;
+                    tc[sl][3] = F.by[id - L.sx] * (float)F.ih.y;
+                    tc[sl][4] = DIM == 3 ? F.bz[id] * (float)F.ih.z : 0.f;
+                    tc[sl][5] = DIM == 3 ? F.bz[id - L.sxy] * (float)F.ih.z : 0.f;
+                    fs[sl * TAIL_THREADS + t] = F.f[id];
+                } else {
+                    for (int q = 0; q < 6; ++q) tc[sl][q] = 0.f;
+                }
+            }
+            __syncthreads();
+            for (int sw = 0; sw < F.nu1; ++sw) { top_gs(0); top_gs(1); }
+            // restriction: a coarse cell's children are its NCH slots, summed in
+            // k_resid_restrict's order (ii fastest); A e in apply_A's order
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const int cc = t + TAIL_THREADS * g;
+                if (cc >= nct) continue;
+                float acc = 0.f;
+#pragma unroll
+                for (int ch = 0; ch < NCH; ++ch) {
+                    const int sl = g * NCH + ch, ii = ch & 1, jj = (ch >> 1) & 1, kk = ch >> 2;
+                    const int li = top_li(sl);
+                    const int oxp = ii ? 1 - stride : stride, oxm = ii ? -stride : stride - 1;
+                    const int oyp = jj ? cnx - 2 * stride : 2 * stride, oym = jj ? -2 * stride : 2 * stride - cnx;
+                    const float ep = es[li];
+                    float ae = tc[sl][0] * (es[li + oxp] - ep) + tc[sl][1] * (es[li + oxm] - ep) +
+                               tc[sl][2] * (es[li + oyp] - ep) + tc[sl][3] * (es[li + oym] - ep);
+                    if (DIM == 3) {
+                        const int ozp = kk ? plane - 4 * stride : 4 * stride, ozm = kk ? -4 * stride : 4 * stride - plane;
+                        ae += tc[sl][4] * (es[li + ozp] - ep) + tc[sl][5] * (es[li + ozm] - ep);
+                    }
+                    acc += fs[sl * TAIL_THREADS + t] - ae;
+                }
+                int I, J, K;
+                cell_coords(C.L, cc, I, J, K);
+                const long cid = C.L.id(I, J, K);
+                C.f[cid] = acc / (float)NCH;
+                C.e[cid] = 0.f;
+            }
+            __syncthreads();
+        }
+    }
     // down: smooth, restrict the residual, clear the coarse error
-    for (int a = 0; a < nl - 1; ++a) {
+    for (int a = a0; a < nl - 1; ++a) {
         const TailLevel<T> &F = A.lv[a];
         const TailLevel<T> &C = A.lv[a + 1];
         for (int s = 0; s < F.nu1; ++s) { gs(F, 0); gs(F, 1); }
@@ -778,7 +1063,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
         __syncthreads();
     }
     // up: prolong (piecewise constant), smooth
-    for (int a = nl - 2; a >= 0; --a) {
+    for (int a = nl - 2; a >= a0; --a) {
         const TailLevel<T> &F = A.lv[a];
         const TailLevel<T> &C = A.lv[a + 1];
         const int n = ncells(F.L);
@@ -789,6 +1074,27 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
         }
         __syncthreads();
         for (int s = 0; s < F.nu2; ++s) { gs(F, 1); gs(F, 0); }
+    }
+    if constexpr (sizeof(T) == 4) {
+        if (A.top) {
+            const TailLevel<T> &F = A.lv[0];
+            const TailLevel<T> &C = A.lv[1];
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const int cc = t + TAIL_THREADS * g;
+                if (cc >= nct) continue;
+                int I, J, K;
+                cell_coords(C.L, cc, I, J, K);
+                const float ec = C.e[C.L.id(I, J, K)];
+#pragma unroll
+                for (int ch = 0; ch < NCH; ++ch) es[top_li(g * NCH + ch)] += ec;
+            }
+            __syncthreads();
+            for (int sw = 0; sw < F.nu2; ++sw) { top_gs(1); top_gs(0); }
+#pragma unroll
+            for (int sl = 0; sl < TAIL_SLOTS; ++sl)
+                if (t + TAIL_THREADS * (sl / NCH) < nct) F.e[top_gid(F, C, sl)] = es[top_li(sl)];
+        }
     }
 }
 
@@ -845,7 +1151,7 @@ static int alloc0(T **p, long n) {
 
 // Build the level hierarchy once per grid (sizes only).  Level 0 describes
 // the field layout; its arrays are the context's fields (FP64) and, for the
-// mixed mode, e32/f32/rho32 in the same ghosted layout.
+// mixed mode, e32/f32/rho32 (and the FP32 z_old) in the colour-split layout.
 static int mg_build(fcfd_ctx *c) {
     const Geom &G = c->G;
     mg_free(c);
@@ -893,16 +1199,17 @@ static int mg_build(fcfd_ctx *c) {
         if (G.dim == 3) FCFD_TRY(alloc0(&M.bz32, n));
     }
     MgLevel &M0 = c->mg[0];
-    FCFD_TRY(alloc0(&M0.e32, G.ntot));
-    FCFD_TRY(alloc0(&M0.f32, G.ntot));
-    FCFD_TRY(alloc0(&c->rho32, G.ntot));
+    const long nsplit = 2 * split_of(G).half;
+    FCFD_TRY(alloc0(&M0.e32, nsplit));
+    FCFD_TRY(alloc0(&M0.f32, nsplit));
+    FCFD_TRY(alloc0(&c->rho32, nsplit));
     FCFD_TRY(alloc0(&M0.f64, G.ntot));   // zero-mean working rhs
     FCFD_TRY(alloc0(&c->kr, G.ntot));
     FCFD_TRY(alloc0(&c->kp, G.ntot));
     FCFD_TRY(alloc0(&c->kq, G.ntot));
     FCFD_TRY(alloc0(&c->kz, G.ntot));
     FCFD_TRY(alloc0(&c->kzo, G.ntot));
-    FCFD_TRY(alloc0(&c->kzo32, G.ntot));
+    FCFD_TRY(alloc0(&c->kzo32, nsplit));
     // scalars + coarse CG workspace (6 vectors) or the dense coarse inverse
     long ncoarse = (long)c->mg.back().nx * c->mg.back().ny * c->mg.back().nz;
     FCFD_TRY(alloc0(&c->mg_scalars, 64 + std::max(6 * ncoarse, ncoarse * ncoarse)));
@@ -940,8 +1247,8 @@ int mg_setup(fcfd_ctx *c) {
         hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, c->stream, C.by64, C.by32, n);
         if (G.dim == 3) hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, c->stream, C.bz64, C.bz32, n);
     }
-    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((G.ntot + 255) / 256)), dim3(256), 0, c->stream, c->f[FCFD_RHO],
-                       c->rho32, G.ntot);
+    hipLaunchKernelGGL(k_rho_split, grd3(G.nx, G.ny, G.nz, blk), blk, 0, c->stream, c->f[FCFD_RHO], lay_of(G),
+                       c->rho32, split_of(G));
     FCFD_CHECK(hipGetLastError());
     {
         MgLevel &M = c->mg.back();
@@ -991,7 +1298,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         T *e; const T *f, *bx, *by, *bz;
         arrays(lv, e, f, bx, by, bz);
         dim3 g((unsigned)(((M.nx + 1) / 2 + blk.x - 1) / blk.x), (M.ny + blk.y - 1) / blk.y, M.nz);
-        const Prol<T> P0 = prol ? *prol : Prol<T>{nullptr, Lay{}, 1, 1, 1};
+        const Prol<T> P0 = prol ? *prol : Prol<T>{nullptr, Lay{}, 0, 0, 0};
         const Dots<T> D0 = dt ? *dt : Dots<T>{nullptr, nullptr, nullptr};
         for (int s = 0; s < sweeps; ++s)
             for (int h = 0; h < 2; ++h) {
@@ -1003,14 +1310,27 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
     hipLaunchKernelGGL((k_smooth<T, DIM, FINE, PM, DM>), g, blk, 0, c->stream, e, f, FINE ? rho0 : (const T *)nullptr, \
                        bx, by, bz, L, ih, color, P0, D0)
                 if (lv == 0) {
-                    if (dm) {
-                        if (pm == 2) FCFD_SMOOTH(true, 2, true);
-                        else FCFD_SMOOTH(true, 0, true);
-                    } else if (pm == 1) FCFD_SMOOTH(true, 1, false);
-                    else if (pm == 2) FCFD_SMOOTH(true, 2, false);
-                    else if (pm == 3) FCFD_SMOOTH(true, 3, false);
-                    else if (pm == 4) FCFD_SMOOTH(true, 4, false);
-                    else FCFD_SMOOTH(true, 0, false);
+                    if constexpr (sizeof(T) == 4) {
+                        // colour-split FP32 level 0: the second half of a
+                        // prolongation or zero-guess sweep is a plain one
+                        const Split S = split_of(G);
+#define FCFD_SPLIT(MD, DM)                                                                                     \
+    hipLaunchKernelGGL((k_smooth_split<DIM, MD, DM>), g, blk, 0, c->stream, e, f, rho0, S, ih, color, P0, D0, L)
+                        if (dm) FCFD_SPLIT(0, true);
+                        else if (pm == 1) FCFD_SPLIT(1, false);
+                        else if (pm == 3) FCFD_SPLIT(2, false);
+                        else FCFD_SPLIT(0, false);
+#undef FCFD_SPLIT
+                    } else {
+                        if (dm) {
+                            if (pm == 2) FCFD_SMOOTH(true, 2, true);
+                            else FCFD_SMOOTH(true, 0, true);
+                        } else if (pm == 1) FCFD_SMOOTH(true, 1, false);
+                        else if (pm == 2) FCFD_SMOOTH(true, 2, false);
+                        else if (pm == 3) FCFD_SMOOTH(true, 3, false);
+                        else if (pm == 4) FCFD_SMOOTH(true, 4, false);
+                        else FCFD_SMOOTH(true, 0, false);
+                    }
                 } else {
                     if (pm == 1) FCFD_SMOOTH(false, 1, false);
                     else if (pm == 2) FCFD_SMOOTH(false, 2, false);
@@ -1033,11 +1353,24 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
     };
     // levels [lt, nl) run inside one single-workgroup launch (k_mg_tail) when
     // they are all small and the coarsest has a dense inverse; else lt = nl
-    int lt = nl;
-    if (nl > 1 && (long)c->mg[nl - 1].nx * c->mg[nl - 1].ny * c->mg[nl - 1].nz <= CDENSE) {
+    int lt = nl, top = 0;
+    auto ncell_of = [&](int lv) { return (long)c->mg[lv].nx * c->mg[lv].ny * c->mg[lv].nz; };
+    if (nl > 1 && ncell_of(nl - 1) <= CDENSE) {
         for (int lv = std::max(l, 1); lv < nl; ++lv)
-            if ((long)c->mg[lv].nx * c->mg[lv].ny * c->mg[lv].nz <= TAIL_MAX) { lt = lv; break; }
-        if (nl - lt > TAIL_LEVELS) lt = nl;
+            if (ncell_of(lv) <= TAIL_MAX) { lt = lv; break; }
+        // FP32: one more level above, resident in LDS and registers
+        if constexpr (sizeof(T) == 4) {
+            const int lv = lt - 1;
+            const MgLevel &F = c->mg[lv];
+            const int nch = DIM == 3 ? 8 : 4;
+            const bool full = F.cx == 2 && F.cy == 2 && (DIM == 2 || F.cz == 2);
+            if (lt < nl && lv >= std::max(l, 1) && full && ncell_of(lv) <= TAIL_TOP) {
+                const MgLevel &C = c->mg[lv + 1];
+                const long plane = DIM == 3 ? (long)C.nx * C.ny : C.nx;
+                if (nch * (ncell_of(lv + 1) + 2 * plane) <= TAIL_LDS) { lt = lv; top = 1; }
+            }
+        }
+        if (nl - lt > TAIL_LEVELS) { lt = nl; top = 0; }
     }
     const int ldown = lt < nl ? lt : nl - 1;   // levels smoothed by separate launches: [l, ldown)
     // down
@@ -1052,10 +1385,14 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         arrays(lv + 1, ec, fc_, a, b, cc);
         dim3 g = grd3(C.nx, C.ny, C.nz, blk);
         ProfScope ps_(c, lv == 0 ? R_RESTRICT0 : R_RESTRICTC);
-        if (lv == 0)
-            hipLaunchKernelGGL((k_resid_restrict<T, DIM, true>), g, blk, 0, c->stream, e, f, rho0, bx, by, bz, L, ih,
-                               (T *)fc_, ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
-        else
+        if (lv == 0) {
+            if constexpr (sizeof(T) == 4)
+                hipLaunchKernelGGL((k_resid_restrict_split<DIM>), g, blk, 0, c->stream, e, f, rho0, split_of(G), ih,
+                                   (T *)fc_, ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
+            else
+                hipLaunchKernelGGL((k_resid_restrict<T, DIM, true>), g, blk, 0, c->stream, e, f, rho0, bx, by, bz, L,
+                                   ih, (T *)fc_, ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
+        } else
             hipLaunchKernelGGL((k_resid_restrict<T, DIM, false>), g, blk, 0, c->stream, e, f, (const T *)nullptr, bx,
                                by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
         FCFD_CHECK(hipGetLastError());
@@ -1063,6 +1400,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
     if (lt < nl) {
         TailArgs<T> A{};
         A.n = nl - lt;
+        A.top = top;
         A.minv = c->mg_scalars + 64;
         for (int q = 0; q < A.n; ++q) {
             const int lv = lt + q;
@@ -1108,9 +1446,10 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         arrays(lv + 1, ec, fc_, a, b, cc);
         if (nu2 > 0) {
             // prolongation folded into the first post-smoothing sweep
-            const Prol<T> P{(const T *)ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1};
+            const Prol<T> P{(const T *)ec, LC, M.cx == 2, M.cy == 2, DIM == 3 && M.cz == 2};
             FCFD_TRY(smooth(lv, sweeps(lv, nu2), true, &P, lv == 0 ? dots : nullptr, false));
         } else {
+            if (lv == 0 && sizeof(T) == 4) return FCFD_ERR_ARG;   // the split level 0 prolongs inside a sweep
             ProfScope ps_(c, lv == 0 ? R_PROLONG0 : R_PROLONGC);
             hipLaunchKernelGGL((k_prolong<T>), grd3(M.nx, M.ny, M.nz, blk), blk, 0, c->stream, e, (const T *)ec, L,
                                LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
@@ -1129,7 +1468,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
 template <int DIM>
 __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p, const double *__restrict__ f,
                                                    const double *__restrict__ rho, Lay L, Inv2 ih,
-                                                   double *__restrict__ r64, float *__restrict__ r32,
+                                                   double *__restrict__ r64, float *__restrict__ r32, Split S,
                                                    double *__restrict__ partial, int with_f) {
     __shared__ double sh[4];
     int bx_, by_, bz_;
@@ -1142,7 +1481,7 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
         rr = r * r;
         if (with_f) ff = f[id] * f[id];
         if (r64) r64[id] = r;
-        if (r32) r32[id] = (float)r;
+        if (r32) r32[S.at(i, j, k)] = (float)r;
     }
     rr = block_sum256(rr, sh);
     ff = block_sum256(ff, sh);
@@ -1157,40 +1496,63 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
 // new direction: p = z + beta p, q = A z + beta q (so q = A p), z_old = z,
 // partial sums of p.q.  The previous iteration's x += alpha p is applied
 // here, where the old p is read anyway (alpha = 0 on the first iteration;
-// the last one is applied by k_fcg_finish).
+// the last one is applied by k_fcg_finish).  z and z_old are FP64 in the
+// field layout, or FP32 in the colour-split layout S.  Each thread walks
+// FCG_KC planes of its (i, j) column and carries z and rho of the planes
+// k-1, k, k+1 in registers: a plane-per-block launch re-fetched both from
+// HBM, because the ten streams of one plane overflow an XCD's 4 MB L2
+// before the next plane needs them again.
+constexpr int FCG_KC = 16;
 template <typename T, int DIM>
 __global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__restrict__ zo,
                                                  double *__restrict__ p, double *__restrict__ q,
                                                  double *__restrict__ x, const double *__restrict__ rho, Lay L,
-                                                 Inv2 ih, const double *__restrict__ sc,
+                                                 Split S, Inv2 ih, const double *__restrict__ sc,
                                                  double *__restrict__ partial) {
     const double beta = sc[SC_BETA], alpha_prev = sc[SC_ALPHA];
     __shared__ double sh[4];
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
-    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y;
+    const int k0 = DIM == 3 ? bz_ * FCG_KC : 0, k1 = DIM == 3 ? min(k0 + FCG_KC, L.nz) : 1;
+    auto zid = [&](int ii, int jj, int kk) -> long {
+        if constexpr (sizeof(T) == 4) return S.at(ii, jj, kk);
+        else return L.id(ii, jj, kk);
+    };
     double pq = 0.0;
     if (i < L.nx && j < L.ny) {
-        long id = L.id(i, j, k);
-        // A z with the FP64 operator; z promoted
-        const double zp = (double)z[id];
-        double r0 = rho[id], acc = 0.0;
-        acc += face<double>(i < L.nx - 1, r0, rho[id + 1], ih.x) * ((double)z[id + 1] - zp);
-        acc += face<double>(i > 0, r0, rho[id - 1], ih.x) * ((double)z[id - 1] - zp);
-        acc += face<double>(j < L.ny - 1, r0, rho[id + L.sx], ih.y) * ((double)z[id + L.sx] - zp);
-        acc += face<double>(j > 0, r0, rho[id - L.sx], ih.y) * ((double)z[id - L.sx] - zp);
+        long id = L.id(i, j, k0);
+        // A z with the FP64 operator; z promoted.  (b, c, t): planes k-1, k, k+1
+        T zc = z[zid(i, j, k0)];
+        double zb = 0.0, rb = 0.0, rc = rho[id];
         if (DIM == 3) {
-            acc += face<double>(k < L.nz - 1, r0, rho[id + L.sxy], ih.z) * ((double)z[id + L.sxy] - zp);
-            acc += face<double>(k > 0, r0, rho[id - L.sxy], ih.z) * ((double)z[id - L.sxy] - zp);
+            zb = (double)z[zid(i, j, k0 - 1)];
+            rb = rho[id - L.sxy];
         }
-        const double po = p[id];
-        x[id] += alpha_prev * po;
-        double pn = zp + beta * po;
-        double qn = acc + beta * q[id];
-        p[id] = pn;
-        q[id] = qn;
-        zo[id] = z[id];
-        pq = pn * qn;
+        for (int k = k0; k < k1; ++k, id += L.sxy) {
+            const double zp = (double)zc;
+            T zt = (T)0;
+            double rt = 0.0, acc = 0.0;
+            acc += face<double>(i < L.nx - 1, rc, rho[id + 1], ih.x) * ((double)z[zid(i + 1, j, k)] - zp);
+            acc += face<double>(i > 0, rc, rho[id - 1], ih.x) * ((double)z[zid(i - 1, j, k)] - zp);
+            acc += face<double>(j < L.ny - 1, rc, rho[id + L.sx], ih.y) * ((double)z[zid(i, j + 1, k)] - zp);
+            acc += face<double>(j > 0, rc, rho[id - L.sx], ih.y) * ((double)z[zid(i, j - 1, k)] - zp);
+            if (DIM == 3) {
+                zt = z[zid(i, j, k + 1)];
+                rt = rho[id + L.sxy];
+                acc += face<double>(k < L.nz - 1, rc, rt, ih.z) * ((double)zt - zp);
+                acc += face<double>(k > 0, rc, rb, ih.z) * (zb - zp);
+            }
+            const double po = p[id];
+            x[id] += alpha_prev * po;
+            double pn = zp + beta * po;
+            double qn = acc + beta * q[id];
+            p[id] = pn;
+            q[id] = qn;
+            zo[zid(i, j, k)] = zc;
+            pq += pn * qn;
+            zb = zp; zc = zt; rb = rc; rc = rt;
+        }
     }
     pq = block_sum256(pq, sh);
     if (threadIdx.x + threadIdx.y == 0)
@@ -1200,7 +1562,7 @@ __global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__r
 // r -= alpha q, ||r||^2; the new residual also becomes the rhs of the next
 // preconditioning cycle (r32 for the FP32 cycle; the FP64 cycle reads r)
 __global__ void __launch_bounds__(256) k_fcg_update(const double *__restrict__ q, double *__restrict__ r, Lay L,
-                                                    const double *__restrict__ sc, float *__restrict__ r32,
+                                                    const double *__restrict__ sc, float *__restrict__ r32, Split S,
                                                     double *__restrict__ partial) {
     const double alpha = sc[SC_ALPHA];
     __shared__ double sh[4];
@@ -1213,7 +1575,7 @@ __global__ void __launch_bounds__(256) k_fcg_update(const double *__restrict__ q
         double rn = r[id] - alpha * q[id];
         r[id] = rn;
         rr = rn * rn;
-        if (r32) r32[id] = (float)rn;
+        if (r32) r32[S.at(i, j, k)] = (float)rn;
     }
     rr = block_sum256(rr, sh);
     if (threadIdx.x + threadIdx.y == 0)
@@ -1311,7 +1673,9 @@ static int fcg_iteration(fcfd_ctx *c, int kind, int inner) {
     dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
     const long nb = (long)g.x * g.y * g.z;
     const long nb_s = (long)(((G.nx + 1) / 2 + blk.x - 1) / blk.x) * ((G.ny + blk.y - 1) / blk.y) * G.nz;
+    const dim3 gdir(g.x, g.y, DIM == 3 ? (G.nz + FCG_KC - 1) / FCG_KC : 1);
     const Lay L0 = lay_of(G);
+    const Split S0 = split_of(G);
     const Inv2 ih = inv2(G.dx, G.dy, G.dz);
     MgLevel &M0 = c->mg[0];
     double *sc = c->mg_scalars, *rho = c->f[FCFD_RHO];
@@ -1330,17 +1694,17 @@ static int fcg_iteration(fcfd_ctx *c, int kind, int inner) {
     {
         ProfScope ps_(c, R_FCG);
         if (mixed)
-            hipLaunchKernelGGL((k_fcg_dir<float, DIM>), g, blk, 0, c->stream, M0.e32, c->kzo32, c->kp, c->kq,
-                               c->f[FCFD_P], rho, L0, ih, sc, c->red.partial);
+            hipLaunchKernelGGL((k_fcg_dir<float, DIM>), gdir, blk, 0, c->stream, M0.e32, c->kzo32, c->kp, c->kq,
+                               c->f[FCFD_P], rho, L0, S0, ih, sc, c->red.partial);
         else
-            hipLaunchKernelGGL((k_fcg_dir<double, DIM>), g, blk, 0, c->stream, c->kz, c->kzo, c->kp, c->kq,
-                               c->f[FCFD_P], rho, L0, ih, sc, c->red.partial);
+            hipLaunchKernelGGL((k_fcg_dir<double, DIM>), gdir, blk, 0, c->stream, c->kz, c->kzo, c->kp, c->kq,
+                               c->f[FCFD_P], rho, L0, S0, ih, sc, c->red.partial);
         FCFD_CHECK(hipGetLastError());
     }
-    FCFD_TRY(sum_partials(c, nb, 1, sc + SC_PQ, SUM_CG_ALPHA));
+    FCFD_TRY(sum_partials(c, (long)gdir.x * gdir.y * gdir.z, 1, sc + SC_PQ, SUM_CG_ALPHA));
     {
         ProfScope ps_(c, R_FCG);
-        hipLaunchKernelGGL(k_fcg_update, g, blk, 0, c->stream, c->kq, c->kr, L0, sc, mixed ? M0.f32 : nullptr,
+        hipLaunchKernelGGL(k_fcg_update, g, blk, 0, c->stream, c->kq, c->kr, L0, sc, mixed ? M0.f32 : nullptr, S0,
                            c->red.partial);
         FCFD_CHECK(hipGetLastError());
     }
@@ -1355,6 +1719,7 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
     const long nb = (long)g.x * g.y * g.z;
     const Lay L0 = lay_of(G);
+    const Split S0 = split_of(G);
     const Inv2 ih = inv2(G.dx, G.dy, G.dz);
     MgLevel &M0 = c->mg[0];
     const double ncell = (double)G.nx * (double)G.ny * (double)G.nz;
@@ -1374,7 +1739,7 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     {
         ProfScope ps_(c, R_RESID0);
         hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, f, rho, L0, ih, krylov ? c->kr : nullptr,
-                           mixed ? M0.f32 : nullptr, c->red.partial, 1);
+                           mixed ? M0.f32 : nullptr, S0, c->red.partial, 1);
         FCFD_CHECK(hipGetLastError());
     }
     double s2[2];
@@ -1390,13 +1755,13 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
             // defect correction (mixed) or plain V-cycles on p (FP64)
             FCFD_TRY(run_graph(c, kind, inner, [&]() { return vcycles<DIM>(c, kind, inner); }));
             if (mixed) {
-                hipLaunchKernelGGL(k_add_f32, g, blk, 0, c->stream, p, M0.e32, L0);
+                hipLaunchKernelGGL(k_add_f32, g, blk, 0, c->stream, p, M0.e32, L0, S0);
                 FCFD_CHECK(hipGetLastError());
             }
             {
                 ProfScope ps_(c, R_RESID0);
                 hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, f, rho, L0, ih, nullptr,
-                                   mixed ? M0.f32 : nullptr, c->red.partial, 0);
+                                   mixed ? M0.f32 : nullptr, S0, c->red.partial, 0);
                 FCFD_CHECK(hipGetLastError());
             }
             FCFD_TRY(sums_to_host(c, nb, 1, s2));
