@@ -749,16 +749,20 @@ __device__ __forceinline__ void cell_coords_g(const Lay &L, int id, int &i, int 
     i = r - jq * sx - 1;
 }
 
-// In-place Gauss-Jordan inversion in LDS (no pivoting: M is SPD); one block
-// of 256 threads; writes M^-1 (symmetric) to minv[n*n].
+// Gauss-Jordan inversion without pivoting (M is SPD) by one block of 256
+// threads, the matrix held in registers: thread t owns column t % 128 of the
+// rows t / 128 + 2q (q = 0..63).  Per pivot only the pivot row and column go
+// through LDS; the update of the owned elements is register arithmetic.
+// Writes M^-1 (symmetric) to minv[n*n].
 template <int DIM>
 __global__ void __launch_bounds__(256) k_coarse_inv(const double *__restrict__ bx, const double *__restrict__ by,
                                                     const double *__restrict__ bz, Lay L, Inv2 ih,
                                                     double *__restrict__ minv) {
-    __shared__ double M[CDENSE * CDENSE];
-    __shared__ double col[CDENSE];
+    constexpr int NQ = CDENSE * CDENSE / 256;   // owned rows per thread
+    __shared__ double col[CDENSE], row[CDENSE];
     __shared__ double sh[4];
     const int t = threadIdx.x, n = L.nx * L.ny * L.nz;
+    const int cc = t % CDENSE, rg = t / CDENSE;
     double dsum = 0.0;
     for (int c = t; c < n; c += 256) {
         int i, j, k;
@@ -769,41 +773,62 @@ __global__ void __launch_bounds__(256) k_coarse_inv(const double *__restrict__ b
         dsum += d;
     }
     const double sigma = block_sum256(dsum, sh) / ((double)n * (double)n);
-    for (int e = t; e < n * n; e += 256) M[e] = sigma;
-    __syncthreads();
-    for (int c = t; c < n; c += 256) {   // row c of B, added to the shift
-        int i, j, k;
-        cell_coords(L, c, i, j, k);
-        const long id = L.id(i, j, k);
-        const double ce_ = bx[id] * ih.x, cw = bx[id - 1] * ih.x, cn = by[id] * ih.y, cs = by[id - L.sx] * ih.y;
-        double d = ce_ + cw + cn + cs;
-        if (i < L.nx - 1) M[c * n + c + 1] -= ce_;
-        if (i > 0) M[c * n + c - 1] -= cw;
-        if (j < L.ny - 1) M[c * n + c + L.nx] -= cn;
-        if (j > 0) M[c * n + c - L.nx] -= cs;
-        if (DIM == 3) {
-            const double ct = bz[id] * ih.z, cb = bz[id - L.sxy] * ih.z;
-            const int pz = L.nx * L.ny;
-            d += ct + cb;
-            if (k < L.nz - 1) M[c * n + c + pz] -= ct;
-            if (k > 0) M[c * n + c - pz] -= cb;
+    // M = B + sigma 1 1^T: element (r, cc) of B from the face coefficients of cell r
+    double m[NQ];
+    {
+        int ic = 0, jc = 0, kc = 0;
+        if (cc < n) cell_coords(L, cc, ic, jc, kc);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int r = rg + 2 * q;
+            double v = sigma;
+            if (r < n && cc < n) {
+                int i, j, k;
+                cell_coords(L, r, i, j, k);
+                const long id = L.id(i, j, k);
+                const double ce_ = bx[id] * ih.x, cw = bx[id - 1] * ih.x, cn = by[id] * ih.y,
+                             cs = by[id - L.sx] * ih.y;
+                double d = ce_ + cw + cn + cs;
+                if (DIM == 3) d += bz[id] * ih.z + bz[id - L.sxy] * ih.z;
+                if (cc == r) v += d;
+                else if (i < L.nx - 1 && cc == r + 1) v -= ce_;
+                else if (i > 0 && cc == r - 1) v -= cw;
+                else if (j < L.ny - 1 && cc == r + L.nx) v -= cn;
+                else if (j > 0 && cc == r - L.nx) v -= cs;
+                else if (DIM == 3 && k < L.nz - 1 && cc == r + L.nx * L.ny) v -= bz[id] * ih.z;
+                else if (DIM == 3 && k > 0 && cc == r - L.nx * L.ny) v -= bz[id - L.sxy] * ih.z;
+            }
+            m[q] = v;
         }
-        M[c * n + c] += d;
     }
-    __syncthreads();
     for (int kk = 0; kk < n; ++kk) {
-        if (t < n) col[t] = M[t * n + kk];
+        // pivot column (owned by the threads of column kk) and pivot row
+        if (cc == kk) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+                if (rg + 2 * q < n) col[rg + 2 * q] = m[q];
+        }
+        if (rg == (kk & 1) && cc < n) {
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) v = q == (kk >> 1) ? m[q] : v;
+            row[cc] = v;
+        }
         __syncthreads();
         const double piv = col[kk];
-        if (t < n) M[kk * n + t] = (t == kk ? 1.0 : M[kk * n + t]) / piv;
-        __syncthreads();
-        for (int e = t; e < n * n; e += 256) {
-            const int r = e / n, cc = e - r * n;
-            if (r != kk) M[e] = (cc == kk ? 0.0 : M[e]) - col[r] * M[kk * n + cc];
+        const double nrow = (cc == kk ? 1.0 : row[cc]) / piv;   // new pivot row, this column
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int r = rg + 2 * q;
+            m[q] = r == kk ? nrow : (cc == kk ? 0.0 : m[q]) - col[r] * nrow;
         }
         __syncthreads();
     }
-    for (int e = t; e < n * n; e += 256) minv[e] = M[e];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int r = rg + 2 * q;
+        if (r < n && cc < n) minv[r * n + cc] = m[q];
+    }
 }
 
 // e = M^-1 b with b = -(f - mean f), then the zero-mean projection of e;
