@@ -749,84 +749,102 @@ __device__ __forceinline__ void cell_coords_g(const Lay &L, int id, int &i, int 
     i = r - jq * sx - 1;
 }
 
-// Gauss-Jordan inversion without pivoting (M is SPD) by one block of 256
-// threads, the matrix held in registers: thread t owns column t % 128 of the
-// rows t / 128 + 2q (q = 0..63).  Per pivot only the pivot row and column go
-// through LDS; the update of the owned elements is register arithmetic.
-// Writes M^-1 (symmetric) to minv[n*n].
+// Gauss-Jordan inversion without pivoting (M is SPD) by one block of
+// CINV_THREADS threads, the matrix held in registers: thread t owns column
+// t % 128 of the rows t / 128 + NRG q.  Per pivot only the pivot row and
+// column go through LDS; the update of the owned elements is register
+// arithmetic.  Writes M^-1 (symmetric) to minv[n*n].
+constexpr int CINV_THREADS = 1024;
 template <int DIM>
-__global__ void __launch_bounds__(256) k_coarse_inv(const double *__restrict__ bx, const double *__restrict__ by,
-                                                    const double *__restrict__ bz, Lay L, Inv2 ih,
-                                                    double *__restrict__ minv) {
-    constexpr int NQ = CDENSE * CDENSE / 256;   // owned rows per thread
-    __shared__ double col[CDENSE], row[CDENSE];
-    __shared__ double sh[4];
+__global__ void __launch_bounds__(CINV_THREADS) k_coarse_inv(const double *__restrict__ bx,
+                                                             const double *__restrict__ by,
+                                                             const double *__restrict__ bz, Lay L, Inv2 ih,
+                                                             double *__restrict__ minv) {
+    constexpr int NRG = CINV_THREADS / CDENSE;           // row groups
+    constexpr int NQ = CDENSE * CDENSE / CINV_THREADS;   // owned rows per thread
+    constexpr int NW = CINV_THREADS / WAVE;
+    __shared__ double colg[NRG][NQ], row[CDENSE];
+    __shared__ double sh[NW];
     const int t = threadIdx.x, n = L.nx * L.ny * L.nz;
     const int cc = t % CDENSE, rg = t / CDENSE;
     double dsum = 0.0;
-    for (int c = t; c < n; c += 256) {
+    if (t < n) {
         int i, j, k;
-        cell_coords(L, c, i, j, k);
+        cell_coords(L, t, i, j, k);
         const long id = L.id(i, j, k);
         double d = (bx[id] + bx[id - 1]) * ih.x + (by[id] + by[id - L.sx]) * ih.y;
         if (DIM == 3) d += (bz[id] + bz[id - L.sxy]) * ih.z;
-        dsum += d;
+        dsum = d;
     }
-    const double sigma = block_sum256(dsum, sh) / ((double)n * (double)n);
-    // M = B + sigma 1 1^T: element (r, cc) of B from the face coefficients of cell r
+    // fixed-order block sum: wave butterflies, then the waves in order
+    for (int o = 32; o > 0; o >>= 1) dsum += __shfl_xor(dsum, o, WAVE);
+    if ((t & (WAVE - 1)) == 0) sh[t / WAVE] = dsum;
+    __syncthreads();
+    double tot = 0.0;
+    for (int w = 0; w < NW; ++w) tot += sh[w];
+    const double sigma = tot / ((double)n * (double)n);
+    // M = B + sigma 1 1^T.  B is symmetric (the face coefficient between two
+    // cells is one stored value), so column cc is formed from cell cc's own
+    // faces: the diagonal d(cc) and -c(face) for the neighbour rows.
     double m[NQ];
     {
-        int ic = 0, jc = 0, kc = 0;
-        if (cc < n) cell_coords(L, cc, ic, jc, kc);
+        int i = 0, j = 0, k = 0;
+        double ce_ = 0.0, cw = 0.0, cn = 0.0, cs = 0.0, ct = 0.0, cb = 0.0, d = 0.0;
+        if (cc < n) {
+            cell_coords(L, cc, i, j, k);
+            const long id = L.id(i, j, k);
+            ce_ = bx[id] * ih.x; cw = bx[id - 1] * ih.x; cn = by[id] * ih.y; cs = by[id - L.sx] * ih.y;
+            d = ce_ + cw + cn + cs;
+            if (DIM == 3) {
+                ct = bz[id] * ih.z;
+                cb = bz[id - L.sxy] * ih.z;
+                d += ct + cb;
+            }
+        }
+        const int pz = L.nx * L.ny;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            const int r = rg + 2 * q;
+            const int r = rg + NRG * q;
             double v = sigma;
-            if (r < n && cc < n) {
-                int i, j, k;
-                cell_coords(L, r, i, j, k);
-                const long id = L.id(i, j, k);
-                const double ce_ = bx[id] * ih.x, cw = bx[id - 1] * ih.x, cn = by[id] * ih.y,
-                             cs = by[id - L.sx] * ih.y;
-                double d = ce_ + cw + cn + cs;
-                if (DIM == 3) d += bz[id] * ih.z + bz[id - L.sxy] * ih.z;
-                if (cc == r) v += d;
-                else if (i < L.nx - 1 && cc == r + 1) v -= ce_;
-                else if (i > 0 && cc == r - 1) v -= cw;
-                else if (j < L.ny - 1 && cc == r + L.nx) v -= cn;
-                else if (j > 0 && cc == r - L.nx) v -= cs;
-                else if (DIM == 3 && k < L.nz - 1 && cc == r + L.nx * L.ny) v -= bz[id] * ih.z;
-                else if (DIM == 3 && k > 0 && cc == r - L.nx * L.ny) v -= bz[id - L.sxy] * ih.z;
-            }
+            if (r == cc) v += d;
+            else if (i < L.nx - 1 && r == cc + 1) v -= ce_;
+            else if (i > 0 && r == cc - 1) v -= cw;
+            else if (j < L.ny - 1 && r == cc + L.nx) v -= cn;
+            else if (j > 0 && r == cc - L.nx) v -= cs;
+            else if (DIM == 3 && k < L.nz - 1 && r == cc + pz) v -= ct;
+            else if (DIM == 3 && k > 0 && r == cc - pz) v -= cb;
             m[q] = v;
         }
     }
     for (int kk = 0; kk < n; ++kk) {
-        // pivot column (owned by the threads of column kk) and pivot row
+        // pivot column (owned by the threads of column kk), stored by row
+        // group so that each thread reads its rows as consecutive words
         if (cc == kk) {
 #pragma unroll
-            for (int q = 0; q < NQ; ++q)
-                if (rg + 2 * q < n) col[rg + 2 * q] = m[q];
+            for (int q = 0; q < NQ; ++q) colg[rg][q] = m[q];
         }
-        if (rg == (kk & 1) && cc < n) {
+        if (rg == kk % NRG && cc < n) {
             double v = 0.0;
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) v = q == (kk >> 1) ? m[q] : v;
+            for (int q = 0; q < NQ; ++q) v = q == kk / NRG ? m[q] : v;
             row[cc] = v;
         }
         __syncthreads();
-        const double piv = col[kk];
+        const double piv = colg[kk % NRG][kk / NRG];
         const double nrow = (cc == kk ? 1.0 : row[cc]) / piv;   // new pivot row, this column
+        double cv[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) cv[q] = colg[rg][q];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            const int r = rg + 2 * q;
-            m[q] = r == kk ? nrow : (cc == kk ? 0.0 : m[q]) - col[r] * nrow;
+            const int r = rg + NRG * q;
+            m[q] = r == kk ? nrow : (cc == kk ? 0.0 : m[q]) - cv[q] * nrow;
         }
         __syncthreads();
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const int r = rg + 2 * q;
+        const int r = rg + NRG * q;
         if (r < n && cc < n) minv[r * n + cc] = m[q];
     }
 }
@@ -1282,11 +1300,11 @@ int mg_setup(fcfd_ctx *c) {
             Lay L = lay_of(M, G.dim);
             Inv2 ih = inv2(M.hx, M.hy, M.hz);
             if (G.dim == 3)
-                hipLaunchKernelGGL(k_coarse_inv<3>, dim3(1), dim3(256), 0, c->stream, M.bx64, M.by64, M.bz64, L, ih,
-                                   c->mg_scalars + 64);
+                hipLaunchKernelGGL(k_coarse_inv<3>, dim3(1), dim3(CINV_THREADS), 0, c->stream, M.bx64, M.by64,
+                                   M.bz64, L, ih, c->mg_scalars + 64);
             else
-                hipLaunchKernelGGL(k_coarse_inv<2>, dim3(1), dim3(256), 0, c->stream, M.bx64, M.by64, nullptr, L, ih,
-                                   c->mg_scalars + 64);
+                hipLaunchKernelGGL(k_coarse_inv<2>, dim3(1), dim3(CINV_THREADS), 0, c->stream, M.bx64, M.by64,
+                                   nullptr, L, ih, c->mg_scalars + 64);
             FCFD_CHECK(hipGetLastError());
         }
     }
