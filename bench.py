@@ -158,6 +158,9 @@ def mg_compare(s, case, variants=MG_VARIANTS) -> dict:
         out.update({"fp64_best": f64, "fp64_ms": conv[f64]["ms"], "mixed_best": mix, "mixed_ms": conv[mix]["ms"],
                     "speedup": round(conv[f64]["ms"] / conv[mix]["ms"], 3),
                     "p_rel_diff": float(np.abs(a - b).max() / max(np.abs(a).max(), 1e-300))})
+    if "fp64_plain" in conv and "mixed_plain" in conv:
+        # the same V(2,2) cycle iterated in both precisions (no Krylov acceleration)
+        out["speedup_plain_vcycle"] = round(conv["fp64_plain"]["ms"] / conv["mixed_plain"]["ms"], 3)
     return out
 
 
