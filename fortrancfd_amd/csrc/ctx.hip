@@ -82,11 +82,21 @@ struct VelBC {
     int nodal;
 };
 
+// For AX = 0 the ghost cells of a row are g consecutive words, so lane x
+// covers layer x % g + 1 of row x / g: the g writes of a row leave as one
+// access instead of g accesses each touching 64 rows.  Within a row only
+// layer 1 of the face-normal component reads a ghost cell (itself), and
+// the same lane then zeroes the wall face, so the layers are independent.
 template <int AX, int DIM>
 __global__ void k_velbc(double *__restrict__ u, double *__restrict__ v, double *__restrict__ w, Geom G, VelBC b) {
     // thread over the 2D set of the two other axes (full ghosted extents)
     const int g = G.g;
     int a = blockIdx.x * blockDim.x + threadIdx.x;
+    int lay = 0;   // AX = 0: the one ghost layer of this lane (1..g)
+    if (AX == 0) {
+        lay = a % g + 1;
+        a /= g;
+    }
     int bb = blockIdx.y * blockDim.y + threadIdx.y;
     int na, nb;
     if (AX == 0) { na = G.ny + 2 * g; nb = G.nz + 2 * G.gz; }
@@ -102,7 +112,7 @@ __global__ void k_velbc(double *__restrict__ u, double *__restrict__ v, double *
         for (int cpt = 0; cpt < DIM; ++cpt) {
             double *f = comp[cpt];
             bool normal = (cpt == AX) && !b.nodal;   // face-normal component
-            for (int ii = 1; ii <= g; ++ii) {
+            for (int ii = AX == 0 ? lay : 1; ii <= (AX == 0 ? lay : g); ++ii) {
                 int dst = side == 0 ? -ii : n - 1 + ii;
                 int src;
                 if (side == 0) src = normal ? ii - 2 : ii - 1;
@@ -113,7 +123,7 @@ __global__ void k_velbc(double *__restrict__ u, double *__restrict__ v, double *
                 else { ld = G.id(a - g, bb - g, dst); ls = G.id(a - g, bb - g, src); }
                 f[ld] = sg * f[ls];
             }
-            if (normal) {
+            if (normal && (AX != 0 || lay == 1)) {
                 int wall = side == 0 ? -1 : n - 1;
                 long lw = AX == 0 ? G.id(wall, a - g, bb - G.gz)
                                   : (AX == 1 ? G.id(a - g, wall, bb - G.gz) : G.id(a - g, bb - g, wall));
@@ -132,7 +142,7 @@ int velbc(fcfd_ctx *c, double *u, double *v, double *w, int nodal) {
         VelBC b;
         const int32_t *t = ax == 0 ? c->pr.ubc : (ax == 1 ? c->pr.vbc : c->pr.wbc);
         b.lo = t[0]; b.hi = t[1]; b.nodal = nodal;
-        int na = ax == 0 ? G.ny + 2 * g : G.nx + 2 * g;
+        int na = ax == 0 ? (G.ny + 2 * g) * g : G.nx + 2 * g;   // x: one lane per ghost layer of a row
         int nb = ax == 2 ? G.ny + 2 * g : G.nz + 2 * G.gz;
         dim3 grd((na + 63) / 64, (nb + 3) / 4);
         if (G.dim == 3) {
