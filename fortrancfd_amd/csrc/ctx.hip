@@ -195,6 +195,10 @@ extern "C" int fcfd_create(const fcfd_grid *grid, const fcfd_params *params, fcf
     G.dy = (grid->yend - grid->ystart) / G.ny;
     G.dz = grid->dim == 3 ? (grid->zend - grid->zstart) / G.nz : 1.0;
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; return FCFD_ERR_HIP; }
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) {
+        delete c;
+        return FCFD_ERR_HIP;
+    }
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return FCFD_ERR_HIP; }
     const long n = G.ntot;
     int rc = FCFD_OK;

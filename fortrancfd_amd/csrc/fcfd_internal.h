@@ -51,8 +51,9 @@ struct MgLevel {
     long sx, sxy, ntot;      // layout with one ghost layer (finest uses Geom)
     // FP64 arrays (level >= 1): solution/error, rhs, residual, face coeffs
     double *e64, *f64, *r64, *bx64, *by64, *bz64;
-    // FP32 mirrors for the mixed-precision inner cycle
-    float *e32, *f32, *r32, *bx32, *by32, *bz32;
+    // FP32 mirrors for the mixed-precision inner cycle; e32b is the second
+    // error buffer of the fused multi-sweep smoother (read one, write the other)
+    float *e32, *f32, *r32, *bx32, *by32, *bz32, *e32b;
 };
 
 // Kernel regions timed by the built-in HIP-event profiler (fcfd_prof_*).
@@ -90,6 +91,7 @@ struct fcfd_ctx {
     fcfd::Geom G;
     hipStream_t stream;
     int device;
+    int ncu;                   // compute units of the device
     // ---- state (device, full ghosted arrays) ----
     double *f[FCFD_NFIELDS];   // indexed by enum fcfd_field; NULL when not held
     // ---- scratch ----

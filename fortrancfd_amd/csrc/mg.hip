@@ -406,6 +406,99 @@ __global__ void k_rho_split(const double *__restrict__ rho, Lay L, float *__rest
     out[S.at(i, j, k)] = (float)rho[L.id(i, j, k)];
 }
 
+// NS full red-black sweeps of a coarse level (stored face coefficients,
+// one-ghost layout) in one launch.  A dependent launch costs ~4 us however
+// small its grid, and the small levels take 6 sweeps each.  Each block owns
+// an output tile and stages the tile plus a halo of H = 2 NS cells of e, f
+// and the face coefficients in LDS; half-sweep h updates its colour's cells
+// within distance 2 NS - 1 - h of the tile, so after the last half-sweep
+// the tile holds exactly what 2 NS single-colour launches produce (same
+// update, same operation order as stencil() + gs_step).  Positions outside
+// the ghosted array are staged as 0 and only interior cells are updated.
+// The result goes to a second array eo: blocks stage their halo from ei
+// while other blocks finish, so writing back into ei would race.
+// prol: the coarse-grid correction ec(parent) is added to every staged
+// value, which is what the folded first post-sweep (PROL 1 then 2) reads.
+template <int DIM>
+struct FusedTile {
+    static constexpr int TX = DIM == 3 ? 16 : 32, TY = DIM == 3 ? 8 : 16, TZ = DIM == 3 ? 8 : 1;
+};
+constexpr int FUSED_THREADS = 1024;
+template <int DIM, int NS>
+__global__ void __launch_bounds__(FUSED_THREADS) k_smooth_fused(const float *__restrict__ e, float *__restrict__ eo,
+                                                                const float *__restrict__ f,
+                                                                const float *__restrict__ bx,
+                                                                const float *__restrict__ by,
+                                                                const float *__restrict__ bz, Lay L, Inv2 ih,
+                                                                int color0, Prol<float> P, int prol) {
+    constexpr int H = 2 * NS, TX = FusedTile<DIM>::TX, TY = FusedTile<DIM>::TY, TZ = FusedTile<DIM>::TZ;
+    constexpr int EX = TX + 2 * H, EY = TY + 2 * H, EZ = DIM == 3 ? TZ + 2 * H : 1, NE = EX * EY * EZ;
+    constexpr int SY = EX, SZ = EX * EY;
+    __shared__ float se[NE], sf[NE], sbx[NE], sby[NE], sbz[DIM == 3 ? NE : 1];
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int t = threadIdx.x;
+    const int i0 = bx_ * TX - H, j0 = by_ * TY - H, k0 = DIM == 3 ? bz_ * TZ - H : 0;
+    const float hx = (float)ih.x, hy = (float)ih.y, hz = (float)ih.z;
+    for (int q = t; q < NE; q += FUSED_THREADS) {
+        const int qx = q % EX, qy = (q / EX) % EY, qz = q / SZ;
+        const int i = i0 + qx, j = j0 + qy, k = k0 + qz;
+        const bool inarr = i >= -1 && i <= L.nx && j >= -1 && j <= L.ny && (DIM == 2 || (k >= -1 && k <= L.nz));
+        float ev = 0.f, fv = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;
+        if (inarr) {
+            const long id = L.id(i, j, k);
+            ev = e[id];
+            if (prol) ev += P.at(i, j, k);
+            fv = f[id];
+            b1 = bx[id];
+            b2 = by[id];
+            if (DIM == 3) b3 = bz[id];
+        }
+        se[q] = ev;
+        sf[q] = fv;
+        sbx[q] = b1;
+        sby[q] = b2;
+        if (DIM == 3) sbz[q] = b3;
+    }
+    __syncthreads();
+    // half-sweep h: cells of colour (color0 + h) & 1 within distance
+    // D = 2 NS - 1 - h of the tile; the loop is unrolled, so the region
+    // extents are constants and only the colour's cells are visited (x
+    // steps by 2 from the row's first cell of that colour)
+#pragma unroll
+    for (int h = 0; h < 2 * NS; ++h) {
+        const int color = (color0 + h) & 1, D = 2 * NS - 1 - h;
+        const int rx = TX + 2 * D, ry = TY + 2 * D, rz = DIM == 3 ? TZ + 2 * D : 1;
+        const int mx = (rx + 1) / 2, nq = mx * ry * rz;
+        const int ox = H - D, oy = H - D, oz = DIM == 3 ? H - D : 0;
+        for (int q = t; q < nq; q += FUSED_THREADS) {
+            const int m = q % mx, r = q / mx, ly = oy + r % ry, lz = oz + r / ry;
+            const int par = (color - i0 - ox - j0 - ly - k0 - lz) & 1;
+            const int lx = ox + par + 2 * m;
+            const int i = i0 + lx, j = j0 + ly, k = k0 + lz;
+            if (lx >= ox + rx || i < 0 || i >= L.nx || j < 0 || j >= L.ny || (DIM == 3 && (k < 0 || k >= L.nz)))
+                continue;
+            const int c = lx + ly * SY + lz * SZ;
+            const float ep = se[c];
+            const float ce_ = sbx[c] * hx, cw = sbx[c - 1] * hx, cn = sby[c] * hy, cs = sby[c - SY] * hy;
+            float sc = ce_ + cw + cn + cs;
+            float sce = ce_ * (se[c + 1] - ep) + cw * (se[c - 1] - ep) + cn * (se[c + SY] - ep) + cs * (se[c - SY] - ep);
+            if (DIM == 3) {
+                const float ct = sbz[c] * hz, cb = sbz[c - SZ] * hz;
+                sc += ct + cb;
+                sce += ct * (se[c + SZ] - ep) + cb * (se[c - SZ] - ep);
+            }
+            se[c] = sc > 0.f ? ep + gs_step<float>(sce - sf[c], sc) : ep;
+        }
+        __syncthreads();
+    }
+    for (int q = t; q < TX * TY * TZ; q += FUSED_THREADS) {
+        const int lx = H + q % TX, ly = H + (q / TX) % TY, lz = (DIM == 3 ? H : 0) + q / (TX * TY);
+        const int i = i0 + lx, j = j0 + ly, k = k0 + lz;
+        if (i < L.nx && j < L.ny && (DIM == 2 || k < L.nz)) eo[L.id(i, j, k)] = se[lx + ly * SY + lz * SZ];
+    }
+}
+
 // residual of the level, restricted to the coarse right-hand side by the
 // mean over the children; the coarse error is zeroed in the same pass
 template <typename T, int DIM, bool FINE>
@@ -1157,7 +1250,8 @@ static Inv2 inv2(double hx, double hy, double hz) {
 
 int mg_free(fcfd_ctx *c) {
     for (auto &M : c->mg) {
-        void *ptrs[] = {M.e64, M.f64, M.r64, M.bx64, M.by64, M.bz64, M.e32, M.f32, M.r32, M.bx32, M.by32, M.bz32};
+        void *ptrs[] = {M.e64, M.f64, M.r64, M.bx64, M.by64, M.bz64, M.e32, M.f32, M.r32, M.bx32, M.by32, M.bz32,
+                        M.e32b};
         for (void *p : ptrs)
             if (p) hipFree(p);
     }
@@ -1237,7 +1331,7 @@ static int mg_build(fcfd_ctx *c) {
         FCFD_TRY(alloc0(&M.e64, n)); FCFD_TRY(alloc0(&M.f64, n));
         FCFD_TRY(alloc0(&M.bx64, n)); FCFD_TRY(alloc0(&M.by64, n));
         if (G.dim == 3) FCFD_TRY(alloc0(&M.bz64, n));
-        FCFD_TRY(alloc0(&M.e32, n)); FCFD_TRY(alloc0(&M.f32, n));
+        FCFD_TRY(alloc0(&M.e32, n)); FCFD_TRY(alloc0(&M.f32, n)); FCFD_TRY(alloc0(&M.e32b, n));
         FCFD_TRY(alloc0(&M.bx32, n)); FCFD_TRY(alloc0(&M.by32, n));
         if (G.dim == 3) FCFD_TRY(alloc0(&M.bz32, n));
     }
@@ -1343,6 +1437,43 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         dim3 g((unsigned)(((M.nx + 1) / 2 + blk.x - 1) / blk.x), (M.ny + blk.y - 1) / blk.y, M.nz);
         const Prol<T> P0 = prol ? *prol : Prol<T>{nullptr, Lay{}, 0, 0, 0};
         const Dots<T> D0 = dt ? *dt : Dots<T>{nullptr, nullptr, nullptr};
+        if constexpr (sizeof(T) == 4) {
+            // small FP32 levels: fused sweeps ping-ponging between e32 and
+            // e32b; an even number of launches ends back in e32.  Two sweeps
+            // per launch (one 123 KB block per CU) while the tiles fit in one
+            // round on the device, else one sweep per launch (two blocks per CU)
+            const long n0c = (long)c->mg[0].nx * c->mg[0].ny * c->mg[0].nz;
+            if (lv >= 1 && M.e32b && sweeps >= 2 && 32L * M.nx * M.ny * M.nz <= n0c && !dt && !zinit) {
+                using FT = FusedTile<DIM>;
+                const dim3 gf((M.nx + FT::TX - 1) / FT::TX, (M.ny + FT::TY - 1) / FT::TY,
+                              DIM == 3 ? (M.nz + FT::TZ - 1) / FT::TZ : 1);
+                const int c0 = reverse ? 1 : 0;
+                int n2, n1;
+                if ((long)gf.x * gf.y * gf.z <= c->ncu) {
+                    n2 = sweeps / 2;
+                    n1 = sweeps % 2;
+                    if ((n2 + n1) % 2) { n2 -= 1; n1 += 2; }
+                } else {
+                    n2 = sweeps % 2;   // one two-sweep launch keeps an odd sweep count even in launches
+                    n1 = sweeps - 2 * n2;
+                }
+                float *bufs[2] = {M.e32, M.e32b};
+                for (int q = 0; q < n2 + n1; ++q) {
+                    ProfScope ps_(c, R_SMOOTHC);
+                    const int pm = prol && q == 0;
+                    const float *ei = bufs[q & 1];
+                    float *eo = bufs[(q + 1) & 1];
+                    if (q < n2)
+                        hipLaunchKernelGGL((k_smooth_fused<DIM, 2>), gf, dim3(FUSED_THREADS), 0, c->stream, ei, eo, f,
+                                           bx, by, bz, L, ih, c0, P0, pm);
+                    else
+                        hipLaunchKernelGGL((k_smooth_fused<DIM, 1>), gf, dim3(FUSED_THREADS), 0, c->stream, ei, eo, f,
+                                           bx, by, bz, L, ih, c0, P0, pm);
+                }
+                FCFD_CHECK(hipGetLastError());
+                return FCFD_OK;
+            }
+        }
         for (int s = 0; s < sweeps; ++s)
             for (int h = 0; h < 2; ++h) {
                 ProfScope ps_(c, lv == 0 ? R_SMOOTH0 : R_SMOOTHC);
