@@ -121,6 +121,11 @@ class Case:
     ppe_nu_coarse: int = 6      # sweeps on levels of <= 1/32 of the finest cells
     imass: float = 0.0
     name: str = "case"
+    # output schedule of the reference driver (manager_read): plot pieces
+    # per axis, stop time and plot interval; host-side only
+    num_of_plot: int = 1
+    t2s: float = 0.0
+    t2p: float = 0.0
 
     # ---- derived (tree.f90:154-172) -----------------------------------
     @property
@@ -237,7 +242,7 @@ class Case:
         lines = [
             "ooo Method to capture interface ( LS / MPLS / CLSVOF )", "2",
             "ooo Number of threads", "1",
-            "ooo Plot split", "1",
+            "ooo Plot split", str(self.num_of_plot),
             "ooo Name of the problem", self.name,
             "ooo Cells per unit length", str(self.ug()),
             "ooo Number of ghost cell", str(self.ghc),
@@ -247,7 +252,7 @@ class Case:
         if self.dim == 3:
             lines += ["ooo Range of domain in Z direction", f"{r(self.zstart)} {r(self.zend)}"]
         lines += [
-            "ooo Time to stop / plot", "1.0 1.0",
+            "ooo Time to stop / plot", f"{r(self.t2s or 1.0)} {r(self.t2p or 1.0)}",
             "ooo Crount number ( advection / redistance )", f"{r(self.cfl)} {r(self.cfl_r)}",
             "ooo PPE iteration ( tolerance / w1 / w2 / valve )", f"{r(self.p_tol_exp)} {r(self.p_w1)} 0.5 4.0",
             "ooo SRK6 iteration ( tolerance / w )", "10.0 0.5",
@@ -284,7 +289,7 @@ def read_reference_input(path: str, dim: int) -> Case:
         raw = fh.read().splitlines()
     vals = [ln.split() for k, ln in enumerate(raw) if k % 2 == 1]
     it = iter(vals)
-    method = int(next(it)[0]); _threads = next(it); _plot = next(it)
+    method = int(next(it)[0]); _threads = next(it); nplot = int(next(it)[0])
     name = next(it)[0]
     ug = int(float(next(it)[0]))
     ghc = int(next(it)[0])
@@ -293,7 +298,7 @@ def read_reference_input(path: str, dim: int) -> Case:
     zs, ze = (0.0, 1.0)
     if dim == 3:
         zs, ze = map(float, next(it)[:2])
-    _t2s = next(it)
+    t2s, t2p = map(float, next(it)[:2])
     cfl, cfl_r = map(float, next(it)[:2])
     ptol, pw1 = map(float, next(it)[:2])
     _srk = next(it)
@@ -334,7 +339,8 @@ def read_reference_input(path: str, dim: int) -> Case:
     return Case(dim=dim, nx=nx, ny=ny, nz=nz, xstart=xs, xend=xe, ystart=ys, yend=ye,
                 zstart=zs, zend=ze, ghc=ghc, cfl=cfl, cfl_r=cfl_r, ls_w=lsw, rho_12=r12,
                 mu_12=m12, re=re, we=we, fr=fr, gvec=gvec, btn_sf=btn_sf, btn_g=btn_g,
-                ubc=ubc, vbc=vbc, wbc=wbc, p_tol_exp=ptol, p_w1=pw1, name=name)
+                ubc=ubc, vbc=vbc, wbc=wbc, p_tol_exp=ptol, p_w1=pw1, name=name,
+                num_of_plot=nplot, t2s=t2s, t2p=t2p)
 
 
 def rising_bubble(dim: int, nx: int, ny: int, nz: int = 1, *, lx: float = 1.0, ly: float = 2.0,

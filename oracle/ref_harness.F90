@@ -7,6 +7,7 @@
 ! vectors for the CPU oracle; it is never part of the product.
 !
 ! usage: ref_harness <op> <config> <indir> <outdir>
+!   (ops plot and output write the reference's files under ./out/)
 !   config  reference-format input file (`.../lib/data/tree.f90:48-106`)
 !   indir   raw little-endian float64 files named after the fields (phi, u ...)
 !           holding the full ghosted arrays; missing files leave zeros
@@ -107,6 +108,18 @@ case ('step')
     call interface_solver
     call harness_ns_solver
     call ls_mv
+case ('plot')
+    ! plot snapshot 0 (plot.f90:10 writes when |time - pid*t2p| <= dt)
+    p%glb%pid = 0
+    p%glb%time = 0.0d0
+    call plot
+case ('output')
+    ! one _MVloss.plt row: ls_mv of the given phi against imass from the
+    ! scalars and ivol = 1.01 * vol (so both loss columns are non-zero)
+    p%glb%time = time_in
+    call ls_mv
+    p%glb%ivol = 1.01d0 * p%glb%vol
+    call output
 case ('init')
     call harness_problem_init(.true.)
 case ('init_nore')

@@ -63,4 +63,7 @@ def run(case, op: str, inputs: dict, imass: float = 0.0, time: float = 0.0, it: 
         out["_piter"] = int(v[4])
         out["_ppe_err"] = float(v[5])
         out["_stdout"] = r.stdout
+        # files the reference wrote under ./out (_MVloss.plt, plot snapshots)
+        od = os.path.join(td, "out")
+        out["_files"] = {fn: open(os.path.join(od, fn), "rb").read() for fn in sorted(os.listdir(od))}
         return out
