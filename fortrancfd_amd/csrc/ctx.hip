@@ -260,6 +260,8 @@ extern "C" int fcfd_destroy(fcfd_ctx *c) {
         if (p) hipFree(p);
     for (auto p : c->ccd_coef)
         if (p) hipFree(p);
+    for (auto p : c->adj)
+        if (p) hipFree(p);
     if (c->red.host) hipHostFree(c->red.host);
     if (c->flag) hipFree(c->flag);
     if (c->flag_host) hipHostFree(c->flag_host);

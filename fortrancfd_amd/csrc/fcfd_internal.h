@@ -102,6 +102,7 @@ struct fcfd_ctx {
     double *dsu, *dsv, *dsw;   // viscous + body force source (sec_part1..3), once per step
     double *un, *vn, *wn;      // predictor output buffer (ping-pong with u, v, w)
     double *ccd_coef[9];       // factorised line matrices [kind * 3 + direction]: CCD, UCCD up, UCCD down
+    double *adj[5];            // advection adjoint: stage states phi1, phi2, adjoints lambda1, lambda2, seed
     // ---- reductions ----
     fcfd::Reduce red;
     unsigned long long *flag;  // device flags (reinit error max bits, done, count)
@@ -205,6 +206,8 @@ int ccd_setup(fcfd_ctx *c);
 int surface_norms(fcfd_ctx *c, const double *phi, int full);
 int curv_kernel(fcfd_ctx *c);
 int ls_advect(fcfd_ctx *c);
+int ls_adv_stage(fcfd_ctx *c, const double *phi, double *phin, int stage);
+int ls_adv_adjoint_step(fcfd_ctx *c, const double *phi, double *lambda, double *ub, double *vb, double *wb);
 int redis(fcfd_ctx *c, int btn, int *iters);
 int mpls(fcfd_ctx *c);
 int ls_mv(fcfd_ctx *c, double *mass, double *vol);

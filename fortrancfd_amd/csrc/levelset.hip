@@ -14,6 +14,7 @@
 // the backward differences, the lambda numerator/denominator) are recomputed
 // or read with clamped indices instead of being stored with ghosts.
 #include "fcfd_internal.h"
+#include "weno.h"
 
 using namespace fcfd;
 
@@ -36,52 +37,6 @@ __device__ __forceinline__ void lsfun(double phi, double w, double &H, double &D
         D = 0.5 * (1.0 + c) / w;
     }
     H = fmax(fmin(H, 1.0), 0.0);
-}
-
-// WENO5-JS nonlinear weights w_k = a_k / sum a, a_k = d_k / (EPS + beta_k)^2,
-// d = (1, 6, 3) (level_set_rk3_solver.f90:170-216).  Multiplying every a_k
-// by q1 q2 q3 (q_k = (EPS + beta_k)^2) leaves the weights unchanged and needs
-// one division instead of six: w_k = (d_k prod_{m != k} q_m) / sum_l (...).
-// With |phi| and |u| of O(1) the products stay far inside the FP64 range
-// (q >= EPS^2 = 1e-26, so a product of two q is >= 1e-52).
-__device__ __forceinline__ void weno_weights(double b1, double b2, double b3, double EPS, double &w1, double &w2,
-                                             double &w3) {
-    const double q1 = (EPS + b1) * (EPS + b1), q2 = (EPS + b2) * (EPS + b2), q3 = (EPS + b3) * (EPS + b3);
-    const double a1 = q2 * q3, a2 = 6.0 * (q1 * q3), a3 = 3.0 * (q1 * q2);
-    const double r = 1.0 / (a1 + a2 + a3);
-    w1 = a1 * r;
-    w2 = a2 * r;
-    w3 = a3 * r;
-}
-
-// WENO5-JS left-biased face value from g(i-2..i+2) (level_set_rk3_solver.f90:170-191)
-__device__ __forceinline__ double weno_left(double g0, double g1, double g2, double g3, double g4, double EPS) {
-    double t1 = g0 - 2.0 * g1 + g2, t2 = g0 - 4.0 * g1 + 3.0 * g2;
-    const double b1 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
-    t1 = g1 - 2.0 * g2 + g3; t2 = g1 - g3;
-    const double b2 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
-    t1 = g2 - 2.0 * g3 + g4; t2 = 3.0 * g2 - 4.0 * g3 + g4;
-    const double b3 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
-    double w1, w2, w3;
-    weno_weights(b1, b2, b3, EPS, w1, w2, w3);
-    constexpr double r6 = 1.0 / 6.0;
-    return w1 * (1.0 / 3.0) * g0 - (7.0 * w1 + w2) * r6 * g1 + (11.0 * w1 + 5.0 * w2 + 2.0 * w3) * r6 * g2 +
-           (2.0 * w2 + 5.0 * w3) * r6 * g3 - w3 * r6 * g4;
-}
-
-// WENO5-JS right-biased face value from f(i-1..i+3) (level_set_rk3_solver.f90:193-216)
-__device__ __forceinline__ double weno_right(double f0, double f1, double f2, double f3, double f4, double EPS) {
-    double t1 = f0 - 2.0 * f1 + f2, t2 = f0 - 4.0 * f1 + 3.0 * f2;
-    const double b3 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
-    t1 = f1 - 2.0 * f2 + f3; t2 = f1 - f3;
-    const double b2 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
-    t1 = f2 - 2.0 * f3 + f4; t2 = 3.0 * f2 - 4.0 * f3 + f4;
-    const double b1 = 13.0 * (t1 * t1) + 3.0 * (t2 * t2);
-    double w1, w2, w3;
-    weno_weights(b1, b2, b3, EPS, w1, w2, w3);
-    constexpr double r6 = 1.0 / 6.0;
-    return w3 * (-f0 + 5.0 * f1 + 2.0 * f2) * r6 + w2 * (2.0 * f1 + 5.0 * f2 - f3) * r6 +
-           w1 * (11.0 * f2 - 7.0 * f3 + 2.0 * f4) * r6;
 }
 
 // ------------------------------------------------------------- properties
@@ -354,6 +309,21 @@ __global__ void __launch_bounds__(256) k_ls_uccd_stage(const double *__restrict_
         src = (-l1[id] - l2[id] + 8.0 * L) / 12.0;
     }
     phin[id] = phi[id] + src * dt;
+}
+
+// one WENO5 SSP-RK3 stage from phi into phin with the context's velocity
+// (the stage sources stay in l1/l2 as in ls_advect); no ghost update
+int ls_adv_stage(fcfd_ctx *c, const double *phi, double *phin, int stage) {
+    const Geom &G = c->G;
+    dim3 blk(64, 4), grd = grid_interior(G, blk);
+    if (G.dim == 3)
+        hipLaunchKernelGGL(k_ls_adv_stage<3>, grd, blk, 0, c->stream, phi, phin, c->f[FCFD_NU], c->f[FCFD_NV],
+                           c->f[FCFD_NW], c->l1, c->l2, G, c->pr.dt, stage);
+    else
+        hipLaunchKernelGGL(k_ls_adv_stage<2>, grd, blk, 0, c->stream, phi, phin, c->f[FCFD_NU], c->f[FCFD_NV],
+                           nullptr, c->l1, c->l2, G, c->pr.dt, stage);
+    FCFD_CHECK(hipGetLastError());
+    return FCFD_OK;
 }
 
 int ls_advect(fcfd_ctx *c) {

@@ -56,6 +56,7 @@ def lib():
         "fcfd_ls_funs": [vp], "fcfd_rho_mu": [vp], "fcfd_ls_mv": [vp, vp, vp],
         "fcfd_level_set_rk3_solver": [vp], "fcfd_level_set_rk3_redis": [vp, i32, vp],
         "fcfd_mass_preserving_level_set": [vp], "fcfd_surface_norms": [vp], "fcfd_curv": [vp], "fcfd_uccd": [vp],
+        "fcfd_ls_adv_adjoint_step": [vp, vp, vp, vp, vp, vp],
         "fcfd_ppe_sor_init": [vp], "fcfd_ppe_sor_solver": [vp, dbl, i32, vp, vp],
         "fcfd_ppe_solve_rhs": [vp, dbl, i32, vp, vp],
         "fcfd_ns_momentum_correction": [vp], "fcfd_ns_ab_predictor": [vp],
@@ -185,6 +186,15 @@ class Solver:
 
     def level_set_rk3_solver(self) -> None:
         self._call("fcfd_level_set_rk3_solver")
+
+    def ls_adv_adjoint_step(self, phi_ptr: int, lam_ptr: int, ub_ptr: int = 0, vb_ptr: int = 0,
+                            wb_ptr: int = 0) -> None:
+        """Adjoint of one level_set_rk3_solver step at the level set phi
+        (device pointer, ghosted layout) with the current NU/NV/NW: lambda
+        (device) goes from dJ/dphi after the step to dJ/dphi before it; the
+        velocity adjoint is added into the u/v/w bar arrays when given."""
+        self._call("fcfd_ls_adv_adjoint_step", C.c_void_p(phi_ptr), C.c_void_p(lam_ptr),
+                   C.c_void_p(ub_ptr or None), C.c_void_p(vb_ptr or None), C.c_void_p(wb_ptr or None))
 
     def level_set_rk3_redis(self, btn: int) -> int:
         it = C.c_int()

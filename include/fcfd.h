@@ -71,6 +71,18 @@ int fcfd_ls_mv(fcfd_ctx *ctx, double *mass, double *vol);
  * upwind-CCD form -(u phi_x + v phi_y + w phi_z) (the commented alternative
  * of level_set_rk3_solver.f90:109,123,137,155-157). */
 int fcfd_level_set_rk3_solver(fcfd_ctx *ctx);
+/* Discrete adjoint of one fcfd_level_set_rk3_solver step (WENO5 scheme, the
+ * velocity NU/NV/NW of the context frozen over the step).  No counterpart in
+ * the reference: the differentiation entry point of the advection.
+ * Device pointers, ghosted field layout:
+ *   phi     the level set at the start of the step (interior read);
+ *   lambda  in: dJ/dphi after the step; out: dJ/dphi before it (interior
+ *           read and written, ghosts untouched);
+ *   u/v/wbar  when not NULL, dJ/d(NU, NV, NW) of this step is ADDED to them
+ *           (interior).
+ * Returns FCFD_ERR_ARG for the UCCD scheme. */
+int fcfd_ls_adv_adjoint_step(fcfd_ctx *ctx, const double *phi, double *lambda, double *ubar, double *vbar,
+                             double *wbar);
 
 /* root_solve("uccd") (`.../lib/solver/ccd_solvers.f90:161-210,308-341,400-412`)
  * of phi along every interior line, upwind by NU/NV/NW (nvel%old): phi_x ->

@@ -1,0 +1,215 @@
+"""Adjoint of the WENO5 SSP-RK3 level-set advection (fcfd_ls_adv_adjoint_step
+and the torch.autograd.Function of fortrancfd_amd.adjoint).
+
+* one step: the adjoint of the level set and of the velocity against
+  torch.autograd of the independent restatement (tests/adjoint_ref.py) on
+  the same state, and <w, dS(phi) v> by central differences of the device
+  forward step against <dS(phi)^T w, v> (the dot-product test of a
+  discrete adjoint).  The perturbations v are smooth fields: WENO-JS with
+  EPS = 1e-13 responds to white noise of size 1e-6 far outside its linear
+  range, so a rough v measures the curvature of the weights, not the
+  derivative;
+* a run with checkpointing: dJ/dphi0 along a random direction and dJ/dA
+  against central finite differences with eps = 1e-6, the acceptance bound
+  1e-6 relative;
+* the gradient fields against an independent FP64 reference gradient:
+  torch.autograd of the same forward restated on the CPU
+  (tests/adjoint_ref.py), whose forward is checked against the device first.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cases import _smooth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _case(n=(20, 16, 18), **kw):
+    from fortrancfd_amd.types import Case
+    nx, ny, nz = n
+    return Case(dim=3, nx=nx, ny=ny, nz=nz, xend=1.0, yend=ny / nx, zend=nz / nx, cfl=0.3, name="adv", **kw)
+
+
+def _phi0(case, r=0.15, c=(0.35, 0.35, 0.35)):
+    from fortrancfd_amd.types import bc_copy, bubble_phi
+    return bc_copy(case, bubble_phi(case, r, c))
+
+
+def _solver(case):
+    from fortrancfd_amd import Solver
+    return Solver(case)
+
+
+def _interior_random(case, rng):
+    a = np.zeros(case.shape)
+    g = case.ghc
+    a[g:-g, g:-g, g:-g] = rng.standard_normal((case.nz, case.ny, case.nx))
+    return a
+
+
+def _step(s, case, phi, vel):
+    for nm, v in zip(("nu", "nv", "nw"), vel):
+        s.push(nm, v)
+    s.push("phi", phi)
+    s.level_set_rk3_solver()
+    return s.pull("phi")
+
+
+def test_adjoint_step_dot_product():
+    from fortrancfd_amd.adjoint import deformation_field
+    case = _case()
+    s = _solver(case)
+    rng = np.random.default_rng(7)
+    phi = _phi0(case)
+    vel = [0.8 * u for u in deformation_field(case)]
+    v = _smooth(case, rng, 1.0)
+    q = [_smooth(case, rng, 1.0) for _ in range(3)]
+    w = _interior_random(case, rng)
+    g = case.ghc
+    inner = lambda a, b: float((a[g:-g, g:-g, g:-g] * b[g:-g, g:-g, g:-g]).sum())
+    eps = 1e-6
+    # directional derivatives of one forward step by central differences
+    dphi = (_step(s, case, phi + eps * v, vel) - _step(s, case, phi - eps * v, vel)) / (2 * eps)
+    dvel = (_step(s, case, phi, [a + eps * b for a, b in zip(vel, q)]) -
+            _step(s, case, phi, [a - eps * b for a, b in zip(vel, q)])) / (2 * eps)
+    # adjoint at the same state
+    for nm, a in zip(("nu", "nv", "nw"), vel):
+        s.push(nm, a)
+    dev = torch.device("cuda")
+    ph_t = torch.from_numpy(phi.reshape(-1).copy()).to(dev)
+    lam = torch.from_numpy(w.reshape(-1).copy()).to(dev)
+    bars = [torch.zeros(case.size, dtype=torch.float64, device=dev) for _ in range(3)]
+    torch.cuda.synchronize()
+    s.ls_adv_adjoint_step(ph_t.data_ptr(), lam.data_ptr(), *(b.data_ptr() for b in bars))
+    s.sync()
+    lam_n = lam.cpu().numpy().reshape(case.shape)
+    bars_n = [b.cpu().numpy().reshape(case.shape) for b in bars]
+    lhs, rhs = inner(w, dphi), inner(lam_n, v)
+    print("phi: <w, dS v> =", lhs, " <dS^T w, v> =", rhs)
+    assert abs(lhs - rhs) <= 1e-7 * abs(lhs)
+    lhs_u = inner(w, dvel)
+    rhs_u = sum(inner(b, c) for b, c in zip(bars_n, q))
+    print("vel: <w, dS q> =", lhs_u, " <dS_u^T w, q> =", rhs_u)
+    assert abs(lhs_u - rhs_u) <= 1e-7 * abs(lhs_u)
+
+
+def test_adjoint_step_against_autograd():
+    import adjoint_ref as R
+    from fortrancfd_amd.adjoint import deformation_field
+    case = _case()
+    g = case.ghc
+    inter = lambda a: np.ascontiguousarray(np.asarray(a).reshape(case.shape)[g:-g, g:-g, g:-g])
+    phi = _phi0(case)
+    vel = [0.8 * u for u in deformation_field(case)]
+    w = _interior_random(case, np.random.default_rng(7))
+    s = _solver(case)
+    for nm, a in zip(("nu", "nv", "nw"), vel):
+        s.push(nm, a)
+    dev = torch.device("cuda")
+    ph_t = torch.from_numpy(phi.reshape(-1).copy()).to(dev)
+    lam = torch.from_numpy(w.reshape(-1).copy()).to(dev)
+    bars = [torch.zeros(case.size, dtype=torch.float64, device=dev) for _ in range(3)]
+    torch.cuda.synchronize()
+    s.ls_adv_adjoint_step(ph_t.data_ptr(), lam.data_ptr(), *(b.data_ptr() for b in bars))
+    s.sync()
+    p = torch.from_numpy(inter(phi)).requires_grad_(True)
+    u = [torch.from_numpy(inter(a)).requires_grad_(True) for a in vel]
+    out = R.rk3_step(p, tuple(u), (case.dx, case.dy, case.dz), case.dt)
+    (out * torch.from_numpy(inter(w))).sum().backward()
+    rel = lambda a, b: float(np.abs(a - b).max() / np.abs(b).max())
+    r = [rel(inter(lam.cpu().numpy()), p.grad.numpy())]
+    r += [rel(inter(b.cpu().numpy()), uu.grad.numpy()) for b, uu in zip(bars, u)]
+    print("adjoint vs autograd (phi, u, v, w):", r)
+    assert max(r) <= 1e-12
+    # ghost entries of lambda are left as given
+    m = np.ones(case.shape, dtype=bool)
+    m[g:-g, g:-g, g:-g] = False
+    assert np.array_equal(lam.cpu().numpy().reshape(case.shape)[m], w[m])
+
+
+def _loss_and_grad(case, phi0, A, target, nsteps, period, every):
+    from fortrancfd_amd.adjoint import advect, l2_loss
+    s = _solver(case)
+    dev = torch.device("cuda")
+    p0 = torch.from_numpy(phi0.reshape(-1).copy()).to(dev).requires_grad_(True)
+    a = torch.tensor(A, dtype=torch.float64, device=dev, requires_grad=True)
+    J = l2_loss(advect(p0, a, s, nsteps, period, every), target, case)
+    J.backward()
+    return float(J), p0.grad.cpu().numpy().reshape(case.shape), float(a.grad)
+
+
+def _loss(case, phi0, A, target, nsteps, period):
+    from fortrancfd_amd.adjoint import advect, l2_loss
+    s = _solver(case)
+    dev = torch.device("cuda")
+    with torch.no_grad():
+        p0 = torch.from_numpy(phi0.reshape(-1).copy()).to(dev)
+        a = torch.tensor(A, dtype=torch.float64, device=dev)
+        return float(l2_loss(advect(p0, a, s, nsteps, period, nsteps), target, case))
+
+
+def test_gradient_against_finite_differences():
+    from fortrancfd_amd.adjoint import advect
+    case = _case((24, 24, 24))
+    nsteps, period, every = 30, 30 * case.dt, 7     # uneven last segment (30 = 4*7 + 2)
+    phi0 = _phi0(case)
+    dev = torch.device("cuda")
+    with torch.no_grad():
+        target = advect(torch.from_numpy(phi0.reshape(-1).copy()).to(dev),
+                        torch.tensor(1.05, dtype=torch.float64, device=dev), _solver(case), nsteps, period, nsteps)
+    J, gphi, gA = _loss_and_grad(case, phi0, 1.0, target, nsteps, period, every)
+    rng = np.random.default_rng(3)
+    v = _smooth(case, rng, 1.0)
+    eps = 1e-6
+    fd_phi = (_loss(case, phi0 + eps * v, 1.0, target, nsteps, period) -
+              _loss(case, phi0 - eps * v, 1.0, target, nsteps, period)) / (2 * eps)
+    ad_phi = float((gphi * v).sum())   # gphi is zero on ghosts: v's ghosts do not count
+    fd_A = (_loss(case, phi0, 1.0 + eps, target, nsteps, period) -
+            _loss(case, phi0, 1.0 - eps, target, nsteps, period)) / (2 * eps)
+    print("J", J, "phi:", ad_phi, fd_phi, "A:", gA, fd_A)
+    assert abs(ad_phi - fd_phi) <= 1e-6 * abs(fd_phi)
+    assert abs(gA - fd_A) <= 1e-6 * abs(fd_A)
+    # ghost cells of phi0 do not enter the advection
+    g = case.ghc
+    m = np.ones(case.shape, dtype=bool)
+    m[g:-g, g:-g, g:-g] = False
+    assert not gphi[m].any()
+
+
+def test_gradient_against_autograd_reference():
+    import adjoint_ref as R
+    from fortrancfd_amd.adjoint import advect, deformation_field
+    case = _case((12, 10, 14))
+    nsteps, period, every = 6, 6 * case.dt, 4
+    phi0 = _phi0(case, r=0.2, c=(0.45, 0.4, 0.5))
+    g = case.ghc
+    inter = lambda a: np.ascontiguousarray(a.reshape(case.shape)[g:-g, g:-g, g:-g])
+    uhat = tuple(torch.from_numpy(inter(u)) for u in deformation_field(case))
+    h = (case.dx, case.dy, case.dz)
+    rng = np.random.default_rng(9)
+    wgt = rng.standard_normal((case.nz, case.ny, case.nx))
+    # reference forward + autograd (CPU, FP64)
+    p0 = torch.from_numpy(inter(phi0)).requires_grad_(True)
+    A = torch.tensor(0.9, dtype=torch.float64, requires_grad=True)
+    pT = R.advect(p0, A, uhat, h, case.dt, nsteps, period)
+    (pT * torch.from_numpy(wgt)).sum().backward()
+    # device forward + hand-written adjoint with the same linear loss
+    dev = torch.device("cuda")
+    q0 = torch.from_numpy(phi0.reshape(-1).copy()).to(dev).requires_grad_(True)
+    B = torch.tensor(0.9, dtype=torch.float64, device=dev, requires_grad=True)
+    qT = advect(q0, B, _solver(case), nsteps, period, every)
+    W = np.zeros(case.shape)
+    W[g:-g, g:-g, g:-g] = wgt
+    (qT * torch.from_numpy(W.reshape(-1)).to(dev)).sum().backward()
+    fwd = np.abs(inter(qT.detach().cpu().numpy()) - pT.detach().numpy()).max() / np.abs(pT.detach().numpy()).max()
+    gp = inter(q0.grad.cpu().numpy())
+    rel = np.abs(gp - p0.grad.numpy()).max() / np.abs(p0.grad.numpy()).max()
+    relA = abs(float(B.grad) - float(A.grad)) / abs(float(A.grad))
+    print("forward", fwd, "grad phi", rel, "grad A", relA)
+    assert fwd <= 1e-13
+    assert rel <= 1e-10
+    assert relA <= 1e-10
