@@ -192,6 +192,79 @@ def poisson_bench(n: int) -> dict:
             "defect correction)", "data": "synthetic", "config": {"workload": f"poisson_{n}^3"}, "mg": res}
 
 
+def adjoint_bench(n: int, nsteps: int = 300, every: int = 20, cfl: float = 0.3) -> dict:
+    """The adjoint advection workload on n^3 cells of [0,1]^3: deformation
+    velocity of amplitude A = 1 and period T = 3, sphere r = 0.15 at
+    (0.35, 0.35, 0.35), nsteps WENO5 SSP-RK3 steps of dt = cfl dx in FP64,
+    J = L2 norm of phi_T - phi_target with phi_target from A = 1.05.
+    Times the forward (checkpoint every `every` steps) and the backward
+    (recompute + hand-written adjoint), and checks dJ/dA and dJ/dphi0 along a
+    smooth direction against central finite differences with eps = 1e-6;
+    then times the same forward in FP32 and reports its deviation from the
+    FP64 phi_T and J."""
+    import torch
+    from fortrancfd_amd import Solver
+    from fortrancfd_amd.adjoint import advect, advect_f32, l2_loss
+    from fortrancfd_amd.types import Case, bc_copy, bubble_phi
+    case = Case(dim=3, nx=n, ny=n, nz=n, xend=1.0, yend=1.0, zend=1.0, cfl=cfl, name="adjoint")
+    s = Solver(case)
+    dev = torch.device("cuda")
+    T = 3.0
+    phi0 = torch.from_numpy(bc_copy(case, bubble_phi(case, 0.15, (0.35, 0.35, 0.35))).reshape(-1)).to(dev)
+
+    def fwd(p, a):
+        with torch.no_grad():
+            return advect(p, torch.tensor(a, dtype=torch.float64, device=dev), s, nsteps, T, nsteps)
+
+    target = fwd(phi0, 1.05)
+    p0 = phi0.clone().requires_grad_(True)
+    A = torch.tensor(1.0, dtype=torch.float64, device=dev, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    phiT = advect(p0, A, s, nsteps, T, every)
+    J = l2_loss(phiT, target, case)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    eps = 1e-6
+    # smooth direction for phi0: a low Fourier mode over the box
+    X, Y, Z = case.coords()
+    v = torch.from_numpy(np.ascontiguousarray(np.sin(np.pi * X) * np.sin(2 * np.pi * Y) * np.cos(np.pi * Z)).reshape(-1)).to(dev)
+    del X, Y, Z
+    loss = lambda p, a: float(l2_loss(fwd(p, a), target, case))
+    fd_A = (loss(phi0, 1.0 + eps) - loss(phi0, 1.0 - eps)) / (2 * eps)
+    fd_v = (loss(phi0 + eps * v, 1.0) - loss(phi0 - eps * v, 1.0)) / (2 * eps)
+    ad_v = float((p0.grad * v).sum())
+    ad_A = float(A.grad)
+    cells = case.ncells
+    # FP32 forward (one untimed step allocates its scratch)
+    advect_f32(phi0, 1.0, s, 1, T)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    p32 = advect_f32(phi0, 1.0, s, nsteps, T)
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    g = case.ghc
+    inner = lambda a: a.detach().reshape(case.shape)[g:-g, g:-g, g:-g].double()
+    err32 = float((inner(p32) - inner(phiT)).abs().max() / inner(phiT).abs().max())
+    J32 = float(l2_loss(p32.double(), target, case))
+    return {"metric": f"adjoint level-set advection {n}^3, {nsteps} steps: gradient vs central FD (eps 1e-6)",
+            "value": max(abs(ad_A - fd_A) / abs(fd_A), abs(ad_v - fd_v) / abs(fd_v)), "unit": "max rel error",
+            "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"adjoint_deformation_{n}^3", "steps": nsteps, "checkpoint_every": every,
+                       "dt_over_dx": cfl, "period": T, "A": 1.0, "A_target": 1.05},
+            "loss": float(J.detach()), "dJdA": ad_A, "fd_dJdA": fd_A, "dJdphi_dir": ad_v, "fd_dJdphi_dir": fd_v,
+            "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+            "forward_Mcell_steps_per_s": round(cells * nsteps / (t1 - t0) / 1e6, 1),
+            "backward_Mcell_steps_per_s": round(cells * nsteps / (t2 - t1) / 1e6, 1),
+            "fp32_forward_s": round(t4 - t3, 3),
+            "fp32_forward_Mcell_steps_per_s": round(cells * nsteps / (t4 - t3) / 1e6, 1),
+            "fp32_phiT_max_rel_dev": err32, "fp32_loss": J32,
+            "fp32_loss_rel_dev": abs(J32 - float(J.detach())) / float(J.detach())}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +275,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--poisson", type=int, default=0)
+    ap.add_argument("--adjoint", type=int, default=0, help="run the adjoint advection workload on N^3 cells")
+    ap.add_argument("--adjoint-steps", type=int, default=300)
     ap.add_argument("--regions-out", default="")
     ap.add_argument("--krylov", type=int, default=1)
     ap.add_argument("--inner", type=int, default=1)
@@ -212,6 +287,9 @@ def main() -> int:
         return 2
     if args.poisson:
         print(json.dumps(poisson_bench(args.poisson)))
+        return 0
+    if args.adjoint:
+        print(json.dumps(adjoint_bench(args.adjoint, args.adjoint_steps)))
         return 0
 
     from fortrancfd_amd import Solver

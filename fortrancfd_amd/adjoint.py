@@ -14,10 +14,14 @@ Step n advances from t_n = n dt with the velocity frozen at t_n (the
 reference's level_set_rk3_solver uses nvel%old over the whole step).
 
 Gradients: with respect to the initial level set (all interior cells) and
-to A.  Since the velocity of step n is A ct_n uhat,
-dJ/dA = sum_n ct_n <ubar_n, uhat>.  The forward keeps the level set of every
+to A.  The fields NU/NV/NW hold A uhat for the whole run and step n uses
+them with the context's velocity scale ct_n (fcfd_set_velocity_scale), so
+the adjoint steps accumulate dJ/d(A uhat) = sum_n ct_n dJ/du_n and
+dJ/dA = <dJ/d(A uhat), uhat>.  The forward keeps the level set of every
 k-th step; the backward restores each segment from its checkpoint, re-runs
 it storing its states, and walks the adjoint steps back through it.
+
+`advect_f32` runs the same forward in FP32 (no gradient).
 """
 from __future__ import annotations
 
@@ -43,6 +47,32 @@ def deformation_field(case: Case) -> Tuple[np.ndarray, ...]:
     return tuple(bc_copy(case, np.ascontiguousarray(a)) for a in (u, v, w))
 
 
+def deformation_field_device(case: Case, device, dtype=None):
+    """deformation_field evaluated on the device with torch (flat ghosted
+    arrays): every ghost cell takes the formula at its clamped interior
+    index, which is the zero-gradient ghost copy."""
+    import torch
+    if case.dim != 3:
+        raise ValueError("the deformation field is the 3D one")
+    dtype = dtype or torch.float64
+    g = case.ghc
+
+    def centres(n, start, h):
+        i = torch.clamp(torch.arange(-g, n + g, device=device, dtype=torch.float64), 0, n - 1)
+        return start + (i + 0.5) * h
+
+    x = centres(case.nx, case.xstart, case.dx).reshape(1, 1, -1)
+    y = centres(case.ny, case.ystart, case.dy).reshape(1, -1, 1)
+    z = centres(case.nz, case.zstart, case.dz).reshape(-1, 1, 1)
+    pi = math.pi
+    sx, sy, sz = torch.sin(pi * x), torch.sin(pi * y), torch.sin(pi * z)
+    s2x, s2y, s2z = torch.sin(2 * pi * x), torch.sin(2 * pi * y), torch.sin(2 * pi * z)
+    u = 2.0 * (sx * sx) * s2y * s2z
+    v = -s2x * (sy * sy) * s2z
+    w = -s2x * s2y * (sz * sz)
+    return tuple(a.to(dtype).contiguous().reshape(-1) for a in (u, v, w))
+
+
 class _Run:
     """Device state shared by the forward and backward of one problem."""
 
@@ -51,33 +81,38 @@ class _Run:
         self.s, self.case = solver, solver.case
         self.nsteps, self.period, self.every = int(nsteps), float(period), max(1, int(every))
         dev = torch.device("cuda")
-        self.uhat = [torch.from_numpy(a.reshape(-1)).to(dev) for a in deformation_field(self.case)]
+        self.uhat = list(deformation_field_device(self.case, dev))
         self.vel = [torch.empty_like(a) for a in self.uhat]
 
     def ct(self, n: int) -> float:
         return math.cos(math.pi * n * self.case.dt / self.period)
 
-    def set_velocity(self, A: float, n: int) -> None:
+    def set_amplitude(self, A: float) -> None:
+        """NU/NV/NW = A uhat (the per-step factor goes in the velocity scale)."""
         import torch
-        f = A * self.ct(n)
         for dst, src in zip(self.vel, self.uhat):
-            torch.mul(src, f, out=dst)
+            torch.mul(src, A, out=dst)
         torch.cuda.synchronize()
         for name, t in zip(("nu", "nv", "nw"), self.vel):
             self.s.push_device(name, t.data_ptr())
 
-    def step(self, A: float, n: int) -> None:
-        self.set_velocity(A, n)
+    def step(self, n: int) -> None:
+        self.s.set_velocity_scale(self.ct(n))
         self.s.level_set_rk3_solver()
 
     def pull_phi(self, out) -> None:
+        """Stream-ordered copy of the solver's phi into `out` (no wait)."""
         self.s.pull_device("phi", out.data_ptr())
-        self.s.sync()
 
     def push_phi(self, t) -> None:
         import torch
         torch.cuda.synchronize()
         self.s.push_device("phi", t.data_ptr())
+
+    def finish(self) -> None:
+        """Wait for the solver's stream and restore the unit velocity scale."""
+        self.s.sync()
+        self.s.set_velocity_scale(1.0)
 
 
 def _function():
@@ -89,6 +124,7 @@ def _function():
         @staticmethod
         def forward(ctx, phi0, A, run: _Run):
             a = float(A.item())
+            run.set_amplitude(a)
             run.push_phi(phi0.detach().contiguous())
             ckpts: List = []
             for n in range(run.nsteps):
@@ -96,18 +132,20 @@ def _function():
                     c = torch.empty_like(phi0)
                     run.pull_phi(c)
                     ckpts.append(c)
-                run.step(a, n)
+                run.step(n)
             out = torch.empty_like(phi0)
             run.pull_phi(out)
+            run.finish()
             ctx.run, ctx.ckpts, ctx.A = run, ckpts, a
             return out
 
         @staticmethod
         def backward(ctx, g):
             run, a = ctx.run, ctx.A
+            run.set_amplitude(a)
             lam = g.detach().clone().contiguous()
-            bars = [torch.zeros_like(u) for u in run.uhat]
-            gA = 0.0
+            bars = [torch.zeros_like(u) for u in run.uhat]   # dJ/d(A uhat), summed over the steps
+            torch.cuda.synchronize()
             nseg = len(ctx.ckpts)
             for seg in range(nseg - 1, -1, -1):
                 start = seg * run.every
@@ -116,20 +154,18 @@ def _function():
                 run.push_phi(ctx.ckpts[seg])
                 states = [ctx.ckpts[seg]]
                 for n in range(start, end - 1):
-                    run.step(a, n)
+                    run.step(n)
                     t = torch.empty_like(lam)
                     run.pull_phi(t)
                     states.append(t)
                 for n in range(end - 1, start - 1, -1):
-                    run.set_velocity(a, n)
-                    for b in bars:
-                        b.zero_()
-                    torch.cuda.synchronize()
+                    run.s.set_velocity_scale(run.ct(n))
                     run.s.ls_adv_adjoint_step(states[n - start].data_ptr(), lam.data_ptr(),
                                               *(b.data_ptr() for b in bars))
-                    run.s.sync()
-                    gA += run.ct(n) * float(sum((b * u).sum() for b, u in zip(bars, run.uhat)))
-            return lam, torch.tensor(gA, dtype=torch.float64, device=g.device), None
+                run.s.sync()   # the segment's states are released after this
+            run.finish()
+            gA = sum((b * u).sum() for b, u in zip(bars, run.uhat))
+            return lam, gA.reshape(()).to(torch.float64), None
 
     return LevelSetAdvection
 
@@ -149,6 +185,26 @@ def advect(phi0, A, solver: Solver, nsteps: int, period: float, every: int = 20)
     shape = phi0.shape
     out = _LSA.apply(phi0.reshape(-1), A, run)
     return out.reshape(shape)
+
+
+def advect_f32(phi0, A: float, solver: Solver, nsteps: int, period: float):
+    """The forward of `advect` in FP32: phi0 (CUDA tensor of the ghosted
+    field) is rounded to float32, advanced nsteps WENO5 SSP-RK3 steps by
+    fcfd_level_set_rk3_solver_f32 with the float32 velocity A uhat times
+    cos(pi n dt / T), and returned as float32.  No gradient."""
+    import torch
+    case = solver.case
+    dev = torch.device("cuda")
+    phi = phi0.detach().reshape(-1).to(torch.float32).contiguous().clone()
+    vel = [(a * float(A)).to(torch.float32) for a in deformation_field_device(case, dev)]
+    ptrs = [v.data_ptr() for v in vel]
+    torch.cuda.synchronize()
+    for n in range(int(nsteps)):
+        solver.set_velocity_scale(math.cos(math.pi * n * case.dt / float(period)))
+        solver.level_set_rk3_solver_f32(phi.data_ptr(), *ptrs)
+    solver.sync()
+    solver.set_velocity_scale(1.0)
+    return phi.reshape(phi0.shape)
 
 
 def l2_loss(phi_T, target, case: Case):

@@ -51,7 +51,8 @@ __global__ void __launch_bounds__(256) k_ls_adv_adj(const double *__restrict__ p
                                                     const double *__restrict__ nv, const double *__restrict__ nw,
                                                     const double *__restrict__ w, const double *__restrict__ base,
                                                     double *__restrict__ out, double *__restrict__ ub,
-                                                    double *__restrict__ vb, double *__restrict__ wb, Geom G) {
+                                                    double *__restrict__ vb, double *__restrict__ wb, Geom G,
+                                                    double vs) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
@@ -74,7 +75,7 @@ __global__ void __launch_bounds__(256) k_ls_adv_adj(const double *__restrict__ p
         for (int o = -5; o <= 5; ++o) {
             const int q = clampi(p0 + o, 0, nd - 1);
             const long a = id + (long)(q - p0) * st[d];
-            const double u = vel[d][a], ph = psi[a];
+            const double u = vs * vel[d][a], ph = psi[a];
             s1[o + 5] = 0.5 * (u + fabs(u)) * ph;
             s2[o + 5] = 0.5 * (u - fabs(u)) * ph;
         }
@@ -98,11 +99,11 @@ __global__ void __launch_bounds__(256) k_ls_adv_adj(const double *__restrict__ p
                 if (clampi(f - 1 + m, 0, nd - 1) == p0) s2b += fb[m];
             }
         }
-        const double u = vel[d][id], ph = psi[id];
+        const double u = vs * vel[d][id], ph = psi[id];
         pb += s1b * 0.5 * (u + fabs(u)) + s2b * 0.5 * (u - fabs(u));
-        if (velb[d]) {
+        if (velb[d]) {   // d/d(stored velocity) = vs d/du
             const double sg = u > 0.0 ? 1.0 : (u < 0.0 ? -1.0 : 0.0);
-            velb[d][id] += (s1b * 0.5 * (1.0 + sg) + s2b * 0.5 * (1.0 - sg)) * ph;
+            velb[d][id] += vs * ((s1b * 0.5 * (1.0 + sg) + s2b * 0.5 * (1.0 - sg)) * ph);
         }
     }
     out[id] = base[id] + pb;
@@ -135,10 +136,10 @@ int ls_adv_adjoint_step(fcfd_ctx *c, const double *phi, double *lambda, double *
     auto adj = [&](const double *psi, const double *base, double *out) {
         if (G.dim == 3)
             hipLaunchKernelGGL(k_ls_adv_adj<3>, grd, blk, 0, c->stream, psi, F[FCFD_NU], F[FCFD_NV], F[FCFD_NW], w,
-                               base, out, ub, vb, wb, G);
+                               base, out, ub, vb, wb, G, c->vscale);
         else
             hipLaunchKernelGGL(k_ls_adv_adj<2>, grd, blk, 0, c->stream, psi, F[FCFD_NU], F[FCFD_NV], nullptr, w,
-                               base, out, ub, vb, nullptr, G);
+                               base, out, ub, vb, nullptr, G, c->vscale);
     };
     // lambda2 = lambda3 + J(phi2)^T (8 dt/12 lambda3)
     hipLaunchKernelGGL(k_adj_seed, grd, blk, 0, c->stream, w, lambda, nullptr, nullptr, 8.0 * dt / 12.0, 0.0, 0.0,
@@ -156,6 +157,14 @@ int ls_adv_adjoint_step(fcfd_ctx *c, const double *phi, double *lambda, double *
 }
 
 }  // namespace fcfd
+
+extern "C" int fcfd_level_set_rk3_solver_f32(fcfd_ctx *ctx, float *phi, const float *nu, const float *nv,
+                                             const float *nw) {
+    if (!ctx || !phi || !nu || !nv || (ctx->G.dim == 3 && !nw)) return FCFD_ERR_ARG;
+    if (ctx->pr.ls_scheme != FCFD_LS_WENO5) return FCFD_ERR_ARG;
+    if (hipSetDevice(ctx->device) != hipSuccess) return FCFD_ERR_HIP;
+    return fcfd::ls_advect_f32(ctx, phi, nu, nv, nw);
+}
 
 extern "C" int fcfd_ls_adv_adjoint_step(fcfd_ctx *ctx, const double *phi, double *lambda, double *ubar,
                                         double *vbar, double *wbar) {

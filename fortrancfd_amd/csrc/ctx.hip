@@ -4,6 +4,7 @@
 // Boundary rules follow `Two phase flows/newOpenMP/3D+IBM/src/others/bcs.f90`
 // (2D: `.../2D/src/others/bcs.f90`).
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 
@@ -19,8 +20,8 @@ namespace fcfd {
 // axes, the final value of a ghost cell is the value of the cell whose
 // indices are clamped to the interior on every axis.  One thread per ghost
 // cell; the ghost shell is enumerated as z slabs, then y slabs, then x slabs.
-template <int DIM>
-__global__ void k_bc_scalar(double *__restrict__ d, Geom G, long nzslab, long nyslab, long total) {
+template <int DIM, typename T>
+__global__ void k_bc_scalar(T *__restrict__ d, Geom G, long nzslab, long nyslab, long total) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
     const int g = G.g, X = G.nx + 2 * g, Y = G.ny + 2 * g;
@@ -53,7 +54,8 @@ __global__ void k_bc_scalar(double *__restrict__ d, Geom G, long nzslab, long ny
     d[G.id(i, j, k)] = d[G.id(ci, cj, ck)];
 }
 
-int bc_scalar(fcfd_ctx *c, double *d) {
+template <typename T>
+static int bc_scalar_t(fcfd_ctx *c, T *d) {
     const Geom &G = c->G;
     const int g = G.g;
     long X = G.nx + 2 * g, Y = G.ny + 2 * g;
@@ -65,12 +67,15 @@ int bc_scalar(fcfd_ctx *c, double *d) {
     long nb = (total + blk - 1) / blk;
     ProfScope ps_(c, R_BC);
     if (G.dim == 3)
-        hipLaunchKernelGGL(k_bc_scalar<3>, dim3((unsigned)nb), dim3(blk), 0, c->stream, d, G, nzslab, nyslab, total);
+        hipLaunchKernelGGL((k_bc_scalar<3, T>), dim3((unsigned)nb), dim3(blk), 0, c->stream, d, G, nzslab, nyslab, total);
     else
-        hipLaunchKernelGGL(k_bc_scalar<2>, dim3((unsigned)nb), dim3(blk), 0, c->stream, d, G, nzslab, nyslab, total);
+        hipLaunchKernelGGL((k_bc_scalar<2, T>), dim3((unsigned)nb), dim3(blk), 0, c->stream, d, G, nzslab, nyslab, total);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
+
+int bc_scalar(fcfd_ctx *c, double *d) { return bc_scalar_t(c, d); }
+int bc_scalar_f32(fcfd_ctx *c, float *d) { return bc_scalar_t(c, d); }
 
 // ------------------------------------------------------------ velbc/nvelbc
 // MAC no-slip (1) / slip (2) rules of velbc (bcs.f90:156-369) and the
@@ -262,6 +267,8 @@ extern "C" int fcfd_destroy(fcfd_ctx *c) {
         if (p) hipFree(p);
     for (auto p : c->adj)
         if (p) hipFree(p);
+    for (auto p : c->adv32)
+        if (p) hipFree(p);
     if (c->red.host) hipHostFree(c->red.host);
     if (c->flag) hipFree(c->flag);
     if (c->flag_host) hipHostFree(c->flag_host);
@@ -345,6 +352,12 @@ extern "C" int fcfd_set_time(fcfd_ctx *c, double time, int iter) {
     if (!c) return FCFD_ERR_ARG;
     c->time = time;
     c->iter = iter;
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_set_velocity_scale(fcfd_ctx *c, double s) {
+    if (!c || !std::isfinite(s)) return FCFD_ERR_ARG;
+    c->vscale = s;
     return FCFD_OK;
 }
 

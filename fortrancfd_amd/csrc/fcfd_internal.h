@@ -103,6 +103,8 @@ struct fcfd_ctx {
     double *un, *vn, *wn;      // predictor output buffer (ping-pong with u, v, w)
     double *ccd_coef[9];       // factorised line matrices [kind * 3 + direction]: CCD, UCCD up, UCCD down
     double *adj[5];            // advection adjoint: stage states phi1, phi2, adjoints lambda1, lambda2, seed
+    float *adv32[4];           // FP32 advection: two stage buffers and the stage sources l1, l2
+    double vscale = 1.0;       // factor on NU/NV/NW in the WENO5 advection and its adjoint
     // ---- reductions ----
     fcfd::Reduce red;
     unsigned long long *flag;  // device flags (reinit error max bits, done, count)
@@ -197,6 +199,7 @@ inline dim3 grid_interior(const Geom &G, dim3 blk) {
 
 // kernels implemented in the .hip files (host wrappers)
 int bc_scalar(fcfd_ctx *c, double *d);
+int bc_scalar_f32(fcfd_ctx *c, float *d);
 int velbc(fcfd_ctx *c, double *u, double *v, double *w, int nodal);
 int ls_props(fcfd_ctx *c, const double *phi, double *H, double *D, double *S, double *rho, double *mu);
 int reduce_sums(fcfd_ctx *c, int kind, const double *a, const double *b, const double *cc, const double *d);
@@ -207,6 +210,7 @@ int surface_norms(fcfd_ctx *c, const double *phi, int full);
 int curv_kernel(fcfd_ctx *c);
 int ls_advect(fcfd_ctx *c);
 int ls_adv_stage(fcfd_ctx *c, const double *phi, double *phin, int stage);
+int ls_advect_f32(fcfd_ctx *c, float *phi, const float *nu, const float *nv, const float *nw);
 int ls_adv_adjoint_step(fcfd_ctx *c, const double *phi, double *lambda, double *ub, double *vb, double *wb);
 int redis(fcfd_ctx *c, int btn, int *iters);
 int mpls(fcfd_ctx *c);

@@ -230,52 +230,55 @@ int mpls(fcfd_ctx *c) {
 // (source_setup :67-96, with the bc() copy of s1/s2 realised by clamped
 // indices), WENO5-JS faces (:162-219), flux divergence (:145-158) and the
 // increment-form stage update (:13-60).
-template <int DIM>
-__global__ void __launch_bounds__(256) k_ls_adv_stage(const double *__restrict__ phi, double *__restrict__ phin,
-                                                      const double *__restrict__ nu, const double *__restrict__ nv,
-                                                      const double *__restrict__ nw, double *__restrict__ l1,
-                                                      double *__restrict__ l2, Geom G, double dt, int stage) {
+// T = float is the single-precision variant (same operation order, FP32
+// weights of weno.h); the velocity is vs (nu, nv, nw).
+template <int DIM, typename T>
+__global__ void __launch_bounds__(256) k_ls_adv_stage(const T *__restrict__ phi, T *__restrict__ phin,
+                                                      const T *__restrict__ nu, const T *__restrict__ nv,
+                                                      const T *__restrict__ nw, T *__restrict__ l1,
+                                                      T *__restrict__ l2, Geom G, T dt, T vs, int stage) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
-    const double *vel[3] = {nu, nv, nw};
+    const T *vel[3] = {nu, nv, nw};
     const int n[3] = {G.nx, G.ny, G.nz};
     const int pos[3] = {i, j, k};
     const long st[3] = {1, G.sx, G.sxy};
-    const double h[3] = {G.dx, G.dy, G.dz};
-    double L = 0.0;
+    const T h[3] = {(T)G.dx, (T)G.dy, (T)G.dz};
+    const T half = (T)0.5;
+    T L = 0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        double s1[7], s2[7];  // positions pos-3 .. pos+3
+        T s1[7], s2[7];  // positions pos-3 .. pos+3
 #pragma unroll
         for (int o = -3; o <= 3; ++o) {
             int q = clampi(pos[d] + o, 0, n[d] - 1);
             long a = id + (long)(q - pos[d]) * st[d];
-            double u = vel[d][a], p = phi[a];
-            s1[o + 3] = 0.5 * (u + fabs(u)) * p;
-            s2[o + 3] = 0.5 * (u - fabs(u)) * p;
+            T u = vs * vel[d][a], p = phi[a];
+            s1[o + 3] = half * (u + fabs(u)) * p;
+            s2[o + 3] = half * (u - fabs(u)) * p;
         }
-        const double EPS = 1.0e-13;
+        const T EPS = (T)1.0e-13;
         // face i+1/2: gm from s1(i-2..i+2), fp from s2(i-1..i+3)
-        double gmR = weno_left(s1[1], s1[2], s1[3], s1[4], s1[5], EPS);
-        double fpR = weno_right(s2[2], s2[3], s2[4], s2[5], s2[6], EPS);
+        T gmR = weno_left(s1[1], s1[2], s1[3], s1[4], s1[5], EPS);
+        T fpR = weno_right(s2[2], s2[3], s2[4], s2[5], s2[6], EPS);
         // face i-1/2
-        double gmL = weno_left(s1[0], s1[1], s1[2], s1[3], s1[4], EPS);
-        double fpL = weno_right(s2[1], s2[2], s2[3], s2[4], s2[5], EPS);
-        double t = -(gmR + fpR - gmL - fpL) / h[d];
+        T gmL = weno_left(s1[0], s1[1], s1[2], s1[3], s1[4], EPS);
+        T fpL = weno_right(s2[1], s2[2], s2[3], s2[4], s2[5], EPS);
+        T t = -(gmR + fpR - gmL - fpL) / h[d];
         L = d == 0 ? t : L + t;
     }
-    double src;
+    T src;
     if (stage == 0) {
         l1[id] = L;
         src = L;
     } else if (stage == 1) {
         l2[id] = L;
-        src = (-3.0 * l1[id] + L) / 4.0;
+        src = ((T)-3.0 * l1[id] + L) / (T)4.0;
     } else {
-        src = (-l1[id] - l2[id] + 8.0 * L) / 12.0;
+        src = (-l1[id] - l2[id] + (T)8.0 * L) / (T)12.0;
     }
     phin[id] = phi[id] + src * dt;
 }
@@ -317,11 +320,11 @@ int ls_adv_stage(fcfd_ctx *c, const double *phi, double *phin, int stage) {
     const Geom &G = c->G;
     dim3 blk(64, 4), grd = grid_interior(G, blk);
     if (G.dim == 3)
-        hipLaunchKernelGGL(k_ls_adv_stage<3>, grd, blk, 0, c->stream, phi, phin, c->f[FCFD_NU], c->f[FCFD_NV],
-                           c->f[FCFD_NW], c->l1, c->l2, G, c->pr.dt, stage);
+        hipLaunchKernelGGL((k_ls_adv_stage<3, double>), grd, blk, 0, c->stream, phi, phin, c->f[FCFD_NU],
+                           c->f[FCFD_NV], c->f[FCFD_NW], c->l1, c->l2, G, c->pr.dt, c->vscale, stage);
     else
-        hipLaunchKernelGGL(k_ls_adv_stage<2>, grd, blk, 0, c->stream, phi, phin, c->f[FCFD_NU], c->f[FCFD_NV],
-                           nullptr, c->l1, c->l2, G, c->pr.dt, stage);
+        hipLaunchKernelGGL((k_ls_adv_stage<2, double>), grd, blk, 0, c->stream, phi, phin, c->f[FCFD_NU],
+                           c->f[FCFD_NV], nullptr, c->l1, c->l2, G, c->pr.dt, c->vscale, stage);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
@@ -331,6 +334,10 @@ int ls_advect(fcfd_ctx *c) {
     dim3 blk(64, 4), grd = grid_interior(G, blk);
     double *a = c->f[FCFD_PHI], *b = c->phiB;
     if (c->pr.ls_scheme == FCFD_LS_UCCD) {
+        if (c->vscale != 1.0) {
+            fprintf(stderr, "libfcfd: the velocity scale applies to the WENO5 scheme only\n");
+            return FCFD_ERR_ARG;
+        }
         // upwind-CCD derivatives park in the predictor's spare velocity buffers
         double *d[3] = {c->un, c->vn, c->wn};
         const double *vel[3] = {c->f[FCFD_NU], c->f[FCFD_NV], c->f[FCFD_NW]};
@@ -357,11 +364,11 @@ int ls_advect(fcfd_ctx *c) {
         {
             ProfScope ps_(c, R_LS_ADV);
             if (G.dim == 3)
-                hipLaunchKernelGGL(k_ls_adv_stage<3>, grd, blk, 0, c->stream, a, b, c->f[FCFD_NU], c->f[FCFD_NV],
-                                   c->f[FCFD_NW], c->l1, c->l2, G, c->pr.dt, s);
+                hipLaunchKernelGGL((k_ls_adv_stage<3, double>), grd, blk, 0, c->stream, a, b, c->f[FCFD_NU],
+                                   c->f[FCFD_NV], c->f[FCFD_NW], c->l1, c->l2, G, c->pr.dt, c->vscale, s);
             else
-                hipLaunchKernelGGL(k_ls_adv_stage<2>, grd, blk, 0, c->stream, a, b, c->f[FCFD_NU], c->f[FCFD_NV],
-                                   nullptr, c->l1, c->l2, G, c->pr.dt, s);
+                hipLaunchKernelGGL((k_ls_adv_stage<2, double>), grd, blk, 0, c->stream, a, b, c->f[FCFD_NU],
+                                   c->f[FCFD_NV], nullptr, c->l1, c->l2, G, c->pr.dt, c->vscale, s);
             FCFD_CHECK(hipGetLastError());
         }
         FCFD_TRY(bc_scalar(c, b));
@@ -370,6 +377,34 @@ int ls_advect(fcfd_ctx *c) {
     // three stages end in the buffer that was phiB: swap roles
     c->phiB = c->f[FCFD_PHI];
     c->f[FCFD_PHI] = a;
+    return FCFD_OK;
+}
+
+// One SSP-RK3 step in FP32 of a caller's level set phi with the caller's
+// FP32 cell velocity times the context's scale: stage 0 phi -> B, stage 1
+// B -> C, stage 2 C -> phi (each stage reads its input and writes another
+// array), the zero-gradient ghost copy after each stage as in ls_advect.
+int ls_advect_f32(fcfd_ctx *c, float *phi, const float *nu, const float *nv, const float *nw) {
+    const Geom &G = c->G;
+    if (!c->adv32[0])
+        for (int q = 0; q < 4; ++q) FCFD_CHECK(hipMalloc((void **)&c->adv32[q], sizeof(float) * G.ntot));
+    float *B = c->adv32[0], *Cb = c->adv32[1], *l1 = c->adv32[2], *l2 = c->adv32[3];
+    dim3 blk(64, 4), grd = grid_interior(G, blk);
+    const float dt = (float)c->pr.dt, vs = (float)c->vscale;
+    float *in[3] = {phi, B, Cb}, *out[3] = {B, Cb, phi};
+    for (int s = 0; s < 3; ++s) {
+        {
+            ProfScope ps_(c, R_LS_ADV);
+            if (G.dim == 3)
+                hipLaunchKernelGGL((k_ls_adv_stage<3, float>), grd, blk, 0, c->stream, in[s], out[s], nu, nv, nw, l1,
+                                   l2, G, dt, vs, s);
+            else
+                hipLaunchKernelGGL((k_ls_adv_stage<2, float>), grd, blk, 0, c->stream, in[s], out[s], nu, nv,
+                                   (const float *)nullptr, l1, l2, G, dt, vs, s);
+            FCFD_CHECK(hipGetLastError());
+        }
+        FCFD_TRY(bc_scalar_f32(c, out[s]));
+    }
     return FCFD_OK;
 }
 

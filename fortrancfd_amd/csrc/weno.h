@@ -52,6 +52,48 @@ __device__ __forceinline__ double weno_right(double f0, double f1, double f2, do
            w1 * (11.0 * f2 - 7.0 * f3 + 2.0 * f4) * r6;
 }
 
+// FP32 forms for the single-precision advection.  The weights use
+// a_k = d_k / q_k: q_k >= EPS^2 = 1e-26 is a normal float, while the product
+// of two q of the FP64 form (>= 1e-52) would underflow to zero where the
+// flux is locally flat (beta = 0 on all three stencils).
+__device__ __forceinline__ void weno_weights(float b1, float b2, float b3, float EPS, float &w1, float &w2,
+                                             float &w3) {
+    const float a1 = 1.0f / ((EPS + b1) * (EPS + b1)), a2 = 6.0f / ((EPS + b2) * (EPS + b2)),
+                a3 = 3.0f / ((EPS + b3) * (EPS + b3));
+    const float r = 1.0f / (a1 + a2 + a3);
+    w1 = a1 * r;
+    w2 = a2 * r;
+    w3 = a3 * r;
+}
+
+__device__ __forceinline__ float weno_left(float g0, float g1, float g2, float g3, float g4, float EPS) {
+    float t1 = g0 - 2.0f * g1 + g2, t2 = g0 - 4.0f * g1 + 3.0f * g2;
+    const float b1 = 13.0f * (t1 * t1) + 3.0f * (t2 * t2);
+    t1 = g1 - 2.0f * g2 + g3; t2 = g1 - g3;
+    const float b2 = 13.0f * (t1 * t1) + 3.0f * (t2 * t2);
+    t1 = g2 - 2.0f * g3 + g4; t2 = 3.0f * g2 - 4.0f * g3 + g4;
+    const float b3 = 13.0f * (t1 * t1) + 3.0f * (t2 * t2);
+    float w1, w2, w3;
+    weno_weights(b1, b2, b3, EPS, w1, w2, w3);
+    constexpr float r6 = 1.0f / 6.0f;
+    return w1 * (1.0f / 3.0f) * g0 - (7.0f * w1 + w2) * r6 * g1 + (11.0f * w1 + 5.0f * w2 + 2.0f * w3) * r6 * g2 +
+           (2.0f * w2 + 5.0f * w3) * r6 * g3 - w3 * r6 * g4;
+}
+
+__device__ __forceinline__ float weno_right(float f0, float f1, float f2, float f3, float f4, float EPS) {
+    float t1 = f0 - 2.0f * f1 + f2, t2 = f0 - 4.0f * f1 + 3.0f * f2;
+    const float b3 = 13.0f * (t1 * t1) + 3.0f * (t2 * t2);
+    t1 = f1 - 2.0f * f2 + f3; t2 = f1 - f3;
+    const float b2 = 13.0f * (t1 * t1) + 3.0f * (t2 * t2);
+    t1 = f2 - 2.0f * f3 + f4; t2 = 3.0f * f2 - 4.0f * f3 + f4;
+    const float b1 = 13.0f * (t1 * t1) + 3.0f * (t2 * t2);
+    float w1, w2, w3;
+    weno_weights(b1, b2, b3, EPS, w1, w2, w3);
+    constexpr float r6 = 1.0f / 6.0f;
+    return w3 * (-f0 + 5.0f * f1 + 2.0f * f2) * r6 + w2 * (2.0f * f1 + 5.0f * f2 - f3) * r6 +
+           w1 * (11.0f * f2 - 7.0f * f3 + 2.0f * f4) * r6;
+}
+
 // ---------------------------------------------------------------- adjoints
 // Reverse mode of weno_weights: given wbar_k = dJ/dw_k, the dJ/dbeta_k.
 // w_k = a_k r with r = 1 / sum a, a1 = q2 q3, a2 = 6 q1 q3, a3 = 3 q1 q2,

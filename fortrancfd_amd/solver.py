@@ -57,6 +57,7 @@ def lib():
         "fcfd_level_set_rk3_solver": [vp], "fcfd_level_set_rk3_redis": [vp, i32, vp],
         "fcfd_mass_preserving_level_set": [vp], "fcfd_surface_norms": [vp], "fcfd_curv": [vp], "fcfd_uccd": [vp],
         "fcfd_ls_adv_adjoint_step": [vp, vp, vp, vp, vp, vp],
+        "fcfd_set_velocity_scale": [vp, dbl], "fcfd_level_set_rk3_solver_f32": [vp, vp, vp, vp, vp],
         "fcfd_ppe_sor_init": [vp], "fcfd_ppe_sor_solver": [vp, dbl, i32, vp, vp],
         "fcfd_ppe_solve_rhs": [vp, dbl, i32, vp, vp],
         "fcfd_ns_momentum_correction": [vp], "fcfd_ns_ab_predictor": [vp],
@@ -187,10 +188,20 @@ class Solver:
     def level_set_rk3_solver(self) -> None:
         self._call("fcfd_level_set_rk3_solver")
 
+    def set_velocity_scale(self, s: float) -> None:
+        """The WENO5 advection (and its FP32 form and adjoint) uses s (NU, NV, NW)."""
+        self._call("fcfd_set_velocity_scale", C.c_double(s))
+
+    def level_set_rk3_solver_f32(self, phi_ptr: int, nu_ptr: int, nv_ptr: int, nw_ptr: int = 0) -> None:
+        """One FP32 SSP-RK3 WENO5 step of the device array phi (ghosted
+        layout, float32) with the float32 cell velocity (device pointers)."""
+        self._call("fcfd_level_set_rk3_solver_f32", C.c_void_p(phi_ptr), C.c_void_p(nu_ptr), C.c_void_p(nv_ptr),
+                   C.c_void_p(nw_ptr or None))
+
     def ls_adv_adjoint_step(self, phi_ptr: int, lam_ptr: int, ub_ptr: int = 0, vb_ptr: int = 0,
                             wb_ptr: int = 0) -> None:
         """Adjoint of one level_set_rk3_solver step at the level set phi
-        (device pointer, ghosted layout) with the current NU/NV/NW: lambda
+        (device pointer, ghosted layout) with the current s (NU, NV, NW): lambda
         (device) goes from dJ/dphi after the step to dJ/dphi before it; the
         velocity adjoint is added into the u/v/w bar arrays when given."""
         self._call("fcfd_ls_adv_adjoint_step", C.c_void_p(phi_ptr), C.c_void_p(lam_ptr),

@@ -71,15 +71,31 @@ int fcfd_ls_mv(fcfd_ctx *ctx, double *mass, double *vol);
  * upwind-CCD form -(u phi_x + v phi_y + w phi_z) (the commented alternative
  * of level_set_rk3_solver.f90:109,123,137,155-157). */
 int fcfd_level_set_rk3_solver(fcfd_ctx *ctx);
+/* Factor s on NU/NV/NW in the WENO5 advection (fcfd_level_set_rk3_solver,
+ * its FP32 form and its adjoint): the velocity of a step is s (NU, NV, NW).
+ * Default 1.  No counterpart in the reference: it lets a prescribed
+ * velocity A c(t) u(x) be set once and modulated per step without copies.
+ * The UCCD scheme rejects s != 1 (FCFD_ERR_ARG at the advection call). */
+int fcfd_set_velocity_scale(fcfd_ctx *ctx, double s);
+/* One SSP-RK3 WENO5 step (the scheme of fcfd_level_set_rk3_solver) in FP32
+ * on the caller's device arrays of the context's ghosted layout: phi (in/out,
+ * ghosts refreshed by the zero-gradient copy after each stage) and the cell
+ * velocity nu, nv[, nw] (times the velocity scale).  dt from the params.  No
+ * counterpart in the reference (its advection is FP64 only); the first call
+ * allocates four FP32 scratch fields in the context.  FCFD_ERR_ARG for the
+ * UCCD scheme. */
+int fcfd_level_set_rk3_solver_f32(fcfd_ctx *ctx, float *phi, const float *nu, const float *nv, const float *nw);
 /* Discrete adjoint of one fcfd_level_set_rk3_solver step (WENO5 scheme, the
- * velocity NU/NV/NW of the context frozen over the step).  No counterpart in
- * the reference: the differentiation entry point of the advection.
+ * velocity s (NU, NV, NW) of the context frozen over the step).  No
+ * counterpart in the reference: the differentiation entry point of the
+ * advection.
  * Device pointers, ghosted field layout:
  *   phi     the level set at the start of the step (interior read);
  *   lambda  in: dJ/dphi after the step; out: dJ/dphi before it (interior
  *           read and written, ghosts untouched);
- *   u/v/wbar  when not NULL, dJ/d(NU, NV, NW) of this step is ADDED to them
- *           (interior).
+ *   u/v/wbar  when not NULL, dJ/d(NU, NV, NW) of this step (the stored
+ *           fields, i.e. s times the derivative with respect to the
+ *           velocity) is ADDED to them (interior).
  * Returns FCFD_ERR_ARG for the UCCD scheme. */
 int fcfd_ls_adv_adjoint_step(fcfd_ctx *ctx, const double *phi, double *lambda, double *ubar, double *vbar,
                              double *wbar);

@@ -47,3 +47,15 @@ def test_reference_gradient_is_linear_in_the_seed():
 
     g = grad(w1 + 2.0 * w2) - grad(w1) - 2.0 * grad(w2)
     assert float(g.abs().max()) <= 1e-12 * float(grad(w1).abs().max())
+
+
+def test_device_deformation_field_is_the_ghost_copied_formula():
+    """deformation_field_device (torch, clamped-index evaluation) equals the
+    numpy field with the zero-gradient ghost copy, on the CPU."""
+    from fortrancfd_amd.adjoint import deformation_field, deformation_field_device
+    from fortrancfd_amd.types import Case
+    case = Case(dim=3, nx=10, ny=7, nz=9, xend=1.0, yend=0.7, zend=0.9, cfl=0.3, name="adv")
+    for a, b in zip(deformation_field(case), deformation_field_device(case, "cpu")):
+        b = b.numpy().reshape(case.shape)
+        assert np.abs(a - b).max() <= 4e-16 * np.abs(a).max()
+        assert np.array_equal(b[0], b[case.ghc]) and np.array_equal(b[:, :, -1], b[:, :, -case.ghc - 1])

@@ -14,7 +14,11 @@ and the torch.autograd.Function of fortrancfd_amd.adjoint).
   1e-6 relative;
 * the gradient fields against an independent FP64 reference gradient:
   torch.autograd of the same forward restated on the CPU
-  (tests/adjoint_ref.py), whose forward is checked against the device first.
+  (tests/adjoint_ref.py), whose forward is checked against the device first;
+* the velocity scale (a step with s (NU, NV, NW) equals a step with the
+  scaled fields, bit for bit when the scaling is exact) and the FP32
+  forward (fcfd_level_set_rk3_solver_f32) against the FP64 path and against
+  the restatement of tests/adjoint_ref.py.
 """
 import math
 
@@ -213,3 +217,49 @@ def test_gradient_against_autograd_reference():
     assert fwd <= 1e-13
     assert rel <= 1e-10
     assert relA <= 1e-10
+
+
+def test_velocity_scale_is_a_factor_on_the_velocity():
+    from fortrancfd_amd.adjoint import deformation_field
+    case = _case()
+    s = _solver(case)
+    phi = _phi0(case)
+    vel = deformation_field(case)
+    ref = _step(s, case, phi, [0.7 * u for u in vel])
+    s.set_velocity_scale(0.5)
+    half = _step(s, case, phi, [1.4 * u for u in vel])    # 0.5 (1.4 u) and 0.7 u differ by rounding
+    s.set_velocity_scale(-1.0)
+    neg = _step(s, case, phi, [-0.7 * u for u in vel])     # exact: a sign flip
+    s.set_velocity_scale(1.0)
+    assert np.array_equal(neg, ref)
+    assert np.abs(half - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+def test_fp32_forward_against_fp64():
+    import adjoint_ref as R
+    from fortrancfd_amd.adjoint import advect, advect_f32, deformation_field
+    case = _case()
+    g = case.ghc
+    inter = lambda a: np.asarray(a, dtype=np.float64).reshape(case.shape)[g:-g, g:-g, g:-g]
+    s = _solver(case)
+    dev = torch.device("cuda")
+    phi0 = _phi0(case)
+    nsteps, period = 12, 3.0
+    p64 = advect(torch.from_numpy(phi0.reshape(-1).copy()).to(dev), torch.tensor(1.0, dtype=torch.float64, device=dev),
+                 s, nsteps, period, nsteps).cpu().numpy()
+    p32 = advect_f32(torch.from_numpy(phi0.reshape(-1).copy()).to(dev), 1.0, s, nsteps, period).cpu().numpy()
+    assert p32.dtype == np.float32
+    scale = np.abs(inter(p64)).max()
+    err = np.abs(inter(p32) - inter(p64)).max() / scale
+    # the restatement in FP32 on the CPU (same formulas, its own rounding)
+    uhat = tuple(torch.from_numpy(np.ascontiguousarray(inter(u))).float() for u in deformation_field(case))
+    pr = R.advect(torch.from_numpy(np.ascontiguousarray(inter(phi0))).float(), torch.tensor(1.0), uhat,
+                  (case.dx, case.dy, case.dz), case.dt, nsteps, period).numpy()
+    err_r = np.abs(inter(p32) - pr).max() / scale
+    print("FP32 vs FP64 device:", err, " FP32 device vs FP32 restatement:", err_r)
+    assert err <= 2e-6
+    assert err_r <= 2e-6
+    # ghosts hold the zero-gradient copy
+    full = p32.reshape(case.shape)
+    assert np.array_equal(full[:, :, 0], full[:, :, g])
+    assert np.array_equal(full[:, :, -1], full[:, :, -g - 1])
