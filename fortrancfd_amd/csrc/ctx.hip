@@ -50,7 +50,8 @@ __global__ void k_bc_scalar(T *__restrict__ d, Geom G, long nzslab, long nyslab,
         int layer = (int)(r % (2 * g));
         i = layer < g ? layer - g : G.nx + (layer - g);
     }
-    int ci = clampi(i, 0, G.nx - 1), cj = clampi(j, 0, G.ny - 1), ck = DIM == 3 ? clampi(k, 0, G.nz - 1) : 0;
+    // 2D: k is the case of a batched stack (0 otherwise), which has no z ghosts
+    int ci = clampi(i, 0, G.nx - 1), cj = clampi(j, 0, G.ny - 1), ck = DIM == 3 ? clampi(k, 0, G.nz - 1) : k;
     d[G.id(i, j, k)] = d[G.id(ci, cj, ck)];
 }
 
@@ -178,20 +179,37 @@ static int alloc_field(double **p, long n) {
     return FCFD_OK;
 }
 
-extern "C" int fcfd_create(const fcfd_grid *grid, const fcfd_params *params, fcfd_ctx **out) {
+namespace fcfd {
+// per-case parameter table of a batched stack (nothing for nb == 1, whose
+// kernels take the scalars of the params)
+int upload_cases(fcfd_ctx *c) {
+    if (c->nb == 1) return FCFD_OK;
+    if (!c->cases_dev) FCFD_CHECK(hipMalloc((void **)&c->cases_dev, sizeof(fcfd_case) * c->nb));
+    FCFD_CHECK(hipMemcpyAsync(c->cases_dev, c->cases.data(), sizeof(fcfd_case) * c->nb, hipMemcpyHostToDevice,
+                              c->stream));
+    return FCFD_OK;
+}
+}  // namespace fcfd
+
+// an ordinary context (ncases = 0) or a batched 2D stack of ncases cases
+static int create_impl(const fcfd_grid *grid, const fcfd_params *params, int ncases, const fcfd_case *cases,
+                       fcfd_ctx **out) {
     if (!grid || !params || !out) return FCFD_ERR_ARG;
     if (grid->dim != 2 && grid->dim != 3) return FCFD_ERR_ARG;
     if (grid->ghc != 3) return FCFD_ERR_ARG;
     if (grid->nx < 4 || grid->ny < 4) return FCFD_ERR_ARG;
     if (grid->dim == 3 && grid->nz < 4) return FCFD_ERR_ARG;
     if (grid->dim == 2 && grid->nz != 1) return FCFD_ERR_ARG;
+    if (ncases > 0 && (grid->dim != 2 || !cases)) return FCFD_ERR_ARG;
     fcfd_ctx *c = new fcfd_ctx();
     memset(c->f, 0, sizeof(c->f));
     c->gd = *grid;
     c->pr = *params;
+    c->nb = ncases > 0 ? ncases : 1;
+    if (ncases > 0) c->cases.assign(cases, cases + ncases);
     Geom &G = c->G;
     G.dim = grid->dim;
-    G.nx = grid->nx; G.ny = grid->ny; G.nz = grid->nz;
+    G.nx = grid->nx; G.ny = grid->ny; G.nz = ncases > 0 ? ncases : grid->nz;
     G.g = grid->ghc; G.gz = grid->dim == 3 ? grid->ghc : 0;
     G.sx = G.nx + 2 * G.g;
     G.sxy = G.sx * (G.ny + 2 * G.g);
@@ -219,22 +237,29 @@ extern "C" int fcfd_create(const fcfd_grid *grid, const fcfd_params *params, fcf
         if (rc != FCFD_OK) break;
         rc = alloc_field(p, n);
     }
-    // reduction partials: 8 slots x 1024 blocks, or two slots per block of an
-    // interior launch with 64x4 blocks, whichever is larger
+    // reduction partials: 8 slots x 1024 blocks per case, or two slots per
+    // block of an interior launch with 64x4 blocks, whichever is larger
+    const long nbc = c->nb;
     {
         long nbi = (long)((G.nx + 63) / 64) * ((G.ny + 3) / 4) * G.nz;
-        long need = std::max(8L * 1024, 2 * nbi + 16);
+        long need = std::max(8L * 1024 * nbc, 2 * nbi + 16);
         c->red.nblocks = (int)((need + 7) / 8);
     }
-    // + 4 slots x 256 second-stage partials of the level-0 reductions
-    if (rc == FCFD_OK) rc = alloc_field(&c->red.partial, 8L * c->red.nblocks + 4 * 256);
+    // + 4 slots x 256 second-stage partials per case of the level-0 reductions
+    if (rc == FCFD_OK) rc = alloc_field(&c->red.partial, 8L * c->red.nblocks + 4 * 256 * nbc);
     if (rc == FCFD_OK) c->red.partial2 = c->red.partial + 8L * c->red.nblocks;
-    if (rc == FCFD_OK) rc = alloc_field(&c->red.result, 16);
-    if (rc == FCFD_OK && hipHostMalloc((void **)&c->red.host, 16 * sizeof(double)) != hipSuccess) rc = FCFD_ERR_HIP;
-    if (rc == FCFD_OK && hipMalloc((void **)&c->flag, (FLAG_SHARD0 + NSHARD) * sizeof(unsigned long long)) != hipSuccess)
+    if (rc == FCFD_OK) rc = alloc_field(&c->red.result, RES_STRIDE * nbc);
+    if (rc == FCFD_OK && hipHostMalloc((void **)&c->red.host, 64 * nbc * sizeof(double)) != hipSuccess)
         rc = FCFD_ERR_HIP;
-    if (rc == FCFD_OK && hipHostMalloc((void **)&c->flag_host, 16 * sizeof(unsigned long long)) != hipSuccess)
+    if (rc == FCFD_OK && hipHostMalloc((void **)&c->diag_host, 2 * nbc * sizeof(double)) != hipSuccess)
         rc = FCFD_ERR_HIP;
+    if (rc == FCFD_OK) memset(c->diag_host, 0, 2 * nbc * sizeof(double));
+    if (rc == FCFD_OK && hipMalloc((void **)&c->flag, FLAG_STRIDE * nbc * sizeof(unsigned long long)) != hipSuccess)
+        rc = FCFD_ERR_HIP;
+    if (rc == FCFD_OK &&
+        hipHostMalloc((void **)&c->flag_host, FLAG_STRIDE * nbc * sizeof(unsigned long long)) != hipSuccess)
+        rc = FCFD_ERR_HIP;
+    if (rc == FCFD_OK) rc = upload_cases(c);
     if (rc == FCFD_OK) rc = ccd_setup(c);
     c->rho32 = nullptr;
     c->mg_scalars = nullptr;
@@ -250,6 +275,30 @@ extern "C" int fcfd_create(const fcfd_grid *grid, const fcfd_params *params, fcf
         return rc;
     }
     *out = c;
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_create(const fcfd_grid *grid, const fcfd_params *params, fcfd_ctx **out) {
+    return create_impl(grid, params, 0, nullptr, out);
+}
+
+extern "C" int fcfd_create_batch(const fcfd_grid *grid, const fcfd_params *params, int ncases,
+                                 const fcfd_case *cases, fcfd_ctx **out) {
+    if (ncases < 1) return FCFD_ERR_ARG;
+    return create_impl(grid, params, ncases, cases, out);
+}
+
+extern "C" int fcfd_batch_size(const fcfd_ctx *c) { return c ? c->nb : 0; }
+
+extern "C" int fcfd_set_cases(fcfd_ctx *c, const fcfd_case *cases) {
+    if (!c || !cases || c->nb == 1) return FCFD_ERR_ARG;
+    c->cases.assign(cases, cases + c->nb);
+    return upload_cases(c);
+}
+
+extern "C" int fcfd_get_cases(const fcfd_ctx *c, fcfd_case *cases) {
+    if (!c || !cases || c->nb == 1) return FCFD_ERR_ARG;
+    std::copy(c->cases.begin(), c->cases.end(), cases);
     return FCFD_OK;
 }
 
@@ -270,6 +319,8 @@ extern "C" int fcfd_destroy(fcfd_ctx *c) {
     for (auto p : c->adv32)
         if (p) hipFree(p);
     if (c->red.host) hipHostFree(c->red.host);
+    if (c->diag_host) hipHostFree(c->diag_host);
+    if (c->cases_dev) hipFree(c->cases_dev);
     if (c->flag) hipFree(c->flag);
     if (c->flag_host) hipHostFree(c->flag_host);
     for (hipEvent_t e : c->prof.pool) hipEventDestroy(e);
@@ -338,13 +389,24 @@ extern "C" int fcfd_get_diag(fcfd_ctx *c, fcfd_diag *out) {
     if (!c || !out) return FCFD_ERR_ARG;
     FCFD_CHECK(hipStreamSynchronize(c->stream));
     if (c->diag_pending) {
-        c->diag.mass = c->red.host[8];
-        c->diag.vol = c->red.host[9];
+        c->diag.mass = c->diag_host[0];
+        c->diag.vol = c->diag_host[1];
         c->diag_pending = 0;
     }
     *out = c->diag;
     out->iter = c->iter;
     out->imass = c->pr.imass;
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_get_case_diag(fcfd_ctx *c, double *mass, double *vol, double *imass) {
+    if (!c) return FCFD_ERR_ARG;
+    FCFD_CHECK(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < c->nb; ++q) {
+        if (mass) mass[q] = c->diag_host[2 * q];
+        if (vol) vol[q] = c->diag_host[2 * q + 1];
+        if (imass) imass[q] = c->nb > 1 ? c->cases[q].imass : c->pr.imass;
+    }
     return FCFD_OK;
 }
 

@@ -105,6 +105,10 @@ struct fcfd_ctx {
     double *adj[5];            // advection adjoint: stage states phi1, phi2, adjoints lambda1, lambda2, seed
     float *adv32[4];           // FP32 advection: two stage buffers and the stage sources l1, l2
     double vscale = 1.0;       // factor on NU/NV/NW in the WENO5 advection and its adjoint
+    // ---- batched 2D stack (fcfd_create_batch): case q is the slice k = q ----
+    int nb = 1;                        // cases in the stack; 1 for an ordinary context
+    fcfd_case *cases_dev = nullptr;    // per-case parameters on the device; NULL when nb == 1
+    std::vector<fcfd_case> cases;      // host copy
     // ---- reductions ----
     fcfd::Reduce red;
     unsigned long long *flag;  // device flags (reinit error max bits, done, count)
@@ -127,7 +131,8 @@ struct fcfd_ctx {
     // ---- bookkeeping ----
     double time;
     int iter;
-    int diag_pending;          // step mass/volume waiting in red.host[8..9]
+    int diag_pending;          // step mass/volume of every case waiting in diag_host
+    double *diag_host;         // pinned: mass, volume of case q at [2q], [2q+1]
     fcfd_diag diag;
     fcfd::Prof prof;
 };
@@ -189,8 +194,16 @@ __device__ __forceinline__ double block_max256(double v, double *sh4) {
 }
 
 // device flag words: [0] unused, [1] reinit done, [2] reinit pseudo-step
-// count, [3] max |grad phi| bits; [FLAG_SHARD0, +NSHARD) per-block max slots
-constexpr int FLAG_SHARD0 = 16, NSHARD = 256;
+// count, [3] max |grad phi| bits; [FLAG_SHARD0, +NSHARD) per-block max slots.
+// A batched stack holds one such block of FLAG_STRIDE words per case.
+constexpr int FLAG_SHARD0 = 16, NSHARD = 256, FLAG_STRIDE = FLAG_SHARD0 + NSHARD;
+// reduced values per case in Reduce::result (mass sums, MPLS lambda, diag)
+constexpr int RES_STRIDE = 16;
+
+// case index of slice k: a batched 2D stack keeps case q in slice k = q;
+// 3D contexts hold one case
+template <int DIM>
+__device__ __forceinline__ int case_of(int k) { return DIM == 2 ? k : 0; }
 
 // launch helpers --------------------------------------------------------
 inline dim3 grid_interior(const Geom &G, dim3 blk) {
@@ -198,6 +211,7 @@ inline dim3 grid_interior(const Geom &G, dim3 blk) {
 }
 
 // kernels implemented in the .hip files (host wrappers)
+int upload_cases(fcfd_ctx *c);
 int bc_scalar(fcfd_ctx *c, double *d);
 int bc_scalar_f32(fcfd_ctx *c, float *d);
 int velbc(fcfd_ctx *c, double *u, double *v, double *w, int nodal);

@@ -13,6 +13,8 @@
 // the reference fills and then ghost-copies with bc() (the flux split s1/s2,
 // the backward differences, the lambda numerator/denominator) are recomputed
 // or read with clamped indices instead of being stored with ghosts.
+#include <algorithm>
+
 #include "fcfd_internal.h"
 #include "weno.h"
 
@@ -41,11 +43,17 @@ __device__ __forceinline__ void lsfun(double phi, double w, double &H, double &D
 
 // ------------------------------------------------------------- properties
 // ls_funs + rho_mu over the full ghosted box (others.f90:56-120)
+// (cp != NULL: a batched 2D stack, case t / slab)
 __global__ void k_ls_props(const double *__restrict__ phi, double *__restrict__ H, double *__restrict__ D,
                            double *__restrict__ S, double *__restrict__ R, double *__restrict__ M, long n,
-                           double w, double r12, double m12) {
+                           double w, double r12, double m12, const fcfd_case *__restrict__ cp, long slab) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
+    if (cp) {
+        const fcfd_case &q = cp[t / slab];
+        r12 = q.rho_12;
+        m12 = q.mu_12;
+    }
     double h, d;
     lsfun(phi[t], w, h, d);
     if (H) H[t] = h;
@@ -59,7 +67,7 @@ int ls_props(fcfd_ctx *c, const double *phi, double *H, double *D, double *S, do
     long n = c->G.ntot;
     ProfScope ps_(c, R_MISC);
     hipLaunchKernelGGL(k_ls_props, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, phi, H, D, S, rho, mu,
-                       n, c->pr.ls_wid, c->pr.rho_12, c->pr.mu_12);
+                       n, c->pr.ls_wid, c->pr.rho_12, c->pr.mu_12, c->cases_dev, c->G.sxy);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
@@ -92,12 +100,16 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[NS], double *parti
         for (int s = 0; s < NS; ++s) partial[s * nblocks + blockIdx.x] = sh[s][0];
 }
 
-// second pass: result[s] = sum_b partial[s*nb + b]  (one block of RTHR threads)
+// second pass, one block of RTHR threads per case q (blockIdx.x):
+// result[q*RES_STRIDE + s] = sum_b partial[s*nb*ncase + q*nb + b]
 __global__ void k_reduce_final(const double *__restrict__ partial, double *__restrict__ result, int ns, int nb) {
     __shared__ double sh[RTHR];
+    const long ntot = (long)nb * gridDim.x;
+    partial += (long)blockIdx.x * nb;
+    result += (long)blockIdx.x * RES_STRIDE;
     for (int s = 0; s < ns; ++s) {
         double a = 0.0;
-        for (int b = threadIdx.x; b < nb; b += RTHR) a += partial[s * nb + b];
+        for (int b = threadIdx.x; b < nb; b += RTHR) a += partial[s * ntot + b];
         sh[threadIdx.x] = a;
         __syncthreads();
         for (int w = RTHR / 2; w > 0; w >>= 1) {
@@ -112,18 +124,23 @@ __global__ void k_reduce_final(const double *__restrict__ partial, double *__res
 // mass / volume (ls_mv, others.f90:129-152) and, with nrm != null, the MPLS
 // lambda denominator sum delta^2 |grad phi| (2(1-rho12)H + rho12) dV with
 // |grad phi| from the CCD normals, written to grad
-// (mass_preserving_level_set.f90:17-34).
+// (mass_preserving_level_set.f90:17-34).  blockIdx.y is the case of a
+// batched 2D stack (its slice; cp holds its rho_12), 0 otherwise; each case
+// reduces over its own RBLK blocks in the same order as a single case.
 template <int DIM>
 __global__ void k_mass_sums(const double *__restrict__ phi, const double *__restrict__ nx_,
                             const double *__restrict__ ny_, const double *__restrict__ nz_,
                             double *__restrict__ grad, double *__restrict__ partial, Geom G, double w,
-                            double r12, int with_lam) {
+                            double r12, int with_lam, const fcfd_case *__restrict__ cp) {
     double acc[3] = {0.0, 0.0, 0.0};
-    const long n = (long)G.nx * G.ny * G.nz;
+    const int q = blockIdx.y;
+    if (cp) r12 = cp[q].rho_12;
+    const long n = DIM == 3 ? (long)G.nx * G.ny * G.nz : (long)G.nx * G.ny;
     const double dv = DIM == 3 ? G.dx * G.dy * G.dz : G.dx * G.dy;
     for (long t = (long)blockIdx.x * RTHR + threadIdx.x; t < n; t += (long)RBLK * RTHR) {
         int i, j, k;
         cell_of(t, G, i, j, k);
+        if (DIM == 2) k = q;
         long id = G.id(i, j, k);
         double h, d;
         lsfun(phi[id], w, h, d);
@@ -145,20 +162,24 @@ __global__ void k_mass_sums(const double *__restrict__ phi, const double *__rest
             acc[2] += plam;
         }
     }
-    block_sum_store<3>(acc, partial, RBLK);
+    block_sum_store<3>(acc, partial + (long)q * RBLK, RBLK * gridDim.y);
 }
 
 // phi += lambda delta |grad phi|, lambda = (imass - mass) / denominator
 // computed here from the reduced sums (mass_preserving_level_set.f90:36-48)
+// (per case of a batched stack: its sums, its imass from cp, its lambda slot)
 template <int DIM>
 __global__ void k_mpls_update(double *__restrict__ phi, const double *__restrict__ grad,
                               const double *__restrict__ sums, double *__restrict__ lam_out, Geom G, double w,
-                              double imass) {
+                              double imass, const fcfd_case *__restrict__ cp) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
+    const int q = case_of<DIM>(k);
+    sums += (long)q * RES_STRIDE;
+    if (cp) imass = cp[q].imass;
     double lam = (imass - sums[0]) / sums[2];
-    if (i == 0 && j == 0 && k == 0) *lam_out = lam;
+    if (i == 0 && j == 0 && (DIM == 2 || k == 0)) lam_out[(long)q * RES_STRIDE] = lam;
     if (i >= G.nx || j >= G.ny) return;
     long id = G.id(i, j, k);
     double h, d;
@@ -171,12 +192,15 @@ int mass_sums(fcfd_ctx *c, const double *phi, int with_lam) {
     ProfScope ps_(c, R_MASS);
     if (G.dim == 3)
         hipLaunchKernelGGL(k_mass_sums<3>, dim3(RBLK), dim3(RTHR), 0, c->stream, phi, c->f[FCFD_NX], c->f[FCFD_NY],
-                           c->f[FCFD_NZ], c->f[FCFD_GRAD], c->red.partial, G, c->pr.ls_wid, c->pr.rho_12, with_lam);
+                           c->f[FCFD_NZ], c->f[FCFD_GRAD], c->red.partial, G, c->pr.ls_wid, c->pr.rho_12, with_lam,
+                           (const fcfd_case *)nullptr);
     else
-        hipLaunchKernelGGL(k_mass_sums<2>, dim3(RBLK), dim3(RTHR), 0, c->stream, phi, c->f[FCFD_NX], c->f[FCFD_NY],
-                           nullptr, c->f[FCFD_GRAD], c->red.partial, G, c->pr.ls_wid, c->pr.rho_12, with_lam);
+        hipLaunchKernelGGL(k_mass_sums<2>, dim3(RBLK, c->nb), dim3(RTHR), 0, c->stream, phi, c->f[FCFD_NX],
+                           c->f[FCFD_NY], nullptr, c->f[FCFD_GRAD], c->red.partial, G, c->pr.ls_wid, c->pr.rho_12,
+                           with_lam, c->cases_dev);
     FCFD_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RTHR), 0, c->stream, c->red.partial, c->red.result, 3, RBLK);
+    hipLaunchKernelGGL(k_reduce_final, dim3(c->nb), dim3(RTHR), 0, c->stream, c->red.partial, c->red.result, 3,
+                       RBLK);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
@@ -187,11 +211,13 @@ int fetch_result(fcfd_ctx *c, int n) {
     return FCFD_OK;
 }
 
+// ls_mv; mass and vol of case 0 (a batched stack leaves every case's pair at
+// red.host[q*RES_STRIDE], [q*RES_STRIDE + 1])
 int ls_mv(fcfd_ctx *c, double *mass, double *vol) {
     FCFD_TRY(ls_props(c, c->f[FCFD_PHI], c->f[FCFD_HEAVY], c->f[FCFD_DELTA], c->f[FCFD_SIGN], c->f[FCFD_RHO],
                       c->f[FCFD_MU]));
     FCFD_TRY(mass_sums(c, c->f[FCFD_PHI], 0));
-    FCFD_TRY(fetch_result(c, 2));
+    FCFD_TRY(fetch_result(c, RES_STRIDE * c->nb));
     c->diag.mass = c->red.host[0];
     c->diag.vol = c->red.host[1];
     if (mass) *mass = c->red.host[0];
@@ -214,10 +240,10 @@ int mpls(fcfd_ctx *c) {
             ProfScope ps_(c, R_MPLS_UPD);
             if (G.dim == 3)
                 hipLaunchKernelGGL(k_mpls_update<3>, grd, blk, 0, c->stream, phi, c->f[FCFD_GRAD], c->red.result,
-                                   lam_dev, G, c->pr.ls_wid, c->pr.imass);
+                                   lam_dev, G, c->pr.ls_wid, c->pr.imass, (const fcfd_case *)nullptr);
             else
                 hipLaunchKernelGGL(k_mpls_update<2>, grd, blk, 0, c->stream, phi, c->f[FCFD_GRAD], c->red.result,
-                                   lam_dev, G, c->pr.ls_wid, c->pr.imass);
+                                   lam_dev, G, c->pr.ls_wid, c->pr.imass, c->cases_dev);
             FCFD_CHECK(hipGetLastError());
         }
         FCFD_TRY(bc_scalar(c, phi));
@@ -419,10 +445,10 @@ __global__ void __launch_bounds__(256) k_redis_a(const double *__restrict__ phi,
                                                  double *__restrict__ grad, double *__restrict__ s2,
                                                  double *__restrict__ s3, const unsigned long long *__restrict__ done,
                                                  Geom G, double w, double r12, int btn) {
-    if (done && done[1]) return;
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
+    if (done && done[(long)case_of<DIM>(k) * FLAG_STRIDE + 1]) return;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
     const int n[3] = {G.nx, G.ny, G.nz};
@@ -470,10 +496,11 @@ __global__ void __launch_bounds__(256) k_redis_b(double *__restrict__ phi, const
                                                  double *__restrict__ l2, const double *__restrict__ phis,
                                                  unsigned long long *__restrict__ flags, Geom G, double w,
                                                  double rdt, int btn, int stage) {
-    if (flags[1]) return;
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
+    flags += (long)case_of<DIM>(k) * FLAG_STRIDE;   // this case's flag block (a block covers one slice)
+    if (flags[1]) return;
     bool in = i < G.nx && j < G.ny;
     double err = 0.0;
     if (in) {
@@ -487,7 +514,7 @@ __global__ void __launch_bounds__(256) k_redis_b(double *__restrict__ phi, const
             double a = wc * s2[id], b = wc * s3[id];
             const int kl = DIM == 3 ? -1 : 0, kh = DIM == 3 ? 1 : 0;
             for (int kk = kl; kk <= kh; ++kk) {
-                int kq = DIM == 3 ? clampi(k + kk, 0, G.nz - 1) : 0;
+                int kq = DIM == 3 ? clampi(k + kk, 0, G.nz - 1) : k;
                 for (int jj = -1; jj <= 1; ++jj) {
                     int jq = clampi(j + jj, 0, G.ny - 1);
                     for (int ii = -1; ii <= 1; ++ii) {
@@ -523,9 +550,10 @@ __global__ void __launch_bounds__(256) k_redis_b(double *__restrict__ phi, const
 }
 
 // end of one pseudo step: count it, stop when the max over the shards is
-// <= tol, clear the shards (one block of NSHARD threads)
+// <= tol, clear the shards (one block of NSHARD threads per case)
 __global__ void __launch_bounds__(NSHARD) k_redis_check(unsigned long long *flags, double tol) {
     __shared__ double sh[4];
+    flags += (long)blockIdx.x * FLAG_STRIDE;
     if (flags[1]) return;
     const int t = threadIdx.x;
     const double e = __longlong_as_double((long long)flags[FLAG_SHARD0 + t]);
@@ -539,10 +567,10 @@ __global__ void __launch_bounds__(NSHARD) k_redis_check(unsigned long long *flag
 
 __global__ void k_copy_interior(double *__restrict__ dst, const double *__restrict__ src,
                                 const unsigned long long *__restrict__ done, Geom G) {
-    if (done && done[1]) return;
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
+    if (done && done[(G.dim == 2 ? (long)k : 0L) * FLAG_STRIDE + 1]) return;
     if (i >= G.nx || j >= G.ny) return;
     long id = G.id(i, j, k);
     dst[id] = src[id];
@@ -572,7 +600,9 @@ __global__ void k_max_interior(const double *__restrict__ a, unsigned long long 
     double v = 0.0;
     if (i < G.nx && j < G.ny) v = a[G.id(i, j, k)];
     v = block_max256(v, sh);
-    if (threadIdx.x + threadIdx.y == 0) atomicMax(&flags[3], (unsigned long long)__double_as_longlong(v));
+    const long q = G.dim == 2 ? k : 0;   // the case of a batched stack
+    if (threadIdx.x + threadIdx.y == 0)
+        atomicMax(&flags[q * FLAG_STRIDE + 3], (unsigned long long)__double_as_longlong(v));
 }
 
 __global__ void k_scale_by_flag(double *__restrict__ phi, const unsigned long long *flags, Geom G) {
@@ -580,7 +610,7 @@ __global__ void k_scale_by_flag(double *__restrict__ phi, const unsigned long lo
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
-    double gmax = __longlong_as_double((long long)flags[3]);
+    double gmax = __longlong_as_double((long long)flags[(G.dim == 2 ? (long)k : 0L) * FLAG_STRIDE + 3]);
     long id = G.id(i, j, k);
     phi[id] = phi[id] / gmax;
 }
@@ -590,7 +620,7 @@ int redis(fcfd_ctx *c, int btn, int *iters) {
     dim3 blk(64, 4), grd = grid_interior(G, blk);
     double *phi = c->f[FCFD_PHI], *grad = c->f[FCFD_GRAD];
     const double w = c->pr.ls_wid, r12 = c->pr.rho_12, rdt = c->pr.rdt;
-    FCFD_CHECK(hipMemsetAsync(c->flag, 0, (FLAG_SHARD0 + NSHARD) * sizeof(unsigned long long), c->stream));
+    FCFD_CHECK(hipMemsetAsync(c->flag, 0, (size_t)FLAG_STRIDE * c->nb * sizeof(unsigned long long), c->stream));
     // level_set_redis_init / stable (:67-136)
     hipLaunchKernelGGL(k_sign_init, grd, blk, 0, c->stream, phi, c->s0, G, w, btn);
     if (btn == 0) {
@@ -647,13 +677,17 @@ int redis(fcfd_ctx *c, int btn, int *iters) {
             FCFD_CHECK(hipGetLastError());
             FCFD_TRY(bc_scalar(c, phi));
         }
-        hipLaunchKernelGGL(k_redis_check, dim3(1), dim3(NSHARD), 0, c->stream, c->flag, tol);
+        hipLaunchKernelGGL(k_redis_check, dim3(c->nb), dim3(NSHARD), 0, c->stream, c->flag, tol);
     }
-    FCFD_CHECK(hipMemcpyAsync(c->flag_host, c->flag, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                              c->stream));
+    // words 0..3 of every case's flag block
+    FCFD_CHECK(hipMemcpy2DAsync(c->flag_host, 4 * sizeof(unsigned long long), c->flag,
+                                FLAG_STRIDE * sizeof(unsigned long long), 4 * sizeof(unsigned long long), c->nb,
+                                hipMemcpyDeviceToHost, c->stream));
     FCFD_CHECK(hipStreamSynchronize(c->stream));
-    c->diag.redis_iters = (int)c->flag_host[2];
-    if (iters) *iters = (int)c->flag_host[2];
+    int most = 0;   // a batched stack reports the largest pseudo-step count of its cases
+    for (int q = 0; q < c->nb; ++q) most = std::max(most, (int)c->flag_host[4 * q + 2]);
+    c->diag.redis_iters = most;
+    if (iters) *iters = most;
     return FCFD_OK;
 }
 

@@ -58,8 +58,19 @@ static Lay lay_of(const MgLevel &M, int dim) {
     return L;
 }
 
+// one case's layout: a 2D level of a batched stack holds one case per slice
+static Lay lay_case(const MgLevel &M, int dim) {
+    Lay L = lay_of(M, dim);
+    if (dim == 2) L.nz = 1;
+    return L;
+}
+
+// cells of one case on level M
+static long case_cells(const MgLevel &M, int dim) { return (long)M.nx * M.ny * (dim == 3 ? M.nz : 1); }
+
 // Colour-split layout of the FP32 level-0 arrays of the mixed cycle.  Cell
-// (i,j,k) has colour c = (i+j+k) & 1 and lives in colour array c (the first
+// (i,j,k) has colour c = (i+j+k) & 1 (2D: (i+j) & 1, k being the case of a
+// batched stack) and lives in colour array c (the first
 // or second `half` elements) at slot m = i >> 1 of row (j,k); each colour
 // array has mh = (nx+1)/2 slots per row plus one ghost slot on every side.
 // All six neighbours of a cell have the other colour, so a red-black
@@ -70,7 +81,7 @@ struct Split {
     long sx, sxy, half;   // row and plane pitch of one colour array; its size
     __host__ __device__ long row(int j, int k) const { return (long)(j + 1) * sx + (long)(k + kz) * sxy; }
     __host__ __device__ long at(int i, int j, int k) const {
-        return ((i + j + k) & 1) * half + (long)((i >> 1) + 1) + row(j, k);
+        return ((i + j + (kz ? k : 0)) & 1) * half + (long)((i >> 1) + 1) + row(j, k);
     }
 };
 
@@ -79,7 +90,7 @@ static Split split_of(const Geom &G) {
     S.nx = G.nx; S.ny = G.ny; S.nz = G.nz; S.kz = G.dim == 3 ? 1 : 0;
     S.sx = (G.nx + 1) / 2 + 2;
     S.sxy = S.sx * (G.ny + 2);
-    S.half = S.sxy * (G.dim == 3 ? G.nz + 2 : 1);
+    S.half = S.sxy * (G.dim == 3 ? G.nz + 2 : G.nz);   // 2D: one plane per case of a batched stack
     return S;
 }
 
@@ -236,7 +247,7 @@ __global__ void __launch_bounds__(256) k_smooth(T *__restrict__ e, const T *__re
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int j = by_ * blockDim.y + threadIdx.y, k = bz_;
-    const int i0 = 2 * (bx_ * blockDim.x + threadIdx.x), par_ = (j + k + color) & 1;
+    const int i0 = 2 * (bx_ * blockDim.x + threadIdx.x), par_ = (j + (DIM == 3 ? k : 0) + color) & 1;
     const int i = i0 + par_;
     const bool upd = i < L.nx && j < L.ny;
     T enew = (T)0;
@@ -266,7 +277,7 @@ __global__ void __launch_bounds__(256) k_smooth(T *__restrict__ e, const T *__re
         rz = block_sum256(rz, sh);
         rzo = block_sum256(rzo, sh);
         if (threadIdx.x + threadIdx.y == 0) {
-            const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+            const long b = bx_ + (long)gridDim.x * (by_ + (long)gridDim.y * bz_);
             const long nb = (long)gridDim.x * gridDim.y * gridDim.z;
             D.partial[b] = rz;
             D.partial[nb + b] = rzo;
@@ -292,7 +303,7 @@ __global__ void __launch_bounds__(256) k_smooth_split(float *__restrict__ e, con
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int m = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
-    const int s = (j + k + color) & 1, i = 2 * m + s;
+    const int s = (j + (DIM == 3 ? k : 0) + color) & 1, i = 2 * m + s;
     const long id = (long)(m + 1) + S.row(j, k);
     const long cc = color ? S.half : 0, oc = S.half - cc;   // this colour, the other colour
     const bool upd = i < S.nx && j < S.ny;
@@ -344,7 +355,7 @@ __global__ void __launch_bounds__(256) k_smooth_split(float *__restrict__ e, con
         rz = block_sum256(rz, sh);
         rzo = block_sum256(rzo, sh);
         if (threadIdx.x + threadIdx.y == 0) {
-            const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+            const long b = bx_ + (long)gridDim.x * (by_ + (long)gridDim.y * bz_);
             const long nb = (long)gridDim.x * gridDim.y * gridDim.z;
             D.partial[b] = rz;
             D.partial[nb + b] = rzo;
@@ -438,7 +449,7 @@ __global__ void __launch_bounds__(FUSED_THREADS) k_smooth_fused(const float *__r
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int t = threadIdx.x;
-    const int i0 = bx_ * TX - H, j0 = by_ * TY - H, k0 = DIM == 3 ? bz_ * TZ - H : 0;
+    const int i0 = bx_ * TX - H, j0 = by_ * TY - H, k0 = DIM == 3 ? bz_ * TZ - H : bz_;   // 2D: the slice
     const float hx = (float)ih.x, hy = (float)ih.y, hz = (float)ih.z;
     for (int q = t; q < NE; q += FUSED_THREADS) {
         const int qx = q % EX, qy = (q / EX) % EY, qz = q / SZ;
@@ -473,7 +484,7 @@ __global__ void __launch_bounds__(FUSED_THREADS) k_smooth_fused(const float *__r
         const int ox = H - D, oy = H - D, oz = DIM == 3 ? H - D : 0;
         for (int q = t; q < nq; q += FUSED_THREADS) {
             const int m = q % mx, r = q / mx, ly = oy + r % ry, lz = oz + r / ry;
-            const int par = (color - i0 - ox - j0 - ly - k0 - lz) & 1;
+            const int par = (color - i0 - ox - j0 - ly - (DIM == 3 ? k0 + lz : 0)) & 1;
             const int lx = ox + par + 2 * m;
             const int i = i0 + lx, j = j0 + ly, k = k0 + lz;
             if (lx >= ox + rx || i < 0 || i >= L.nx || j < 0 || j >= L.ny || (DIM == 3 && (k < 0 || k >= L.nz)))
@@ -598,27 +609,36 @@ __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, lo
     if (t < n) b[t] = (float)a[t];
 }
 
-// Fixed-order two-stage sum of the block partials of a level-0 launch:
-// stage 1, block (b, s) sums the contiguous chunk b of slot s into
-// partial2[s*256 + b]; stage 2 (one block) sums the 256 chunk sums per slot.
+// Fixed-order two-stage sum of the block partials of a level-0 launch, per
+// case q (blockIdx.z; a batched stack's case owns the contiguous logical
+// blocks [q nb, (q+1) nb) of every slot): stage 1, block (b, s, q) sums the
+// contiguous chunk b of slot s into partial2[(q ns + s) 256 + b]; stage 2
+// (one block per case) sums the 256 chunk sums per slot.
 __global__ void __launch_bounds__(256) k_sum_stage1(const double *__restrict__ partial, long nb, long chunk,
                                                     double *__restrict__ partial2) {
     __shared__ double sh[4];
-    const int s = blockIdx.y;
+    const int s = blockIdx.y, q = blockIdx.z;
+    const long ntot = nb * gridDim.z;
     const long b0 = (long)blockIdx.x * chunk, b1 = b0 + chunk < nb ? b0 + chunk : nb;
     double a = 0.0;
-    for (long b = b0 + threadIdx.x; b < b1; b += 256) a += partial[s * nb + b];
+    for (long b = b0 + threadIdx.x; b < b1; b += 256) a += partial[s * ntot + q * nb + b];
     a = block_sum256(a, sh);
-    if (threadIdx.x == 0) partial2[s * 256 + blockIdx.x] = a;
+    if (threadIdx.x == 0) partial2[((long)q * gridDim.y + s) * 256 + blockIdx.x] = a;
 }
 
 // Device scalars of a solve (mg_scalars[0..63]).  The flexible-CG
 // coefficients are formed on the device so an iteration needs one host
 // read (the residual norm and the breakdown flag) and replays as one graph.
+// A batched stack holds one block of SCS scalars per case; the dense coarse
+// inverses follow the nb blocks.  SC_DONE freezes a case whose solve has
+// ended while others of the stack go on: its alpha and beta are 0 from then
+// on, so the x += alpha p still pending from its last iteration is the only
+// update it receives, as at the end of a single solve.
 enum {
     SC_RZ = 0, SC_RZO = 1, SC_PQ = 2, SC_RR = 3, SC_MEAN = 4, SC_RZ_OLD = 5, SC_BETA = 6, SC_ALPHA = 7,
-    SC_FIRST = 8, SC_BREAK = 9, SC_NHOST = 10
+    SC_FIRST = 8, SC_BREAK = 9, SC_DONE = 10, SC_NHOST = 11
 };
+constexpr int SCS = 64;
 enum { SUM_STORE = 0, SUM_CG_BETA = 1, SUM_CG_ALPHA = 2 };
 
 // op SUM_CG_BETA: out = {r.z, r.zold} -> beta = (r.z - r.zold) / rz_old
@@ -630,31 +650,41 @@ enum { SUM_STORE = 0, SUM_CG_BETA = 1, SUM_CG_ALPHA = 2 };
 __global__ void __launch_bounds__(256) k_sum_stage2(const double *__restrict__ partial2, int ns,
                                                     double *__restrict__ out, int op, double *__restrict__ sc) {
     __shared__ double sh[4];
+    const int q = blockIdx.x;   // case
+    partial2 += (long)q * ns * 256;
+    out += (long)q * SCS;
+    sc += (long)q * SCS;
     for (int s = 0; s < ns; ++s) {
         double a = block_sum256(partial2[s * 256 + threadIdx.x], sh);
         if (threadIdx.x == 0) out[s] = a;
     }
+    const bool done = sc[SC_DONE] != 0.0;
     if (threadIdx.x == 0 && op == SUM_CG_BETA) {
         const double rz = out[0], rzo = out[1];
-        sc[SC_BETA] = sc[SC_FIRST] != 0.0 ? 0.0 : (rz - rzo) / sc[SC_RZ_OLD];
+        sc[SC_BETA] = (sc[SC_FIRST] != 0.0 || done) ? 0.0 : (rz - rzo) / sc[SC_RZ_OLD];
         sc[SC_RZ_OLD] = rz;
         sc[SC_FIRST] = 0.0;
     } else if (threadIdx.x == 0 && op == SUM_CG_ALPHA) {
         const double pq = out[0], rz = sc[SC_RZ_OLD];
-        const bool ok = pq < 0.0 && rz < 0.0;
+        const bool ok = pq < 0.0 && rz < 0.0 && !done;
         sc[SC_ALPHA] = ok ? rz / pq : 0.0;
-        if (!ok) sc[SC_BREAK] = 1.0;
+        if (!ok && !done) sc[SC_BREAK] = 1.0;
     }
 }
 
-// start of a flexible-CG solve: first iteration, no breakdown
+// start of a flexible-CG solve: first iteration, no breakdown (case blockIdx.x)
 __global__ void k_cg_init(double *sc) {
+    sc += (long)blockIdx.x * SCS;
     sc[SC_FIRST] = 1.0;
     sc[SC_BREAK] = 0.0;
     sc[SC_RZ_OLD] = 0.0;
     sc[SC_BETA] = 0.0;
     sc[SC_ALPHA] = 0.0;
+    sc[SC_DONE] = 0.0;
 }
+
+// freeze case q of a batched solve (see SC_DONE)
+__global__ void k_cg_done(double *sc, int q) { sc[(long)q * SCS + SC_DONE] = 1.0; }
 
 // sum of an interior field into block partials
 __global__ void __launch_bounds__(256) k_sum(const double *__restrict__ a, Lay L, double *__restrict__ partial) {
@@ -665,37 +695,39 @@ __global__ void __launch_bounds__(256) k_sum(const double *__restrict__ a, Lay L
     double v = 0.0;
     if (i < L.nx && j < L.ny) v = a[L.id(i, j, k)];
     v = block_sum256(v, sh);
-    if (threadIdx.x + threadIdx.y == 0)
-        partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = v;
+    if (threadIdx.x + threadIdx.y == 0) partial[bx_ + (long)gridDim.x * (by_ + (long)gridDim.y * bz_)] = v;
 }
 
-// a -= scale * s[0]  (interior)
-__global__ void k_sub_scalar(double *__restrict__ a, Lay L, const double *__restrict__ s, double scale) {
+// a -= scale * s[0]  (interior; s[q SCS] for case q of a batched stack)
+__global__ void k_sub_scalar(double *__restrict__ a, Lay L, const double *__restrict__ s, double scale, int dim) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= L.nx || j >= L.ny) return;
-    a[L.id(i, j, k)] -= s[0] * scale;
+    a[L.id(i, j, k)] -= s[(dim == 2 ? (long)k : 0L) * SCS] * scale;
 }
 
 // copy f into the working rhs minus its mean (compatibility of the Neumann
 // problem): out = f - s[0]*scale
 __global__ void k_center_copy(const double *__restrict__ f, double *__restrict__ out, Lay L,
-                              const double *__restrict__ s, double scale) {
+                              const double *__restrict__ s, double scale, int dim) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= L.nx || j >= L.ny) return;
     long id = L.id(i, j, k);
-    out[id] = f[id] - s[0] * scale;
+    out[id] = f[id] - s[(dim == 2 ? (long)k : 0L) * SCS] * scale;
 }
 
-// p += e (FP32 colour-split correction into the FP64 solution)
-__global__ void k_add_f32(double *__restrict__ p, const float *__restrict__ e, Lay L, Split S) {
+// p += e (FP32 colour-split correction into the FP64 solution), except for
+// the frozen cases of a batched stack (SC_DONE of case k in sc)
+__global__ void k_add_f32(double *__restrict__ p, const float *__restrict__ e, Lay L, Split S,
+                          const double *__restrict__ sc, int dim) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= L.nx || j >= L.ny) return;
+    if (sc[(dim == 2 ? (long)k : 0L) * SCS + SC_DONE] != 0.0) return;
     p[L.id(i, j, k)] += (double)e[S.at(i, j, k)];
 }
 
@@ -860,10 +892,14 @@ __global__ void __launch_bounds__(CINV_THREADS) k_coarse_inv(const double *__res
     __shared__ double sh[NW];
     const int t = threadIdx.x, n = L.nx * L.ny * L.nz;
     const int cc = t % CDENSE, rg = t / CDENSE;
+    // a batched 2D stack inverts case blockIdx.x's block (L: one slice)
+    const int kq = DIM == 2 ? (int)blockIdx.x : 0;
+    minv += (long)kq * n * n;
     double dsum = 0.0;
     if (t < n) {
         int i, j, k;
         cell_coords(L, t, i, j, k);
+        k += kq;
         const long id = L.id(i, j, k);
         double d = (bx[id] + bx[id - 1]) * ih.x + (by[id] + by[id - L.sx]) * ih.y;
         if (DIM == 3) d += (bz[id] + bz[id - L.sxy]) * ih.z;
@@ -885,6 +921,7 @@ __global__ void __launch_bounds__(CINV_THREADS) k_coarse_inv(const double *__res
         double ce_ = 0.0, cw = 0.0, cn = 0.0, cs = 0.0, ct = 0.0, cb = 0.0, d = 0.0;
         if (cc < n) {
             cell_coords(L, cc, i, j, k);
+            k += kq;
             const long id = L.id(i, j, k);
             ce_ = bx[id] * ih.x; cw = bx[id - 1] * ih.x; cn = by[id] * ih.y; cs = by[id - L.sx] * ih.y;
             d = ce_ + cw + cn + cs;
@@ -946,15 +983,18 @@ __global__ void __launch_bounds__(CINV_THREADS) k_coarse_inv(const double *__res
 // one block of CDENSE threads, one cell per thread
 template <typename T>
 __global__ void __launch_bounds__(CDENSE) k_coarse_dense(T *__restrict__ eout, const T *__restrict__ fin,
-                                                         const double *__restrict__ minv, Lay L) {
+                                                         const double *__restrict__ minv, Lay L, int dim) {
     constexpr int NW = CDENSE / WAVE;
     __shared__ double b[CDENSE];
     __shared__ double ws[NW];
     const int t = threadIdx.x, n = L.nx * L.ny * L.nz;
-    int i = 0, j = 0, k = 0;
+    const int kq = dim == 2 ? (int)blockIdx.x : 0;   // case of a batched 2D stack (L: one slice)
+    minv += (long)kq * n * n;
+    int i = 0, j = 0, k = kq;
     double f = 0.0;
     if (t < n) {
         cell_coords(L, t, i, j, k);
+        k += kq;
         f = (double)fin[L.id(i, j, k)];
     }
     // fixed-order block sum: butterfly per wave (lane 0's order), then waves in order
@@ -1018,6 +1058,13 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
     __shared__ double red[TAIL_THREADS / WAVE];
     __shared__ double bvec[CDENSE];
     const int t = threadIdx.x;
+    // a batched 2D stack runs one workgroup per case on its slice kq; the
+    // level layouts describe one slice
+    const int kq = DIM == 2 ? (int)blockIdx.x : 0;
+    auto cell_coords = [kq](const Lay &L, int c, int &i, int &j, int &k) {
+        fcfd::cell_coords(L, c, i, j, k);
+        k += kq;
+    };
     auto ncells = [](const Lay &L) { return L.nx * L.ny * L.nz; };
     auto gs = [&](const TailLevel<T> &V, int color) {
         const Lay &L = V.L;
@@ -1026,7 +1073,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
         for (int c = t; c < n; c += TAIL_THREADS) {
             int i, j, k;
             cell_coords(L, c, i, j, k);
-            if (((i + j + k) & 1) != color) continue;
+            if (((i + j + (DIM == 3 ? k : 0)) & 1) != color) continue;   // 2D: k is the case
             const long id = L.id(i, j, k);
             T sc, sce, ep;
             stencil<T, DIM, false, 0>(V.e, nullptr, V.bx, V.by, V.bz, L, V.ih, i, j, k, id, P0, sc, sce, ep);
@@ -1091,7 +1138,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
         const int ch = sl % NCH;
         int I, J, K;
         cell_coords(C.L, t + TAIL_THREADS * (sl / NCH), I, J, K);
-        return F.L.id(2 * I + (ch & 1), 2 * J + ((ch >> 1) & 1), DIM == 3 ? 2 * K + (ch >> 2) : 0);
+        return F.L.id(2 * I + (ch & 1), 2 * J + ((ch >> 1) & 1), DIM == 3 ? 2 * K + (ch >> 2) : K);
     };
     if constexpr (sizeof(T) == 4) {
         __shared__ float es_[TAIL_LDS], fs_[TAIL_TOP];
@@ -1182,7 +1229,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
     {
         const TailLevel<T> &V = A.lv[nl - 1];
         const int n = ncells(V.L);
-        int i = 0, j = 0, k = 0;
+        int i = 0, j = 0, k = kq;
         double fv = 0.0;
         if (t < n) {
             cell_coords(V.L, t, i, j, k);
@@ -1192,8 +1239,9 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
         if (t < n) bvec[t] = fm - fv;
         __syncthreads();
         double x = 0.0;
+        const double *minv = A.minv + (long)kq * n * n;
         if (t < n)
-            for (int cc = 0; cc < n; ++cc) x += A.minv[cc * n + t] * bvec[cc];
+            for (int cc = 0; cc < n; ++cc) x += minv[cc * n + t] * bvec[cc];
         const double xm = block_sum(t < n ? x : 0.0) / (double)n;
         if (t < n) V.e[V.L.id(i, j, k)] = (T)(x - xm);
         __syncthreads();
@@ -1299,7 +1347,7 @@ static int mg_build(fcfd_ctx *c) {
     c->mg.push_back(L0);
     for (;;) {
         MgLevel &F = c->mg.back();
-        long ncell = (long)F.nx * F.ny * F.nz;
+        long ncell = case_cells(F, G.dim);
         if (ncell <= CDENSE && c->mg.size() > 1) break;
         double hmin = G.dim == 3 ? std::min({F.hx, F.hy, F.hz}) : std::min(F.hx, F.hy);
         int cf[3] = {1, 1, 1};
@@ -1317,7 +1365,7 @@ static int mg_build(fcfd_ctx *c) {
         }
         F.cx = cf[0]; F.cy = cf[1]; F.cz = cf[2];
         MgLevel C = {};
-        C.nx = F.nx / cf[0]; C.ny = F.ny / cf[1]; C.nz = G.dim == 3 ? F.nz / cf[2] : 1;
+        C.nx = F.nx / cf[0]; C.ny = F.ny / cf[1]; C.nz = G.dim == 3 ? F.nz / cf[2] : F.nz;   // 2D: the cases
         C.hx = F.hx * cf[0]; C.hy = F.hy * cf[1]; C.hz = F.hz * cf[2];
         C.sx = C.nx + 2;
         C.sxy = C.sx * (C.ny + 2);
@@ -1347,11 +1395,19 @@ static int mg_build(fcfd_ctx *c) {
     FCFD_TRY(alloc0(&c->kz, G.ntot));
     FCFD_TRY(alloc0(&c->kzo, G.ntot));
     FCFD_TRY(alloc0(&c->kzo32, nsplit));
-    // scalars + coarse CG workspace (6 vectors) or the dense coarse inverse
-    long ncoarse = (long)c->mg.back().nx * c->mg.back().ny * c->mg.back().nz;
-    FCFD_TRY(alloc0(&c->mg_scalars, 64 + std::max(6 * ncoarse, ncoarse * ncoarse)));
+    // per-case scalars + coarse CG workspace (6 vectors) or the dense coarse
+    // inverse of every case
+    long ncoarse = case_cells(c->mg.back(), G.dim);
+    if (c->nb > 1 && (c->mg.size() < 2 || ncoarse > CDENSE)) {
+        fprintf(stderr, "libfcfd: a batched stack needs a coarsest level of <= %d cells per case\n", CDENSE);
+        return FCFD_ERR_ARG;
+    }
+    FCFD_TRY(alloc0(&c->mg_scalars, SCS * c->nb + std::max(6 * ncoarse, c->nb * ncoarse * ncoarse)));
     return FCFD_OK;
 }
+
+// dense coarse inverses (case q at + q n^2) after the per-case scalars
+static double *coarse_inv(fcfd_ctx *c) { return c->mg_scalars + SCS * c->nb; }
 
 // ppe_sor_init: coefficients of every level from rho%now
 int mg_setup(fcfd_ctx *c) {
@@ -1389,16 +1445,16 @@ int mg_setup(fcfd_ctx *c) {
     FCFD_CHECK(hipGetLastError());
     {
         MgLevel &M = c->mg.back();
-        const long n = (long)M.nx * M.ny * M.nz;
+        const long n = case_cells(M, G.dim);
         if (c->mg.size() > 1 && n <= CDENSE) {
-            Lay L = lay_of(M, G.dim);
+            Lay L = lay_case(M, G.dim);
             Inv2 ih = inv2(M.hx, M.hy, M.hz);
             if (G.dim == 3)
                 hipLaunchKernelGGL(k_coarse_inv<3>, dim3(1), dim3(CINV_THREADS), 0, c->stream, M.bx64, M.by64,
-                                   M.bz64, L, ih, c->mg_scalars + 64);
+                                   M.bz64, L, ih, coarse_inv(c));
             else
-                hipLaunchKernelGGL(k_coarse_inv<2>, dim3(1), dim3(CINV_THREADS), 0, c->stream, M.bx64, M.by64,
-                                   nullptr, L, ih, c->mg_scalars + 64);
+                hipLaunchKernelGGL(k_coarse_inv<2>, dim3(c->nb), dim3(CINV_THREADS), 0, c->stream, M.bx64, M.by64,
+                                   nullptr, L, ih, coarse_inv(c));
             FCFD_CHECK(hipGetLastError());
         }
     }
@@ -1446,7 +1502,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
             if (lv >= 1 && M.e32b && sweeps >= 2 && 32L * M.nx * M.ny * M.nz <= n0c && !dt && !zinit) {
                 using FT = FusedTile<DIM>;
                 const dim3 gf((M.nx + FT::TX - 1) / FT::TX, (M.ny + FT::TY - 1) / FT::TY,
-                              DIM == 3 ? (M.nz + FT::TZ - 1) / FT::TZ : 1);
+                              DIM == 3 ? (M.nz + FT::TZ - 1) / FT::TZ : M.nz);
                 const int c0 = reverse ? 1 : 0;
                 int n2, n1;
                 if ((long)gf.x * gf.y * gf.z <= c->ncu) {
@@ -1528,7 +1584,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
     // levels [lt, nl) run inside one single-workgroup launch (k_mg_tail) when
     // they are all small and the coarsest has a dense inverse; else lt = nl
     int lt = nl, top = 0;
-    auto ncell_of = [&](int lv) { return (long)c->mg[lv].nx * c->mg[lv].ny * c->mg[lv].nz; };
+    auto ncell_of = [&](int lv) { return case_cells(c->mg[lv], G.dim); };   // per case
     if (nl > 1 && ncell_of(nl - 1) <= CDENSE) {
         for (int lv = std::max(l, 1); lv < nl; ++lv)
             if (ncell_of(lv) <= TAIL_MAX) { lt = lv; break; }
@@ -1575,7 +1631,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         TailArgs<T> A{};
         A.n = nl - lt;
         A.top = top;
-        A.minv = c->mg_scalars + 64;
+        A.minv = coarse_inv(c);
         for (int q = 0; q < A.n; ++q) {
             const int lv = lt + q;
             MgLevel &M = c->mg[lv];
@@ -1583,14 +1639,14 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
             T *e; const T *f, *bx, *by, *bz;
             arrays(lv, e, f, bx, by, bz);
             V.e = e; V.f = (T *)f; V.bx = bx; V.by = by; V.bz = bz;
-            V.L = lay_of(M, G.dim);
+            V.L = lay_case(M, G.dim);
             V.ih = inv2(M.hx, M.hy, M.hz);
             V.cx = M.cx; V.cy = M.cy; V.cz = DIM == 3 ? M.cz : 1;
             V.nu1 = sweeps(lv, nu1);
             V.nu2 = sweeps(lv, nu2);
         }
         ProfScope ps_(c, R_COARSE);
-        hipLaunchKernelGGL((k_mg_tail<T, DIM>), dim3(1), dim3(TAIL_THREADS), 0, c->stream, A);
+        hipLaunchKernelGGL((k_mg_tail<T, DIM>), dim3(DIM == 2 ? c->nb : 1), dim3(TAIL_THREADS), 0, c->stream, A);
         FCFD_CHECK(hipGetLastError());
     } else {
         // coarsest: dense inverse or FP64 CG in one workgroup
@@ -1599,14 +1655,14 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         Inv2 ih = inv2(M.hx, M.hy, M.hz);
         T *e; const T *f, *bx, *by, *bz;
         arrays(nl - 1, e, f, bx, by, bz);
-        long n = (long)M.nx * M.ny * M.nz;
+        long n = case_cells(M, G.dim);
         ProfScope ps_(c, R_COARSE);
         if (nl > 1 && n <= CDENSE)
-            hipLaunchKernelGGL((k_coarse_dense<T>), dim3(1), dim3(CDENSE), 0, c->stream, e, f,
-                               (const double *)(c->mg_scalars + 64), L);
-        else
+            hipLaunchKernelGGL((k_coarse_dense<T>), dim3(DIM == 2 ? c->nb : 1), dim3(CDENSE), 0, c->stream, e, f,
+                               (const double *)coarse_inv(c), lay_case(M, G.dim), DIM);
+        else   // one case (mg_build refuses a batched stack without a dense coarsest level)
             hipLaunchKernelGGL((k_coarse_cg<T, DIM>), dim3(1), dim3(n >= 1024 ? 1024 : 256), 0, c->stream, e, f, bx,
-                               by, bz, M.bx64, M.by64, M.bz64, c->mg_scalars + 64, L, ih, (int)std::min<long>(n, 96),
+                               by, bz, M.bx64, M.by64, M.bz64, coarse_inv(c), L, ih, (int)std::min<long>(n, 96),
                                0.0);
         FCFD_CHECK(hipGetLastError());
     }
@@ -1660,7 +1716,7 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
     rr = block_sum256(rr, sh);
     ff = block_sum256(ff, sh);
     if (threadIdx.x + threadIdx.y == 0) {
-        long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+        long b = bx_ + (long)gridDim.x * (by_ + (long)gridDim.y * bz_);
         long nb = (long)gridDim.x * gridDim.y * gridDim.z;
         partial[b] = rr;
         partial[nb + b] = ff;
@@ -1683,12 +1739,14 @@ __global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__r
                                                  double *__restrict__ x, const double *__restrict__ rho, Lay L,
                                                  Split S, Inv2 ih, const double *__restrict__ sc,
                                                  double *__restrict__ partial) {
-    const double beta = sc[SC_BETA], alpha_prev = sc[SC_ALPHA];
     __shared__ double sh[4];
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y;
-    const int k0 = DIM == 3 ? bz_ * FCG_KC : 0, k1 = DIM == 3 ? min(k0 + FCG_KC, L.nz) : 1;
+    // 3D: FCG_KC planes per block; 2D: the block's slice (the case of a batched stack)
+    const int k0 = DIM == 3 ? bz_ * FCG_KC : bz_, k1 = DIM == 3 ? min(k0 + FCG_KC, L.nz) : bz_ + 1;
+    sc += (long)case_of<DIM>(bz_) * SCS;
+    const double beta = sc[SC_BETA], alpha_prev = sc[SC_ALPHA];
     auto zid = [&](int ii, int jj, int kk) -> long {
         if constexpr (sizeof(T) == 4) return S.at(ii, jj, kk);
         else return L.id(ii, jj, kk);
@@ -1729,20 +1787,19 @@ __global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__r
         }
     }
     pq = block_sum256(pq, sh);
-    if (threadIdx.x + threadIdx.y == 0)
-        partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = pq;
+    if (threadIdx.x + threadIdx.y == 0) partial[bx_ + (long)gridDim.x * (by_ + (long)gridDim.y * bz_)] = pq;
 }
 
 // r -= alpha q, ||r||^2; the new residual also becomes the rhs of the next
 // preconditioning cycle (r32 for the FP32 cycle; the FP64 cycle reads r)
 __global__ void __launch_bounds__(256) k_fcg_update(const double *__restrict__ q, double *__restrict__ r, Lay L,
                                                     const double *__restrict__ sc, float *__restrict__ r32, Split S,
-                                                    double *__restrict__ partial) {
-    const double alpha = sc[SC_ALPHA];
+                                                    double *__restrict__ partial, int dim) {
     __shared__ double sh[4];
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
+    const double alpha = sc[(dim == 2 ? (long)k : 0L) * SCS + SC_ALPHA];
     double rr = 0.0;
     if (i < L.nx && j < L.ny) {
         long id = L.id(i, j, k);
@@ -1752,43 +1809,45 @@ __global__ void __launch_bounds__(256) k_fcg_update(const double *__restrict__ q
         if (r32) r32[S.at(i, j, k)] = (float)rn;
     }
     rr = block_sum256(rr, sh);
-    if (threadIdx.x + threadIdx.y == 0)
-        partial[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = rr;
+    if (threadIdx.x + threadIdx.y == 0) partial[bx_ + (long)gridDim.x * (by_ + (long)gridDim.y * bz_)] = rr;
 }
 
 // the last iteration's x += alpha p
 __global__ void __launch_bounds__(256) k_fcg_finish(double *__restrict__ x, const double *__restrict__ p, Lay L,
-                                                    const double *__restrict__ sc) {
+                                                    const double *__restrict__ sc, int dim) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= L.nx || j >= L.ny) return;
     const long id = L.id(i, j, k);
-    x[id] += sc[SC_ALPHA] * p[id];
+    x[id] += sc[(dim == 2 ? (long)k : 0L) * SCS + SC_ALPHA] * p[id];
 }
 
-// out[0..ns) = sums of the ns slots of the block partials of a level-0
-// launch, then the scalar step `op` (SUM_*)
+// out[q SCS + s] = sums of the ns slots of the block partials of a level-0
+// launch of nb blocks, per case q of the context, then the scalar step `op`
+// (SUM_*) on each case's scalars
 static int sum_partials(fcfd_ctx *c, long nb, int ns, double *out, int op = SUM_STORE) {
-    const long chunk = (nb + 255) / 256;
-    hipLaunchKernelGGL(k_sum_stage1, dim3(256, ns), dim3(256), 0, c->stream, c->red.partial, nb, chunk,
+    const long nbq = nb / c->nb, chunk = (nbq + 255) / 256;
+    hipLaunchKernelGGL(k_sum_stage1, dim3(256, ns, c->nb), dim3(256), 0, c->stream, c->red.partial, nbq, chunk,
                        c->red.partial2);
-    hipLaunchKernelGGL(k_sum_stage2, dim3(1), dim3(256), 0, c->stream, c->red.partial2, ns, out, op,
+    hipLaunchKernelGGL(k_sum_stage2, dim3(c->nb), dim3(256), 0, c->stream, c->red.partial2, ns, out, op,
                        c->mg_scalars);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
 
-// sum ns slots of the block partials of a level-0 launch and bring them to the host
+// sum ns slots of the block partials of a level-0 launch and bring them to
+// the host: out[q ns + s] for case q
 static int sums_to_host(fcfd_ctx *c, long nb, int ns, double *out) {
     FCFD_TRY(sum_partials(c, nb, ns, c->mg_scalars));
-    FCFD_CHECK(hipMemcpyAsync(c->red.host, c->mg_scalars, ns * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FCFD_CHECK(hipMemcpy2DAsync(c->red.host, ns * sizeof(double), c->mg_scalars, SCS * sizeof(double),
+                                ns * sizeof(double), c->nb, hipMemcpyDeviceToHost, c->stream));
     FCFD_CHECK(hipStreamSynchronize(c->stream));
-    for (int s = 0; s < ns; ++s) out[s] = c->red.host[s];
+    for (int q = 0; q < c->nb * ns; ++q) out[q] = c->red.host[q];
     return FCFD_OK;
 }
 
-// mean of an interior FP64 field into mg_scalars[4] (device)
+// mean of an interior FP64 field into mg_scalars[q SCS + SC_MEAN] per case (device)
 static int field_mean(fcfd_ctx *c, const double *a) {
     const Geom &G = c->G;
     dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
@@ -1847,7 +1906,7 @@ static int fcg_iteration(fcfd_ctx *c, int kind, int inner) {
     dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
     const long nb = (long)g.x * g.y * g.z;
     const long nb_s = (long)(((G.nx + 1) / 2 + blk.x - 1) / blk.x) * ((G.ny + blk.y - 1) / blk.y) * G.nz;
-    const dim3 gdir(g.x, g.y, DIM == 3 ? (G.nz + FCG_KC - 1) / FCG_KC : 1);
+    const dim3 gdir(g.x, g.y, DIM == 3 ? (G.nz + FCG_KC - 1) / FCG_KC : G.nz);
     const Lay L0 = lay_of(G);
     const Split S0 = split_of(G);
     const Inv2 ih = inv2(G.dx, G.dy, G.dz);
@@ -1879,14 +1938,22 @@ static int fcg_iteration(fcfd_ctx *c, int kind, int inner) {
     {
         ProfScope ps_(c, R_FCG);
         hipLaunchKernelGGL(k_fcg_update, g, blk, 0, c->stream, c->kq, c->kr, L0, sc, mixed ? M0.f32 : nullptr, S0,
-                           c->red.partial);
+                           c->red.partial, DIM);
         FCFD_CHECK(hipGetLastError());
     }
     FCFD_TRY(sum_partials(c, nb, 1, sc + SC_RR, SUM_STORE));
-    FCFD_CHECK(hipMemcpyAsync(c->red.host, sc, SC_NHOST * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // the first SC_NHOST scalars of every case, packed
+    FCFD_CHECK(hipMemcpy2DAsync(c->red.host, SC_NHOST * sizeof(double), sc, SCS * sizeof(double),
+                                SC_NHOST * sizeof(double), c->nb, hipMemcpyDeviceToHost, c->stream));
     return FCFD_OK;
 }
 
+// The solve runs the loop of a single system for every case of the context
+// at once (one case unless it is a batched 2D stack).  Each case keeps its
+// own stopping state; once a case's loop would have ended it is frozen on
+// the device (k_cg_done; the mixed defect correction skips its update), so
+// its result is that of its own solve.  FP64 plain V-cycles cannot freeze a
+// case and go on improving finished ones until the last case stops.
 template <int DIM>
 static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel) {
     const Geom &G = c->G;
@@ -1896,7 +1963,8 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     const Split S0 = split_of(G);
     const Inv2 ih = inv2(G.dx, G.dy, G.dz);
     MgLevel &M0 = c->mg[0];
-    const double ncell = (double)G.nx * (double)G.ny * (double)G.nz;
+    const int nq = c->nb;
+    const double ncell = DIM == 3 ? (double)G.nx * (double)G.ny * (double)G.nz : (double)G.nx * (double)G.ny;
     double *p = c->f[FCFD_P], *f = M0.f64, *rho = c->f[FCFD_RHO];
     const bool mixed = precision == FCFD_PPE_MIXED;
     const bool krylov = c->pr.ppe_krylov != 0;
@@ -1904,10 +1972,10 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     const int kind = mixed ? (krylov ? 3 : 2) : (krylov ? 1 : 0);
     const int maxc = c->pr.ppe_max_cycles > 0 ? c->pr.ppe_max_cycles : 200;
     static const bool verbose = getenv("FCFD_MG_VERBOSE") != nullptr;
-    // compatibility of the Neumann problem: f = src - mean(src)
+    // compatibility of the Neumann problem: f = src - mean(src), per case
     FCFD_TRY(field_mean(c, c->f[FCFD_PPE_SRC]));
     hipLaunchKernelGGL(k_center_copy, g, blk, 0, c->stream, c->f[FCFD_PPE_SRC], f, L0, c->mg_scalars + SC_MEAN,
-                       1.0 / ncell);
+                       1.0 / ncell, DIM);
     FCFD_CHECK(hipGetLastError());
     // r0 = f - A p
     {
@@ -1916,20 +1984,42 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
                            mixed ? M0.f32 : nullptr, S0, c->red.partial, 1);
         FCFD_CHECK(hipGetLastError());
     }
-    double s2[2];
-    FCFD_TRY(sums_to_host(c, nb, 2, s2));
-    const double fn = s2[1] > 0.0 ? sqrt(s2[1]) : 1.0;
-    double r = sqrt(s2[0]) / fn, best = r;
-    int k = 0, since_best = 0, rc = FCFD_OK;
-    if (verbose) fprintf(stderr, "fcfd mg[%s%s]: it 0 rel %.3e\n", mixed ? "mixed" : "fp64", krylov ? "+fcg" : "", r);
-    if (!std::isfinite(r)) return FCFD_ERR_NAN;
-    if (krylov) hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, c->stream, c->mg_scalars);
-    while (r > rel_tol && k < maxc) {
+    std::vector<double> s2(2 * nq);
+    FCFD_TRY(sums_to_host(c, nb, 2, s2.data()));
+    struct Case_ {
+        double fn, r, best;
+        int k, since;
+        bool done;
+    };
+    std::vector<Case_> cs(nq);
+    int active = 0;
+    for (int q = 0; q < nq; ++q) {
+        Case_ &a = cs[q];
+        a.fn = s2[2 * q + 1] > 0.0 ? sqrt(s2[2 * q + 1]) : 1.0;
+        a.r = sqrt(s2[2 * q]) / a.fn;
+        a.best = a.r;
+        a.k = a.since = 0;
+        if (!std::isfinite(a.r)) return FCFD_ERR_NAN;
+        a.done = !(a.r > rel_tol && a.k < maxc);
+        active += !a.done;
+    }
+    auto rmax = [&]() {
+        double m = 0.0;
+        for (const Case_ &a : cs) m = std::max(m, a.r);
+        return m;
+    };
+    if (verbose) fprintf(stderr, "fcfd mg[%s%s]: it 0 rel %.3e\n", mixed ? "mixed" : "fp64", krylov ? "+fcg" : "", rmax());
+    hipLaunchKernelGGL(k_cg_init, dim3(nq), dim3(1), 0, c->stream, c->mg_scalars);
+    for (int q = 0; q < nq; ++q)
+        if (cs[q].done) hipLaunchKernelGGL(k_cg_done, dim3(1), dim3(1), 0, c->stream, c->mg_scalars, q);
+    FCFD_CHECK(hipGetLastError());
+    int it = 0;
+    while (active > 0) {
         if (!krylov) {
             // defect correction (mixed) or plain V-cycles on p (FP64)
             FCFD_TRY(run_graph(c, kind, inner, [&]() { return vcycles<DIM>(c, kind, inner); }));
             if (mixed) {
-                hipLaunchKernelGGL(k_add_f32, g, blk, 0, c->stream, p, M0.e32, L0, S0);
+                hipLaunchKernelGGL(k_add_f32, g, blk, 0, c->stream, p, M0.e32, L0, S0, c->mg_scalars, DIM);
                 FCFD_CHECK(hipGetLastError());
             }
             {
@@ -1938,41 +2028,61 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
                                    mixed ? M0.f32 : nullptr, S0, c->red.partial, 0);
                 FCFD_CHECK(hipGetLastError());
             }
-            FCFD_TRY(sums_to_host(c, nb, 1, s2));
+            FCFD_TRY(sums_to_host(c, nb, 1, s2.data()));
             FCFD_TRY(prof_graph_accum(c, kind, inner));
         } else {
             // flexible CG preconditioned by the V-cycle(s), one graph per iteration
             FCFD_TRY(run_graph(c, kind, inner, [&]() { return fcg_iteration<DIM>(c, kind, inner); }));
             FCFD_CHECK(hipStreamSynchronize(c->stream));
             FCFD_TRY(prof_graph_accum(c, kind, inner));
-            if (c->red.host[SC_BREAK] != 0.0) {
-                if (verbose) fprintf(stderr, "fcfd mg: breakdown rz %.3e pq %.3e\n", c->red.host[SC_RZ_OLD],
-                                     c->red.host[SC_PQ]);
-                break;
-            }
-            s2[0] = c->red.host[SC_RR];
         }
-        ++k;
-        r = sqrt(s2[0]) / fn;
-        if (verbose) fprintf(stderr, "fcfd mg[%s%s]: it %d rel %.3e\n", mixed ? "mixed" : "fp64", krylov ? "+fcg" : "", k, r);
-        if (!std::isfinite(r)) return FCFD_ERR_NAN;
-        // stagnation: the FP64 residual has reached its round-off floor
-        if (r < 0.9 * best) { best = r; since_best = 0; }
-        else if (++since_best >= 4) break;
+        ++it;
+        for (int q = 0; q < nq; ++q) {
+            Case_ &a = cs[q];
+            if (a.done) continue;
+            if (krylov) {
+                const double *h = c->red.host + (long)q * SC_NHOST;
+                if (h[SC_BREAK] != 0.0) {
+                    if (verbose) fprintf(stderr, "fcfd mg: case %d breakdown rz %.3e pq %.3e\n", q, h[SC_RZ_OLD],
+                                         h[SC_PQ]);
+                    a.done = true;
+                    continue;
+                }
+                s2[q] = h[SC_RR];
+            }
+            ++a.k;
+            a.r = sqrt(s2[q]) / a.fn;
+            if (!std::isfinite(a.r)) return FCFD_ERR_NAN;
+            // stagnation: the FP64 residual has reached its round-off floor
+            if (a.r < 0.9 * a.best) { a.best = a.r; a.since = 0; }
+            else if (++a.since >= 4) a.done = true;
+            if (!(a.r > rel_tol && a.k < maxc)) a.done = true;
+        }
+        active = 0;
+        for (const Case_ &a : cs) active += !a.done;
+        if (verbose) fprintf(stderr, "fcfd mg[%s%s]: it %d rel %.3e\n", mixed ? "mixed" : "fp64", krylov ? "+fcg" : "", it, rmax());
+        if (active > 0)
+            for (int q = 0; q < nq; ++q)
+                if (cs[q].done) hipLaunchKernelGGL(k_cg_done, dim3(1), dim3(1), 0, c->stream, c->mg_scalars, q);
     }
-    if (krylov && k > 0) {
-        hipLaunchKernelGGL(k_fcg_finish, g, blk, 0, c->stream, p, c->kp, L0, c->mg_scalars);
+    if (krylov && it > 0) {
+        hipLaunchKernelGGL(k_fcg_finish, g, blk, 0, c->stream, p, c->kp, L0, c->mg_scalars, DIM);
         FCFD_CHECK(hipGetLastError());
     }
-    if (r > rel_tol) rc = (r <= 10.0 * rel_tol && since_best >= 4) ? FCFD_OK : FCFD_ERR_NOT_CONVERGED;
-    // mean(p) = 0 as the reference's iteration enforces, then ghost copy
+    int rc = FCFD_OK, kmax = 0;
+    for (const Case_ &a : cs) {
+        kmax = std::max(kmax, a.k);
+        if (a.r > rel_tol && !(a.r <= 10.0 * rel_tol && a.since >= 4)) rc = FCFD_ERR_NOT_CONVERGED;
+    }
+    const double r = rmax();
+    // mean(p) = 0 per case as the reference's iteration enforces, then ghost copy
     FCFD_TRY(field_mean(c, p));
-    hipLaunchKernelGGL(k_sub_scalar, g, blk, 0, c->stream, p, L0, c->mg_scalars + SC_MEAN, 1.0 / ncell);
+    hipLaunchKernelGGL(k_sub_scalar, g, blk, 0, c->stream, p, L0, c->mg_scalars + SC_MEAN, 1.0 / ncell, DIM);
     FCFD_CHECK(hipGetLastError());
     FCFD_TRY(bc_scalar(c, p));
-    if (cycles) *cycles = k;
+    if (cycles) *cycles = kmax;
     if (rel) *rel = r;
-    c->diag.ppe_cycles = k;
+    c->diag.ppe_cycles = kmax;
     c->diag.ppe_rel_resid = r;
     return rc;
 }

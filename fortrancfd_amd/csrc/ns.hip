@@ -48,11 +48,16 @@ __global__ void __launch_bounds__(256) k_diff_source(const double *__restrict__ 
                                                      const double *__restrict__ Co, const double *__restrict__ NXo,
                                                      const double *__restrict__ NYo, const double *__restrict__ NZo,
                                                      double *__restrict__ Tx, double *__restrict__ Ty,
-                                                     double *__restrict__ Tz, Geom G, Phys P) {
+                                                     double *__restrict__ Tz, Geom G, Phys P,
+                                                     const fcfd_case *__restrict__ cp) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
+    if (cp) {   // a batched stack: this slice's case
+        const fcfd_case &q = cp[case_of<DIM>(k)];
+        P.re = q.re; P.m12 = q.mu_12; P.r12 = q.rho_12; P.fr = q.fr; P.we = q.we;
+    }
     const long id = G.id(i, j, k);
     const long sx = G.sx, sxy = G.sxy;
     const double dx = G.dx, dy = G.dy, dz = G.dz;
@@ -303,10 +308,11 @@ int ns_diff_source(fcfd_ctx *c) {
     if (G.dim == 3)
         hipLaunchKernelGGL(k_diff_source<3>, grd, blk, 0, c->stream, F[FCFD_PHI_OLD], F[FCFD_U_OLD], F[FCFD_V_OLD],
                            F[FCFD_W_OLD], F[FCFD_CURV], F[FCFD_NX], F[FCFD_NY], F[FCFD_NZ], c->dsu, c->dsv, c->dsw,
-                           G, P);
+                           G, P, (const fcfd_case *)nullptr);
     else
         hipLaunchKernelGGL(k_diff_source<2>, grd, blk, 0, c->stream, F[FCFD_PHI_OLD], F[FCFD_U_OLD], F[FCFD_V_OLD],
-                           nullptr, F[FCFD_CURV], F[FCFD_NX], F[FCFD_NY], nullptr, c->dsu, c->dsv, nullptr, G, P);
+                           nullptr, F[FCFD_CURV], F[FCFD_NX], F[FCFD_NY], nullptr, c->dsu, c->dsv, nullptr, G, P,
+                           c->cases_dev);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }

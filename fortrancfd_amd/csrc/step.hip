@@ -179,7 +179,8 @@ extern "C" int fcfd_step(fcfd_ctx *c) {
     if (rc != FCFD_OK && rc != FCFD_ERR_NOT_CONVERGED) return rc;
     // ls_mv of the main loop: mass and volume of the new phi
     FCFD_TRY(mass_sums(c, c->f[FCFD_PHI], 0));
-    FCFD_CHECK(hipMemcpyAsync(c->red.host + 8, c->red.result, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FCFD_CHECK(hipMemcpy2DAsync(c->diag_host, 2 * sizeof(double), c->red.result, RES_STRIDE * sizeof(double),
+                                2 * sizeof(double), c->nb, hipMemcpyDeviceToHost, c->stream));
     c->diag_pending = 1;
     return rc;
 }
@@ -210,6 +211,14 @@ extern "C" int fcfd_problem_init(fcfd_ctx *c, int redis0) {
     FCFD_TRY(ls_mv(c, &mass, &vol));
     c->pr.imass = mass;
     c->diag.imass = mass;
+    for (int q = 0; q < c->nb; ++q) {
+        c->diag_host[2 * q] = c->red.host[q * RES_STRIDE];
+        c->diag_host[2 * q + 1] = c->red.host[q * RES_STRIDE + 1];
+    }
+    if (c->nb > 1) {   // every case of a batched stack keeps its own initial mass
+        for (int q = 0; q < c->nb; ++q) c->cases[q].imass = c->red.host[q * RES_STRIDE];
+        FCFD_TRY(upload_cases(c));
+    }
     c->time = 0.0;
     c->iter = 0;
     return FCFD_OK;

@@ -21,7 +21,7 @@ import os
 
 import numpy as np
 
-from .types import FIELD_ID, PPE_FP64, STATUS, Case, FcfdDiag
+from .types import FIELD_ID, PPE_FP64, STATUS, Case, FcfdCase, FcfdDiag
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfcfd.so")
@@ -64,6 +64,8 @@ def lib():
         "fcfd_node_vel": [vp], "fcfd_ns_check_convergence_div": [vp], "fcfd_ns_solver": [vp],
         "fcfd_switch": [vp], "fcfd_interface_solver": [vp], "fcfd_step": [vp],
         "fcfd_problem_init": [vp, i32],
+        "fcfd_create_batch": [vp, vp, i32, vp, C.POINTER(vp)], "fcfd_set_cases": [vp, vp],
+        "fcfd_get_cases": [vp, vp], "fcfd_get_case_diag": [vp, vp, vp, vp], "fcfd_batch_size": [vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -265,3 +267,76 @@ class Solver:
 
     def problem_init(self, redis0: bool = True) -> None:
         self._call("fcfd_problem_init", 1 if redis0 else 0)
+
+
+# parameters that may differ between the cases of one batched context
+CASE_KEYS = ("rho_12", "mu_12", "re", "we", "fr")
+
+
+class BatchSolver(Solver):
+    """Many independent 2D cases on one grid in one context
+    (fcfd_create_batch): case q is slice q of every field, and every call
+    advances all of them in the same launches.  The cases must agree on the
+    grid and on every shared parameter (time steps, gravity, switches,
+    boundary conditions, solver settings); density / viscosity ratios, Re,
+    We and Fr may differ.  Fields are pushed and pulled as arrays of shape
+    (ncases, ny + 6, nx + 6)."""
+
+    def __init__(self, cases, allow_unconverged: bool = False):
+        cases = list(cases)
+        if not cases:
+            raise ValueError("no cases")
+        c0 = cases[0]
+        if c0.dim != 2:
+            raise ValueError("batched contexts hold 2D cases")
+        shared = c0.param_struct()
+        for c in cases[1:]:
+            p = c.param_struct()
+            for name, _ in shared._fields_:
+                if name in CASE_KEYS or name in ("imass", "pad_"):
+                    continue
+                a, b = getattr(shared, name), getattr(p, name)
+                a = list(a) if hasattr(a, "__len__") else a
+                b = list(b) if hasattr(b, "__len__") else b
+                if a != b:
+                    raise ValueError(f"cases differ in the shared parameter {name}: {a} vs {b}")
+            if (c.nx, c.ny, c.xstart, c.xend, c.ystart, c.yend) != (c0.nx, c0.ny, c0.xstart, c0.xend, c0.ystart,
+                                                                   c0.yend):
+                raise ValueError("cases of one batch must share the grid")
+        self.case = c0
+        self.cases = cases
+        self.n = len(cases)
+        self.allow_unconverged = allow_unconverged
+        self._grid = c0.grid_struct()
+        self._params = shared
+        self._cases = (FcfdCase * self.n)(*[self._case_struct(c) for c in cases])
+        h = C.c_void_p()
+        self._check("fcfd_create_batch", lib().fcfd_create_batch(C.byref(self._grid), C.byref(self._params), self.n,
+                                                                 self._cases, C.byref(h)))
+        self.h = h
+
+    @staticmethod
+    def _case_struct(c: Case) -> FcfdCase:
+        p = c.param_struct()
+        return FcfdCase(**{k: getattr(p, k) for k in CASE_KEYS}, imass=p.imass)
+
+    @property
+    def shape(self):
+        return (self.n,) + tuple(self.case.shape)
+
+    def push(self, name: str, a) -> None:
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.float64)).reshape(-1)
+        if a.size != self.n * self.case.size:
+            raise ValueError(f"{name}: {a.size} values, the batch holds {self.n} x {self.case.size}")
+        self._call("fcfd_push", FIELD_ID[name], a.ctypes.data)
+
+    def pull(self, name: str) -> np.ndarray:
+        a = np.empty(self.n * self.case.size, dtype=np.float64)
+        self._call("fcfd_pull", FIELD_ID[name], a.ctypes.data)
+        return a.reshape(self.shape)
+
+    def case_diag(self) -> dict:
+        """Per-case mass, volume (after the last step or problem_init) and MPLS target mass."""
+        m, v, im = (np.zeros(self.n) for _ in range(3))
+        self._call("fcfd_get_case_diag", m.ctypes.data, v.ctypes.data, im.ctypes.data)
+        return {"mass": m, "vol": v, "imass": im}

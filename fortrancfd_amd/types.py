@@ -61,6 +61,14 @@ class FcfdParams(C.Structure):
     ]
 
 
+class FcfdCase(C.Structure):
+    """Per-case parameters of a batched 2D stack (fcfd_case)."""
+    _fields_ = [
+        ("rho_12", C.c_double), ("mu_12", C.c_double),
+        ("re", C.c_double), ("we", C.c_double), ("fr", C.c_double), ("imass", C.c_double),
+    ]
+
+
 class FcfdDiag(C.Structure):
     _fields_ = [
         ("mass", C.c_double), ("vol", C.c_double), ("imass", C.c_double),

@@ -32,6 +32,28 @@ typedef struct fcfd_ctx fcfd_ctx;
  * Fields start at zero. */
 int fcfd_create(const fcfd_grid *grid, const fcfd_params *params, fcfd_ctx **out);
 int fcfd_destroy(fcfd_ctx *ctx);
+/* Batched 2D cases: ncases independent 2D problems on the same grid (grid->dim
+ * = 2, nz = 1) in one context.  No counterpart in the reference, which runs
+ * one case per process; this is the multi-case context of the batched
+ * rising-bubble workload.  Case q is slice k = q of every field: a field
+ * holds ncases consecutive (nx+6)(ny+6) planes, each with its own ghosts, and
+ * fcfd_push / fcfd_pull move all of them.  Every entry point below advances
+ * all cases in the same launches; reductions, the MPLS and reinitialisation
+ * stop rules and the pressure solve's stopping test are per case, so each
+ * case ends as it would alone.  params holds what the cases share; cases[q]
+ * their density / viscosity ratios, Re, We, Fr and MPLS mass (problem_init
+ * resets imass). */
+int fcfd_create_batch(const fcfd_grid *grid, const fcfd_params *params, int ncases, const fcfd_case *cases,
+                      fcfd_ctx **out);
+/* Number of cases of a context (1 for fcfd_create). */
+int fcfd_batch_size(const fcfd_ctx *ctx);
+/* Replace / read the per-case parameters of a batched context (FCFD_ERR_ARG otherwise). */
+int fcfd_set_cases(fcfd_ctx *ctx, const fcfd_case *cases);
+int fcfd_get_cases(const fcfd_ctx *ctx, fcfd_case *cases);
+/* Per-case mass and volume after the last fcfd_step (or fcfd_problem_init)
+ * and the MPLS reference mass, into arrays of fcfd_batch_size entries (any
+ * may be NULL). */
+int fcfd_get_case_diag(fcfd_ctx *ctx, double *mass, double *vol, double *imass);
 /* Replace the parameters (dt, tolerances ...) of an existing context. */
 int fcfd_set_params(fcfd_ctx *ctx, const fcfd_params *params);
 /* Number of float64 values of one ghosted field. */

@@ -123,6 +123,16 @@ typedef struct fcfd_diag {
     int32_t iter;                /* time step counter (p%glb%iter)             */
 } fcfd_diag;
 
+/* Per-case parameters of a batched 2D context (fcfd_create_batch): the
+ * physical numbers that differ between the cases of one stack.  Every other
+ * parameter (dt, rdt, ls_wid, gravity, switches, tolerances, BCs) is shared
+ * and comes from fcfd_params. */
+typedef struct fcfd_case {
+    double rho_12, mu_12;        /* density / viscosity ratio                 */
+    double re, we, fr;           /* Reynolds, Weber, Froude numbers            */
+    double imass;                /* MPLS reference mass                        */
+} fcfd_case;
+
 #ifdef __cplusplus
 }
 #endif

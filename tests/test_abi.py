@@ -54,7 +54,7 @@ def _c_layout(struct: str, fields: list[str]) -> dict:
 
 
 @pytest.mark.parametrize("cname,pyname", [("fcfd_grid", "FcfdGrid"), ("fcfd_params", "FcfdParams"),
-                                          ("fcfd_diag", "FcfdDiag")])
+                                          ("fcfd_diag", "FcfdDiag"), ("fcfd_case", "FcfdCase")])
 def test_ctypes_structs_match_the_c_layout(cname, pyname):
     from fortrancfd_amd import types
     py = getattr(types, pyname)
