@@ -1499,13 +1499,15 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
             // per launch (one 123 KB block per CU) while the tiles fit in one
             // round on the device, else one sweep per launch (two blocks per CU)
             const long n0c = (long)c->mg[0].nx * c->mg[0].ny * c->mg[0].nz;
-            if (lv >= 1 && M.e32b && sweeps >= 2 && 32L * M.nx * M.ny * M.nz <= n0c && !dt && !zinit) {
+            const int fused = c->pr.ppe_fused;   // 0 auto, 1 / 2 sweeps per launch, 3 off
+            if (fused != 3 && lv >= 1 && M.e32b && sweeps >= 2 && 32L * M.nx * M.ny * M.nz <= n0c && !dt &&
+                !zinit) {
                 using FT = FusedTile<DIM>;
                 const dim3 gf((M.nx + FT::TX - 1) / FT::TX, (M.ny + FT::TY - 1) / FT::TY,
                               DIM == 3 ? (M.nz + FT::TZ - 1) / FT::TZ : M.nz);
                 const int c0 = reverse ? 1 : 0;
                 int n2, n1;
-                if ((long)gf.x * gf.y * gf.z <= c->ncu) {
+                if (fused == 2 || (fused == 0 && (long)gf.x * gf.y * gf.z <= c->ncu)) {
                     n2 = sweeps / 2;
                     n1 = sweeps % 2;
                     if ((n2 + n1) % 2) { n2 -= 1; n1 += 2; }

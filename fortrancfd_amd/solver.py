@@ -120,6 +120,9 @@ class Solver:
 
     # ---- parameters / fields -------------------------------------------
     def set_params(self, **kw):
+        if "imass" not in kw:
+            # keep the MPLS target that problem_init measured on the device
+            self.case.imass = self.diag()["imass"]
         for k, v in kw.items():
             setattr(self.case, k, v)
         self._params = self.case.param_struct()
@@ -293,7 +296,7 @@ class BatchSolver(Solver):
         for c in cases[1:]:
             p = c.param_struct()
             for name, _ in shared._fields_:
-                if name in CASE_KEYS or name in ("imass", "pad_"):
+                if name in CASE_KEYS or name == "imass":
                     continue
                 a, b = getattr(shared, name), getattr(p, name)
                 a = list(a) if hasattr(a, "__len__") else a
@@ -314,6 +317,31 @@ class BatchSolver(Solver):
         self._check("fcfd_create_batch", lib().fcfd_create_batch(C.byref(self._grid), C.byref(self._params), self.n,
                                                                  self._cases, C.byref(h)))
         self.h = h
+
+    def set_params(self, **kw):
+        """Change shared parameters of every case (per-case ones go through set_cases)."""
+        bad = [k for k in kw if k in CASE_KEYS or k == "imass"]
+        if bad:
+            raise ValueError(f"per-case parameters {bad}: use set_cases")
+        for c in self.cases:
+            for k, v in kw.items():
+                setattr(c, k, v)
+        self._params = self.case.param_struct()
+        self._call("fcfd_set_params", C.byref(self._params))
+
+    def set_cases(self, **per_case) -> None:
+        """Per-case parameters: keyword -> sequence of n values (rho_12, mu_12, re, we, fr)."""
+        cur = (FcfdCase * self.n)()
+        self._call("fcfd_get_cases", cur)
+        for k, vals in per_case.items():
+            if k not in CASE_KEYS:
+                raise ValueError(f"{k} is not a per-case parameter")
+            if len(vals) != self.n:
+                raise ValueError(f"{k}: {len(vals)} values for {self.n} cases")
+            for q, v in enumerate(vals):
+                setattr(cur[q], k, float(v))
+                setattr(self.cases[q], k, float(v))
+        self._call("fcfd_set_cases", cur)
 
     @staticmethod
     def _case_struct(c: Case) -> FcfdCase:

@@ -69,7 +69,10 @@ typedef struct fcfd_params {
     int32_t ppe_nu1, ppe_nu2;    /* pre/post RB-GS sweeps per level (0: 2, 2)  */
     int32_t ppe_nu_coarse;       /* pre/post sweeps on levels of <= 1/32 of the
                                     finest cells (0: 6)                         */
-    int32_t pad_;
+    int32_t ppe_fused;           /* FP32 coarse-level sweeps: 0 auto (two per
+                                    launch while the tiles fit the device in
+                                    one round, else one), 1 one per launch,
+                                    2 two per launch, 3 one colour per launch */
 } fcfd_params;
 
 /* Level-set advection source of level_set_rk3_solver: the reference's WENO5

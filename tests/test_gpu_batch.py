@@ -71,3 +71,36 @@ def test_batch_matches_single_cases_bitwise(precision):
     # the cases are genuinely different problems
     assert not np.array_equal(b.pull("p")[0], b.pull("p")[2])
 
+
+
+def test_autotuner_keeps_the_state_and_picks_a_candidate():
+    from fortrancfd_amd.autotune import SCHEDULES, tune_ppe_schedule
+    from fortrancfd_amd.solver import BatchSolver
+    cs = _cases(precision=1)
+    b = BatchSolver([c for c, _ in cs])
+    b.push("phi", np.stack([phi for _, phi in cs]))
+    b.problem_init(False)
+    b.step()
+    before = {k: b.pull(k) for k in ("phi", "u", "v", "p")}
+    imass = b.case_diag()["imass"].copy()
+    res = tune_ppe_schedule(b, reps=1)
+    assert res["best"] in SCHEDULES and len(res["ms"]) >= 1
+    for k, a in before.items():
+        assert np.array_equal(b.pull(k), a), k
+    assert np.array_equal(b.case_diag()["imass"], imass)
+    for k, v in res["best"].items():
+        assert all(getattr(c, k) == v for c in b.cases)
+    b.step()   # steps on with the chosen schedule
+    assert np.isfinite(b.pull("p")).all()
+
+
+def test_set_params_keeps_the_measured_mpls_mass():
+    from fortrancfd_amd import Solver
+    c, phi = _cases()[0]
+    s = Solver(c)
+    s.push("phi", phi)
+    s.problem_init(False)
+    m0 = s.diag()["imass"]
+    assert m0 > 0
+    s.set_params(ppe_nu_coarse=4)
+    assert s.diag()["imass"] == m0
