@@ -44,9 +44,11 @@ def region_bytes(case, mixed: bool) -> dict:
     """Algorithmic bytes per launch of each kernel region: every distinct
     array a kernel reads or writes counted once per interior cell (ghosts,
     halo re-reads and reduction partials excluded), 8 B per FP64 value,
-    4 B per FP32 value."""
+    4 B per FP32 value, 2 B per FP16 value (the level-0 density of the FP32
+    cycle with ppe_rho_half)."""
     N = case.nx * case.ny * case.nz
     b = 4 if mixed else 8
+    br = 2 if (mixed and case.ppe_rho_half) else b   # density bytes on level 0
     nchild0 = 4 if case.dz > 1.5 * case.dx else 8   # first coarsening: x,y only when dz = 2 dx
     return {
         "ls_adv_stage": 8 * N * 20 / 3,        # R{phi,nu,nv,nw,l1..} W{phi',l} averaged over 3 stages
@@ -61,13 +63,13 @@ def region_bytes(case, mixed: bool) -> dict:
         "node_vel": 8 * N * 6,
         "mpls_update": 8 * N * 3,
         # per colour (colour-split FP32 arrays in the mixed cycle): R{e of the
-        # other colour (N/2), f of this colour (N/2), rho (N)} W{e of this
-        # colour (N/2)}; averaged over the 8 launches of a V(2,2) cycle:
+        # other colour (N/2), f of this colour (N/2), rho (N, FP32 or FP16)}
+        # W{e of this colour (N/2)}; averaged over the 8 launches of a V(2,2) cycle:
         # + R{ec} (N/nchild0 coarse values) in the first post-sweep half
         # (prolongation folded in) and + R{r (FP64), z_old} in the last
         # half-sweep (CG dots folded in)
-        "mg_smooth_l0": N * (8 * b * 2.5 + b / nchild0 + (8 + b)) / 8,
-        "mg_restrict_l0": b * N * (3 + 2.0 / nchild0),   # R{e,f,rho} W{fc,ec}
+        "mg_smooth_l0": N * (8 * (1.5 * b + br) + b / nchild0 + (8 + b)) / 8,
+        "mg_restrict_l0": N * (2 * b + br + b * 2.0 / nchild0),   # R{e,f,rho} W{fc,ec}
         "mg_resid_l0": N * (3 * 8 + (2 * 4 if mixed else 0)),  # R{p,f,rho} (W{r32,e32})
         # flexible CG, averaged over its two launches per iteration:
         # dir R{z, p, q, x, rho} W{p, q, x, z_old}; update R{q, r} W{r, r32}
@@ -414,6 +416,7 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=0, help="run the batched 2D workload with N cases")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--regions-out", default="")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default smoother schedule")
     ap.add_argument("--krylov", type=int, default=1)
     ap.add_argument("--inner", type=int, default=1)
     args = ap.parse_args()
@@ -442,6 +445,13 @@ def main() -> int:
     for w in range(args.warmup):
         s.step()
         print(f"warmup step {w + 1}/{args.warmup} {s.diag()}", file=sys.stderr, flush=True)
+    tuned = None
+    if not args.no_tune:
+        # smoother schedule for this system (untimed; the pressure is restored)
+        from fortrancfd_amd.autotune import tune_ppe_schedule
+        tuned = tune_ppe_schedule(s)
+        print(f"autotuned schedule {tuned['best']} ({tuned['ms']} ms)", file=sys.stderr, flush=True)
+        s.step()   # captures the chosen schedule's cycle graphs
     s.sync()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -501,14 +511,18 @@ def main() -> int:
         "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"rising_bubble_3d_{case.nx}x{case.ny}x{case.nz}", "grid": [case.nx, case.ny, case.nz],
-                   "domain": [1.0, 1.0, 2.0], "ppe": ("fp32" if mixed else "fp64") + f" V(2,2) x{args.inner}" + (" preconditioning fp64 flexible CG" if args.krylov else (" in fp64 defect correction" if mixed else "")),
+                   "domain": [1.0, 1.0, 2.0], "ppe": ("fp32" if mixed else "fp64") + f" V({case.ppe_nu1},{case.ppe_nu2}) x{case.ppe_inner}" + (" preconditioning fp64 flexible CG" if args.krylov else (" in fp64 defect correction" if mixed else "")),
                    "ppe_tol": 1e-10, "picard": 4, "redis_interval": 5},
         "roofline": roofline,
         "step_hbm_frac_measured_kernels": round(tot_bytes / (tot_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms_per_step": round(tot_ms / args.steps, 3),
         "profiled_ms_per_step": round((tp1 - tp0) * 1e3 / args.steps, 3),
         "ppe_cycles_last_step": d["ppe_cycles_step"],
+        "ppe_schedule": {k: getattr(case, k) for k in ("ppe_nu1", "ppe_nu2", "ppe_nu_coarse", "ppe_inner",
+                                                        "ppe_fused", "ppe_rho_half")},
     }
+    if tuned:
+        out["autotune_candidate_ms"] = tuned["ms"]
     if not args.no_mg:
         out["mg"] = mg_compare(s, case)
     if args.regions_out:

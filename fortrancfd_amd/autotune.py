@@ -2,8 +2,9 @@
 
 The multigrid's cost per solve depends on the smoother schedule (sweeps per
 level, extra sweeps on the small levels, V-cycles per flexible-CG iteration)
-and on how the FP32 coarse-level sweeps are tiled (two sweeps per launch
-with a 4-cell halo, one with a 2-cell halo, or one colour per launch), and
+on how the FP32 coarse-level sweeps are tiled (two sweeps per launch with
+a 4-cell halo, one with a 2-cell halo, or one colour per launch) and on
+whether the FP32 level-0 sweeps read the density in FP16, and
 the best choice moves with the grid and the number of cases in a batched
 context: fewer, larger launches pay off where the device is underfilled.
 `tune_ppe_schedule` times the solve of the solver's current pressure system
@@ -18,19 +19,17 @@ import time
 from .solver import FcfdError
 
 # (pre, post) sweeps, sweeps on the small levels, V-cycles per outer
-# iteration, FP32 coarse tiling (0 auto, 1 / 2 sweeps per launch, 3 colour launches)
+# iteration, FP32 coarse tiling (0 auto, 1 / 2 sweeps per launch, 3 colour
+# launches), FP16 density in the FP32 level-0 sweeps
+def _s(nu1, nu2, nuc, inner, fused, half=1):
+    return dict(ppe_nu1=nu1, ppe_nu2=nu2, ppe_nu_coarse=nuc, ppe_inner=inner, ppe_fused=fused, ppe_rho_half=half)
+
+
 SCHEDULES = [
-    dict(ppe_nu1=2, ppe_nu2=2, ppe_nu_coarse=6, ppe_inner=1, ppe_fused=0),
-    dict(ppe_nu1=2, ppe_nu2=2, ppe_nu_coarse=6, ppe_inner=1, ppe_fused=1),
-    dict(ppe_nu1=2, ppe_nu2=2, ppe_nu_coarse=6, ppe_inner=1, ppe_fused=2),
-    dict(ppe_nu1=2, ppe_nu2=2, ppe_nu_coarse=6, ppe_inner=1, ppe_fused=3),
-    dict(ppe_nu1=1, ppe_nu2=1, ppe_nu_coarse=4, ppe_inner=1, ppe_fused=0),
-    dict(ppe_nu1=2, ppe_nu2=2, ppe_nu_coarse=4, ppe_inner=1, ppe_fused=0),
-    dict(ppe_nu1=2, ppe_nu2=2, ppe_nu_coarse=8, ppe_inner=1, ppe_fused=0),
-    dict(ppe_nu1=3, ppe_nu2=3, ppe_nu_coarse=6, ppe_inner=1, ppe_fused=0),
-    dict(ppe_nu1=1, ppe_nu2=1, ppe_nu_coarse=6, ppe_inner=2, ppe_fused=0),
+    _s(2, 2, 6, 1, 0), _s(2, 2, 6, 1, 0, half=0), _s(2, 2, 6, 1, 1), _s(2, 2, 6, 1, 2), _s(2, 2, 6, 1, 3),
+    _s(1, 1, 4, 1, 0), _s(2, 2, 4, 1, 0), _s(2, 2, 8, 1, 0), _s(3, 3, 6, 1, 0), _s(1, 1, 6, 2, 0),
 ]
-KEYS = ("ppe_nu1", "ppe_nu2", "ppe_nu_coarse", "ppe_inner", "ppe_fused")
+KEYS = ("ppe_nu1", "ppe_nu2", "ppe_nu_coarse", "ppe_inner", "ppe_fused", "ppe_rho_half")
 
 
 def tune_ppe_schedule(solver, candidates=SCHEDULES, reps: int = 2, tol: float = None, precision: int = None) -> dict:

@@ -262,6 +262,7 @@ static int create_impl(const fcfd_grid *grid, const fcfd_params *params, int nca
     if (rc == FCFD_OK) rc = upload_cases(c);
     if (rc == FCFD_OK) rc = ccd_setup(c);
     c->rho32 = nullptr;
+    c->rho16 = nullptr;
     c->mg_scalars = nullptr;
     c->mg_ready = 0;
     c->time = 0.0;

@@ -116,6 +116,7 @@ struct fcfd_ctx {
     // ---- multigrid ----
     std::vector<fcfd::MgLevel> mg;
     float *rho32;              // FP32 copy of rho for the mixed inner cycle
+    _Float16 *rho16;           // FP16 copy (params.ppe_rho_half)
     double *mg_scalars;        // device scalars of the solve (norms)
     int mg_ready;
     // level-0 Krylov vectors of the flexible CG (FP64) and FP32 previous z

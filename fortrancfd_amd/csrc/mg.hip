@@ -296,9 +296,9 @@ __global__ void __launch_bounds__(256) k_smooth(T *__restrict__ e, const T *__re
 // updated); 2: zero initial guess, the neighbours are 0 (first pre-sweep
 // half, so the error array needs no clearing).  DOTS: the CG dots of the
 // thread's pair (2m, 2m+1) as in k_smooth.
-template <int DIM, int MODE, bool DOTS>
+template <int DIM, int MODE, bool DOTS, typename RT>
 __global__ void __launch_bounds__(256) k_smooth_split(float *__restrict__ e, const float *__restrict__ f,
-                                                      const float *__restrict__ rho, Split S, Inv2 ih, int color,
+                                                      const RT *__restrict__ rho, Split S, Inv2 ih, int color,
                                                       Prol<float> P, Dots<float> D, Lay L) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
@@ -309,18 +309,19 @@ __global__ void __launch_bounds__(256) k_smooth_split(float *__restrict__ e, con
     const bool upd = i < S.nx && j < S.ny;
     float enew = 0.f;
     if (upd) {
-        const float *ro = rho + oc, *eo = e + oc;
-        const float r = rho[cc + id];
+        const RT *ro = rho + oc;
+        const float *eo = e + oc;
+        const float r = (float)rho[cc + id];
         // the x neighbours i-1 and i+1 are slots m-1+s and m+s of the other colour
         const long xw = id - 1 + s, xe = id + s;
-        const float cE = face<float>(i < S.nx - 1, r, ro[xe], (float)ih.x);
-        const float cW = face<float>(i > 0, r, ro[xw], (float)ih.x);
-        const float cN = face<float>(j < S.ny - 1, r, ro[id + S.sx], (float)ih.y);
-        const float cS = face<float>(j > 0, r, ro[id - S.sx], (float)ih.y);
+        const float cE = face<float>(i < S.nx - 1, r, (float)ro[xe], (float)ih.x);
+        const float cW = face<float>(i > 0, r, (float)ro[xw], (float)ih.x);
+        const float cN = face<float>(j < S.ny - 1, r, (float)ro[id + S.sx], (float)ih.y);
+        const float cS = face<float>(j > 0, r, (float)ro[id - S.sx], (float)ih.y);
         float cT = 0.f, cB = 0.f;
         if (DIM == 3) {
-            cT = face<float>(k < S.nz - 1, r, ro[id + S.sxy], (float)ih.z);
-            cB = face<float>(k > 0, r, ro[id - S.sxy], (float)ih.z);
+            cT = face<float>(k < S.nz - 1, r, (float)ro[id + S.sxy], (float)ih.z);
+            cB = face<float>(k > 0, r, (float)ro[id - S.sxy], (float)ih.z);
         }
         auto nb = [&](long o, int ii, int jj, int kk) -> float {
             float v = eo[o];
@@ -364,32 +365,32 @@ __global__ void __launch_bounds__(256) k_smooth_split(float *__restrict__ e, con
 }
 
 // (A e)_P of the colour-split FP32 level 0, difference form as apply_A
-template <int DIM>
-__device__ __forceinline__ float apply_A_split(const float *__restrict__ e, const float *__restrict__ rho,
+template <int DIM, typename RT>
+__device__ __forceinline__ float apply_A_split(const float *__restrict__ e, const RT *__restrict__ rho,
                                                const Split &S, Inv2 ih, int i, int j, int k) {
     const long a = S.at(i, j, k);
-    const float ep = e[a], r = rho[a];
+    const float ep = e[a], r = (float)rho[a];
     long n = S.at(i + 1, j, k);
-    float acc = face<float>(i < S.nx - 1, r, rho[n], (float)ih.x) * (e[n] - ep);
+    float acc = face<float>(i < S.nx - 1, r, (float)rho[n], (float)ih.x) * (e[n] - ep);
     n = S.at(i - 1, j, k);
-    acc += face<float>(i > 0, r, rho[n], (float)ih.x) * (e[n] - ep);
+    acc += face<float>(i > 0, r, (float)rho[n], (float)ih.x) * (e[n] - ep);
     n = S.at(i, j + 1, k);
-    acc += face<float>(j < S.ny - 1, r, rho[n], (float)ih.y) * (e[n] - ep);
+    acc += face<float>(j < S.ny - 1, r, (float)rho[n], (float)ih.y) * (e[n] - ep);
     n = S.at(i, j - 1, k);
-    acc += face<float>(j > 0, r, rho[n], (float)ih.y) * (e[n] - ep);
+    acc += face<float>(j > 0, r, (float)rho[n], (float)ih.y) * (e[n] - ep);
     if (DIM == 3) {
         n = S.at(i, j, k + 1);
-        acc += face<float>(k < S.nz - 1, r, rho[n], (float)ih.z) * (e[n] - ep);
+        acc += face<float>(k < S.nz - 1, r, (float)rho[n], (float)ih.z) * (e[n] - ep);
         n = S.at(i, j, k - 1);
-        acc += face<float>(k > 0, r, rho[n], (float)ih.z) * (e[n] - ep);
+        acc += face<float>(k > 0, r, (float)rho[n], (float)ih.z) * (e[n] - ep);
     }
     return acc;
 }
 
 // k_resid_restrict for the colour-split FP32 level 0
-template <int DIM>
+template <int DIM, typename RT>
 __global__ void __launch_bounds__(256) k_resid_restrict_split(const float *__restrict__ e, const float *__restrict__ f,
-                                                              const float *__restrict__ rho, Split S, Inv2 ih,
+                                                              const RT *__restrict__ rho, Split S, Inv2 ih,
                                                               float *__restrict__ fc, float *__restrict__ ec, Lay C,
                                                               int cx, int cy, int cz) {
     int bx_, by_, bz_;
@@ -401,20 +402,24 @@ __global__ void __launch_bounds__(256) k_resid_restrict_split(const float *__res
         for (int jj = 0; jj < cy; ++jj)
             for (int ii = 0; ii < cx; ++ii) {
                 const int i = I * cx + ii, j = J * cy + jj, k = K * cz + kk;
-                acc += f[S.at(i, j, k)] - apply_A_split<DIM>(e, rho, S, ih, i, j, k);
+                acc += f[S.at(i, j, k)] - apply_A_split<DIM, RT>(e, rho, S, ih, i, j, k);
             }
     const long cid = C.id(I, J, K);
     fc[cid] = acc / (float)(cx * cy * cz);
     ec[cid] = 0.f;
 }
 
-// FP32 density of level 0 in the colour-split layout
-__global__ void k_rho_split(const double *__restrict__ rho, Lay L, float *__restrict__ out, Split S) {
+// FP32 (and, for the FP16-density smoother, FP16) density of level 0 in the
+// colour-split layout
+__global__ void k_rho_split(const double *__restrict__ rho, Lay L, float *__restrict__ out,
+                            _Float16 *__restrict__ out16, Split S) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= L.nx || j >= L.ny) return;
-    out[S.at(i, j, k)] = (float)rho[L.id(i, j, k)];
+    const double r = rho[L.id(i, j, k)];
+    out[S.at(i, j, k)] = (float)r;
+    if (out16) out16[S.at(i, j, k)] = (_Float16)r;
 }
 
 // NS full red-black sweeps of a coarse level (stored face coefficients,
@@ -1305,6 +1310,7 @@ int mg_free(fcfd_ctx *c) {
     }
     c->mg.clear();
     if (c->rho32) hipFree(c->rho32);
+    if (c->rho16) hipFree(c->rho16);
     if (c->mg_scalars) hipFree(c->mg_scalars);
     void *kv[] = {c->kr, c->kp, c->kq, c->kz, c->kzo, c->kzo32};
     for (void *p : kv)
@@ -1312,6 +1318,7 @@ int mg_free(fcfd_ctx *c) {
     c->kr = c->kp = c->kq = c->kz = c->kzo = nullptr;
     c->kzo32 = nullptr;
     c->rho32 = nullptr;
+    c->rho16 = nullptr;
     c->mg_scalars = nullptr;
     c->mg_ready = 0;
     return mg_drop_graphs(c);
@@ -1388,6 +1395,7 @@ static int mg_build(fcfd_ctx *c) {
     FCFD_TRY(alloc0(&M0.e32, nsplit));
     FCFD_TRY(alloc0(&M0.f32, nsplit));
     FCFD_TRY(alloc0(&c->rho32, nsplit));
+    FCFD_TRY(alloc0(&c->rho16, nsplit));
     FCFD_TRY(alloc0(&M0.f64, G.ntot));   // zero-mean working rhs
     FCFD_TRY(alloc0(&c->kr, G.ntot));
     FCFD_TRY(alloc0(&c->kp, G.ntot));
@@ -1441,7 +1449,7 @@ int mg_setup(fcfd_ctx *c) {
         if (G.dim == 3) hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, c->stream, C.bz64, C.bz32, n);
     }
     hipLaunchKernelGGL(k_rho_split, grd3(G.nx, G.ny, G.nz, blk), blk, 0, c->stream, c->f[FCFD_RHO], lay_of(G),
-                       c->rho32, split_of(G));
+                       c->rho32, c->rho16, split_of(G));
     FCFD_CHECK(hipGetLastError());
     {
         MgLevel &M = c->mg.back();
@@ -1471,6 +1479,8 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
     dim3 blk(64, 4);
     const int nl = (int)c->mg.size();
     if (dots && nu2 < 1) return FCFD_ERR_ARG;        // the dots come from the last post-smoothing sweep
+    // FP32 cycle: level 0 may read the density in FP16 (the split copy rho16)
+    const bool half_rho = sizeof(T) == 4 && c->pr.ppe_rho_half != 0 && c->rho16;
     if (zero_init && nu1 < 1) return FCFD_ERR_ARG;   // the first pre-sweep stands in for clearing e
     auto arrays = [&](int lv, T *&e, const T *&f, const T *&bx, const T *&by, const T *&bz) {
         MgLevel &M = c->mg[lv];
@@ -1547,7 +1557,14 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
                         // prolongation or zero-guess sweep is a plain one
                         const Split S = split_of(G);
 #define FCFD_SPLIT(MD, DM)                                                                                     \
-    hipLaunchKernelGGL((k_smooth_split<DIM, MD, DM>), g, blk, 0, c->stream, e, f, rho0, S, ih, color, P0, D0, L)
+    do {                                                                                                       \
+        if (half_rho)                                                                                          \
+            hipLaunchKernelGGL((k_smooth_split<DIM, MD, DM, _Float16>), g, blk, 0, c->stream, e, f,            \
+                               (const _Float16 *)c->rho16, S, ih, color, P0, D0, L);                          \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_smooth_split<DIM, MD, DM, float>), g, blk, 0, c->stream, e, f,               \
+                               (const float *)rho0, S, ih, color, P0, D0, L);                                 \
+    } while (0)
                         if (dm) FCFD_SPLIT(0, true);
                         else if (pm == 1) FCFD_SPLIT(1, false);
                         else if (pm == 3) FCFD_SPLIT(2, false);
@@ -1618,9 +1635,16 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         dim3 g = grd3(C.nx, C.ny, C.nz, blk);
         ProfScope ps_(c, lv == 0 ? R_RESTRICT0 : R_RESTRICTC);
         if (lv == 0) {
-            if constexpr (sizeof(T) == 4)
-                hipLaunchKernelGGL((k_resid_restrict_split<DIM>), g, blk, 0, c->stream, e, f, rho0, split_of(G), ih,
-                                   (T *)fc_, ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
+            if constexpr (sizeof(T) == 4) {
+                if (half_rho)
+                    hipLaunchKernelGGL((k_resid_restrict_split<DIM, _Float16>), g, blk, 0, c->stream, e, f,
+                                       (const _Float16 *)c->rho16, split_of(G), ih, (T *)fc_, ec, LC, M.cx, M.cy,
+                                       DIM == 3 ? M.cz : 1);
+                else
+                    hipLaunchKernelGGL((k_resid_restrict_split<DIM, float>), g, blk, 0, c->stream, e, f,
+                                       (const float *)rho0, split_of(G), ih, (T *)fc_, ec, LC, M.cx, M.cy,
+                                       DIM == 3 ? M.cz : 1);
+            }
             else
                 hipLaunchKernelGGL((k_resid_restrict<T, DIM, true>), g, blk, 0, c->stream, e, f, rho0, bx, by, bz, L,
                                    ih, (T *)fc_, ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);

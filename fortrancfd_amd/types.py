@@ -57,7 +57,7 @@ class FcfdParams(C.Structure):
         ("redis_interval", C.c_int32), ("picard_iters", C.c_int32),
         ("ppe_precision", C.c_int32), ("ppe_max_cycles", C.c_int32), ("ppe_krylov", C.c_int32),
         ("ppe_inner", C.c_int32), ("ls_scheme", C.c_int32), ("ppe_nu1", C.c_int32), ("ppe_nu2", C.c_int32),
-        ("ppe_nu_coarse", C.c_int32), ("ppe_fused", C.c_int32),
+        ("ppe_nu_coarse", C.c_int32), ("ppe_fused", C.c_int32), ("ppe_rho_half", C.c_int32),
     ]
 
 
@@ -128,6 +128,7 @@ class Case:
     ppe_nu2: int = 2
     ppe_nu_coarse: int = 6      # sweeps on levels of <= 1/32 of the finest cells
     ppe_fused: int = 0          # coarse FP32 sweeps: 0 auto, 1 / 2 per launch, 3 colour launches
+    ppe_rho_half: int = 0       # 1: FP32 level-0 smoother reads the density in FP16
     imass: float = 0.0
     name: str = "case"
     # output schedule of the reference driver (manager_read): plot pieces
@@ -232,6 +233,7 @@ class Case:
         p.ls_scheme = self.ls_scheme
         p.ppe_nu1, p.ppe_nu2, p.ppe_nu_coarse = self.ppe_nu1, self.ppe_nu2, self.ppe_nu_coarse
         p.ppe_fused = self.ppe_fused
+        p.ppe_rho_half = self.ppe_rho_half
         return p
 
     def ug(self) -> int:
