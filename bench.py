@@ -37,7 +37,7 @@ def rising_case(n: int, precision: int, tol: float):
     from fortrancfd_amd.types import Case
     return Case(dim=3, nx=n, ny=n, nz=n, xend=1.0, yend=1.0, zend=2.0, gvec=(0.0, 0.0, -1.0),
                 cfl=0.1, cfl_r=0.1, ppe_tol=tol, ppe_precision=precision, redis_interval=5,
-                picard_iters=4, ppe_max_cycles=100, name="rising_bubble_3d")
+                picard_iters=4, ppe_max_cycles=100, ppe_rho_half=1, name="rising_bubble_3d")
 
 
 def region_bytes(case, mixed: bool) -> dict:

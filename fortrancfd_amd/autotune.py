@@ -32,10 +32,14 @@ SCHEDULES = [
 KEYS = ("ppe_nu1", "ppe_nu2", "ppe_nu_coarse", "ppe_inner", "ppe_fused", "ppe_rho_half")
 
 
-def tune_ppe_schedule(solver, candidates=SCHEDULES, reps: int = 2, tol: float = None, precision: int = None) -> dict:
+def tune_ppe_schedule(solver, candidates=SCHEDULES, reps: int = 2, tol: float = None, precision: int = None,
+                      margin: float = 0.02) -> dict:
     """Pick the fastest smoother schedule for `solver` (a Solver or
     BatchSolver whose pressure operator is set up, e.g. after a step) and
-    apply it.  Returns {"best": schedule, "ms": {index: milliseconds}}."""
+    apply it.  The solver's current schedule is timed first and is replaced
+    only by a candidate at least `margin` faster, so timing noise does not
+    flip the choice between runs.  Returns {"best": schedule, "ms":
+    {"current" | index: milliseconds}}."""
     import numpy as np
     case = solver.case
     tol = case.ppe_tol if tol is None else tol
@@ -43,8 +47,8 @@ def tune_ppe_schedule(solver, candidates=SCHEDULES, reps: int = 2, tol: float = 
     keep = {k: getattr(case, k) for k in KEYS}
     p_saved = solver.pull("p")
     zeros = np.zeros_like(p_saved)
-    times = {}
-    for n, sched in enumerate(candidates):
+
+    def timed(sched):
         solver.set_params(**sched)
         best = None
         try:
@@ -58,13 +62,24 @@ def tune_ppe_schedule(solver, candidates=SCHEDULES, reps: int = 2, tol: float = 
                 if rep > 0:
                     best = ms if best is None else min(best, ms)
         except FcfdError:
-            best = None                          # did not converge: not a candidate
-        if best is not None:
-            times[n] = round(best, 3)
+            return None                          # did not converge: not a candidate
+        return round(best, 3)
+
+    times = {}
+    base = timed(keep)
+    if base is not None:
+        times["current"] = base
+    for n, sched in enumerate(candidates):
+        if sched == keep:
+            continue
+        t = timed(sched)
+        if t is not None:
+            times[n] = t
     solver.push("p", p_saved)
-    if not times:
-        solver.set_params(**keep)
-        return {"best": keep, "ms": times}
-    win = min(times, key=times.get)
-    solver.set_params(**candidates[win])
-    return {"best": dict(candidates[win]), "ms": times}
+    cand = {k: v for k, v in times.items() if k != "current"}
+    win = min(cand, key=cand.get) if cand else None
+    if win is not None and (base is None or cand[win] < (1.0 - margin) * base):
+        solver.set_params(**candidates[win])
+        return {"best": dict(candidates[win]), "ms": times}
+    solver.set_params(**keep)
+    return {"best": keep, "ms": times}
