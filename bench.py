@@ -309,11 +309,12 @@ def batch_cases(ncases: int = 64, seed: int = 0x5EEDF00D, precision: int = 1):
 
 
 def batch_bench(ncases: int = 64, steps: int = 500, warmup: int = 2, loop_steps: int = 5,
-                precision: int = 1) -> dict:
+                precision: int = 1, serial: bool = False) -> dict:
     """Aggregate throughput of the batched 2D workload: one batched context
     per grid size, each stepped by its own host thread (its own HIP stream),
     against the same cases stepped one after another in ordinary contexts
-    (a bounded sample of loop_steps steps per case)."""
+    (a bounded sample of loop_steps steps per case).  serial: the contexts
+    are stepped one after another from the main thread instead."""
     import threading
     from fortrancfd_amd import Solver
     from fortrancfd_amd.solver import BatchSolver
@@ -346,12 +347,16 @@ def batch_bench(ncases: int = 64, steps: int = 500, warmup: int = 2, loop_steps:
                                          f"{sorted(set(bad))}: " + ", ".join(
                                              f"rho_12={b.cases[q].rho_12:.4g} we={b.cases[q].we:.3g}"
                                              for q in sorted(set(bad)))))
-        th = [threading.Thread(target=worker, args=(b,)) for b in solvers]
         t = time.perf_counter()
-        for x in th:
-            x.start()
-        for x in th:
-            x.join()
+        if serial:
+            for b in solvers:
+                worker(b)
+        else:
+            th = [threading.Thread(target=worker, args=(b,)) for b in solvers]
+            for x in th:
+                x.start()
+            for x in th:
+                x.join()
         dt = time.perf_counter() - t
         if errs:
             raise errs[0]
@@ -391,6 +396,7 @@ def batch_bench(ncases: int = 64, steps: int = 500, warmup: int = 2, loop_steps:
             "unit": "Mcell-steps/s", "higher_is_better": True, "n_gpus": 1,
             "dtype": "f64 (FP32 V-cycles in FP64 flexible CG)" if precision else "f64", "data": "synthetic",
             "config": {"workload": f"batched_2d_rising_bubbles_{ncases}", "cases": ncases, "steps": steps,
+                       "host_threads": 1 if serial else len(solvers),
                        "grids": {f"{k[0]}x{k[1]}": len(v) for k, v in sorted(groups.items())},
                        "dt_over_dx": {f"{k[0]}x{k[1]}": round(v[0][0].cfl, 5) for k, v in sorted(groups.items())},
                        "total_cells": cells},
@@ -415,6 +421,7 @@ def main() -> int:
     ap.add_argument("--adjoint-steps", type=int, default=300)
     ap.add_argument("--batch", type=int, default=0, help="run the batched 2D workload with N cases")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--batch-serial", action="store_true", help="step the batched contexts from one host thread")
     ap.add_argument("--regions-out", default="")
     ap.add_argument("--no-tune", action="store_true", help="keep the default smoother schedule")
     ap.add_argument("--krylov", type=int, default=1)
@@ -431,7 +438,8 @@ def main() -> int:
         print(json.dumps(adjoint_bench(args.adjoint, args.adjoint_steps)))
         return 0
     if args.batch:
-        print(json.dumps(batch_bench(args.batch, args.batch_steps, precision=1 if args.precision == "mixed" else 0)))
+        print(json.dumps(batch_bench(args.batch, args.batch_steps, precision=1 if args.precision == "mixed" else 0,
+                                     serial=args.batch_serial)))
         return 0
 
     from fortrancfd_amd import Solver

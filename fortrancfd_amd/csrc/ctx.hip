@@ -180,6 +180,27 @@ static int alloc_field(double **p, long n) {
 }
 
 namespace fcfd {
+__global__ void k_gather8(const unsigned long long *__restrict__ src, long pitch, int width, int rows,
+                          unsigned long long *__restrict__ dst) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= width * rows) return;
+    dst[t] = src[(long)(t / width) * pitch + t % width];
+}
+
+int gather_to_host(fcfd_ctx *c, void *host, const void *dev, long pitch, int width, int rows) {
+    if (rows == 1) {
+        FCFD_CHECK(hipMemcpyAsync(host, dev, sizeof(unsigned long long) * width, hipMemcpyDeviceToHost, c->stream));
+        return FCFD_OK;
+    }
+    if (width * rows > 16 * c->nb) return FCFD_ERR_ARG;
+    hipLaunchKernelGGL(k_gather8, dim3((width * rows + 255) / 256), dim3(256), 0, c->stream,
+                       (const unsigned long long *)dev, pitch, width, rows, c->pack);
+    FCFD_CHECK(hipGetLastError());
+    FCFD_CHECK(hipMemcpyAsync(host, c->pack, sizeof(unsigned long long) * width * rows, hipMemcpyDeviceToHost,
+                              c->stream));
+    return FCFD_OK;
+}
+
 // per-case parameter table of a batched stack (nothing for nb == 1, whose
 // kernels take the scalars of the params)
 int upload_cases(fcfd_ctx *c) {
@@ -254,6 +275,8 @@ static int create_impl(const fcfd_grid *grid, const fcfd_params *params, int nca
     if (rc == FCFD_OK && hipHostMalloc((void **)&c->diag_host, 2 * nbc * sizeof(double)) != hipSuccess)
         rc = FCFD_ERR_HIP;
     if (rc == FCFD_OK) memset(c->diag_host, 0, 2 * nbc * sizeof(double));
+    if (rc == FCFD_OK && hipMalloc((void **)&c->pack, 16 * nbc * sizeof(unsigned long long)) != hipSuccess)
+        rc = FCFD_ERR_HIP;
     if (rc == FCFD_OK && hipMalloc((void **)&c->flag, FLAG_STRIDE * nbc * sizeof(unsigned long long)) != hipSuccess)
         rc = FCFD_ERR_HIP;
     if (rc == FCFD_OK &&
@@ -322,6 +345,7 @@ extern "C" int fcfd_destroy(fcfd_ctx *c) {
     if (c->red.host) hipHostFree(c->red.host);
     if (c->diag_host) hipHostFree(c->diag_host);
     if (c->cases_dev) hipFree(c->cases_dev);
+    if (c->pack) hipFree(c->pack);
     if (c->flag) hipFree(c->flag);
     if (c->flag_host) hipHostFree(c->flag_host);
     for (hipEvent_t e : c->prof.pool) hipEventDestroy(e);

@@ -134,6 +134,7 @@ struct fcfd_ctx {
     int iter;
     int diag_pending;          // step mass/volume of every case waiting in diag_host
     double *diag_host;         // pinned: mass, volume of case q at [2q], [2q+1]
+    unsigned long long *pack;  // device staging for per-case values gathered to the host (16 words per case)
     fcfd_diag diag;
     fcfd::Prof prof;
 };
@@ -213,6 +214,10 @@ inline dim3 grid_interior(const Geom &G, dim3 blk) {
 
 // kernels implemented in the .hip files (host wrappers)
 int upload_cases(fcfd_ctx *c);
+// rows of `width` 8-byte words, `pitch` words apart on the device, into
+// consecutive host words (one contiguous copy; a gather kernel first when
+// there is more than one row); stream ordered
+int gather_to_host(fcfd_ctx *c, void *host, const void *dev, long pitch, int width, int rows);
 int bc_scalar(fcfd_ctx *c, double *d);
 int bc_scalar_f32(fcfd_ctx *c, float *d);
 int velbc(fcfd_ctx *c, double *u, double *v, double *w, int nodal);

@@ -680,9 +680,7 @@ int redis(fcfd_ctx *c, int btn, int *iters) {
         hipLaunchKernelGGL(k_redis_check, dim3(c->nb), dim3(NSHARD), 0, c->stream, c->flag, tol);
     }
     // words 0..3 of every case's flag block
-    FCFD_CHECK(hipMemcpy2DAsync(c->flag_host, 4 * sizeof(unsigned long long), c->flag,
-                                FLAG_STRIDE * sizeof(unsigned long long), 4 * sizeof(unsigned long long), c->nb,
-                                hipMemcpyDeviceToHost, c->stream));
+    FCFD_TRY(gather_to_host(c, c->flag_host, c->flag, FLAG_STRIDE, 4, c->nb));
     FCFD_CHECK(hipStreamSynchronize(c->stream));
     int most = 0;   // a batched stack reports the largest pseudo-step count of its cases
     for (int q = 0; q < c->nb; ++q) most = std::max(most, (int)c->flag_host[4 * q + 2]);

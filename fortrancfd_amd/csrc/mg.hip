@@ -1866,8 +1866,7 @@ static int sum_partials(fcfd_ctx *c, long nb, int ns, double *out, int op = SUM_
 // the host: out[q ns + s] for case q
 static int sums_to_host(fcfd_ctx *c, long nb, int ns, double *out) {
     FCFD_TRY(sum_partials(c, nb, ns, c->mg_scalars));
-    FCFD_CHECK(hipMemcpy2DAsync(c->red.host, ns * sizeof(double), c->mg_scalars, SCS * sizeof(double),
-                                ns * sizeof(double), c->nb, hipMemcpyDeviceToHost, c->stream));
+    FCFD_TRY(gather_to_host(c, c->red.host, c->mg_scalars, SCS, ns, c->nb));
     FCFD_CHECK(hipStreamSynchronize(c->stream));
     for (int q = 0; q < c->nb * ns; ++q) out[q] = c->red.host[q];
     return FCFD_OK;
@@ -1969,8 +1968,7 @@ static int fcg_iteration(fcfd_ctx *c, int kind, int inner) {
     }
     FCFD_TRY(sum_partials(c, nb, 1, sc + SC_RR, SUM_STORE));
     // the first SC_NHOST scalars of every case, packed
-    FCFD_CHECK(hipMemcpy2DAsync(c->red.host, SC_NHOST * sizeof(double), sc, SCS * sizeof(double),
-                                SC_NHOST * sizeof(double), c->nb, hipMemcpyDeviceToHost, c->stream));
+    FCFD_TRY(gather_to_host(c, c->red.host, sc, SCS, SC_NHOST, c->nb));
     return FCFD_OK;
 }
 

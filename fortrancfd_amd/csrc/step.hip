@@ -179,8 +179,7 @@ extern "C" int fcfd_step(fcfd_ctx *c) {
     if (rc != FCFD_OK && rc != FCFD_ERR_NOT_CONVERGED) return rc;
     // ls_mv of the main loop: mass and volume of the new phi
     FCFD_TRY(mass_sums(c, c->f[FCFD_PHI], 0));
-    FCFD_CHECK(hipMemcpy2DAsync(c->diag_host, 2 * sizeof(double), c->red.result, RES_STRIDE * sizeof(double),
-                                2 * sizeof(double), c->nb, hipMemcpyDeviceToHost, c->stream));
+    FCFD_TRY(gather_to_host(c, c->diag_host, c->red.result, RES_STRIDE, 2, c->nb));
     c->diag_pending = 1;
     return rc;
 }
