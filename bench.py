@@ -309,12 +309,14 @@ def batch_cases(ncases: int = 64, seed: int = 0x5EEDF00D, precision: int = 1):
 
 
 def batch_bench(ncases: int = 64, steps: int = 500, warmup: int = 2, loop_steps: int = 5,
-                precision: int = 1, serial: bool = False) -> dict:
+                precision: int = 1, serial: bool = False, regions: bool = False) -> dict:
     """Aggregate throughput of the batched 2D workload: one batched context
     per grid size, each stepped by its own host thread (its own HIP stream),
     against the same cases stepped one after another in ordinary contexts
     (a bounded sample of loop_steps steps per case).  serial: the contexts
-    are stepped one after another from the main thread instead."""
+    are stepped one after another from the main thread instead.  regions:
+    afterwards each context steps 5 more times with the library's HIP-event
+    regions on (graphs off) and their ms per step are reported per grid."""
     import threading
     from fortrancfd_amd import Solver
     from fortrancfd_amd.solver import BatchSolver
@@ -392,6 +394,16 @@ def batch_bench(ncases: int = 64, steps: int = 500, warmup: int = 2, loop_steps:
     t_loop = time.perf_counter() - t
     rate_b = cells * steps / t_batch / 1e6
     rate_l = cells * loop_steps / t_loop / 1e6
+    region_ms = {}
+    if regions:
+        for b in solvers:
+            b.prof_enable(True)
+            for _ in range(5):
+                b.step()
+            b.sync()
+            pr = b.prof_read()
+            b.prof_enable(False)
+            region_ms[f"{b.case.nx}x{b.case.ny}"] = {k: [round(v[0] / 5, 4), v[1] // 5] for k, v in pr.items() if v[1]}
     return {"metric": "batched 2D rising bubbles: aggregate Mcell-steps/s", "value": round(rate_b, 2),
             "unit": "Mcell-steps/s", "higher_is_better": True, "n_gpus": 1,
             "dtype": "f64 (FP32 V-cycles in FP64 flexible CG)" if precision else "f64", "data": "synthetic",
@@ -404,6 +416,7 @@ def batch_bench(ncases: int = 64, steps: int = 500, warmup: int = 2, loop_steps:
             "per_case_loop": {"value": round(rate_l, 2), "unit": "Mcell-steps/s", "steps_per_case": loop_steps,
                               "seconds": round(t_loop, 3)},
             "speedup_vs_per_case_loop": round(rate_b / rate_l, 2), "autotuned": tuned,
+            **({"region_ms_per_step_and_launches": region_ms} if regions else {}),
             "max_rel_mass_drift": mass_drift}
 
 
@@ -422,6 +435,7 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=0, help="run the batched 2D workload with N cases")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--batch-serial", action="store_true", help="step the batched contexts from one host thread")
+    ap.add_argument("--batch-regions", action="store_true", help="add the per-grid HIP-event region breakdown")
     ap.add_argument("--regions-out", default="")
     ap.add_argument("--no-tune", action="store_true", help="keep the default smoother schedule")
     ap.add_argument("--krylov", type=int, default=1)
@@ -439,7 +453,7 @@ def main() -> int:
         return 0
     if args.batch:
         print(json.dumps(batch_bench(args.batch, args.batch_steps, precision=1 if args.precision == "mixed" else 0,
-                                     serial=args.batch_serial)))
+                                     serial=args.batch_serial, regions=args.batch_regions)))
         return 0
 
     from fortrancfd_amd import Solver
