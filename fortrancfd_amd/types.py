@@ -248,12 +248,12 @@ class Case:
         return ug
 
     # ---- reference-format input file (tree.f90:48-106) ------------------
-    def to_reference_input(self) -> str:
+    def to_reference_input(self, threads: int = 1) -> str:
         r = repr
         gx, gy = self.gvec[0], self.gvec[1]
         lines = [
             "ooo Method to capture interface ( LS / MPLS / CLSVOF )", "2",
-            "ooo Number of threads", "1",
+            "ooo Number of threads", str(int(threads)),
             "ooo Plot split", str(self.num_of_plot),
             "ooo Name of the problem", self.name,
             "ooo Cells per unit length", str(self.ug()),

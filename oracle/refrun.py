@@ -35,21 +35,21 @@ def available(dim: int) -> bool:
 
 
 def run(case, op: str, inputs: dict, imass: float = 0.0, time: float = 0.0, it: int = 0,
-        timeout: float = 600.0) -> dict:
+        timeout: float = 600.0, threads: int = 1) -> dict:
     names = HARNESS_FIELDS_3D if case.dim == 3 else HARNESS_FIELDS_2D
     with tempfile.TemporaryDirectory(prefix="fcfdref_") as td:
         os.makedirs(os.path.join(td, "in"))
         os.makedirs(os.path.join(td, "outf"))
         os.makedirs(os.path.join(td, "out"))
         with open(os.path.join(td, "case.txt"), "w") as fh:
-            fh.write(case.to_reference_input())
+            fh.write(case.to_reference_input(threads))
         for k, a in inputs.items():
             if k not in names:
                 raise KeyError(k)
             np.ascontiguousarray(a, dtype="<f8").reshape(-1).tofile(os.path.join(td, "in", k))
         with open(os.path.join(td, "in", "scalars.txt"), "w") as fh:
             fh.write(f"{imass!r} {time!r} {it}\n")
-        env = dict(os.environ, OMP_NUM_THREADS="1")
+        env = dict(os.environ, OMP_NUM_THREADS=str(int(threads)))
         r = subprocess.run([harness_path(case.dim), op, "case.txt", "in", "outf"], cwd=td, env=env,
                            capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
