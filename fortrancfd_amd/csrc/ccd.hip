@@ -109,7 +109,9 @@ int ccd_setup(fcfd_ctx *c) {
 // asgn_src_ccd + TWIN_BKS along one line; ghost line ends included.  One
 // thread per line.  The packed coefficient table of the direction is staged
 // in LDS once per workgroup (every line reads the same row at the same step,
-// so the reads are LDS broadcasts).  Each sweep moves through the line in
+// so the reads are LDS broadcasts); a line too long for the LDS table
+// (in_lds = 0) reads the same rows from global memory, where they stay in
+// the caches for the same reason.  Each sweep moves through the line in
 // chunks of CH points: the loads of chunk c+1 are issued before chunk c's
 // dependent recurrence runs, so memory latency overlaps the arithmetic.
 constexpr int CCD_THREADS = 256;
@@ -118,8 +120,8 @@ constexpr long CCD_LDS_MAX = 64 * 1024;
 template <int DIR, int CH>
 __global__ void __launch_bounds__(CCD_THREADS) k_ccd_lines(const double *__restrict__ f, double *__restrict__ fx,
                                                            double *__restrict__ fxx, const double *__restrict__ cog,
-                                                           Geom G) {
-    extern __shared__ double co[];
+                                                           Geom G, int in_lds) {
+    extern __shared__ double lds_co[];
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int a, b, N;
     long base, st, nlines;
@@ -131,8 +133,12 @@ __global__ void __launch_bounds__(CCD_THREADS) k_ccd_lines(const double *__restr
     } else {
         nlines = (long)G.nx * G.ny; N = G.nz + 2 * G.g; st = G.sxy; dx = G.dz;
     }
-    for (int e = threadIdx.x; e < N * NPACK; e += CCD_THREADS) co[e] = cog[e];
-    __syncthreads();
+    const double *co = cog;
+    if (in_lds) {
+        for (int e = threadIdx.x; e < N * NPACK; e += CCD_THREADS) lds_co[e] = cog[e];
+        __syncthreads();
+        co = lds_co;
+    }
     if (t >= nlines) return;
     if (DIR == 0) {
         a = (int)(t % G.ny); b = (int)(t / G.ny);
@@ -401,11 +407,9 @@ int ccd_dir(fcfd_ctx *c, int dir, const double *f, double *fx, double *fxx) {
     const Geom &G = c->G;
     long nl = dir == 0 ? (long)G.ny * G.nz : (dir == 1 ? (long)G.nx * G.nz : (long)G.nx * G.ny);
     const int N = (dir == 0 ? G.nx : (dir == 1 ? G.ny : G.nz)) + 2 * G.g;
-    const size_t lds = sizeof(double) * NPACK * N;
-    if ((long)lds > CCD_LDS_MAX) {
-        fprintf(stderr, "libfcfd: CCD line of %d points exceeds the LDS coefficient table\n", N);
-        return FCFD_ERR_ARG;
-    }
+    const size_t need = sizeof(double) * NPACK * N;
+    const int in_lds = (long)need <= CCD_LDS_MAX;   // else the table is read from global memory
+    const size_t lds = in_lds ? need : 0;
     constexpr int CH = 8;
     dim3 grd((unsigned)((nl + CCD_THREADS - 1) / CCD_THREADS));
     ProfScope ps_(c, R_CCD);
@@ -414,9 +418,9 @@ int ccd_dir(fcfd_ctx *c, int dir, const double *f, double *fx, double *fxx) {
         hipLaunchKernelGGL(k_ccd_lines_x, dim3((unsigned)((nl + CCDX_LINES - 1) / CCDX_LINES)), dim3(CCDX_LINES), 0,
                            c->stream, f, fx, fxx, co, G);
     else if (dir == 1)
-        hipLaunchKernelGGL((k_ccd_lines<1, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, co, G);
+        hipLaunchKernelGGL((k_ccd_lines<1, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, co, G, in_lds);
     else
-        hipLaunchKernelGGL((k_ccd_lines<2, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, co, G);
+        hipLaunchKernelGGL((k_ccd_lines<2, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, co, G, in_lds);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }

@@ -110,3 +110,34 @@ def test_set_params_keeps_the_measured_mpls_mass():
     assert m0 > 0
     s.set_params(ppe_nu_coarse=4)
     assert s.diag()["imass"] == m0
+
+
+def test_batch_with_fused_coarse_levels_matches_single_cases():
+    """At 512 x 1024 per case the hierarchy has a coarse level (64 x 128,
+    8192 cells per case) below 1/32 of the finest that is not part of the
+    single-workgroup tail, so the FP32 cycle smooths it with the fused
+    multi-sweep tiles, one z-slice (case) per tile.  Two cases, one step,
+    bit for bit against their own contexts."""
+    from fortrancfd_amd import Solver
+    from fortrancfd_amd.solver import BatchSolver
+    from fortrancfd_amd.types import bc_copy, bubble_phi
+    cs = []
+    for r12, we, rad in ((0.01, 20.0, 0.25), (0.001, 80.0, 0.3)):
+        c = case2d(512, rho_12=r12, we=we, ppe_precision=1, name="batch")
+        cs.append((c, bc_copy(c, bubble_phi(c, rad, (0.5, 0.5)))))
+    ref = []
+    for c, phi in cs:
+        s = Solver(c)
+        s.push("phi", phi)
+        s.problem_init(False)
+        s.step()
+        ref.append({k: s.pull(k) for k in ("phi", "u", "v", "p")})
+        s.close()
+    b = BatchSolver([c for c, _ in cs])
+    b.push("phi", np.stack([phi for _, phi in cs]))
+    b.problem_init(False)
+    b.step()
+    for k in ("phi", "u", "v", "p"):
+        got = b.pull(k)
+        for q in range(2):
+            assert np.array_equal(got[q], ref[q][k]), (k, q, float(np.abs(got[q] - ref[q][k]).max()))

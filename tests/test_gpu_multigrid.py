@@ -63,3 +63,28 @@ def test_fused_coarse_smoothing_plain_cycles():
         assert rel <= 1e-10, (name, rel)
         cyc[name] = c
     assert cyc["mixed"] <= cyc["fp64"] + 1
+
+
+def test_every_tuner_schedule_reaches_the_same_pressure():
+    """Each smoother schedule the autotuner may choose (sweep counts, coarse
+    FP32 tiling 0..3, FP16 density in the FP32 level-0 sweeps, two V-cycles
+    per outer iteration) is a preconditioner of the same FP64 flexible CG:
+    every one must reach the tolerance and the same pressure to it."""
+    from fortrancfd_amd.autotune import SCHEDULES
+    from fortrancfd_amd.types import PPE_MIXED
+    case, s = _system()
+    zeros = np.zeros(case.size)
+    ref = None
+    g = case.ghc
+    for sched in SCHEDULES:
+        s.set_params(**sched)
+        s.push("p", zeros)
+        cyc, rel = s.ppe_solve_rhs(1e-10, PPE_MIXED)
+        p = s.pull("p")[g:-g, g:-g, g:-g]
+        print(sched, cyc, rel)
+        assert rel <= 1e-10 and cyc <= 20, (sched, cyc, rel)
+        if ref is None:
+            ref = p
+        else:
+            d = np.abs(p - ref).max() / np.abs(ref).max()
+            assert d <= 1e-8, (sched, d)

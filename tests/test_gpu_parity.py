@@ -200,3 +200,15 @@ def test_time_steps(tag, mk):
     _cmp(s, o, names, 1e-8)
     d, e = s.diag(), o.diag()
     assert abs(d["mass"] - e["mass"]) <= 1e-11 * e["mass"]
+
+
+def test_ccd_lines_longer_than_the_lds_table():
+    """y lines of 1030 points (1024 cells + ghosts) exceed the LDS copy of the
+    line-matrix table, so the line kernel reads the table from global memory;
+    the derivatives and curvature must match the oracle as on short lines."""
+    from fortrancfd_amd.types import Case
+    case = Case(dim=2, nx=16, ny=1024, nz=1, xend=1.0, yend=64.0, gvec=(0.0, -1.0, 0.0), name="t2d")  # dx = dy
+    s, o = _pair(case, {"phi": bubble(case, 0.3, 1)})
+    s.curv(); o.call("curv")
+    _cmp(s, o, ["nx", "ny", "nxx", "nyy", "nxy"], 1e-12)
+    _cmp(s, o, ["curv"], 1e-11)
