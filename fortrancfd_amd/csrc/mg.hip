@@ -738,6 +738,8 @@ __global__ void k_add_f32(double *__restrict__ p, const float *__restrict__ e, L
 
 // Coarsest level: Jacobi-preconditioned CG in FP64 inside one workgroup.
 // The rhs is projected to zero mean; the solution is returned with zero mean.
+// A batched 2D stack runs one workgroup per case on its slice (L describes
+// one slice; workspace 6 n per case).
 template <typename T, int DIM>
 __global__ void __launch_bounds__(1024) k_coarse_cg(T *__restrict__ eout, const T *__restrict__ fin,
                                                     const T *__restrict__ bxT, const T *__restrict__ byT,
@@ -748,6 +750,8 @@ __global__ void __launch_bounds__(1024) k_coarse_cg(T *__restrict__ eout, const 
     __shared__ double red[1024];
     const int nt = blockDim.x, t = threadIdx.x;
     const long n = (long)L.nx * L.ny * L.nz;
+    const int kq = DIM == 2 ? (int)blockIdx.x : 0;
+    ws += (long)kq * 6 * n;
     double *x = ws, *r = ws + n, *z = ws + 2 * n, *q = ws + 3 * n, *Aq = ws + 4 * n, *dg = ws + 5 * n;
     auto block_sum = [&](double v) -> double {
         red[t] = v;
@@ -764,7 +768,7 @@ __global__ void __launch_bounds__(1024) k_coarse_cg(T *__restrict__ eout, const 
         i = (int)(c % L.nx);
         long rr = c / L.nx;
         j = (int)(rr % L.ny);
-        k = (int)(rr / L.ny);
+        k = (int)(rr / L.ny) + kq;
     };
     // Apply -A (positive semi-definite) to v at cell c
     auto negA = [&](const double *v, long c) -> double {
@@ -1406,11 +1410,11 @@ static int mg_build(fcfd_ctx *c) {
     // per-case scalars + coarse CG workspace (6 vectors) or the dense coarse
     // inverse of every case
     long ncoarse = case_cells(c->mg.back(), G.dim);
-    if (c->nb > 1 && (c->mg.size() < 2 || ncoarse > CDENSE)) {
-        fprintf(stderr, "libfcfd: a batched stack needs a coarsest level of <= %d cells per case\n", CDENSE);
+    if (c->mg.size() < 2) {
+        fprintf(stderr, "libfcfd: the PPE multigrid needs at least two levels\n");
         return FCFD_ERR_ARG;
     }
-    FCFD_TRY(alloc0(&c->mg_scalars, SCS * c->nb + std::max(6 * ncoarse, c->nb * ncoarse * ncoarse)));
+    FCFD_TRY(alloc0(&c->mg_scalars, SCS * c->nb + c->nb * std::max(6 * ncoarse, ncoarse * ncoarse)));
     return FCFD_OK;
 }
 
@@ -1686,10 +1690,10 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         if (nl > 1 && n <= CDENSE)
             hipLaunchKernelGGL((k_coarse_dense<T>), dim3(DIM == 2 ? c->nb : 1), dim3(CDENSE), 0, c->stream, e, f,
                                (const double *)coarse_inv(c), lay_case(M, G.dim), DIM);
-        else   // one case (mg_build refuses a batched stack without a dense coarsest level)
-            hipLaunchKernelGGL((k_coarse_cg<T, DIM>), dim3(1), dim3(n >= 1024 ? 1024 : 256), 0, c->stream, e, f, bx,
-                               by, bz, M.bx64, M.by64, M.bz64, coarse_inv(c), L, ih, (int)std::min<long>(n, 96),
-                               0.0);
+        else   // one workgroup per case
+            hipLaunchKernelGGL((k_coarse_cg<T, DIM>), dim3(DIM == 2 ? c->nb : 1), dim3(n >= 1024 ? 1024 : 256), 0,
+                               c->stream, e, f, bx, by, bz, M.bx64, M.by64, M.bz64, coarse_inv(c),
+                               lay_case(M, G.dim), ih, (int)std::min<long>(n, 96), 0.0);
         FCFD_CHECK(hipGetLastError());
     }
     // up

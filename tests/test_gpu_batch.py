@@ -141,3 +141,34 @@ def test_batch_with_fused_coarse_levels_matches_single_cases():
         got = b.pull(k)
         for q in range(2):
             assert np.array_equal(got[q], ref[q][k]), (k, q, float(np.abs(got[q] - ref[q][k]).max()))
+
+
+def test_batch_with_iterative_coarsest_level_matches_single_cases():
+    """52 x 104 coarsens to 13 x 13 = 169 cells, above the dense-inverse
+    limit, so the coarsest level is solved by the one-workgroup FP64 CG, one
+    workgroup per case in the batch; bit for bit against single runs."""
+    from fortrancfd_amd import Solver
+    from fortrancfd_amd.solver import BatchSolver
+    from fortrancfd_amd.types import bc_copy, bubble_phi
+    cs = []
+    for r12, we, rad in ((0.1, 5.0, 0.2), (0.001, 50.0, 0.3), (0.02, 20.0, 0.25)):
+        c = case2d(52, rho_12=r12, we=we, ppe_precision=1, redis_interval=2, name="batch")
+        cs.append((c, bc_copy(c, bubble_phi(c, rad, (0.5, 0.5)))))
+    ref = []
+    for c, phi in cs:
+        s = Solver(c)
+        s.push("phi", phi)
+        s.problem_init(False)
+        for _ in range(3):
+            s.step()
+        ref.append({k: s.pull(k) for k in ("phi", "u", "v", "p")})
+        s.close()
+    b = BatchSolver([c for c, _ in cs])
+    b.push("phi", np.stack([phi for _, phi in cs]))
+    b.problem_init(False)
+    for _ in range(3):
+        b.step()
+    for k in ("phi", "u", "v", "p"):
+        got = b.pull(k)
+        for q in range(3):
+            assert np.array_equal(got[q], ref[q][k]), (k, q, float(np.abs(got[q] - ref[q][k]).max()))
