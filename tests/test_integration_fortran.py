@@ -32,6 +32,25 @@ def test_field_parameters_match_the_header_enum():
             assert types.FIELD_ID[name.lower()] == int(val), name
 
 
+def _fortran_members(typename: str) -> list[str]:
+    body = re.search(rf"type, bind\(c\) :: {typename}\n(.*?)end type", fortran_module(), flags=re.S).group(1)
+    names = []
+    for line in body.splitlines():
+        if "::" in line:
+            names += [re.sub(r"\(.*\)", "", n).strip() for n in line.split("::")[1].split(",")]
+    return names
+
+
+@pytest.mark.parametrize("fname,pyname", [("fcfd_grid", "FcfdGrid"), ("fcfd_params", "FcfdParams"),
+                                          ("fcfd_diag", "FcfdDiag")])
+def test_fortran_members_match_the_c_structs(fname, pyname):
+    """Same members in the same order as the ctypes mirrors (whose layout
+    tests/test_abi.py checks against the header): a size match alone can
+    hide a renamed or missing member in the trailing padding."""
+    from fortrancfd_amd import types
+    assert _fortran_members(fname) == [n for n, _ in getattr(types, pyname)._fields_]
+
+
 @pytest.mark.skipif(not os.path.exists(FLANG), reason="AMD flang not installed")
 @pytest.mark.skipif(not os.path.exists(os.path.join(LIBDIR, "libfcfd.so")), reason="libfcfd.so not built")
 def test_fortran_binding_compiles_links_and_matches_c():
