@@ -39,74 +39,230 @@ __global__ void k_adj_seed(double *__restrict__ w, const double *__restrict__ l1
     w[id] = v;
 }
 
-// out = base + J(psi)^T w over the interior, and the velocity adjoint of
-// the stage added into ub/vb/wb (when given).  Gather form: cell j visits
-// every face f of each direction whose clamped WENO stencil can contain j
-// (f = j-3 .. j+2, the faces -1/2 .. n-1/2 existing), forms
-// Fbar_f = (w_{f+1} - w_f)/h (the two cells the face enters with -1/h and
-// +1/h), runs both reconstructions in reverse and keeps the parts of the
-// stencil slots that are j.  Deterministic: no atomics.
+// out = base + J(psi)^T w over the interior, and the velocity adjoint of the
+// stage added into ub/vb/wb (when given), one direction per pass: the x
+// pass writes out = base + (x part), the y and z passes add theirs.  Each
+// face's reverse reconstructions are computed once (the value of face f,
+// between cells f and f+1, is WENO_left(s1 at f-2..f+2) +
+// WENO_right(s2 at f-1..f+3); its seed is Fbar_f = (w_{f+1} - w_f)/h, w = 0
+// outside the line) and cell c collects the slots of faces c-3 .. c+2 whose
+// clamped stencil position is c, so boundary cells receive every slot that
+// the zero-gradient clamp folds onto them.  Deterministic: no atomics.
+constexpr double ADJ_EPS = 1.0e-13;
+
+// the velocity and level-set adjoints of one cell from its s1, s2 adjoints
+__device__ __forceinline__ void adj_cell(double s1b, double s2b, double u, double ph, double vs, double &pb,
+                                         double &velb) {
+    pb = s1b * 0.5 * (u + fabs(u)) + s2b * 0.5 * (u - fabs(u));
+    const double sg = u > 0.0 ? 1.0 : (u < 0.0 ? -1.0 : 0.0);
+    velb = vs * ((s1b * 0.5 * (1.0 + sg) + s2b * 0.5 * (1.0 - sg)) * ph);
+}
+
+// reverse reconstructions of one face from its line samples: s1, s2 at the
+// clamped positions f-2 .. f+3
+__device__ __forceinline__ void face_adj(const double *s1, const double *s2, double fbar, double *gb, double *fb) {
+    weno_left_adj(s1, ADJ_EPS, fbar, gb);       // s1 at f-2 .. f+2
+    weno_right_adj(s2 + 1, ADJ_EPS, fbar, fb);  // s2 at f-1 .. f+3
+}
+
+constexpr int ADJ_TX = 64, ADJ_TY = 4;
+constexpr int ADJ_NFX = ADJ_TX + 5;   // x faces i0-3 .. i0+65 of a 64-cell row
+constexpr int ADJ_NFY = ADJ_TY + 5;   // y face rows j0-3 .. j0+5 of a 4-row block
+constexpr int ADJ_KC = 16;            // z cells per thread
+
+// x pass: block 64 x 4 cells of one plane; the 69 faces of each row are
+// computed by the row's wave (two passes) into LDS, then each cell gathers
 template <int DIM>
-__global__ void __launch_bounds__(256) k_ls_adv_adj(const double *__restrict__ psi, const double *__restrict__ nu,
-                                                    const double *__restrict__ nv, const double *__restrict__ nw,
-                                                    const double *__restrict__ w, const double *__restrict__ base,
-                                                    double *__restrict__ out, double *__restrict__ ub,
-                                                    double *__restrict__ vb, double *__restrict__ wb, Geom G,
-                                                    double vs) {
+__global__ void __launch_bounds__(256) k_adj_x(const double *__restrict__ psi, const double *__restrict__ nu,
+                                               const double *__restrict__ w, const double *__restrict__ base,
+                                               double *__restrict__ out, double *__restrict__ ub, Geom G,
+                                               double vs) {
+    __shared__ double sg[ADJ_TY][ADJ_NFX][5], sf[ADJ_TY][ADJ_NFX][5];
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
-    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
-    if (i >= G.nx || j >= G.ny) return;
-    const long id = G.id(i, j, k);
-    const double *vel[3] = {nu, nv, nw};
-    double *velb[3] = {ub, vb, wb};
-    const int n[3] = {G.nx, G.ny, G.nz};
-    const int pos[3] = {i, j, k};
-    const long st[3] = {1, G.sx, G.sxy};
-    const double h[3] = {G.dx, G.dy, G.dz};
-    const double EPS = 1.0e-13;
-    double pb = 0.0;
+    const int lane = threadIdx.x, r = threadIdx.y;
+    const int i0 = bx_ * ADJ_TX, j = by_ * ADJ_TY + r, k = bz_;
+    const int n = G.nx;
+    const bool rowok = j < G.ny;
+    const long row = rowok ? G.id(0, j, k) : 0;
+    for (int t = lane; t < ADJ_NFX; t += ADJ_TX) {
+        const int f = i0 - 3 + t;
+        double gb[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, fb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (rowok && f >= -1 && f <= n - 1) {
+            const double wl = f >= 0 ? w[row + f] : 0.0, wr = f + 1 <= n - 1 ? w[row + f + 1] : 0.0;
+            double s1[6], s2[6];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        const int p0 = pos[d], nd = n[d];
-        // s1, s2 at the clamped positions p0-5 .. p0+5, the seed at p0-3 .. p0+3
-        double s1[11], s2[11], wv[7];
-#pragma unroll
-        for (int o = -5; o <= 5; ++o) {
-            const int q = clampi(p0 + o, 0, nd - 1);
-            const long a = id + (long)(q - p0) * st[d];
-            const double u = vs * vel[d][a], ph = psi[a];
-            s1[o + 5] = 0.5 * (u + fabs(u)) * ph;
-            s2[o + 5] = 0.5 * (u - fabs(u)) * ph;
-        }
-#pragma unroll
-        for (int o = -3; o <= 3; ++o) {
-            const int q = p0 + o;
-            wv[o + 3] = (q >= 0 && q < nd) ? w[id + (long)o * st[d]] : 0.0;
-        }
-        double s1b = 0.0, s2b = 0.0;
-#pragma unroll
-        for (int fo = -3; fo <= 2; ++fo) {   // face f = p0 + fo between cells f and f+1
-            const int f = p0 + fo;
-            if (f < -1 || f > nd - 1) continue;
-            const double fbar = (wv[fo + 4] - wv[fo + 3]) / h[d];
-            double gb[5], fb[5];
-            weno_left_adj(&s1[fo + 3], EPS, fbar, gb);    // s1 at f-2 .. f+2
-            weno_right_adj(&s2[fo + 4], EPS, fbar, fb);   // s2 at f-1 .. f+3
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                if (clampi(f - 2 + m, 0, nd - 1) == p0) s1b += gb[m];
-                if (clampi(f - 1 + m, 0, nd - 1) == p0) s2b += fb[m];
+            for (int m = 0; m < 6; ++m) {
+                const long a = row + clampi(f - 2 + m, 0, n - 1);
+                const double u = vs * nu[a], ph = psi[a];
+                s1[m] = 0.5 * (u + fabs(u)) * ph;
+                s2[m] = 0.5 * (u - fabs(u)) * ph;
             }
+            face_adj(s1, s2, (wr - wl) / G.dx, gb, fb);
         }
-        const double u = vs * vel[d][id], ph = psi[id];
-        pb += s1b * 0.5 * (u + fabs(u)) + s2b * 0.5 * (u - fabs(u));
-        if (velb[d]) {   // d/d(stored velocity) = vs d/du
-            const double sg = u > 0.0 ? 1.0 : (u < 0.0 ? -1.0 : 0.0);
-            velb[d][id] += vs * ((s1b * 0.5 * (1.0 + sg) + s2b * 0.5 * (1.0 - sg)) * ph);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            sg[r][t][m] = gb[m];
+            sf[r][t][m] = fb[m];
         }
     }
+    __syncthreads();
+    const int i = i0 + lane;
+    if (!rowok || i >= n) return;
+    double s1b = 0.0, s2b = 0.0;
+#pragma unroll
+    for (int fo = -3; fo <= 2; ++fo) {
+        const int f = i + fo, t = lane + fo + 3;
+        if (f < -1 || f > n - 1) continue;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            if (clampi(f - 2 + m, 0, n - 1) == i) s1b += sg[r][t][m];
+            if (clampi(f - 1 + m, 0, n - 1) == i) s2b += sf[r][t][m];
+        }
+    }
+    const long id = row + i;
+    double pb, vb;
+    adj_cell(s1b, s2b, vs * nu[id], psi[id], vs, pb, vb);
     out[id] = base[id] + pb;
+    if (ub) ub[id] += vb;
+}
+
+// y pass: block 64 x 4 cells of one plane; the 9 x 64 faces the block's
+// cells need are computed by its 256 threads (three passes) into LDS
+template <int DIM>
+__global__ void __launch_bounds__(256) k_adj_y(const double *__restrict__ psi, const double *__restrict__ nv,
+                                               const double *__restrict__ w, double *__restrict__ out,
+                                               double *__restrict__ vbar, Geom G, double vs) {
+    __shared__ double sg[ADJ_NFY][ADJ_TX][5], sf[ADJ_NFY][ADJ_TX][5];
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int lane = threadIdx.x, r = threadIdx.y;
+    const int j0 = by_ * ADJ_TY, k = bz_;
+    const int n = G.ny;
+    const long sx = G.sx;
+    for (int e = r * ADJ_TX + lane; e < ADJ_NFY * ADJ_TX; e += ADJ_TX * ADJ_TY) {
+        const int t = e / ADJ_TX, col = e % ADJ_TX, ii = bx_ * ADJ_TX + col;
+        const int f = j0 - 3 + t;
+        double gb[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, fb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (ii < G.nx && f >= -1 && f <= n - 1) {
+            const long c0 = G.id(ii, 0, k);
+            const double wl = f >= 0 ? w[c0 + f * sx] : 0.0, wr = f + 1 <= n - 1 ? w[c0 + (f + 1) * sx] : 0.0;
+            double s1[6], s2[6];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                const long a = c0 + clampi(f - 2 + m, 0, n - 1) * sx;
+                const double u = vs * nv[a], ph = psi[a];
+                s1[m] = 0.5 * (u + fabs(u)) * ph;
+                s2[m] = 0.5 * (u - fabs(u)) * ph;
+            }
+            face_adj(s1, s2, (wr - wl) / G.dy, gb, fb);
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            sg[t][col][m] = gb[m];
+            sf[t][col][m] = fb[m];
+        }
+    }
+    __syncthreads();
+    const int i = bx_ * ADJ_TX + lane, j = j0 + r;
+    if (i >= G.nx || j >= n) return;
+    double s1b = 0.0, s2b = 0.0;
+#pragma unroll
+    for (int fo = -3; fo <= 2; ++fo) {
+        const int f = j + fo, t = r + fo + 3;
+        if (f < -1 || f > n - 1) continue;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            if (clampi(f - 2 + m, 0, n - 1) == j) s1b += sg[t][lane][m];
+            if (clampi(f - 1 + m, 0, n - 1) == j) s2b += sf[t][lane][m];
+        }
+    }
+    const long id = G.id(i, j, k);
+    double pb, vb;
+    adj_cell(s1b, s2b, vs * nv[id], psi[id], vs, pb, vb);
+    out[id] = out[id] + pb;
+    if (vbar) vbar[id] += vb;
+}
+
+// z pass (3D): each thread walks the faces of its (i, j) column that reach
+// its ADJ_KC cells, keeping the samples of positions f-2 .. f+3 and the
+// s1 / s2 adjoints of cells f-2 .. f+3 in registers; after face f no later
+// face reaches cell f-2, which is then complete
+__global__ void __launch_bounds__(256) k_adj_z(const double *__restrict__ psi, const double *__restrict__ nw,
+                                               const double *__restrict__ w, double *__restrict__ out,
+                                               double *__restrict__ wbar, Geom G, double vs) {
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y;
+    const int k0 = bz_ * ADJ_KC, k1 = min(k0 + ADJ_KC, G.nz);
+    if (i >= G.nx || j >= G.ny) return;
+    const int n = G.nz;
+    const long c0 = G.id(i, j, 0), st = G.sxy;
+    const int fa = max(-1, k0 - 3), fz = min(n - 1, k1 + 1);
+    double s1[6], s2[6], acc1[6], acc2[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        const long a = c0 + clampi(fa - 2 + m, 0, n - 1) * st;
+        const double u = vs * nw[a], ph = psi[a];
+        s1[m] = 0.5 * (u + fabs(u)) * ph;
+        s2[m] = 0.5 * (u - fabs(u)) * ph;
+        acc1[m] = 0.0;
+        acc2[m] = 0.0;
+    }
+    auto finish = [&](int c, double s1b, double s2b) {
+        const long id = c0 + c * st;
+        double pb, vb;
+        adj_cell(s1b, s2b, vs * nw[id], psi[id], vs, pb, vb);
+        out[id] = out[id] + pb;
+        if (wbar) wbar[id] += vb;
+    };
+    double wl = fa >= 0 ? w[c0 + fa * st] : 0.0;
+    for (int f = fa; f <= fz; ++f) {
+        const double wr = f + 1 <= n - 1 ? w[c0 + (f + 1) * st] : 0.0;
+        double gb[5], fb[5];
+        face_adj(s1, s2, (wr - wl) / G.dz, gb, fb);
+        const int b = f - 2;   // cell of window slot 0
+        if (f - 2 >= 0 && f + 3 <= n - 1) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                acc1[m] += gb[m];
+                acc2[m + 1] += fb[m];
+            }
+        } else {   // near a wall: slots the clamp folds onto the boundary cell
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const int c1 = clampi(f - 2 + m, 0, n - 1) - b, c2 = clampi(f - 1 + m, 0, n - 1) - b;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    acc1[q] += c1 == q ? gb[m] : 0.0;
+                    acc2[q] += c2 == q ? fb[m] : 0.0;
+                }
+            }
+        }
+        if (b >= k0 && b < k1) finish(b, acc1[0], acc2[0]);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            acc1[m] = acc1[m + 1];
+            acc2[m] = acc2[m + 1];
+            s1[m] = s1[m + 1];
+            s2[m] = s2[m + 1];
+        }
+        acc1[5] = 0.0;
+        acc2[5] = 0.0;
+        {
+            const long a = c0 + clampi(f + 4, 0, n - 1) * st;   // position (f+1) + 3
+            const double u = vs * nw[a], ph = psi[a];
+            s1[5] = 0.5 * (u + fabs(u)) * ph;
+            s2[5] = 0.5 * (u - fabs(u)) * ph;
+        }
+        wl = wr;
+    }
+    // at the top wall the last cells complete after the last face
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const int c = fz - 1 + m;
+        if (c >= k0 && c < k1 && c <= n - 1 && fz - 2 < c) finish(c, acc1[m], acc2[m]);
+    }
 }
 
 static int adj_scratch(fcfd_ctx *c) {
@@ -133,13 +289,16 @@ int ls_adv_adjoint_step(fcfd_ctx *c, const double *phi, double *lambda, double *
     // stage states phi1, phi2 of the forward step (interior; L reads no ghost)
     FCFD_TRY(ls_adv_stage(c, phi, phi1, 0));
     FCFD_TRY(ls_adv_stage(c, phi1, phi2, 1));
+    const dim3 gz(grd.x, grd.y, (G.nz + ADJ_KC - 1) / ADJ_KC);
     auto adj = [&](const double *psi, const double *base, double *out) {
-        if (G.dim == 3)
-            hipLaunchKernelGGL(k_ls_adv_adj<3>, grd, blk, 0, c->stream, psi, F[FCFD_NU], F[FCFD_NV], F[FCFD_NW], w,
-                               base, out, ub, vb, wb, G, c->vscale);
-        else
-            hipLaunchKernelGGL(k_ls_adv_adj<2>, grd, blk, 0, c->stream, psi, F[FCFD_NU], F[FCFD_NV], nullptr, w,
-                               base, out, ub, vb, nullptr, G, c->vscale);
+        if (G.dim == 3) {
+            hipLaunchKernelGGL(k_adj_x<3>, grd, blk, 0, c->stream, psi, F[FCFD_NU], w, base, out, ub, G, c->vscale);
+            hipLaunchKernelGGL(k_adj_y<3>, grd, blk, 0, c->stream, psi, F[FCFD_NV], w, out, vb, G, c->vscale);
+            hipLaunchKernelGGL(k_adj_z, gz, blk, 0, c->stream, psi, F[FCFD_NW], w, out, wb, G, c->vscale);
+        } else {
+            hipLaunchKernelGGL(k_adj_x<2>, grd, blk, 0, c->stream, psi, F[FCFD_NU], w, base, out, ub, G, c->vscale);
+            hipLaunchKernelGGL(k_adj_y<2>, grd, blk, 0, c->stream, psi, F[FCFD_NV], w, out, vb, G, c->vscale);
+        }
     };
     // lambda2 = lambda3 + J(phi2)^T (8 dt/12 lambda3)
     hipLaunchKernelGGL(k_adj_seed, grd, blk, 0, c->stream, w, lambda, nullptr, nullptr, 8.0 * dt / 12.0, 0.0, 0.0,
