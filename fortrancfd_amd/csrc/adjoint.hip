@@ -67,7 +67,6 @@ __device__ __forceinline__ void face_adj(const double *s1, const double *s2, dou
 
 constexpr int ADJ_TX = 64, ADJ_TY = 4;
 constexpr int ADJ_NFX = ADJ_TX + 5;   // x faces i0-3 .. i0+65 of a 64-cell row
-constexpr int ADJ_NFY = ADJ_TY + 5;   // y face rows j0-3 .. j0+5 of a 4-row block
 constexpr int ADJ_KC = 16;            // z cells per thread
 
 // x pass: block 64 x 4 cells of one plane; the 69 faces of each row are
@@ -83,12 +82,13 @@ __global__ void __launch_bounds__(256) k_adj_x(const double *__restrict__ psi, c
     const int lane = threadIdx.x, r = threadIdx.y;
     const int i0 = bx_ * ADJ_TX, j = by_ * ADJ_TY + r, k = bz_;
     const int n = G.nx;
-    const bool rowok = j < G.ny;
-    const long row = rowok ? G.id(0, j, k) : 0;
-    for (int t = lane; t < ADJ_NFX; t += ADJ_TX) {
+    // the 4 x 69 faces over the 256 threads (one wave takes a second turn)
+    for (int e = r * ADJ_TX + lane; e < ADJ_TY * ADJ_NFX; e += ADJ_TX * ADJ_TY) {
+        const int rr = e / ADJ_NFX, t = e - rr * ADJ_NFX, jj = by_ * ADJ_TY + rr;
         const int f = i0 - 3 + t;
         double gb[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, fb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (rowok && f >= -1 && f <= n - 1) {
+        if (jj < G.ny && f >= -1 && f <= n - 1) {
+            const long row = G.id(0, jj, k);
             const double wl = f >= 0 ? w[row + f] : 0.0, wr = f + 1 <= n - 1 ? w[row + f + 1] : 0.0;
             double s1[6], s2[6];
 #pragma unroll
@@ -102,11 +102,13 @@ __global__ void __launch_bounds__(256) k_adj_x(const double *__restrict__ psi, c
         }
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
-            sg[r][t][m] = gb[m];
-            sf[r][t][m] = fb[m];
+            sg[rr][t][m] = gb[m];
+            sf[rr][t][m] = fb[m];
         }
     }
     __syncthreads();
+    const bool rowok = j < G.ny;
+    const long row = rowok ? G.id(0, j, k) : 0;
     const int i = i0 + lane;
     if (!rowok || i >= n) return;
     double s1b = 0.0, s2b = 0.0;
@@ -127,83 +129,29 @@ __global__ void __launch_bounds__(256) k_adj_x(const double *__restrict__ psi, c
     if (ub) ub[id] += vb;
 }
 
-// y pass: block 64 x 4 cells of one plane; the 9 x 64 faces the block's
-// cells need are computed by its 256 threads (three passes) into LDS
-template <int DIM>
-__global__ void __launch_bounds__(256) k_adj_y(const double *__restrict__ psi, const double *__restrict__ nv,
-                                               const double *__restrict__ w, double *__restrict__ out,
-                                               double *__restrict__ vbar, Geom G, double vs) {
-    __shared__ double sg[ADJ_NFY][ADJ_TX][5], sf[ADJ_NFY][ADJ_TX][5];
+// y and z passes: each thread walks the faces of one line along axis AX
+// (1: y, 2: z) that reach its ADJ_KC cells, keeping the samples of positions
+// f-2 .. f+3 and the s1 / s2 adjoints of cells f-2 .. f+3 in registers;
+// after face f no later face reaches cell f-2, which is then complete.
+// Threads run along x (coalesced); blockIdx.y picks the other index.
+template <int AX>
+__global__ void __launch_bounds__(256) k_adj_walk(const double *__restrict__ psi, const double *__restrict__ vel,
+                                                  const double *__restrict__ w, double *__restrict__ out,
+                                                  double *__restrict__ velbar, Geom G, double vs) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
-    const int lane = threadIdx.x, r = threadIdx.y;
-    const int j0 = by_ * ADJ_TY, k = bz_;
-    const int n = G.ny;
-    const long sx = G.sx;
-    for (int e = r * ADJ_TX + lane; e < ADJ_NFY * ADJ_TX; e += ADJ_TX * ADJ_TY) {
-        const int t = e / ADJ_TX, col = e % ADJ_TX, ii = bx_ * ADJ_TX + col;
-        const int f = j0 - 3 + t;
-        double gb[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, fb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (ii < G.nx && f >= -1 && f <= n - 1) {
-            const long c0 = G.id(ii, 0, k);
-            const double wl = f >= 0 ? w[c0 + f * sx] : 0.0, wr = f + 1 <= n - 1 ? w[c0 + (f + 1) * sx] : 0.0;
-            double s1[6], s2[6];
-#pragma unroll
-            for (int m = 0; m < 6; ++m) {
-                const long a = c0 + clampi(f - 2 + m, 0, n - 1) * sx;
-                const double u = vs * nv[a], ph = psi[a];
-                s1[m] = 0.5 * (u + fabs(u)) * ph;
-                s2[m] = 0.5 * (u - fabs(u)) * ph;
-            }
-            face_adj(s1, s2, (wr - wl) / G.dy, gb, fb);
-        }
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            sg[t][col][m] = gb[m];
-            sf[t][col][m] = fb[m];
-        }
-    }
-    __syncthreads();
-    const int i = bx_ * ADJ_TX + lane, j = j0 + r;
-    if (i >= G.nx || j >= n) return;
-    double s1b = 0.0, s2b = 0.0;
-#pragma unroll
-    for (int fo = -3; fo <= 2; ++fo) {
-        const int f = j + fo, t = r + fo + 3;
-        if (f < -1 || f > n - 1) continue;
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            if (clampi(f - 2 + m, 0, n - 1) == j) s1b += sg[t][lane][m];
-            if (clampi(f - 1 + m, 0, n - 1) == j) s2b += sf[t][lane][m];
-        }
-    }
-    const long id = G.id(i, j, k);
-    double pb, vb;
-    adj_cell(s1b, s2b, vs * nv[id], psi[id], vs, pb, vb);
-    out[id] = out[id] + pb;
-    if (vbar) vbar[id] += vb;
-}
-
-// z pass (3D): each thread walks the faces of its (i, j) column that reach
-// its ADJ_KC cells, keeping the samples of positions f-2 .. f+3 and the
-// s1 / s2 adjoints of cells f-2 .. f+3 in registers; after face f no later
-// face reaches cell f-2, which is then complete
-__global__ void __launch_bounds__(256) k_adj_z(const double *__restrict__ psi, const double *__restrict__ nw,
-                                               const double *__restrict__ w, double *__restrict__ out,
-                                               double *__restrict__ wbar, Geom G, double vs) {
-    int bx_, by_, bz_;
-    xcd_block(bx_, by_, bz_);
-    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y;
-    const int k0 = bz_ * ADJ_KC, k1 = min(k0 + ADJ_KC, G.nz);
-    if (i >= G.nx || j >= G.ny) return;
-    const int n = G.nz;
-    const long c0 = G.id(i, j, 0), st = G.sxy;
+    const int i = bx_ * blockDim.x + threadIdx.x, o = by_ * blockDim.y + threadIdx.y;
+    const int n = AX == 1 ? G.ny : G.nz;
+    const int k0 = bz_ * ADJ_KC, k1 = min(k0 + ADJ_KC, n);
+    if (i >= G.nx || o >= (AX == 1 ? G.nz : G.ny)) return;
+    const long c0 = AX == 1 ? G.id(i, 0, o) : G.id(i, o, 0), st = AX == 1 ? G.sx : G.sxy;
+    const double h = AX == 1 ? G.dy : G.dz;
     const int fa = max(-1, k0 - 3), fz = min(n - 1, k1 + 1);
     double s1[6], s2[6], acc1[6], acc2[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
         const long a = c0 + clampi(fa - 2 + m, 0, n - 1) * st;
-        const double u = vs * nw[a], ph = psi[a];
+        const double u = vs * vel[a], ph = psi[a];
         s1[m] = 0.5 * (u + fabs(u)) * ph;
         s2[m] = 0.5 * (u - fabs(u)) * ph;
         acc1[m] = 0.0;
@@ -212,15 +160,15 @@ __global__ void __launch_bounds__(256) k_adj_z(const double *__restrict__ psi, c
     auto finish = [&](int c, double s1b, double s2b) {
         const long id = c0 + c * st;
         double pb, vb;
-        adj_cell(s1b, s2b, vs * nw[id], psi[id], vs, pb, vb);
+        adj_cell(s1b, s2b, vs * vel[id], psi[id], vs, pb, vb);
         out[id] = out[id] + pb;
-        if (wbar) wbar[id] += vb;
+        if (velbar) velbar[id] += vb;
     };
     double wl = fa >= 0 ? w[c0 + fa * st] : 0.0;
     for (int f = fa; f <= fz; ++f) {
         const double wr = f + 1 <= n - 1 ? w[c0 + (f + 1) * st] : 0.0;
         double gb[5], fb[5];
-        face_adj(s1, s2, (wr - wl) / G.dz, gb, fb);
+        face_adj(s1, s2, (wr - wl) / h, gb, fb);
         const int b = f - 2;   // cell of window slot 0
         if (f - 2 >= 0 && f + 3 <= n - 1) {
 #pragma unroll
@@ -251,17 +199,17 @@ __global__ void __launch_bounds__(256) k_adj_z(const double *__restrict__ psi, c
         acc2[5] = 0.0;
         {
             const long a = c0 + clampi(f + 4, 0, n - 1) * st;   // position (f+1) + 3
-            const double u = vs * nw[a], ph = psi[a];
+            const double u = vs * vel[a], ph = psi[a];
             s1[5] = 0.5 * (u + fabs(u)) * ph;
             s2[5] = 0.5 * (u - fabs(u)) * ph;
         }
         wl = wr;
     }
-    // at the top wall the last cells complete after the last face
+    // at the far wall the last cells complete after the last face
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         const int c = fz - 1 + m;
-        if (c >= k0 && c < k1 && c <= n - 1 && fz - 2 < c) finish(c, acc1[m], acc2[m]);
+        if (c >= k0 && c < k1 && c <= n - 1) finish(c, acc1[m], acc2[m]);
     }
 }
 
@@ -289,16 +237,18 @@ int ls_adv_adjoint_step(fcfd_ctx *c, const double *phi, double *lambda, double *
     // stage states phi1, phi2 of the forward step (interior; L reads no ghost)
     FCFD_TRY(ls_adv_stage(c, phi, phi1, 0));
     FCFD_TRY(ls_adv_stage(c, phi1, phi2, 1));
-    const dim3 gz(grd.x, grd.y, (G.nz + ADJ_KC - 1) / ADJ_KC);
+    // walkers: 64 x 4 threads over (x, z) or (x, y) in 3D, 256 along x in 2D
+    const dim3 bw = G.dim == 3 ? dim3(64, 4) : dim3(256, 1);
+    const dim3 gy((G.nx + bw.x - 1) / bw.x, (G.nz + bw.y - 1) / bw.y, (G.ny + ADJ_KC - 1) / ADJ_KC);
+    const dim3 gz((G.nx + bw.x - 1) / bw.x, (G.ny + bw.y - 1) / bw.y, (G.nz + ADJ_KC - 1) / ADJ_KC);
     auto adj = [&](const double *psi, const double *base, double *out) {
-        if (G.dim == 3) {
+        if (G.dim == 3)
             hipLaunchKernelGGL(k_adj_x<3>, grd, blk, 0, c->stream, psi, F[FCFD_NU], w, base, out, ub, G, c->vscale);
-            hipLaunchKernelGGL(k_adj_y<3>, grd, blk, 0, c->stream, psi, F[FCFD_NV], w, out, vb, G, c->vscale);
-            hipLaunchKernelGGL(k_adj_z, gz, blk, 0, c->stream, psi, F[FCFD_NW], w, out, wb, G, c->vscale);
-        } else {
+        else
             hipLaunchKernelGGL(k_adj_x<2>, grd, blk, 0, c->stream, psi, F[FCFD_NU], w, base, out, ub, G, c->vscale);
-            hipLaunchKernelGGL(k_adj_y<2>, grd, blk, 0, c->stream, psi, F[FCFD_NV], w, out, vb, G, c->vscale);
-        }
+        hipLaunchKernelGGL(k_adj_walk<1>, gy, bw, 0, c->stream, psi, F[FCFD_NV], w, out, vb, G, c->vscale);
+        if (G.dim == 3)
+            hipLaunchKernelGGL(k_adj_walk<2>, gz, bw, 0, c->stream, psi, F[FCFD_NW], w, out, wb, G, c->vscale);
     };
     // lambda2 = lambda3 + J(phi2)^T (8 dt/12 lambda3)
     hipLaunchKernelGGL(k_adj_seed, grd, blk, 0, c->stream, w, lambda, nullptr, nullptr, 8.0 * dt / 12.0, 0.0, 0.0,

@@ -1761,10 +1761,13 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
 // FCG_KC planes of its (i, j) column and carries z and rho of the planes
 // k-1, k, k+1 in registers: a plane-per-block launch re-fetched both from
 // HBM, because the ten streams of one plane overflow an XCD's 4 MB L2
-// before the next plane needs them again.
+// before the next plane needs them again.  At the default occupancy target
+// (8 waves per SIMD, <= 64 VGPRs) the compiler issues the neighbour loads of
+// a plane one at a time, each followed by a wait; allowing 5 waves per SIMD
+// lets it issue them together (about 7 % faster at 256^3).
 constexpr int FCG_KC = 16;
 template <typename T, int DIM>
-__global__ void __launch_bounds__(256) k_fcg_dir(const T *__restrict__ z, T *__restrict__ zo,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 5))) k_fcg_dir(const T *__restrict__ z, T *__restrict__ zo,
                                                  double *__restrict__ p, double *__restrict__ q,
                                                  double *__restrict__ x, const double *__restrict__ rho, Lay L,
                                                  Split S, Inv2 ih, const double *__restrict__ sc,
