@@ -348,6 +348,8 @@ extern "C" int fcfd_destroy(fcfd_ctx *c) {
     if (c->pack) hipFree(c->pack);
     if (c->flag) hipFree(c->flag);
     if (c->flag_host) hipHostFree(c->flag_host);
+    for (hipEvent_t e : c->redis_ev)
+        if (e) hipEventDestroy(e);
     for (hipEvent_t e : c->prof.pool) hipEventDestroy(e);
     mg_free(c);
     hipStreamDestroy(c->stream);

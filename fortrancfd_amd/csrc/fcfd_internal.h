@@ -113,6 +113,7 @@ struct fcfd_ctx {
     fcfd::Reduce red;
     unsigned long long *flag;  // device flags (reinit error max bits, done, count)
     unsigned long long *flag_host;
+    hipEvent_t redis_ev[2] = {nullptr, nullptr};   // reinit stop polls in flight
     // ---- multigrid ----
     std::vector<fcfd::MgLevel> mg;
     float *rho32;              // FP32 copy of rho for the mixed inner cycle

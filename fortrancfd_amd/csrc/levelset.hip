@@ -643,6 +643,111 @@ __global__ void __launch_bounds__(256) k_redis_b(double *__restrict__ phi, const
     }
 }
 
+// Pass B in 3D, RB_KC planes per thread: the 3x3x3 sums of s2 and s3 are
+// carried as 3x3 plane sums of planes k-1, k, k+1 (clamped), so a cell
+// loads 9 values of each instead of 27.  Same update as k_redis_b; the sums
+// associate differently (plane by plane), a change of rounding only.  The
+// stage and btn are template arguments so that every load of a plane is
+// issued at the top of its iteration, before any of them is consumed.
+constexpr int RB_KC = 16;
+template <int STAGE, bool BTN>
+__global__ void __launch_bounds__(256) k_redis_b_km(double *__restrict__ phi, const double *__restrict__ S0,
+                                                    const double *__restrict__ grad, const double *__restrict__ s2,
+                                                    const double *__restrict__ s3, double *__restrict__ l1,
+                                                    double *__restrict__ l2, const double *__restrict__ phis,
+                                                    unsigned long long *__restrict__ flags, Geom G, double w,
+                                                    double rdt) {
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y;
+    const int k0 = bz_ * RB_KC, k1 = min(k0 + RB_KC, G.nz);
+    if (flags[1]) return;
+    const bool in = i < G.nx && j < G.ny;
+    double err = 0.0;
+    if (in) {
+        const int im = max(i - 1, 0), ip = min(i + 1, G.nx - 1), jm = max(j - 1, 0), jp = min(j + 1, G.ny - 1);
+        const long c0 = G.id(i, j, 0);
+        const long o[9] = {G.id(im, jm, 0) - c0, G.id(i, jm, 0) - c0, G.id(ip, jm, 0) - c0,
+                           G.id(im, j, 0) - c0,  0,                    G.id(ip, j, 0) - c0,
+                           G.id(im, jp, 0) - c0, G.id(i, jp, 0) - c0,  G.id(ip, jp, 0) - c0};
+        auto plane = [&](long c, double &a, double &b) {
+            double t2[9], t3[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                t2[q] = s2[c + o[q]];
+                t3[q] = s3[c + o[q]];
+            }
+            a = 0.0;
+            b = 0.0;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                a += t2[q];
+                b += t3[q];
+            }
+        };
+        double am = 0.0, bm = 0.0, ac = 0.0, bc = 0.0;
+        if (BTN) {
+            plane(G.id(i, j, max(k0 - 1, 0)), am, bm);
+            plane(G.id(i, j, k0), ac, bc);
+        }
+        long id = G.id(i, j, k0);
+        for (int k = k0; k < k1; ++k, id += G.sxy) {
+            // every load of the plane first
+            const double p0 = phi[id], S = S0[id], gr = grad[id];
+            const double l1o = STAGE >= 1 ? l1[id] : 0.0, l2o = STAGE == 2 ? l2[id] : 0.0;
+            const double pst = STAGE == 2 ? phis[id] : 0.0;
+            double c2 = 0.0, c3 = 0.0, ap = ac, bp = bc;
+            if (BTN) {
+                c2 = s2[id];
+                c3 = s3[id];
+                if (k + 1 <= G.nz - 1) plane(id + G.sxy, ap, bp);
+            }
+            double hh, dd;
+            lsfun(p0, w, hh, dd);
+            double s1 = 0.0;
+            if (BTN) {
+                const double a = 51.0 * c2 + am + ac + ap, b = 51.0 * c3 + bm + bc + bp;
+                if (fabs(b) > 1.0e-12) s1 = a / b * dd;
+                am = ac; bm = bc; ac = ap; bc = bp;
+            }
+            const double l = (S - s1) * gr - S;
+            double src;
+            if (STAGE == 0) { l1[id] = l; src = l; }
+            else if (STAGE == 1) { l2[id] = l; src = (-3.0 * l1o + l) / 4.0; }
+            else { src = (-l1o - l2o + 8.0 * l) / 12.0; }
+            const double pn = p0 - rdt * src;
+            phi[id] = pn;
+            if (STAGE == 2) err = fmax(err, fabs(pst - pn));
+        }
+    }
+    if (STAGE == 2) {
+        __shared__ double sh[4];
+        const double m = block_max256(err, sh);
+        if (threadIdx.x + threadIdx.y == 0) {
+            const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+            atomicMax(&flags[FLAG_SHARD0 + b % NSHARD], (unsigned long long)__double_as_longlong(m));
+        }
+    }
+}
+
+static void launch_redis_b_km(fcfd_ctx *c, dim3 g, dim3 blk, double *phi, const double *grad, int btn, int stage) {
+    const Geom &G = c->G;
+    const double w = c->pr.ls_wid, rdt = c->pr.rdt;
+#define FCFD_RB(S_, B_)                                                                                              \
+    hipLaunchKernelGGL((k_redis_b_km<S_, B_>), g, blk, 0, c->stream, phi, c->s0, grad, c->s2, c->s3, c->l1, c->l2, \
+                       c->phiS, c->flag, G, w, rdt)
+    if (btn) {
+        if (stage == 0) FCFD_RB(0, true);
+        else if (stage == 1) FCFD_RB(1, true);
+        else FCFD_RB(2, true);
+    } else {
+        if (stage == 0) FCFD_RB(0, false);
+        else if (stage == 1) FCFD_RB(1, false);
+        else FCFD_RB(2, false);
+    }
+#undef FCFD_RB
+}
+
 // end of one pseudo step: count it, stop when the max over the shards is
 // <= tol, clear the shards (one block of NSHARD threads per case)
 __global__ void __launch_bounds__(NSHARD) k_redis_check(unsigned long long *flags, double tol) {
@@ -747,7 +852,22 @@ int redis(fcfd_ctx *c, int btn, int *iters) {
         if (nmax > 10000000) return FCFD_ERR_ARG;
     }
     const double tol = G.dim == 3 ? 1.0e-5 : 1.0e-8;
+    const dim3 gkm(grd.x, grd.y, (G.nz + RB_KC - 1) / RB_KC);
+    // The stop test runs on the device (k_redis_check); a stopped case's
+    // kernels return at once.  The done words of pseudo-step `it` are copied
+    // to the host behind it, and before step it+2 is queued the host reads
+    // them, so it stops queueing once every case has stopped while one step
+    // stays queued ahead.  Slots of flag_host past the 4 nb words read below.
+    for (int q = 0; q < 2; ++q)
+        if (!c->redis_ev[q]) FCFD_CHECK(hipEventCreateWithFlags(&c->redis_ev[q], hipEventDisableTiming));
+    unsigned long long *poll[2] = {c->flag_host + 4 * c->nb, c->flag_host + 5 * c->nb};
     for (int it = 0; it < nmax; ++it) {
+        if (it >= 2) {
+            FCFD_CHECK(hipEventSynchronize(c->redis_ev[it & 1]));
+            bool all = true;
+            for (int q = 0; q < c->nb; ++q) all = all && poll[it & 1][q] != 0ull;
+            if (all) break;
+        }
         hipLaunchKernelGGL(k_copy_interior, grd, blk, 0, c->stream, c->phiS, phi, c->flag, G);
         for (int s = 0; s < 3; ++s) {
             {
@@ -762,8 +882,7 @@ int redis(fcfd_ctx *c, int btn, int *iters) {
             {
                 ProfScope ps_(c, R_REDIS_B);
                 if (G.dim == 3)
-                    hipLaunchKernelGGL(k_redis_b<3>, grd, blk, 0, c->stream, phi, c->s0, grad, c->s2, c->s3, c->l1,
-                                       c->l2, c->phiS, c->flag, G, w, rdt, btn, s);
+                    launch_redis_b_km(c, gkm, blk, phi, grad, btn, s);
                 else
                     hipLaunchKernelGGL(k_redis_b<2>, grd, blk, 0, c->stream, phi, c->s0, grad, c->s2, c->s3, c->l1,
                                        c->l2, c->phiS, c->flag, G, w, rdt, btn, s);
@@ -772,6 +891,8 @@ int redis(fcfd_ctx *c, int btn, int *iters) {
             FCFD_TRY(bc_scalar(c, phi));
         }
         hipLaunchKernelGGL(k_redis_check, dim3(c->nb), dim3(NSHARD), 0, c->stream, c->flag, tol);
+        FCFD_TRY(gather_to_host(c, poll[it & 1], c->flag + 1, FLAG_STRIDE, 1, c->nb));
+        FCFD_CHECK(hipEventRecord(c->redis_ev[it & 1], c->stream));
     }
     // words 0..3 of every case's flag block
     FCFD_TRY(gather_to_host(c, c->flag_host, c->flag, FLAG_STRIDE, 4, c->nb));
