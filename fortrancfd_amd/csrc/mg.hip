@@ -285,6 +285,33 @@ __global__ void __launch_bounds__(256) k_smooth(T *__restrict__ e, const T *__re
     }
 }
 
+// Gauss-Seidel value of one cell of the colour-split FP32 level 0 from its
+// density r, the neighbour densities (E, W, N, S, T, B) and the neighbour
+// values e*: (sum c e_nb - f) / sum c, the walls closing faces.  ZERO: the
+// neighbour values are 0 (zero initial guess).  The sums use explicit fmaf,
+// so the rounding does not depend on how the caller is compiled.
+template <int DIM, bool ZERO>
+__device__ __forceinline__ float split_gs(float r, float rE, float rW, float rN, float rS, float rT, float rB,
+                                          float eE, float eW, float eN, float eS, float eT, float eB, float fv,
+                                          int i, int j, int k, const Split &S, Inv2 ih) {
+    const float cE = face<float>(i < S.nx - 1, r, rE, (float)ih.x);
+    const float cW = face<float>(i > 0, r, rW, (float)ih.x);
+    const float cN = face<float>(j < S.ny - 1, r, rN, (float)ih.y);
+    const float cS = face<float>(j > 0, r, rS, (float)ih.y);
+    float cT = 0.f, cB = 0.f;
+    if (DIM == 3) {
+        cT = face<float>(k < S.nz - 1, r, rT, (float)ih.z);
+        cB = face<float>(k > 0, r, rB, (float)ih.z);
+    }
+    float sc = cE + cW + cN + cS, sce = 0.f;
+    if (!ZERO) sce = fmaf(cS, eS, fmaf(cN, eN, fmaf(cW, eW, cE * eE)));
+    if (DIM == 3) {
+        sc += cT + cB;
+        if (!ZERO) sce += fmaf(cB, eB, cT * eT);
+    }
+    return sc > 0.f ? (sce - fv) * __builtin_amdgcn_rcpf(sc) : 0.f;
+}
+
 // One colour of the red-black Gauss-Seidel sweep on the colour-split FP32
 // level 0 (the mixed cycle): e_P = (sum c e_nb - f) / sum c.  A half-sweep
 // reads the other colour's e, this colour's f and both colours' rho and
@@ -311,32 +338,21 @@ __global__ void __launch_bounds__(256) k_smooth_split(float *__restrict__ e, con
     if (upd) {
         const RT *ro = rho + oc;
         const float *eo = e + oc;
-        const float r = (float)rho[cc + id];
         // the x neighbours i-1 and i+1 are slots m-1+s and m+s of the other colour
         const long xw = id - 1 + s, xe = id + s;
-        const float cE = face<float>(i < S.nx - 1, r, (float)ro[xe], (float)ih.x);
-        const float cW = face<float>(i > 0, r, (float)ro[xw], (float)ih.x);
-        const float cN = face<float>(j < S.ny - 1, r, (float)ro[id + S.sx], (float)ih.y);
-        const float cS = face<float>(j > 0, r, (float)ro[id - S.sx], (float)ih.y);
-        float cT = 0.f, cB = 0.f;
-        if (DIM == 3) {
-            cT = face<float>(k < S.nz - 1, r, (float)ro[id + S.sxy], (float)ih.z);
-            cB = face<float>(k > 0, r, (float)ro[id - S.sxy], (float)ih.z);
-        }
         auto nb = [&](long o, int ii, int jj, int kk) -> float {
+            if (MODE == 2) return 0.f;
             float v = eo[o];
             if (MODE == 1) v += P.at(ii, jj, kk);
             return v;
         };
-        float sc = cE + cW + cN + cS, sce = 0.f;
-        if (MODE != 2)
-            sce = cE * nb(xe, i + 1, j, k) + cW * nb(xw, i - 1, j, k) + cN * nb(id + S.sx, i, j + 1, k) +
-                  cS * nb(id - S.sx, i, j - 1, k);
-        if (DIM == 3) {
-            sc += cT + cB;
-            if (MODE != 2) sce += cT * nb(id + S.sxy, i, j, k + 1) + cB * nb(id - S.sxy, i, j, k - 1);
-        }
-        enew = sc > 0.f ? (sce - f[cc + id]) * __builtin_amdgcn_rcpf(sc) : 0.f;
+        const long ot = DIM == 3 ? id + S.sxy : id, ob = DIM == 3 ? id - S.sxy : id;
+        enew = split_gs<DIM, MODE == 2>((float)rho[cc + id], (float)ro[xe], (float)ro[xw], (float)ro[id + S.sx],
+                                        (float)ro[id - S.sx], DIM == 3 ? (float)ro[ot] : 0.f,
+                                        DIM == 3 ? (float)ro[ob] : 0.f, nb(xe, i + 1, j, k), nb(xw, i - 1, j, k),
+                                        nb(id + S.sx, i, j + 1, k), nb(id - S.sx, i, j - 1, k),
+                                        DIM == 3 ? nb(ot, i, j, k + 1) : 0.f, DIM == 3 ? nb(ob, i, j, k - 1) : 0.f,
+                                        f[cc + id], i, j, k, S, ih);
         e[cc + id] = enew;
     }
     if (DOTS) {
