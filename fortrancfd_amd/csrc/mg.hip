@@ -403,8 +403,8 @@ __device__ __forceinline__ float apply_A_split(const float *__restrict__ e, cons
     return acc;
 }
 
-// k_resid_restrict for the colour-split FP32 level 0
-template <int DIM, typename RT>
+// k_resid_restrict for the colour-split FP32 level 0 (C2 as there)
+template <int DIM, typename RT, bool C2 = false>
 __global__ void __launch_bounds__(256) k_resid_restrict_split(const float *__restrict__ e, const float *__restrict__ f,
                                                               const RT *__restrict__ rho, Split S, Inv2 ih,
                                                               float *__restrict__ fc, float *__restrict__ ec, Lay C,
@@ -413,10 +413,14 @@ __global__ void __launch_bounds__(256) k_resid_restrict_split(const float *__res
     xcd_block(bx_, by_, bz_);
     const int I = bx_ * blockDim.x + threadIdx.x, J = by_ * blockDim.y + threadIdx.y, K = bz_;
     if (I >= C.nx || J >= C.ny) return;
+    if (C2) { cx = 2; cy = 2; cz = DIM == 3 ? 2 : 1; }
     float acc = 0.f;
-    for (int kk = 0; kk < cz; ++kk)
-        for (int jj = 0; jj < cy; ++jj)
-            for (int ii = 0; ii < cx; ++ii) {
+#pragma unroll
+    for (int kk = 0; kk < (C2 ? (DIM == 3 ? 2 : 1) : cz); ++kk)
+#pragma unroll
+        for (int jj = 0; jj < (C2 ? 2 : cy); ++jj)
+#pragma unroll
+            for (int ii = 0; ii < (C2 ? 2 : cx); ++ii) {
                 const int i = I * cx + ii, j = J * cy + jj, k = K * cz + kk;
                 acc += f[S.at(i, j, k)] - apply_A_split<DIM, RT>(e, rho, S, ih, i, j, k);
             }
@@ -532,8 +536,11 @@ __global__ void __launch_bounds__(FUSED_THREADS) k_smooth_fused(const float *__r
 }
 
 // residual of the level, restricted to the coarse right-hand side by the
-// mean over the children; the coarse error is zeroed in the same pass
-template <typename T, int DIM, bool FINE>
+// mean over the children; the coarse error is zeroed in the same pass.
+// C2: the factors are 2 per axis (1 along the cases of a 2D stack), known at
+// compile time, so the children's loads are issued together rather than one
+// child per memory round trip; same summation order as the runtime form.
+template <typename T, int DIM, bool FINE, bool C2 = false>
 __global__ void __launch_bounds__(256) k_resid_restrict(const T *__restrict__ e, const T *__restrict__ f,
                                                         const T *__restrict__ rho, const T *__restrict__ bx,
                                                         const T *__restrict__ by, const T *__restrict__ bz, Lay L,
@@ -543,10 +550,14 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const T *__restrict__ e,
     xcd_block(bx_, by_, bz_);
     const int I = bx_ * blockDim.x + threadIdx.x, J = by_ * blockDim.y + threadIdx.y, K = bz_;
     if (I >= C.nx || J >= C.ny) return;
+    if (C2) { cx = 2; cy = 2; cz = DIM == 3 ? 2 : 1; }
     T acc = 0;
-    for (int kk = 0; kk < cz; ++kk)
-        for (int jj = 0; jj < cy; ++jj)
-            for (int ii = 0; ii < cx; ++ii) {
+#pragma unroll
+    for (int kk = 0; kk < (C2 ? (DIM == 3 ? 2 : 1) : cz); ++kk)
+#pragma unroll
+        for (int jj = 0; jj < (C2 ? 2 : cy); ++jj)
+#pragma unroll
+            for (int ii = 0; ii < (C2 ? 2 : cx); ++ii) {
                 int i = I * cx + ii, j = J * cy + jj, k = K * cz + kk;
                 long id = L.id(i, j, k);
                 acc += f[id] - apply_A<T, DIM, FINE>(e, rho, bx, by, bz, L, ih, i, j, k, id);
@@ -1654,23 +1665,43 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         arrays(lv + 1, ec, fc_, a, b, cc);
         dim3 g = grd3(C.nx, C.ny, C.nz, blk);
         ProfScope ps_(c, lv == 0 ? R_RESTRICT0 : R_RESTRICTC);
+        const int mz = DIM == 3 ? M.cz : 1;
+        const bool c2 = M.cx == 2 && M.cy == 2 && mz == (DIM == 3 ? 2 : 1);
         if (lv == 0) {
             if constexpr (sizeof(T) == 4) {
-                if (half_rho)
-                    hipLaunchKernelGGL((k_resid_restrict_split<DIM, _Float16>), g, blk, 0, c->stream, e, f,
-                                       (const _Float16 *)c->rho16, split_of(G), ih, (T *)fc_, ec, LC, M.cx, M.cy,
-                                       DIM == 3 ? M.cz : 1);
+                const Split S0 = split_of(G);
+                if (half_rho) {
+                    const _Float16 *r16 = (const _Float16 *)c->rho16;
+                    if (c2)
+                        hipLaunchKernelGGL((k_resid_restrict_split<DIM, _Float16, true>), g, blk, 0, c->stream, e, f,
+                                           r16, S0, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+                    else
+                        hipLaunchKernelGGL((k_resid_restrict_split<DIM, _Float16, false>), g, blk, 0, c->stream, e,
+                                           f, r16, S0, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+                } else {
+                    const float *r32 = (const float *)rho0;
+                    if (c2)
+                        hipLaunchKernelGGL((k_resid_restrict_split<DIM, float, true>), g, blk, 0, c->stream, e, f,
+                                           r32, S0, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+                    else
+                        hipLaunchKernelGGL((k_resid_restrict_split<DIM, float, false>), g, blk, 0, c->stream, e, f,
+                                           r32, S0, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+                }
+            } else {
+                if (c2)
+                    hipLaunchKernelGGL((k_resid_restrict<T, DIM, true, true>), g, blk, 0, c->stream, e, f, rho0, bx,
+                                       by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
                 else
-                    hipLaunchKernelGGL((k_resid_restrict_split<DIM, float>), g, blk, 0, c->stream, e, f,
-                                       (const float *)rho0, split_of(G), ih, (T *)fc_, ec, LC, M.cx, M.cy,
-                                       DIM == 3 ? M.cz : 1);
+                    hipLaunchKernelGGL((k_resid_restrict<T, DIM, true, false>), g, blk, 0, c->stream, e, f, rho0, bx,
+                                       by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
             }
-            else
-                hipLaunchKernelGGL((k_resid_restrict<T, DIM, true>), g, blk, 0, c->stream, e, f, rho0, bx, by, bz, L,
-                                   ih, (T *)fc_, ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
-        } else
-            hipLaunchKernelGGL((k_resid_restrict<T, DIM, false>), g, blk, 0, c->stream, e, f, (const T *)nullptr, bx,
-                               by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
+        } else if (c2) {
+            hipLaunchKernelGGL((k_resid_restrict<T, DIM, false, true>), g, blk, 0, c->stream, e, f,
+                               (const T *)nullptr, bx, by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+        } else {
+            hipLaunchKernelGGL((k_resid_restrict<T, DIM, false, false>), g, blk, 0, c->stream, e, f,
+                               (const T *)nullptr, bx, by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+        }
         FCFD_CHECK(hipGetLastError());
     }
     if (lt < nl) {
