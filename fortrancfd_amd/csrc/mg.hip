@@ -157,6 +157,22 @@ __device__ __forceinline__ void stencil(const T *__restrict__ e, const T *__rest
                                         const T *__restrict__ bx, const T *__restrict__ by,
                                         const T *__restrict__ bz, const Lay &L, Inv2 ih, int i, int j, int k,
                                         long id, const Prol<T> &P, T &sum_c, T &sum_ce, T &ep) {
+    // the values first, so that every load of the cell issues before the
+    // coefficient arithmetic (one memory round trip instead of two)
+    auto nb = [&](long o, int ii, int jj, int kk) -> T {
+        if (PROL == 3) return (T)0;
+        T v = e[id + o];
+        if (PROL == 1) v += P.at(ii, jj, kk);
+        return v;
+    };
+    const T vE = nb(1, i + 1, j, k), vW = nb(-1, i - 1, j, k), vN = nb(L.sx, i, j + 1, k),
+            vS = nb(-L.sx, i, j - 1, k);
+    const T vT = DIM == 3 ? nb(L.sxy, i, j, k + 1) : (T)0, vB = DIM == 3 ? nb(-L.sxy, i, j, k - 1) : (T)0;
+    if (PROL >= 3) ep = (T)0;
+    else {
+        ep = e[id];
+        if (PROL != 0) ep += P.at(i, j, k);
+    }
     T cw, ce_, cs, cn, cb = 0, ct = 0;
     if (FINE) {
         T r = rho[id];
@@ -178,23 +194,11 @@ __device__ __forceinline__ void stencil(const T *__restrict__ e, const T *__rest
             cb = bz[id - L.sxy] * (T)ih.z;
         }
     }
-    auto nb = [&](long o, int ii, int jj, int kk) -> T {
-        if (PROL == 3) return (T)0;
-        T v = e[id + o];
-        if (PROL == 1) v += P.at(ii, jj, kk);
-        return v;
-    };
-    if (PROL >= 3) ep = (T)0;
-    else {
-        ep = e[id];
-        if (PROL != 0) ep += P.at(i, j, k);
-    }
     sum_c = ce_ + cw + cn + cs;
-    sum_ce = ce_ * (nb(1, i + 1, j, k) - ep) + cw * (nb(-1, i - 1, j, k) - ep) + cn * (nb(L.sx, i, j + 1, k) - ep) +
-             cs * (nb(-L.sx, i, j - 1, k) - ep);
+    sum_ce = ce_ * (vE - ep) + cw * (vW - ep) + cn * (vN - ep) + cs * (vS - ep);
     if (DIM == 3) {
         sum_c += ct + cb;
-        sum_ce += ct * (nb(L.sxy, i, j, k + 1) - ep) + cb * (nb(-L.sxy, i, j, k - 1) - ep);
+        sum_ce += ct * (vT - ep) + cb * (vB - ep);
     }
 }
 
@@ -256,8 +260,11 @@ __global__ void __launch_bounds__(256) k_smooth(T *__restrict__ e, const T *__re
         T sc, sce, ep;
         stencil<T, DIM, FINE, PROL>(e, rho, bx, by, bz, L, ih, i, j, k, id, P, sc, sce, ep);
         // correction form e_P += (sum c (e_nb - e_P) - f) / sum c: the Gauss-Seidel
-        // update without cancelling terms of size c |e|
-        enew = sc > (T)0 ? ep + gs_step<T>(sce - f[id], sc) : ep;
+        // update without cancelling terms of size c |e|.  Formed for every
+        // cell and selected (sc = 0 only off the grid), so that no branch
+        // holds back the value loads until the coefficients have arrived.
+        const T d = gs_step<T>(sce - f[id], sc);
+        enew = sc > (T)0 ? ep + d : ep;
         e[id] = enew;
     }
     if (DOTS) {
@@ -309,7 +316,10 @@ __device__ __forceinline__ float split_gs(float r, float rE, float rW, float rN,
         sc += cT + cB;
         if (!ZERO) sce += fmaf(cB, eB, cT * eT);
     }
-    return sc > 0.f ? (sce - fv) * __builtin_amdgcn_rcpf(sc) : 0.f;
+    // formed unconditionally and selected (sc = 0 only off the grid): a
+    // branch here would hold back the value loads behind the density loads
+    const float d = (sce - fv) * __builtin_amdgcn_rcpf(sc);
+    return sc > 0.f ? d : 0.f;
 }
 
 // One colour of the red-black Gauss-Seidel sweep on the colour-split FP32
