@@ -396,6 +396,98 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage_km(const T *__restrict__ p
     }
 }
 
+// The same stage in 3D with every x and y face of a plane computed once: a
+// 64 x 4 block needs the 4 x 65 x-faces and 5 x 64 y-faces of its tile, which
+// its 256 threads compute into LDS (580 faces instead of the 1024 that two
+// faces per cell and direction take), then each cell differences them; the z
+// faces stay in the per-column register window.  The LDS plane is double
+// buffered, one barrier per plane.  Face value F = WENO_left(s1 at f-2..f+2)
+// + WENO_right(s2 at f-1..f+3), clamped positions, as k_ls_adv_stage_km.
+template <typename T>
+__global__ void __launch_bounds__(256) k_ls_adv_stage_kf(const T *__restrict__ phi, T *__restrict__ phin,
+                                                         const T *__restrict__ nu, const T *__restrict__ nv,
+                                                         const T *__restrict__ nw, T *__restrict__ l1,
+                                                         T *__restrict__ l2, Geom G, T dt, T vs, int stage) {
+    constexpr int TX = 64, TY = 4, NFX = TY * (TX + 1), NFY = (TY + 1) * TX;
+    __shared__ T fxs[2][TY][TX + 1], fys[2][TY + 1][TX];
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * TX + tx;
+    const int i0 = bx_ * TX, j0 = by_ * TY, i = i0 + tx, j = j0 + ty;
+    const int k0 = bz_ * LS_KC, k1 = min(k0 + LS_KC, G.nz);
+    const bool in = i < G.nx && j < G.ny;
+    const int ic = min(i, G.nx - 1), jc = min(j, G.ny - 1);
+    const T half = (T)0.5, EPS = (T)1.0e-13;
+    const T hx = (T)G.dx, hy = (T)G.dy, hz = (T)G.dz;
+    // face f + 1/2 of a line: positions clamp(f - 2 + m), m = 0..5, at stride st from the line start a0
+    auto face = [&](const T *__restrict__ vel, long a0, long st, int f, int n) -> T {
+        T s1[6], s2[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            const long a = a0 + (long)clampi(f - 2 + m, 0, n - 1) * st;
+            const T u = vs * vel[a], p = phi[a];
+            s1[m] = half * (u + fabs(u)) * p;
+            s2[m] = half * (u - fabs(u)) * p;
+        }
+        return weno_left(s1[0], s1[1], s1[2], s1[3], s1[4], EPS) + weno_right(s2[1], s2[2], s2[3], s2[4], s2[5], EPS);
+    };
+    // z window of this column: positions clamp(k - 3 + m), m = 0..6
+    T s1z[7], s2z[7];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+        const long a = G.id(ic, jc, clampi(k0 - 3 + m, 0, G.nz - 1));
+        const T u = vs * nw[a], p = phi[a];
+        s1z[m] = half * (u + fabs(u)) * p;
+        s2z[m] = half * (u - fabs(u)) * p;
+    }
+    T Fl = weno_left(s1z[0], s1z[1], s1z[2], s1z[3], s1z[4], EPS) +
+           weno_right(s2z[1], s2z[2], s2z[3], s2z[4], s2z[5], EPS);
+    for (int k = k0; k < k1; ++k) {
+        const int b = k & 1;
+        for (int e = t; e < NFX + NFY; e += TX * TY) {
+            if (e < NFX) {   // x-face i0 - 1 + c + 1/2 of row j0 + r
+                const int r = e / (TX + 1), c = e - r * (TX + 1);
+                fxs[b][r][c] = face(nu, G.id(0, min(j0 + r, G.ny - 1), k), 1, i0 - 1 + c, G.nx);
+            } else {         // y-face j0 - 1 + r + 1/2 of column i0 + c
+                const int q = e - NFX, r = q / TX, c = q - r * TX;
+                fys[b][r][c] = face(nv, G.id(min(i0 + c, G.nx - 1), 0, k), G.sx, j0 - 1 + r, G.ny);
+            }
+        }
+        __syncthreads();
+        // face k + 1/2: gm from s1(k-2..k+2), fp from s2(k-1..k+3)
+        const T Fr = weno_left(s1z[1], s1z[2], s1z[3], s1z[4], s1z[5], EPS) +
+                     weno_right(s2z[2], s2z[3], s2z[4], s2z[5], s2z[6], EPS);
+        if (in) {
+            const long id = G.id(i, j, k);
+            T L = -(fxs[b][ty][tx + 1] - fxs[b][ty][tx]) / hx;
+            L = L + (-(fys[b][ty + 1][tx] - fys[b][ty][tx]) / hy);
+            L = L + (-(Fr - Fl) / hz);
+            T src;
+            if (stage == 0) {
+                l1[id] = L;
+                src = L;
+            } else if (stage == 1) {
+                l2[id] = L;
+                src = ((T)-3.0 * l1[id] + L) / (T)4.0;
+            } else {
+                src = (-l1[id] - l2[id] + (T)8.0 * L) / (T)12.0;
+            }
+            phin[id] = phi[id] + src * dt;
+        }
+        Fl = Fr;
+        // slide the z window to plane k + 1
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            s1z[m] = s1z[m + 1];
+            s2z[m] = s2z[m + 1];
+        }
+        const long a = G.id(ic, jc, clampi(k + 4, 0, G.nz - 1));
+        const T u = vs * nw[a], p = phi[a];
+        s1z[6] = half * (u + fabs(u)) * p;
+        s2z[6] = half * (u - fabs(u)) * p;
+    }
+}
+
 // One SSP-RK3 stage with the UCCD source L = -(u phi_x + v phi_y + w phi_z)
 // (the reference's commented alternative source,
 // `.../interface_solver/level_set_rk3_solver.f90:155-157`), from the upwind
@@ -438,7 +530,7 @@ int ls_adv_stage(fcfd_ctx *c, const double *phi, double *phin, int stage) {
     const Geom &G = c->G;
     dim3 blk(64, 4), grd = grid_interior(G, blk);
     if (G.dim == 3)
-        hipLaunchKernelGGL((k_ls_adv_stage_km<double>), grid_km(G, blk), blk, 0, c->stream, phi, phin,
+        hipLaunchKernelGGL((k_ls_adv_stage_kf<double>), grid_km(G, blk), blk, 0, c->stream, phi, phin,
                            c->f[FCFD_NU], c->f[FCFD_NV], c->f[FCFD_NW], c->l1, c->l2, G, c->pr.dt, c->vscale, stage);
     else
         hipLaunchKernelGGL((k_ls_adv_stage<2, double>), grd, blk, 0, c->stream, phi, phin, c->f[FCFD_NU],
@@ -482,7 +574,7 @@ int ls_advect(fcfd_ctx *c) {
         {
             ProfScope ps_(c, R_LS_ADV);
             if (G.dim == 3)
-                hipLaunchKernelGGL((k_ls_adv_stage_km<double>), grid_km(G, blk), blk, 0, c->stream, a, b,
+                hipLaunchKernelGGL((k_ls_adv_stage_kf<double>), grid_km(G, blk), blk, 0, c->stream, a, b,
                                    c->f[FCFD_NU], c->f[FCFD_NV], c->f[FCFD_NW], c->l1, c->l2, G, c->pr.dt, c->vscale,
                                    s);
             else
@@ -515,7 +607,7 @@ int ls_advect_f32(fcfd_ctx *c, float *phi, const float *nu, const float *nv, con
         {
             ProfScope ps_(c, R_LS_ADV);
             if (G.dim == 3)
-                hipLaunchKernelGGL((k_ls_adv_stage_km<float>), grid_km(G, blk), blk, 0, c->stream, in[s], out[s], nu,
+                hipLaunchKernelGGL((k_ls_adv_stage_kf<float>), grid_km(G, blk), blk, 0, c->stream, in[s], out[s], nu,
                                    nv, nw, l1,
                                    l2, G, dt, vs, s);
             else
