@@ -309,100 +309,18 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage(const T *__restrict__ phi,
     phin[id] = phi[id] + src * dt;
 }
 
-// 3D form of k_ls_adv_stage: each thread walks LS_KC planes of its (i, j)
-// column.  The z flux of face k+1/2 is computed once and is the k-1/2 flux
-// of the next plane, and the z stencil is a window of 7 planes carried in
-// registers (one new (w, phi) pair loaded per plane): 10 reconstructions per
-// cell instead of 12 and one z load instead of seven.  Same formulas; the z
-// part is summed as F(k+1/2) - F(k-1/2), so results differ from the
-// one-plane kernel only by round-off.
-constexpr int LS_KC = 16;
-template <typename T>
-__global__ void __launch_bounds__(256) k_ls_adv_stage_km(const T *__restrict__ phi, T *__restrict__ phin,
-                                                         const T *__restrict__ nu, const T *__restrict__ nv,
-                                                         const T *__restrict__ nw, T *__restrict__ l1,
-                                                         T *__restrict__ l2, Geom G, T dt, T vs, int stage) {
-    int bx_, by_, bz_;
-    xcd_block(bx_, by_, bz_);
-    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y;
-    const int k0 = bz_ * LS_KC, k1 = min(k0 + LS_KC, G.nz);
-    if (i >= G.nx || j >= G.ny) return;
-    const T half = (T)0.5, EPS = (T)1.0e-13;
-    const T hx = (T)G.dx, hy = (T)G.dy, hz = (T)G.dz;
-    // z window: positions clamp(k - 3 + m), m = 0..6
-    T s1z[7], s2z[7];
-#pragma unroll
-    for (int m = 0; m < 7; ++m) {
-        const long a = G.id(i, j, clampi(k0 - 3 + m, 0, G.nz - 1));
-        const T u = vs * nw[a], p = phi[a];
-        s1z[m] = half * (u + fabs(u)) * p;
-        s2z[m] = half * (u - fabs(u)) * p;
-    }
-    // face k0 - 1/2: gm from s1(k0-3..k0+1), fp from s2(k0-2..k0+2)
-    T Fl = weno_left(s1z[0], s1z[1], s1z[2], s1z[3], s1z[4], EPS) +
-           weno_right(s2z[1], s2z[2], s2z[3], s2z[4], s2z[5], EPS);
-    long id = G.id(i, j, k0);
-    for (int k = k0; k < k1; ++k, id += G.sxy) {
-        const T *vel[2] = {nu, nv};
-        const int n[2] = {G.nx, G.ny};
-        const int pos[2] = {i, j};
-        const long st[2] = {1, G.sx};
-        const T h[2] = {hx, hy};
-        T L = 0;
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-            T s1[7], s2[7];
-#pragma unroll
-            for (int o = -3; o <= 3; ++o) {
-                const int q = clampi(pos[d] + o, 0, n[d] - 1);
-                const long a = id + (long)(q - pos[d]) * st[d];
-                const T u = vs * vel[d][a], p = phi[a];
-                s1[o + 3] = half * (u + fabs(u)) * p;
-                s2[o + 3] = half * (u - fabs(u)) * p;
-            }
-            const T gmR = weno_left(s1[1], s1[2], s1[3], s1[4], s1[5], EPS);
-            const T fpR = weno_right(s2[2], s2[3], s2[4], s2[5], s2[6], EPS);
-            const T gmL = weno_left(s1[0], s1[1], s1[2], s1[3], s1[4], EPS);
-            const T fpL = weno_right(s2[1], s2[2], s2[3], s2[4], s2[5], EPS);
-            const T t = -(gmR + fpR - gmL - fpL) / h[d];
-            L = d == 0 ? t : L + t;
-        }
-        // face k + 1/2: gm from s1(k-2..k+2), fp from s2(k-1..k+3)
-        const T Fr = weno_left(s1z[1], s1z[2], s1z[3], s1z[4], s1z[5], EPS) +
-                     weno_right(s2z[2], s2z[3], s2z[4], s2z[5], s2z[6], EPS);
-        L = L + (-(Fr - Fl) / hz);
-        Fl = Fr;
-        T src;
-        if (stage == 0) {
-            l1[id] = L;
-            src = L;
-        } else if (stage == 1) {
-            l2[id] = L;
-            src = ((T)-3.0 * l1[id] + L) / (T)4.0;
-        } else {
-            src = (-l1[id] - l2[id] + (T)8.0 * L) / (T)12.0;
-        }
-        phin[id] = phi[id] + src * dt;
-        // slide the z window to plane k + 1
-#pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            s1z[m] = s1z[m + 1];
-            s2z[m] = s2z[m + 1];
-        }
-        const long a = G.id(i, j, clampi(k + 4, 0, G.nz - 1));
-        const T u = vs * nw[a], p = phi[a];
-        s1z[6] = half * (u + fabs(u)) * p;
-        s2z[6] = half * (u - fabs(u)) * p;
-    }
-}
+constexpr int LS_KC = 16;   // planes per block of the 3D stage
 
-// The same stage in 3D with every x and y face of a plane computed once: a
-// 64 x 4 block needs the 4 x 65 x-faces and 5 x 64 y-faces of its tile, which
-// its 256 threads compute into LDS (580 faces instead of the 1024 that two
-// faces per cell and direction take), then each cell differences them; the z
-// faces stay in the per-column register window.  The LDS plane is double
-// buffered, one barrier per plane.  Face value F = WENO_left(s1 at f-2..f+2)
-// + WENO_right(s2 at f-1..f+3), clamped positions, as k_ls_adv_stage_km.
+// 3D form of k_ls_adv_stage with every face computed once.  Each block walks
+// LS_KC planes of a 64 x 4 column tile.  Per plane, its 256 threads compute
+// the 4 x 65 x-faces and 5 x 64 y-faces of the tile into LDS (580 faces
+// where two faces per cell and direction take 1024), then each cell
+// differences them.  The z flux of face k+1/2 is the k-1/2 flux of the next
+// plane, from a 7-plane window of (w, phi) splits carried in registers.  The
+// LDS plane is double buffered, one barrier per plane.  Face value F =
+// WENO_left(s1 at f-2..f+2) + WENO_right(s2 at f-1..f+3) at clamped
+// positions; L sums F(+1/2) - F(-1/2) per direction, so results differ from
+// the one-plane kernel only by round-off.
 template <typename T>
 __global__ void __launch_bounds__(256) k_ls_adv_stage_kf(const T *__restrict__ phi, T *__restrict__ phin,
                                                          const T *__restrict__ nu, const T *__restrict__ nv,
@@ -519,7 +437,7 @@ __global__ void __launch_bounds__(256) k_ls_uccd_stage(const double *__restrict_
     phin[id] = phi[id] + src * dt;
 }
 
-// launch grid of k_ls_adv_stage_km: LS_KC planes per block
+// launch grid of k_ls_adv_stage_kf: LS_KC planes per block
 static dim3 grid_km(const Geom &G, dim3 blk) {
     return dim3((G.nx + blk.x - 1) / blk.x, (G.ny + blk.y - 1) / blk.y, (G.nz + LS_KC - 1) / LS_KC);
 }
