@@ -66,9 +66,10 @@ def region_bytes(case, mixed: bool) -> dict:
         # other colour (N/2), f of this colour (N/2), rho (N, FP32 or FP16)}
         # W{e of this colour (N/2)}; averaged over the 8 launches of a V(2,2) cycle:
         # + R{ec} (N/nchild0 coarse values) in the first post-sweep half
-        # (prolongation folded in) and + R{r (FP64), z_old} in the last
-        # half-sweep (CG dots folded in)
-        "mg_smooth_l0": N * (8 * (1.5 * b + br) + b / nchild0 + (8 + b)) / 8,
+        # (prolongation folded in) and, in the last half-sweep (CG dots
+        # folded in), + R{z_old} and the residual of the pair's other cell
+        # (FP32 split f, N/2, in the mixed cycle; FP64 r otherwise)
+        "mg_smooth_l0": N * (8 * (1.5 * b + br) + b / nchild0 + ((0.5 * b + b) if mixed else (8 + b))) / 8,
         "mg_restrict_l0": N * (2 * b + br + b * 2.0 / nchild0),   # R{e,f,rho} W{fc,ec}
         "mg_resid_l0": N * (3 * 8 + (2 * 4 if mixed else 0)),  # R{p,f,rho} (W{r32,e32})
         # flexible CG, averaged over its two launches per iteration:

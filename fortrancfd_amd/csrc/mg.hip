@@ -366,16 +366,21 @@ __global__ void __launch_bounds__(256) k_smooth_split(float *__restrict__ e, con
         e[cc + id] = enew;
     }
     if (DOTS) {
+        // r is taken as f, the FP32 right-hand side of this cycle (r rounded
+        // to FP32), which the update has just read for this cell: the dots only
+        // set the flexible-CG step lengths, while the FP64 residual update
+        // keeps the outer iteration exact, and the pair costs 4 B per cell
+        // instead of the 8 B of the FP64 residual
         __shared__ double sh[4];
         double rz = 0.0, rzo = 0.0;
         if (upd) {
-            const double rv = D.r[L.id(i, j, k)];
+            const double rv = (double)f[cc + id];
             rz = rv * (double)enew;
             rzo = rv * (double)D.zo[cc + id];
         }
         const int ip = 2 * m + 1 - s;   // the pair's other cell: final from the previous half-sweep
         if (ip < S.nx && j < S.ny) {
-            const double rv = D.r[L.id(ip, j, k)];
+            const double rv = (double)f[oc + id];
             rz += rv * (double)e[oc + id];
             rzo += rv * (double)D.zo[oc + id];
         }
