@@ -54,6 +54,9 @@ struct MgLevel {
     // FP32 mirrors for the mixed-precision inner cycle; e32b is the second
     // error buffer of the fused multi-sweep smoother (read one, write the other)
     float *e32, *f32, *r32, *bx32, *by32, *bz32, *e32b;
+    // FP16 face coefficients of level 1 in 3D (params.ppe_rho_half), read by
+    // its FP32 smoothing and restriction; NULL elsewhere
+    _Float16 *bx16, *by16, *bz16;
 };
 
 // Kernel regions timed by the built-in HIP-event profiler (fcfd_prof_*).

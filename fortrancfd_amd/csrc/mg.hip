@@ -152,10 +152,10 @@ struct Prol {
 // 3 and 4: the same two half-sweeps for a zero initial guess: the values are
 // taken as 0 instead of read (3: centre and neighbours, 4: centre only), so
 // the error array needs no clearing between cycles.
-template <typename T, int DIM, bool FINE, int PROL>
+template <typename T, int DIM, bool FINE, int PROL, typename CT = T>
 __device__ __forceinline__ void stencil(const T *__restrict__ e, const T *__restrict__ rho,
-                                        const T *__restrict__ bx, const T *__restrict__ by,
-                                        const T *__restrict__ bz, const Lay &L, Inv2 ih, int i, int j, int k,
+                                        const CT *__restrict__ bx, const CT *__restrict__ by,
+                                        const CT *__restrict__ bz, const Lay &L, Inv2 ih, int i, int j, int k,
                                         long id, const Prol<T> &P, T &sum_c, T &sum_ce, T &ep) {
     // the values first, so that every load of the cell issues before the
     // coefficient arithmetic (one memory round trip instead of two)
@@ -185,13 +185,13 @@ __device__ __forceinline__ void stencil(const T *__restrict__ e, const T *__rest
             cb = face<T>(k > 0, r, rho[id - L.sxy], (T)ih.z);
         }
     } else {
-        ce_ = bx[id] * (T)ih.x;
-        cw = bx[id - 1] * (T)ih.x;
-        cn = by[id] * (T)ih.y;
-        cs = by[id - L.sx] * (T)ih.y;
+        ce_ = (T)bx[id] * (T)ih.x;
+        cw = (T)bx[id - 1] * (T)ih.x;
+        cn = (T)by[id] * (T)ih.y;
+        cs = (T)by[id - L.sx] * (T)ih.y;
         if (DIM == 3) {
-            ct = bz[id] * (T)ih.z;
-            cb = bz[id - L.sxy] * (T)ih.z;
+            ct = (T)bz[id] * (T)ih.z;
+            cb = (T)bz[id - L.sxy] * (T)ih.z;
         }
     }
     sum_c = ce_ + cw + cn + cs;
@@ -206,9 +206,9 @@ __device__ __forceinline__ void stencil(const T *__restrict__ e, const T *__rest
 // gas (rho ~ 1e-3) c_nb is ~1000x larger than in the liquid, and the form
 // sum c e_nb - (sum c) e_P would cancel terms of size c |e| and put the
 // round-off floor of the residual above 1e-10 relative on fine grids.
-template <typename T, int DIM, bool FINE>
-__device__ __forceinline__ T apply_A(const T *__restrict__ e, const T *__restrict__ rho, const T *__restrict__ bx,
-                                     const T *__restrict__ by, const T *__restrict__ bz, const Lay &L, Inv2 ih,
+template <typename T, int DIM, bool FINE, typename CT = T>
+__device__ __forceinline__ T apply_A(const T *__restrict__ e, const T *__restrict__ rho, const CT *__restrict__ bx,
+                                     const CT *__restrict__ by, const CT *__restrict__ bz, const Lay &L, Inv2 ih,
                                      int i, int j, int k, long id) {
     const T ep = e[id];
     T acc;
@@ -223,10 +223,10 @@ __device__ __forceinline__ T apply_A(const T *__restrict__ e, const T *__restric
             acc += face<T>(k > 0, r, rho[id - L.sxy], (T)ih.z) * (e[id - L.sxy] - ep);
         }
     } else {
-        acc = bx[id] * (T)ih.x * (e[id + 1] - ep) + bx[id - 1] * (T)ih.x * (e[id - 1] - ep) +
-              by[id] * (T)ih.y * (e[id + L.sx] - ep) + by[id - L.sx] * (T)ih.y * (e[id - L.sx] - ep);
+        acc = (T)bx[id] * (T)ih.x * (e[id + 1] - ep) + (T)bx[id - 1] * (T)ih.x * (e[id - 1] - ep) +
+              (T)by[id] * (T)ih.y * (e[id + L.sx] - ep) + (T)by[id - L.sx] * (T)ih.y * (e[id - L.sx] - ep);
         if (DIM == 3)
-            acc += bz[id] * (T)ih.z * (e[id + L.sxy] - ep) + bz[id - L.sxy] * (T)ih.z * (e[id - L.sxy] - ep);
+            acc += (T)bz[id] * (T)ih.z * (e[id + L.sxy] - ep) + (T)bz[id - L.sxy] * (T)ih.z * (e[id - L.sxy] - ep);
     }
     return acc;
 }
@@ -243,10 +243,10 @@ struct Dots {
 
 // one colour of a red-black Gauss-Seidel sweep: e_P = (sum c e_nb - f) / sum c.
 // Thread x covers two cells of a row; it updates the one whose colour matches.
-template <typename T, int DIM, bool FINE, int PROL, bool DOTS>
+template <typename T, int DIM, bool FINE, int PROL, bool DOTS, typename CT = T>
 __global__ void __launch_bounds__(256) k_smooth(T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ rho,
-                                                const T *__restrict__ bx, const T *__restrict__ by,
-                                                const T *__restrict__ bz, Lay L, Inv2 ih, int color, Prol<T> P,
+                                                const CT *__restrict__ bx, const CT *__restrict__ by,
+                                                const CT *__restrict__ bz, Lay L, Inv2 ih, int color, Prol<T> P,
                                                 Dots<T> D) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(256) k_smooth(T *__restrict__ e, const T *__re
     if (upd) {
         const long id = L.id(i, j, k);
         T sc, sce, ep;
-        stencil<T, DIM, FINE, PROL>(e, rho, bx, by, bz, L, ih, i, j, k, id, P, sc, sce, ep);
+        stencil<T, DIM, FINE, PROL, CT>(e, rho, bx, by, bz, L, ih, i, j, k, id, P, sc, sce, ep);
         // correction form e_P += (sum c (e_nb - e_P) - f) / sum c: the Gauss-Seidel
         // update without cancelling terms of size c |e|.  Formed for every
         // cell and selected (sc = 0 only off the grid), so that no branch
@@ -555,10 +555,10 @@ __global__ void __launch_bounds__(FUSED_THREADS) k_smooth_fused(const float *__r
 // C2: the factors are 2 per axis (1 along the cases of a 2D stack), known at
 // compile time, so the children's loads are issued together rather than one
 // child per memory round trip; same summation order as the runtime form.
-template <typename T, int DIM, bool FINE, bool C2 = false>
+template <typename T, int DIM, bool FINE, bool C2 = false, typename CT = T>
 __global__ void __launch_bounds__(256) k_resid_restrict(const T *__restrict__ e, const T *__restrict__ f,
-                                                        const T *__restrict__ rho, const T *__restrict__ bx,
-                                                        const T *__restrict__ by, const T *__restrict__ bz, Lay L,
+                                                        const T *__restrict__ rho, const CT *__restrict__ bx,
+                                                        const CT *__restrict__ by, const CT *__restrict__ bz, Lay L,
                                                         Inv2 ih, T *__restrict__ fc, T *__restrict__ ec, Lay C,
                                                         int cx, int cy, int cz) {
     int bx_, by_, bz_;
@@ -575,7 +575,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const T *__restrict__ e,
             for (int ii = 0; ii < (C2 ? 2 : cx); ++ii) {
                 int i = I * cx + ii, j = J * cy + jj, k = K * cz + kk;
                 long id = L.id(i, j, k);
-                acc += f[id] - apply_A<T, DIM, FINE>(e, rho, bx, by, bz, L, ih, i, j, k, id);
+                acc += f[id] - apply_A<T, DIM, FINE, CT>(e, rho, bx, by, bz, L, ih, i, j, k, id);
             }
     const long cid = C.id(I, J, K);
     fc[cid] = acc / (T)(cx * cy * cz);
@@ -654,6 +654,11 @@ __global__ void k_coef_restrict(const double *__restrict__ rho, const double *__
 __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, long n) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) b[t] = (float)a[t];
+}
+
+__global__ void k_to_f16(const double *__restrict__ a, _Float16 *__restrict__ b, long n) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = (_Float16)a[t];
 }
 
 // Fixed-order two-stage sum of the block partials of a level-0 launch, per
@@ -1127,7 +1132,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
             if (((i + j + (DIM == 3 ? k : 0)) & 1) != color) continue;   // 2D: k is the case
             const long id = L.id(i, j, k);
             T sc, sce, ep;
-            stencil<T, DIM, false, 0>(V.e, nullptr, V.bx, V.by, V.bz, L, V.ih, i, j, k, id, P0, sc, sce, ep);
+            stencil<T, DIM, false, 0, T>(V.e, nullptr, V.bx, V.by, V.bz, L, V.ih, i, j, k, id, P0, sc, sce, ep);
             V.e[id] = sc > (T)0 ? ep + gs_step<T>(sce - V.f[id], sc) : ep;
         }
         __syncthreads();
@@ -1268,7 +1273,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
                     for (int ii = 0; ii < F.cx; ++ii) {
                         const int i = I * F.cx + ii, j = J * F.cy + jj, k = K * F.cz + kk;
                         const long id = F.L.id(i, j, k);
-                        acc += F.f[id] - apply_A<T, DIM, false>(F.e, nullptr, F.bx, F.by, F.bz, F.L, F.ih, i, j, k, id);
+                        acc += F.f[id] - apply_A<T, DIM, false, T>(F.e, nullptr, F.bx, F.by, F.bz, F.L, F.ih, i, j, k, id);
                     }
             const long cid = C.L.id(I, J, K);
             C.f[cid] = acc / (T)nch;
@@ -1350,7 +1355,7 @@ static Inv2 inv2(double hx, double hy, double hz) {
 int mg_free(fcfd_ctx *c) {
     for (auto &M : c->mg) {
         void *ptrs[] = {M.e64, M.f64, M.r64, M.bx64, M.by64, M.bz64, M.e32, M.f32, M.r32, M.bx32, M.by32, M.bz32,
-                        M.e32b};
+                        M.e32b, M.bx16, M.by16, M.bz16};
         for (void *p : ptrs)
             if (p) hipFree(p);
     }
@@ -1435,6 +1440,9 @@ static int mg_build(fcfd_ctx *c) {
         FCFD_TRY(alloc0(&M.e32, n)); FCFD_TRY(alloc0(&M.f32, n)); FCFD_TRY(alloc0(&M.e32b, n));
         FCFD_TRY(alloc0(&M.bx32, n)); FCFD_TRY(alloc0(&M.by32, n));
         if (G.dim == 3) FCFD_TRY(alloc0(&M.bz32, n));
+        if (G.dim == 3 && l == 1) {   // FP16 coefficients of level 1 (ppe_rho_half)
+            FCFD_TRY(alloc0(&M.bx16, n)); FCFD_TRY(alloc0(&M.by16, n)); FCFD_TRY(alloc0(&M.bz16, n));
+        }
     }
     MgLevel &M0 = c->mg[0];
     const long nsplit = 2 * split_of(G).half;
@@ -1493,6 +1501,11 @@ int mg_setup(fcfd_ctx *c) {
         hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, c->stream, C.bx64, C.bx32, n);
         hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, c->stream, C.by64, C.by32, n);
         if (G.dim == 3) hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, c->stream, C.bz64, C.bz32, n);
+        if (C.bx16) {
+            hipLaunchKernelGGL(k_to_f16, dim3(nb), dim3(256), 0, c->stream, C.bx64, C.bx16, n);
+            hipLaunchKernelGGL(k_to_f16, dim3(nb), dim3(256), 0, c->stream, C.by64, C.by16, n);
+            hipLaunchKernelGGL(k_to_f16, dim3(nb), dim3(256), 0, c->stream, C.bz64, C.bz16, n);
+        }
     }
     hipLaunchKernelGGL(k_rho_split, grd3(G.nx, G.ny, G.nz, blk), blk, 0, c->stream, c->f[FCFD_RHO], lay_of(G),
                        c->rho32, c->rho16, split_of(G));
@@ -1627,9 +1640,25 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
                         else FCFD_SMOOTH(true, 0, false);
                     }
                 } else {
-                    if (pm == 1) FCFD_SMOOTH(false, 1, false);
-                    else if (pm == 2) FCFD_SMOOTH(false, 2, false);
-                    else FCFD_SMOOTH(false, 0, false);
+                    bool done16 = false;
+                    if constexpr (sizeof(T) == 4) {
+                        if (half_rho && M.bx16) {   // FP16 face coefficients (level 1 in 3D)
+                            const _Float16 *hx = M.bx16, *hy = M.by16, *hz = M.bz16;
+#define FCFD_SMOOTH16(PM)                                                                                      \
+    hipLaunchKernelGGL((k_smooth<T, DIM, false, PM, false, _Float16>), g, blk, 0, c->stream, e, f,            \
+                       (const T *)nullptr, hx, hy, hz, L, ih, color, P0, D0)
+                            if (pm == 1) FCFD_SMOOTH16(1);
+                            else if (pm == 2) FCFD_SMOOTH16(2);
+                            else FCFD_SMOOTH16(0);
+#undef FCFD_SMOOTH16
+                            done16 = true;
+                        }
+                    }
+                    if (!done16) {
+                        if (pm == 1) FCFD_SMOOTH(false, 1, false);
+                        else if (pm == 2) FCFD_SMOOTH(false, 2, false);
+                        else FCFD_SMOOTH(false, 0, false);
+                    }
                 }
 #undef FCFD_SMOOTH
             }
@@ -1709,6 +1738,18 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
                 else
                     hipLaunchKernelGGL((k_resid_restrict<T, DIM, true, false>), g, blk, 0, c->stream, e, f, rho0, bx,
                                        by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+            }
+        } else if (sizeof(T) == 4 && half_rho && M.bx16) {
+            // the FP16 face coefficients the level's smoothing reads
+            if constexpr (sizeof(T) == 4) {
+                if (c2)
+                    hipLaunchKernelGGL((k_resid_restrict<T, DIM, false, true, _Float16>), g, blk, 0, c->stream, e,
+                                       f, (const T *)nullptr, (const _Float16 *)M.bx16, (const _Float16 *)M.by16,
+                                       (const _Float16 *)M.bz16, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+                else
+                    hipLaunchKernelGGL((k_resid_restrict<T, DIM, false, false, _Float16>), g, blk, 0, c->stream, e,
+                                       f, (const T *)nullptr, (const _Float16 *)M.bx16, (const _Float16 *)M.by16,
+                                       (const _Float16 *)M.bz16, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
             }
         } else if (c2) {
             hipLaunchKernelGGL((k_resid_restrict<T, DIM, false, true>), g, blk, 0, c->stream, e, f,
@@ -1799,7 +1840,7 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
     double rr = 0.0, ff = 0.0;
     if (i < L.nx && j < L.ny) {
         long id = L.id(i, j, k);
-        double r = f[id] - apply_A<double, DIM, true>(p, rho, nullptr, nullptr, nullptr, L, ih, i, j, k, id);
+        double r = f[id] - apply_A<double, DIM, true, double>(p, rho, nullptr, nullptr, nullptr, L, ih, i, j, k, id);
         rr = r * r;
         if (with_f) ff = f[id] * f[id];
         if (r64) r64[id] = r;
