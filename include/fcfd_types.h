@@ -74,7 +74,8 @@ typedef struct fcfd_params {
                                     one round, else one), 1 one per launch,
                                     2 two per launch, 3 one colour per launch */
     int32_t ppe_rho_half;        /* 1: the FP32 level-0 smoother and restriction
-                                    read the density in FP16 (mixed mode) */
+                                    read the density in FP16, and in 3D those of
+                                    level 1 its face coefficients (mixed mode) */
 } fcfd_params;
 
 /* Level-set advection source of level_set_rk3_solver: the reference's WENO5
