@@ -419,7 +419,7 @@ __device__ __forceinline__ float apply_A_split(const float *__restrict__ e, cons
 }
 
 // k_resid_restrict for the colour-split FP32 level 0 (C2 as there)
-template <int DIM, typename RT, bool C2 = false>
+template <int DIM, typename RT, int C2 = 0>
 __global__ void __launch_bounds__(256) k_resid_restrict_split(const float *__restrict__ e, const float *__restrict__ f,
                                                               const RT *__restrict__ rho, Split S, Inv2 ih,
                                                               float *__restrict__ fc, float *__restrict__ ec, Lay C,
@@ -428,10 +428,10 @@ __global__ void __launch_bounds__(256) k_resid_restrict_split(const float *__res
     xcd_block(bx_, by_, bz_);
     const int I = bx_ * blockDim.x + threadIdx.x, J = by_ * blockDim.y + threadIdx.y, K = bz_;
     if (I >= C.nx || J >= C.ny) return;
-    if (C2) { cx = 2; cy = 2; cz = DIM == 3 ? 2 : 1; }
+    if (C2) { cx = 2; cy = 2; cz = C2; }
     float acc = 0.f;
 #pragma unroll
-    for (int kk = 0; kk < (C2 ? (DIM == 3 ? 2 : 1) : cz); ++kk)
+    for (int kk = 0; kk < (C2 ? C2 : cz); ++kk)
 #pragma unroll
         for (int jj = 0; jj < (C2 ? 2 : cy); ++jj)
 #pragma unroll
@@ -552,10 +552,11 @@ __global__ void __launch_bounds__(FUSED_THREADS) k_smooth_fused(const float *__r
 
 // residual of the level, restricted to the coarse right-hand side by the
 // mean over the children; the coarse error is zeroed in the same pass.
-// C2: the factors are 2 per axis (1 along the cases of a 2D stack), known at
-// compile time, so the children's loads are issued together rather than one
-// child per memory round trip; same summation order as the runtime form.
-template <typename T, int DIM, bool FINE, bool C2 = false, typename CT = T>
+// C2 (1 or 2): the factors are 2 in x and y and C2 in z (1 along the cases
+// of a 2D stack), known at compile time, so the children's loads are issued
+// together rather than one child per memory round trip; same summation order
+// as the runtime form (C2 = 0).
+template <typename T, int DIM, bool FINE, int C2 = 0, typename CT = T>
 __global__ void __launch_bounds__(256) k_resid_restrict(const T *__restrict__ e, const T *__restrict__ f,
                                                         const T *__restrict__ rho, const CT *__restrict__ bx,
                                                         const CT *__restrict__ by, const CT *__restrict__ bz, Lay L,
@@ -565,10 +566,10 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const T *__restrict__ e,
     xcd_block(bx_, by_, bz_);
     const int I = bx_ * blockDim.x + threadIdx.x, J = by_ * blockDim.y + threadIdx.y, K = bz_;
     if (I >= C.nx || J >= C.ny) return;
-    if (C2) { cx = 2; cy = 2; cz = DIM == 3 ? 2 : 1; }
+    if (C2) { cx = 2; cy = 2; cz = C2; }
     T acc = 0;
 #pragma unroll
-    for (int kk = 0; kk < (C2 ? (DIM == 3 ? 2 : 1) : cz); ++kk)
+    for (int kk = 0; kk < (C2 ? C2 : cz); ++kk)
 #pragma unroll
         for (int jj = 0; jj < (C2 ? 2 : cy); ++jj)
 #pragma unroll
@@ -1710,54 +1711,47 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         dim3 g = grd3(C.nx, C.ny, C.nz, blk);
         ProfScope ps_(c, lv == 0 ? R_RESTRICT0 : R_RESTRICTC);
         const int mz = DIM == 3 ? M.cz : 1;
-        const bool c2 = M.cx == 2 && M.cy == 2 && mz == (DIM == 3 ? 2 : 1);
+        const int c2 = M.cx == 2 && M.cy == 2 && (mz == 1 || mz == 2) ? mz : 0;   // compile-time children
+        // the kernel instances by compile-time children (0: runtime factors)
+#define FCFD_RR3(KERNEL_0, KERNEL_1, KERNEL_2, ...)                                                          \
+    do {                                                                                                       \
+        if (c2 == 2) hipLaunchKernelGGL(KERNEL_2, g, blk, 0, c->stream, __VA_ARGS__);                           \
+        else if (c2 == 1) hipLaunchKernelGGL(KERNEL_1, g, blk, 0, c->stream, __VA_ARGS__);                      \
+        else hipLaunchKernelGGL(KERNEL_0, g, blk, 0, c->stream, __VA_ARGS__);                                   \
+    } while (0)
         if (lv == 0) {
             if constexpr (sizeof(T) == 4) {
                 const Split S0 = split_of(G);
                 if (half_rho) {
                     const _Float16 *r16 = (const _Float16 *)c->rho16;
-                    if (c2)
-                        hipLaunchKernelGGL((k_resid_restrict_split<DIM, _Float16, true>), g, blk, 0, c->stream, e, f,
-                                           r16, S0, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
-                    else
-                        hipLaunchKernelGGL((k_resid_restrict_split<DIM, _Float16, false>), g, blk, 0, c->stream, e,
-                                           f, r16, S0, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+                    FCFD_RR3((k_resid_restrict_split<DIM, _Float16, 0>), (k_resid_restrict_split<DIM, _Float16, 1>),
+                             (k_resid_restrict_split<DIM, _Float16, 2>), e, f, r16, S0, ih, (T *)fc_, ec, LC, M.cx,
+                             M.cy, mz);
                 } else {
                     const float *r32 = (const float *)rho0;
-                    if (c2)
-                        hipLaunchKernelGGL((k_resid_restrict_split<DIM, float, true>), g, blk, 0, c->stream, e, f,
-                                           r32, S0, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
-                    else
-                        hipLaunchKernelGGL((k_resid_restrict_split<DIM, float, false>), g, blk, 0, c->stream, e, f,
-                                           r32, S0, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+                    FCFD_RR3((k_resid_restrict_split<DIM, float, 0>), (k_resid_restrict_split<DIM, float, 1>),
+                             (k_resid_restrict_split<DIM, float, 2>), e, f, r32, S0, ih, (T *)fc_, ec, LC, M.cx,
+                             M.cy, mz);
                 }
             } else {
-                if (c2)
-                    hipLaunchKernelGGL((k_resid_restrict<T, DIM, true, true>), g, blk, 0, c->stream, e, f, rho0, bx,
-                                       by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
-                else
-                    hipLaunchKernelGGL((k_resid_restrict<T, DIM, true, false>), g, blk, 0, c->stream, e, f, rho0, bx,
-                                       by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+                FCFD_RR3((k_resid_restrict<T, DIM, true, 0>), (k_resid_restrict<T, DIM, true, 1>),
+                         (k_resid_restrict<T, DIM, true, 2>), e, f, rho0, bx, by, bz, L, ih, (T *)fc_, ec, LC, M.cx,
+                         M.cy, mz);
             }
         } else if (sizeof(T) == 4 && half_rho && M.bx16) {
             // the FP16 face coefficients the level's smoothing reads
             if constexpr (sizeof(T) == 4) {
-                if (c2)
-                    hipLaunchKernelGGL((k_resid_restrict<T, DIM, false, true, _Float16>), g, blk, 0, c->stream, e,
-                                       f, (const T *)nullptr, (const _Float16 *)M.bx16, (const _Float16 *)M.by16,
-                                       (const _Float16 *)M.bz16, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
-                else
-                    hipLaunchKernelGGL((k_resid_restrict<T, DIM, false, false, _Float16>), g, blk, 0, c->stream, e,
-                                       f, (const T *)nullptr, (const _Float16 *)M.bx16, (const _Float16 *)M.by16,
-                                       (const _Float16 *)M.bz16, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+                const _Float16 *hx = M.bx16, *hy = M.by16, *hz = M.bz16;
+                FCFD_RR3((k_resid_restrict<T, DIM, false, 0, _Float16>), (k_resid_restrict<T, DIM, false, 1, _Float16>),
+                         (k_resid_restrict<T, DIM, false, 2, _Float16>), e, f, (const T *)nullptr, hx, hy, hz, L, ih,
+                         (T *)fc_, ec, LC, M.cx, M.cy, mz);
             }
-        } else if (c2) {
-            hipLaunchKernelGGL((k_resid_restrict<T, DIM, false, true>), g, blk, 0, c->stream, e, f,
-                               (const T *)nullptr, bx, by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
         } else {
-            hipLaunchKernelGGL((k_resid_restrict<T, DIM, false, false>), g, blk, 0, c->stream, e, f,
-                               (const T *)nullptr, bx, by, bz, L, ih, (T *)fc_, ec, LC, M.cx, M.cy, mz);
+            FCFD_RR3((k_resid_restrict<T, DIM, false, 0>), (k_resid_restrict<T, DIM, false, 1>),
+                     (k_resid_restrict<T, DIM, false, 2>), e, f, (const T *)nullptr, bx, by, bz, L, ih, (T *)fc_, ec,
+                     LC, M.cx, M.cy, mz);
         }
+#undef FCFD_RR3
         FCFD_CHECK(hipGetLastError());
     }
     if (lt < nl) {
