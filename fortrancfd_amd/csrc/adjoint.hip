@@ -1,5 +1,5 @@
 // adjoint.hip — discrete adjoint of the WENO5 flux-split level-set advection
-// with SSP-RK3 (the forward is k_ls_adv_stage in levelset.hip, following
+// with SSP-RK3 (the forward is k_ls_adv_stage_kf in levelset.hip, following
 // `Two phase flows/newOpenMP/3D+IBM/src/interface_solver/
 // level_set_rk3_solver.f90:1-219`).  There is no counterpart in the
 // reference: this is the differentiation entry point of the path.

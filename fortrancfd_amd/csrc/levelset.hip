@@ -252,76 +252,24 @@ int mpls(fcfd_ctx *c) {
 }
 
 // ------------------------------------------------------------- advection
+constexpr int LS_KC = 16;   // planes per block of the 3D stage
+
 // One SSP-RK3 stage of level_set_rk3_solver, fused: velocity flux split
 // (source_setup :67-96, with the bc() copy of s1/s2 realised by clamped
 // indices), WENO5-JS faces (:162-219), flux divergence (:145-158) and the
-// increment-form stage update (:13-60).
+// increment-form stage update (:13-60), with every face computed once.
 // T = float is the single-precision variant (same operation order, FP32
 // weights of weno.h); the velocity is vs (nu, nv, nw).
-template <int DIM, typename T>
-__global__ void __launch_bounds__(256) k_ls_adv_stage(const T *__restrict__ phi, T *__restrict__ phin,
-                                                      const T *__restrict__ nu, const T *__restrict__ nv,
-                                                      const T *__restrict__ nw, T *__restrict__ l1,
-                                                      T *__restrict__ l2, Geom G, T dt, T vs, int stage) {
-    int bx_, by_, bz_;
-    xcd_block(bx_, by_, bz_);
-    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
-    if (i >= G.nx || j >= G.ny) return;
-    const long id = G.id(i, j, k);
-    const T *vel[3] = {nu, nv, nw};
-    const int n[3] = {G.nx, G.ny, G.nz};
-    const int pos[3] = {i, j, k};
-    const long st[3] = {1, G.sx, G.sxy};
-    const T h[3] = {(T)G.dx, (T)G.dy, (T)G.dz};
-    const T half = (T)0.5;
-    T L = 0;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        T s1[7], s2[7];  // positions pos-3 .. pos+3
-#pragma unroll
-        for (int o = -3; o <= 3; ++o) {
-            int q = clampi(pos[d] + o, 0, n[d] - 1);
-            long a = id + (long)(q - pos[d]) * st[d];
-            T u = vs * vel[d][a], p = phi[a];
-            s1[o + 3] = half * (u + fabs(u)) * p;
-            s2[o + 3] = half * (u - fabs(u)) * p;
-        }
-        const T EPS = (T)1.0e-13;
-        // face i+1/2: gm from s1(i-2..i+2), fp from s2(i-1..i+3)
-        T gmR = weno_left(s1[1], s1[2], s1[3], s1[4], s1[5], EPS);
-        T fpR = weno_right(s2[2], s2[3], s2[4], s2[5], s2[6], EPS);
-        // face i-1/2
-        T gmL = weno_left(s1[0], s1[1], s1[2], s1[3], s1[4], EPS);
-        T fpL = weno_right(s2[1], s2[2], s2[3], s2[4], s2[5], EPS);
-        T t = -(gmR + fpR - gmL - fpL) / h[d];
-        L = d == 0 ? t : L + t;
-    }
-    T src;
-    if (stage == 0) {
-        l1[id] = L;
-        src = L;
-    } else if (stage == 1) {
-        l2[id] = L;
-        src = ((T)-3.0 * l1[id] + L) / (T)4.0;
-    } else {
-        src = (-l1[id] - l2[id] + (T)8.0 * L) / (T)12.0;
-    }
-    phin[id] = phi[id] + src * dt;
-}
-
-constexpr int LS_KC = 16;   // planes per block of the 3D stage
-
-// 3D form of k_ls_adv_stage with every face computed once.  Each block walks
-// LS_KC planes of a 64 x 4 column tile.  Per plane, its 256 threads compute
-// the 4 x 65 x-faces and 5 x 64 y-faces of the tile into LDS (580 faces
-// where two faces per cell and direction take 1024), then each cell
+// A block covers a 64 x 4 tile and, in 3D, walks LS_KC planes of it (2D:
+// the block's slice, a case of a batched stack).  Per plane its 256 threads
+// compute the 4 x 65 x-faces and 5 x 64 y-faces of the tile into LDS (580
+// faces where two faces per cell and direction take 1024), then each cell
 // differences them.  The z flux of face k+1/2 is the k-1/2 flux of the next
 // plane, from a 7-plane window of (w, phi) splits carried in registers.  The
 // LDS plane is double buffered, one barrier per plane.  Face value F =
 // WENO_left(s1 at f-2..f+2) + WENO_right(s2 at f-1..f+3) at clamped
-// positions; L sums F(+1/2) - F(-1/2) per direction, so results differ from
-// the one-plane kernel only by round-off.
-template <typename T>
+// positions; L sums F(+1/2) - F(-1/2) per direction.
+template <int DIM, typename T>
 __global__ void __launch_bounds__(256) k_ls_adv_stage_kf(const T *__restrict__ phi, T *__restrict__ phin,
                                                          const T *__restrict__ nu, const T *__restrict__ nv,
                                                          const T *__restrict__ nw, T *__restrict__ l1,
@@ -332,7 +280,8 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage_kf(const T *__restrict__ p
     xcd_block(bx_, by_, bz_);
     const int tx = threadIdx.x, ty = threadIdx.y, t = ty * TX + tx;
     const int i0 = bx_ * TX, j0 = by_ * TY, i = i0 + tx, j = j0 + ty;
-    const int k0 = bz_ * LS_KC, k1 = min(k0 + LS_KC, G.nz);
+    // 3D: LS_KC planes per block; 2D: the block's slice (a case of a batched stack)
+    const int k0 = DIM == 3 ? bz_ * LS_KC : bz_, k1 = DIM == 3 ? min(k0 + LS_KC, G.nz) : bz_ + 1;
     const bool in = i < G.nx && j < G.ny;
     const int ic = min(i, G.nx - 1), jc = min(j, G.ny - 1);
     const T half = (T)0.5, EPS = (T)1.0e-13;
@@ -350,16 +299,18 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage_kf(const T *__restrict__ p
         return weno_left(s1[0], s1[1], s1[2], s1[3], s1[4], EPS) + weno_right(s2[1], s2[2], s2[3], s2[4], s2[5], EPS);
     };
     // z window of this column: positions clamp(k - 3 + m), m = 0..6
-    T s1z[7], s2z[7];
+    T s1z[7], s2z[7], Fl = 0;
+    if (DIM == 3) {
 #pragma unroll
-    for (int m = 0; m < 7; ++m) {
-        const long a = G.id(ic, jc, clampi(k0 - 3 + m, 0, G.nz - 1));
-        const T u = vs * nw[a], p = phi[a];
-        s1z[m] = half * (u + fabs(u)) * p;
-        s2z[m] = half * (u - fabs(u)) * p;
+        for (int m = 0; m < 7; ++m) {
+            const long a = G.id(ic, jc, clampi(k0 - 3 + m, 0, G.nz - 1));
+            const T u = vs * nw[a], p = phi[a];
+            s1z[m] = half * (u + fabs(u)) * p;
+            s2z[m] = half * (u - fabs(u)) * p;
+        }
+        Fl = weno_left(s1z[0], s1z[1], s1z[2], s1z[3], s1z[4], EPS) +
+             weno_right(s2z[1], s2z[2], s2z[3], s2z[4], s2z[5], EPS);
     }
-    T Fl = weno_left(s1z[0], s1z[1], s1z[2], s1z[3], s1z[4], EPS) +
-           weno_right(s2z[1], s2z[2], s2z[3], s2z[4], s2z[5], EPS);
     for (int k = k0; k < k1; ++k) {
         const int b = k & 1;
         for (int e = t; e < NFX + NFY; e += TX * TY) {
@@ -373,13 +324,15 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage_kf(const T *__restrict__ p
         }
         __syncthreads();
         // face k + 1/2: gm from s1(k-2..k+2), fp from s2(k-1..k+3)
-        const T Fr = weno_left(s1z[1], s1z[2], s1z[3], s1z[4], s1z[5], EPS) +
-                     weno_right(s2z[2], s2z[3], s2z[4], s2z[5], s2z[6], EPS);
+        T Fr = 0;
+        if (DIM == 3)
+            Fr = weno_left(s1z[1], s1z[2], s1z[3], s1z[4], s1z[5], EPS) +
+                 weno_right(s2z[2], s2z[3], s2z[4], s2z[5], s2z[6], EPS);
         if (in) {
             const long id = G.id(i, j, k);
             T L = -(fxs[b][ty][tx + 1] - fxs[b][ty][tx]) / hx;
             L = L + (-(fys[b][ty + 1][tx] - fys[b][ty][tx]) / hy);
-            L = L + (-(Fr - Fl) / hz);
+            if (DIM == 3) L = L + (-(Fr - Fl) / hz);
             T src;
             if (stage == 0) {
                 l1[id] = L;
@@ -392,17 +345,19 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage_kf(const T *__restrict__ p
             }
             phin[id] = phi[id] + src * dt;
         }
-        Fl = Fr;
-        // slide the z window to plane k + 1
+        if (DIM == 3) {
+            Fl = Fr;
+            // slide the z window to plane k + 1
 #pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            s1z[m] = s1z[m + 1];
-            s2z[m] = s2z[m + 1];
+            for (int m = 0; m < 6; ++m) {
+                s1z[m] = s1z[m + 1];
+                s2z[m] = s2z[m + 1];
+            }
+            const long a = G.id(ic, jc, clampi(k + 4, 0, G.nz - 1));
+            const T u = vs * nw[a], p = phi[a];
+            s1z[6] = half * (u + fabs(u)) * p;
+            s2z[6] = half * (u - fabs(u)) * p;
         }
-        const long a = G.id(ic, jc, clampi(k + 4, 0, G.nz - 1));
-        const T u = vs * nw[a], p = phi[a];
-        s1z[6] = half * (u + fabs(u)) * p;
-        s2z[6] = half * (u - fabs(u)) * p;
     }
 }
 
@@ -448,10 +403,10 @@ int ls_adv_stage(fcfd_ctx *c, const double *phi, double *phin, int stage) {
     const Geom &G = c->G;
     dim3 blk(64, 4), grd = grid_interior(G, blk);
     if (G.dim == 3)
-        hipLaunchKernelGGL((k_ls_adv_stage_kf<double>), grid_km(G, blk), blk, 0, c->stream, phi, phin,
+        hipLaunchKernelGGL((k_ls_adv_stage_kf<3, double>), grid_km(G, blk), blk, 0, c->stream, phi, phin,
                            c->f[FCFD_NU], c->f[FCFD_NV], c->f[FCFD_NW], c->l1, c->l2, G, c->pr.dt, c->vscale, stage);
     else
-        hipLaunchKernelGGL((k_ls_adv_stage<2, double>), grd, blk, 0, c->stream, phi, phin, c->f[FCFD_NU],
+        hipLaunchKernelGGL((k_ls_adv_stage_kf<2, double>), grd, blk, 0, c->stream, phi, phin, c->f[FCFD_NU],
                            c->f[FCFD_NV], nullptr, c->l1, c->l2, G, c->pr.dt, c->vscale, stage);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
@@ -492,11 +447,11 @@ int ls_advect(fcfd_ctx *c) {
         {
             ProfScope ps_(c, R_LS_ADV);
             if (G.dim == 3)
-                hipLaunchKernelGGL((k_ls_adv_stage_kf<double>), grid_km(G, blk), blk, 0, c->stream, a, b,
+                hipLaunchKernelGGL((k_ls_adv_stage_kf<3, double>), grid_km(G, blk), blk, 0, c->stream, a, b,
                                    c->f[FCFD_NU], c->f[FCFD_NV], c->f[FCFD_NW], c->l1, c->l2, G, c->pr.dt, c->vscale,
                                    s);
             else
-                hipLaunchKernelGGL((k_ls_adv_stage<2, double>), grd, blk, 0, c->stream, a, b, c->f[FCFD_NU],
+                hipLaunchKernelGGL((k_ls_adv_stage_kf<2, double>), grd, blk, 0, c->stream, a, b, c->f[FCFD_NU],
                                    c->f[FCFD_NV], nullptr, c->l1, c->l2, G, c->pr.dt, c->vscale, s);
             FCFD_CHECK(hipGetLastError());
         }
@@ -525,11 +480,11 @@ int ls_advect_f32(fcfd_ctx *c, float *phi, const float *nu, const float *nv, con
         {
             ProfScope ps_(c, R_LS_ADV);
             if (G.dim == 3)
-                hipLaunchKernelGGL((k_ls_adv_stage_kf<float>), grid_km(G, blk), blk, 0, c->stream, in[s], out[s], nu,
+                hipLaunchKernelGGL((k_ls_adv_stage_kf<3, float>), grid_km(G, blk), blk, 0, c->stream, in[s], out[s], nu,
                                    nv, nw, l1,
                                    l2, G, dt, vs, s);
             else
-                hipLaunchKernelGGL((k_ls_adv_stage<2, float>), grd, blk, 0, c->stream, in[s], out[s], nu, nv,
+                hipLaunchKernelGGL((k_ls_adv_stage_kf<2, float>), grd, blk, 0, c->stream, in[s], out[s], nu, nv,
                                    (const float *)nullptr, l1, l2, G, dt, vs, s);
             FCFD_CHECK(hipGetLastError());
         }
