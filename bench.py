@@ -78,6 +78,25 @@ def region_bytes(case, mixed: bool) -> dict:
     }
 
 
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (every lane an FMA each cycle)
+
+
+def region_flops(case) -> dict:
+    """Algorithmic FP64 flops per launch of the FP64-ALU-bound regions (a
+    divide, square root or sine counts as one flop).  One WENO5-JS
+    reconstruction as the kernels write it: three smoothness indicators
+    (~33), q_k = (EPS + beta_k)^2 (6), the product-form weights (5), their
+    normalisation (3 + 3) and the stencil combination (~25): ~75.
+    WENO stage: 3 faces per cell (each shared by two cells) x 2
+    reconstructions = 450, flux splits 3 x 7, face sums, divergence and the
+    increment-form update ~20: ~490 per cell.  Reinit pass A: 2 one-sided
+    reconstructions per direction = 450, 7 backward differences per
+    direction x 2 = 42, Godunov selection and |grad| ~15, smoothed
+    Heaviside / delta and the lambda terms ~20: ~530 per cell."""
+    N = case.nx * case.ny * case.nz
+    return {"ls_adv_stage": N * 490, "redis_pass_a": N * 530}
+
+
 def profiled_traffic(region: str):
     """HBM bytes per launch of a kernel region as measured by the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/*.txt, line
@@ -555,6 +574,15 @@ def main() -> int:
         roofline["traffic_source"] = "profiles/" + tsrc
     if dom_any != dom:
         roofline["costliest_region_profiled"] = dom_any
+    # the FP64-ALU-bound regions against the FP64 vector peak (their launch
+    # means include the reinit launches queued after its stop, a few per run)
+    alu = {}
+    for name, fl in region_flops(case).items():
+        if name in regions:
+            tf = fl / (regions[name]["ms_per_launch"] * 1e-3) / 1e12
+            alu[name] = {"bound": "fp64-valu", "achieved": round(tf, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tf / FP64_PEAK_TFLOPS, 4), "flops_per_launch": fl,
+                         "ms_per_launch": regions[name]["ms_per_launch"]}
     out = {
         "metric": "Mcell-steps/s @256^3 two-phase, %HBM peak; MG ms to 1e-10 mixed vs FP64",
         "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -564,6 +592,7 @@ def main() -> int:
                    "domain": [1.0, 1.0, 2.0], "ppe": ("fp32" if mixed else "fp64") + f" V({case.ppe_nu1},{case.ppe_nu2}) x{case.ppe_inner}" + (" preconditioning fp64 flexible CG" if args.krylov else (" in fp64 defect correction" if mixed else "")),
                    "ppe_tol": 1e-10, "picard": 4, "redis_interval": 5},
         "roofline": roofline,
+        "alu_roofline": alu,
         "step_hbm_frac_measured_kernels": round(tot_bytes / (tot_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms_per_step": round(tot_ms / args.steps, 3),
         "profiled_ms_per_step": round((tp1 - tp0) * 1e3 / args.steps, 3),
