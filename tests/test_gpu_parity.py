@@ -116,6 +116,23 @@ def test_reinitialisation(tag, mk, btn):
     _cmp(s, o, ["phi"], 1e-11)
 
 
+def test_reinitialisation_stops_on_its_test():
+    """At 48^3 a signed-distance sphere converges: the pseudo-time loop ends on
+    max |phi - phi_start| <= 1e-5 after 13 of its 31 steps.  The stop is
+    decided on the device and the host stops queueing steps once it has read
+    the stop flags, so the pseudo-step count and the field must equal the
+    oracle's (whose loop breaks there)."""
+    from fortrancfd_amd.types import bc_copy, bubble_phi
+    case = case3d(48)
+    phi = bc_copy(case, bubble_phi(case, 0.25, (0.5, 0.5, 0.5)))
+    s, o = _pair(case, {"phi": phi})
+    it = s.level_set_rk3_redis(1)
+    o.call("level_set_rk3_redis", 1)
+    ref = o.diag()["redis_iters"]
+    assert it == ref and it < 31, (it, ref)
+    _cmp(s, o, ["phi"], 1e-11)
+
+
 @pytest.mark.parametrize("tag,mk", CASES)
 def test_mass_preserving_correction(tag, mk):
     case = mk()
