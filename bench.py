@@ -33,11 +33,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, GB/s
 
 
-def rising_case(n: int, precision: int, tol: float):
-    from fortrancfd_amd.types import Case
-    return Case(dim=3, nx=n, ny=n, nz=n, xend=1.0, yend=1.0, zend=2.0, gvec=(0.0, 0.0, -1.0),
-                cfl=0.1, cfl_r=0.1, ppe_tol=tol, ppe_precision=precision, redis_interval=5,
-                picard_iters=4, ppe_max_cycles=100, ppe_rho_half=1, name="rising_bubble_3d")
+from fortrancfd_amd.workloads import (BATCH_SIZES, batch_cases, poisson_case, rising_case,  # noqa: E402
+                                     seeded_rhs, sphere_density)
 
 
 def region_bytes(case, mixed: bool) -> dict:
@@ -215,25 +212,10 @@ def mg_compare(s, case, variants=MG_VARIANTS) -> dict:
 
 def poisson_bench(n: int) -> dict:
     from fortrancfd_amd import Solver
-    from fortrancfd_amd.types import PPE_FP64, PPE_MIXED, Case
-    case = Case(dim=3, nx=n, ny=n, nz=n, xend=1.0, yend=1.0, zend=1.0, ppe_max_cycles=200, ppe_rho_half=1,
-                name="poisson")
+    case = poisson_case(n, ppe_rho_half=1)
     s = Solver(case, allow_unconverged=True)
-    X, Y, Z = case.coords()
-    r = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2)
-    eps = 1.5 * case.dx
-    x = (r - 0.3) / eps
-    H = np.where(x > 1, 1.0, np.where(x < -1, 0.0, 0.5 * (1 + x + np.sin(np.pi * x) / np.pi)))
-    rho = 1.0 + 999.0 * H
-    del X, Y, Z, r, x, H
-    s.push("rho", rho)
-    del rho
-    rng = np.random.default_rng(0x5EEDF00D)
-    f = rng.uniform(-1.0, 1.0, size=case.shape)
-    fi = case.interior(f)
-    fi -= fi.mean()
-    s.push("ppe_src", f)
-    del f
+    s.push("rho", sphere_density(case))
+    s.push("ppe_src", seeded_rhs(case))
     s.ppe_sor_init()
     res = mg_compare(s, case)
     return {"metric": f"MG ms to 1e-10, {n}^3 variable-coefficient Poisson", "value": res.get("mixed_ms"),
@@ -312,47 +294,6 @@ def adjoint_bench(n: int, nsteps: int = 300, every: int = 20, cfl: float = 0.3) 
             "fp32_forward_Mcell_steps_per_s": round(cells * nsteps / (t4 - t3) / 1e6, 1),
             "fp32_phiT_max_rel_dev": err32, "fp32_loss": J32,
             "fp32_loss_rel_dev": abs(J32 - float(J.detach())) / float(J.detach())}
-
-
-BATCH_SIZES = ((64, 128), (128, 256), (256, 512))
-
-
-def batch_cases(ncases: int = 64, seed: int = 0x5EEDF00D, precision: int = 1):
-    """Seeded batched rising-bubble cases on [0,1]x[0,2]: grid from
-    64x128 / 128x256 / 256x512, bubble r ~ U[0.15, 0.3] at (0.5, 0.5),
-    density ratio ~ U[10, 1000], Eotvos number ~ U[1, 100] with We = Eo
-    (L = D, Fr = 1), viscosity ratio 100, Re = 1e4, reinit every 5 steps,
-    PPE to 1e-10 (mixed: FP32 V-cycles in FP64 flexible CG).  The cases of
-    one grid share dt (one context): dt = min(0.1 dx, half the explicit
-    capillary limit sqrt(rho_eff We dx^3 / (2 pi)) of every case of that
-    grid), with rho_eff = (1 - rho_12) / ln(1 / rho_12) the delta-weighted
-    harmonic mean of the density across the smeared interface (the CSF
-    force is divided by the local density).  We down to 1 is far below the
-    CSF stability bound at dt = 0.1 dx; with the plain mean density as
-    rho_eff a case with density ratio 747 and We = 11.7 grew parasitic
-    currents to |u| ~ 10 and failed after ~160 steps."""
-    from fortrancfd_amd.types import Case
-    rng = np.random.default_rng(seed)
-    draw = []
-    for q in range(ncases):
-        nx, ny = BATCH_SIZES[int(rng.integers(0, len(BATCH_SIZES)))]
-        r = float(rng.uniform(0.15, 0.3))
-        ratio = float(rng.uniform(10.0, 1000.0))
-        eo = float(rng.uniform(1.0, 100.0))
-        draw.append((nx, ny, r, 1.0 / ratio, eo))
-    cfl = {}
-    for nx, ny, r, r12, we in draw:
-        dx = 1.0 / nx
-        rho_eff = (1.0 - r12) / np.log(1.0 / r12)
-        lim = 0.5 * np.sqrt(rho_eff * we * dx ** 3 / (2.0 * np.pi)) / dx
-        cfl[nx] = min(cfl.get(nx, 0.1), lim)
-    out = []
-    for q, (nx, ny, r, r12, we) in enumerate(draw):
-        c = Case(dim=2, nx=nx, ny=ny, nz=1, xend=1.0, yend=2.0, gvec=(0.0, -1.0, 0.0), rho_12=r12,
-                 mu_12=0.01, re=1.0e4, we=we, fr=1.0, cfl=float(cfl[nx]), cfl_r=0.1, redis_interval=5,
-                 ppe_tol=1e-10, ppe_precision=precision, name=f"batch{q}")
-        out.append((c, r))
-    return out
 
 
 def batch_bench(ncases: int = 64, steps: int = 500, warmup: int = 2, loop_steps: int = 5,

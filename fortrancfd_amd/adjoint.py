@@ -115,6 +115,30 @@ class _Run:
         self.s.set_velocity_scale(1.0)
 
 
+def _fold_ghosts(a, g: int):
+    """Adjoint of the zero-gradient ghost copy (bc): every ghost of the
+    ghosted array `a` (shaped (k, j, i) or (j, i)) holds the value of the
+    interior cell its indices clamp to, so its cotangent is added to that
+    cell; the ghosts are then zero.  Axis by axis, so corner and edge ghosts
+    reach their cell through the faces they are copied from."""
+    a = a.clone()
+    for ax in range(a.dim()):
+        n = a.shape[ax]
+        a.narrow(ax, g, 1).add_(a.narrow(ax, 0, g).sum(dim=ax, keepdim=True))
+        a.narrow(ax, n - g - 1, 1).add_(a.narrow(ax, n - g, g).sum(dim=ax, keepdim=True))
+        a.narrow(ax, 0, g).zero_()
+        a.narrow(ax, n - g, g).zero_()
+    return a
+
+
+def _zero_ghosts(a, g: int):
+    for ax in range(a.dim()):
+        n = a.shape[ax]
+        a.narrow(ax, 0, g).zero_()
+        a.narrow(ax, n - g, g).zero_()
+    return a
+
+
 def _function():
     import torch
 
@@ -143,7 +167,10 @@ def _function():
         def backward(ctx, g):
             run, a = ctx.run, ctx.A
             run.set_amplitude(a)
-            lam = g.detach().clone().contiguous()
+            case = run.s.case
+            # phi_T's ghosts are zero-gradient copies made by the last stage:
+            # their cotangents belong to the interior cells they copy
+            lam = _fold_ghosts(g.detach().reshape(case.shape), case.ghc).reshape(-1).contiguous()
             bars = [torch.zeros_like(u) for u in run.uhat]   # dJ/d(A uhat), summed over the steps
             torch.cuda.synchronize()
             nseg = len(ctx.ckpts)
@@ -165,6 +192,8 @@ def _function():
                 run.s.sync()   # the segment's states are released after this
             run.finish()
             gA = sum((b * u).sum() for b, u in zip(bars, run.uhat))
+            # the advection never reads phi0's ghosts
+            lam = _zero_ghosts(lam.reshape(case.shape), case.ghc).reshape(-1)
             return lam, gA.reshape(()).to(torch.float64), None
 
     return LevelSetAdvection

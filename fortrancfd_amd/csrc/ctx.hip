@@ -213,6 +213,33 @@ int upload_cases(fcfd_ctx *c) {
 }  // namespace fcfd
 
 // an ordinary context (ncases = 0) or a batched 2D stack of ncases cases
+extern "C" int fcfd_ppe_fp16_density_ok(double rho_min, double rho_max) {
+    // FP16's normal range is [6.104e-5, 65504].  Level 0 stores rho itself;
+    // level 1 (3D) stores averaged face coefficients 2/(rho_a + rho_b), which
+    // lie in [1/rho_max, 1/rho_min].  A small margin keeps the rounding of
+    // those averages inside the range too.
+    const double lo = 6.2e-5, hi = 6.5e4;
+    if (!(rho_min > 0.0) || !(rho_max >= rho_min)) return 0;
+    return rho_min >= lo && rho_max <= hi && 1.0 / rho_min <= hi && 1.0 / rho_max >= lo;
+}
+
+// params.ppe_rho_half is a request: the FP16 copies are only made when the
+// density of every case, between rho_12 and 1 (rho = H + rho_12 (1 - H)),
+// fits FP16 as fcfd_ppe_fp16_density_ok checks; otherwise the FP32 copies
+// are read and a note is printed once per context.
+static void guard_fp16(fcfd_ctx *c) {
+    if (!c->pr.ppe_rho_half) return;
+    double lo = std::min(1.0, c->pr.rho_12), hi = std::max(1.0, c->pr.rho_12);
+    for (const fcfd_case &q : c->cases) {
+        lo = std::min(lo, std::min(1.0, q.rho_12));
+        hi = std::max(hi, std::max(1.0, q.rho_12));
+    }
+    if (!fcfd_ppe_fp16_density_ok(lo, hi)) {
+        fprintf(stderr, "libfcfd: density range [%g, %g] exceeds FP16; ppe_rho_half ignored\n", lo, hi);
+        c->pr.ppe_rho_half = 0;
+    }
+}
+
 static int create_impl(const fcfd_grid *grid, const fcfd_params *params, int ncases, const fcfd_case *cases,
                        fcfd_ctx **out) {
     if (!grid || !params || !out) return FCFD_ERR_ARG;
@@ -228,6 +255,7 @@ static int create_impl(const fcfd_grid *grid, const fcfd_params *params, int nca
     c->pr = *params;
     c->nb = ncases > 0 ? ncases : 1;
     if (ncases > 0) c->cases.assign(cases, cases + ncases);
+    guard_fp16(c);
     Geom &G = c->G;
     G.dim = grid->dim;
     G.nx = grid->nx; G.ny = grid->ny; G.nz = ncases > 0 ? ncases : grid->nz;
@@ -272,9 +300,9 @@ static int create_impl(const fcfd_grid *grid, const fcfd_params *params, int nca
     if (rc == FCFD_OK) rc = alloc_field(&c->red.result, RES_STRIDE * nbc);
     if (rc == FCFD_OK && hipHostMalloc((void **)&c->red.host, 64 * nbc * sizeof(double)) != hipSuccess)
         rc = FCFD_ERR_HIP;
-    if (rc == FCFD_OK && hipHostMalloc((void **)&c->diag_host, 2 * nbc * sizeof(double)) != hipSuccess)
+    if (rc == FCFD_OK && hipHostMalloc((void **)&c->diag_host, DIAG_W * nbc * sizeof(double)) != hipSuccess)
         rc = FCFD_ERR_HIP;
-    if (rc == FCFD_OK) memset(c->diag_host, 0, 2 * nbc * sizeof(double));
+    if (rc == FCFD_OK) memset(c->diag_host, 0, DIAG_W * nbc * sizeof(double));
     if (rc == FCFD_OK && hipMalloc((void **)&c->pack, 16 * nbc * sizeof(unsigned long long)) != hipSuccess)
         rc = FCFD_ERR_HIP;
     if (rc == FCFD_OK && hipMalloc((void **)&c->flag, FLAG_STRIDE * nbc * sizeof(unsigned long long)) != hipSuccess)
@@ -317,6 +345,8 @@ extern "C" int fcfd_batch_size(const fcfd_ctx *c) { return c ? c->nb : 0; }
 extern "C" int fcfd_set_cases(fcfd_ctx *c, const fcfd_case *cases) {
     if (!c || !cases || c->nb == 1) return FCFD_ERR_ARG;
     c->cases.assign(cases, cases + c->nb);
+    guard_fp16(c);
+    FCFD_TRY(mg_drop_graphs(c));
     return upload_cases(c);
 }
 
@@ -360,6 +390,7 @@ extern "C" int fcfd_destroy(fcfd_ctx *c) {
 extern "C" int fcfd_set_params(fcfd_ctx *c, const fcfd_params *p) {
     if (!c || !p) return FCFD_ERR_ARG;
     c->pr = *p;
+    guard_fp16(c);
     c->diag.imass = p->imass;
     // captured V-cycles bake in the sweep counts: capture again on next use
     return mg_drop_graphs(c);
@@ -418,6 +449,9 @@ extern "C" int fcfd_get_diag(fcfd_ctx *c, fcfd_diag *out) {
     if (c->diag_pending) {
         c->diag.mass = c->diag_host[0];
         c->diag.vol = c->diag_host[1];
+        c->diag.mpls_lambda = c->diag_host[RES_LAMBDA];
+        c->diag.vel_div = c->diag_host[RES_VDIV];
+        c->diag.vel_sdiv = c->diag_host[RES_VDIV + 1];
         c->diag_pending = 0;
     }
     *out = c->diag;
@@ -430,8 +464,8 @@ extern "C" int fcfd_get_case_diag(fcfd_ctx *c, double *mass, double *vol, double
     if (!c) return FCFD_ERR_ARG;
     FCFD_CHECK(hipStreamSynchronize(c->stream));
     for (int q = 0; q < c->nb; ++q) {
-        if (mass) mass[q] = c->diag_host[2 * q];
-        if (vol) vol[q] = c->diag_host[2 * q + 1];
+        if (mass) mass[q] = c->diag_host[(long)q * DIAG_W];
+        if (vol) vol[q] = c->diag_host[(long)q * DIAG_W + 1];
         if (imass) imass[q] = c->nb > 1 ? c->cases[q].imass : c->pr.imass;
     }
     return FCFD_OK;

@@ -136,8 +136,8 @@ struct fcfd_ctx {
     // ---- bookkeeping ----
     double time;
     int iter;
-    int diag_pending;          // step mass/volume of every case waiting in diag_host
-    double *diag_host;         // pinned: mass, volume of case q at [2q], [2q+1]
+    int diag_pending;          // step diagnostics of every case waiting in diag_host
+    double *diag_host;         // pinned: result words 0..DIAG_W-1 of case q at [q*DIAG_W]
     unsigned long long *pack;  // device staging for per-case values gathered to the host (16 words per case)
     fcfd_diag diag;
     fcfd::Prof prof;
@@ -203,8 +203,11 @@ __device__ __forceinline__ double block_max256(double v, double *sh4) {
 // count, [3] max |grad phi| bits; [FLAG_SHARD0, +NSHARD) per-block max slots.
 // A batched stack holds one such block of FLAG_STRIDE words per case.
 constexpr int FLAG_SHARD0 = 16, NSHARD = 256, FLAG_STRIDE = FLAG_SHARD0 + NSHARD;
-// reduced values per case in Reduce::result (mass sums, MPLS lambda, diag)
-constexpr int RES_STRIDE = 16;
+// reduced values per case in Reduce::result: [0..2] mass sums, [RES_LAMBDA]
+// the last MPLS multiplier, [RES_VDIV], [RES_VDIV + 1] max / mean |div u|
+// (case 0 only); the first DIAG_W words of every case are gathered to the
+// host after each step
+constexpr int RES_STRIDE = 16, RES_LAMBDA = 8, RES_VDIV = 9, DIAG_W = 11;
 
 // case index of slice k: a batched 2D stack keeps case q in slice k = q;
 // 3D contexts hold one case
@@ -244,7 +247,7 @@ int ns_predictor(fcfd_ctx *c, const double *ut, const double *vt, const double *
 int ppe_src(fcfd_ctx *c);
 int projection(fcfd_ctx *c);
 int node_vel(fcfd_ctx *c);
-int div_check(fcfd_ctx *c);
+int div_check(fcfd_ctx *c, int to_host);
 int mg_setup(fcfd_ctx *c);
 int mg_free(fcfd_ctx *c);
 int mg_drop_graphs(fcfd_ctx *c);

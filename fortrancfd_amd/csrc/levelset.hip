@@ -229,7 +229,7 @@ int mpls(fcfd_ctx *c) {
     const Geom &G = c->G;
     double *phi = c->f[FCFD_PHI];
     dim3 blk(64, 4), grd = grid_interior(G, blk);
-    double *lam_dev = c->red.result + 8;
+    double *lam_dev = c->red.result + RES_LAMBDA;
     for (int it = 0; it < 2; ++it) {
         // surface_norms: only the first derivatives enter the correction
         FCFD_TRY(ccd_dir(c, 0, phi, c->f[FCFD_NX], c->cx));

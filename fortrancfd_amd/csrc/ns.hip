@@ -418,7 +418,37 @@ int node_vel(fcfd_ctx *c) {
     return velbc(c, F[FCFD_NU], F[FCFD_NV], F[FCFD_NW], 1);
 }
 
-int div_check(fcfd_ctx *c) {
+// the block partials of k_div reduced in a fixed order into
+// out[0] = max |div u|, out[1] = mean |div u|
+__global__ void k_div_final(const double *__restrict__ partial, int nblocks, double inv_ncell,
+                            double *__restrict__ out) {
+    __shared__ double smax[256], ssum[256];
+    double mx = 0.0, sm = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) {
+        mx = fmax(mx, partial[b]);
+        sm += partial[nblocks + b];
+    }
+    smax[threadIdx.x] = mx;
+    ssum[threadIdx.x] = sm;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+            ssum[threadIdx.x] += ssum[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = smax[0];
+        out[1] = ssum[0] * inv_ncell;
+    }
+}
+
+// ns_check_convergence_div: max and mean |div u| over the interior (of every
+// case of a batched stack together) into result slots RES_VDIV, RES_VDIV + 1
+// of case 0; with to_host the diagnostics are read back at once, otherwise
+// they travel with the step's end-of-step gather.
+int div_check(fcfd_ctx *c, int to_host) {
     const Geom &G = c->G;
     double **F = c->f;
     const int nb = 256;
@@ -429,16 +459,15 @@ int div_check(fcfd_ctx *c) {
         hipLaunchKernelGGL(k_div<2>, dim3(nb), dim3(256), 0, c->stream, F[FCFD_U], F[FCFD_V], nullptr,
                            c->red.partial, G, nb);
     FCFD_CHECK(hipGetLastError());
-    std::vector<double> h(2 * nb);
-    FCFD_CHECK(hipMemcpyAsync(h.data(), c->red.partial, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream));
+    hipLaunchKernelGGL(k_div_final, dim3(1), dim3(256), 0, c->stream, c->red.partial, nb,
+                       1.0 / ((double)G.nx * (double)G.ny * (double)G.nz), c->red.result + RES_VDIV);
+    FCFD_CHECK(hipGetLastError());
+    if (!to_host) return FCFD_OK;
+    double h[2];
+    FCFD_CHECK(hipMemcpyAsync(h, c->red.result + RES_VDIV, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     FCFD_CHECK(hipStreamSynchronize(c->stream));
-    double mx = 0.0, sm = 0.0;
-    for (int b = 0; b < nb; ++b) {
-        mx = fmax(mx, h[b]);
-        sm += h[nb + b];
-    }
-    c->diag.vel_div = mx;
-    c->diag.vel_sdiv = sm / ((double)G.nx * (double)G.ny * (double)G.nz);
+    c->diag.vel_div = h[0];
+    c->diag.vel_sdiv = h[1];
     return FCFD_OK;
 }
 

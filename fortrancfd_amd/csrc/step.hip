@@ -110,7 +110,7 @@ extern "C" int fcfd_node_vel(fcfd_ctx *c) {
 
 extern "C" int fcfd_ns_check_convergence_div(fcfd_ctx *c) {
     CTX_OK(c);
-    return div_check(c);
+    return div_check(c, 1);
 }
 
 // manager_switch for the fields whose old level the step reads, plus the
@@ -165,6 +165,7 @@ extern "C" int fcfd_ns_solver(fcfd_ctx *c) {
         FCFD_TRY(projection(c));
     }
     c->diag.ppe_cycles_step = total;
+    FCFD_TRY(div_check(c, 0));
     FCFD_TRY(node_vel(c));
     return rc;
 }
@@ -179,7 +180,7 @@ extern "C" int fcfd_step(fcfd_ctx *c) {
     if (rc != FCFD_OK && rc != FCFD_ERR_NOT_CONVERGED) return rc;
     // ls_mv of the main loop: mass and volume of the new phi
     FCFD_TRY(mass_sums(c, c->f[FCFD_PHI], 0));
-    FCFD_TRY(gather_to_host(c, c->diag_host, c->red.result, RES_STRIDE, 2, c->nb));
+    FCFD_TRY(gather_to_host(c, c->diag_host, c->red.result, RES_STRIDE, DIAG_W, c->nb));
     c->diag_pending = 1;
     return rc;
 }
@@ -211,8 +212,8 @@ extern "C" int fcfd_problem_init(fcfd_ctx *c, int redis0) {
     c->pr.imass = mass;
     c->diag.imass = mass;
     for (int q = 0; q < c->nb; ++q) {
-        c->diag_host[2 * q] = c->red.host[q * RES_STRIDE];
-        c->diag_host[2 * q + 1] = c->red.host[q * RES_STRIDE + 1];
+        c->diag_host[(long)q * DIAG_W] = c->red.host[q * RES_STRIDE];
+        c->diag_host[(long)q * DIAG_W + 1] = c->red.host[q * RES_STRIDE + 1];
     }
     if (c->nb > 1) {   // every case of a batched stack keeps its own initial mass
         for (int q = 0; q < c->nb; ++q) c->cases[q].imass = c->red.host[q * RES_STRIDE];

@@ -54,6 +54,15 @@ int fcfd_get_cases(const fcfd_ctx *ctx, fcfd_case *cases);
  * and the MPLS reference mass, into arrays of fcfd_batch_size entries (any
  * may be NULL). */
 int fcfd_get_case_diag(fcfd_ctx *ctx, double *mass, double *vol, double *imass);
+/* 1 when densities in [rho_min, rho_max] and the averaged face coefficients
+ * 2 / (rho_a + rho_b) of the second multigrid level fit FP16's normal range
+ * (with a small margin), else 0.  Host-only; no counterpart in the
+ * reference.  params.ppe_rho_half is honoured only when the context's
+ * density range [min(1, rho_12), max(1, rho_12)] over its cases passes this
+ * test; otherwise the FP32 copies are read.  A density pushed by the caller
+ * (the standalone Poisson problem) must itself lie in such a range when
+ * ppe_rho_half is set. */
+int fcfd_ppe_fp16_density_ok(double rho_min, double rho_max);
 /* Replace the parameters (dt, tolerances ...) of an existing context. */
 int fcfd_set_params(fcfd_ctx *ctx, const fcfd_params *params);
 /* Number of float64 values of one ghosted field. */

@@ -5,7 +5,7 @@
 set -euo pipefail
 here="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 mkdir -p "$here/_build"
-gcc -O2 -fPIC -fopenmp -ffp-contract=off -shared -o "$here/_build/liboracle.so.tmp" "$here/fcfd_oracle.c" -lm
+gcc -O2 -fPIC -fopenmp -ffp-contract=off -shared -o "$here/_build/liboracle.so.tmp" "$here/fcfd_oracle.c" "$here/m3_oracle.c" -lm
 mv "$here/_build/liboracle.so.tmp" "$here/_build/liboracle.so"
 echo "oracle: built $here/_build/liboracle.so"
 if [ "${FCFD_SKIP_REF:-0}" != "1" ]; then

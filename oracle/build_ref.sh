@@ -2,7 +2,9 @@
 # Build the compiled Fortran reference harness into oracle/_ref/ (test
 # infrastructure only).  The reference sources are compiled where they lie
 # under the reference checkout; nothing is copied.  Needs AMD flang
-# (/opt/rocm/lib/llvm/bin/flang).  Outputs: oracle/_ref/ref2d, oracle/_ref/ref3d.
+# (/opt/rocm/lib/llvm/bin/flang).  Outputs: oracle/_ref/ref2d, oracle/_ref/ref3d
+# (newOpenMP 2D / 3D+IBM through ref_harness.F90) and oracle/_ref/refm3
+# (MPLS_RisingBubble/20190612 through m3_harness.F90).
 #
 # usage: oracle/build_ref.sh [REFERENCE_ROOT]   (default /root/reference)
 set -euo pipefail
@@ -40,3 +42,13 @@ for d in 2D 3D; do
     "$FC" $FFLAGS -cpp $def -module-dir "$b/mod" -I"$b/mod" -I"$src/src" "$here/ref_harness.F90" "${objs[@]}" -o "$out/$exe"
     echo "build_ref: built $out/$exe"
 done
+# MPLS_RisingBubble (3D, two bubbles): its modules, then the harness, which
+# INCLUDEs the reference's internal subroutines from where they lie
+m3="$ref_root/MPLS_RisingBubble/20190612"
+if [ -d "$m3" ]; then
+    b="$out/build_M3"
+    mkdir -p "$b"
+    "$FC" $FFLAGS -c -module-dir "$b" "$m3/DATA_MOD.F90" -o "$b/DATA_MOD.o"
+    "$FC" $FFLAGS -cpp -module-dir "$b" -I"$b" -I"$m3" "$here/m3_harness.F90" "$b/DATA_MOD.o" -o "$out/refm3" 2> "$b/warnings.txt"
+    echo "build_ref: built $out/refm3"
+fi

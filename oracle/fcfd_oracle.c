@@ -20,6 +20,7 @@
  * Parity pinning of each function is listed in DESIGN.md (oracle section).
  */
 #include "fcfd_oracle.h"
+#include "band.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -52,6 +53,11 @@ struct orc_state {
     double *t_s1[3], *t_s2[3], *t_s3[3], *t_ss1[3], *t_ss2[3], *t_l1, *t_l2, *t_l3;
     /* ccd line work space */
     double *wk;
+    /* Cholesky factor of the pinned operator of the last ppe_sor_init (band
+     * storage, see ppe_band_factor); NULL until a direct solve needs it */
+    double *band;
+    long band_n, band_b;
+    int band_valid;
 };
 
 #define ID(s, i, j, k) ((long)((i) + (s)->g) + (long)((j) + (s)->g) * (s)->sx + (long)((k) + (s)->gz) * (s)->sxy)
@@ -114,6 +120,7 @@ void orc_destroy(orc_state *s) {
         free(s->t_s1[d]); free(s->t_s2[d]); free(s->t_s3[d]); free(s->t_ss1[d]); free(s->t_ss2[d]);
     }
     free(s->t_l1); free(s->t_l2); free(s->t_l3);
+    free(s->band);
     free(s);
 }
 
@@ -187,6 +194,7 @@ int orc_set(orc_state *s, int f, const double *src) {
     double *p = field_ptr(s, f);
     if (!p || !src) return FCFD_ERR_FIELD;
     memcpy(p, src, sizeof(double) * (size_t)s->ntot);
+    s->band_valid = 0;
     return FCFD_OK;
 }
 
@@ -1127,6 +1135,7 @@ int orc_ppe_sor_init(orc_state *s) {
                 s->coe_r[id] = r; s->coe_l[id] = l; s->coe_f[id] = f; s->coe_b[id] = b;
                 s->coe_u[id] = u; s->coe_d[id] = d; s->coe_c[id] = c;
             }
+    s->band_valid = 0;
     return FCFD_OK;
 }
 
@@ -1229,12 +1238,179 @@ int orc_ppe_apply(orc_state *s, const double *P, double *out) {
     return FCFD_OK;
 }
 
-/* Converged solve of the same operator (not in the reference): Jacobi
- * preconditioned CG on -A p = -(src - mean src) to rel_tol, then mean(p) = 0,
- * then ns_momentum_correction.  This is the fixed point the reference
- * relaxation converges to (its iterate satisfies A p = src, mean p = 0). */
-int orc_ppe_exact_solver(orc_state *s, double rel_tol) {
-    ppe_src(s);
+/* ------------------------------------------------ converged PPE solve
+ * Not in the reference: the fixed point its relaxation converges to, solved
+ * directly.  The ppe_sor_init operator A (c p + sum c_nb p_nb over the
+ * interior, zero-flux walls) is symmetric with the constants as null space,
+ * and the relaxation's iterate satisfies A p = src - mean(src), mean(p) = 0.
+ *
+ * M = -A with the first cell pinned (its row and column replaced by the
+ * identity) is symmetric positive definite and banded: cells are numbered
+ * i + nx (j + ny k), so the couplings of a row lie within b = nx (2D) or
+ * nx ny (3D) of the diagonal.  ppe_band_factor forms its Cholesky factor in
+ * band storage (row r keeps columns r-b..r at r(b+1) + (c - r + b)),
+ * ppe_band_solve the two triangular sweeps.  Backward stable, so the
+ * solution is exact to the operator's conditioning; two steps of iterative
+ * refinement with the reference-form residual remove the rest.  Dropping
+ * the pinned cell's equation loses nothing: the rhs sums to zero and the
+ * columns of A do, so that equation follows from the others. */
+
+static long ppe_cell(const orc_state *s, int i, int j, int k) {
+    return (long)i + (long)s->nx * ((long)j + (long)s->ny * (long)k);
+}
+
+/* ---- band Cholesky (shared with the MPLS_RisingBubble oracle, band.h) ---- */
+int orc_band_alloc(orc_band *B, long n, long b) {
+    const long w = b + 1;
+    if (!B->L || B->n != n || B->b != b) {
+        free(B->L);
+        B->L = (double *)malloc(sizeof(double) * (size_t)(n * w));
+        if (!B->L) return FCFD_ERR_ARG;
+        B->n = n;
+        B->b = b;
+    }
+    memset(B->L, 0, sizeof(double) * (size_t)(n * w));
+    return FCFD_OK;
+}
+
+/* right-looking band Cholesky in place: column j scales, then updates rows
+ * j+1..j+b */
+int orc_band_cholesky(orc_band *B) {
+    const long n = B->n, b = B->b, w = b + 1;
+    double *L = B->L;
+    double *col = (double *)malloc(sizeof(double) * (size_t)w);
+    for (long j = 0; j < n; ++j) {
+        double *rj = L + j * w + b;
+        if (!(rj[0] > 0.0)) { free(col); return FCFD_ERR_SINGULAR; }
+        const double d = sqrt(rj[0]);
+        rj[0] = d;
+        const long iend = j + b < n - 1 ? j + b : n - 1;
+        const long m = iend - j;
+        for (long t = 1; t <= m; ++t) {
+            double *e = L + (j + t) * w + b - t;   /* M[j+t][j] */
+            *e /= d;
+            col[t] = *e;
+        }
+#pragma omp parallel for schedule(static) if (m >= 256)
+        for (long t = 1; t <= m; ++t) {
+            const long i = j + t;
+            const double lij = col[t];
+            if (lij == 0.0) continue;
+            double *ri = L + i * w + b - i;   /* ri[c] = M[i][c] */
+            for (long q = 1; q <= t; ++q) ri[j + q] -= lij * col[q];
+        }
+    }
+    free(col);
+    return FCFD_OK;
+}
+
+/* x <- M^-1 x, x[0] pinned to 0 (row 0 of M is the identity) */
+void orc_band_solve(const orc_band *B, double *x) {
+    const long n = B->n, b = B->b, w = b + 1;
+    const double *L = B->L;
+    x[0] = 0.0;
+    for (long i = 0; i < n; ++i) {   /* L y = x */
+        const long c0 = i - b > 0 ? i - b : 0;
+        const double *ri = L + i * w + b - i;
+        double acc = x[i];
+        for (long c = c0; c < i; ++c) acc -= ri[c] * x[c];
+        x[i] = acc / ri[i];
+    }
+    for (long i = n - 1; i >= 0; --i) {   /* L^T x = y */
+        const double *ri = L + i * w + b - i;
+        x[i] /= ri[i];
+        const long c0 = i - b > 0 ? i - b : 0;
+        const double xi = x[i];
+        for (long c = c0; c < i; ++c) x[c] -= ri[c] * xi;
+    }
+}
+
+static int ppe_band_factor(orc_state *s) {
+    const long n = (long)s->nx * s->ny * s->nz;
+    const long b = s->dim == 3 ? (long)s->nx * s->ny : s->nx;
+    const long w = b + 1;
+    orc_band B = {s->band, s->band_n, s->band_b};
+    int rc = orc_band_alloc(&B, n, b);
+    s->band = B.L; s->band_n = B.n; s->band_b = B.b;
+    if (rc != FCFD_OK) return rc;
+    double *L = s->band;
+    /* lower band of M = -A: left (-1), back (-nx), down (-nx ny) and the diagonal */
+    FOR_INT(s) {
+        const long id = ID(s, i, j, k), r = ppe_cell(s, i, j, k);
+        double *row = L + r * w + b;   /* row[c - r] for c <= r */
+        row[0] = -s->coe_c[id];
+        if (i > 0) row[-1] = -s->coe_l[id];
+        if (j > 0) row[-(long)s->nx] = -s->coe_b[id];
+        if (s->dim == 3 && k > 0) row[-b] = -s->coe_d[id];
+    }
+    /* pin cell 0: identity row and column */
+    L[b] = 1.0;
+    if (n > 1) L[1 * w + b - 1] = 0.0;
+    if (n > s->nx) L[(long)s->nx * w + b - s->nx] = 0.0;
+    if (s->dim == 3 && n > b) L[b * w] = 0.0;
+    rc = orc_band_cholesky(&B);
+    if (rc != FCFD_OK) return rc;
+    s->band_valid = 1;
+    return FCFD_OK;
+}
+
+static void ppe_band_solve(const orc_state *s, double *x) {
+    const orc_band B = {s->band, s->band_n, s->band_b};
+    orc_band_solve(&B, x);
+}
+
+/* A P = coe_src - mean(coe_src), mean(P) = 0 (interior), ghosts by bc. */
+static int ppe_direct(orc_state *s) {
+    const long n = (long)s->nx * s->ny * s->nz;
+    double *P = s->p.lv[L_NOW];
+    if (!s->band_valid) {
+        int rc = ppe_band_factor(s);
+        if (rc != FCFD_OK) return rc;
+    }
+    double *f = falloc(s->ntot), *Ap = falloc(s->ntot), *x = falloc(n);
+    const double ncell = (double)n;
+    double mean = 0.0;
+    FOR_INT(s) mean += s->coe_src[ID(s, i, j, k)];
+    mean /= ncell;
+    FOR_INT(s) { long id = ID(s, i, j, k); f[id] = s->coe_src[id] - mean; }
+    FOR_INT(s) { long id = ID(s, i, j, k); P[id] = 0.0; }
+    for (int pass = 0; pass < 3; ++pass) {
+        /* x = M^-1 (A P - f) = -A^-1 r with r = f - A P */
+        orc_ppe_apply(s, P, Ap);
+        FOR_INT(s) { long id = ID(s, i, j, k); x[ppe_cell(s, i, j, k)] = Ap[id] - f[id]; }
+        ppe_band_solve(s, x);
+        FOR_INT(s) { long id = ID(s, i, j, k); P[id] += x[ppe_cell(s, i, j, k)]; }
+    }
+    double pm = 0.0;
+    FOR_INT(s) pm += P[ID(s, i, j, k)];
+    pm /= ncell;
+    FOR_INT(s) P[ID(s, i, j, k)] -= pm;
+    bc(s, P);
+    orc_ppe_apply(s, P, Ap);
+    double rn = 0.0, fn = 0.0;
+    FOR_INT(s) {
+        long id = ID(s, i, j, k);
+        double r = f[id] - Ap[id];
+        rn += r * r;
+        fn += f[id] * f[id];
+    }
+    s->ppe_rel = fn > 0.0 ? sqrt(rn / fn) : 0.0;
+    s->piter += 3;
+    free(f); free(Ap); free(x);
+    return FCFD_OK;
+}
+
+/* Largest band factor the converged solve forms (doubles); bigger systems
+ * use the projected Jacobi-PCG below.  Settable so tests can force either. */
+static long orc_band_max = 400L * 1000L * 1000L;
+void orc_set_band_max(long ndoubles) { orc_band_max = ndoubles; }
+
+/* Jacobi-preconditioned CG on -A p = -(src - mean src) to rel_tol, the fallback
+ * of the converged solve for systems whose band factor is too large.  The
+ * preconditioned residual is projected onto zero mean (the range of A), so
+ * the search directions carry no null-space component that rounding could
+ * let grow. */
+static int ppe_pcg(orc_state *s, double rel_tol) {
     const long n = s->ntot;
     double *P = s->p.lv[L_NOW];
     double *r = falloc(n), *z = falloc(n), *q = falloc(n), *Aq = falloc(n), *f = falloc(n);
@@ -1246,7 +1422,7 @@ int orc_ppe_exact_solver(orc_state *s, double rel_tol) {
     PFOR_SUM(s, fn) { long id = ID(s, i, j, k); f[id] = -(s->coe_src[id] - mean); fn += f[id] * f[id]; }
     fn = sqrt(fn);
     orc_ppe_apply(s, P, Aq);
-    double rn = 0.0, rz = 0.0;
+    double rn = 0.0, rz = 0.0, zm = 0.0;
     PFOR_SUM(s, rn) {
         long id = ID(s, i, j, k);
         r[id] = f[id] + Aq[id];
@@ -1255,7 +1431,9 @@ int orc_ppe_exact_solver(orc_state *s, double rel_tol) {
     int it = 0, rc = FCFD_OK;
     if (fn == 0.0) fn = 1.0;
     if (sqrt(rn) / fn > rel_tol) {
-        PFOR_SUM(s, rz) { long id = ID(s, i, j, k); z[id] = r[id] / (-s->coe_c[id]); q[id] = z[id]; rz += r[id] * z[id]; }
+        PFOR_SUM(s, zm) { long id = ID(s, i, j, k); z[id] = r[id] / (-s->coe_c[id]); zm += z[id]; }
+        zm /= ncell;
+        PFOR_SUM(s, rz) { long id = ID(s, i, j, k); z[id] -= zm; q[id] = z[id]; rz += r[id] * z[id]; }
         for (it = 1; it <= 200000; ++it) {
             orc_ppe_apply(s, q, Aq);
             double qAq = 0.0;
@@ -1270,7 +1448,10 @@ int orc_ppe_exact_solver(orc_state *s, double rel_tol) {
             }
             if (sqrt(rn) / fn <= rel_tol) break;
             double rz2 = 0.0;
-            PFOR_SUM(s, rz2) { long id = ID(s, i, j, k); z[id] = r[id] / (-s->coe_c[id]); rz2 += r[id] * z[id]; }
+            zm = 0.0;
+            PFOR_SUM(s, zm) { long id = ID(s, i, j, k); z[id] = r[id] / (-s->coe_c[id]); zm += z[id]; }
+            zm /= ncell;
+            PFOR_SUM(s, rz2) { long id = ID(s, i, j, k); z[id] -= zm; rz2 += r[id] * z[id]; }
             double beta = rz2 / rz;
             rz = rz2;
             PFOR_INT(s) { long id = ID(s, i, j, k); q[id] = z[id] + beta * q[id]; }
@@ -1285,6 +1466,24 @@ int orc_ppe_exact_solver(orc_state *s, double rel_tol) {
     s->ppe_rel = sqrt(rn) / fn;
     s->piter += it;
     free(r); free(z); free(q); free(Aq); free(f);
+    return rc;
+}
+
+/* The converged solve of coe_src, then mean(p) = 0: direct while the band
+ * factor fits orc_band_max doubles (rel_tol unused: the solution is exact to
+ * rounding), otherwise projected Jacobi-PCG to rel_tol. */
+int orc_ppe_solve_rhs(orc_state *s, double rel_tol) {
+    const long n = (long)s->nx * s->ny * s->nz;
+    const long b = s->dim == 3 ? (long)s->nx * s->ny : s->nx;
+    if (n * (b + 1) <= orc_band_max) return ppe_direct(s);
+    return ppe_pcg(s, rel_tol);
+}
+
+/* Converged replacement of ppe_sor_solver: src = div(u*)/dt, the converged
+ * solve, then ns_momentum_correction, as the relaxation ends with it. */
+int orc_ppe_exact_solver(orc_state *s, double rel_tol) {
+    ppe_src(s);
+    int rc = orc_ppe_solve_rhs(s, rel_tol);
     orc_ns_momentum_correction(s);
     return rc;
 }

@@ -53,6 +53,8 @@ int orc_curv(orc_state *s);
 int orc_ppe_sor_init(orc_state *s);
 int orc_ppe_sor_solver(orc_state *s, double tol);     /* reference relaxation */
 int orc_ppe_exact_solver(orc_state *s, double rel_tol);/* converged solve      */
+int orc_ppe_solve_rhs(orc_state *s, double rel_tol);  /* converged solve of coe_src alone */
+void orc_set_band_max(long ndoubles);  /* direct solve up to this band size, else PCG */
 int orc_ns_momentum_correction(orc_state *s);
 int orc_ns_linearize(orc_state *s);
 int orc_ns_ab_setup(orc_state *s);

@@ -52,7 +52,8 @@ def lib():
         L.orc_set_imass.argtypes = [C.c_void_p, C.c_double]
         L.orc_get_diag.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_set_threads.argtypes = [C.c_int]
-        for nm in ("orc_ppe_sor_solver", "orc_ppe_exact_solver"):
+        L.orc_set_band_max.argtypes = [C.c_long]
+        for nm in ("orc_ppe_sor_solver", "orc_ppe_exact_solver", "orc_ppe_solve_rhs"):
             getattr(L, nm).argtypes = [C.c_void_p, C.c_double]
         for nm in ("orc_level_set_rk3_redis", "orc_ns_solver", "orc_step", "orc_problem_init", "orc_bc"):
             getattr(L, nm).argtypes = [C.c_void_p, C.c_int]

@@ -30,6 +30,14 @@ def case3d(n: int = 12, **kw) -> Case:
     return Case(dim=3, nx=n, ny=n, nz=2 * n, xend=1.0, yend=1.0, zend=2.0, gvec=(0.0, 0.0, -1.0), **kw)
 
 
+def case3d_odd(**kw) -> Case:
+    """3D box of 20 x 18 x 37 cells (dx = dy = dz = 0.05): no extent is a
+    multiple of the 16-plane walks or of the 64 x 4 tiles of the kernels, so
+    their remainder planes and partial tiles are exercised."""
+    kw.setdefault("name", "t3d")
+    return Case(dim=3, nx=20, ny=18, nz=37, xend=1.0, yend=0.9, zend=1.85, gvec=(0.0, 0.0, -1.0), **kw)
+
+
 def _smooth(case: Case, rng, amp: float, nmodes: int = 3) -> np.ndarray:
     """a smooth seeded field: sum of a few low Fourier modes over the box."""
     if case.dim == 3:

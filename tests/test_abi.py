@@ -74,3 +74,19 @@ def test_status_and_field_enums_match_the_header():
     body = src[src.index("enum fcfd_field"):src.index("FCFD_NFIELDS")]
     names = re.findall(r"FCFD_(\w+)", body)
     assert [n.lower() for n in names] == list(types.FIELDS)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libfcfd.so not built (fortrancfd_amd/build.py)")
+def test_fp16_density_guard():
+    """FP16 copies of the density (level 0) and of the level-1 face
+    coefficients 2/(rho_a + rho_b) are only used while both fit FP16's
+    normal range [6.1e-5, 65504]; host-only, no device call."""
+    f = C.CDLL(LIB).fcfd_ppe_fp16_density_ok
+    f.argtypes = [C.c_double, C.c_double]
+    assert f(1e-3, 1.0) == 1            # the rising bubble, ratio 1000
+    assert f(1.0, 1000.0) == 1          # the standalone Poisson density
+    assert f(1e-4, 1.0) == 1
+    assert f(5e-5, 1.0) == 0            # rho subnormal in FP16
+    assert f(1e-5, 1.0) == 0            # 1/rho overflows FP16 on level 1
+    assert f(1.0, 1.6e4) == 1 and f(1.0, 2e4) == 0   # 1/rho on level 1 subnormal in FP16
+    assert f(0.0, 1.0) == 0 and f(2.0, 1.0) == 0

@@ -59,3 +59,31 @@ def test_device_deformation_field_is_the_ghost_copied_formula():
         b = b.numpy().reshape(case.shape)
         assert np.abs(a - b).max() <= 4e-16 * np.abs(a).max()
         assert np.array_equal(b[0], b[case.ghc]) and np.array_equal(b[:, :, -1], b[:, :, -case.ghc - 1])
+
+
+def test_ghost_fold_is_the_adjoint_of_the_ghost_copy():
+    """<W, bc(x)> = <fold(W), x> for every ghosted x: fold adds each ghost's
+    weight to the interior cell the zero-gradient copy takes it from."""
+    import torch
+    from fortrancfd_amd.adjoint import _fold_ghosts
+    from fortrancfd_amd.types import Case, bc_copy
+    for case in (Case(dim=3, nx=5, ny=4, nz=6, name="t"), Case(dim=2, nx=5, ny=7, nz=1, name="t")):
+        rng = np.random.default_rng(1)
+        W = rng.standard_normal(case.shape)
+        fW = _fold_ghosts(torch.from_numpy(W), case.ghc).numpy()
+        assert not fW[~_interior_mask(case)].any()
+        for _ in range(3):
+            x = rng.standard_normal(case.shape)
+            lhs = float((W * bc_copy(case, x.copy())).sum())
+            rhs = float((fW * x).sum())
+            assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs)), (lhs, rhs)
+
+
+def _interior_mask(case):
+    m = np.zeros(case.shape, dtype=bool)
+    g = case.ghc
+    if case.dim == 3:
+        m[g:-g, g:-g, g:-g] = True
+    else:
+        m[g:-g, g:-g] = True
+    return m

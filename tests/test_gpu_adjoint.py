@@ -219,6 +219,45 @@ def test_gradient_against_autograd_reference():
     assert relA <= 1e-10
 
 
+def test_gradient_with_weights_on_the_ghosts():
+    """A loss that weights every entry of the ghosted phi_T, ghosts included:
+    the ghosts of phi_T are zero-gradient copies of the boundary cells, so
+    their weights reach dJ/dphi0 and dJ/dA through those cells.  Against
+    torch.autograd of the restatement with the same copy (replicate padding)
+    in its forward; dJ/dphi0 is zero on phi0's ghosts, which the advection
+    never reads."""
+    import adjoint_ref as R
+    from fortrancfd_amd.adjoint import advect, deformation_field
+    case = _case((12, 10, 14))
+    nsteps, period, every = 5, 5 * case.dt, 3
+    phi0 = _phi0(case, r=0.2, c=(0.3, 0.4, 0.5))
+    g = case.ghc
+    inter = lambda a: np.ascontiguousarray(a.reshape(case.shape)[g:-g, g:-g, g:-g])
+    uhat = tuple(torch.from_numpy(inter(u)) for u in deformation_field(case))
+    h = (case.dx, case.dy, case.dz)
+    W = np.random.default_rng(11).standard_normal(case.shape)
+    p0 = torch.from_numpy(inter(phi0)).requires_grad_(True)
+    A = torch.tensor(1.1, dtype=torch.float64, requires_grad=True)
+    pT = R.advect(p0, A, uhat, h, case.dt, nsteps, period)
+    full = torch.nn.functional.pad(pT[None, None], (g,) * 6, mode="replicate")[0, 0]
+    (full * torch.from_numpy(W)).sum().backward()
+    dev = torch.device("cuda")
+    q0 = torch.from_numpy(phi0.reshape(-1).copy()).to(dev).requires_grad_(True)
+    B = torch.tensor(1.1, dtype=torch.float64, device=dev, requires_grad=True)
+    qT = advect(q0, B, _solver(case), nsteps, period, every)
+    fwd = np.abs(qT.detach().cpu().numpy().reshape(case.shape) - full.detach().numpy()).max()
+    (qT * torch.from_numpy(W.reshape(-1)).to(dev)).sum().backward()
+    gq = q0.grad.cpu().numpy().reshape(case.shape)
+    rel = np.abs(inter(gq) - p0.grad.numpy()).max() / np.abs(p0.grad.numpy()).max()
+    relA = abs(float(B.grad) - float(A.grad)) / abs(float(A.grad))
+    m = np.ones(case.shape, dtype=bool)
+    m[g:-g, g:-g, g:-g] = False
+    print("forward", fwd, "grad phi", rel, "grad A", relA)
+    assert fwd <= 1e-13 * np.abs(full.detach().numpy()).max()
+    assert rel <= 1e-10 and relA <= 1e-10
+    assert not gq[m].any()
+
+
 def test_velocity_scale_is_a_factor_on_the_velocity():
     from fortrancfd_amd.adjoint import deformation_field
     case = _case()

@@ -172,3 +172,56 @@ def test_batch_with_iterative_coarsest_level_matches_single_cases():
         got = b.pull(k)
         for q in range(3):
             assert np.array_equal(got[q], ref[q][k]), (k, q, float(np.abs(got[q] - ref[q][k]).max()))
+
+
+def test_config_grid_batches_against_the_oracle():
+    """The batched workload's own grids (64x128, 128x256, 256x512) with its
+    seeded cases (bubble radius, density ratio and Eotvos number drawn as the
+    benchmark draws them): two cases per grid in one batched context, five
+    steps (the fifth reinitialises), FP64 and mixed pressure modes, each case
+    against the oracle run alone with its converged (direct) pressure solve.
+    phi, u, v, nu, nv and p to 1e-10 relative, mass to 1e-12."""
+    from fortrancfd_amd.solver import BatchSolver
+    from fortrancfd_amd.types import bc_copy, bubble_phi
+    from fortrancfd_amd.workloads import batch_cases
+    from oracle.oracle import Oracle
+    Oracle.threads(16)
+    groups = {}
+    for c, r in batch_cases(64, tol=1e-13):
+        g = groups.setdefault((c.nx, c.ny), [])
+        if len(g) < 2:
+            g.append((c, r))
+    assert sorted(groups) == [(64, 128), (128, 256), (256, 512)]
+    nsteps, names = 5, ("phi", "u", "v", "nu", "nv", "p")
+    for key in sorted(groups):
+        cs = groups[key]
+        phis = [bc_copy(c, bubble_phi(c, r, (0.5, 0.5))) for c, r in cs]
+        ref, ref_mass = [], []
+        for (c, _), phi in zip(cs, phis):
+            o = Oracle(c)
+            o.set("phi", phi)
+            o.call("problem_init", 0)
+            for _ in range(nsteps):
+                o.call("step", 1)
+            ref.append({n: o.get(n) for n in names})
+            ref_mass.append(o.diag()["mass"])
+            del o
+        for prec in (0, 1):
+            for c, _ in cs:
+                c.ppe_precision = prec
+            b = BatchSolver([c for c, _ in cs])
+            b.push("phi", np.stack(phis))
+            b.problem_init(False)
+            for _ in range(nsteps):
+                b.step()
+            mass = b.case_diag()["mass"]
+            for n in names:
+                got = b.pull(n)
+                for q, (c, _) in enumerate(cs):
+                    a, e = c.interior(got[q]), c.interior(ref[q][n])
+                    err = float(np.abs(a - e).max() / np.abs(e).max())
+                    print(key, "precision", prec, "case", q, n, f"{err:.2e}", flush=True)
+                    assert err <= 1e-10, (key, prec, q, n, err)
+            for q in range(len(cs)):
+                assert abs(mass[q] - ref_mass[q]) <= 1e-12 * ref_mass[q], (key, prec, q)
+            b.close()

@@ -6,19 +6,29 @@ Tolerances, relative to the max-norm of the oracle field:
   * copies and ghost rules: exact;
   * single kernels: 1e-12 (the GPU's sin/cos, pow and FMA contraction differ
     from the host's by an ulp; reductions run in a different order);
-  * the pressure solve: the multigrid stops at ||r||/||f|| <= 1e-12 and the
-    oracle's CG at 1e-13, so p and the projected velocity agree to 1e-8;
-  * whole steps: 1e-8 (bounded by the pressure tolerance).
+  * the pressure solve and whole steps: 1e-10.  The multigrid solves to
+    ||r||/||f|| <= 1e-13 and the oracle solves the same operator directly
+    (band Cholesky, exact to rounding), so what remains is the operator's
+    conditioning times rounding: ~1e-12 at these sizes with density ratio
+    1000 (test_oracle_ppe.py measures the same spread between two direct
+    solvers).
+
+Grids: 2D 32 x 64, 3D 16 x 16 x 32 and 3D 20 x 18 x 37, whose extents are
+not multiples of the kernels' 16-plane walks and 64 x 4 tiles.
 """
 import numpy as np
 import pytest
 
-from cases import bubble, case2d, case3d, max_rel, node_velocities, seeded, vel_names, velocities
+from cases import bubble, case2d, case3d, case3d_odd, max_rel, node_velocities, seeded, vel_names, velocities
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
-CASES = [("2d", lambda **kw: case2d(32, **kw)), ("3d", lambda **kw: case3d(16, **kw))]
+CASES = [("2d", lambda **kw: case2d(32, **kw)), ("3d", lambda **kw: case3d(16, **kw)),
+         ("3d_odd", lambda **kw: case3d_odd(**kw))]
+
+# device pressure tolerance of the solve and step tests
+PPE_TOL = 1e-13
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -68,8 +78,10 @@ def test_ls_funs_rho_mu(tag, mk):
     _cmp(s, o, ["heavy", "delta", "sign", "rho", "mu"], 1e-14)
     m, v = s.ls_mv(); o.call("ls_mv")
     d = o.diag()
-    assert abs(m - d["mass"]) <= 1e-13 * d["mass"]
-    assert abs(v - d["vol"]) <= 1e-13 * d["vol"]
+    # sums of ~1e4 positive terms in another order: a few 1e-14 typically,
+    # 1.2e-13 measured at 20 x 18 x 37
+    assert abs(m - d["mass"]) <= 1e-12 * d["mass"]
+    assert abs(v - d["vol"]) <= 1e-12 * d["vol"]
 
 
 @pytest.mark.parametrize("tag,mk", CASES)
@@ -186,37 +198,46 @@ def test_pressure_solve_and_projection(tag, mk, precision):
     s, o = _pair(case, {"rho": rho, **velocities(case, 6)})
     s.velbc(); o.call("velbc")
     s.ppe_sor_init(); o.call("ppe_sor_init")
-    cyc, rel = s.ppe_sor_solver(1e-12, precision)
-    o.call("ppe_exact_solver", 1e-13)
+    cyc, rel = s.ppe_sor_solver(PPE_TOL, precision)
+    o.call("ppe_exact_solver", 1e-15)
     print("cycles", cyc, "rel", rel)
-    assert rel <= 1e-12
-    _cmp(s, o, ["p"], 1e-8, case)
-    _cmp(s, o, vel_names(case), 1e-8)
+    assert rel <= 10 * PPE_TOL
+    _cmp(s, o, ["p"], 1e-10, case)
+    _cmp(s, o, vel_names(case), 1e-10)
     s.ns_check_convergence_div()
     o.call("ns_check_convergence_div")
     d = s.diag()
-    assert d["vel_div"] <= 1e-8 * np.abs(case.interior(o.get("ppe_src"))).max() * case.dt
+    assert d["vel_div"] <= 1e-10 * np.abs(case.interior(o.get("ppe_src"))).max() * case.dt
 
 
 @pytest.mark.parametrize("tag,mk", CASES)
-def test_time_steps(tag, mk):
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_time_steps(tag, mk, precision):
     """problem_init + 5 main-loop steps (step 5 reinitialises) with the
-    converged pressure solve on both sides."""
-    case = mk(ppe_tol=1e-12)
+    converged pressure solve on both sides: the device multigrid in FP64 or
+    with FP32 V-cycles, the oracle's direct solve; every field after every
+    step to 1e-10."""
+    case = mk(ppe_tol=PPE_TOL, ppe_precision=precision)
     from fortrancfd_amd import Solver
     phi = bubble(case, 0.0, 1)
     s = Solver(case)
     o = Oracle(case)
     s.push("phi", phi); o.set("phi", phi)
     s.problem_init(False); o.call("problem_init", 0)
-    assert abs(s.diag()["imass"] - o.diag()["imass"]) <= 1e-13 * o.diag()["imass"]
+    assert abs(s.diag()["imass"] - o.diag()["imass"]) <= 1e-12 * o.diag()["imass"]
+    names = ["phi", "p", "nu", "nv"] + vel_names(case) + (["nw"] if case.dim == 3 else [])
     for it in range(5):
         s.step()
         o.call("step", 1)
-    names = ["phi", "p", "nu", "nv"] + vel_names(case)
-    _cmp(s, o, names, 1e-8)
+        print("step", it + 1)
+        _cmp(s, o, names, 1e-10)
     d, e = s.diag(), o.diag()
-    assert abs(d["mass"] - e["mass"]) <= 1e-11 * e["mass"]
+    assert abs(d["mass"] - e["mass"]) <= 1e-12 * e["mass"]
+    # the step's own diagnostics: max |div u| after the projection and the
+    # last MPLS multiplier (both at round-off level here, as the oracle's)
+    print("vel_div", d["vel_div"], e["vel_div"], "mpls_lambda", d["mpls_lambda"], e["mpls_lambda"])
+    assert 0.0 < d["vel_div"] <= 1e-8 and d["vel_sdiv"] <= d["vel_div"]
+    assert np.isfinite(d["mpls_lambda"]) and d["mpls_lambda"] != 0.0
 
 
 def test_ccd_lines_longer_than_the_lds_table():
