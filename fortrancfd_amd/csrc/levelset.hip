@@ -14,6 +14,7 @@
 // the backward differences, the lambda numerator/denominator) are recomputed
 // or read with clamped indices instead of being stored with ghosts.
 #include <algorithm>
+#include <mutex>
 
 #include "fcfd_internal.h"
 #include "weno.h"
@@ -224,6 +225,122 @@ int ls_mv(fcfd_ctx *c, double *mass, double *vol) {
     if (vol) *vol = c->red.host[1];
     return FCFD_OK;
 }
+
+// ------------------------------------------------ ls_mv without allocation
+// fcfd_ls_mv_noalloc: the kernels of ls_mv (k_ls_props, k_mass_sums,
+// k_reduce_final) on caller-owned fields, with their reduction partials,
+// reduced sums and per-case ratios in one file-scope device array instead of
+// context memory.  Every call on a device shares that device's copy of the
+// array, so calls that overlap on one device (two streams, two threads) race;
+// the array keeps the last call's contents, and nothing in it is read before
+// this call has written it.
+constexpr int NOALLOC_MAX_CASES = FCFD_LS_MV_NOALLOC_MAX_CASES;
+constexpr long NA_PARTIAL = 3L * RBLK * NOALLOC_MAX_CASES;            // k_mass_sums block partials
+constexpr long NA_RESULT = (long)RES_STRIDE * NOALLOC_MAX_CASES;      // k_reduce_final output
+constexpr long NA_CASES = (long)(sizeof(fcfd_case) / sizeof(double)) * NOALLOC_MAX_CASES;
+constexpr long NA_WORDS = NA_PARTIAL + NA_RESULT + NA_CASES;          // 1.58 MB
+static __device__ __align__(256) double scratch[NA_WORDS];
+
+struct CaseRatios {
+    double r12[NOALLOC_MAX_CASES], m12[NOALLOC_MAX_CASES];
+};
+
+__global__ void k_put_cases(CaseRatios cr, int n, fcfd_case *__restrict__ dst) {
+    const int q = threadIdx.x;
+    if (q >= n) return;
+    fcfd_case c{};
+    c.rho_12 = cr.r12[q];
+    c.mu_12 = cr.m12[q];
+    dst[q] = c;
+}
+
+// sums[2q], sums[2q + 1] = mass, volume of case q
+__global__ void k_put_sums(const double *__restrict__ result, int n, double *__restrict__ sums) {
+    const int t = threadIdx.x;
+    if (t >= 2 * n) return;
+    sums[t] = result[(long)(t >> 1) * RES_STRIDE + (t & 1)];
+}
+
+}  // namespace fcfd
+
+extern "C" int fcfd_ls_mv_noalloc(const fcfd_grid *grid, const fcfd_params *params, int ncases,
+                                  const fcfd_case *cases, const double *phi, double *heavy, double *delta,
+                                  double *sign, double *rho, double *mu, double *sums, void *stream) {
+    using namespace fcfd;
+    if (!grid || !params || !phi || !sums) return FCFD_ERR_ARG;
+    if (grid->dim != 2 && grid->dim != 3) return FCFD_ERR_ARG;
+    if (grid->ghc != 3 || grid->nx < 4 || grid->ny < 4) return FCFD_ERR_ARG;
+    if (grid->dim == 3 && (grid->nz < 4 || ncases != 1 || cases)) return FCFD_ERR_ARG;
+    if (grid->dim == 2 && grid->nz != 1) return FCFD_ERR_ARG;
+    if (ncases < 1) return FCFD_ERR_ARG;
+    if (ncases > NOALLOC_MAX_CASES) {
+        fprintf(stderr, "libfcfd: fcfd_ls_mv_noalloc: %d cases exceed its fixed scratch of %d cases\n", ncases,
+                NOALLOC_MAX_CASES);
+        return FCFD_ERR_ARG;
+    }
+    if (ncases > 1 && !cases) return FCFD_ERR_ARG;
+    // the device address of this device's copy of the scratch array, looked
+    // up once per device (the array's name is not a device address on the host)
+    static std::mutex mu_;
+    static double *dev_scratch[64] = {};
+    int dev = 0;
+    FCFD_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return FCFD_ERR_ARG;
+    double *base;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!dev_scratch[dev]) {
+            void *ptr = nullptr;
+            FCFD_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            dev_scratch[dev] = (double *)ptr;
+        }
+        base = dev_scratch[dev];
+    }
+    double *partial = base, *result = base + NA_PARTIAL;
+    fcfd_case *cp = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    Geom G;
+    G.dim = grid->dim;
+    G.nx = grid->nx; G.ny = grid->ny; G.nz = grid->dim == 3 ? grid->nz : ncases;
+    G.g = grid->ghc; G.gz = grid->dim == 3 ? grid->ghc : 0;
+    G.sx = G.nx + 2 * G.g;
+    G.sxy = G.sx * (G.ny + 2 * G.g);
+    G.ntot = G.sxy * (G.nz + 2 * G.gz);
+    G.dx = (grid->xend - grid->xstart) / G.nx;
+    G.dy = (grid->yend - grid->ystart) / G.ny;
+    G.dz = grid->dim == 3 ? (grid->zend - grid->zstart) / G.nz : 1.0;
+    if (ncases > 1) {
+        CaseRatios cr{};
+        for (int q = 0; q < ncases; ++q) {
+            cr.r12[q] = cases[q].rho_12;
+            cr.m12[q] = cases[q].mu_12;
+        }
+        cp = (fcfd_case *)(base + NA_PARTIAL + NA_RESULT);
+        hipLaunchKernelGGL(k_put_cases, dim3(1), dim3(NOALLOC_MAX_CASES), 0, st, cr, ncases, cp);
+        FCFD_CHECK(hipGetLastError());
+    }
+    const long n = G.ntot;
+    hipLaunchKernelGGL(k_ls_props, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, phi, heavy, delta, sign,
+                       rho, mu, n, params->ls_wid, params->rho_12, params->mu_12, (const fcfd_case *)cp, G.sxy);
+    FCFD_CHECK(hipGetLastError());
+    if (G.dim == 3)
+        hipLaunchKernelGGL(k_mass_sums<3>, dim3(RBLK), dim3(RTHR), 0, st, phi, (const double *)nullptr,
+                           (const double *)nullptr, (const double *)nullptr, (double *)nullptr, partial, G,
+                           params->ls_wid, params->rho_12, 0, (const fcfd_case *)nullptr);
+    else
+        hipLaunchKernelGGL(k_mass_sums<2>, dim3(RBLK, ncases), dim3(RTHR), 0, st, phi, (const double *)nullptr,
+                           (const double *)nullptr, (const double *)nullptr, (double *)nullptr, partial, G,
+                           params->ls_wid, params->rho_12, 0, (const fcfd_case *)cp);
+    FCFD_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_final, dim3(ncases), dim3(RTHR), 0, st, (const double *)partial, result, 3, RBLK);
+    FCFD_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_put_sums, dim3(1), dim3(2 * NOALLOC_MAX_CASES), 0, st, (const double *)result, ncases,
+                       sums);
+    FCFD_CHECK(hipGetLastError());
+    return FCFD_OK;
+}
+
+namespace fcfd {
 
 int mpls(fcfd_ctx *c) {
     const Geom &G = c->G;

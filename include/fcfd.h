@@ -95,6 +95,27 @@ int fcfd_rho_mu(fcfd_ctx *ctx);
 /* ls_mv: rho_mu + mass and volume sums (others.f90:122-176). */
 int fcfd_ls_mv(fcfd_ctx *ctx, double *mass, double *vol);
 
+/* ls_mv without allocation: ls_funs + rho_mu over the ghosted box of phi
+ * (H, delta, sign, rho, mu; any output may be NULL) and the mass and volume
+ * sums into sums[2q], sums[2q + 1] for each case q, with the kernels of
+ * fcfd_ls_mv and bit for bit its results.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It takes
+ * no context: every pointer is the caller's device memory in the layout of
+ * fcfd_types.h, and the call allocates nothing.  Its reduction partials live
+ * in a fixed device array of the library (1.6 MB per device, held until the
+ * process ends), so ncases is at most FCFD_LS_MV_NOALLOC_MAX_CASES
+ * (FCFD_ERR_ARG beyond); calls that overlap on one device race on that array.
+ * grid->dim = 3: one case (ncases = 1, cases = NULL) with params->rho_12 /
+ * mu_12.  grid->dim = 2: ncases consecutive planes as a batched context holds
+ * them, with the ratios of cases[q] when ncases > 1 (host array).  params
+ * supplies ls_wid (and the ratios of a single case).  stream is a
+ * hipStream_t (NULL: the null stream); the call only queues work on it. */
+#define FCFD_LS_MV_NOALLOC_MAX_CASES 64
+int fcfd_ls_mv_noalloc(const fcfd_grid *grid, const fcfd_params *params, int ncases, const fcfd_case *cases,
+                       const double *phi, double *heavy, double *delta, double *sign, double *rho, double *mu,
+                       double *sums, void *stream);
+
 /* ---- level-set advection (`.../interface_solver/level_set_rk3_solver.f90:1-219`) ---- */
 /* level_set_rk3_solver: SSP-RK3 of phi with the cell velocity nvel (fields
  * NU/NV/NW hold nvel%old of the reference); the source is the WENO5-JS flux
