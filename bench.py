@@ -114,13 +114,15 @@ def profiled_traffic(region: str):
     return None, None
 
 
-def cpu_baseline_reference(n: int = 32, threads: int = 0) -> dict:
+def cpu_baseline_reference(n: int = 40, steps: int = 3, threads: int = 0) -> dict:
     """The compiled Fortran reference (oracle/_ref/ref3d, its own OpenMP
     build) on a bounded sample: the rising bubble on a uniform n x n x 2n grid
     of [0,1]^2 x [0,2] (the reference reads cells per unit length, so the
     dz = 2 dx grid of the GPU workload has no reference input), problem_init
-    untimed, then one timed main-loop step with the reference's own pressure
-    relaxation (max |dp| < 1e-7, p_w1 = 1.5)."""
+    untimed, then `steps` main-loop steps in one process, timed inside it
+    around the step loop only (no field I/O), with the reference's own
+    pressure relaxation (max |dp| < 1e-7, p_w1 = 1.5).  Threads: the box's
+    CPU share for one GPU (16), bound to cores."""
     from oracle import refrun
     from fortrancfd_amd.types import Case, bc_copy, bubble_phi
     if not refrun.available(3):
@@ -131,14 +133,14 @@ def cpu_baseline_reference(n: int = 32, threads: int = 0) -> dict:
     phi = bc_copy(case, bubble_phi(case, 0.2, (0.5, 0.5, 0.5)))
     out = refrun.run(case, "init_nore", {"phi": phi}, threads=th)
     state = {k: out[k] for k in refrun.HARNESS_FIELDS_3D}
-    t = time.perf_counter()
-    o2 = refrun.run(case, "step", state, imass=out["_imass"], time=0.0, it=0, threads=th, timeout=900)
-    dt = time.perf_counter() - t
-    return {"value": round(case.ncells / dt / 1e6, 6), "unit": "Mcell-steps/s", "cores": th, "kind": "reference",
-            "sample": f"compiled Fortran reference (newOpenMP 3D, flang -O2 -fopenmp) main-loop step on the rising "
-                      f"bubble at {n}x{n}x{2 * n} (uniform grid), its own pressure relaxation to max|dp| < 1e-7 "
-                      f"({o2['_piter']} relaxation sweeps in the step), {th} OpenMP threads, {dt:.1f} s including "
-                      f"the harness's field file I/O"}
+    r = refrun.run_steps_timed(case, state, steps, out["_imass"], th)
+    dt = r["seconds"]
+    return {"value": round(case.ncells * steps / dt / 1e6, 6), "unit": "Mcell-steps/s", "cores": th,
+            "kind": "reference",
+            "sample": f"compiled Fortran reference (newOpenMP 3D, flang -O2 -fopenmp) main loop on the rising bubble "
+                      f"at {n}x{n}x{2 * n} (uniform grid), {steps} steps timed inside the program (no file I/O), "
+                      f"its own pressure relaxation to max|dp| < 1e-7 (p%glb%piter {r['piter']} at the end), {th} OpenMP "
+                      f"threads bound to cores, {dt:.1f} s"}
 
 
 def cpu_baseline(steps: int = 5, n: int = 96, threads: int = 0) -> dict:
@@ -208,6 +210,46 @@ def mg_compare(s, case, variants=MG_VARIANTS) -> dict:
         # the same V(2,2) cycle iterated in both precisions (no Krylov acceleration)
         out["speedup_plain_vcycle"] = round(conv["fp64_plain"]["ms"] / conv["mixed_plain"]["ms"], 3)
     return out
+
+
+def config2_run(n: int, steps: int = 200) -> dict:
+    """The 3D rising bubble of the benchmark from rest for `steps` steps (the
+    whole configuration), once with the FP64 multigrid and once with FP32
+    V-cycles inside the FP64 outer iteration: wall time, pressure iterations
+    per step, the mixed run's final u, v, w, p, phi against the FP64 run's
+    (the FP64 path is pinned to the oracle at 1e-10 by the tests), and the
+    bubble volume drift sum(1 - H) dV of each run."""
+    from fortrancfd_amd import Solver
+    from fortrancfd_amd.types import PPE_FP64, PPE_MIXED, bc_copy, bubble_phi
+    runs, fields = {}, {}
+    for name, prec in (("fp64", PPE_FP64), ("mixed", PPE_MIXED)):
+        case = rising_case(n, prec, 1e-10)
+        s = Solver(case)
+        s.push("phi", bc_copy(case, bubble_phi(case, 0.2, (0.5, 0.5, 0.5))))
+        s.problem_init(False)
+        d0 = s.diag()
+        vol_tot = float(case.nx * case.ny * case.nz) * case.dx * case.dy * case.dz
+        bub0 = vol_tot - d0["vol"]
+        cyc = []
+        s.sync()
+        t = time.perf_counter()
+        for _ in range(steps):
+            s.step()
+            cyc.append(s.diag()["ppe_cycles_step"])
+        s.sync()
+        dt = time.perf_counter() - t
+        d = s.diag()
+        bub = vol_tot - d["vol"]
+        runs[name] = {"seconds": round(dt, 3), "ms_per_step": round(dt * 1e3 / steps, 3),
+                      "ppe_iters_per_step": {"min": int(min(cyc)), "mean": round(float(np.mean(cyc)), 2),
+                                             "max": int(max(cyc))},
+                      "bubble_volume_drift": float(bub / bub0 - 1.0), "mass_drift": float(d["mass"] / d0["mass"] - 1.0)}
+        fields[name] = {f: case.interior(s.pull(f)).copy() for f in ("u", "v", "w", "p", "phi")}
+        s.close()
+        print(f"config-2 {name}: {steps} steps in {dt:.2f} s", file=sys.stderr, flush=True)
+    err = {f: float(np.abs(fields["mixed"][f] - fields["fp64"][f]).max() / np.abs(fields["fp64"][f]).max())
+           for f in fields["fp64"]}
+    return {"steps": steps, "grid": [n, n, n], "runs": runs, "mixed_vs_fp64_max_rel": err}
 
 
 def poisson_bench(n: int) -> dict:
@@ -426,6 +468,8 @@ def main() -> int:
     ap.add_argument("--batch-regions", action="store_true", help="add the per-grid HIP-event region breakdown")
     ap.add_argument("--regions-out", default="")
     ap.add_argument("--no-tune", action="store_true", help="keep the default smoother schedule")
+    ap.add_argument("--no-config2", action="store_true", help="skip the 200-step FP64 vs mixed run")
+    ap.add_argument("--config2-steps", type=int, default=200)
     ap.add_argument("--krylov", type=int, default=1)
     ap.add_argument("--inner", type=int, default=1)
     args = ap.parse_args()
@@ -449,7 +493,7 @@ def main() -> int:
     mixed = args.precision == "mixed"
     case = rising_case(args.n, PPE_MIXED if mixed else PPE_FP64, 1e-10)
     case.ppe_krylov, case.ppe_inner = args.krylov, args.inner
-    s = Solver(case, allow_unconverged=True)
+    s = Solver(case)   # an unconverged pressure solve raises
     s.push("phi", bc_copy(case, bubble_phi(case, 0.2, (0.5, 0.5, 0.5))))
     s.problem_init(False)
     for w in range(args.warmup):
@@ -469,6 +513,9 @@ def main() -> int:
     s.sync()
     t1 = time.perf_counter()
     d = s.diag()
+    # the pressure iterations of each step of the profiled repeat below (a
+    # diag read synchronises, so it stays out of the timed region)
+    cyc_prof = []
     print(f"timed {args.steps} steps in {t1 - t0:.3f} s; last diag {d}", file=sys.stderr, flush=True)
     # the same number of steps again with a HIP event pair around every kernel
     # launch on the solver's stream (V-cycles then launch directly instead of
@@ -477,6 +524,7 @@ def main() -> int:
     tp0 = time.perf_counter()
     for k in range(args.steps):
         s.step()
+        cyc_prof.append(s.diag()["ppe_cycles_step"])
     s.sync()
     tp1 = time.perf_counter()
     prof = s.prof_read()
@@ -545,10 +593,14 @@ def main() -> int:
         out["autotune_candidate_ms"] = tuned["ms"]
     if not args.no_mg:
         out["mg"] = mg_compare(s, case)
+    out["ppe_iters_per_step_profiled_run"] = {"min": int(min(cyc_prof)), "mean": round(float(np.mean(cyc_prof)), 2),
+                                              "max": int(max(cyc_prof))}
     if args.regions_out:
         with open(args.regions_out, "w") as fh:
             json.dump(regions, fh, indent=1)
     s.close()
+    if not args.no_config2 and case.nx == 256:
+        out["config2"] = config2_run(case.nx, args.config2_steps)
     if not args.no_cpu:
         # the reference's own OpenMP build; the C restatement (same operation
         # order, pressure solved to the same 1e-10) is reported beside it

@@ -32,8 +32,11 @@ namespace fcfd {
 // the reciprocal products are folded in here once).
 enum { PF_A1, PF_B1, PF_AA1, PF_BB1, PB_A3, PB_B3, PB_AA3, PB_BB3, PB_A2D, PB_AA2D, PB_B2D, PB_BB2D, NPACK };
 
-// line matrices of the CCD family (working_matrix rows 1..3 = sub, diag, super)
-enum { MAT_CCD = 0, MAT_UCCD_UP = 1, MAT_UCCD_DOWN = 2, NMAT = 3 };
+// line matrices of the CCD family (working_matrix rows 1..3 = sub, diag, super):
+// the newOpenMP set on ghosted lines (n + 6 points) and the MPLS_RisingBubble
+// set on interior lines (n points; its UCCD has the M-DRPCCD coefficients,
+// `MPLS_RisingBubble/20190612/SCHEME_COMPACT.F90:189-200,482-490`)
+enum { MAT_CCD = 0, MAT_UCCD_UP = 1, MAT_UCCD_DOWN = 2, MAT_M3_CCD = 3, MAT_M3_UP = 4, MAT_M3_DOWN = 5, NMAT = 6 };
 
 // Matrix of one kind on a line of N points, spacing dx: interior rows of
 // init_ccd / init_uccd, the closure rows of working_matrix_bc, then TWIN_DEC
@@ -42,7 +45,12 @@ static void ccd_factor(int kind, int N, double dx, std::vector<double> &pk) {
     std::vector<double> A(3 * N), B(3 * N), AA(3 * N), BB(3 * N);
     auto M = [&](std::vector<double> &X, int r, int i) -> double & { return X[i * 3 + r - 1]; };
     const int is = 0, ie = N - 1;
-    const double A1 = 0.875, B1 = 0.1251282341599089, B2 = -0.2487176584009104, B3 = 0.0001282341599089;
+    const bool m3 = kind >= MAT_M3_CCD;
+    if (m3) kind -= MAT_M3_CCD;
+    const double A1 = 0.875;
+    const double B1 = m3 ? 0.125100442885824 : 0.1251282341599089;
+    const double B2 = m3 ? -0.248995571141759 : -0.2487176584009104;
+    const double B3 = m3 ? 0.0001004428858241 : 0.0001282341599089;
     for (int i = is + 1; i <= ie - 1; ++i) {
         if (kind == MAT_CCD) {
             M(A, 1, i) = 7.0 / 16.0; M(A, 2, i) = 1.0; M(A, 3, i) = 7.0 / 16.0;
@@ -99,7 +107,7 @@ int ccd_setup(fcfd_ctx *c) {
             dst = nullptr;
             if (d >= G.dim) continue;
             std::vector<double> pk;
-            ccd_factor(kind, n[d] + 2 * G.g, h[d], pk);
+            ccd_factor(kind, n[d] + (kind >= MAT_M3_CCD ? 0 : 2 * G.g), h[d], pk);
             FCFD_CHECK(hipMalloc((void **)&dst, sizeof(double) * pk.size()));
             FCFD_CHECK(hipMemcpy(dst, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
         }
@@ -120,18 +128,18 @@ constexpr long CCD_LDS_MAX = 64 * 1024;
 template <int DIR, int CH>
 __global__ void __launch_bounds__(CCD_THREADS) k_ccd_lines(const double *__restrict__ f, double *__restrict__ fx,
                                                            double *__restrict__ fxx, const double *__restrict__ cog,
-                                                           Geom G, int in_lds) {
+                                                           Geom G, int in_lds, int ext) {
     extern __shared__ double lds_co[];
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int a, b, N;
     long base, st, nlines;
     double dx;
     if (DIR == 0) {
-        nlines = (long)G.ny * G.nz; N = G.nx + 2 * G.g; st = 1; dx = G.dx;
+        nlines = (long)G.ny * G.nz; N = G.nx + 2 * ext; st = 1; dx = G.dx;
     } else if (DIR == 1) {
-        nlines = (long)G.nx * G.nz; N = G.ny + 2 * G.g; st = G.sx; dx = G.dy;
+        nlines = (long)G.nx * G.nz; N = G.ny + 2 * ext; st = G.sx; dx = G.dy;
     } else {
-        nlines = (long)G.nx * G.ny; N = G.nz + 2 * G.g; st = G.sxy; dx = G.dz;
+        nlines = (long)G.nx * G.ny; N = G.nz + 2 * ext; st = G.sxy; dx = G.dz;
     }
     const double *co = cog;
     if (in_lds) {
@@ -142,13 +150,13 @@ __global__ void __launch_bounds__(CCD_THREADS) k_ccd_lines(const double *__restr
     if (t >= nlines) return;
     if (DIR == 0) {
         a = (int)(t % G.ny); b = (int)(t / G.ny);
-        base = G.id(-G.g, a, b);
+        base = G.id(-ext, a, b);
     } else if (DIR == 1) {
         a = (int)(t % G.nx); b = (int)(t / G.nx);
-        base = G.id(a, -G.g, b);
+        base = G.id(a, -ext, b);
     } else {
         a = (int)(t % G.nx); b = (int)(t / G.nx);
-        base = G.id(a, b, -G.g);
+        base = G.id(a, b, -ext);
     }
     const double *F = f + base;
     double *X = fx + base, *XX = fxx + base;
@@ -253,14 +261,14 @@ static_assert(CCDX_LINES == WAVE && CCDX_LINES % CCDX_CH == 0, "one wave per blo
 
 __global__ void __launch_bounds__(CCDX_LINES) k_ccd_lines_x(const double *__restrict__ f, double *__restrict__ fx,
                                                             double *__restrict__ fxx, const double *__restrict__ co,
-                                                            Geom G) {
+                                                            Geom G, int ext) {
     __shared__ double tF[CCDX_LINES * CCDX_TS], tX[CCDX_LINES * CCDX_TS], tXX[CCDX_LINES * CCDX_TS];
     const int lane = threadIdx.x;
     const long nlines = (long)G.ny * G.nz, L0 = (long)blockIdx.x * CCDX_LINES, t = L0 + lane;
-    const int N = G.nx + 2 * G.g;
+    const int N = G.nx + 2 * ext;
     const bool valid = t < nlines;
     long mybase = 0;
-    if (valid) mybase = G.id(-G.g, (int)(t % G.ny), (int)(t / G.ny));
+    if (valid) mybase = G.id(-ext, (int)(t % G.ny), (int)(t / G.ny));
     // tile element e = r*64 + lane of a cooperative access: line l = e / CX, point u = e % CX
     constexpr int RPI = CCDX_LINES / CCDX_CH;   // lines per wave instruction
     const int lsub = lane / CCDX_CH, usub = lane % CCDX_CH;
@@ -403,27 +411,47 @@ __global__ void __launch_bounds__(CCDX_LINES) k_ccd_lines_x(const double *__rest
     }
 }
 
-int ccd_dir(fcfd_ctx *c, int dir, const double *f, double *fx, double *fxx) {
+// CCD of f along direction dir: set 0 the newOpenMP lines through the ghosts
+// (n + 6 points), set 1 the MPLS_RisingBubble interior lines (n points)
+int ccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, double *fx, double *fxx) {
     const Geom &G = c->G;
+    const int ext = set ? 0 : G.g;
     long nl = dir == 0 ? (long)G.ny * G.nz : (dir == 1 ? (long)G.nx * G.nz : (long)G.nx * G.ny);
-    const int N = (dir == 0 ? G.nx : (dir == 1 ? G.ny : G.nz)) + 2 * G.g;
+    const int N = (dir == 0 ? G.nx : (dir == 1 ? G.ny : G.nz)) + 2 * ext;
     const size_t need = sizeof(double) * NPACK * N;
     const int in_lds = (long)need <= CCD_LDS_MAX;   // else the table is read from global memory
     const size_t lds = in_lds ? need : 0;
     constexpr int CH = 8;
     dim3 grd((unsigned)((nl + CCD_THREADS - 1) / CCD_THREADS));
     ProfScope ps_(c, R_CCD);
-    const double *co = c->ccd_coef[MAT_CCD * 3 + dir];
+    const double *co = c->ccd_coef[(set ? MAT_M3_CCD : MAT_CCD) * 3 + dir];
     if (dir == 0)
         hipLaunchKernelGGL(k_ccd_lines_x, dim3((unsigned)((nl + CCDX_LINES - 1) / CCDX_LINES)), dim3(CCDX_LINES), 0,
-                           c->stream, f, fx, fxx, co, G);
+                           c->stream, f, fx, fxx, co, G, ext);
     else if (dir == 1)
-        hipLaunchKernelGGL((k_ccd_lines<1, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, co, G, in_lds);
+        hipLaunchKernelGGL((k_ccd_lines<1, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, co, G, in_lds,
+                           ext);
     else
-        hipLaunchKernelGGL((k_ccd_lines<2, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, co, G, in_lds);
+        hipLaunchKernelGGL((k_ccd_lines<2, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, fx, fxx, co, G, in_lds,
+                           ext);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
+
+int ccd_dir(fcfd_ctx *c, int dir, const double *f, double *fx, double *fxx) {
+    return ccd_lines_set(c, 0, dir, f, fx, fxx);
+}
+
+// first-derivative source coefficients of the upwind system (the downwind
+// one mirrors them) and the choice at u = 0: newOpenMP takes the downwind
+// value there (ccd_solvers.f90:400-412), MPLS_RisingBubble the upwind one
+// (SCHEME_COMPACT.F90:17-25)
+struct UccdSrc {
+    double c1, c2, c3;
+    int ge;
+};
+constexpr UccdSrc UCCD_N3{-1.93596119008109, 1.99692238016218, -0.06096119008109, 0};
+constexpr UccdSrc UCCD_M3{-1.93629468537011, 1.99758937102742, -0.061294685370110, 1};
 
 // root_solve("uccd") along one line (`.../lib/solver/ccd_solvers.f90:
 // 308-341,400-412`): the up- and down-wind systems share the second-
@@ -437,15 +465,16 @@ __global__ void __launch_bounds__(CCD_THREADS) k_uccd_lines(const double *__rest
                                                             double *__restrict__ fx, double *__restrict__ fxx,
                                                             double *__restrict__ t1, double *__restrict__ t2,
                                                             double *__restrict__ t3, const double *__restrict__ cog_u,
-                                                            const double *__restrict__ cog_d, Geom G, int use_lds) {
+                                                            const double *__restrict__ cog_d, Geom G, int use_lds,
+                                                            int ext, UccdSrc cs) {
     extern __shared__ double lds[];
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long nlines, st;
     int N;
     double dx;
-    if (DIR == 0) { nlines = (long)G.ny * G.nz; N = G.nx + 2 * G.g; st = 1; dx = G.dx; }
-    else if (DIR == 1) { nlines = (long)G.nx * G.nz; N = G.ny + 2 * G.g; st = G.sx; dx = G.dy; }
-    else { nlines = (long)G.nx * G.ny; N = G.nz + 2 * G.g; st = G.sxy; dx = G.dz; }
+    if (DIR == 0) { nlines = (long)G.ny * G.nz; N = G.nx + 2 * ext; st = 1; dx = G.dx; }
+    else if (DIR == 1) { nlines = (long)G.nx * G.nz; N = G.ny + 2 * ext; st = G.sx; dx = G.dy; }
+    else { nlines = (long)G.nx * G.ny; N = G.nz + 2 * ext; st = G.sxy; dx = G.dz; }
     const double *cu = cog_u, *cd = cog_d;
     if (use_lds) {
         for (int e = threadIdx.x; e < N * NPACK; e += CCD_THREADS) {
@@ -458,13 +487,13 @@ __global__ void __launch_bounds__(CCD_THREADS) k_uccd_lines(const double *__rest
     }
     if (t >= nlines) return;
     long base;
-    if (DIR == 0) base = G.id(-G.g, (int)(t % G.ny), (int)(t / G.ny));
-    else if (DIR == 1) base = G.id((int)(t % G.nx), -G.g, (int)(t / G.nx));
-    else base = G.id((int)(t % G.nx), (int)(t / G.nx), -G.g);
+    if (DIR == 0) base = G.id(-ext, (int)(t % G.ny), (int)(t / G.ny));
+    else if (DIR == 1) base = G.id((int)(t % G.nx), -ext, (int)(t / G.nx));
+    else base = G.id((int)(t % G.nx), (int)(t / G.nx), -ext);
     const double *F = f + base, *U = u + base;
     double *X = fx + base, *XX = fxx ? fxx + base : nullptr, *T1 = t1 + base, *T2 = t2 + base, *T3 = t3 + base;
     const double idx = 1.0 / dx, idx2 = idx * idx;
-    const double c3 = -0.06096119008109, c2 = 1.99692238016218, c1 = -1.93596119008109;
+    const double c1 = cs.c1, c2 = cs.c2, c3 = cs.c3;
     // boundary rows (shared by both systems)
     const double f0 = F[0], f1 = F[st], f2 = F[2 * st], f3 = F[3 * st];
     double Su = (-3.5 * f0 + 4.0 * f1 - 0.5 * f2) * idx;
@@ -518,7 +547,7 @@ __global__ void __launch_bounds__(CCD_THREADS) k_uccd_lines(const double *__rest
         Sd = b[PB_BB2D] * Sdn - b[PB_B2D] * SSdn;
         SSd = b[PB_A2D] * SSdn - b[PB_AA2D] * Sdn;
         const long o = (long)ie * st;
-        X[o] = U[o] > 0.0 ? Su : Sd;
+        X[o] = (U[o] > 0.0 || (cs.ge && U[o] == 0.0)) ? Su : Sd;
         if (XX) XX[o] = 0.5 * (SSu + SSd);
     }
     // back substitution of both systems
@@ -544,37 +573,45 @@ __global__ void __launch_bounds__(CCD_THREADS) k_uccd_lines(const double *__rest
                 Sd = b[PB_BB2D] * Sid - b[PB_B2D] * SSid;
                 SSd = b[PB_A2D] * SSid - b[PB_AA2D] * Sid;
                 const long o = (long)i * st;
-                X[o] = uv[q] > 0.0 ? Su : Sd;
+                X[o] = (uv[q] > 0.0 || (cs.ge && uv[q] == 0.0)) ? Su : Sd;
                 if (XX) XX[o] = 0.5 * (SSu + SSd);
             }
         }
     }
 }
 
-// UCCD of f along direction dir on every interior line (ghost ends
-// included), upwind by u; scratch: the context's cx, cy, cz
-int uccd_dir(fcfd_ctx *c, int dir, const double *f, const double *u, double *fx, double *fxx) {
+// UCCD of f along direction dir, upwind by u; set 0: the newOpenMP lines
+// through the ghosts, set 1: the MPLS_RisingBubble interior lines with its
+// M-DRPCCD coefficients; scratch: the context's cx, cy, cz
+int uccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, const double *u, double *fx, double *fxx) {
     const Geom &G = c->G;
+    const int ext = set ? 0 : G.g;
     const long nl = dir == 0 ? (long)G.ny * G.nz : (dir == 1 ? (long)G.nx * G.nz : (long)G.nx * G.ny);
-    const int N = (dir == 0 ? G.nx : (dir == 1 ? G.ny : G.nz)) + 2 * G.g;
+    const int N = (dir == 0 ? G.nx : (dir == 1 ? G.ny : G.nz)) + 2 * ext;
     const size_t need = 2 * sizeof(double) * NPACK * N;
     const int use_lds = (long)need <= CCD_LDS_MAX;
     const size_t lds = use_lds ? need : 0;
     constexpr int CH = 8;
     dim3 grd((unsigned)((nl + CCD_THREADS - 1) / CCD_THREADS));
-    const double *cu = c->ccd_coef[MAT_UCCD_UP * 3 + dir], *cd = c->ccd_coef[MAT_UCCD_DOWN * 3 + dir];
+    const int ku = set ? MAT_M3_UP : MAT_UCCD_UP, kd = set ? MAT_M3_DOWN : MAT_UCCD_DOWN;
+    const double *cu = c->ccd_coef[ku * 3 + dir], *cd = c->ccd_coef[kd * 3 + dir];
+    const UccdSrc cs = set ? UCCD_M3 : UCCD_N3;
     ProfScope ps_(c, R_CCD);
     if (dir == 0)
         hipLaunchKernelGGL((k_uccd_lines<0, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, u, fx, fxx, c->cx, c->cy,
-                           c->cz, cu, cd, G, use_lds);
+                           c->cz, cu, cd, G, use_lds, ext, cs);
     else if (dir == 1)
         hipLaunchKernelGGL((k_uccd_lines<1, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, u, fx, fxx, c->cx, c->cy,
-                           c->cz, cu, cd, G, use_lds);
+                           c->cz, cu, cd, G, use_lds, ext, cs);
     else
         hipLaunchKernelGGL((k_uccd_lines<2, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, u, fx, fxx, c->cx, c->cy,
-                           c->cz, cu, cd, G, use_lds);
+                           c->cz, cu, cd, G, use_lds, ext, cs);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
+}
+
+int uccd_dir(fcfd_ctx *c, int dir, const double *f, const double *u, double *fx, double *fxx) {
+    return uccd_lines_set(c, 0, dir, f, u, fx, fxx);
 }
 
 // surface_norms: the reference solves every line of the ghosted box and then

@@ -278,6 +278,8 @@ static int create_impl(const fcfd_grid *grid, const fcfd_params *params, int nca
         bool zf = f == FCFD_W || f == FCFD_W_OLD || f == FCFD_NW || f == FCFD_SRC_W || f == FCFD_SRC_W_OLD ||
                   f == FCFD_NZ || f == FCFD_NZZ || f == FCFD_NXZ || f == FCFD_NYZ;
         if (zf && G.dim == 2) continue;
+        // the MPLS_RisingBubble fields are allocated by its first call
+        if (f == FCFD_PHI1 || f == FCFD_PHI2 || f == FCFD_RHO_OLD || f == FCFD_MU_OLD) continue;
         rc = alloc_field(&c->f[f], n);
     }
     double **scr[] = {&c->phiB, &c->phiS, &c->l1, &c->l2, &c->s2, &c->s3, &c->s0, &c->cx, &c->cy,
@@ -400,6 +402,8 @@ extern "C" int64_t fcfd_field_size(const fcfd_ctx *c) { return c ? (int64_t)c->G
 
 static double *field_or_null(fcfd_ctx *c, int f) {
     if (!c || f < 0 || f >= FCFD_NFIELDS) return nullptr;
+    const bool m3f = f == FCFD_PHI1 || f == FCFD_PHI2 || f == FCFD_RHO_OLD || f == FCFD_MU_OLD;
+    if (m3f && !c->f[f] && fcfd::m3_fields(c) != FCFD_OK) return nullptr;
     return c->f[f];
 }
 

@@ -104,7 +104,8 @@ struct fcfd_ctx {
     double *cx, *cy, *cz;      // CCD second-derivative scratch / cross-derivative temp
     double *dsu, *dsv, *dsw;   // viscous + body force source (sec_part1..3), once per step
     double *un, *vn, *wn;      // predictor output buffer (ping-pong with u, v, w)
-    double *ccd_coef[9];       // factorised line matrices [kind * 3 + direction]: CCD, UCCD up, UCCD down
+    double *ccd_coef[18];      // factorised line matrices [kind * 3 + direction]: CCD, UCCD up / down on
+                               // ghosted lines, the same on interior lines (MPLS_RisingBubble set)
     double *adj[5];            // advection adjoint: stage states phi1, phi2, adjoints lambda1, lambda2, seed
     float *adv32[4];           // FP32 advection: two stage buffers and the stage sources l1, l2
     double vscale = 1.0;       // factor on NU/NV/NW in the WENO5 advection and its adjoint
@@ -140,6 +141,7 @@ struct fcfd_ctx {
     double *diag_host;         // pinned: result words 0..DIAG_W-1 of case q at [q*DIAG_W]
     unsigned long long *pack;  // device staging for per-case values gathered to the host (16 words per case)
     fcfd_diag diag;
+    fcfd_m3_state m3 = {{0.0, 0.0}, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0e-5, 0, 0};   // MPLS_RisingBubble step
     fcfd::Prof prof;
 };
 
@@ -231,6 +233,9 @@ int velbc(fcfd_ctx *c, double *u, double *v, double *w, int nodal);
 int ls_props(fcfd_ctx *c, const double *phi, double *H, double *D, double *S, double *rho, double *mu);
 int reduce_sums(fcfd_ctx *c, int kind, const double *a, const double *b, const double *cc, const double *d);
 int ccd_dir(fcfd_ctx *c, int dir, const double *f, double *fx, double *fxx);
+int ccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, double *fx, double *fxx);
+int m3_fields(fcfd_ctx *c);
+int uccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, const double *u, double *fx, double *fxx);
 int uccd_dir(fcfd_ctx *c, int dir, const double *f, const double *u, double *fx, double *fxx);
 int ccd_setup(fcfd_ctx *c);
 int surface_norms(fcfd_ctx *c, const double *phi, int full);

@@ -21,7 +21,7 @@ import os
 
 import numpy as np
 
-from .types import FIELD_ID, PPE_FP64, STATUS, Case, FcfdCase, FcfdDiag
+from .types import FIELD_ID, PPE_FP64, STATUS, Case, FcfdCase, FcfdDiag, FcfdM3State
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfcfd.so")
@@ -66,6 +66,11 @@ def lib():
         "fcfd_problem_init": [vp, i32],
         "fcfd_create_batch": [vp, vp, i32, vp, C.POINTER(vp)], "fcfd_set_cases": [vp, vp],
         "fcfd_get_cases": [vp, vp], "fcfd_get_case_diag": [vp, vp, vp, vp], "fcfd_batch_size": [vp],
+        "fcfd_m3_get_state": [vp, vp], "fcfd_m3_set_state": [vp, vp], "fcfd_m3_amurho": [vp],
+        "fcfd_m3_ls_ccd": [vp], "fcfd_m3_ls_rk3_solver": [vp], "fcfd_m3_ls_redistance": [vp],
+        "fcfd_m3_mpls": [vp, dbl], "fcfd_m3_solve_ls": [vp], "fcfd_m3_solve_ns": [vp], "fcfd_m3_find_mass": [vp],
+        "fcfd_m3_store_data": [vp], "fcfd_m3_setup": [vp], "fcfd_m3_step": [vp],
+        "fcfd_ls_mv_noalloc": [vp, vp, i32, vp] + [vp] * 7 + [vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -274,6 +279,62 @@ class Solver:
 
 # parameters that may differ between the cases of one batched context
 CASE_KEYS = ("rho_12", "mu_12", "re", "we", "fr")
+
+
+class M3Solver(Solver):
+    """The MPLS_RisingBubble step (`MPLS_RisingBubble/20190612/`) on one 3D
+    grid: the reference's subroutines under their own names (AMURHO,
+    LS_CCD, LS_RK3_SOLVER, LS_REDISTANCE, MPLS, SOLVE_LS, SOLVE_NS,
+    FIND_MASS, STORE_DATA), setup() for the part of SETUP after the initial
+    level sets are pushed and step() for one pass of its main loop."""
+
+    def state(self) -> dict:
+        st = FcfdM3State()
+        self._call("fcfd_m3_get_state", C.byref(st))
+        return st.as_dict()
+
+    def set_state(self, **kw) -> None:
+        st = FcfdM3State()
+        self._call("fcfd_m3_get_state", C.byref(st))
+        for k, v in kw.items():
+            if k == "bubble_imass":
+                st.bubble_imass[0], st.bubble_imass[1] = v
+            else:
+                setattr(st, k, v)
+        self._call("fcfd_m3_set_state", C.byref(st))
+
+    def amurho(self) -> None:
+        self._call("fcfd_m3_amurho")
+
+    def ls_ccd(self) -> None:
+        self._call("fcfd_m3_ls_ccd")
+
+    def ls_rk3_solver(self) -> None:
+        self._call("fcfd_m3_ls_rk3_solver")
+
+    def ls_redistance(self) -> None:
+        self._call("fcfd_m3_ls_redistance")
+
+    def mpls(self, im: float) -> None:
+        self._call("fcfd_m3_mpls", C.c_double(im))
+
+    def solve_ls(self) -> None:
+        self._call("fcfd_m3_solve_ls")
+
+    def solve_ns(self) -> None:
+        self._call("fcfd_m3_solve_ns")
+
+    def find_mass(self) -> None:
+        self._call("fcfd_m3_find_mass")
+
+    def store_data(self) -> None:
+        self._call("fcfd_m3_store_data")
+
+    def setup(self) -> None:
+        self._call("fcfd_m3_setup")
+
+    def step(self) -> None:
+        self._call("fcfd_m3_step")
 
 
 class BatchSolver(Solver):

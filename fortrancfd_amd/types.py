@@ -22,6 +22,7 @@ FIELDS: List[str] = [
     "src_u", "src_v", "src_w", "src_u_old", "src_v_old", "src_w_old", "p",
     "heavy", "delta", "sign", "rho", "mu", "grad", "nx", "ny", "nz",
     "nxx", "nyy", "nzz", "nxy", "nxz", "nyz", "curv", "ppe_src",
+    "phi1", "phi2", "rho_old", "mu_old",
 ]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 Z_FIELDS = {"w", "w_old", "nw", "src_w", "src_w_old", "nz", "nzz", "nxz", "nyz"}
@@ -67,6 +68,20 @@ class FcfdCase(C.Structure):
         ("rho_12", C.c_double), ("mu_12", C.c_double),
         ("re", C.c_double), ("we", C.c_double), ("fr", C.c_double), ("imass", C.c_double),
     ]
+
+
+class FcfdM3State(C.Structure):
+    """struct fcfd_m3_state: bookkeeping of the MPLS_RisingBubble step."""
+    _fields_ = [
+        ("bubble_imass", C.c_double * 2), ("imass", C.c_double), ("ivol", C.c_double),
+        ("mass", C.c_double), ("vol", C.c_double), ("div", C.c_double), ("l2sum", C.c_double),
+        ("picard_tol", C.c_double), ("rec_mass", C.c_int32), ("picard", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["bubble_imass"] = (self.bubble_imass[0], self.bubble_imass[1])
+        return d
 
 
 class FcfdDiag(C.Structure):

@@ -224,6 +224,53 @@ int fcfd_step(fcfd_ctx *ctx);
  * p%loc%init, ns_ab_setup, ls_mv, imass = mass. */
 int fcfd_problem_init(fcfd_ctx *ctx, int redis0);
 
+/* ---- MPLS_RisingBubble (`MPLS_RisingBubble/20190612/`, 3D, two bubbles) ----
+ * The reference's second code base: each bubble keeps its own level set
+ * (FCFD_PHI1, FCFD_PHI2), advected with the upwind combined compact scheme
+ * (M-DRPCCD coefficients) on interior lines, maintained and corrected to its
+ * own mass, then merged by max; phi > 0 is the bubble (rho = rho_12 H + 1 - H,
+ * AMURHO.F90:18-22).  Parameters come from fcfd_params: rho_12 / mu_12 =
+ * RATIO_RHO / RATIO_AMU, re / we / fr, dt, rdt, ls_wid = INTERFACE_WIDTH,
+ * btn_sf / btn_g = ST_FORCE / G_FORCE, no-slip walls (ubc = vbc = wbc = 1),
+ * ppe_tol / ppe_precision for the pressure solve, which replaces the
+ * reference's SOR (PRESS_POISSON) by the multigrid of fcfd_ppe_solve_rhs on
+ * the same operator.  3D contexts only (FCFD_ERR_ARG otherwise). */
+int fcfd_m3_get_state(fcfd_ctx *ctx, fcfd_m3_state *s);
+int fcfd_m3_set_state(fcfd_ctx *ctx, const fcfd_m3_state *s);
+/* AMURHO (AMURHO.F90:1-46): HEAVY, DELTA, RHO, MU of PHI. */
+int fcfd_m3_amurho(fcfd_ctx *ctx);
+/* LS_CCD (FIND_CURVATURE.F90:99-199): CCD normals NX..NZ, second and cross
+ * derivatives, CURV of PHI. */
+int fcfd_m3_ls_ccd(fcfd_ctx *ctx);
+/* LS_RK3_SOLVER + CELL_FACE_VELOCITY (LS_SOLVER.F90:39-228): SSP-RK3 of PHI
+ * with the UCCD source by the cell velocity of U, V, W (into NU, NV, NW). */
+int fcfd_m3_ls_rk3_solver(fcfd_ctx *ctx);
+/* LS_REDISTANCE(PHI, 1) (LS_REDISTANCE.F90:1-211): pseudo-time steps with the
+ * RIS_MASS constraint until the pseudo time exceeds 2 ls_wid. */
+int fcfd_m3_ls_redistance(fcfd_ctx *ctx);
+/* MPLS(im) (LS_SOLVER.F90:257-321): five corrections of PHI to mass im. */
+int fcfd_m3_mpls(fcfd_ctx *ctx, double im);
+/* SOLVE_LS (LS_SOLVER.F90:1-37): both bubbles advected, maintained every
+ * fifth step (LS_MAINTAIN.F90) and corrected, merged, reinitialised. */
+int fcfd_m3_solve_ls(fcfd_ctx *ctx);
+/* SOLVE_NS (SOLVE_NS.F90:1-49): AMURHO, curvature, up to four Picard
+ * iterations of second-order upwind source + Adams-Bashforth
+ * (SECOND_ORDER_UPWIND.F90), pressure solve, projection, no-slip velocity
+ * ghosts, with the reference's early exit. */
+int fcfd_m3_solve_ns(fcfd_ctx *ctx);
+/* FIND_MASS (FIND_MASS.F90:1-162), level-set sums; the first call sets the
+ * bubble targets from FCFD_PHI1, FCFD_PHI2. */
+int fcfd_m3_find_mass(fcfd_ctx *ctx);
+/* STORE_DATA (STORE_DATA.F90:1-33). */
+int fcfd_m3_store_data(fcfd_ctx *ctx);
+/* the part of SETUP after the initial level sets are pushed (SETUP.F90:
+ * 210-222): ghost copies, reinitialisation, AMURHO, STORE_DATA, the first
+ * momentum source, STORE_DATA, FIND_MASS. */
+int fcfd_m3_setup(fcfd_ctx *ctx);
+/* one pass of the main loop (MAIN.F90:10-26 without output): time, iter,
+ * SOLVE_LS, SOLVE_NS, FIND_MASS, STORE_DATA. */
+int fcfd_m3_step(fcfd_ctx *ctx);
+
 /* ---- built-in kernel timing (no reference counterpart; replaces the
  * reference's system_clock buckets ls_adv / ls_red / ppe / ns,
  * `.../lib/data/branches.f90:36`, printed by print_CPU_info) ---- */

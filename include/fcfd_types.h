@@ -113,6 +113,11 @@ enum fcfd_field {
     FCFD_NXY, FCFD_NXZ, FCFD_NYZ,           /* normals%{xy,xz,yz}%now          */
     FCFD_CURV,           /* normals%curv%now                                   */
     FCFD_PPE_SRC,        /* coe%src (div u* / dt)                              */
+    /* held by the MPLS_RisingBubble step only (allocated on its first call):
+     * the two bubbles' level sets and the old density / viscosity
+     * (`MPLS_RisingBubble/20190612/DATA_MOD.F90:26-31,52`) */
+    FCFD_PHI1, FCFD_PHI2,
+    FCFD_RHO_OLD, FCFD_MU_OLD,
     FCFD_NFIELDS
 };
 
@@ -128,6 +133,21 @@ typedef struct fcfd_diag {
     int32_t redis_iters;         /* pseudo steps of the last reinitialisation  */
     int32_t iter;                /* time step counter (p%glb%iter)             */
 } fcfd_diag;
+
+/* Bookkeeping of the MPLS_RisingBubble step (`MPLS_RisingBubble/20190612/`
+ * DATA_MOD.F90 LS_DATA / PROBLEM_DEF scalars): the per-bubble MPLS targets
+ * B1_iM, B2_iM, the initial totals, the last sums and Picard diagnostics.
+ * fcfd_m3_setup fills it; it can be read and replaced (restart, tests). */
+typedef struct fcfd_m3_state {
+    double bubble_imass[2];      /* B1_iM, B2_iM (FIND_MASS.F90:92-118)        */
+    double imass, ivol;          /* IMASS_LS, IVOL_LS                          */
+    double mass, vol;            /* MASS_LS, VOL_LS of the last FIND_MASS      */
+    double div;                  /* DIV: max |div u| of the last iteration     */
+    double l2sum;                /* L2_U + L2_V + L2_W of the last iteration   */
+    double picard_tol;           /* TOL_M: Picard exit when l2sum < it (1e-5)  */
+    int32_t rec_mass;            /* REC_MASS: 0 until the first FIND_MASS      */
+    int32_t picard;              /* Picard iterations of the last SOLVE_NS     */
+} fcfd_m3_state;
 
 /* Per-case parameters of a batched 2D context (fcfd_create_batch): the
  * physical numbers that differ between the cases of one stack.  Every other

@@ -62,7 +62,7 @@ def lib():
 
 
 def field_id(name: str) -> int:
-    return FIELD_ID[name] if name in FIELD_ID else EXTRA_ID[name]
+    return EXTRA_ID[name] if name in EXTRA_ID else FIELD_ID[name]
 
 
 class Oracle:

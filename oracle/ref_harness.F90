@@ -6,7 +6,7 @@
 ! pulled in through the reference's own `included.f90`).  It writes golden
 ! vectors for the CPU oracle; it is never part of the product.
 !
-! usage: ref_harness <op> <config> <indir> <outdir>
+! usage: ref_harness <op> <config> <indir> <outdir> [nsteps]
 !   (ops plot and output write the reference's files under ./out/)
 !   config  reference-format input file (`.../lib/data/tree.f90:48-106`)
 !   indir   raw little-endian float64 files named after the fields (phi, u ...)
@@ -25,7 +25,9 @@ use all
 implicit none
 character(len=512) :: op, cfg, indir, outdir
 real(8) :: imass_in, time_in
-integer :: iter_in, ios, u
+integer :: iter_in, ios, u, nsteps, istep
+integer(8) :: clk0, clk1, clk_rate
+character(len=32) :: nstr
 
 call get_command_argument(1, op)
 call get_command_argument(2, cfg)
@@ -108,6 +110,25 @@ case ('step')
     call interface_solver
     call harness_ns_solver
     call ls_mv
+case ('steps_timed')
+    ! argument 5 main-loop steps, timed inside the program (no file I/O in
+    ! the timed region) with system_clock; timing.txt gets the seconds, the
+    ! step count and the summed pressure relaxation sweeps
+    call get_command_argument(5, nstr)
+    read(nstr, *) nsteps
+    call system_clock(clk0, clk_rate)
+    do istep = 1, nsteps
+        p%glb%time = p%glb%time + p%glb%dt
+        p%glb%iter = p%glb%iter + 1
+        call p%switch
+        call interface_solver
+        call harness_ns_solver
+        call ls_mv
+    end do
+    call system_clock(clk1)
+    open(newunit=u, file=trim(outdir)//'/timing.txt', status='replace')
+    write(u, '(ES24.16, 2I12)') real(clk1 - clk0, 8) / real(clk_rate, 8), nsteps, p%glb%piter
+    close(u)
 case ('plot')
     ! plot snapshot 0 (plot.f90:10 writes when |time - pid*t2p| <= dt)
     p%glb%pid = 0

@@ -67,3 +67,28 @@ def run(case, op: str, inputs: dict, imass: float = 0.0, time: float = 0.0, it: 
         od = os.path.join(td, "out")
         out["_files"] = {fn: open(os.path.join(od, fn), "rb").read() for fn in sorted(os.listdir(od))}
         return out
+
+
+def run_steps_timed(case, inputs: dict, nsteps: int, imass: float, threads: int, timeout: float = 900.0) -> dict:
+    """nsteps main-loop steps of the compiled reference in one process, timed
+    inside it (system_clock around the loop only, no field I/O); returns
+    {"seconds", "steps", "piter"}."""
+    names = HARNESS_FIELDS_3D if case.dim == 3 else HARNESS_FIELDS_2D
+    with tempfile.TemporaryDirectory(prefix="fcfdref_") as td:
+        for d in ("in", "outf", "out"):
+            os.makedirs(os.path.join(td, d))
+        with open(os.path.join(td, "case.txt"), "w") as fh:
+            fh.write(case.to_reference_input(threads))
+        for k, a in inputs.items():
+            if k not in names:
+                raise KeyError(k)
+            np.ascontiguousarray(a, dtype="<f8").reshape(-1).tofile(os.path.join(td, "in", k))
+        with open(os.path.join(td, "in", "scalars.txt"), "w") as fh:
+            fh.write(f"{imass!r} 0.0 0\n")
+        env = dict(os.environ, OMP_NUM_THREADS=str(int(threads)), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        r = subprocess.run([harness_path(case.dim), "steps_timed", "case.txt", "in", "outf", str(int(nsteps))],
+                           cwd=td, env=env, capture_output=True, text=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference harness failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
+        v = open(os.path.join(td, "outf", "timing.txt")).read().split()
+        return {"seconds": float(v[0]), "steps": int(v[1]), "piter": int(v[2])}

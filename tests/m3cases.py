@@ -95,3 +95,26 @@ def golden_inputs(case: M3Case, bubbles_only: bool = False):
     for k in ("u", "v", "w"):
         inp[k + "_old"] = 0.9 * inp[k]
     return inp, state
+
+
+# oracle array name -> device field name (the reference's arrays on the
+# context fields of the MPLS_RisingBubble step, include/fcfd.h)
+M3_TO_DEV = {
+    "phi": "phi", "phi1": "phi1", "phi2": "phi2", "phi_old": "phi_old", "u": "u", "v": "v", "w": "w",
+    "u_old": "u_old", "v_old": "v_old", "w_old": "w_old", "uh": "nu", "vh": "nv", "wh": "nw",
+    "us": "src_u", "vs": "src_v", "ws": "src_w", "us_old": "src_u_old", "vs_old": "src_v_old",
+    "ws_old": "src_w_old", "rho": "rho", "rho_old": "rho_old", "amu": "mu", "amu_old": "mu_old", "p": "p",
+    "rhs": "ppe_src", "heavy": "heavy", "delta": "delta", "grad": "grad", "sgn": "sign", "normal_x": "nx",
+    "normal_y": "ny", "normal_z": "nz", "curv": "curv",
+}
+
+
+def device_case(case: M3Case):
+    """the fcfd Case of an M3Case (3D, no-slip walls, the reference's
+    INTERFACE_WIDTH = 1.5 dx, RDT = 0.5 dx)."""
+    from fortrancfd_amd.types import Case
+    return Case(dim=3, nx=case.nx, ny=case.ny, nz=case.nz, xend=case.nx * case.dx, yend=case.ny * case.dx,
+                zend=case.nz * case.dx, cfl=case.cfl, cfl_r=0.5, ls_w=1.5, rho_12=case.ratio_rho,
+                mu_12=case.ratio_amu, re=case.re, we=case.we, fr=case.fr, btn_sf=float(case.st_force),
+                btn_g=float(case.g_force), gvec=(0.0, 0.0, -1.0), ppe_tol=case.ppe_tol,
+                ppe_precision=case.ppe_precision, name="m3")
