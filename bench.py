@@ -450,6 +450,65 @@ def batch_bench(ncases: int = 64, steps: int = 500, warmup: int = 2, loop_steps:
             "max_rel_mass_drift": mass_drift}
 
 
+def m3_bench(unit_grid: int, steps: int, warmup: int, precision: int = 1) -> dict:
+    """The MPLS_RisingBubble two-bubble step at unit_grid cells per unit
+    length (35 is the reference's 140 x 140 x 280): setup (reinitialisation,
+    first momentum source, mass targets) untimed, warmup steps, then `steps`
+    main-loop steps timed, then the same steps again with the event regions."""
+    from fortrancfd_amd.solver import M3Solver
+    from fortrancfd_amd.workloads import m3_bubbles, m3_case
+    case = m3_case(unit_grid, precision)
+    s = M3Solver(case)
+    for k, a in m3_bubbles(case).items():
+        s.push(k, a)
+    s.set_time(0.0, 0)
+    t0 = time.perf_counter()
+    s.setup()
+    s.sync()
+    t_setup = time.perf_counter() - t0
+    st0 = s.state()
+    for w in range(warmup):
+        s.step()
+        print(f"m3 warmup step {w + 1}/{warmup} picard {s.state()['picard']}", file=sys.stderr, flush=True)
+    s.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step()
+    s.sync()
+    t1 = time.perf_counter()
+    st = s.state()
+    picard, cycles = [], []
+    s.prof_enable(True)
+    tp0 = time.perf_counter()
+    for _ in range(steps):
+        s.step()
+        picard.append(s.state()["picard"])
+        cycles.append(s.diag()["ppe_cycles_step"])
+    s.sync()
+    tp1 = time.perf_counter()
+    prof = s.prof_read()
+    s.prof_enable(False)
+    regions = {k: {"ms_per_step": round(ms / steps, 3), "launches_per_step": round(cnt / steps, 1)}
+               for k, (ms, cnt) in prof.items() if cnt}
+    s.close()
+    value = case.ncells * steps / (t1 - t0) / 1e6
+    return {
+        "metric": "Mcell-steps/s, MPLS_RisingBubble two-bubble step", "value": round(value, 3),
+        "unit": "Mcell-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round((t1 - t0) * 1e3 / steps, 3), "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic (the reference's initial condition)",
+        "config": {"workload": f"mpls_rising_bubble_{case.nx}x{case.ny}x{case.nz}",
+                   "grid": [case.nx, case.ny, case.nz], "domain": [4.0, 4.0, 8.0],
+                   "ppe": ("fp32 V-cycles in fp64 flexible CG" if precision else "fp64") + " to 1e-10"},
+        "setup_s": round(t_setup, 3),
+        "picard_per_step_profiled_run": picard, "ppe_cycles_per_step_profiled_run": cycles,
+        "mass_drift": {"total": abs(st["mass"] - st0["imass"]) / st0["imass"]},
+        "bubble_imass": list(st0["bubble_imass"]),
+        "profiled_ms_per_step": round((tp1 - tp0) * 1e3 / steps, 3),
+        "regions": regions,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -470,6 +529,7 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="keep the default smoother schedule")
     ap.add_argument("--no-config2", action="store_true", help="skip the 200-step FP64 vs mixed run")
     ap.add_argument("--config2-steps", type=int, default=200)
+    ap.add_argument("--m3", type=int, default=0, help="run the MPLS_RisingBubble step at N cells per unit length")
     ap.add_argument("--krylov", type=int, default=1)
     ap.add_argument("--inner", type=int, default=1)
     args = ap.parse_args()
@@ -482,6 +542,9 @@ def main() -> int:
         return 0
     if args.adjoint:
         print(json.dumps(adjoint_bench(args.adjoint, args.adjoint_steps)))
+        return 0
+    if args.m3:
+        print(json.dumps(m3_bench(args.m3, args.steps, args.warmup, 1 if args.precision == "mixed" else 0)))
         return 0
     if args.batch:
         print(json.dumps(batch_bench(args.batch, args.batch_steps, precision=1 if args.precision == "mixed" else 0,
@@ -499,10 +562,14 @@ def main() -> int:
     for w in range(args.warmup):
         s.step()
         print(f"warmup step {w + 1}/{args.warmup} {s.diag()}", file=sys.stderr, flush=True)
-    tuned = None
+    tuned = tiles = None
     if not args.no_tune:
-        # smoother schedule for this system (untimed; the pressure is restored)
-        from fortrancfd_amd.autotune import tune_ppe_schedule
+        # thread-block tiles of the stencil kernels (untimed steps that
+        # advance the run as any step does), then the smoother schedule for
+        # this system (untimed; the pressure is restored)
+        from fortrancfd_amd.autotune import tune_ppe_schedule, tune_tiles
+        tiles = tune_tiles(s)
+        print(f"autotuned tiles {tiles['best']}", file=sys.stderr, flush=True)
         tuned = tune_ppe_schedule(s)
         print(f"autotuned schedule {tuned['best']} ({tuned['ms']} ms)", file=sys.stderr, flush=True)
         s.step()   # captures the chosen schedule's cycle graphs
@@ -591,6 +658,9 @@ def main() -> int:
     }
     if tuned:
         out["autotune_candidate_ms"] = tuned["ms"]
+    if tiles:
+        out["tiles"] = {k: f"{v[0]}x{v[1]}/{v[2]}" for k, v in tiles["best"].items()}
+        out["tile_candidate_us"] = tiles["us"]
     if not args.no_mg:
         out["mg"] = mg_compare(s, case)
     out["ppe_iters_per_step_profiled_run"] = {"min": int(min(cyc_prof)), "mean": round(float(np.mean(cyc_prof)), 2),

@@ -21,7 +21,11 @@ dJ/dA = <dJ/d(A uhat), uhat>.  The forward keeps the level set of every
 k-th step; the backward restores each segment from its checkpoint, re-runs
 it storing its states, and walks the adjoint steps back through it.
 
-`advect_f32` runs the same forward in FP32 (no gradient).
+`advect` differentiates in the precision of phi0: float64 runs the FP64
+steps and their adjoint; float32 the FP32 steps
+(fcfd_level_set_rk3_solver_f32) and their FP32 adjoint
+(fcfd_ls_adv_adjoint_step_f32), with the velocity and checkpoints in FP32.
+`advect_f32` runs the FP32 forward alone (no gradient).
 """
 from __future__ import annotations
 
@@ -199,19 +203,116 @@ def _function():
     return LevelSetAdvection
 
 
+class _Run32:
+    """FP32 state of one problem: the float32 velocity A uhat lives in
+    caller tensors (the FP32 step takes pointers), the level set in a
+    tensor of the run."""
+
+    def __init__(self, solver: Solver, nsteps: int, period: float, every: int):
+        import torch
+        self.s, self.case = solver, solver.case
+        self.nsteps, self.period, self.every = int(nsteps), float(period), max(1, int(every))
+        self.uhat = [u.to(torch.float32) for u in deformation_field_device(self.case, torch.device("cuda"))]
+        self.vel = [torch.empty_like(a) for a in self.uhat]
+
+    def ct(self, n: int) -> float:
+        return math.cos(math.pi * n * self.case.dt / self.period)
+
+    def set_amplitude(self, A: float) -> None:
+        import torch
+        for dst, src in zip(self.vel, self.uhat):
+            torch.mul(src, A, out=dst)
+        torch.cuda.synchronize()
+
+    def ptrs(self):
+        return [v.data_ptr() for v in self.vel]
+
+    def step(self, n: int, phi) -> None:
+        self.s.set_velocity_scale(self.ct(n))
+        self.s.level_set_rk3_solver_f32(phi.data_ptr(), *self.ptrs())
+
+    def finish(self) -> None:
+        self.s.sync()
+        self.s.set_velocity_scale(1.0)
+
+
+def _function32():
+    import torch
+
+    class LevelSetAdvection32(torch.autograd.Function):
+        """phi_T = advect(phi_0, A) in FP32; see the module docs."""
+
+        @staticmethod
+        def forward(ctx, phi0, A, run: _Run32):
+            a = float(A.item())
+            run.set_amplitude(a)
+            phi = phi0.detach().contiguous().clone()
+            ckpts: List = []
+            torch.cuda.synchronize()
+            for n in range(run.nsteps):
+                if n % run.every == 0:
+                    run.s.sync()
+                    ckpts.append(phi.clone())
+                    torch.cuda.synchronize()
+                run.step(n, phi)
+            run.finish()
+            ctx.run, ctx.ckpts, ctx.A, ctx.A_dtype = run, ckpts, a, A.dtype
+            return phi
+
+        @staticmethod
+        def backward(ctx, g):
+            run, a = ctx.run, ctx.A
+            run.set_amplitude(a)
+            case = run.s.case
+            lam = _fold_ghosts(g.detach().reshape(case.shape), case.ghc).reshape(-1).contiguous()
+            bars = [torch.zeros_like(u) for u in run.uhat]
+            torch.cuda.synchronize()
+            for seg in range(len(ctx.ckpts) - 1, -1, -1):
+                start = seg * run.every
+                end = min(start + run.every, run.nsteps)
+                states = [ctx.ckpts[seg]]
+                cur = ctx.ckpts[seg].clone()
+                torch.cuda.synchronize()
+                for n in range(start, end - 1):
+                    run.step(n, cur)
+                    run.s.sync()
+                    states.append(cur.clone())
+                    torch.cuda.synchronize()
+                for n in range(end - 1, start - 1, -1):
+                    run.s.set_velocity_scale(run.ct(n))
+                    run.s.ls_adv_adjoint_step_f32(states[n - start].data_ptr(), lam.data_ptr(),
+                                                  *(b.data_ptr() for b in bars), *run.ptrs())
+                run.s.sync()
+            run.finish()
+            gA = sum((b.double() * u.double()).sum() for b, u in zip(bars, run.uhat))
+            lam = _zero_ghosts(lam.reshape(case.shape), case.ghc).reshape(-1)
+            return lam, gA.reshape(()).to(ctx.A_dtype), None
+
+    return LevelSetAdvection32
+
+
 _LSA = None
+_LSA32 = None
 
 
 def advect(phi0, A, solver: Solver, nsteps: int, period: float, every: int = 20):
-    """Differentiable advection of phi0 (a CUDA float64 tensor of the ghosted
-    field, flat or shaped) by the deformation field of amplitude A (a 0-d
-    CUDA float64 tensor) over nsteps steps of solver.case.dt; checkpoints
-    every `every` steps.  Returns phi_T with the shape of phi0."""
-    global _LSA
+    """Differentiable advection of phi0 (a CUDA float64 or float32 tensor of
+    the ghosted field, flat or shaped) by the deformation field of amplitude
+    A (a 0-d CUDA tensor) over nsteps steps of solver.case.dt, in phi0's
+    precision; checkpoints every `every` steps.  Returns phi_T with the shape
+    and dtype of phi0."""
+    import torch
+    global _LSA, _LSA32
+    shape = phi0.shape
+    if phi0.dtype == torch.float32:
+        if _LSA32 is None:
+            _LSA32 = _function32()
+        run = _Run32(solver, nsteps, period, every)
+        out = _LSA32.apply(phi0.reshape(-1), A, run)
+        return out.reshape(shape)
     if _LSA is None:
         _LSA = _function()
     run = _Run(solver, nsteps, period, every)
-    shape = phi0.shape
     out = _LSA.apply(phi0.reshape(-1), A, run)
     return out.reshape(shape)
 

@@ -1,4 +1,5 @@
-"""Smoother-schedule autotuner for the pressure solve.
+"""Autotuners: the smoother schedule of the pressure solve and the
+thread-block tiles of the stencil kernels.
 
 The multigrid's cost per solve depends on the smoother schedule (sweeps per
 level, extra sweeps on the small levels, V-cycles per flexible-CG iteration)
@@ -11,6 +12,13 @@ context: fewer, larger launches pay off where the device is underfilled.
 from p = 0 under each candidate and keeps the fastest one that converges.
 The system (operator of the current density, rhs in PPE_SRC) is left as it
 was and the pressure is restored, so tuning does not change the run.
+
+`tune_tiles` picks the thread-block tile of each stencil kernel of the step
+(predictor, viscous / body-force source, PPE source, projection, node
+velocities).  A tile does not change any cell's arithmetic, so the results
+are bit for bit the same under every tile, and the tuner times real steps:
+each candidate runs whole steps with the built-in event regions on, every
+kernel keeps the candidate whose mean launch time was least.
 """
 from __future__ import annotations
 
@@ -83,3 +91,50 @@ def tune_ppe_schedule(solver, candidates=SCHEDULES, reps: int = 2, tol: float = 
         return {"best": dict(candidates[win]), "ms": times}
     solver.set_params(**keep)
     return {"best": keep, "ms": times}
+
+
+# tiles (threads along i, along j, slab rows of the XCD order; 0: whole
+# planes) tried for every stencil kernel
+TILES = [(64, 4, 0), (64, 2, 0), (128, 2, 0), (256, 1, 0), (64, 4, 4), (64, 4, 8), (64, 4, 16), (128, 2, 8),
+         (256, 1, 16)]
+
+
+def tune_tiles(solver, candidates=TILES, steps: int = 1, margin: float = 0.02) -> dict:
+    """Pick, for each tunable stencil kernel, the fastest thread-block tile
+    over `steps` real steps per candidate (the steps advance the run exactly
+    as untuned steps would: a tile changes no result).  A kernel keeps its
+    current tile unless a candidate is at least `margin` faster.  Returns
+    {"best": {region: (tx, ty, slab)}, "us": {region: {"tx x ty / slab": microseconds}}}."""
+    from .types import TILE_KERNELS
+    keep = {k: solver.get_tile(k) for k in TILE_KERNELS}
+    order = [None] + [t for t in candidates]
+    us = {r: {} for r in TILE_KERNELS.values()}
+    for t in order:
+        for k in TILE_KERNELS:
+            solver.set_tile(k, *(keep[k] if t is None else t))
+        solver.prof_enable(True)
+        for _ in range(steps):
+            solver.step()
+        solver.sync()
+        prof = solver.prof_read()
+        solver.prof_enable(False)
+        for k, region in TILE_KERNELS.items():
+            ms, cnt = prof.get(region, (0.0, 0))
+            if cnt:
+                tt = keep[k] if t is None else t
+                key = f"{tt[0]}x{tt[1]}/{tt[2]}"
+                v = round(ms / cnt * 1e3, 3)
+                us[region][key] = min(v, us[region].get(key, v))
+    best = {}
+    for k, region in TILE_KERNELS.items():
+        cur = f"{keep[k][0]}x{keep[k][1]}/{keep[k][2]}"
+        times = us[region]
+        pick = keep[k]
+        if times and cur in times:
+            win = min(times, key=times.get)
+            if times[win] < (1.0 - margin) * times[cur]:
+                a, b = win.split("/")
+                pick = tuple(int(x) for x in a.split("x")) + (int(b),)
+        solver.set_tile(k, *pick)
+        best[region] = pick
+    return {"best": best, "us": us}

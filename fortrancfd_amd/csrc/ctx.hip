@@ -254,6 +254,7 @@ static int create_impl(const fcfd_grid *grid, const fcfd_params *params, int nca
     c->gd = *grid;
     c->pr = *params;
     c->nb = ncases > 0 ? ncases : 1;
+    for (auto &t : c->tile) { t[0] = 64; t[1] = 4; t[2] = 0; }
     if (ncases > 0) c->cases.assign(cases, cases + ncases);
     guard_fp16(c);
     Geom &G = c->G;
@@ -374,6 +375,8 @@ extern "C" int fcfd_destroy(fcfd_ctx *c) {
         if (p) hipFree(p);
     for (auto p : c->adv32)
         if (p) hipFree(p);
+    for (auto p : c->adj32)
+        if (p) hipFree(p);
     if (c->red.host) hipHostFree(c->red.host);
     if (c->diag_host) hipHostFree(c->diag_host);
     if (c->cases_dev) hipFree(c->cases_dev);
@@ -479,6 +482,24 @@ extern "C" int fcfd_set_time(fcfd_ctx *c, double time, int iter) {
     if (!c) return FCFD_ERR_ARG;
     c->time = time;
     c->iter = iter;
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_set_tile(fcfd_ctx *c, int kernel, int tx, int ty, int slab) {
+    if (!c || kernel < 0 || kernel >= FCFD_NTILE) return FCFD_ERR_ARG;
+    if ((tx != 64 && tx != 128 && tx != 256) || ty < 1 || tx * ty > 256) return FCFD_ERR_ARG;
+    if (slab < 0 || slab > 4096) return FCFD_ERR_ARG;
+    c->tile[kernel][0] = tx;
+    c->tile[kernel][1] = ty;
+    c->tile[kernel][2] = slab;
+    return FCFD_OK;
+}
+
+extern "C" int fcfd_get_tile(const fcfd_ctx *c, int kernel, int *tx, int *ty, int *slab) {
+    if (!c || kernel < 0 || kernel >= FCFD_NTILE || !tx || !ty) return FCFD_ERR_ARG;
+    *tx = c->tile[kernel][0];
+    *ty = c->tile[kernel][1];
+    if (slab) *slab = c->tile[kernel][2];
     return FCFD_OK;
 }
 

@@ -36,6 +36,11 @@ struct Geom {
     int dim, nx, ny, nz, g, gz;
     long sx, sxy, ntot;
     double dx, dy, dz;
+    // Ghost depth that holds a neighbour's data on each face (x low / high,
+    // y, z): 0 on a physical face, where kernels read clamped indices as the
+    // reference's zero-gradient copies; up to g on a face between blocks of
+    // a multi-block layout, whose ghosts a halo exchange fills.
+    int ox0 = 0, ox1 = 0, oy0 = 0, oy1 = 0, oz0 = 0, oz1 = 0;
     __host__ __device__ long id(int i, int j, int k) const {
         return (long)(i + g) + (long)(j + g) * sx + (long)(k + gz) * sxy;
     }
@@ -92,6 +97,7 @@ struct fcfd_ctx {
     fcfd_grid gd;
     fcfd_params pr;
     fcfd::Geom G;
+    int tile[FCFD_NTILE][3];   // thread-block tile (x, y) and slab rows per enum fcfd_tile_kernel
     hipStream_t stream;
     int device;
     int ncu;                   // compute units of the device
@@ -108,6 +114,7 @@ struct fcfd_ctx {
                                // ghosted lines, the same on interior lines (MPLS_RisingBubble set)
     double *adj[5];            // advection adjoint: stage states phi1, phi2, adjoints lambda1, lambda2, seed
     float *adv32[4];           // FP32 advection: two stage buffers and the stage sources l1, l2
+    float *adj32[7];           // FP32 adjoint: phi1, phi2, lambda1, lambda2, seed, stage sources l1, l2
     double vscale = 1.0;       // factor on NU/NV/NW in the WENO5 advection and its adjoint
     // ---- batched 2D stack (fcfd_create_batch): case q is the slice k = q ----
     int nb = 1;                        // cases in the stack; 1 for an ordinary context
@@ -189,6 +196,33 @@ __device__ __forceinline__ void xcd_block(int &bx, int &by, int &bz) {
     bz = (int)(r / nby);
 }
 
+// xcd_block with another order of the logical blocks: with slab = S > 0 the
+// tile rows are grouped in slabs of S rows, and an XCD's contiguous range
+// walks the planes of one slab (x fastest, then the slab's rows, then the
+// planes) before the next slab.  A plane's footprint per XCD is then S tile
+// rows instead of the whole plane, so the k-2 .. k+2 planes a z-stencil
+// re-reads are still in the XCD's L2.  slab = 0 is xcd_block's order.
+__device__ __forceinline__ void xcd_block_slab(int &bx, int &by, int &bz, int slab) {
+    const unsigned nbx = gridDim.x, nby = gridDim.y, nbz = gridDim.z, nb = nbx * nby * nbz;
+    const unsigned w = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    const unsigned q = nb / NXCD, rem = nb % NXCD, x = w % NXCD;
+    const unsigned l = x * q + (x < rem ? x : rem) + w / NXCD;
+    if (slab <= 0) {
+        bx = (int)(l % nbx);
+        const unsigned r = l / nbx;
+        by = (int)(r % nby);
+        bz = (int)(r / nby);
+        return;
+    }
+    const unsigned S = (unsigned)slab, per = nbx * S * nbz;
+    const unsigned sl = l / per, rows = min(S, nby - sl * S);
+    const unsigned within = l - sl * per;
+    bx = (int)(within % nbx);
+    const unsigned r = within / nbx;
+    by = (int)(sl * S + r % rows);
+    bz = (int)(r / rows);
+}
+
 // Max over the threads of a 256-thread block (values >= 0 or any order-free
 // max); every thread gets the result; every thread of the block must call it.
 __device__ __forceinline__ double block_max256(double v, double *sh4) {
@@ -217,6 +251,7 @@ template <int DIM>
 __device__ __forceinline__ int case_of(int k) { return DIM == 2 ? k : 0; }
 
 // launch helpers --------------------------------------------------------
+inline dim3 tile_blk(const fcfd_ctx *c, int kernel) { return dim3(c->tile[kernel][0], c->tile[kernel][1]); }
 inline dim3 grid_interior(const Geom &G, dim3 blk) {
     return dim3((G.nx + blk.x - 1) / blk.x, (G.ny + blk.y - 1) / blk.y, G.nz);
 }
@@ -237,6 +272,12 @@ int ccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, double *fx, do
 int m3_fields(fcfd_ctx *c);
 int uccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, const double *u, double *fx, double *fxx);
 int uccd_dir(fcfd_ctx *c, int dir, const double *f, const double *u, double *fx, double *fxx);
+// one WENO5 SSP-RK3 stage of the FP64 advection on the field geometry G
+// (which may carry open faces) with caller pointers, queued on st
+int ls_adv_stage_f32(fcfd_ctx *c, const float *phi, float *phin, const float *nu, const float *nv, const float *nw,
+                     float *l1, float *l2, int stage);
+int ls_adv_stage_geom(hipStream_t st, const Geom &G, const double *phi, double *phin, const double *nu,
+                      const double *nv, const double *nw, double *l1, double *l2, double dt, int stage);
 int ccd_setup(fcfd_ctx *c);
 int surface_norms(fcfd_ctx *c, const double *phi, int full);
 int curv_kernel(fcfd_ctx *c);

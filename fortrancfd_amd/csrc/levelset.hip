@@ -391,6 +391,14 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage_kf(const T *__restrict__ p
                                                          const T *__restrict__ nu, const T *__restrict__ nv,
                                                          const T *__restrict__ nw, T *__restrict__ l1,
                                                          T *__restrict__ l2, Geom G, T dt, T vs, int stage) {
+    // Products are fused into adds only within one statement here: the flux
+    // splits s1, s2 of the z window are formed in the plane that loads them,
+    // and with fusion across statements a face whose splits were formed in
+    // the same plane (the first face of a walk) rounded differently from the
+    // same face formed from carried splits.  A face now has one value
+    // wherever a walk starts, which keeps block-by-block runs bit for bit
+    // equal to whole-field ones.
+#pragma clang fp contract(on)
     constexpr int TX = 64, TY = 4, NFX = TY * (TX + 1), NFY = (TY + 1) * TX;
     __shared__ T fxs[2][TY][TX + 1], fys[2][TY + 1][TX];
     int bx_, by_, bz_;
@@ -403,12 +411,14 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage_kf(const T *__restrict__ p
     const int ic = min(i, G.nx - 1), jc = min(j, G.ny - 1);
     const T half = (T)0.5, EPS = (T)1.0e-13;
     const T hx = (T)G.dx, hy = (T)G.dy, hz = (T)G.dz;
-    // face f + 1/2 of a line: positions clamp(f - 2 + m), m = 0..5, at stride st from the line start a0
-    auto face = [&](const T *__restrict__ vel, long a0, long st, int f, int n) -> T {
+    // face f + 1/2 of a line: positions clamp(f - 2 + m), m = 0..5, into
+    // [-lo, n - 1 + hi] (lo, hi = 0 on physical faces), at stride st from the
+    // line start a0
+    auto face = [&](const T *__restrict__ vel, long a0, long st, int f, int lo, int hi) -> T {
         T s1[6], s2[6];
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
-            const long a = a0 + (long)clampi(f - 2 + m, 0, n - 1) * st;
+            const long a = a0 + (long)clampi(f - 2 + m, lo, hi) * st;
             const T u = vs * vel[a], p = phi[a];
             s1[m] = half * (u + fabs(u)) * p;
             s2[m] = half * (u - fabs(u)) * p;
@@ -420,7 +430,7 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage_kf(const T *__restrict__ p
     if (DIM == 3) {
 #pragma unroll
         for (int m = 0; m < 7; ++m) {
-            const long a = G.id(ic, jc, clampi(k0 - 3 + m, 0, G.nz - 1));
+            const long a = G.id(ic, jc, clampi(k0 - 3 + m, -G.oz0, G.nz - 1 + G.oz1));
             const T u = vs * nw[a], p = phi[a];
             s1z[m] = half * (u + fabs(u)) * p;
             s2z[m] = half * (u - fabs(u)) * p;
@@ -433,10 +443,11 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage_kf(const T *__restrict__ p
         for (int e = t; e < NFX + NFY; e += TX * TY) {
             if (e < NFX) {   // x-face i0 - 1 + c + 1/2 of row j0 + r
                 const int r = e / (TX + 1), c = e - r * (TX + 1);
-                fxs[b][r][c] = face(nu, G.id(0, min(j0 + r, G.ny - 1), k), 1, i0 - 1 + c, G.nx);
+                fxs[b][r][c] = face(nu, G.id(0, min(j0 + r, G.ny - 1), k), 1, i0 - 1 + c, -G.ox0, G.nx - 1 + G.ox1);
             } else {         // y-face j0 - 1 + r + 1/2 of column i0 + c
                 const int q = e - NFX, r = q / TX, c = q - r * TX;
-                fys[b][r][c] = face(nv, G.id(min(i0 + c, G.nx - 1), 0, k), G.sx, j0 - 1 + r, G.ny);
+                fys[b][r][c] = face(nv, G.id(min(i0 + c, G.nx - 1), 0, k), G.sx, j0 - 1 + r, -G.oy0,
+                                    G.ny - 1 + G.oy1);
             }
         }
         __syncthreads();
@@ -470,7 +481,7 @@ __global__ void __launch_bounds__(256) k_ls_adv_stage_kf(const T *__restrict__ p
                 s1z[m] = s1z[m + 1];
                 s2z[m] = s2z[m + 1];
             }
-            const long a = G.id(ic, jc, clampi(k + 4, 0, G.nz - 1));
+            const long a = G.id(ic, jc, clampi(k + 4, -G.oz0, G.nz - 1 + G.oz1));
             const T u = vs * nw[a], p = phi[a];
             s1z[6] = half * (u + fabs(u)) * p;
             s2z[6] = half * (u - fabs(u)) * p;
@@ -529,6 +540,19 @@ int ls_adv_stage(fcfd_ctx *c, const double *phi, double *phin, int stage) {
     return FCFD_OK;
 }
 
+int ls_adv_stage_geom(hipStream_t st, const Geom &G, const double *phi, double *phin, const double *nu,
+                      const double *nv, const double *nw, double *l1, double *l2, double dt, int stage) {
+    dim3 blk(64, 4);
+    if (G.dim == 3)
+        hipLaunchKernelGGL((k_ls_adv_stage_kf<3, double>), grid_km(G, blk), blk, 0, st, phi, phin, nu, nv, nw, l1, l2,
+                           G, dt, 1.0, stage);
+    else
+        hipLaunchKernelGGL((k_ls_adv_stage_kf<2, double>), grid_interior(G, blk), blk, 0, st, phi, phin, nu, nv,
+                           nullptr, l1, l2, G, dt, 1.0, stage);
+    FCFD_CHECK(hipGetLastError());
+    return FCFD_OK;
+}
+
 int ls_advect(fcfd_ctx *c) {
     const Geom &G = c->G;
     dim3 blk(64, 4), grd = grid_interior(G, blk);
@@ -578,6 +602,23 @@ int ls_advect(fcfd_ctx *c) {
     // three stages end in the buffer that was phiB: swap roles
     c->phiB = c->f[FCFD_PHI];
     c->f[FCFD_PHI] = a;
+    return FCFD_OK;
+}
+
+// one FP32 WENO5 SSP-RK3 stage from phi into phin with caller arrays (the
+// velocity times the context's scale); no ghost update
+int ls_adv_stage_f32(fcfd_ctx *c, const float *phi, float *phin, const float *nu, const float *nv, const float *nw,
+                     float *l1, float *l2, int stage) {
+    const Geom &G = c->G;
+    dim3 blk(64, 4);
+    const float dt = (float)c->pr.dt, vs = (float)c->vscale;
+    if (G.dim == 3)
+        hipLaunchKernelGGL((k_ls_adv_stage_kf<3, float>), grid_km(G, blk), blk, 0, c->stream, phi, phin, nu, nv, nw,
+                           l1, l2, G, dt, vs, stage);
+    else
+        hipLaunchKernelGGL((k_ls_adv_stage_kf<2, float>), grid_interior(G, blk), blk, 0, c->stream, phi, phin, nu,
+                           nv, (const float *)nullptr, l1, l2, G, dt, vs, stage);
+    FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
 

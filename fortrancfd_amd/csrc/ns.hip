@@ -49,9 +49,9 @@ __global__ void __launch_bounds__(256) k_diff_source(const double *__restrict__ 
                                                      const double *__restrict__ NYo, const double *__restrict__ NZo,
                                                      double *__restrict__ Tx, double *__restrict__ Ty,
                                                      double *__restrict__ Tz, Geom G, Phys P,
-                                                     const fcfd_case *__restrict__ cp) {
+                                                     const fcfd_case *__restrict__ cp, int slab) {
     int bx_, by_, bz_;
-    xcd_block(bx_, by_, bz_);
+    xcd_block_slab(bx_, by_, bz_, slab);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     if (cp) {   // a batched stack: this slice's case
@@ -70,7 +70,6 @@ __global__ void __launch_bounds__(256) k_diff_source(const double *__restrict__ 
     double hc, dc;
     lsfun2(Po[id], P.w, hc, dc);
     const double rc = hc + r12 * (1.0 - hc), mc = hc + m12 * (1.0 - hc);
-    double out[3];
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
         double hn, dn;
@@ -134,10 +133,8 @@ __global__ void __launch_bounds__(256) k_diff_source(const double *__restrict__ 
         double curv = (Co[id] + Co[id + nb[c]]) / 2.0;
         double ph = 0.5 * (N[c][id] + N[c][id + nb[c]]);
         v = v + P.g[c] * P.btn_g / P.fr - P.btn_sf * curv * delta * ph / (P.we * rho);
-        out[c] = v;
+        T[c][id] = v;
     }
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) T[c][id] = out[c];
 }
 
 // ns_linearize + ns_ab_adv_source_sec + ns_ab_setup + ns_ab_predictor fused:
@@ -153,9 +150,9 @@ __global__ void __launch_bounds__(256) k_predictor(
     const double *__restrict__ Sox, const double *__restrict__ Soy, const double *__restrict__ Soz,
     double *__restrict__ Sx, double *__restrict__ Sy, double *__restrict__ Sz, double *__restrict__ Un,
     double *__restrict__ Vn, double *__restrict__ Wn, Geom G, double dt, int zero_e, int zero_n, int zero_t,
-    int setup_only) {
+    int setup_only, int slab) {
     int bx_, by_, bz_;
-    xcd_block(bx_, by_, bz_);
+    xcd_block_slab(bx_, by_, bz_, slab);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
@@ -211,9 +208,9 @@ __global__ void __launch_bounds__(256) k_predictor(
 // coe%src = div(u*) / dt (ns_ppe_solver.f90:275-287)
 template <int DIM>
 __global__ void k_ppe_src(const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
-                          double *__restrict__ f, Geom G, double dt) {
+                          double *__restrict__ f, Geom G, double dt, int slab) {
     int bx_, by_, bz_;
-    xcd_block(bx_, by_, bz_);
+    xcd_block_slab(bx_, by_, bz_, slab);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
@@ -225,9 +222,10 @@ __global__ void k_ppe_src(const double *__restrict__ U, const double *__restrict
 // ns_momentum_correction (ns_ppe_solver.f90:379-399)
 template <int DIM>
 __global__ void k_projection(double *__restrict__ U, double *__restrict__ V, double *__restrict__ W,
-                             const double *__restrict__ P, const double *__restrict__ R, Geom G, double dt) {
+                             const double *__restrict__ P, const double *__restrict__ R, Geom G, double dt,
+                             int slab) {
     int bx_, by_, bz_;
-    xcd_block(bx_, by_, bz_);
+    xcd_block_slab(bx_, by_, bz_, slab);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
@@ -246,9 +244,10 @@ __global__ void k_projection(double *__restrict__ U, double *__restrict__ V, dou
 // node_vel (ns_solvers.f90:133-143)
 template <int DIM>
 __global__ void k_node_vel(const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
-                           double *__restrict__ NU, double *__restrict__ NV, double *__restrict__ NW, Geom G) {
+                           double *__restrict__ NU, double *__restrict__ NV, double *__restrict__ NW, Geom G,
+                           int slab) {
     int bx_, by_, bz_;
-    xcd_block(bx_, by_, bz_);
+    xcd_block_slab(bx_, by_, bz_, slab);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
@@ -302,17 +301,17 @@ static Phys phys_of(fcfd_ctx *c) {
 int ns_diff_source(fcfd_ctx *c) {
     const Geom &G = c->G;
     double **F = c->f;
-    dim3 blk(64, 4), grd = grid_interior(G, blk);
+    dim3 blk = tile_blk(c, FCFD_TILE_DIFF_SOURCE), grd = grid_interior(G, blk);
     Phys P = phys_of(c);
     ProfScope ps_(c, R_DIFF);
     if (G.dim == 3)
         hipLaunchKernelGGL(k_diff_source<3>, grd, blk, 0, c->stream, F[FCFD_PHI_OLD], F[FCFD_U_OLD], F[FCFD_V_OLD],
                            F[FCFD_W_OLD], F[FCFD_CURV], F[FCFD_NX], F[FCFD_NY], F[FCFD_NZ], c->dsu, c->dsv, c->dsw,
-                           G, P, (const fcfd_case *)nullptr);
+                           G, P, (const fcfd_case *)nullptr, c->tile[FCFD_TILE_DIFF_SOURCE][2]);
     else
         hipLaunchKernelGGL(k_diff_source<2>, grd, blk, 0, c->stream, F[FCFD_PHI_OLD], F[FCFD_U_OLD], F[FCFD_V_OLD],
                            nullptr, F[FCFD_CURV], F[FCFD_NX], F[FCFD_NY], nullptr, c->dsu, c->dsv, nullptr, G, P,
-                           c->cases_dev);
+                           c->cases_dev, c->tile[FCFD_TILE_DIFF_SOURCE][2]);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
@@ -322,7 +321,7 @@ int ns_diff_source(fcfd_ctx *c) {
 int ns_predictor(fcfd_ctx *c, const double *ut, const double *vt, const double *wt) {
     const Geom &G = c->G;
     double **F = c->f;
-    dim3 blk(64, 4), grd = grid_interior(G, blk);
+    dim3 blk = tile_blk(c, FCFD_TILE_PREDICTOR), grd = grid_interior(G, blk);
     int ze = c->pr.ubc[1] == 1 || c->pr.ubc[1] == 2;
     int zn = c->pr.vbc[1] == 1 || c->pr.vbc[1] == 2;
     int zt = c->pr.wbc[1] == 1 || c->pr.wbc[1] == 2;
@@ -332,12 +331,12 @@ int ns_predictor(fcfd_ctx *c, const double *ut, const double *vt, const double *
             hipLaunchKernelGGL(k_predictor<3>, grd, blk, 0, c->stream, ut, vt, wt, F[FCFD_U_OLD], F[FCFD_V_OLD],
                                F[FCFD_W_OLD], c->dsu, c->dsv, c->dsw, F[FCFD_SRC_U_OLD], F[FCFD_SRC_V_OLD],
                                F[FCFD_SRC_W_OLD], F[FCFD_SRC_U], F[FCFD_SRC_V], F[FCFD_SRC_W], c->un, c->vn, c->wn, G,
-                               c->pr.dt, ze, zn, zt, 0);
+                               c->pr.dt, ze, zn, zt, 0, c->tile[FCFD_TILE_PREDICTOR][2]);
         else
             hipLaunchKernelGGL(k_predictor<2>, grd, blk, 0, c->stream, ut, vt, nullptr, F[FCFD_U_OLD], F[FCFD_V_OLD],
                                nullptr, c->dsu, c->dsv, nullptr, F[FCFD_SRC_U_OLD], F[FCFD_SRC_V_OLD], nullptr,
                                F[FCFD_SRC_U], F[FCFD_SRC_V], nullptr, c->un, c->vn, nullptr, G, c->pr.dt, ze, zn, zt,
-                               0);
+                               0, c->tile[FCFD_TILE_PREDICTOR][2]);
         FCFD_CHECK(hipGetLastError());
     }
     // the predicted field becomes vel now; its ghost values follow velbc
@@ -351,7 +350,7 @@ int ns_predictor(fcfd_ctx *c, const double *ut, const double *vt, const double *
 int ns_setup_only(fcfd_ctx *c) {
     const Geom &G = c->G;
     double **F = c->f;
-    dim3 blk(64, 4), grd = grid_interior(G, blk);
+    dim3 blk = tile_blk(c, FCFD_TILE_PREDICTOR), grd = grid_interior(G, blk);
     int ze = c->pr.ubc[1] == 1 || c->pr.ubc[1] == 2;
     int zn = c->pr.vbc[1] == 1 || c->pr.vbc[1] == 2;
     int zt = c->pr.wbc[1] == 1 || c->pr.wbc[1] == 2;
@@ -359,12 +358,12 @@ int ns_setup_only(fcfd_ctx *c) {
         hipLaunchKernelGGL(k_predictor<3>, grd, blk, 0, c->stream, F[FCFD_U], F[FCFD_V], F[FCFD_W], F[FCFD_U_OLD],
                            F[FCFD_V_OLD], F[FCFD_W_OLD], c->dsu, c->dsv, c->dsw, F[FCFD_SRC_U_OLD],
                            F[FCFD_SRC_V_OLD], F[FCFD_SRC_W_OLD], F[FCFD_SRC_U], F[FCFD_SRC_V], F[FCFD_SRC_W],
-                           c->un, c->vn, c->wn, G, c->pr.dt, ze, zn, zt, 1);
+                           c->un, c->vn, c->wn, G, c->pr.dt, ze, zn, zt, 1, c->tile[FCFD_TILE_PREDICTOR][2]);
     else
         hipLaunchKernelGGL(k_predictor<2>, grd, blk, 0, c->stream, F[FCFD_U], F[FCFD_V], nullptr, F[FCFD_U_OLD],
                            F[FCFD_V_OLD], nullptr, c->dsu, c->dsv, nullptr, F[FCFD_SRC_U_OLD], F[FCFD_SRC_V_OLD],
                            nullptr, F[FCFD_SRC_U], F[FCFD_SRC_V], nullptr, c->un, c->vn, nullptr, G, c->pr.dt, ze,
-                           zn, zt, 1);
+                           zn, zt, 1, c->tile[FCFD_TILE_PREDICTOR][2]);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
@@ -372,14 +371,14 @@ int ns_setup_only(fcfd_ctx *c) {
 int ppe_src(fcfd_ctx *c) {
     const Geom &G = c->G;
     double **F = c->f;
-    dim3 blk(64, 4), grd = grid_interior(G, blk);
+    dim3 blk = tile_blk(c, FCFD_TILE_PPE_SRC), grd = grid_interior(G, blk);
     ProfScope ps_(c, R_PPE_SRC);
     if (G.dim == 3)
         hipLaunchKernelGGL(k_ppe_src<3>, grd, blk, 0, c->stream, F[FCFD_U], F[FCFD_V], F[FCFD_W], F[FCFD_PPE_SRC], G,
-                           c->pr.dt);
+                           c->pr.dt, c->tile[FCFD_TILE_PPE_SRC][2]);
     else
         hipLaunchKernelGGL(k_ppe_src<2>, grd, blk, 0, c->stream, F[FCFD_U], F[FCFD_V], nullptr, F[FCFD_PPE_SRC], G,
-                           c->pr.dt);
+                           c->pr.dt, c->tile[FCFD_TILE_PPE_SRC][2]);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
@@ -387,15 +386,15 @@ int ppe_src(fcfd_ctx *c) {
 int projection(fcfd_ctx *c) {
     const Geom &G = c->G;
     double **F = c->f;
-    dim3 blk(64, 4), grd = grid_interior(G, blk);
+    dim3 blk = tile_blk(c, FCFD_TILE_PROJECTION), grd = grid_interior(G, blk);
     {
         ProfScope ps_(c, R_PROJ);
         if (G.dim == 3)
             hipLaunchKernelGGL(k_projection<3>, grd, blk, 0, c->stream, F[FCFD_U], F[FCFD_V], F[FCFD_W], F[FCFD_P],
-                               F[FCFD_RHO], G, c->pr.dt);
+                               F[FCFD_RHO], G, c->pr.dt, c->tile[FCFD_TILE_PROJECTION][2]);
         else
             hipLaunchKernelGGL(k_projection<2>, grd, blk, 0, c->stream, F[FCFD_U], F[FCFD_V], nullptr, F[FCFD_P],
-                               F[FCFD_RHO], G, c->pr.dt);
+                               F[FCFD_RHO], G, c->pr.dt, c->tile[FCFD_TILE_PROJECTION][2]);
         FCFD_CHECK(hipGetLastError());
     }
     return velbc(c, F[FCFD_U], F[FCFD_V], F[FCFD_W], 0);
@@ -404,15 +403,15 @@ int projection(fcfd_ctx *c) {
 int node_vel(fcfd_ctx *c) {
     const Geom &G = c->G;
     double **F = c->f;
-    dim3 blk(64, 4), grd = grid_interior(G, blk);
+    dim3 blk = tile_blk(c, FCFD_TILE_NODE_VEL), grd = grid_interior(G, blk);
     {
         ProfScope ps_(c, R_NODE_VEL);
         if (G.dim == 3)
             hipLaunchKernelGGL(k_node_vel<3>, grd, blk, 0, c->stream, F[FCFD_U], F[FCFD_V], F[FCFD_W], F[FCFD_NU],
-                               F[FCFD_NV], F[FCFD_NW], G);
+                               F[FCFD_NV], F[FCFD_NW], G, c->tile[FCFD_TILE_NODE_VEL][2]);
         else
             hipLaunchKernelGGL(k_node_vel<2>, grd, blk, 0, c->stream, F[FCFD_U], F[FCFD_V], nullptr, F[FCFD_NU],
-                               F[FCFD_NV], nullptr, G);
+                               F[FCFD_NV], nullptr, G, c->tile[FCFD_TILE_NODE_VEL][2]);
         FCFD_CHECK(hipGetLastError());
     }
     return velbc(c, F[FCFD_NU], F[FCFD_NV], F[FCFD_NW], 1);

@@ -115,73 +115,93 @@ __device__ __forceinline__ void weno_weights_adj(double b1, double b2, double b3
     bb3 = 2.0 * e3 * qb3;
 }
 
+// FP32 form of weno_weights_adj for the weights of the FP32 forward,
+// a_k = d_k / q_k (same function, other arithmetic): da_k/db_k = -2 a_k / e_k.
+__device__ __forceinline__ void weno_weights_adj(float b1, float b2, float b3, float EPS, float wb1, float wb2,
+                                                 float wb3, float &bb1, float &bb2, float &bb3) {
+    const float e1 = EPS + b1, e2 = EPS + b2, e3 = EPS + b3;
+    const float a1 = 1.0f / (e1 * e1), a2 = 6.0f / (e2 * e2), a3 = 3.0f / (e3 * e3);
+    const float r = 1.0f / (a1 + a2 + a3);
+    const float rb = wb1 * a1 + wb2 * a2 + wb3 * a3;
+    const float sb = -rb * r * r;
+    const float ab1 = wb1 * r + sb, ab2 = wb2 * r + sb, ab3 = wb3 * r + sb;
+    bb1 = -2.0f * ab1 * a1 / e1;
+    bb2 = -2.0f * ab2 * a2 / e2;
+    bb3 = -2.0f * ab3 * a3 / e3;
+}
+
 // d(weno_left)/dg_m times vb, into gb[0..4]: the value is
 // w1 P1 + w2 P2 + w3 P3 with P1 = (2g0 - 7g1 + 11g2)/6, P2 = (-g1 + 5g2 + 2g3)/6,
 // P3 = (2g2 + 5g3 - g4)/6, the weights from the smoothness indicators b_k.
-__device__ __forceinline__ void weno_left_adj(const double *g, double EPS, double vb, double *gb) {
-    const double t11 = g[0] - 2.0 * g[1] + g[2], t12 = g[0] - 4.0 * g[1] + 3.0 * g[2];
-    const double t21 = g[1] - 2.0 * g[2] + g[3], t22 = g[1] - g[3];
-    const double t31 = g[2] - 2.0 * g[3] + g[4], t32 = 3.0 * g[2] - 4.0 * g[3] + g[4];
-    const double b1 = 13.0 * (t11 * t11) + 3.0 * (t12 * t12);
-    const double b2 = 13.0 * (t21 * t21) + 3.0 * (t22 * t22);
-    const double b3 = 13.0 * (t31 * t31) + 3.0 * (t32 * t32);
-    double w1, w2, w3;
+// T = double or float (the weights of that precision's forward).
+template <typename T>
+__device__ __forceinline__ void weno_left_adj(const T *g, T EPS, T vb, T *gb) {
+    const T c2 = 2, c3 = 3, c4 = 4, c5 = 5, c6 = 6, c7 = 7, c11 = 11, c13 = 13, c26 = 26;
+    const T t11 = g[0] - c2 * g[1] + g[2], t12 = g[0] - c4 * g[1] + c3 * g[2];
+    const T t21 = g[1] - c2 * g[2] + g[3], t22 = g[1] - g[3];
+    const T t31 = g[2] - c2 * g[3] + g[4], t32 = c3 * g[2] - c4 * g[3] + g[4];
+    const T b1 = c13 * (t11 * t11) + c3 * (t12 * t12);
+    const T b2 = c13 * (t21 * t21) + c3 * (t22 * t22);
+    const T b3 = c13 * (t31 * t31) + c3 * (t32 * t32);
+    T w1, w2, w3;
     weno_weights(b1, b2, b3, EPS, w1, w2, w3);
-    constexpr double r6 = 1.0 / 6.0;
-    const double P1 = (2.0 * g[0] - 7.0 * g[1] + 11.0 * g[2]) * r6;
-    const double P2 = (-g[1] + 5.0 * g[2] + 2.0 * g[3]) * r6;
-    const double P3 = (2.0 * g[2] + 5.0 * g[3] - g[4]) * r6;
-    double bb1, bb2, bb3;
+    const T r6 = (T)1 / c6;
+    const T P1 = (c2 * g[0] - c7 * g[1] + c11 * g[2]) * r6;
+    const T P2 = (-g[1] + c5 * g[2] + c2 * g[3]) * r6;
+    const T P3 = (c2 * g[2] + c5 * g[3] - g[4]) * r6;
+    T bb1, bb2, bb3;
     weno_weights_adj(b1, b2, b3, EPS, vb * P1, vb * P2, vb * P3, bb1, bb2, bb3);
-    const double p1 = vb * w1 * r6, p2 = vb * w2 * r6, p3 = vb * w3 * r6;
+    const T p1 = vb * w1 * r6, p2 = vb * w2 * r6, p3 = vb * w3 * r6;
     // through the candidate stencils
-    gb[0] = 2.0 * p1;
-    gb[1] = -7.0 * p1 - p2;
-    gb[2] = 11.0 * p1 + 5.0 * p2 + 2.0 * p3;
-    gb[3] = 2.0 * p2 + 5.0 * p3;
+    gb[0] = c2 * p1;
+    gb[1] = -c7 * p1 - p2;
+    gb[2] = c11 * p1 + c5 * p2 + c2 * p3;
+    gb[3] = c2 * p2 + c5 * p3;
     gb[4] = -p3;
     // through the smoothness indicators
-    const double u11 = 26.0 * t11 * bb1, u12 = 6.0 * t12 * bb1;
-    const double u21 = 26.0 * t21 * bb2, u22 = 6.0 * t22 * bb2;
-    const double u31 = 26.0 * t31 * bb3, u32 = 6.0 * t32 * bb3;
+    const T u11 = c26 * t11 * bb1, u12 = c6 * t12 * bb1;
+    const T u21 = c26 * t21 * bb2, u22 = c6 * t22 * bb2;
+    const T u31 = c26 * t31 * bb3, u32 = c6 * t32 * bb3;
     gb[0] += u11 + u12;
-    gb[1] += -2.0 * u11 - 4.0 * u12 + u21 + u22;
-    gb[2] += u11 + 3.0 * u12 - 2.0 * u21 + u31 + 3.0 * u32;
-    gb[3] += u21 - u22 - 2.0 * u31 - 4.0 * u32;
+    gb[1] += -c2 * u11 - c4 * u12 + u21 + u22;
+    gb[2] += u11 + c3 * u12 - c2 * u21 + u31 + c3 * u32;
+    gb[3] += u21 - u22 - c2 * u31 - c4 * u32;
     gb[4] += u31 + u32;
 }
 
 // d(weno_right)/df_m times vb, into fb[0..4]: the value is
 // w3 R3 + w2 R2 + w1 R1 with R3 = (-f0 + 5f1 + 2f2)/6, R2 = (2f1 + 5f2 - f3)/6,
 // R1 = (11f2 - 7f3 + 2f4)/6; b3 from f0..f2, b2 from f1..f3, b1 from f2..f4.
-__device__ __forceinline__ void weno_right_adj(const double *f, double EPS, double vb, double *fb) {
-    const double t31 = f[0] - 2.0 * f[1] + f[2], t32 = f[0] - 4.0 * f[1] + 3.0 * f[2];
-    const double t21 = f[1] - 2.0 * f[2] + f[3], t22 = f[1] - f[3];
-    const double t11 = f[2] - 2.0 * f[3] + f[4], t12 = 3.0 * f[2] - 4.0 * f[3] + f[4];
-    const double b3 = 13.0 * (t31 * t31) + 3.0 * (t32 * t32);
-    const double b2 = 13.0 * (t21 * t21) + 3.0 * (t22 * t22);
-    const double b1 = 13.0 * (t11 * t11) + 3.0 * (t12 * t12);
-    double w1, w2, w3;
+template <typename T>
+__device__ __forceinline__ void weno_right_adj(const T *f, T EPS, T vb, T *fb) {
+    const T c2 = 2, c3 = 3, c4 = 4, c5 = 5, c6 = 6, c7 = 7, c11 = 11, c13 = 13, c26 = 26;
+    const T t31 = f[0] - c2 * f[1] + f[2], t32 = f[0] - c4 * f[1] + c3 * f[2];
+    const T t21 = f[1] - c2 * f[2] + f[3], t22 = f[1] - f[3];
+    const T t11 = f[2] - c2 * f[3] + f[4], t12 = c3 * f[2] - c4 * f[3] + f[4];
+    const T b3 = c13 * (t31 * t31) + c3 * (t32 * t32);
+    const T b2 = c13 * (t21 * t21) + c3 * (t22 * t22);
+    const T b1 = c13 * (t11 * t11) + c3 * (t12 * t12);
+    T w1, w2, w3;
     weno_weights(b1, b2, b3, EPS, w1, w2, w3);
-    constexpr double r6 = 1.0 / 6.0;
-    const double R3 = (-f[0] + 5.0 * f[1] + 2.0 * f[2]) * r6;
-    const double R2 = (2.0 * f[1] + 5.0 * f[2] - f[3]) * r6;
-    const double R1 = (11.0 * f[2] - 7.0 * f[3] + 2.0 * f[4]) * r6;
-    double bb1, bb2, bb3;
+    const T r6 = (T)1 / c6;
+    const T R3 = (-f[0] + c5 * f[1] + c2 * f[2]) * r6;
+    const T R2 = (c2 * f[1] + c5 * f[2] - f[3]) * r6;
+    const T R1 = (c11 * f[2] - c7 * f[3] + c2 * f[4]) * r6;
+    T bb1, bb2, bb3;
     weno_weights_adj(b1, b2, b3, EPS, vb * R1, vb * R2, vb * R3, bb1, bb2, bb3);
-    const double p1 = vb * w1 * r6, p2 = vb * w2 * r6, p3 = vb * w3 * r6;
+    const T p1 = vb * w1 * r6, p2 = vb * w2 * r6, p3 = vb * w3 * r6;
     fb[0] = -p3;
-    fb[1] = 5.0 * p3 + 2.0 * p2;
-    fb[2] = 2.0 * p3 + 5.0 * p2 + 11.0 * p1;
-    fb[3] = -p2 - 7.0 * p1;
-    fb[4] = 2.0 * p1;
-    const double u31 = 26.0 * t31 * bb3, u32 = 6.0 * t32 * bb3;
-    const double u21 = 26.0 * t21 * bb2, u22 = 6.0 * t22 * bb2;
-    const double u11 = 26.0 * t11 * bb1, u12 = 6.0 * t12 * bb1;
+    fb[1] = c5 * p3 + c2 * p2;
+    fb[2] = c2 * p3 + c5 * p2 + c11 * p1;
+    fb[3] = -p2 - c7 * p1;
+    fb[4] = c2 * p1;
+    const T u31 = c26 * t31 * bb3, u32 = c6 * t32 * bb3;
+    const T u21 = c26 * t21 * bb2, u22 = c6 * t22 * bb2;
+    const T u11 = c26 * t11 * bb1, u12 = c6 * t12 * bb1;
     fb[0] += u31 + u32;
-    fb[1] += -2.0 * u31 - 4.0 * u32 + u21 + u22;
-    fb[2] += u31 + 3.0 * u32 - 2.0 * u21 + u11 + 3.0 * u12;
-    fb[3] += u21 - u22 - 2.0 * u11 - 4.0 * u12;
+    fb[1] += -c2 * u31 - c4 * u32 + u21 + u22;
+    fb[2] += u31 + c3 * u32 - c2 * u21 + u11 + c3 * u12;
+    fb[3] += u21 - u22 - c2 * u11 - c4 * u12;
     fb[4] += u11 + u12;
 }
 

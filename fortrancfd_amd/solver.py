@@ -57,6 +57,7 @@ def lib():
         "fcfd_level_set_rk3_solver": [vp], "fcfd_level_set_rk3_redis": [vp, i32, vp],
         "fcfd_mass_preserving_level_set": [vp], "fcfd_surface_norms": [vp], "fcfd_curv": [vp], "fcfd_uccd": [vp],
         "fcfd_ls_adv_adjoint_step": [vp, vp, vp, vp, vp, vp],
+        "fcfd_ls_adv_adjoint_step_f32": [vp, vp, vp, vp, vp, vp, vp, vp, vp],
         "fcfd_set_velocity_scale": [vp, dbl], "fcfd_level_set_rk3_solver_f32": [vp, vp, vp, vp, vp],
         "fcfd_ppe_sor_init": [vp], "fcfd_ppe_sor_solver": [vp, dbl, i32, vp, vp],
         "fcfd_ppe_solve_rhs": [vp, dbl, i32, vp, vp],
@@ -71,6 +72,13 @@ def lib():
         "fcfd_m3_mpls": [vp, dbl], "fcfd_m3_solve_ls": [vp], "fcfd_m3_solve_ns": [vp], "fcfd_m3_find_mass": [vp],
         "fcfd_m3_store_data": [vp], "fcfd_m3_setup": [vp], "fcfd_m3_step": [vp],
         "fcfd_ls_mv_noalloc": [vp, vp, i32, vp] + [vp] * 7 + [vp],
+        "fcfd_set_tile": [vp, i32, i32, i32, i32], "fcfd_get_tile": [vp, i32, vp, vp, vp],
+        "fcfd_blocks_create": [vp, vp, i32, i32, i32, i32, i32, i32, C.POINTER(vp)],
+        "fcfd_blocks_destroy": [vp], "fcfd_blocks_range": [vp, i32, i32, i32, vp, vp],
+        "fcfd_blocks_scatter": [vp, vp, vp, vp], "fcfd_blocks_gather": [vp, vp, vp, vp],
+        "fcfd_blocks_bc": [vp, vp, vp], "fcfd_blocks_sync": [vp, vp, vp],
+        "fcfd_blocks_check_per": [vp, vp, vp, vp],
+        "fcfd_blocks_level_set_rk3_solver": [vp, vp, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -78,6 +86,8 @@ def lib():
         fn.restype = C.c_int
     L.fcfd_field_size.argtypes = [vp]
     L.fcfd_field_size.restype = C.c_int64
+    L.fcfd_blocks_stack_size.argtypes = [vp]
+    L.fcfd_blocks_stack_size.restype = C.c_int64
     L.fcfd_prof_nregions.argtypes = []
     L.fcfd_prof_nregions.restype = C.c_int
     L.fcfd_prof_region_name.argtypes = [C.c_int]
@@ -170,6 +180,17 @@ class Solver:
         self._call("fcfd_prof_read", ms.ctypes.data, cnt.ctypes.data, n)
         return {lib().fcfd_prof_region_name(r).decode(): (float(ms[r]), int(cnt[r])) for r in range(n)}
 
+    def set_tile(self, kernel: int, tx: int, ty: int, slab: int = 0) -> None:
+        """thread-block tile and XCD slab order of one stencil kernel
+        (types.TILE_KERNELS)"""
+        self._call("fcfd_set_tile", int(kernel), int(tx), int(ty), int(slab))
+
+    def get_tile(self, kernel: int):
+        """(tx, ty, slab)"""
+        tx, ty, sl = C.c_int(), C.c_int(), C.c_int()
+        self._call("fcfd_get_tile", int(kernel), C.byref(tx), C.byref(ty), C.byref(sl))
+        return tx.value, ty.value, sl.value
+
     def set_time(self, time: float, it: int) -> None:
         self._call("fcfd_set_time", C.c_double(time), it)
 
@@ -207,6 +228,15 @@ class Solver:
         layout, float32) with the float32 cell velocity (device pointers)."""
         self._call("fcfd_level_set_rk3_solver_f32", C.c_void_p(phi_ptr), C.c_void_p(nu_ptr), C.c_void_p(nv_ptr),
                    C.c_void_p(nw_ptr or None))
+
+    def ls_adv_adjoint_step_f32(self, phi_ptr: int, lam_ptr: int, ub_ptr: int, vb_ptr: int, wb_ptr: int,
+                                nu_ptr: int, nv_ptr: int, nw_ptr: int = 0) -> None:
+        """FP32 adjoint of one level_set_rk3_solver_f32 step at phi (float32
+        device arrays) with the float32 cell velocity nu, nv, nw times the
+        current scale; u/v/w bars (0: not wanted) accumulate."""
+        self._call("fcfd_ls_adv_adjoint_step_f32", C.c_void_p(phi_ptr), C.c_void_p(lam_ptr),
+                   C.c_void_p(ub_ptr or None), C.c_void_p(vb_ptr or None), C.c_void_p(wb_ptr or None),
+                   C.c_void_p(nu_ptr), C.c_void_p(nv_ptr), C.c_void_p(nw_ptr or None))
 
     def ls_adv_adjoint_step(self, phi_ptr: int, lam_ptr: int, ub_ptr: int = 0, vb_ptr: int = 0,
                             wb_ptr: int = 0) -> None:

@@ -25,6 +25,8 @@ FIELDS: List[str] = [
     "phi1", "phi2", "rho_old", "mu_old",
 ]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+# enum fcfd_tile_kernel -> the built-in profiler region that times it
+TILE_KERNELS = {0: "predictor", 1: "diff_source", 2: "ppe_src", 3: "projection", 4: "node_vel"}
 Z_FIELDS = {"w", "w_old", "nw", "src_w", "src_w_old", "nz", "nzz", "nxz", "nyz"}
 
 FCFD_OK = 0

@@ -1,6 +1,7 @@
 """Synthetic workloads of the benchmark and of the tests that run them at
 their real sizes: the 3D rising bubble, the standalone variable-coefficient
-Poisson problem and the batched 2D rising-bubble cases.  Inputs are
+Poisson problem, the batched 2D rising-bubble cases and the two-bubble
+MPLS_RisingBubble problem.  Inputs are
 deterministic (seeded numpy generators); the fluid starts at rest.
 """
 from __future__ import annotations
@@ -89,3 +90,28 @@ def batch_cases(ncases: int = 64, seed: int = SEED, precision: int = 1, tol: flo
                  ppe_tol=tol, ppe_precision=precision, name=f"batch{q}")
         out.append((c, r))
     return out
+
+
+def m3_case(unit_grid: int, precision: int = 1, tol: float = 1e-10) -> Case:
+    """The MPLS_RisingBubble problem (`MPLS_RisingBubble/20190612/SETUP.F90:14-85`):
+    a [0,4] x [0,4] x [0,8] box with unit_grid cells per unit length (35 in
+    the reference: 140 x 140 x 280), density / viscosity ratio 0.001 / 0.01,
+    Re = 67.27, We = 16, Fr = 1, surface tension and gravity (-z) on,
+    dt = 0.005 dx, rdt = 0.5 dx, interface half-width 1.5 dx, no-slip walls;
+    the pressure to ||r||/||f|| <= tol."""
+    n = int(unit_grid)
+    return Case(dim=3, nx=4 * n, ny=4 * n, nz=8 * n, xend=4.0, yend=4.0, zend=8.0, cfl=0.005, cfl_r=0.5,
+                ls_w=1.5, rho_12=0.001, mu_12=0.01, re=67.27, we=16.0, fr=1.0, btn_sf=1.0, btn_g=1.0,
+                gvec=(0.0, 0.0, -1.0), ppe_tol=tol, ppe_precision=precision, ppe_max_cycles=100,
+                name="mpls_rising_bubble")
+
+
+def m3_bubbles(case: Case) -> dict:
+    """the two bubbles of SETUP.F90:174-182: spheres of radius 0.5 at
+    (2, 2, 1) and (2, 2, 2.5), phi > 0 inside; phi = max(phi1, phi2).  Full
+    ghosted arrays (cell centres (I - 0.5) dx continued into the ghosts)."""
+    X, Y, Z = case.coords()
+    p1 = -np.sqrt((X - 2.0) ** 2 + (Y - 2.0) ** 2 + (Z - 1.0) ** 2) + 0.5
+    p2 = -np.sqrt((X - 2.0) ** 2 + (Y - 2.0) ** 2 + (Z - 2.5) ** 2) + 0.5
+    return {"phi1": np.ascontiguousarray(p1), "phi2": np.ascontiguousarray(p2),
+            "phi": np.ascontiguousarray(np.maximum(p1, p2))}

@@ -65,6 +65,16 @@ int fcfd_get_case_diag(fcfd_ctx *ctx, double *mass, double *vol, double *imass);
 int fcfd_ppe_fp16_density_ok(double rho_min, double rho_max);
 /* Replace the parameters (dt, tolerances ...) of an existing context. */
 int fcfd_set_params(fcfd_ctx *ctx, const fcfd_params *params);
+/* Thread-block tile of one stencil kernel (enum fcfd_tile_kernel): tx
+ * threads along i (64, 128 or 256) times ty along j, tx * ty <= 256, and the
+ * order in which the XCDs take the tiles: slab = 0 plane by plane, slab = S
+ * > 0 in slabs of S tile rows, each XCD walking the planes of its slab (the
+ * z-stencil's neighbour planes then stay in that XCD's L2).  Default 64 x 4,
+ * slab 0.  No counterpart in the reference (its loops are OpenMP over k);
+ * the tile autotuner sets these.  FCFD_ERR_ARG for another shape or a slab
+ * outside 0..4096; slab may be NULL in fcfd_get_tile. */
+int fcfd_set_tile(fcfd_ctx *ctx, int kernel, int tx, int ty, int slab);
+int fcfd_get_tile(const fcfd_ctx *ctx, int kernel, int *tx, int *ty, int *slab);
 /* Number of float64 values of one ghosted field. */
 int64_t fcfd_field_size(const fcfd_ctx *ctx);
 
@@ -151,6 +161,14 @@ int fcfd_level_set_rk3_solver_f32(fcfd_ctx *ctx, float *phi, const float *nu, co
  * Returns FCFD_ERR_ARG for the UCCD scheme. */
 int fcfd_ls_adv_adjoint_step(fcfd_ctx *ctx, const double *phi, double *lambda, double *ubar, double *vbar,
                              double *wbar);
+/* The same adjoint in FP32 for the FP32 step (fcfd_level_set_rk3_solver_f32):
+ * phi, lambda, u/v/wbar as above in float, and the caller's FP32 cell
+ * velocity nu, nv[, nw] of that step (times the context's velocity scale);
+ * its reconstructions use the FP32 forward's weights.  No counterpart in the
+ * reference.  The first call allocates seven FP32 scratch fields in the
+ * context.  FCFD_ERR_ARG for the UCCD scheme. */
+int fcfd_ls_adv_adjoint_step_f32(fcfd_ctx *ctx, const float *phi, float *lambda, float *ubar, float *vbar,
+                                 float *wbar, const float *nu, const float *nv, const float *nw);
 
 /* root_solve("uccd") (`.../lib/solver/ccd_solvers.f90:161-210,308-341,400-412`)
  * of phi along every interior line, upwind by NU/NV/NW (nvel%old): phi_x ->
@@ -270,6 +288,63 @@ int fcfd_m3_setup(fcfd_ctx *ctx);
 /* one pass of the main loop (MAIN.F90:10-26 without output): time, iter,
  * SOLVE_LS, SOLVE_NS, FIND_MASS, STORE_DATA. */
 int fcfd_m3_step(fcfd_ctx *ctx);
+
+/* ---- multi-block layout (`DomainDecomposition/3D/lib/pointer/ptr_roots.f90`,
+ * block ranges of `.../lib/data/branches.f90:84-121`) ----
+ * A grid cut into gx x gy x gz blocks (the reference's grid_x, grid_y,
+ * grid_z), block (bx, by, bz) owning the 1-based ranges
+ * is = bx N / gx + 1 .. ie = (bx + 1) N / gx per axis, each stored as a
+ * ghosted array (ghost width ghc; none in z in 2D) in the column-major
+ * reference layout.  The blocks of one field lie back to back in a caller's
+ * device array of fcfd_blocks_stack_size doubles (a "stack"); a block's
+ * offset is given by fcfd_blocks_range.  Every call that takes a stream only
+ * queues work on it (NULL: the null stream). */
+typedef struct fcfd_blocks fcfd_blocks;
+#define FCFD_BLOCKS_MAX_PER_AXIS 16
+/* ptrpart_init (`ptr_roots.f90:58-99`) with job_init's ranges: the layout of
+ * grid cut into gx x gy x gz blocks (1..FCFD_BLOCKS_MAX_PER_AXIS each; gz = 1
+ * in 2D), periodic along the axes whose flag is set (xper, yper, zper of the
+ * reference); params->dt is the step of fcfd_blocks_level_set_rk3_solver
+ * (params may be NULL otherwise).  FCFD_ERR_ARG when a block is thinner than
+ * the ghost width (the reference stops there too, branches.f90:115-118). */
+int fcfd_blocks_create(const fcfd_grid *grid, const fcfd_params *params, int gx, int gy, int gz, int xper,
+                       int yper, int zper, fcfd_blocks **out);
+int fcfd_blocks_destroy(fcfd_blocks *blocks);
+/* doubles in one stack (-1 for NULL) */
+int64_t fcfd_blocks_stack_size(const fcfd_blocks *blocks);
+/* the 1-based interior ranges is, ie, js, je, ks, ke of block (bx, by, bz)
+ * and the offset of its array in a stack */
+int fcfd_blocks_range(const fcfd_blocks *blocks, int bx, int by, int bz, int32_t range[6], int64_t *offset);
+/* every cell of every block (ghosts included) from the ghosted global field
+ * at the same index / the blocks' interiors back into the global field. */
+int fcfd_blocks_scatter(fcfd_blocks *blocks, const double *field, double *stack, void *stream);
+int fcfd_blocks_gather(fcfd_blocks *blocks, const double *stack, double *field, void *stream);
+/* job_bc(0) (`.../lib/data/branches.f90:240-290`) on every block: the
+ * zero-gradient ghost copy on all six faces of each block. */
+int fcfd_blocks_bc(fcfd_blocks *blocks, double *stack, void *stream);
+/* ptrpart_sync (`ptr_roots.f90:157-320`): the halo exchange.  Every ghost
+ * cell takes the value of the cell of its global index in the block that
+ * owns it (wrapped on periodic axes; outside the domain on another axis, the
+ * boundary block's own ghost).  Face ghosts equal the reference's after one
+ * of its passes; edges and corners equal its settled state (the reference
+ * copies them from neighbours that its loop may not have refreshed yet). */
+int fcfd_blocks_sync(fcfd_blocks *blocks, double *stack, void *stream);
+/* ptrpart_check_periodic (`ptr_roots.f90:322-404`): into *err the sum over
+ * the periodic axes of the squared differences between the periodic ghost
+ * layers and the cells they mirror (0 after fcfd_blocks_sync); synchronises
+ * the stream. */
+int fcfd_blocks_check_per(fcfd_blocks *blocks, const double *stack, double *err, void *stream);
+/* The WENO5 SSP-RK3 step of fcfd_level_set_rk3_solver run block by block as
+ * the multi-block code runs it (`DomainDecomposition/3D/src/interface_solver/
+ * level_set_rk3_solver.f90:1-80`: per-block stage, job_bc(0), sync after
+ * each stage), on stacks: phi in/out, the cell velocity nu, nv[, nw] (ghosts
+ * filled, e.g. by fcfd_blocks_scatter + fcfd_blocks_sync).  Blocks read their
+ * neighbours' data through the ghosts of shared faces and the clamped
+ * zero-gradient values on physical faces, so on a non-periodic layout the
+ * result equals fcfd_level_set_rk3_solver on the whole field bit for bit.
+ * The first call allocates four scratch stacks. */
+int fcfd_blocks_level_set_rk3_solver(fcfd_blocks *blocks, double *phi, const double *nu, const double *nv,
+                                     const double *nw, void *stream);
 
 /* ---- built-in kernel timing (no reference counterpart; replaces the
  * reference's system_clock buckets ls_adv / ls_red / ppe / ns,

@@ -87,6 +87,18 @@ enum fcfd_ls_scheme {
     FCFD_LS_UCCD = 1
 };
 
+/* Stencil kernels whose thread-block tile (x threads along i, y along j;
+ * one plane per block) can be chosen with fcfd_set_tile.  Every tile gives
+ * the same results bit for bit: a cell's arithmetic does not depend on it. */
+enum fcfd_tile_kernel {
+    FCFD_TILE_PREDICTOR = 0,     /* ns_linearize + upwind advection + AB2      */
+    FCFD_TILE_DIFF_SOURCE,       /* sec_part1..3                               */
+    FCFD_TILE_PPE_SRC,           /* div u* / dt                                */
+    FCFD_TILE_PROJECTION,        /* ns_momentum_correction                     */
+    FCFD_TILE_NODE_VEL,          /* node_vel                                   */
+    FCFD_NTILE
+};
+
 enum fcfd_ppe_precision {
     FCFD_PPE_FP64 = 0,           /* FP64 smoothing/transfer, FP64 everywhere   */
     FCFD_PPE_MIXED = 1           /* FP32 V-cycles inside FP64 defect correction*/

@@ -42,6 +42,23 @@ for d in 2D 3D; do
     "$FC" $FFLAGS -cpp $def -module-dir "$b/mod" -I"$b/mod" -I"$src/src" "$here/ref_harness.F90" "${objs[@]}" -o "$out/$exe"
     echo "build_ref: built $out/$exe"
 done
+# the drop-in: the newOpenMP 2D main loop through the fcfd_c binding module
+# (the block printed in INTEGRATION.md) against libfcfd.so
+lib="$here/../fortrancfd_amd/libfcfd.so"
+if [ -f "$lib" ]; then
+    b="$out/build_2D"
+    python3 - "$here/../INTEGRATION.md" "$b/fcfd_c.f90" <<'PYEOF'
+import re, sys
+md = open(sys.argv[1]).read()
+blk = [x for x in re.findall(r"```fortran\n(.*?)```", md, flags=re.S) if x.lstrip().startswith("module fcfd_c")][0]
+open(sys.argv[2], "w").write(blk)
+PYEOF
+    "$FC" $FFLAGS -c -module-dir "$b/mod" "$b/fcfd_c.f90" -o "$b/fcfd_c.o"
+    "$FC" $FFLAGS -cpp -module-dir "$b/mod" -I"$b/mod" "$here/n2_dropin.F90" "$b/fcfd_c.o" "$b/modules.o" \
+        "$b/ir_precision.o" "$b/lib_pack_data.o" "$b/lib_base64.o" "$b/lib_vtk_io.o" \
+        -L"$(dirname "$lib")" -lfcfd -Wl,-rpath,"$(cd "$(dirname "$lib")" && pwd)" -o "$out/n2_dropin"
+    echo "build_ref: built $out/n2_dropin"
+fi
 # MPLS_RisingBubble (3D, two bubbles): its modules, then the harness, which
 # INCLUDEs the reference's internal subroutines from where they lie
 m3="$ref_root/MPLS_RisingBubble/20190612"

@@ -302,3 +302,76 @@ def test_fp32_forward_against_fp64():
     full = p32.reshape(case.shape)
     assert np.array_equal(full[:, :, 0], full[:, :, g])
     assert np.array_equal(full[:, :, -1], full[:, :, -g - 1])
+
+
+def test_fp32_adjoint_step_against_fp64():
+    """The FP32 adjoint of one FP32 step (fcfd_ls_adv_adjoint_step_f32) at a
+    state representable in FP32, against the FP64 adjoint at the same state:
+    the two differ by FP32 rounding of the same derivative."""
+    from fortrancfd_amd.adjoint import deformation_field
+    case = _case()
+    s = _solver(case)
+    rng = np.random.default_rng(7)
+    phi = _phi0(case).astype(np.float32).astype(np.float64)
+    vel = [(0.8 * u).astype(np.float32).astype(np.float64) for u in deformation_field(case)]
+    w = _interior_random(case, rng).astype(np.float32).astype(np.float64)
+    dev = torch.device("cuda")
+    for nm, a in zip(("nu", "nv", "nw"), vel):
+        s.push(nm, a)
+    res = {}
+    for dt_, nm in ((torch.float64, "fp64"), (torch.float32, "fp32")):
+        ph = torch.from_numpy(phi.reshape(-1).copy()).to(dev, dt_)
+        lam = torch.from_numpy(w.reshape(-1).copy()).to(dev, dt_)
+        bars = [torch.zeros(case.size, dtype=dt_, device=dev) for _ in range(3)]
+        vt = [torch.from_numpy(a.reshape(-1).copy()).to(dev, dt_) for a in vel]
+        torch.cuda.synchronize()
+        if dt_ == torch.float64:
+            s.ls_adv_adjoint_step(ph.data_ptr(), lam.data_ptr(), *(b.data_ptr() for b in bars))
+        else:
+            s.ls_adv_adjoint_step_f32(ph.data_ptr(), lam.data_ptr(), *(b.data_ptr() for b in bars),
+                                      *(v.data_ptr() for v in vt))
+        s.sync()
+        res[nm] = [lam.double().cpu().numpy()] + [b.double().cpu().numpy() for b in bars]
+    g = case.ghc
+    inter = lambda a: a.reshape(case.shape)[g:-g, g:-g, g:-g]
+    rel = [float(np.abs(inter(a) - inter(b)).max() / np.abs(inter(b)).max()) for a, b in zip(res["fp32"], res["fp64"])]
+    print("FP32 vs FP64 adjoint step (phi, u, v, w):", rel)
+    assert max(rel) <= 1e-4
+
+
+def test_fp32_gradient_against_fp64():
+    """dJ/dA and dJ/dphi0 (along a smooth direction and as fields) of the
+    FP32 differentiable advection against the FP64 one: 24^3, 30 steps,
+    checkpoints every 7 (uneven last segment)."""
+    from fortrancfd_amd.adjoint import advect, l2_loss
+    case = _case((24, 24, 24))
+    nsteps, period, every = 30, 30 * case.dt, 7
+    phi0 = _phi0(case)
+    dev = torch.device("cuda")
+    with torch.no_grad():
+        target = advect(torch.from_numpy(phi0.reshape(-1).copy()).to(dev),
+                        torch.tensor(1.05, dtype=torch.float64, device=dev), _solver(case), nsteps, period, nsteps)
+    out = {}
+    for dt_ in (torch.float64, torch.float32):
+        p0 = torch.from_numpy(phi0.reshape(-1).copy()).to(dev, dt_).requires_grad_(True)
+        a = torch.tensor(1.0, dtype=dt_, device=dev, requires_grad=True)
+        pT = advect(p0, a, _solver(case), nsteps, period, every)
+        assert pT.dtype == dt_
+        J = l2_loss(pT.double(), target, case)
+        J.backward()
+        out[dt_] = (float(J), p0.grad.double().cpu().numpy().reshape(case.shape), float(a.grad))
+    J64, g64, a64 = out[torch.float64]
+    J32, g32, a32 = out[torch.float32]
+    v = _smooth(case, np.random.default_rng(3), 1.0)
+    d64, d32 = float((g64 * v).sum()), float((g32 * v).sum())
+    g = case.ghc
+    inter = lambda a: a[g:-g, g:-g, g:-g]
+    rel_field = float(np.abs(inter(g32) - inter(g64)).max() / np.abs(inter(g64)).max())
+    print("J", J64, J32, "dA", a64, a32, "dphi.v", d64, d32, "field", rel_field)
+    assert abs(J32 - J64) <= 1e-4 * abs(J64)
+    assert abs(a32 - a64) <= 1e-3 * abs(a64)
+    assert abs(d32 - d64) <= 1e-3 * abs(d64)
+    assert rel_field <= 1e-2
+    m = np.ones(case.shape, dtype=bool)
+    m[g:-g, g:-g, g:-g] = False
+    assert not g32[m].any()
