@@ -168,10 +168,12 @@ def cpu_baseline(steps: int = 5, n: int = 96, threads: int = 0) -> dict:
                       f"{used} OpenMP threads, {dt:.1f} s"}
 
 
-MG_VARIANTS = [  # (name, precision, krylov, V-cycles per outer iteration, pre/post sweeps)
+MG_VARIANTS = [  # (name, precision, krylov, V-cycles per outer iteration, pre/post sweeps[, FP16 density])
     ("fp64_plain", 0, 0, 1, 2, 2), ("fp64_fcg", 0, 1, 1, 2, 2), ("fp64_fcg2", 0, 1, 2, 2, 2),
     ("mixed_plain", 1, 0, 1, 2, 2), ("mixed_plain2", 1, 0, 2, 2, 2), ("mixed_fcg", 1, 1, 1, 2, 2),
     ("mixed_fcg2", 1, 1, 2, 2, 2),
+    # the FP32 cycle with every array in FP32 (no FP16 density / face coefficients)
+    ("mixed_fcg_fp32", 1, 1, 1, 2, 2, 0), ("mixed_plain_fp32", 1, 0, 1, 2, 2, 0),
 ]
 
 
@@ -180,12 +182,13 @@ def mg_compare(s, case, variants=MG_VARIANTS) -> dict:
     of rho now and the rhs in PPE_SRC), every variant from p = 0; the best
     FP64 and the best mixed variant are compared.  Restores the case's own
     solver settings afterwards."""
-    keep = (case.ppe_krylov, case.ppe_inner, case.ppe_nu1, case.ppe_nu2)
+    keep = (case.ppe_krylov, case.ppe_inner, case.ppe_nu1, case.ppe_nu2, case.ppe_rho_half)
     out = {"variants": {}}
     zeros = np.zeros(case.size)
     sols = {}
-    for name, prec, kry, inner, nu1, nu2 in variants:
-        s.set_params(ppe_krylov=kry, ppe_inner=inner, ppe_nu1=nu1, ppe_nu2=nu2)
+    for name, prec, kry, inner, nu1, nu2, *half in variants:
+        s.set_params(ppe_krylov=kry, ppe_inner=inner, ppe_nu1=nu1, ppe_nu2=nu2,
+                     ppe_rho_half=half[0] if half else keep[4])
         best = None
         for rep in range(2):
             s.push("p", zeros)
@@ -197,10 +200,16 @@ def mg_compare(s, case, variants=MG_VARIANTS) -> dict:
             best = ms if best is None else min(best, ms)
         sols[name] = case.interior(s.pull("p")).copy()
         out["variants"][name] = {"ms": round(best, 3), "outer_iters": cyc, "rel_resid": rel}
-    s.set_params(ppe_krylov=keep[0], ppe_inner=keep[1], ppe_nu1=keep[2], ppe_nu2=keep[3])
+    s.set_params(ppe_krylov=keep[0], ppe_inner=keep[1], ppe_nu1=keep[2], ppe_nu2=keep[3], ppe_rho_half=keep[4])
     conv = {k: v for k, v in out["variants"].items() if v["rel_resid"] <= 1e-10}
     f64 = min((k for k in conv if k.startswith("fp64")), key=lambda k: conv[k]["ms"], default=None)
     mix = min((k for k in conv if k.startswith("mixed")), key=lambda k: conv[k]["ms"], default=None)
+    m32 = min((k for k in conv if k.startswith("mixed") and k.endswith("_fp32")), key=lambda k: conv[k]["ms"],
+              default=None)
+    if f64 and m32:
+        # every array of the FP32 cycle in FP32 (the precision the mixed scheme names)
+        out["speedup_fp32_only"] = round(conv[f64]["ms"] / conv[m32]["ms"], 3)
+        out["mixed_fp32_only_best"] = m32
     if f64 and mix:
         a, b = sols[f64], sols[mix]
         out.update({"fp64_best": f64, "fp64_ms": conv[f64]["ms"], "mixed_best": mix, "mixed_ms": conv[mix]["ms"],

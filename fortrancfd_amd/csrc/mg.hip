@@ -430,6 +430,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict_split(const float *__res
     if (I >= C.nx || J >= C.ny) return;
     if (C2) { cx = 2; cy = 2; cz = C2; }
     float acc = 0.f;
+    int nch = 0;
 #pragma unroll
     for (int kk = 0; kk < (C2 ? C2 : cz); ++kk)
 #pragma unroll
@@ -437,10 +438,13 @@ __global__ void __launch_bounds__(256) k_resid_restrict_split(const float *__res
 #pragma unroll
             for (int ii = 0; ii < (C2 ? 2 : cx); ++ii) {
                 const int i = I * cx + ii, j = J * cy + jj, k = K * cz + kk;
+                // the last aggregate of an odd axis has one child
+                if (i >= S.nx || j >= S.ny || (DIM == 3 && k >= S.nz)) continue;
                 acc += f[S.at(i, j, k)] - apply_A_split<DIM, RT>(e, rho, S, ih, i, j, k);
+                ++nch;
             }
     const long cid = C.id(I, J, K);
-    fc[cid] = acc / (float)(cx * cy * cz);
+    fc[cid] = acc / (float)nch;
     ec[cid] = 0.f;
 }
 
@@ -568,6 +572,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const T *__restrict__ e,
     if (I >= C.nx || J >= C.ny) return;
     if (C2) { cx = 2; cy = 2; cz = C2; }
     T acc = 0;
+    int nch = 0;
 #pragma unroll
     for (int kk = 0; kk < (C2 ? C2 : cz); ++kk)
 #pragma unroll
@@ -575,11 +580,14 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const T *__restrict__ e,
 #pragma unroll
             for (int ii = 0; ii < (C2 ? 2 : cx); ++ii) {
                 int i = I * cx + ii, j = J * cy + jj, k = K * cz + kk;
+                // the last aggregate of an odd axis has one child
+                if (i >= L.nx || j >= L.ny || (DIM == 3 && k >= L.nz)) continue;
                 long id = L.id(i, j, k);
                 acc += f[id] - apply_A<T, DIM, FINE, CT>(e, rho, bx, by, bz, L, ih, i, j, k, id);
+                ++nch;
             }
     const long cid = C.id(I, J, K);
-    fc[cid] = acc / (T)(cx * cy * cz);
+    fc[cid] = acc / (T)nch;
     ec[cid] = (T)0;
 }
 
@@ -607,48 +615,61 @@ __global__ void k_coef_restrict(const double *__restrict__ rho, const double *__
     const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
     if (I >= C.nx || J >= C.ny) return;
     const long cid = C.id(I, J, K);
+    // the last aggregate of an odd axis has one child: fine indices past the
+    // level are not children, and each mean runs over the faces that exist
+    // (the +x face of the last coarse cell is the wall either way)
+    const int nz_f = DIM == 3 ? L.nz : 1 << 30;
     // x faces at fine i = I*cx + cx - 1
     {
         double s = 0.0;
+        int m = 0;
         int i = I * cx + cx - 1;
         for (int kk = 0; kk < cz; ++kk)
             for (int jj = 0; jj < cy; ++jj) {
                 int j = J * cy + jj, k = K * cz + kk;
+                if (j >= L.ny || k >= nz_f) continue;
                 long id = L.id(i, j, k);
                 double b;
                 if (FINE) b = i < L.nx - 1 ? 2.0 / (rho[id] + rho[id + 1]) : 0.0;
                 else b = bxf[id];
                 s += b;
+                ++m;
             }
-        bxc[cid] = I < C.nx - 1 ? s / (cy * cz) : 0.0;
+        bxc[cid] = I < C.nx - 1 ? s / m : 0.0;
     }
     {
         double s = 0.0;
+        int m = 0;
         int j = J * cy + cy - 1;
         for (int kk = 0; kk < cz; ++kk)
             for (int ii = 0; ii < cx; ++ii) {
                 int i = I * cx + ii, k = K * cz + kk;
+                if (i >= L.nx || k >= nz_f) continue;
                 long id = L.id(i, j, k);
                 double b;
                 if (FINE) b = j < L.ny - 1 ? 2.0 / (rho[id] + rho[id + L.sx]) : 0.0;
                 else b = byf[id];
                 s += b;
+                ++m;
             }
-        byc[cid] = J < C.ny - 1 ? s / (cx * cz) : 0.0;
+        byc[cid] = J < C.ny - 1 ? s / m : 0.0;
     }
     if (DIM == 3) {
         double s = 0.0;
+        int m = 0;
         int k = K * cz + cz - 1;
         for (int jj = 0; jj < cy; ++jj)
             for (int ii = 0; ii < cx; ++ii) {
                 int i = I * cx + ii, j = J * cy + jj;
+                if (i >= L.nx || j >= L.ny) continue;
                 long id = L.id(i, j, k);
                 double b;
                 if (FINE) b = k < L.nz - 1 ? 2.0 / (rho[id] + rho[id + L.sxy]) : 0.0;
                 else b = bzf[id];
                 s += b;
+                ++m;
             }
-        bzc[cid] = K < C.nz - 1 ? s / (cx * cy) : 0.0;
+        bzc[cid] = K < C.nz - 1 ? s / m : 0.0;
     }
 }
 
@@ -1264,17 +1285,21 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
         const TailLevel<T> &F = A.lv[a];
         const TailLevel<T> &C = A.lv[a + 1];
         for (int s = 0; s < F.nu1; ++s) { gs(F, 0); gs(F, 1); }
-        const int nc = ncells(C.L), nch = F.cx * F.cy * F.cz;
+        const int nc = ncells(C.L);
         for (int cc = t; cc < nc; cc += TAIL_THREADS) {
             int I, J, K;
             cell_coords(C.L, cc, I, J, K);
             T acc = 0;
+            int nch = 0;
             for (int kk = 0; kk < F.cz; ++kk)
                 for (int jj = 0; jj < F.cy; ++jj)
                     for (int ii = 0; ii < F.cx; ++ii) {
                         const int i = I * F.cx + ii, j = J * F.cy + jj, k = K * F.cz + kk;
+                        // the last aggregate of an odd axis has one child (2D: k is the case)
+                        if (i >= F.L.nx || j >= F.L.ny || (DIM == 3 && k >= F.L.nz)) continue;
                         const long id = F.L.id(i, j, k);
                         acc += F.f[id] - apply_A<T, DIM, false, T>(F.e, nullptr, F.bx, F.by, F.bz, F.L, F.ih, i, j, k, id);
+                        ++nch;
                     }
             const long cid = C.L.id(I, J, K);
             C.f[cid] = acc / (T)nch;
@@ -1413,18 +1438,21 @@ static int mg_build(fcfd_ctx *c) {
         const int n[3] = {F.nx, F.ny, F.nz};
         const double h[3] = {F.hx, F.hy, F.hz};
         bool any = false;
+        // an axis of n >= 3 cells pairs its cells into (n + 1) / 2 aggregates;
+        // on an odd axis the last aggregate is a single cell
         for (int d = 0; d < G.dim; ++d)
-            if (n[d] % 2 == 0 && n[d] >= 4 && h[d] <= 1.5 * hmin) { cf[d] = 2; any = true; }
+            if (n[d] >= 3 && h[d] <= 1.5 * hmin) { cf[d] = 2; any = true; }
         if (!any) {
             if (c->mg.size() == 1) {
-                fprintf(stderr, "libfcfd: PPE multigrid needs an even cell count (>= 4) on some axis\n");
+                fprintf(stderr, "libfcfd: PPE multigrid needs at least 3 cells on some axis\n");
                 return FCFD_ERR_ARG;
             }
             break;
         }
         F.cx = cf[0]; F.cy = cf[1]; F.cz = cf[2];
         MgLevel C = {};
-        C.nx = F.nx / cf[0]; C.ny = F.ny / cf[1]; C.nz = G.dim == 3 ? F.nz / cf[2] : F.nz;   // 2D: the cases
+        C.nx = (F.nx + cf[0] - 1) / cf[0]; C.ny = (F.ny + cf[1] - 1) / cf[1];
+        C.nz = G.dim == 3 ? (F.nz + cf[2] - 1) / cf[2] : F.nz;   // 2D: the cases
         C.hx = F.hx * cf[0]; C.hy = F.hy * cf[1]; C.hz = F.hz * cf[2];
         C.sx = C.nx + 2;
         C.sxy = C.sx * (C.ny + 2);
@@ -1688,7 +1716,8 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
             const int lv = lt - 1;
             const MgLevel &F = c->mg[lv];
             const int nch = DIM == 3 ? 8 : 4;
-            const bool full = F.cx == 2 && F.cy == 2 && (DIM == 2 || F.cz == 2);
+            const bool full = F.cx == 2 && F.cy == 2 && (DIM == 2 || F.cz == 2) && F.nx % 2 == 0 &&
+                              F.ny % 2 == 0 && (DIM == 2 || F.nz % 2 == 0);
             if (lt < nl && lv >= std::max(l, 1) && full && ncell_of(lv) <= TAIL_TOP) {
                 const MgLevel &C = c->mg[lv + 1];
                 const long plane = DIM == 3 ? (long)C.nx * C.ny : C.nx;

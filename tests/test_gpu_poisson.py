@@ -96,3 +96,47 @@ def test_poisson_512_both_precisions_reach_1e10():
     d = max_rel(sol[1], sol[0], case)
     print("mixed vs fp64", d)
     assert d <= 1e-8
+
+
+@pytest.mark.parametrize("dims", [(35, 27, 41), (33, 65, 1)], ids=["3d_35x27x41", "2d_33x65"])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_odd_grids_against_the_oracle(dims, precision):
+    """Odd extents on every axis: the multigrid pairs cells into (n + 1) / 2
+    aggregates, the last one a single cell, down to a small coarsest level
+    (an even-only hierarchy stopped at 35 x 27 x 41 itself).  p against the
+    oracle's direct solve to 1e-10, and the cycle count stays that of an
+    even grid of the same size."""
+    from fortrancfd_amd import Solver
+    from fortrancfd_amd.types import Case
+    from fortrancfd_amd.workloads import seeded_rhs, sphere_density
+    from oracle.oracle import Oracle
+    nx, ny, nz = dims
+    h = 1.0 / nx
+    if nz > 1:
+        case = Case(dim=3, nx=nx, ny=ny, nz=nz, xend=1.0, yend=ny * h, zend=nz * h, ppe_max_cycles=200,
+                    name="poisson_odd")
+        rho = sphere_density(case)
+    else:
+        case = Case(dim=2, nx=nx, ny=ny, nz=1, xend=1.0, yend=ny * h, ppe_max_cycles=200, name="poisson_odd")
+        X, Y = case.coords()
+        x = (np.sqrt((X - 0.5) ** 2 + (Y - 0.9) ** 2) - 0.3) / (1.5 * case.dx)
+        H = np.where(x > 1, 1.0, np.where(x < -1, 0.0, 0.5 * (1 + x + np.sin(np.pi * x) / np.pi)))
+        rho = np.ascontiguousarray(1.0 + 999.0 * H)
+    f = seeded_rhs(case)
+    s = Solver(case)
+    s.push("rho", rho)
+    s.push("ppe_src", f)
+    s.ppe_sor_init()
+    cyc, rel = s.ppe_solve_rhs(1e-13, precision)
+    print(dims, "device cycles", cyc, "rel", rel)
+    assert rel <= 1e-12
+    assert cyc <= 30
+    Oracle.threads(16)
+    o = Oracle(case)
+    o.set("rho", rho)
+    o.call("ppe_sor_init")
+    o.set("ppe_src", f)
+    o.call("ppe_solve_rhs", 1e-14)
+    r = max_rel(s.pull("p"), o.get("p"), case)
+    print("p vs oracle", r)
+    assert r <= 1e-10
