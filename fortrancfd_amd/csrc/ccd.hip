@@ -39,11 +39,22 @@ enum { PF_A1, PF_B1, PF_AA1, PF_BB1, PB_A3, PB_B3, PB_AA3, PB_BB3, PB_A2D, PB_AA
 enum { MAT_CCD = 0, MAT_UCCD_UP = 1, MAT_UCCD_DOWN = 2, MAT_M3_CCD = 3, MAT_M3_UP = 4, MAT_M3_DOWN = 5, NMAT = 6 };
 
 // Matrix of one kind on a line of N points, spacing dx: interior rows of
-// init_ccd / init_uccd, the closure rows of working_matrix_bc, then TWIN_DEC
-// (`.../lib/solver/ccd_solvers.f90:76-210`, `matrix_solver.f90:6-48`), packed.
-static void ccd_factor(int kind, int N, double dx, std::vector<double> &pk) {
-    std::vector<double> A(3 * N), B(3 * N), AA(3 * N), BB(3 * N);
+// init_ccd / init_uccd and the closure rows of working_matrix_bc
+// (`.../lib/solver/ccd_solvers.f90:76-210`).  Row i: eq. 1 (first
+// derivative) A(r, i) f'(i-2+r) + B(r, i) f''(i-2+r), eq. 2 (second
+// derivative) AA(r, i) f'(.) + BB(r, i) f''(.), r = 1 sub, 2 diagonal, 3 super.
+struct LineMat {
+    int N;
+    std::vector<double> A, B, AA, BB;
+    double &M(std::vector<double> &X, int r, int i) { return X[i * 3 + r - 1]; }
+};
+
+static LineMat ccd_matrix(int kind, int N, double dx) {
+    LineMat L;
+    L.N = N;
+    L.A.assign(3 * N, 0.0); L.B.assign(3 * N, 0.0); L.AA.assign(3 * N, 0.0); L.BB.assign(3 * N, 0.0);
     auto M = [&](std::vector<double> &X, int r, int i) -> double & { return X[i * 3 + r - 1]; };
+    std::vector<double> &A = L.A, &B = L.B, &AA = L.AA, &BB = L.BB;
     const int is = 0, ie = N - 1;
     const bool m3 = kind >= MAT_M3_CCD;
     if (m3) kind -= MAT_M3_CCD;
@@ -71,7 +82,21 @@ static void ccd_factor(int kind, int N, double dx, std::vector<double> &pk) {
     M(A, 1, ie) = 2.0; M(B, 1, ie) = dx; M(AA, 1, ie) = 2.5 / dx; M(BB, 1, ie) = 8.5;
     M(A, 1, is) = 0.0; M(AA, 1, is) = 0.0; M(A, 3, ie) = 0.0; M(AA, 3, ie) = 0.0;
     M(B, 1, is) = 0.0; M(BB, 1, is) = 0.0; M(B, 3, ie) = 0.0; M(BB, 3, ie) = 0.0;
-    for (int i = is + 1; i <= ie; ++i) {
+    return L;
+}
+
+// TWIN_DEC (`.../lib/solver/matrix_solver.f90:6-48`) in place, packed per
+// point; with seg > 0 the elimination restarts at every multiple of seg (the
+// factors of each seg-point segment alone, no coupling to the previous one).
+static void ccd_pack(LineMat L, int seg, std::vector<double> &pk) {
+    const int N = L.N;
+    auto M = [&](std::vector<double> &X, int r, int i) -> double & { return X[i * 3 + r - 1]; };
+    std::vector<double> &A = L.A, &B = L.B, &AA = L.AA, &BB = L.BB;
+    for (int i = 1; i < N; ++i) {
+        if (seg > 0 && i % seg == 0) {   // segment start: no elimination
+            M(A, 1, i) = 0.0; M(B, 1, i) = 0.0; M(AA, 1, i) = 0.0; M(BB, 1, i) = 0.0;
+            continue;
+        }
         double Den = M(A, 2, i - 1) * M(BB, 2, i - 1) - M(AA, 2, i - 1) * M(B, 2, i - 1);
         double SC1 = -(M(AA, 2, i - 1) * M(B, 1, i) - M(A, 1, i) * M(BB, 2, i - 1)) / Den;
         double SC2 = (M(B, 1, i) * M(A, 2, i - 1) - M(B, 2, i - 1) * M(A, 1, i)) / Den;
@@ -95,6 +120,160 @@ static void ccd_factor(int kind, int N, double dx, std::vector<double> &pk) {
         q[PB_A2D] = M(A, 2, i) / D; q[PB_AA2D] = M(AA, 2, i) / D;
         q[PB_B2D] = M(B, 2, i) / D; q[PB_BB2D] = M(BB, 2, i) / D;
     }
+}
+
+static void ccd_factor(int kind, int N, double dx, std::vector<double> &pk) {
+    ccd_pack(ccd_matrix(kind, N, dx), 0, pk);
+}
+
+// ---- partitioned line solve (y and z lines) ----------------------------
+// A line of N points is cut into segments of SEG points.  With x0 the
+// solution of each segment's own system (its rows without the couplings to
+// the neighbouring segments), the line's solution is
+//   x(i) = x0(i) - V(i) x(a-1) - W(i) x(b+1)      (i in segment [a, b]),
+// V, W the 2 x 2 "spikes" M_s^-1 (coupling block), constant for the grid.
+// The values x(b_s), x(a_{s+1}) at the P - 1 segment interfaces solve a
+// block-tridiagonal system with 4 x 4 blocks, also constant: its block LU is
+// precomputed, so a line's reduced solve is two short sweeps.  Kernels:
+// (1) per segment: x0 at its two ends; (2) per line: the reduced solve;
+// (3) per segment: x0 again, corrected by the spikes, written out.  f is
+// read twice and fx, fxx written once, and every line has P independent
+// threads for the sweeps.
+constexpr int SEG = 16;
+// spike table per point: V(2 x 2) then W(2 x 2), rows (f', f''), columns the
+// (f', f'') of x(a-1) / x(b+1)
+constexpr int NSP = 8;
+// reduced block LU per interface: multiplier m (4 x 4), inverse of the
+// modified diagonal Dinv (4 x 4), super block U (4 x 4)
+constexpr int NRED = 48;
+
+// the local solve of segment [a, b] with the packed local factors for a
+// right side given per point (r1, r2), as the kernels do it
+static void seg_solve_host(const std::vector<double> &lt, int a, int b, const double *r, double *x) {
+    const int n = b - a + 1;
+    std::vector<double> S(n), SS(n);
+    double s = 0.0, ss = 0.0;
+    for (int q = 0; q < n; ++q) {
+        const double *c = &lt[(size_t)(a + q) * NPACK];
+        const double Sn = r[2 * q] - (c[PF_A1] * s + c[PF_B1] * ss);
+        const double SSn = r[2 * q + 1] - (c[PF_AA1] * s + c[PF_BB1] * ss);
+        s = Sn; ss = SSn;
+        S[q] = s; SS[q] = ss;
+    }
+    double xs = 0.0, xss = 0.0;
+    for (int q = n - 1; q >= 0; --q) {
+        const double *c = &lt[(size_t)(a + q) * NPACK];
+        const double Si = S[q] - (c[PB_A3] * xs + c[PB_B3] * xss);
+        const double SSi = SS[q] - (c[PB_AA3] * xs + c[PB_BB3] * xss);
+        xs = c[PB_BB2D] * Si - c[PB_B2D] * SSi;
+        xss = c[PB_A2D] * SSi - c[PB_AA2D] * Si;
+        x[2 * q] = xs; x[2 * q + 1] = xss;
+    }
+}
+
+// 4 x 4 inverse by Gauss-Jordan with partial pivoting (host)
+static void inv4(const double *m, double *out) {
+    double a[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 8; ++c) a[r][c] = c < 4 ? m[r * 4 + c] : (c - 4 == r ? 1.0 : 0.0);
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (fabs(a[r][c]) > fabs(a[p][c])) p = r;
+        for (int k = 0; k < 8; ++k) std::swap(a[c][k], a[p][k]);
+        const double d = a[c][c];
+        for (int k = 0; k < 8; ++k) a[c][k] /= d;
+        for (int r = 0; r < 4; ++r)
+            if (r != c) {
+                const double f = a[r][c];
+                for (int k = 0; k < 8; ++k) a[r][k] -= f * a[c][k];
+            }
+    }
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) out[r * 4 + c] = a[r][c + 4];
+}
+
+static void mul4(const double *a, const double *b, double *c) {
+    for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 4; ++k) {
+            double s = 0.0;
+            for (int j = 0; j < 4; ++j) s += a[r * 4 + j] * b[j * 4 + k];
+            c[r * 4 + k] = s;
+        }
+}
+
+// local factors, spikes and reduced block LU of one line matrix, in one
+// buffer: [lt: N x NPACK][sp: N x NSP][red: (P - 1) x NRED]
+static void ccd_seg_tables(int kind, int N, double dx, std::vector<double> &out) {
+    const LineMat M0 = ccd_matrix(kind, N, dx);
+    std::vector<double> lt;
+    ccd_pack(M0, SEG, lt);
+    const int P = (N + SEG - 1) / SEG;
+    std::vector<double> sp((size_t)N * NSP, 0.0);
+    auto Mv = [&](const std::vector<double> &X, int r, int i) { return X[i * 3 + r - 1]; };
+    std::vector<double> r(2 * SEG), x(2 * SEG);
+    for (int s = 0; s < P; ++s) {
+        const int a = s * SEG, b = std::min(a + SEG, N) - 1, n = b - a + 1;
+        for (int col = 0; col < 2; ++col) {
+            if (a > 0) {   // V column: coupling of row a to x(a-1)
+                std::fill(r.begin(), r.end(), 0.0);
+                r[0] = col == 0 ? Mv(M0.A, 1, a) : Mv(M0.B, 1, a);
+                r[1] = col == 0 ? Mv(M0.AA, 1, a) : Mv(M0.BB, 1, a);
+                seg_solve_host(lt, a, b, r.data(), x.data());
+                for (int q = 0; q < n; ++q) {
+                    sp[(size_t)(a + q) * NSP + 0 + col] = x[2 * q];       // V row f'
+                    sp[(size_t)(a + q) * NSP + 2 + col] = x[2 * q + 1];   // V row f''
+                }
+            }
+            if (b < N - 1) {   // W column: coupling of row b to x(b+1)
+                std::fill(r.begin(), r.end(), 0.0);
+                r[2 * (n - 1)] = col == 0 ? Mv(M0.A, 3, b) : Mv(M0.B, 3, b);
+                r[2 * (n - 1) + 1] = col == 0 ? Mv(M0.AA, 3, b) : Mv(M0.BB, 3, b);
+                seg_solve_host(lt, a, b, r.data(), x.data());
+                for (int q = 0; q < n; ++q) {
+                    sp[(size_t)(a + q) * NSP + 4 + col] = x[2 * q];
+                    sp[(size_t)(a + q) * NSP + 6 + col] = x[2 * q + 1];
+                }
+            }
+        }
+    }
+    // reduced system over interfaces s = 0 .. P-2, u_s = (x(b_s), x(a_{s+1})):
+    //   x(b_s)     + V_s(b_s) x(b_{s-1})     + W_s(b_s) x(a_{s+1})     = x0(b_s)
+    //   x(a_{s+1}) + V_{s+1}(a_{s+1}) x(b_s) + W_{s+1}(a_{s+1}) x(a_{s+2}) = x0(a_{s+1})
+    const int NI = P - 1;
+    std::vector<double> red((size_t)std::max(NI, 0) * NRED, 0.0);
+    std::vector<double> Dp(16), Dinv_prev(16), Uprev(16);
+    for (int s = 0; s < NI; ++s) {
+        const int bs = std::min((s + 1) * SEG, N) - 1, an = (s + 1) * SEG;
+        const double *Vb = &sp[(size_t)bs * NSP], *Wb = Vb + 4;
+        const double *Va = &sp[(size_t)an * NSP], *Wa = Va + 4;
+        double D[16] = {0}, L[16] = {0}, U[16] = {0};
+        for (int k = 0; k < 4; ++k) D[k * 4 + k] = 1.0;
+        // rows 0-1: x(b_s); columns 2-3 of u_s: x(a_{s+1}) by W_s(b_s)
+        D[0 * 4 + 2] = Wb[0]; D[0 * 4 + 3] = Wb[1]; D[1 * 4 + 2] = Wb[2]; D[1 * 4 + 3] = Wb[3];
+        // rows 2-3: x(a_{s+1}); columns 0-1 of u_s: x(b_s) by V_{s+1}(a_{s+1})
+        D[2 * 4 + 0] = Va[0]; D[2 * 4 + 1] = Va[1]; D[3 * 4 + 0] = Va[2]; D[3 * 4 + 1] = Va[3];
+        // L: rows 0-1 couple to the first half of u_{s-1} (x(b_{s-1})) by V_s(b_s)
+        if (s > 0) { L[0] = Vb[0]; L[1] = Vb[1]; L[4] = Vb[2]; L[5] = Vb[3]; }
+        // U: rows 2-3 couple to the second half of u_{s+1} (x(a_{s+2})) by W_{s+1}(a_{s+1})
+        if (s < NI - 1) { U[2 * 4 + 2] = Wa[0]; U[2 * 4 + 3] = Wa[1]; U[3 * 4 + 2] = Wa[2]; U[3 * 4 + 3] = Wa[3]; }
+        double m[16] = {0}, Dm[16];
+        if (s > 0) {
+            mul4(L, Dinv_prev.data(), m);
+            double mU[16];
+            mul4(m, Uprev.data(), mU);
+            for (int k = 0; k < 16; ++k) D[k] -= mU[k];
+        }
+        inv4(D, Dm);
+        double *o = &red[(size_t)s * NRED];
+        for (int k = 0; k < 16; ++k) { o[k] = m[k]; o[16 + k] = Dm[k]; o[32 + k] = U[k]; }
+        std::copy(Dm, Dm + 16, Dinv_prev.begin());
+        std::copy(U, U + 16, Uprev.begin());
+    }
+    out.clear();
+    out.insert(out.end(), lt.begin(), lt.end());
+    out.insert(out.end(), sp.begin(), sp.end());
+    out.insert(out.end(), red.begin(), red.end());
 }
 
 int ccd_setup(fcfd_ctx *c) {
@@ -583,6 +762,191 @@ __global__ void __launch_bounds__(CCD_THREADS) k_uccd_lines(const double *__rest
 // UCCD of f along direction dir, upwind by u; set 0: the newOpenMP lines
 // through the ghosts, set 1: the MPLS_RisingBubble interior lines with its
 // M-DRPCCD coefficients; scratch: the context's cx, cy, cz
+// k_uccd_lines along x, the contiguous axis: one wave owns 64 lines and
+// moves them through CX-point LDS tiles, as k_ccd_lines_x does, so global
+// accesses are row segments of CX values (CX = 8: four 9 x 64 tiles, 18 KB,
+// keep two waves per SIMD resident).  Forward: a chunk of f comes in,
+// each lane runs both systems over it and leaves SU, SSU, SD, SSD in four
+// tiles (SSD over the f tile, read just before), which go out together.
+// Backward: the four forward tiles of a chunk come in, each lane back-
+// substitutes both systems and leaves SU, SD and (SSU + SSD) / 2; the choice
+// between SU and SD by the sign of u is made as the tile goes out, where the
+// loads of u are coalesced.  Same arithmetic per point as k_uccd_lines.
+template <int CX>
+__global__ void __launch_bounds__(CCDX_LINES) k_uccd_lines_x(const double *__restrict__ f,
+                                                             const double *__restrict__ u,
+                                                             double *__restrict__ fx, double *__restrict__ fxx,
+                                                             double *__restrict__ t1, double *__restrict__ t2,
+                                                             double *__restrict__ t3, const double *__restrict__ cu,
+                                                             const double *__restrict__ cd, Geom G, int ext,
+                                                             UccdSrc cs) {
+    constexpr int TS = CX + 1;
+    __shared__ double tA[CCDX_LINES * TS], tB[CCDX_LINES * TS], tC[CCDX_LINES * TS], tD[CCDX_LINES * TS];
+    const int lane = threadIdx.x;
+    const long nlines = (long)G.ny * G.nz, L0 = (long)blockIdx.x * CCDX_LINES, t = L0 + lane;
+    const int N = G.nx + 2 * ext;
+    const bool valid = t < nlines;
+    long mybase = 0;
+    if (valid) mybase = G.id(-ext, (int)(t % G.ny), (int)(t / G.ny));
+    constexpr int RPI = CCDX_LINES / CX;
+    const int lsub = lane / CX, usub = lane % CX;
+    int rbase[CX];   // row starts (the caller keeps fields below 2^31 values)
+    bool rval[CX];
+#pragma unroll
+    for (int r = 0; r < CX; ++r) {
+        const int l = r * RPI + lsub;
+        rbase[r] = __shfl((int)mybase, l, CCDX_LINES);
+        rval[r] = L0 + l < nlines;
+    }
+    const double idx = 1.0 / G.dx, idx2 = idx * idx;
+    const double c1 = cs.c1, c2 = cs.c2, c3 = cs.c3;
+    const double *F = f + mybase;
+    auto put = [&](double *tile, const double *v) {
+#pragma unroll
+        for (int r = 0; r < CX; ++r) tile[(r * RPI + lsub) * TS + usub] = v[r];
+    };
+    // ---- forward sweep: boundary row, then the chunks
+    double Su = 0.0, SSu = 0.0, Sd = 0.0, SSd = 0.0, fm = 0.0, fc = 0.0;
+    if (valid) {
+        const double f0 = F[0], f1 = F[1], f2 = F[2], f3 = F[3];
+        Su = (-3.5 * f0 + 4.0 * f1 - 0.5 * f2) * idx;
+        SSu = (34.0 / 3.0 * f0 - 83.0 / 4.0 * f1 + 10.0 * f2 - 7.0 / 12.0 * f3) * idx2;
+        Sd = Su;
+        SSd = SSu;
+        fx[mybase] = Su; t1[mybase] = SSu; t2[mybase] = Sd; t3[mybase] = SSd;
+        fm = f0;
+        fc = f1;
+    }
+    double pf[CX];
+    auto load_f = [&](int j0) {
+#pragma unroll
+        for (int r = 0; r < CX; ++r)
+            if (rval[r] && j0 + usub < N) pf[r] = f[rbase[r] + j0 + usub];
+    };
+    load_f(2);
+    for (int j0 = 2; j0 < N; j0 += CX) {
+        put(tD, pf);
+        __syncthreads();
+        if (j0 + CX < N) load_f(j0 + CX);
+        if (valid) {
+            double *ra = tA + lane * TS, *rb = tB + lane * TS, *rc = tC + lane * TS,
+                   *rd = tD + lane * TS;
+#pragma unroll 2
+            for (int q = 0; q < CX; ++q) {
+                const int i = j0 - 1 + q;
+                if (i <= N - 2) {
+                    const double *a = cu + i * NPACK, *b = cd + i * NPACK;
+                    const double fp = rd[q];
+                    const double ss = (3.0 * fm - 6.0 * fc + 3.0 * fp) * idx2;
+                    const double su = (c1 * fm + c2 * fc + c3 * fp) * idx;
+                    const double sd = -(c3 * fm + c2 * fc + c1 * fp) * idx;
+                    const double Sun = su - (a[PF_A1] * Su + a[PF_B1] * SSu);
+                    const double SSun = ss - (a[PF_AA1] * Su + a[PF_BB1] * SSu);
+                    const double Sdn = sd - (b[PF_A1] * Sd + b[PF_B1] * SSd);
+                    const double SSdn = ss - (b[PF_AA1] * Sd + b[PF_BB1] * SSd);
+                    Su = Sun; SSu = SSun; Sd = Sdn; SSd = SSdn;
+                    ra[q] = Su; rb[q] = SSu; rc[q] = Sd; rd[q] = SSd;
+                    fm = fc; fc = fp;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < CX; ++r) {
+            const int l = r * RPI + lsub, i = j0 - 1 + usub;
+            if (rval[r] && i <= N - 2) {
+                const long o = rbase[r] + i;
+                fx[o] = tA[l * TS + usub];
+                t1[o] = tB[l * TS + usub];
+                t2[o] = tC[l * TS + usub];
+                t3[o] = tD[l * TS + usub];
+            }
+        }
+        __syncthreads();
+    }
+    const int ie = N - 1;
+    if (valid) {
+        const double e0 = F[ie], e1 = F[ie - 1], e2 = F[ie - 2], e3 = F[ie - 3];
+        const double s = -(-3.5 * e0 + 4.0 * e1 - 0.5 * e2) * idx;
+        const double ss = (34.0 / 3.0 * e0 - 83.0 / 4.0 * e1 + 10.0 * e2 - 7.0 / 12.0 * e3) * idx2;
+        const double *a = cu + ie * NPACK, *b = cd + ie * NPACK;
+        const double Sun = s - (a[PF_A1] * Su + a[PF_B1] * SSu);
+        const double SSun = ss - (a[PF_AA1] * Su + a[PF_BB1] * SSu);
+        const double Sdn = s - (b[PF_A1] * Sd + b[PF_B1] * SSd);
+        const double SSdn = ss - (b[PF_AA1] * Sd + b[PF_BB1] * SSd);
+        Su = a[PB_BB2D] * Sun - a[PB_B2D] * SSun;
+        SSu = a[PB_A2D] * SSun - a[PB_AA2D] * Sun;
+        Sd = b[PB_BB2D] * Sdn - b[PB_B2D] * SSdn;
+        SSd = b[PB_A2D] * SSdn - b[PB_AA2D] * Sdn;
+        const double uv = u[mybase + ie];
+        fx[mybase + ie] = (uv > 0.0 || (cs.ge && uv == 0.0)) ? Su : Sd;
+        if (fxx) fxx[mybase + ie] = 0.5 * (SSu + SSd);
+    }
+    // ---- back substitution over i = N-2 .. 1 by the chunks in reverse, then i = 0
+    // (the four forward arrays of a chunk go straight into the tiles: held
+    // in registers for the next chunk they would take 128 VGPRs)
+    const int nch = (N - 2 + CX - 1) / CX;
+    __syncthreads();
+    for (int cix = nch - 1; cix >= 0; --cix) {
+        const int j0 = 2 + cix * CX;
+#pragma unroll
+        for (int r = 0; r < CX; ++r) {
+            const int i = j0 - 1 + usub, e = (r * RPI + lsub) * TS + usub;
+            if (rval[r] && i <= N - 2) {
+                const long o = rbase[r] + i;
+                tA[e] = fx[o]; tB[e] = t1[o]; tC[e] = t2[o]; tD[e] = t3[o];
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            double *ra = tA + lane * TS, *rb = tB + lane * TS, *rc = tC + lane * TS,
+                   *rd = tD + lane * TS;
+#pragma unroll 2
+            for (int q = CX - 1; q >= 0; --q) {
+                const int i = j0 - 1 + q;
+                if (i <= N - 2) {
+                    const double *a = cu + i * NPACK, *b = cd + i * NPACK;
+                    const double Siu = ra[q] - (a[PB_A3] * Su + a[PB_B3] * SSu);
+                    const double SSiu = rb[q] - (a[PB_AA3] * Su + a[PB_BB3] * SSu);
+                    const double Sid = rc[q] - (b[PB_A3] * Sd + b[PB_B3] * SSd);
+                    const double SSid = rd[q] - (b[PB_AA3] * Sd + b[PB_BB3] * SSd);
+                    Su = a[PB_BB2D] * Siu - a[PB_B2D] * SSiu;
+                    SSu = a[PB_A2D] * SSiu - a[PB_AA2D] * Siu;
+                    Sd = b[PB_BB2D] * Sid - b[PB_B2D] * SSid;
+                    SSd = b[PB_A2D] * SSid - b[PB_AA2D] * Sid;
+                    ra[q] = Su; rc[q] = Sd; rb[q] = 0.5 * (SSu + SSd);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < CX; ++r) {
+            const int l = r * RPI + lsub, i = j0 - 1 + usub;
+            if (rval[r] && i <= N - 2) {
+                const long o = rbase[r] + i;
+                const double uv = u[o];
+                fx[o] = (uv > 0.0 || (cs.ge && uv == 0.0)) ? tA[l * TS + usub] : tC[l * TS + usub];
+                if (fxx) fxx[o] = tB[l * TS + usub];
+            }
+        }
+        __syncthreads();
+    }
+    if (valid) {
+        const double *a = cu, *b = cd;
+        const double Siu = fx[mybase] - (a[PB_A3] * Su + a[PB_B3] * SSu);
+        const double SSiu = t1[mybase] - (a[PB_AA3] * Su + a[PB_BB3] * SSu);
+        const double Sid = t2[mybase] - (b[PB_A3] * Sd + b[PB_B3] * SSd);
+        const double SSid = t3[mybase] - (b[PB_AA3] * Sd + b[PB_BB3] * SSd);
+        Su = a[PB_BB2D] * Siu - a[PB_B2D] * SSiu;
+        SSu = a[PB_A2D] * SSiu - a[PB_AA2D] * Siu;
+        Sd = b[PB_BB2D] * Sid - b[PB_B2D] * SSid;
+        SSd = b[PB_A2D] * SSid - b[PB_AA2D] * Sid;
+        const double uv = u[mybase];
+        fx[mybase] = (uv > 0.0 || (cs.ge && uv == 0.0)) ? Su : Sd;
+        if (fxx) fxx[mybase] = 0.5 * (SSu + SSd);
+    }
+}
+
 int uccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, const double *u, double *fx, double *fxx) {
     const Geom &G = c->G;
     const int ext = set ? 0 : G.g;
@@ -597,7 +961,10 @@ int uccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, const double 
     const double *cu = c->ccd_coef[ku * 3 + dir], *cd = c->ccd_coef[kd * 3 + dir];
     const UccdSrc cs = set ? UCCD_M3 : UCCD_N3;
     ProfScope ps_(c, R_CCD);
-    if (dir == 0)
+    if (dir == 0 && G.ntot < (1L << 31))
+        hipLaunchKernelGGL(k_uccd_lines_x<8>, dim3((unsigned)((nl + CCDX_LINES - 1) / CCDX_LINES)), dim3(CCDX_LINES), 0,
+                           c->stream, f, u, fx, fxx, c->cx, c->cy, c->cz, cu, cd, G, ext, cs);
+    else if (dir == 0)
         hipLaunchKernelGGL((k_uccd_lines<0, CH>), grd, dim3(CCD_THREADS), lds, c->stream, f, u, fx, fxx, c->cx, c->cy,
                            c->cz, cu, cd, G, use_lds, ext, cs);
     else if (dir == 1)

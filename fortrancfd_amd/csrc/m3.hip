@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "fcfd_internal.h"
+#include "weno.h"
 
 using namespace fcfd;
 
@@ -116,47 +117,27 @@ __global__ void k_m3_sign(const double *__restrict__ phi, double *__restrict__ S
     S[id] = 2.0 * (h - 0.5);
 }
 
-// weno_js (SCHEME_WENO.F90:47-101) at one point from the five / six values
-// of the backward differences: fm from f(i-2..i+2), fp from f(i-1..i+3)
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
-__device__ __forceinline__ double weno_m(double a, double b, double c, double d, double e) {
-    const double b1 = 13.0 * sq(a - 2.0 * b + c) + 3.0 * sq(a - 4.0 * b + 3.0 * c);
-    const double b2 = 13.0 * sq(b - 2.0 * c + d) + 3.0 * sq(b - d);
-    const double b3 = 13.0 * sq(c - 2.0 * d + e) + 3.0 * sq(3.0 * c - 4.0 * d + e);
-    const double a1 = 1.0 / sq(kEps + b1), a2 = 6.0 / sq(kEps + b2), a3 = 3.0 / sq(kEps + b3);
-    const double s = a1 + a2 + a3;
-    const double w1 = a1 / s, w2 = a2 / s, w3 = a3 / s;
-    return w1 / 3.0 * a - (7.0 * w1 + w2) / 6.0 * b + (11.0 * w1 + 5.0 * w2 + 2.0 * w3) / 6.0 * c +
-           (2.0 * w2 + 5.0 * w3) / 6.0 * d - w3 / 6.0 * e;
-}
-
-__device__ __forceinline__ double weno_p(double a, double b, double c, double d, double e) {
-    // a..e = f(i-1), f(i), f(i+1), f(i+2), f(i+3)
-    const double b3 = 13.0 * sq(a - 2.0 * b + c) + 3.0 * sq(a - 4.0 * b + 3.0 * c);
-    const double b2 = 13.0 * sq(b - 2.0 * c + d) + 3.0 * sq(b - d);
-    const double b1 = 13.0 * sq(c - 2.0 * d + e) + 3.0 * sq(3.0 * c - 4.0 * d + e);
-    const double a1 = 1.0 / sq(kEps + b1), a2 = 6.0 / sq(kEps + b2), a3 = 3.0 / sq(kEps + b3);
-    const double s = a1 + a2 + a3;
-    const double w1 = a1 / s, w2 = a2 / s, w3 = a3 / s;
-    return w3 * (-a + 5.0 * b + 2.0 * c) / 6.0 + w2 * (2.0 * b + 5.0 * c - d) / 6.0 +
-           w1 * (11.0 * c - 7.0 * d + 2.0 * e) / 6.0;
-}
+// weno_js (SCHEME_WENO.F90:47-101) at one point is the WENO5-JS pair of
+// weno.h with EPS = 1e-12: the left-biased value from f(i-2..i+2), the
+// right-biased one from f(i-1..i+3), the same smoothness indicators and
+// weights d = (1, 6, 3), normalised there with one division instead of six.
 
 // the one-sided WENO values of the backward differences along a line:
 // F(m) = (T(m) - T(m-1)) / h for interior m, F of a ghost index = F of the
 // nearest interior index (FV_DERI's bc3d); n the line's interior length
-__device__ __forceinline__ void m3_line_weno(const double *__restrict__ T, long id, long st, int i, int n, double h,
+__device__ __forceinline__ void m3_line_weno(const double *__restrict__ T, long id, long st, int i, int n, double ih,
                                              double &up, double &um) {
-    double f[6];   // F(i-2) .. F(i+3)
+    double f[6];   // F(i-2) .. F(i+3); the differences times 1 / h
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const int m = clampi(i - 2 + q, 0, n - 1);
         const long o = id + (long)(m - i) * st;
-        f[q] = (T[o] - T[o - st]) / h;
+        f[q] = (T[o] - T[o - st]) * ih;
     }
-    um = weno_m(f[0], f[1], f[2], f[3], f[4]);
-    up = weno_p(f[1], f[2], f[3], f[4], f[5]);
+    um = weno_left(f[0], f[1], f[2], f[3], f[4], kEps);
+    up = weno_right(f[1], f[2], f[3], f[4], f[5], kEps);
 }
 
 // GRADFI + the pointwise part of RIS_MASS (LS_REDISTANCE.F90:148-238):
@@ -168,9 +149,9 @@ __global__ void k_m3_redis_a(const double *__restrict__ T, const double *__restr
     if (!cell_of_block(G, i, j, k)) return;
     const long id = G.id(i, j, k);
     double up, um, vp, vm, wp, wm;
-    m3_line_weno(T, id, 1, i, G.nx, G.dx, up, um);
-    m3_line_weno(T, id, G.sx, j, G.ny, G.dy, vp, vm);
-    m3_line_weno(T, id, G.sxy, k, G.nz, G.dz, wp, wm);
+    m3_line_weno(T, id, 1, i, G.nx, 1.0 / G.dx, up, um);
+    m3_line_weno(T, id, G.sx, j, G.ny, 1.0 / G.dy, vp, vm);
+    m3_line_weno(T, id, G.sxy, k, G.nz, 1.0 / G.dz, wp, wm);
     const double sg = S[id];
     double g;
     if (sg >= 0.0)
@@ -190,36 +171,58 @@ __global__ void k_m3_redis_a(const double *__restrict__ T, const double *__restr
 // RIS_MASS's 27-point constraint (a, b zero-gradient beyond the walls, as
 // bc3d leaves them) and one RK stage of LS_REDIS (LS_REDISTANCE.F90:68-146)
 // on T in place: s = (SGN - C) |grad| - SGN, C = delta (51 a + sum a) /
-// (51 b + sum b)
-__global__ void k_m3_redis_b(double *__restrict__ T, const double *__restrict__ S, const double *__restrict__ Gr,
-                             const double *__restrict__ D, const double *__restrict__ A, const double *__restrict__ B,
-                             double *__restrict__ s0, double *__restrict__ s1, Geom G, double rdt, int stage) {
-    int i, j, k;
-    if (!cell_of_block(G, i, j, k)) return;
-    const long id = G.id(i, j, k);
-    double sa = 0.0, sb = 0.0;
-    for (int kk = -1; kk <= 1; ++kk) {
-        const int kc = clampi(k + kk, 0, G.nz - 1);
-        for (int jj = -1; jj <= 1; ++jj) {
-            const int jc = clampi(j + jj, 0, G.ny - 1);
-            for (int ii = -1; ii <= 1; ++ii) {
-                const long q = G.id(clampi(i + ii, 0, G.nx - 1), jc, kc);
-                sa += A[q];
-                sb += B[q];
-            }
+// (51 b + sum b).  A thread walks M3_KC planes of one column and carries the
+// 3 x 3 plane sums of planes k-1, k, k+1, so each plane's nine values of a
+// and b are loaded once instead of three times.
+constexpr int M3_KC = 16;
+__global__ void __launch_bounds__(256) k_m3_redis_b(double *__restrict__ T, const double *__restrict__ S,
+                                                     const double *__restrict__ Gr, const double *__restrict__ D,
+                                                     const double *__restrict__ A, const double *__restrict__ B,
+                                                     double *__restrict__ s0, double *__restrict__ s1, Geom G,
+                                                     double rdt, int stage) {
+    int bx_, by_, bz_;
+    xcd_block(bx_, by_, bz_);
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y;
+    if (i >= G.nx || j >= G.ny) return;
+    const int k0 = bz_ * M3_KC, k1 = min(k0 + M3_KC, G.nz);
+    const int im = clampi(i - 1, 0, G.nx - 1), ip = clampi(i + 1, 0, G.nx - 1);
+    const int jm = clampi(j - 1, 0, G.ny - 1), jp = clampi(j + 1, 0, G.ny - 1);
+    // 3 x 3 sums of a and b over plane k (clamped), rows j-1, j, j+1
+    auto plane = [&](int k, double &pa, double &pb) {
+        const int kc = clampi(k, 0, G.nz - 1);
+        pa = 0.0;
+        pb = 0.0;
+        const int js[3] = {jm, j, jp};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const long row = G.id(0, js[r], kc);
+            pa += A[row + im];
+            pa += A[row + i];
+            pa += A[row + ip];
+            pb += B[row + im];
+            pb += B[row + i];
+            pb += B[row + ip];
         }
-    }
-    const double suma = 51.0 * A[id] + sa, sumb = 51.0 * B[id] + sb;
-    const double c = (sumb != 0.0 ? suma / sumb : 0.0) * D[id];
-    const double s = (S[id] - c) * Gr[id] - S[id];
-    if (stage == 0) {
-        s0[id] = s;
-        T[id] = T[id] - rdt * s;
-    } else if (stage == 1) {
-        s1[id] = s;
-        T[id] = T[id] - rdt / 4.0 * (-3.0 * s0[id] + s);
-    } else {
-        T[id] = T[id] - rdt / 12.0 * (-s0[id] - s1[id] + 8.0 * s);
+    };
+    double am, bm, ac, bc, ap, bp;
+    plane(k0 - 1, am, bm);
+    plane(k0, ac, bc);
+    for (int k = k0; k < k1; ++k) {
+        plane(k + 1, ap, bp);
+        const long id = G.id(i, j, k);
+        const double suma = 51.0 * A[id] + (am + ac + ap), sumb = 51.0 * B[id] + (bm + bc + bp);
+        const double c = (sumb != 0.0 ? suma / sumb : 0.0) * D[id];
+        const double s = (S[id] - c) * Gr[id] - S[id];
+        if (stage == 0) {
+            s0[id] = s;
+            T[id] = T[id] - rdt * s;
+        } else if (stage == 1) {
+            s1[id] = s;
+            T[id] = T[id] - rdt / 4.0 * (-3.0 * s0[id] + s);
+        } else {
+            T[id] = T[id] - rdt / 12.0 * (-s0[id] - s1[id] + 8.0 * s);
+        }
+        am = ac; bm = bc; ac = ap; bc = bp;
     }
 }
 
@@ -551,7 +554,8 @@ static int m3_redistance(fcfd_ctx *c, double *T) {
             }
             {
                 ProfScope ps_(c, R_REDIS_B);
-                hipLaunchKernelGGL(k_m3_redis_b, grd, blk_i(), 0, c->stream, T, F[FCFD_SIGN], F[FCFD_GRAD],
+                const dim3 gkm(grd.x, grd.y, (G.nz + M3_KC - 1) / M3_KC);
+                hipLaunchKernelGGL(k_m3_redis_b, gkm, blk_i(), 0, c->stream, T, F[FCFD_SIGN], F[FCFD_GRAD],
                                    F[FCFD_DELTA], c->s2, c->s3, c->l1, c->l2, G, c->pr.rdt, s);
                 FCFD_CHECK(hipGetLastError());
             }
