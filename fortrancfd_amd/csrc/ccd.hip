@@ -85,18 +85,24 @@ static LineMat ccd_matrix(int kind, int N, double dx) {
     return L;
 }
 
+// first point of segment s of a line of N points cut into P segments whose
+// lengths differ by at most one
+__host__ __device__ inline int seg_start(int s, int N, int P) { return (int)((long)s * N / P); }
+
 // TWIN_DEC (`.../lib/solver/matrix_solver.f90:6-48`) in place, packed per
-// point; with seg > 0 the elimination restarts at every multiple of seg (the
-// factors of each seg-point segment alone, no coupling to the previous one).
-static void ccd_pack(LineMat L, int seg, std::vector<double> &pk) {
+// point.  With P > 1 the elimination restarts at the first point of every
+// segment: the factors are those of each segment's own system, and the
+// forward coefficients of a segment's first row keep the row's coupling to
+// the previous point (unused by a sweep that starts from zero, applied to the
+// known neighbour value by one that does not).
+static void ccd_pack(LineMat L, int P, std::vector<double> &pk) {
     const int N = L.N;
     auto M = [&](std::vector<double> &X, int r, int i) -> double & { return X[i * 3 + r - 1]; };
     std::vector<double> &A = L.A, &B = L.B, &AA = L.AA, &BB = L.BB;
+    std::vector<char> start(N, 0);
+    for (int s = 0; s < P; ++s) start[seg_start(s, N, P)] = 1;
     for (int i = 1; i < N; ++i) {
-        if (seg > 0 && i % seg == 0) {   // segment start: no elimination
-            M(A, 1, i) = 0.0; M(B, 1, i) = 0.0; M(AA, 1, i) = 0.0; M(BB, 1, i) = 0.0;
-            continue;
-        }
+        if (start[i]) continue;
         double Den = M(A, 2, i - 1) * M(BB, 2, i - 1) - M(AA, 2, i - 1) * M(B, 2, i - 1);
         double SC1 = -(M(AA, 2, i - 1) * M(B, 1, i) - M(A, 1, i) * M(BB, 2, i - 1)) / Den;
         double SC2 = (M(B, 1, i) * M(A, 2, i - 1) - M(B, 2, i - 1) * M(A, 1, i)) / Den;
@@ -123,32 +129,33 @@ static void ccd_pack(LineMat L, int seg, std::vector<double> &pk) {
 }
 
 static void ccd_factor(int kind, int N, double dx, std::vector<double> &pk) {
-    ccd_pack(ccd_matrix(kind, N, dx), 0, pk);
+    ccd_pack(ccd_matrix(kind, N, dx), 1, pk);
 }
 
 // ---- partitioned line solve (y and z lines) ----------------------------
-// A line of N points is cut into segments of SEG points.  With x0 the
-// solution of each segment's own system (its rows without the couplings to
-// the neighbouring segments), the line's solution is
-//   x(i) = x0(i) - V(i) x(a-1) - W(i) x(b+1)      (i in segment [a, b]),
-// V, W the 2 x 2 "spikes" M_s^-1 (coupling block), constant for the grid.
-// The values x(b_s), x(a_{s+1}) at the P - 1 segment interfaces solve a
-// block-tridiagonal system with 4 x 4 blocks, also constant: its block LU is
-// precomputed, so a line's reduced solve is two short sweeps.  Kernels:
-// (1) per segment: x0 at its two ends; (2) per line: the reduced solve;
-// (3) per segment: x0 again, corrected by the spikes, written out.  f is
-// read twice and fx, fxx written once, and every line has P independent
-// threads for the sweeps.
+// One thread per line leaves a y or z sweep with only nx * nz or nx * ny
+// threads, too few to cover the memory latency of the chip (140 x 140 lines
+// are 77 workgroups of 256).  The line of N points is cut into P segments
+// of at most SEG points.  With x0 the solution of a segment's own system
+// (its rows without their couplings to the neighbouring segments), the
+// line's solution on segment [a, b] is
+//   x(i) = x0(i) - V(i) x(a-1) - W(i) x(b+1),
+// V, W the 2 x 2 "spikes" (the segment's inverse times its coupling blocks),
+// fixed by the grid.  The values at the P - 1 interfaces, u_s = (x(b_s),
+// x(a_{s+1})), solve a block-tridiagonal system of 4 x 4 blocks, also fixed
+// by the grid, whose block LU is formed here once.  The solve is then
+//   k_seg_lines<.., false>: per segment, x0 at its two ends;
+//   k_seg_reduce:           per line, the reduced forward / backward sweeps;
+//   k_seg_lines<.., true>:  per segment, the segment's own system once more
+//                           with x(a-1) and x(b+1) known, written out.
+// Every line has P threads in each sweep and f is read twice.
 constexpr int SEG = 16;
-// spike table per point: V(2 x 2) then W(2 x 2), rows (f', f''), columns the
-// (f', f'') of x(a-1) / x(b+1)
-constexpr int NSP = 8;
 // reduced block LU per interface: multiplier m (4 x 4), inverse of the
-// modified diagonal Dinv (4 x 4), super block U (4 x 4)
+// modified diagonal Dinv (4 x 4), super block U (4 x 4), row-major
 constexpr int NRED = 48;
 
-// the local solve of segment [a, b] with the packed local factors for a
-// right side given per point (r1, r2), as the kernels do it
+// the local solve of segment [a, b] (starting from zero neighbours) for a
+// right side given per point (r1, r2), with the packed segment factors
 static void seg_solve_host(const std::vector<double> &lt, int a, int b, const double *r, double *x) {
     const int n = b - a + 1;
     std::vector<double> S(n), SS(n);
@@ -171,7 +178,7 @@ static void seg_solve_host(const std::vector<double> &lt, int a, int b, const do
     }
 }
 
-// 4 x 4 inverse by Gauss-Jordan with partial pivoting (host)
+// 4 x 4 inverse by Gauss-Jordan with partial pivoting
 static void inv4(const double *m, double *out) {
     double a[4][8];
     for (int r = 0; r < 4; ++r)
@@ -202,18 +209,22 @@ static void mul4(const double *a, const double *b, double *c) {
         }
 }
 
-// local factors, spikes and reduced block LU of one line matrix, in one
-// buffer: [lt: N x NPACK][sp: N x NSP][red: (P - 1) x NRED]
+static int seg_count(int N) { return (N + SEG - 1) / SEG; }
+
+// segment factors and reduced block LU of one line matrix, in one buffer:
+// [N x NPACK segment factors][(P - 1) x NRED reduced LU]
 static void ccd_seg_tables(int kind, int N, double dx, std::vector<double> &out) {
     const LineMat M0 = ccd_matrix(kind, N, dx);
+    const int P = seg_count(N);
     std::vector<double> lt;
-    ccd_pack(M0, SEG, lt);
-    const int P = (N + SEG - 1) / SEG;
-    std::vector<double> sp((size_t)N * NSP, 0.0);
+    ccd_pack(M0, P, lt);
+    // spikes per point: V (2 x 2) then W (2 x 2), rows (f', f''), columns
+    // the (f', f'') of x(a-1) / x(b+1)
+    std::vector<double> sp((size_t)N * 8, 0.0);
     auto Mv = [&](const std::vector<double> &X, int r, int i) { return X[i * 3 + r - 1]; };
     std::vector<double> r(2 * SEG), x(2 * SEG);
     for (int s = 0; s < P; ++s) {
-        const int a = s * SEG, b = std::min(a + SEG, N) - 1, n = b - a + 1;
+        const int a = seg_start(s, N, P), b = seg_start(s + 1, N, P) - 1, n = b - a + 1;
         for (int col = 0; col < 2; ++col) {
             if (a > 0) {   // V column: coupling of row a to x(a-1)
                 std::fill(r.begin(), r.end(), 0.0);
@@ -221,8 +232,8 @@ static void ccd_seg_tables(int kind, int N, double dx, std::vector<double> &out)
                 r[1] = col == 0 ? Mv(M0.AA, 1, a) : Mv(M0.BB, 1, a);
                 seg_solve_host(lt, a, b, r.data(), x.data());
                 for (int q = 0; q < n; ++q) {
-                    sp[(size_t)(a + q) * NSP + 0 + col] = x[2 * q];       // V row f'
-                    sp[(size_t)(a + q) * NSP + 2 + col] = x[2 * q + 1];   // V row f''
+                    sp[(size_t)(a + q) * 8 + 0 + col] = x[2 * q];
+                    sp[(size_t)(a + q) * 8 + 2 + col] = x[2 * q + 1];
                 }
             }
             if (b < N - 1) {   // W column: coupling of row b to x(b+1)
@@ -231,48 +242,42 @@ static void ccd_seg_tables(int kind, int N, double dx, std::vector<double> &out)
                 r[2 * (n - 1) + 1] = col == 0 ? Mv(M0.AA, 3, b) : Mv(M0.BB, 3, b);
                 seg_solve_host(lt, a, b, r.data(), x.data());
                 for (int q = 0; q < n; ++q) {
-                    sp[(size_t)(a + q) * NSP + 4 + col] = x[2 * q];
-                    sp[(size_t)(a + q) * NSP + 6 + col] = x[2 * q + 1];
+                    sp[(size_t)(a + q) * 8 + 4 + col] = x[2 * q];
+                    sp[(size_t)(a + q) * 8 + 6 + col] = x[2 * q + 1];
                 }
             }
         }
     }
-    // reduced system over interfaces s = 0 .. P-2, u_s = (x(b_s), x(a_{s+1})):
-    //   x(b_s)     + V_s(b_s) x(b_{s-1})     + W_s(b_s) x(a_{s+1})     = x0(b_s)
+    // reduced rows of interface s (unknowns u_s = (x(b_s), x(a_{s+1}))):
+    //   x(b_s)     + V_s(b_s) x(b_{s-1})     + W_s(b_s) x(a_{s+1})         = x0(b_s)
     //   x(a_{s+1}) + V_{s+1}(a_{s+1}) x(b_s) + W_{s+1}(a_{s+1}) x(a_{s+2}) = x0(a_{s+1})
     const int NI = P - 1;
-    std::vector<double> red((size_t)std::max(NI, 0) * NRED, 0.0);
-    std::vector<double> Dp(16), Dinv_prev(16), Uprev(16);
+    std::vector<double> red((size_t)NI * NRED, 0.0);
+    double Dinv_prev[16] = {0}, Uprev[16] = {0};
     for (int s = 0; s < NI; ++s) {
-        const int bs = std::min((s + 1) * SEG, N) - 1, an = (s + 1) * SEG;
-        const double *Vb = &sp[(size_t)bs * NSP], *Wb = Vb + 4;
-        const double *Va = &sp[(size_t)an * NSP], *Wa = Va + 4;
+        const int an = seg_start(s + 1, N, P), bs = an - 1;
+        const double *Vb = &sp[(size_t)bs * 8], *Wb = Vb + 4;
+        const double *Va = &sp[(size_t)an * 8], *Wa = Va + 4;
         double D[16] = {0}, L[16] = {0}, U[16] = {0};
         for (int k = 0; k < 4; ++k) D[k * 4 + k] = 1.0;
-        // rows 0-1: x(b_s); columns 2-3 of u_s: x(a_{s+1}) by W_s(b_s)
         D[0 * 4 + 2] = Wb[0]; D[0 * 4 + 3] = Wb[1]; D[1 * 4 + 2] = Wb[2]; D[1 * 4 + 3] = Wb[3];
-        // rows 2-3: x(a_{s+1}); columns 0-1 of u_s: x(b_s) by V_{s+1}(a_{s+1})
         D[2 * 4 + 0] = Va[0]; D[2 * 4 + 1] = Va[1]; D[3 * 4 + 0] = Va[2]; D[3 * 4 + 1] = Va[3];
-        // L: rows 0-1 couple to the first half of u_{s-1} (x(b_{s-1})) by V_s(b_s)
         if (s > 0) { L[0] = Vb[0]; L[1] = Vb[1]; L[4] = Vb[2]; L[5] = Vb[3]; }
-        // U: rows 2-3 couple to the second half of u_{s+1} (x(a_{s+2})) by W_{s+1}(a_{s+1})
         if (s < NI - 1) { U[2 * 4 + 2] = Wa[0]; U[2 * 4 + 3] = Wa[1]; U[3 * 4 + 2] = Wa[2]; U[3 * 4 + 3] = Wa[3]; }
         double m[16] = {0}, Dm[16];
         if (s > 0) {
-            mul4(L, Dinv_prev.data(), m);
+            mul4(L, Dinv_prev, m);
             double mU[16];
-            mul4(m, Uprev.data(), mU);
+            mul4(m, Uprev, mU);
             for (int k = 0; k < 16; ++k) D[k] -= mU[k];
         }
         inv4(D, Dm);
         double *o = &red[(size_t)s * NRED];
         for (int k = 0; k < 16; ++k) { o[k] = m[k]; o[16 + k] = Dm[k]; o[32 + k] = U[k]; }
-        std::copy(Dm, Dm + 16, Dinv_prev.begin());
-        std::copy(U, U + 16, Uprev.begin());
+        std::copy(Dm, Dm + 16, Dinv_prev);
+        std::copy(U, U + 16, Uprev);
     }
-    out.clear();
-    out.insert(out.end(), lt.begin(), lt.end());
-    out.insert(out.end(), sp.begin(), sp.end());
+    out = lt;
     out.insert(out.end(), red.begin(), red.end());
 }
 
@@ -280,16 +285,27 @@ int ccd_setup(fcfd_ctx *c) {
     const Geom &G = c->G;
     const int n[3] = {G.nx, G.ny, G.nz};
     const double h[3] = {G.dx, G.dy, G.dz};
+    const long nl[3] = {(long)G.ny * G.nz, (long)G.nx * G.nz, (long)G.nx * G.ny};
+    long zmax = 0;
     for (int kind = 0; kind < NMAT; ++kind)
         for (int d = 0; d < 3; ++d) {
             double *&dst = c->ccd_coef[kind * 3 + d];
-            dst = nullptr;
+            double *&seg = c->ccd_seg[kind * 3 + d];
+            dst = seg = nullptr;
             if (d >= G.dim) continue;
+            const int N = n[d] + (kind >= MAT_M3_CCD ? 0 : 2 * G.g);
             std::vector<double> pk;
-            ccd_factor(kind, n[d] + (kind >= MAT_M3_CCD ? 0 : 2 * G.g), h[d], pk);
+            ccd_factor(kind, N, h[d], pk);
             FCFD_CHECK(hipMalloc((void **)&dst, sizeof(double) * pk.size()));
             FCFD_CHECK(hipMemcpy(dst, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
+            if (d == 0) continue;   // x lines keep the one-thread-per-line sweeps
+            ccd_seg_tables(kind, N, h[d], pk);
+            FCFD_CHECK(hipMalloc((void **)&seg, sizeof(double) * pk.size()));
+            FCFD_CHECK(hipMemcpy(seg, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
+            zmax = std::max(zmax, 2L * seg_count(N) * 4 * nl[d]);
         }
+    c->ccd_z = nullptr;
+    if (zmax) FCFD_CHECK(hipMalloc((void **)&c->ccd_z, sizeof(double) * zmax));
     return FCFD_OK;
 }
 
@@ -592,6 +608,220 @@ __global__ void __launch_bounds__(CCDX_LINES) k_ccd_lines_x(const double *__rest
 
 // CCD of f along direction dir: set 0 the newOpenMP lines through the ghosts
 // (n + 6 points), set 1 the MPLS_RisingBubble interior lines (n points)
+// first-derivative source coefficients of the upwind system (the downwind
+// one mirrors them) and the choice at u = 0: newOpenMP takes the downwind
+// value there (ccd_solvers.f90:400-412), MPLS_RisingBubble the upwind one
+// (SCHEME_COMPACT.F90:17-25)
+struct UccdSrc {
+    double c1, c2, c3;
+    int ge;
+};
+constexpr UccdSrc UCCD_N3{-1.93596119008109, 1.99692238016218, -0.06096119008109, 0};
+constexpr UccdSrc UCCD_M3{-1.93629468537011, 1.99758937102742, -0.061294685370110, 1};
+
+// Partitioned y / z line solve, one thread per (line, segment); the wave's
+// 64 lanes are 64 consecutive lines (consecutive i: coalesced row accesses)
+// of one segment, so every coefficient read is uniform over the wave.  NS =
+// 1: CCD (asgn_src_ccd); NS = 2: the up- and down-wind UCCD systems side by
+// side, as in k_uccd_lines.  FINAL = false: the segment's own solution x0
+// (no neighbours), whose end values go to zb; FINAL = true: the segment's
+// system with the neighbour values x(a-1), x(b+1) that k_seg_reduce left in
+// zb, the result written to fx, fxx.  zb holds per system, segment s and
+// line t four values at zb[((sys * P + s) * 4 + c) * nl + t]: c = 0, 1 the
+// (f', f'') of the segment's first point, c = 2, 3 of its last.
+constexpr int SEGT = 256;
+
+template <int DIR, int NS, bool FINAL>
+__global__ void __launch_bounds__(SEGT) k_seg_lines(const double *__restrict__ f, const double *__restrict__ u,
+                                                    double *__restrict__ fx, double *__restrict__ fxx,
+                                                    double *__restrict__ zb, const double *__restrict__ tab_u,
+                                                    const double *__restrict__ tab_d, Geom G, int ext, int P,
+                                                    UccdSrc cs) {
+    long nl, st;
+    int N;
+    double dx;
+    if (DIR == 1) { nl = (long)G.nx * G.nz; N = G.ny + 2 * ext; st = G.sx; dx = G.dy; }
+    else { nl = (long)G.nx * G.ny; N = G.nz + 2 * ext; st = G.sxy; dx = G.dz; }
+    const long t = (long)blockIdx.x * SEGT + threadIdx.x;
+    if (t >= nl) return;
+    const int s = blockIdx.y;
+    const int a = seg_start(s, N, P), n = seg_start(s + 1, N, P) - a;
+    const int la = (int)(t % G.nx), lb = (int)(t / G.nx);
+    const long base = (DIR == 1 ? G.id(la, -ext, lb) : G.id(la, lb, -ext)) + (long)a * st;
+    const double idx = 1.0 / dx, idx2 = idx * idx;
+    const double *tab[2] = {tab_u, tab_d};
+    // f(a - 1 .. a + n) where the line has them; fv[k] = f(a - 1 + k)
+    double fv[SEG + 2];
+#pragma unroll
+    for (int k = 0; k < SEG + 2; ++k) {
+        const int i = a - 1 + k;
+        fv[k] = (k <= n + 1 && i >= 0 && i < N) ? f[base + (long)(k - 1) * st] : 0.0;
+    }
+    // upwind choice per point, from u loaded up front
+    unsigned up = 0;
+    if (FINAL && NS == 2) {
+#pragma unroll
+        for (int q = 0; q < SEG; ++q)
+            if (q < n) {
+                const double uv = u[base + (long)q * st];
+                up |= (uv > 0.0 || (cs.ge && uv == 0.0)) ? (1u << q) : 0u;
+            }
+    }
+    // x(a-1) seeds the forward sweep, x(b+1) the backward one
+    double p1[NS], p2[NS], x1[NS], x2[NS];
+#pragma unroll
+    for (int y = 0; y < NS; ++y) {
+        p1[y] = p2[y] = x1[y] = x2[y] = 0.0;
+        if (FINAL) {
+            const double *Z = zb + (long)y * P * 4 * nl + t;
+            if (s > 0) { p1[y] = Z[((long)(s - 1) * 4 + 2) * nl]; p2[y] = Z[((long)(s - 1) * 4 + 3) * nl]; }
+            if (s < P - 1) { x1[y] = Z[((long)(s + 1) * 4 + 0) * nl]; x2[y] = Z[((long)(s + 1) * 4 + 1) * nl]; }
+        }
+    }
+    double S[NS][SEG], SS[NS][SEG];
+#pragma unroll
+    for (int q = 0; q < SEG; ++q) {
+        if (q < n) {
+            const int i = a + q;
+            double r1[NS], r2;
+            if (i == 0) {
+                const double f0 = fv[q + 1], f1 = fv[q + 2], f2 = fv[q + 3], f3 = fv[q + 4 < SEG + 2 ? q + 4 : SEG + 1];
+                r1[0] = (-3.5 * f0 + 4.0 * f1 - 0.5 * f2) * idx;
+                r2 = (34.0 / 3.0 * f0 - 83.0 / 4.0 * f1 + 10.0 * f2 - 7.0 / 12.0 * f3) * idx2;
+                if (NS == 2) r1[NS - 1] = r1[0];
+            } else if (i == N - 1) {
+                const double e0 = fv[q + 1], e1 = fv[q], e2 = fv[q >= 1 ? q - 1 : 0], e3 = fv[q >= 2 ? q - 2 : 0];
+                r1[0] = -(-3.5 * e0 + 4.0 * e1 - 0.5 * e2) * idx;
+                r2 = (34.0 / 3.0 * e0 - 83.0 / 4.0 * e1 + 10.0 * e2 - 7.0 / 12.0 * e3) * idx2;
+                if (NS == 2) r1[NS - 1] = r1[0];
+            } else {
+                const double fm = fv[q], fc = fv[q + 1], fp = fv[q + 2];
+                r2 = (3.0 * fm - 6.0 * fc + 3.0 * fp) * idx2;
+                if (NS == 1) {
+                    r1[0] = 15.0 / 16.0 * (fp - fm) * idx;
+                } else {
+                    r1[0] = (cs.c1 * fm + cs.c2 * fc + cs.c3 * fp) * idx;
+                    r1[NS - 1] = -(cs.c3 * fm + cs.c2 * fc + cs.c1 * fp) * idx;
+                }
+            }
+#pragma unroll
+            for (int y = 0; y < NS; ++y) {
+                const double *co = tab[y] + (long)i * NPACK;
+                const double Sn = r1[y] - (co[PF_A1] * p1[y] + co[PF_B1] * p2[y]);
+                const double SSn = r2 - (co[PF_AA1] * p1[y] + co[PF_BB1] * p2[y]);
+                p1[y] = S[y][q] = Sn;
+                p2[y] = SS[y][q] = SSn;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = SEG - 1; q >= 0; --q) {
+        if (q < n) {
+            const int i = a + q;
+#pragma unroll
+            for (int y = 0; y < NS; ++y) {
+                const double *co = tab[y] + (long)i * NPACK;
+                const double Si = S[y][q] - (co[PB_A3] * x1[y] + co[PB_B3] * x2[y]);
+                const double SSi = SS[y][q] - (co[PB_AA3] * x1[y] + co[PB_BB3] * x2[y]);
+                x1[y] = co[PB_BB2D] * Si - co[PB_B2D] * SSi;
+                x2[y] = co[PB_A2D] * SSi - co[PB_AA2D] * Si;
+            }
+            if (FINAL) {
+                const long o = base + (long)q * st;
+                if (NS == 1) {
+                    fx[o] = x1[0];
+                    fxx[o] = x2[0];
+                } else {
+                    fx[o] = ((up >> q) & 1u) ? x1[0] : x1[NS - 1];
+                    if (fxx) fxx[o] = 0.5 * (x2[0] + x2[NS - 1]);
+                }
+            } else if (q == n - 1 || q == 0) {
+#pragma unroll
+                for (int y = 0; y < NS; ++y) {
+                    double *Z = zb + (long)y * P * 4 * nl + t;
+                    const int c0 = q == 0 ? 0 : 2;
+                    Z[((long)s * 4 + c0) * nl] = x1[y];
+                    Z[((long)s * 4 + c0 + 1) * nl] = x2[y];
+                }
+            }
+        }
+    }
+}
+
+// The reduced block-tridiagonal solve of one line (per system): forward
+// d'_s = d_s - m_s d'_{s-1}, backward u_s = Dinv_s (d'_s - U_s u_{s+1}),
+// with d_s = (x0(b_s), x0(a_{s+1})) read from zb and u_s = (x(b_s),
+// x(a_{s+1})) written back over them.
+template <int NS>
+__global__ void __launch_bounds__(SEGT) k_seg_reduce(double *__restrict__ zb, const double *__restrict__ red_u,
+                                                     const double *__restrict__ red_d, long nl, int P) {
+    const long t = (long)blockIdx.x * SEGT + threadIdx.x;
+    if (t >= nl) return;
+    const int NI = P - 1;
+#pragma unroll
+    for (int y = 0; y < NS; ++y) {
+        const double *R = y ? red_d : red_u;
+        double *Z = zb + (long)y * P * 4 * nl + t;
+        // slots of interface s: (s, 2), (s, 3), (s + 1, 0), (s + 1, 1)
+        auto slot = [&](int s, int k) -> double & {
+            return Z[(k < 2 ? (long)s * 4 + 2 + k : (long)(s + 1) * 4 + k - 2) * nl];
+        };
+        double dp[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int s = 0; s < NI; ++s) {
+            const double *m = R + (long)s * NRED;
+            double d[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) d[k] = slot(s, k);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                d[r] -= m[r * 4 + 0] * dp[0] + m[r * 4 + 1] * dp[1] + m[r * 4 + 2] * dp[2] + m[r * 4 + 3] * dp[3];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (s < NI - 1) slot(s, k) = d[k];
+                dp[k] = d[k];
+            }
+        }
+        double un[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int s = NI - 1; s >= 0; --s) {
+            const double *Di = R + (long)s * NRED + 16, *U = Di + 16;
+            double w[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[k] = s == NI - 1 ? dp[k] : slot(s, k);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                w[r] -= U[r * 4 + 0] * un[0] + U[r * 4 + 1] * un[1] + U[r * 4 + 2] * un[2] + U[r * 4 + 3] * un[3];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                un[r] = Di[r * 4 + 0] * w[0] + Di[r * 4 + 1] * w[1] + Di[r * 4 + 2] * w[2] + Di[r * 4 + 3] * w[3];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) slot(s, k) = un[k];
+        }
+    }
+}
+
+// the partitioned solve of the y (dir 1) or z (dir 2) lines: NS = 1 CCD
+// with table k0, NS = 2 UCCD with tables k0 (up) and k1 (down)
+template <int DIR, int NS>
+static int seg_solve(fcfd_ctx *c, int ext, int k0, int k1, const double *f, const double *u, double *fx, double *fxx,
+                     UccdSrc cs) {
+    const Geom &G = c->G;
+    const long nl = DIR == 1 ? (long)G.nx * G.nz : (long)G.nx * G.ny;
+    const int N = (DIR == 1 ? G.ny : G.nz) + 2 * ext;
+    const int P = seg_count(N);
+    const double *tu = c->ccd_seg[k0 * 3 + DIR], *td = c->ccd_seg[k1 * 3 + DIR];
+    const unsigned gx = (unsigned)((nl + SEGT - 1) / SEGT);
+    if (P > 1) {
+        hipLaunchKernelGGL((k_seg_lines<DIR, NS, false>), dim3(gx, (unsigned)P), dim3(SEGT), 0, c->stream, f, u, fx,
+                           fxx, c->ccd_z, tu, td, G, ext, P, cs);
+        hipLaunchKernelGGL((k_seg_reduce<NS>), dim3(gx), dim3(SEGT), 0, c->stream, c->ccd_z, tu + (long)N * NPACK,
+                           td + (long)N * NPACK, nl, P);
+    }
+    hipLaunchKernelGGL((k_seg_lines<DIR, NS, true>), dim3(gx, (unsigned)P), dim3(SEGT), 0, c->stream, f, u, fx, fxx,
+                       c->ccd_z, tu, td, G, ext, P, cs);
+    FCFD_CHECK(hipGetLastError());
+    return FCFD_OK;
+}
+
 int ccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, double *fx, double *fxx) {
     const Geom &G = c->G;
     const int ext = set ? 0 : G.g;
@@ -603,7 +833,10 @@ int ccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, double *fx, do
     constexpr int CH = 8;
     dim3 grd((unsigned)((nl + CCD_THREADS - 1) / CCD_THREADS));
     ProfScope ps_(c, R_CCD);
-    const double *co = c->ccd_coef[(set ? MAT_M3_CCD : MAT_CCD) * 3 + dir];
+    const int kc = set ? MAT_M3_CCD : MAT_CCD;
+    const double *co = c->ccd_coef[kc * 3 + dir];
+    if (dir == 1 && c->ccd_seg[kc * 3 + 1]) return seg_solve<1, 1>(c, ext, kc, kc, f, nullptr, fx, fxx, UccdSrc{});
+    if (dir == 2 && c->ccd_seg[kc * 3 + 2]) return seg_solve<2, 1>(c, ext, kc, kc, f, nullptr, fx, fxx, UccdSrc{});
     if (dir == 0)
         hipLaunchKernelGGL(k_ccd_lines_x, dim3((unsigned)((nl + CCDX_LINES - 1) / CCDX_LINES)), dim3(CCDX_LINES), 0,
                            c->stream, f, fx, fxx, co, G, ext);
@@ -620,17 +853,6 @@ int ccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, double *fx, do
 int ccd_dir(fcfd_ctx *c, int dir, const double *f, double *fx, double *fxx) {
     return ccd_lines_set(c, 0, dir, f, fx, fxx);
 }
-
-// first-derivative source coefficients of the upwind system (the downwind
-// one mirrors them) and the choice at u = 0: newOpenMP takes the downwind
-// value there (ccd_solvers.f90:400-412), MPLS_RisingBubble the upwind one
-// (SCHEME_COMPACT.F90:17-25)
-struct UccdSrc {
-    double c1, c2, c3;
-    int ge;
-};
-constexpr UccdSrc UCCD_N3{-1.93596119008109, 1.99692238016218, -0.06096119008109, 0};
-constexpr UccdSrc UCCD_M3{-1.93629468537011, 1.99758937102742, -0.061294685370110, 1};
 
 // root_solve("uccd") along one line (`.../lib/solver/ccd_solvers.f90:
 // 308-341,400-412`): the up- and down-wind systems share the second-
@@ -961,6 +1183,8 @@ int uccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, const double 
     const double *cu = c->ccd_coef[ku * 3 + dir], *cd = c->ccd_coef[kd * 3 + dir];
     const UccdSrc cs = set ? UCCD_M3 : UCCD_N3;
     ProfScope ps_(c, R_CCD);
+    if (dir == 1 && c->ccd_seg[ku * 3 + 1]) return seg_solve<1, 2>(c, ext, ku, kd, f, u, fx, fxx, cs);
+    if (dir == 2 && c->ccd_seg[ku * 3 + 2]) return seg_solve<2, 2>(c, ext, ku, kd, f, u, fx, fxx, cs);
     if (dir == 0 && G.ntot < (1L << 31))
         hipLaunchKernelGGL(k_uccd_lines_x<8>, dim3((unsigned)((nl + CCDX_LINES - 1) / CCDX_LINES)), dim3(CCDX_LINES), 0,
                            c->stream, f, u, fx, fxx, c->cx, c->cy, c->cz, cu, cd, G, ext, cs);

@@ -371,6 +371,9 @@ extern "C" int fcfd_destroy(fcfd_ctx *c) {
         if (p) hipFree(p);
     for (auto p : c->ccd_coef)
         if (p) hipFree(p);
+    for (auto p : c->ccd_seg)
+        if (p) hipFree(p);
+    if (c->ccd_z) hipFree(c->ccd_z);
     for (auto p : c->adj)
         if (p) hipFree(p);
     for (auto p : c->adv32)

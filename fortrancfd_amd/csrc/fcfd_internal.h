@@ -112,6 +112,8 @@ struct fcfd_ctx {
     double *un, *vn, *wn;      // predictor output buffer (ping-pong with u, v, w)
     double *ccd_coef[18];      // factorised line matrices [kind * 3 + direction]: CCD, UCCD up / down on
                                // ghosted lines, the same on interior lines (MPLS_RisingBubble set)
+    double *ccd_seg[18];       // the same per segment for the partitioned y / z solve, + reduced block LU
+    double *ccd_z;             // partitioned solve: segment end values / interface values per line
     double *adj[5];            // advection adjoint: stage states phi1, phi2, adjoints lambda1, lambda2, seed
     float *adv32[4];           // FP32 advection: two stage buffers and the stage sources l1, l2
     float *adj32[7];           // FP32 adjoint: phi1, phi2, lambda1, lambda2, seed, stage sources l1, l2
