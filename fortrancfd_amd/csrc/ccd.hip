@@ -18,6 +18,7 @@
 // segments (measured at 256^3: ~0.35 ms per x sweep, from ~0.88 ms with
 // per-lane strided access; y or z ~0.19 ms).
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "fcfd_internal.h"
@@ -87,7 +88,7 @@ static LineMat ccd_matrix(int kind, int N, double dx) {
 
 // first point of segment s of a line of N points cut into P segments whose
 // lengths differ by at most one
-__host__ __device__ inline int seg_start(int s, int N, int P) { return (int)((long)s * N / P); }
+__host__ __device__ inline int seg_start(int s, int N, int P) { return (int)((unsigned)s * (unsigned)N / (unsigned)P); }
 
 // TWIN_DEC (`.../lib/solver/matrix_solver.f90:6-48`) in place, packed per
 // point.  With P > 1 the elimination restarts at the first point of every
@@ -132,10 +133,10 @@ static void ccd_factor(int kind, int N, double dx, std::vector<double> &pk) {
     ccd_pack(ccd_matrix(kind, N, dx), 1, pk);
 }
 
-// ---- partitioned line solve (y and z lines) ----------------------------
-// One thread per line leaves a y or z sweep with only nx * nz or nx * ny
-// threads, too few to cover the memory latency of the chip (140 x 140 lines
-// are 77 workgroups of 256).  The line of N points is cut into P segments
+// ---- partitioned line solve ---------------------------------------------
+// One thread per line leaves a sweep with only as many threads as lines,
+// too few to cover the memory latency of the chip (140 x 140 z lines are 77
+// workgroups of 256).  The line of N points is cut into P segments
 // of at most SEG points.  With x0 the solution of a segment's own system
 // (its rows without their couplings to the neighbouring segments), the
 // line's solution on segment [a, b] is
@@ -298,7 +299,8 @@ int ccd_setup(fcfd_ctx *c) {
             ccd_factor(kind, N, h[d], pk);
             FCFD_CHECK(hipMalloc((void **)&dst, sizeof(double) * pk.size()));
             FCFD_CHECK(hipMemcpy(dst, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
-            if (d == 0) continue;   // x lines keep the one-thread-per-line sweeps
+            // the partitioned solve numbers lines and segment points in 32 bits
+            if (nl[d] >= (1L << 31) || seg_count(N) > 65535) continue;
             ccd_seg_tables(kind, N, h[d], pk);
             FCFD_CHECK(hipMalloc((void **)&seg, sizeof(double) * pk.size()));
             FCFD_CHECK(hipMemcpy(seg, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
@@ -631,54 +633,18 @@ constexpr UccdSrc UCCD_M3{-1.93629468537011, 1.99758937102742, -0.06129468537011
 // (f', f'') of the segment's first point, c = 2, 3 of its last.
 constexpr int SEGT = 256;
 
-template <int DIR, int NS, bool FINAL>
-__global__ void __launch_bounds__(SEGT) k_seg_lines(const double *__restrict__ f, const double *__restrict__ u,
-                                                    double *__restrict__ fx, double *__restrict__ fxx,
-                                                    double *__restrict__ zb, const double *__restrict__ tab_u,
-                                                    const double *__restrict__ tab_d, Geom G, int ext, int P,
-                                                    UccdSrc cs) {
-    long nl, st;
-    int N;
-    double dx;
-    if (DIR == 1) { nl = (long)G.nx * G.nz; N = G.ny + 2 * ext; st = G.sx; dx = G.dy; }
-    else { nl = (long)G.nx * G.ny; N = G.nz + 2 * ext; st = G.sxy; dx = G.dz; }
-    const long t = (long)blockIdx.x * SEGT + threadIdx.x;
-    if (t >= nl) return;
-    const int s = blockIdx.y;
-    const int a = seg_start(s, N, P), n = seg_start(s + 1, N, P) - a;
-    const int la = (int)(t % G.nx), lb = (int)(t / G.nx);
-    const long base = (DIR == 1 ? G.id(la, -ext, lb) : G.id(la, lb, -ext)) + (long)a * st;
-    const double idx = 1.0 / dx, idx2 = idx * idx;
+// The sweeps of one segment (n <= SEG points from line index a, N points on
+// the line) for NS systems, fv[k] = f(a - 1 + k) where the line has it.
+// p1, p2 seed the forward sweep (x(a-1); zero for the segment's own
+// system), x1, x2 the backward one (x(b+1)).  On return S[y][q], SS[y][q]
+// hold the (f', f'') of system y at point a + q, and e1, e2 those at b.
+template <int NS>
+__device__ __forceinline__ void seg_sweep(const double (&fv)[SEG + 2], int a, int n, int N, double idx, double idx2,
+                                          const double *tab_u, const double *tab_d, const UccdSrc &cs,
+                                          double (&p1)[NS], double (&p2)[NS], double (&x1)[NS], double (&x2)[NS],
+                                          double (&S)[NS][SEG], double (&SS)[NS][SEG], double (&e1)[NS],
+                                          double (&e2)[NS]) {
     const double *tab[2] = {tab_u, tab_d};
-    // f(a - 1 .. a + n) where the line has them; fv[k] = f(a - 1 + k)
-    double fv[SEG + 2];
-#pragma unroll
-    for (int k = 0; k < SEG + 2; ++k) {
-        const int i = a - 1 + k;
-        fv[k] = (k <= n + 1 && i >= 0 && i < N) ? f[base + (long)(k - 1) * st] : 0.0;
-    }
-    // upwind choice per point, from u loaded up front
-    unsigned up = 0;
-    if (FINAL && NS == 2) {
-#pragma unroll
-        for (int q = 0; q < SEG; ++q)
-            if (q < n) {
-                const double uv = u[base + (long)q * st];
-                up |= (uv > 0.0 || (cs.ge && uv == 0.0)) ? (1u << q) : 0u;
-            }
-    }
-    // x(a-1) seeds the forward sweep, x(b+1) the backward one
-    double p1[NS], p2[NS], x1[NS], x2[NS];
-#pragma unroll
-    for (int y = 0; y < NS; ++y) {
-        p1[y] = p2[y] = x1[y] = x2[y] = 0.0;
-        if (FINAL) {
-            const double *Z = zb + (long)y * P * 4 * nl + t;
-            if (s > 0) { p1[y] = Z[((long)(s - 1) * 4 + 2) * nl]; p2[y] = Z[((long)(s - 1) * 4 + 3) * nl]; }
-            if (s < P - 1) { x1[y] = Z[((long)(s + 1) * 4 + 0) * nl]; x2[y] = Z[((long)(s + 1) * 4 + 1) * nl]; }
-        }
-    }
-    double S[NS][SEG], SS[NS][SEG];
 #pragma unroll
     for (int q = 0; q < SEG; ++q) {
         if (q < n) {
@@ -690,9 +656,9 @@ __global__ void __launch_bounds__(SEGT) k_seg_lines(const double *__restrict__ f
                 r2 = (34.0 / 3.0 * f0 - 83.0 / 4.0 * f1 + 10.0 * f2 - 7.0 / 12.0 * f3) * idx2;
                 if (NS == 2) r1[NS - 1] = r1[0];
             } else if (i == N - 1) {
-                const double e0 = fv[q + 1], e1 = fv[q], e2 = fv[q >= 1 ? q - 1 : 0], e3 = fv[q >= 2 ? q - 2 : 0];
-                r1[0] = -(-3.5 * e0 + 4.0 * e1 - 0.5 * e2) * idx;
-                r2 = (34.0 / 3.0 * e0 - 83.0 / 4.0 * e1 + 10.0 * e2 - 7.0 / 12.0 * e3) * idx2;
+                const double e0 = fv[q + 1], e1_ = fv[q], e2_ = fv[q >= 1 ? q - 1 : 0], e3 = fv[q >= 2 ? q - 2 : 0];
+                r1[0] = -(-3.5 * e0 + 4.0 * e1_ - 0.5 * e2_) * idx;
+                r2 = (34.0 / 3.0 * e0 - 83.0 / 4.0 * e1_ + 10.0 * e2_ - 7.0 / 12.0 * e3) * idx2;
                 if (NS == 2) r1[NS - 1] = r1[0];
             } else {
                 const double fm = fv[q], fc = fv[q + 1], fp = fv[q + 2];
@@ -725,78 +691,324 @@ __global__ void __launch_bounds__(SEGT) k_seg_lines(const double *__restrict__ f
                 const double SSi = SS[y][q] - (co[PB_AA3] * x1[y] + co[PB_BB3] * x2[y]);
                 x1[y] = co[PB_BB2D] * Si - co[PB_B2D] * SSi;
                 x2[y] = co[PB_A2D] * SSi - co[PB_AA2D] * Si;
-            }
-            if (FINAL) {
-                const long o = base + (long)q * st;
-                if (NS == 1) {
-                    fx[o] = x1[0];
-                    fxx[o] = x2[0];
-                } else {
-                    fx[o] = ((up >> q) & 1u) ? x1[0] : x1[NS - 1];
-                    if (fxx) fxx[o] = 0.5 * (x2[0] + x2[NS - 1]);
-                }
-            } else if (q == n - 1 || q == 0) {
-#pragma unroll
-                for (int y = 0; y < NS; ++y) {
-                    double *Z = zb + (long)y * P * 4 * nl + t;
-                    const int c0 = q == 0 ? 0 : 2;
-                    Z[((long)s * 4 + c0) * nl] = x1[y];
-                    Z[((long)s * 4 + c0 + 1) * nl] = x2[y];
+                S[y][q] = x1[y];
+                SS[y][q] = x2[y];
+                if (q == n - 1) {
+                    e1[y] = x1[y];
+                    e2[y] = x2[y];
                 }
             }
         }
     }
 }
 
-// The reduced block-tridiagonal solve of one line (per system): forward
-// d'_s = d_s - m_s d'_{s-1}, backward u_s = Dinv_s (d'_s - U_s u_{s+1}),
-// with d_s = (x0(b_s), x0(a_{s+1})) read from zb and u_s = (x(b_s),
-// x(a_{s+1})) written back over them.
+// x(a-1) and x(b+1) of line t's segment s from the reduced solve (FINAL)
 template <int NS>
+__device__ __forceinline__ void seg_neighbours(const double *__restrict__ zb, long t, long nl, int s, int P,
+                                               double (&p1)[NS], double (&p2)[NS], double (&x1)[NS],
+                                               double (&x2)[NS]) {
+#pragma unroll
+    for (int y = 0; y < NS; ++y) {
+        p1[y] = p2[y] = x1[y] = x2[y] = 0.0;
+        const double *Z = zb + (long)y * P * 4 * nl + t;
+        if (s > 0) { p1[y] = Z[((long)(s - 1) * 4 + 2) * nl]; p2[y] = Z[((long)(s - 1) * 4 + 3) * nl]; }
+        if (s < P - 1) { x1[y] = Z[((long)(s + 1) * 4 + 0) * nl]; x2[y] = Z[((long)(s + 1) * 4 + 1) * nl]; }
+    }
+}
+
+// the segment's end values x0(a), x0(b) into zb (not FINAL)
+template <int NS>
+__device__ __forceinline__ void seg_store_ends(double *__restrict__ zb, long t, long nl, int s, int P,
+                                               const double (&S)[NS][SEG], const double (&SS)[NS][SEG],
+                                               const double (&e1)[NS], const double (&e2)[NS]) {
+#pragma unroll
+    for (int y = 0; y < NS; ++y) {
+        double *Z = zb + (long)y * P * 4 * nl + t;
+        Z[((long)s * 4 + 0) * nl] = S[y][0];
+        Z[((long)s * 4 + 1) * nl] = SS[y][0];
+        Z[((long)s * 4 + 2) * nl] = e1[y];
+        Z[((long)s * 4 + 3) * nl] = e2[y];
+    }
+}
+
+template <int DIR, int NS, bool FINAL>
+__global__ void __launch_bounds__(SEGT) k_seg_lines(const double *__restrict__ f, const double *__restrict__ u,
+                                                    double *__restrict__ fx, double *__restrict__ fxx,
+                                                    double *__restrict__ zb, const double *__restrict__ tab_u,
+                                                    const double *__restrict__ tab_d, Geom G, int ext, int P,
+                                                    UccdSrc cs) {
+    long nl, st;
+    int N;
+    double dx;
+    if constexpr (DIR == 1) { nl = (long)G.nx * G.nz; N = G.ny + 2 * ext; st = G.sx; dx = G.dy; }
+    else { nl = (long)G.nx * G.ny; N = G.nz + 2 * ext; st = G.sxy; dx = G.dz; }
+    const long t = (long)blockIdx.x * SEGT + threadIdx.x;
+    if (t >= nl) return;
+    const int s = blockIdx.y;
+    const int a = seg_start(s, N, P), n = seg_start(s + 1, N, P) - a;
+    const int lb = (int)((unsigned)t / (unsigned)G.nx), la = (int)((unsigned)t - (unsigned)lb * (unsigned)G.nx);
+    const long line = DIR == 1 ? G.id(la, -ext, lb) : G.id(la, lb, -ext);
+    const long base = line + (long)a * st;
+    const double idx = 1.0 / dx, idx2 = idx * idx;
+    // f(a - 1 .. a + n): all SEG + 2 loads unconditional, at indices clamped
+    // to the line, so they are issued together; the sweeps read none of the
+    // values past the segment's range or the line's ends
+    double fv[SEG + 2];
+#pragma unroll
+    for (int k = 0; k < SEG + 2; ++k) fv[k] = f[line + (long)min(max(a - 1 + k, 0), N - 1) * st];
+    // upwind choice per point, from u loaded up front
+    unsigned up = 0;
+    if (FINAL && NS == 2) {
+        double uv[SEG];   // unconditional loads (clamped to the line), issued together
+#pragma unroll
+        for (int q = 0; q < SEG; ++q) uv[q] = u[line + (long)min(a + q, N - 1) * st];
+#pragma unroll
+        for (int q = 0; q < SEG; ++q)
+            up |= (q < n && (uv[q] > 0.0 || (cs.ge && uv[q] == 0.0))) ? (1u << q) : 0u;
+    }
+    double p1[NS], p2[NS], x1[NS], x2[NS], e1[NS], e2[NS];
+    if (FINAL) {
+        seg_neighbours<NS>(zb, t, nl, s, P, p1, p2, x1, x2);
+    } else {
+#pragma unroll
+        for (int y = 0; y < NS; ++y) p1[y] = p2[y] = x1[y] = x2[y] = 0.0;
+    }
+    double S[NS][SEG], SS[NS][SEG];
+    seg_sweep<NS>(fv, a, n, N, idx, idx2, tab_u, tab_d, cs, p1, p2, x1, x2, S, SS, e1, e2);
+    if (!FINAL) {
+        seg_store_ends<NS>(zb, t, nl, s, P, S, SS, e1, e2);
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < SEG; ++q)
+        if (q < n) {
+            const long o = base + (long)q * st;
+            if (NS == 1) {
+                fx[o] = S[0][q];
+                fxx[o] = SS[0][q];
+            } else {
+                fx[o] = ((up >> q) & 1u) ? S[0][q] : S[NS - 1][q];
+                if (fxx) fxx[o] = 0.5 * (SS[0][q] + SS[NS - 1][q]);
+            }
+        }
+}
+
+// The partitioned solve along x, the contiguous axis.  A block is one wave:
+// 64 consecutive x lines (rows) and one segment.  A row's segment values are
+// contiguous, so the wave moves them through an LDS tile of 64 rows x SEGX_TS
+// (odd, against bank conflicts on the per-row reads): the loads and stores
+// number the tile's elements row by row across the wave's lanes, a few rows
+// per instruction, each row a contiguous run; each lane then sweeps its own
+// row from registers.
+constexpr int SEGX_TS = SEG + 3;
+
+template <int NS, bool FINAL>
+__global__ void __launch_bounds__(WAVE) k_seg_lines_x(const double *__restrict__ f, const double *__restrict__ u,
+                                                      double *__restrict__ fx, double *__restrict__ fxx,
+                                                      double *__restrict__ zb, const double *__restrict__ tab_u,
+                                                      const double *__restrict__ tab_d, Geom G, int ext, int P,
+                                                      UccdSrc cs) {
+    __shared__ double tile[WAVE * SEGX_TS];
+    __shared__ long rowb[WAVE];
+    const long nl = (long)G.ny * G.nz;
+    const int N = G.nx + 2 * ext;
+    const int lane = threadIdx.x;
+    const long t0 = (long)blockIdx.x * WAVE, t = t0 + lane;
+    const int rows = (int)min((long)WAVE, nl - t0);
+    const int s = blockIdx.y;
+    const int a = seg_start(s, N, P), n = seg_start(s + 1, N, P) - a;
+    if (t < nl) {
+        const unsigned k = (unsigned)t / (unsigned)G.ny;
+        rowb[lane] = G.id(-ext, (int)((unsigned)t - k * (unsigned)G.ny), (int)k);
+    } else {
+        rowb[lane] = 0;
+    }
+    __syncthreads();
+    // f(a - 1 .. a + n) of every row, clipped to the line: tile columns k0 .. k1
+    const int k0 = a > 0 ? 0 : 1, k1 = a + n < N ? n + 1 : n;
+    const int W = k1 - k0 + 1, tot = rows * W;
+    {
+        // at most SEG + 2 elements per lane, all loads issued before the
+        // first is waited for (surplus lanes repeat the last element)
+        double v[SEG + 2];
+#pragma unroll
+        for (int m = 0; m < SEG + 2; ++m) {
+            const int e = min(lane + m * WAVE, tot - 1), r = e / W;
+            v[m] = f[rowb[r] + a - 1 + k0 + (e - r * W)];
+        }
+#pragma unroll
+        for (int m = 0; m < SEG + 2; ++m) {
+            const int e = lane + m * WAVE, r = e / W;
+            if (e < tot) tile[r * SEGX_TS + k0 + (e - r * W)] = v[m];
+        }
+    }
+    __syncthreads();
+    double fv[SEG + 2];
+#pragma unroll
+    for (int k = 0; k < SEG + 2; ++k) fv[k] = (k >= k0 && k <= k1) ? tile[lane * SEGX_TS + k] : 0.0;
+    // upwind choice per point: u through the same tile
+    unsigned up = 0;
+    if (FINAL && NS == 2) {
+        __syncthreads();
+        const int totu = rows * n;
+        double v[SEG];
+#pragma unroll
+        for (int m = 0; m < SEG; ++m) {
+            const int e = min(lane + m * WAVE, totu - 1), r = e / n;
+            v[m] = u[rowb[r] + a + (e - r * n)];
+        }
+#pragma unroll
+        for (int m = 0; m < SEG; ++m) {
+            const int e = lane + m * WAVE, r = e / n;
+            if (e < totu) tile[r * SEGX_TS + (e - r * n)] = v[m];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < SEG; ++q)
+            if (q < n) {
+                const double uv = tile[lane * SEGX_TS + q];
+                up |= (uv > 0.0 || (cs.ge && uv == 0.0)) ? (1u << q) : 0u;
+            }
+    }
+    const double idx = 1.0 / G.dx, idx2 = idx * idx;
+    double p1[NS], p2[NS], x1[NS], x2[NS], e1[NS], e2[NS];
+    double S[NS][SEG], SS[NS][SEG];
+    if (t < nl) {
+        if (FINAL) {
+            seg_neighbours<NS>(zb, t, nl, s, P, p1, p2, x1, x2);
+        } else {
+#pragma unroll
+            for (int y = 0; y < NS; ++y) p1[y] = p2[y] = x1[y] = x2[y] = 0.0;
+        }
+        seg_sweep<NS>(fv, a, n, N, idx, idx2, tab_u, tab_d, cs, p1, p2, x1, x2, S, SS, e1, e2);
+        if (!FINAL) seg_store_ends<NS>(zb, t, nl, s, P, S, SS, e1, e2);
+    }
+    if (!FINAL) return;
+    // fx, then fxx, out through the tile
+#pragma unroll
+    for (int o2 = 0; o2 < 2; ++o2) {
+        if (o2 == 1 && !fxx) break;
+        __syncthreads();   // the tile's previous contents are consumed
+        if (t < nl) {
+#pragma unroll
+            for (int q = 0; q < SEG; ++q)
+                if (q < n) {
+                    double v;
+                    if (NS == 1) v = o2 == 0 ? S[0][q] : SS[0][q];
+                    else v = o2 == 0 ? (((up >> q) & 1u) ? S[0][q] : S[NS - 1][q]) : 0.5 * (SS[0][q] + SS[NS - 1][q]);
+                    tile[lane * SEGX_TS + q] = v;
+                }
+        }
+        __syncthreads();
+        double *out = o2 == 0 ? fx : fxx;
+        for (int e = lane; e < rows * n; e += WAVE) {
+            const int r = e / n, q = e - r * n;
+            out[rowb[r] + a + q] = tile[r * SEGX_TS + q];
+        }
+    }
+}
+
+// The reduced block-tridiagonal solve of one line and system (blockIdx.y:
+// 0 the CCD or up-wind system, 1 the down-wind one): forward d'_s = d_s -
+// m_s d'_{s-1}, backward u_s = Dinv_s (d'_s - U_s u_{s+1}), with d_s =
+// (x0(b_s), x0(a_{s+1})) read from zb and u_s = (x(b_s), x(a_{s+1})) written
+// back over them.
 __global__ void __launch_bounds__(SEGT) k_seg_reduce(double *__restrict__ zb, const double *__restrict__ red_u,
                                                      const double *__restrict__ red_d, long nl, int P) {
     const long t = (long)blockIdx.x * SEGT + threadIdx.x;
     if (t >= nl) return;
-    const int NI = P - 1;
+    const int NI = P - 1, y = blockIdx.y;
+    const double *R = y ? red_d : red_u;
+    double *Z = zb + (long)y * P * 4 * nl + t;
+    // slots of interface s: (s, 2), (s, 3), (s + 1, 0), (s + 1, 1)
+    auto slot = [&](int s, int k) -> double & {
+        return Z[(k < 2 ? (long)s * 4 + 2 + k : (long)(s + 1) * 4 + k - 2) * nl];
+    };
+    double dp[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < NI; ++s) {
+        const double *m = R + (long)s * NRED;
+        double d[4];
 #pragma unroll
-    for (int y = 0; y < NS; ++y) {
-        const double *R = y ? red_d : red_u;
-        double *Z = zb + (long)y * P * 4 * nl + t;
-        // slots of interface s: (s, 2), (s, 3), (s + 1, 0), (s + 1, 1)
-        auto slot = [&](int s, int k) -> double & {
-            return Z[(k < 2 ? (long)s * 4 + 2 + k : (long)(s + 1) * 4 + k - 2) * nl];
-        };
-        double dp[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int s = 0; s < NI; ++s) {
+        for (int k = 0; k < 4; ++k) d[k] = slot(s, k);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            d[r] -= m[r * 4 + 0] * dp[0] + m[r * 4 + 1] * dp[1] + m[r * 4 + 2] * dp[2] + m[r * 4 + 3] * dp[3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (s < NI - 1) slot(s, k) = d[k];
+            dp[k] = d[k];
+        }
+    }
+    double un[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int s = NI - 1; s >= 0; --s) {
+        const double *Di = R + (long)s * NRED + 16, *U = Di + 16;
+        double w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[k] = s == NI - 1 ? dp[k] : slot(s, k);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            w[r] -= U[r * 4 + 0] * un[0] + U[r * 4 + 1] * un[1] + U[r * 4 + 2] * un[2] + U[r * 4 + 3] * un[3];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            un[r] = Di[r * 4 + 0] * w[0] + Di[r * 4 + 1] * w[1] + Di[r * 4 + 2] * w[2] + Di[r * 4 + 3] * w[3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) slot(s, k) = un[k];
+    }
+}
+
+// k_seg_reduce for lines of at most MAXI interfaces: all d_s are loaded up
+// front into registers, so the loads of a line overlap instead of costing one
+// memory latency per interface and sweep (MAXI = 20: 160 of the registers),
+// and the coefficients come from LDS.
+constexpr int SEG_MAXI = 20;
+template <int MAXI>
+__global__ void __launch_bounds__(SEGT) k_seg_reduce_r(double *__restrict__ zb, const double *__restrict__ red_u,
+                                                       const double *__restrict__ red_d, long nl, int P) {
+    // the system's reduced LU staged in LDS: read through the scalar cache it
+    // costs one cache-miss latency per interface and sweep on the critical path
+    __shared__ double R[MAXI * NRED];
+    const int NI = P - 1, y = blockIdx.y;
+    const double *Rg = y ? red_d : red_u;
+    for (int e = threadIdx.x; e < NI * NRED; e += SEGT) R[e] = Rg[e];
+    __syncthreads();
+    const long t = (long)blockIdx.x * SEGT + threadIdx.x;
+    if (t >= nl) return;
+    double *Z = zb + (long)y * P * 4 * nl + t;
+    auto slot = [&](int s, int k) -> double & {
+        return Z[(k < 2 ? (long)s * 4 + 2 + k : (long)(s + 1) * 4 + k - 2) * nl];
+    };
+    double d[MAXI][4];
+#pragma unroll
+    for (int s = 0; s < MAXI; ++s)
+        if (s < NI) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) d[s][k] = slot(s, k);
+        }
+#pragma unroll
+    for (int s = 1; s < MAXI; ++s)
+        if (s < NI) {
             const double *m = R + (long)s * NRED;
-            double d[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) d[k] = slot(s, k);
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                d[r] -= m[r * 4 + 0] * dp[0] + m[r * 4 + 1] * dp[1] + m[r * 4 + 2] * dp[2] + m[r * 4 + 3] * dp[3];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (s < NI - 1) slot(s, k) = d[k];
-                dp[k] = d[k];
-            }
+                d[s][r] -= m[r * 4 + 0] * d[s - 1][0] + m[r * 4 + 1] * d[s - 1][1] + m[r * 4 + 2] * d[s - 1][2] +
+                           m[r * 4 + 3] * d[s - 1][3];
         }
-        double un[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int s = NI - 1; s >= 0; --s) {
+    double un[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = MAXI - 1; s >= 0; --s)
+        if (s < NI) {
             const double *Di = R + (long)s * NRED + 16, *U = Di + 16;
             double w[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) w[k] = s == NI - 1 ? dp[k] : slot(s, k);
-#pragma unroll
             for (int r = 0; r < 4; ++r)
-                w[r] -= U[r * 4 + 0] * un[0] + U[r * 4 + 1] * un[1] + U[r * 4 + 2] * un[2] + U[r * 4 + 3] * un[3];
+                w[r] = d[s][r] -
+                       (U[r * 4 + 0] * un[0] + U[r * 4 + 1] * un[1] + U[r * 4 + 2] * un[2] + U[r * 4 + 3] * un[3]);
 #pragma unroll
             for (int r = 0; r < 4; ++r)
                 un[r] = Di[r * 4 + 0] * w[0] + Di[r * 4 + 1] * w[1] + Di[r * 4 + 2] * w[2] + Di[r * 4 + 3] * w[3];
 #pragma unroll
             for (int k = 0; k < 4; ++k) slot(s, k) = un[k];
         }
-    }
 }
 
 // the partitioned solve of the y (dir 1) or z (dir 2) lines: NS = 1 CCD
@@ -805,19 +1017,30 @@ template <int DIR, int NS>
 static int seg_solve(fcfd_ctx *c, int ext, int k0, int k1, const double *f, const double *u, double *fx, double *fxx,
                      UccdSrc cs) {
     const Geom &G = c->G;
-    const long nl = DIR == 1 ? (long)G.nx * G.nz : (long)G.nx * G.ny;
-    const int N = (DIR == 1 ? G.ny : G.nz) + 2 * ext;
+    const long nl = DIR == 0 ? (long)G.ny * G.nz : (DIR == 1 ? (long)G.nx * G.nz : (long)G.nx * G.ny);
+    const int N = (DIR == 0 ? G.nx : (DIR == 1 ? G.ny : G.nz)) + 2 * ext;
     const int P = seg_count(N);
     const double *tu = c->ccd_seg[k0 * 3 + DIR], *td = c->ccd_seg[k1 * 3 + DIR];
-    const unsigned gx = (unsigned)((nl + SEGT - 1) / SEGT);
+    const unsigned gx = (unsigned)((nl + SEGT - 1) / SEGT), gw = (unsigned)((nl + WAVE - 1) / WAVE);
+    auto lines = [&](auto fin) {
+        constexpr bool FIN = decltype(fin)::value;
+        if constexpr (DIR == 0)
+            hipLaunchKernelGGL((k_seg_lines_x<NS, FIN>), dim3(gw, (unsigned)P), dim3(WAVE), 0, c->stream, f, u, fx,
+                               fxx, c->ccd_z, tu, td, G, ext, P, cs);
+        else
+            hipLaunchKernelGGL((k_seg_lines<DIR, NS, FIN>), dim3(gx, (unsigned)P), dim3(SEGT), 0, c->stream, f, u,
+                               fx, fxx, c->ccd_z, tu, td, G, ext, P, cs);
+    };
     if (P > 1) {
-        hipLaunchKernelGGL((k_seg_lines<DIR, NS, false>), dim3(gx, (unsigned)P), dim3(SEGT), 0, c->stream, f, u, fx,
-                           fxx, c->ccd_z, tu, td, G, ext, P, cs);
-        hipLaunchKernelGGL((k_seg_reduce<NS>), dim3(gx), dim3(SEGT), 0, c->stream, c->ccd_z, tu + (long)N * NPACK,
-                           td + (long)N * NPACK, nl, P);
+        lines(std::false_type{});
+        if (P - 1 <= SEG_MAXI)
+            hipLaunchKernelGGL(k_seg_reduce_r<SEG_MAXI>, dim3(gx, NS), dim3(SEGT), 0, c->stream, c->ccd_z,
+                               tu + (long)N * NPACK, td + (long)N * NPACK, nl, P);
+        else
+            hipLaunchKernelGGL(k_seg_reduce, dim3(gx, NS), dim3(SEGT), 0, c->stream, c->ccd_z, tu + (long)N * NPACK,
+                               td + (long)N * NPACK, nl, P);
     }
-    hipLaunchKernelGGL((k_seg_lines<DIR, NS, true>), dim3(gx, (unsigned)P), dim3(SEGT), 0, c->stream, f, u, fx, fxx,
-                       c->ccd_z, tu, td, G, ext, P, cs);
+    lines(std::true_type{});
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
@@ -835,6 +1058,7 @@ int ccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, double *fx, do
     ProfScope ps_(c, R_CCD);
     const int kc = set ? MAT_M3_CCD : MAT_CCD;
     const double *co = c->ccd_coef[kc * 3 + dir];
+    if (dir == 0 && c->ccd_seg[kc * 3 + 0]) return seg_solve<0, 1>(c, ext, kc, kc, f, nullptr, fx, fxx, UccdSrc{});
     if (dir == 1 && c->ccd_seg[kc * 3 + 1]) return seg_solve<1, 1>(c, ext, kc, kc, f, nullptr, fx, fxx, UccdSrc{});
     if (dir == 2 && c->ccd_seg[kc * 3 + 2]) return seg_solve<2, 1>(c, ext, kc, kc, f, nullptr, fx, fxx, UccdSrc{});
     if (dir == 0)
@@ -1183,6 +1407,7 @@ int uccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, const double 
     const double *cu = c->ccd_coef[ku * 3 + dir], *cd = c->ccd_coef[kd * 3 + dir];
     const UccdSrc cs = set ? UCCD_M3 : UCCD_N3;
     ProfScope ps_(c, R_CCD);
+    if (dir == 0 && c->ccd_seg[ku * 3 + 0]) return seg_solve<0, 2>(c, ext, ku, kd, f, u, fx, fxx, cs);
     if (dir == 1 && c->ccd_seg[ku * 3 + 1]) return seg_solve<1, 2>(c, ext, ku, kd, f, u, fx, fxx, cs);
     if (dir == 2 && c->ccd_seg[ku * 3 + 2]) return seg_solve<2, 2>(c, ext, ku, kd, f, u, fx, fxx, cs);
     if (dir == 0 && G.ntot < (1L << 31))

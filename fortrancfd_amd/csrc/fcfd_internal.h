@@ -254,6 +254,30 @@ __device__ __forceinline__ int case_of(int k) { return DIM == 2 ? k : 0; }
 
 // launch helpers --------------------------------------------------------
 inline dim3 tile_blk(const fcfd_ctx *c, int kernel) { return dim3(c->tile[kernel][0], c->tile[kernel][1]); }
+// interior cell (i, j, k) of a one-dimensional launch over the nx * ny * nz
+// interior cells in memory order (x fastest; no idle lanes at the row ends
+// as a 2D tile has when nx is not a multiple of its width), the blocks in
+// xcd_block's XCD-contiguous order; false past the last cell.  The launch
+// (grid_flat) needs nx * ny * nz < 2^32.
+__device__ __forceinline__ bool cell_of_flat(const Geom &G, int &i, int &j, int &k) {
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const unsigned t = (unsigned)bx * blockDim.x + threadIdx.x;
+    const unsigned nxy = (unsigned)G.nx * (unsigned)G.ny;
+    if (t >= nxy * (unsigned)G.nz) return false;
+    const unsigned kk = t / nxy, r = t - kk * nxy, jj = r / (unsigned)G.nx;
+    i = (int)(r - jj * (unsigned)G.nx);
+    j = (int)jj;
+    k = (int)kk;
+    return true;
+}
+
+inline bool flat_fits(const Geom &G) { return (double)G.nx * G.ny * G.nz < 4294967296.0 - 1024.0; }
+
+inline dim3 grid_flat(const Geom &G, int blk) {
+    return dim3((unsigned)(((long)G.nx * G.ny * G.nz + blk - 1) / blk));
+}
+
 inline dim3 grid_interior(const Geom &G, dim3 blk) {
     return dim3((G.nx + blk.x - 1) / blk.x, (G.ny + blk.y - 1) / blk.y, G.nz);
 }

@@ -130,11 +130,20 @@ __device__ __forceinline__ double sq(double x) { return x * x; }
 __device__ __forceinline__ void m3_line_weno(const double *__restrict__ T, long id, long st, int i, int n, double ih,
                                              double &up, double &um) {
     double f[6];   // F(i-2) .. F(i+3); the differences times 1 / h
+    if (i >= 2 && i <= n - 4) {
+        // no index clamped: the six differences share the seven values T(i-3 .. i+3)
+        double g[7];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const int m = clampi(i - 2 + q, 0, n - 1);
-        const long o = id + (long)(m - i) * st;
-        f[q] = (T[o] - T[o - st]) * ih;
+        for (int r = 0; r < 7; ++r) g[r] = T[id + (long)(r - 3) * st];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) f[q] = (g[q + 1] - g[q]) * ih;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int m = clampi(i - 2 + q, 0, n - 1);
+            const long o = id + (long)(m - i) * st;
+            f[q] = (T[o] - T[o - st]) * ih;
+        }
     }
     um = weno_left(f[0], f[1], f[2], f[3], f[4], kEps);
     up = weno_right(f[1], f[2], f[3], f[4], f[5], kEps);
@@ -146,7 +155,7 @@ __device__ __forceinline__ void m3_line_weno(const double *__restrict__ T, long 
 __global__ void k_m3_redis_a(const double *__restrict__ T, const double *__restrict__ S, double *__restrict__ Gr,
                              double *__restrict__ D, double *__restrict__ A, double *__restrict__ B, Geom G, double w) {
     int i, j, k;
-    if (!cell_of_block(G, i, j, k)) return;
+    if (!cell_of_flat(G, i, j, k)) return;
     const long id = G.id(i, j, k);
     double up, um, vp, vm, wp, wm;
     m3_line_weno(T, id, 1, i, G.nx, 1.0 / G.dx, up, um);
@@ -173,18 +182,20 @@ __global__ void k_m3_redis_a(const double *__restrict__ T, const double *__restr
 // on T in place: s = (SGN - C) |grad| - SGN, C = delta (51 a + sum a) /
 // (51 b + sum b).  A thread walks M3_KC planes of one column and carries the
 // 3 x 3 plane sums of planes k-1, k, k+1, so each plane's nine values of a
-// and b are loaded once instead of three times.
-constexpr int M3_KC = 16;
-__global__ void __launch_bounds__(256) k_m3_redis_b(double *__restrict__ T, const double *__restrict__ S,
+// and b are loaded once instead of three times.  The columns (i, j) are
+// numbered in one dimension (x fastest), blockIdx.y picks the planes.
+constexpr int M3_KC = 16, M3_BT = 256;
+__global__ void __launch_bounds__(M3_BT) k_m3_redis_b(double *__restrict__ T, const double *__restrict__ S,
                                                      const double *__restrict__ Gr, const double *__restrict__ D,
                                                      const double *__restrict__ A, const double *__restrict__ B,
                                                      double *__restrict__ s0, double *__restrict__ s1, Geom G,
                                                      double rdt, int stage) {
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
-    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y;
-    if (i >= G.nx || j >= G.ny) return;
-    const int k0 = bz_ * M3_KC, k1 = min(k0 + M3_KC, G.nz);
+    const unsigned t = (unsigned)bx_ * M3_BT + threadIdx.x;
+    if (t >= (unsigned)G.nx * (unsigned)G.ny) return;
+    const int j = (int)(t / (unsigned)G.nx), i = (int)(t - (unsigned)j * (unsigned)G.nx);
+    const int k0 = by_ * M3_KC, k1 = min(k0 + M3_KC, G.nz);
     const int im = clampi(i - 1, 0, G.nx - 1), ip = clampi(i + 1, 0, G.nx - 1);
     const int jm = clampi(j - 1, 0, G.ny - 1), jp = clampi(j + 1, 0, G.ny - 1);
     // 3 x 3 sums of a and b over plane k (clamped), rows j-1, j, j+1
@@ -541,21 +552,23 @@ static int m3_redistance(fcfd_ctx *c, double *T) {
     const Geom &G = c->G;
     double **F = c->f;
     dim3 grd = grid_interior(G, blk_i());
+    if (!flat_fits(G)) return FCFD_ERR_ARG;
     hipLaunchKernelGGL(k_m3_sign, grd, blk_i(), 0, c->stream, T, F[FCFD_SIGN], G, c->pr.ls_wid);
     FCFD_CHECK(hipGetLastError());
+    const dim3 ga = grid_flat(G, M3_BT);
+    const dim3 gb((unsigned)(((long)G.nx * G.ny + M3_BT - 1) / M3_BT), (unsigned)((G.nz + M3_KC - 1) / M3_KC));
     const int n = m3_redis_steps(c);
     for (int it = 0; it < n; ++it)
         for (int s = 0; s < 3; ++s) {
             {
                 ProfScope ps_(c, R_REDIS_A);
-                hipLaunchKernelGGL(k_m3_redis_a, grd, blk_i(), 0, c->stream, T, F[FCFD_SIGN], F[FCFD_GRAD],
+                hipLaunchKernelGGL(k_m3_redis_a, ga, dim3(M3_BT), 0, c->stream, T, F[FCFD_SIGN], F[FCFD_GRAD],
                                    F[FCFD_DELTA], c->s2, c->s3, G, c->pr.ls_wid);
                 FCFD_CHECK(hipGetLastError());
             }
             {
                 ProfScope ps_(c, R_REDIS_B);
-                const dim3 gkm(grd.x, grd.y, (G.nz + M3_KC - 1) / M3_KC);
-                hipLaunchKernelGGL(k_m3_redis_b, gkm, blk_i(), 0, c->stream, T, F[FCFD_SIGN], F[FCFD_GRAD],
+                hipLaunchKernelGGL(k_m3_redis_b, gb, dim3(M3_BT), 0, c->stream, T, F[FCFD_SIGN], F[FCFD_GRAD],
                                    F[FCFD_DELTA], c->s2, c->s3, c->l1, c->l2, G, c->pr.rdt, s);
                 FCFD_CHECK(hipGetLastError());
             }
