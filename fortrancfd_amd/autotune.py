@@ -28,7 +28,7 @@ from .solver import FcfdError
 
 # (pre, post) sweeps, sweeps on the small levels, V-cycles per outer
 # iteration, FP32 coarse tiling (0 auto, 1 / 2 sweeps per launch, 3 colour
-# launches), FP16 density in the FP32 level-0 sweeps
+# launches, 4 auto from level 1 down), FP16 density in the FP32 level-0 sweeps
 def _s(nu1, nu2, nuc, inner, fused, half=1):
     return dict(ppe_nu1=nu1, ppe_nu2=nu2, ppe_nu_coarse=nuc, ppe_inner=inner, ppe_fused=fused, ppe_rho_half=half)
 
@@ -36,6 +36,7 @@ def _s(nu1, nu2, nuc, inner, fused, half=1):
 SCHEDULES = [
     _s(2, 2, 6, 1, 0), _s(2, 2, 6, 1, 0, half=0), _s(2, 2, 6, 1, 1), _s(2, 2, 6, 1, 2), _s(2, 2, 6, 1, 3),
     _s(1, 1, 4, 1, 0), _s(2, 2, 4, 1, 0), _s(2, 2, 8, 1, 0), _s(3, 3, 6, 1, 0), _s(1, 1, 6, 2, 0),
+    _s(2, 2, 6, 1, 4),
 ]
 KEYS = ("ppe_nu1", "ppe_nu2", "ppe_nu_coarse", "ppe_inner", "ppe_fused", "ppe_rho_half")
 

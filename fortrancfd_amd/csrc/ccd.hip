@@ -39,6 +39,17 @@ enum { PF_A1, PF_B1, PF_AA1, PF_BB1, PB_A3, PB_B3, PB_AA3, PB_BB3, PB_A2D, PB_AA
 // `MPLS_RisingBubble/20190612/SCHEME_COMPACT.F90:189-200,482-490`)
 enum { MAT_CCD = 0, MAT_UCCD_UP = 1, MAT_UCCD_DOWN = 2, MAT_M3_CCD = 3, MAT_M3_UP = 4, MAT_M3_DOWN = 5, NMAT = 6 };
 
+// first-derivative source coefficients of the upwind system (the downwind
+// one mirrors them) and the choice at u = 0: newOpenMP takes the downwind
+// value there (ccd_solvers.f90:400-412), MPLS_RisingBubble the upwind one
+// (SCHEME_COMPACT.F90:17-25)
+struct UccdSrc {
+    double c1, c2, c3;
+    int ge;
+};
+constexpr UccdSrc UCCD_N3{-1.93596119008109, 1.99692238016218, -0.06096119008109, 0};
+constexpr UccdSrc UCCD_M3{-1.93629468537011, 1.99758937102742, -0.061294685370110, 1};
+
 // Matrix of one kind on a line of N points, spacing dx: interior rows of
 // init_ccd / init_uccd and the closure rows of working_matrix_bc
 // (`.../lib/solver/ccd_solvers.f90:76-210`).  Row i: eq. 1 (first
@@ -212,8 +223,9 @@ static void mul4(const double *a, const double *b, double *c) {
 
 static int seg_count(int N) { return (N + SEG - 1) / SEG; }
 
-// segment factors and reduced block LU of one line matrix, in one buffer:
-// [N x NPACK segment factors][(P - 1) x NRED reduced LU]
+// segment factors, reduced block LU and segment end functionals of one line
+// matrix, in one buffer: [N x NPACK segment factors][(P - 1) x NRED reduced
+// LU][P x 4 x (SEG + 2) end functionals]
 static void ccd_seg_tables(int kind, int N, double dx, std::vector<double> &out) {
     const LineMat M0 = ccd_matrix(kind, N, dx);
     const int P = seg_count(N);
@@ -278,8 +290,52 @@ static void ccd_seg_tables(int kind, int N, double dx, std::vector<double> &out)
         std::copy(Dm, Dm + 16, Dinv_prev);
         std::copy(U, U + 16, Uprev);
     }
+    // end functionals: x0(a) and x0(b) of segment s as weights on f(a - 1 ..
+    // a + n) (the sources are linear in f), E[(s * 4 + c) * (SEG + 2) + k],
+    // c = 0, 1 the (f', f'') at a, 2, 3 at b; found by solving for unit f
+    std::vector<double> E((size_t)P * 4 * (SEG + 2), 0.0);
+    {
+        const double idx = 1.0 / dx, idx2 = idx * idx;
+        const int form = kind % 3;   // 0 CCD, 1 up-wind, 2 down-wind
+        const UccdSrc cs = kind >= MAT_M3_CCD ? UCCD_M3 : UCCD_N3;
+        for (int s = 0; s < P; ++s) {
+            const int a = seg_start(s, N, P), b = seg_start(s + 1, N, P) - 1, n = b - a + 1;
+            for (int k = 0; k <= n + 1; ++k) {
+                const int fi = a - 1 + k;
+                if (fi < 0 || fi >= N) continue;
+                auto F = [&](int j) { return j == fi ? 1.0 : 0.0; };
+                for (int q = 0; q < n; ++q) {
+                    const int i = a + q;
+                    double r1, r2;
+                    if (i == 0) {
+                        r1 = (-3.5 * F(0) + 4.0 * F(1) - 0.5 * F(2)) * idx;
+                        r2 = (34.0 / 3.0 * F(0) - 83.0 / 4.0 * F(1) + 10.0 * F(2) - 7.0 / 12.0 * F(3)) * idx2;
+                    } else if (i == N - 1) {
+                        const double e0 = F(N - 1), e1 = F(N - 2), e2 = F(N - 3), e3 = F(N - 4);
+                        r1 = -(-3.5 * e0 + 4.0 * e1 - 0.5 * e2) * idx;
+                        r2 = (34.0 / 3.0 * e0 - 83.0 / 4.0 * e1 + 10.0 * e2 - 7.0 / 12.0 * e3) * idx2;
+                    } else {
+                        const double fm = F(i - 1), fc = F(i), fp = F(i + 1);
+                        r2 = (3.0 * fm - 6.0 * fc + 3.0 * fp) * idx2;
+                        r1 = form == 0 ? 15.0 / 16.0 * (fp - fm) * idx
+                                       : (form == 1 ? (cs.c1 * fm + cs.c2 * fc + cs.c3 * fp) * idx
+                                                    : -(cs.c3 * fm + cs.c2 * fc + cs.c1 * fp) * idx);
+                    }
+                    r[2 * q] = r1;
+                    r[2 * q + 1] = r2;
+                }
+                seg_solve_host(lt, a, b, r.data(), x.data());
+                double *e = &E[(size_t)s * 4 * (SEG + 2) + k];
+                e[0] = x[0];
+                e[SEG + 2] = x[1];
+                e[2 * (SEG + 2)] = x[2 * (n - 1)];
+                e[3 * (SEG + 2)] = x[2 * (n - 1) + 1];
+            }
+        }
+    }
     out = lt;
     out.insert(out.end(), red.begin(), red.end());
+    out.insert(out.end(), E.begin(), E.end());
 }
 
 int ccd_setup(fcfd_ctx *c) {
@@ -608,42 +664,77 @@ __global__ void __launch_bounds__(CCDX_LINES) k_ccd_lines_x(const double *__rest
     }
 }
 
-// CCD of f along direction dir: set 0 the newOpenMP lines through the ghosts
-// (n + 6 points), set 1 the MPLS_RisingBubble interior lines (n points)
-// first-derivative source coefficients of the upwind system (the downwind
-// one mirrors them) and the choice at u = 0: newOpenMP takes the downwind
-// value there (ccd_solvers.f90:400-412), MPLS_RisingBubble the upwind one
-// (SCHEME_COMPACT.F90:17-25)
-struct UccdSrc {
-    double c1, c2, c3;
-    int ge;
-};
-constexpr UccdSrc UCCD_N3{-1.93596119008109, 1.99692238016218, -0.06096119008109, 0};
-constexpr UccdSrc UCCD_M3{-1.93629468537011, 1.99758937102742, -0.061294685370110, 1};
-
-// Partitioned y / z line solve, one thread per (line, segment); the wave's
-// 64 lanes are 64 consecutive lines (consecutive i: coalesced row accesses)
-// of one segment, so every coefficient read is uniform over the wave.  NS =
-// 1: CCD (asgn_src_ccd); NS = 2: the up- and down-wind UCCD systems side by
-// side, as in k_uccd_lines.  FINAL = false: the segment's own solution x0
-// (no neighbours), whose end values go to zb; FINAL = true: the segment's
-// system with the neighbour values x(a-1), x(b+1) that k_seg_reduce left in
-// zb, the result written to fx, fxx.  zb holds per system, segment s and
-// line t four values at zb[((sys * P + s) * 4 + c) * nl + t]: c = 0, 1 the
-// (f', f'') of the segment's first point, c = 2, 3 of its last.
+// Partitioned line solve kernels.  zb holds per system y, segment s and line
+// t four values at zb[((y * P + s) * 4 + c) * nl + t]: c = 0, 1 the
+// (f', f'') of the segment's first point, c = 2, 3 of its last: the ends of
+// the segment's own solution x0 (k_seg_ends*), then, after k_seg_reduce*,
+// the line's solution at the interfaces.  NS = 1: CCD (asgn_src_ccd); NS =
+// 2: the up- and down-wind UCCD systems side by side (root_solve "uccd").
+// For y and z lines a thread is one (line, segment) and the wave's 64 lanes
+// are 64 consecutive lines (consecutive i: coalesced row accesses) of one
+// segment, so every coefficient read is uniform over the wave.
 constexpr int SEGT = 256;
 
+// line geometry of the y (DIR 1) or z (DIR 2) lines
+template <int DIR>
+__device__ __forceinline__ void seg_line_geom(const Geom &G, int ext, long &nl, int &N, long &st, double &dx) {
+    if constexpr (DIR == 1) { nl = (long)G.nx * G.nz; N = G.ny + 2 * ext; st = G.sx; dx = G.dy; }
+    else { nl = (long)G.nx * G.ny; N = G.nz + 2 * ext; st = G.sxy; dx = G.dz; }
+}
+
+// x0(a), x0(b) of every system from the end functionals E (uniform over the
+// wave: one segment), fv[k] = f(a - 1 + k)
+template <int NS>
+__device__ __forceinline__ void seg_ends_store(const double (&fv)[SEG + 2], const double *end_u,
+                                               const double *end_d, double *__restrict__ zb, long t, long nl, int s,
+                                               int P) {
+#pragma unroll
+    for (int y = 0; y < NS; ++y) {
+        const double *E = (y ? end_d : end_u) + (long)s * 4 * (SEG + 2);
+        double *Z = zb + (long)y * P * 4 * nl + t;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < SEG + 2; ++k) acc = fma(E[c * (SEG + 2) + k], fv[k], acc);
+            Z[((long)s * 4 + c) * nl] = acc;
+        }
+    }
+}
+
+// the segment ends of the y / z lines: SEG + 2 loads and 4 NS dot products
+// per thread, no recurrence
+template <int DIR, int NS>
+__global__ void __launch_bounds__(SEGT) k_seg_ends(const double *__restrict__ f, double *__restrict__ zb,
+                                                   const double *__restrict__ end_u,
+                                                   const double *__restrict__ end_d, Geom G, int ext, int P) {
+    long nl, st;
+    int N;
+    double dx;
+    seg_line_geom<DIR>(G, ext, nl, N, st, dx);
+    const long t = (long)blockIdx.x * SEGT + threadIdx.x;
+    if (t >= nl) return;
+    const int s = blockIdx.y, a = seg_start(s, N, P);
+    const int lb = (int)((unsigned)t / (unsigned)G.nx), la = (int)((unsigned)t - (unsigned)lb * (unsigned)G.nx);
+    const long line = DIR == 1 ? G.id(la, -ext, lb) : G.id(la, lb, -ext);
+    // all loads unconditional (indices clamped to the line), issued together;
+    // the functionals weigh the positions off the line with 0
+    double fv[SEG + 2];
+#pragma unroll
+    for (int k = 0; k < SEG + 2; ++k) fv[k] = f[line + (long)min(max(a - 1 + k, 0), N - 1) * st];
+    seg_ends_store<NS>(fv, end_u, end_d, zb, t, nl, s, P);
+}
+
 // The sweeps of one segment (n <= SEG points from line index a, N points on
-// the line) for NS systems, fv[k] = f(a - 1 + k) where the line has it.
-// p1, p2 seed the forward sweep (x(a-1); zero for the segment's own
-// system), x1, x2 the backward one (x(b+1)).  On return S[y][q], SS[y][q]
-// hold the (f', f'') of system y at point a + q, and e1, e2 those at b.
+// the line) for NS systems with the segment's neighbours known: fv[k] =
+// f(a - 1 + k) where the line has it, p1, p2 = x(a-1) seed the forward
+// sweep, x1, x2 = x(b+1) the backward one (zero past the line's ends).  On
+// return S[y][q], SS[y][q] hold the (f', f'') of system y at point a + q.
 template <int NS>
 __device__ __forceinline__ void seg_sweep(const double (&fv)[SEG + 2], int a, int n, int N, double idx, double idx2,
                                           const double *tab_u, const double *tab_d, const UccdSrc &cs,
                                           double (&p1)[NS], double (&p2)[NS], double (&x1)[NS], double (&x2)[NS],
-                                          double (&S)[NS][SEG], double (&SS)[NS][SEG], double (&e1)[NS],
-                                          double (&e2)[NS]) {
+                                          double (&S)[NS][SEG], double (&SS)[NS][SEG]) {
     const double *tab[2] = {tab_u, tab_d};
 #pragma unroll
     for (int q = 0; q < SEG; ++q) {
@@ -656,9 +747,9 @@ __device__ __forceinline__ void seg_sweep(const double (&fv)[SEG + 2], int a, in
                 r2 = (34.0 / 3.0 * f0 - 83.0 / 4.0 * f1 + 10.0 * f2 - 7.0 / 12.0 * f3) * idx2;
                 if (NS == 2) r1[NS - 1] = r1[0];
             } else if (i == N - 1) {
-                const double e0 = fv[q + 1], e1_ = fv[q], e2_ = fv[q >= 1 ? q - 1 : 0], e3 = fv[q >= 2 ? q - 2 : 0];
-                r1[0] = -(-3.5 * e0 + 4.0 * e1_ - 0.5 * e2_) * idx;
-                r2 = (34.0 / 3.0 * e0 - 83.0 / 4.0 * e1_ + 10.0 * e2_ - 7.0 / 12.0 * e3) * idx2;
+                const double e0 = fv[q + 1], e1 = fv[q], e2 = fv[q >= 1 ? q - 1 : 0], e3 = fv[q >= 2 ? q - 2 : 0];
+                r1[0] = -(-3.5 * e0 + 4.0 * e1 - 0.5 * e2) * idx;
+                r2 = (34.0 / 3.0 * e0 - 83.0 / 4.0 * e1 + 10.0 * e2 - 7.0 / 12.0 * e3) * idx2;
                 if (NS == 2) r1[NS - 1] = r1[0];
             } else {
                 const double fm = fv[q], fc = fv[q + 1], fp = fv[q + 2];
@@ -693,16 +784,12 @@ __device__ __forceinline__ void seg_sweep(const double (&fv)[SEG + 2], int a, in
                 x2[y] = co[PB_A2D] * SSi - co[PB_AA2D] * Si;
                 S[y][q] = x1[y];
                 SS[y][q] = x2[y];
-                if (q == n - 1) {
-                    e1[y] = x1[y];
-                    e2[y] = x2[y];
-                }
             }
         }
     }
 }
 
-// x(a-1) and x(b+1) of line t's segment s from the reduced solve (FINAL)
+// x(a-1) and x(b+1) of line t's segment s from the reduced solve
 template <int NS>
 __device__ __forceinline__ void seg_neighbours(const double *__restrict__ zb, long t, long nl, int s, int P,
                                                double (&p1)[NS], double (&p2)[NS], double (&x1)[NS],
@@ -716,32 +803,18 @@ __device__ __forceinline__ void seg_neighbours(const double *__restrict__ zb, lo
     }
 }
 
-// the segment's end values x0(a), x0(b) into zb (not FINAL)
-template <int NS>
-__device__ __forceinline__ void seg_store_ends(double *__restrict__ zb, long t, long nl, int s, int P,
-                                               const double (&S)[NS][SEG], const double (&SS)[NS][SEG],
-                                               const double (&e1)[NS], const double (&e2)[NS]) {
-#pragma unroll
-    for (int y = 0; y < NS; ++y) {
-        double *Z = zb + (long)y * P * 4 * nl + t;
-        Z[((long)s * 4 + 0) * nl] = S[y][0];
-        Z[((long)s * 4 + 1) * nl] = SS[y][0];
-        Z[((long)s * 4 + 2) * nl] = e1[y];
-        Z[((long)s * 4 + 3) * nl] = e2[y];
-    }
-}
-
-template <int DIR, int NS, bool FINAL>
+// the segment's system with its neighbours known, written to fx, fxx (UCCD:
+// fx the up-wind value where u > 0, else the down-wind one, fxx their mean)
+template <int DIR, int NS>
 __global__ void __launch_bounds__(SEGT) k_seg_lines(const double *__restrict__ f, const double *__restrict__ u,
                                                     double *__restrict__ fx, double *__restrict__ fxx,
-                                                    double *__restrict__ zb, const double *__restrict__ tab_u,
+                                                    const double *__restrict__ zb, const double *__restrict__ tab_u,
                                                     const double *__restrict__ tab_d, Geom G, int ext, int P,
                                                     UccdSrc cs) {
     long nl, st;
     int N;
     double dx;
-    if constexpr (DIR == 1) { nl = (long)G.nx * G.nz; N = G.ny + 2 * ext; st = G.sx; dx = G.dy; }
-    else { nl = (long)G.nx * G.ny; N = G.nz + 2 * ext; st = G.sxy; dx = G.dz; }
+    seg_line_geom<DIR>(G, ext, nl, N, st, dx);
     const long t = (long)blockIdx.x * SEGT + threadIdx.x;
     if (t >= nl) return;
     const int s = blockIdx.y;
@@ -758,7 +831,7 @@ __global__ void __launch_bounds__(SEGT) k_seg_lines(const double *__restrict__ f
     for (int k = 0; k < SEG + 2; ++k) fv[k] = f[line + (long)min(max(a - 1 + k, 0), N - 1) * st];
     // upwind choice per point, from u loaded up front
     unsigned up = 0;
-    if (FINAL && NS == 2) {
+    if (NS == 2) {
         double uv[SEG];   // unconditional loads (clamped to the line), issued together
 #pragma unroll
         for (int q = 0; q < SEG; ++q) uv[q] = u[line + (long)min(a + q, N - 1) * st];
@@ -766,19 +839,10 @@ __global__ void __launch_bounds__(SEGT) k_seg_lines(const double *__restrict__ f
         for (int q = 0; q < SEG; ++q)
             up |= (q < n && (uv[q] > 0.0 || (cs.ge && uv[q] == 0.0))) ? (1u << q) : 0u;
     }
-    double p1[NS], p2[NS], x1[NS], x2[NS], e1[NS], e2[NS];
-    if (FINAL) {
-        seg_neighbours<NS>(zb, t, nl, s, P, p1, p2, x1, x2);
-    } else {
-#pragma unroll
-        for (int y = 0; y < NS; ++y) p1[y] = p2[y] = x1[y] = x2[y] = 0.0;
-    }
+    double p1[NS], p2[NS], x1[NS], x2[NS];
+    seg_neighbours<NS>(zb, t, nl, s, P, p1, p2, x1, x2);
     double S[NS][SEG], SS[NS][SEG];
-    seg_sweep<NS>(fv, a, n, N, idx, idx2, tab_u, tab_d, cs, p1, p2, x1, x2, S, SS, e1, e2);
-    if (!FINAL) {
-        seg_store_ends<NS>(zb, t, nl, s, P, S, SS, e1, e2);
-        return;
-    }
+    seg_sweep<NS>(fv, a, n, N, idx, idx2, tab_u, tab_d, cs, p1, p2, x1, x2, S, SS);
 #pragma unroll
     for (int q = 0; q < SEG; ++q)
         if (q < n) {
@@ -798,25 +862,17 @@ __global__ void __launch_bounds__(SEGT) k_seg_lines(const double *__restrict__ f
 // contiguous, so the wave moves them through an LDS tile of 64 rows x SEGX_TS
 // (odd, against bank conflicts on the per-row reads): the loads and stores
 // number the tile's elements row by row across the wave's lanes, a few rows
-// per instruction, each row a contiguous run; each lane then sweeps its own
-// row from registers.
+// per instruction, each row a contiguous run; each lane then works on its
+// own row from registers.
 constexpr int SEGX_TS = SEG + 3;
 
-template <int NS, bool FINAL>
-__global__ void __launch_bounds__(WAVE) k_seg_lines_x(const double *__restrict__ f, const double *__restrict__ u,
-                                                      double *__restrict__ fx, double *__restrict__ fxx,
-                                                      double *__restrict__ zb, const double *__restrict__ tab_u,
-                                                      const double *__restrict__ tab_d, Geom G, int ext, int P,
-                                                      UccdSrc cs) {
-    __shared__ double tile[WAVE * SEGX_TS];
-    __shared__ long rowb[WAVE];
-    const long nl = (long)G.ny * G.nz;
-    const int N = G.nx + 2 * ext;
+// the block's rows (x lines t0 ..) and f(a - 1 .. a + n) of each, clipped to
+// the line, into the tile; fv[k] = f(a - 1 + k) of this lane's row
+__device__ __forceinline__ void segx_stage(const double *__restrict__ f, double *tile, long *rowb, const Geom &G,
+                                           int ext, long nl, long t0, int rows, int a, int n, int N,
+                                           double (&fv)[SEG + 2]) {
     const int lane = threadIdx.x;
-    const long t0 = (long)blockIdx.x * WAVE, t = t0 + lane;
-    const int rows = (int)min((long)WAVE, nl - t0);
-    const int s = blockIdx.y;
-    const int a = seg_start(s, N, P), n = seg_start(s + 1, N, P) - a;
+    const long t = t0 + lane;
     if (t < nl) {
         const unsigned k = (unsigned)t / (unsigned)G.ny;
         rowb[lane] = G.id(-ext, (int)((unsigned)t - k * (unsigned)G.ny), (int)k);
@@ -824,8 +880,7 @@ __global__ void __launch_bounds__(WAVE) k_seg_lines_x(const double *__restrict__
         rowb[lane] = 0;
     }
     __syncthreads();
-    // f(a - 1 .. a + n) of every row, clipped to the line: tile columns k0 .. k1
-    const int k0 = a > 0 ? 0 : 1, k1 = a + n < N ? n + 1 : n;
+    const int k0 = a > 0 ? 0 : 1, k1 = a + n < N ? n + 1 : n;   // tile columns k0 .. k1
     const int W = k1 - k0 + 1, tot = rows * W;
     {
         // at most SEG + 2 elements per lane, all loads issued before the
@@ -843,12 +898,47 @@ __global__ void __launch_bounds__(WAVE) k_seg_lines_x(const double *__restrict__
         }
     }
     __syncthreads();
-    double fv[SEG + 2];
 #pragma unroll
     for (int k = 0; k < SEG + 2; ++k) fv[k] = (k >= k0 && k <= k1) ? tile[lane * SEGX_TS + k] : 0.0;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(WAVE) k_seg_ends_x(const double *__restrict__ f, double *__restrict__ zb,
+                                                     const double *__restrict__ end_u,
+                                                     const double *__restrict__ end_d, Geom G, int ext, int P) {
+    __shared__ double tile[WAVE * SEGX_TS];
+    __shared__ long rowb[WAVE];
+    const long nl = (long)G.ny * G.nz;
+    const int N = G.nx + 2 * ext;
+    const long t0 = (long)blockIdx.x * WAVE, t = t0 + threadIdx.x;
+    const int rows = (int)min((long)WAVE, nl - t0);
+    const int s = blockIdx.y;
+    const int a = seg_start(s, N, P), n = seg_start(s + 1, N, P) - a;
+    double fv[SEG + 2];
+    segx_stage(f, tile, rowb, G, ext, nl, t0, rows, a, n, N, fv);
+    if (t < nl) seg_ends_store<NS>(fv, end_u, end_d, zb, t, nl, s, P);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(WAVE) k_seg_lines_x(const double *__restrict__ f, const double *__restrict__ u,
+                                                      double *__restrict__ fx, double *__restrict__ fxx,
+                                                      const double *__restrict__ zb, const double *__restrict__ tab_u,
+                                                      const double *__restrict__ tab_d, Geom G, int ext, int P,
+                                                      UccdSrc cs) {
+    __shared__ double tile[WAVE * SEGX_TS];
+    __shared__ long rowb[WAVE];
+    const long nl = (long)G.ny * G.nz;
+    const int N = G.nx + 2 * ext;
+    const int lane = threadIdx.x;
+    const long t0 = (long)blockIdx.x * WAVE, t = t0 + lane;
+    const int rows = (int)min((long)WAVE, nl - t0);
+    const int s = blockIdx.y;
+    const int a = seg_start(s, N, P), n = seg_start(s + 1, N, P) - a;
+    double fv[SEG + 2];
+    segx_stage(f, tile, rowb, G, ext, nl, t0, rows, a, n, N, fv);
     // upwind choice per point: u through the same tile
     unsigned up = 0;
-    if (FINAL && NS == 2) {
+    if (NS == 2) {
         __syncthreads();
         const int totu = rows * n;
         double v[SEG];
@@ -871,19 +961,12 @@ __global__ void __launch_bounds__(WAVE) k_seg_lines_x(const double *__restrict__
             }
     }
     const double idx = 1.0 / G.dx, idx2 = idx * idx;
-    double p1[NS], p2[NS], x1[NS], x2[NS], e1[NS], e2[NS];
+    double p1[NS], p2[NS], x1[NS], x2[NS];
     double S[NS][SEG], SS[NS][SEG];
     if (t < nl) {
-        if (FINAL) {
-            seg_neighbours<NS>(zb, t, nl, s, P, p1, p2, x1, x2);
-        } else {
-#pragma unroll
-            for (int y = 0; y < NS; ++y) p1[y] = p2[y] = x1[y] = x2[y] = 0.0;
-        }
-        seg_sweep<NS>(fv, a, n, N, idx, idx2, tab_u, tab_d, cs, p1, p2, x1, x2, S, SS, e1, e2);
-        if (!FINAL) seg_store_ends<NS>(zb, t, nl, s, P, S, SS, e1, e2);
+        seg_neighbours<NS>(zb, t, nl, s, P, p1, p2, x1, x2);
+        seg_sweep<NS>(fv, a, n, N, idx, idx2, tab_u, tab_d, cs, p1, p2, x1, x2, S, SS);
     }
-    if (!FINAL) return;
     // fx, then fxx, out through the tile
 #pragma unroll
     for (int o2 = 0; o2 < 2; ++o2) {
@@ -1013,6 +1096,8 @@ __global__ void __launch_bounds__(SEGT) k_seg_reduce_r(double *__restrict__ zb, 
 
 // the partitioned solve of the y (dir 1) or z (dir 2) lines: NS = 1 CCD
 // with table k0, NS = 2 UCCD with tables k0 (up) and k1 (down)
+// the partitioned solve of the lines along DIR: NS = 1 CCD with table k0,
+// NS = 2 UCCD with tables k0 (up) and k1 (down)
 template <int DIR, int NS>
 static int seg_solve(fcfd_ctx *c, int ext, int k0, int k1, const double *f, const double *u, double *fx, double *fxx,
                      UccdSrc cs) {
@@ -1021,30 +1106,34 @@ static int seg_solve(fcfd_ctx *c, int ext, int k0, int k1, const double *f, cons
     const int N = (DIR == 0 ? G.nx : (DIR == 1 ? G.ny : G.nz)) + 2 * ext;
     const int P = seg_count(N);
     const double *tu = c->ccd_seg[k0 * 3 + DIR], *td = c->ccd_seg[k1 * 3 + DIR];
+    const long red_off = (long)N * NPACK, end_off = red_off + (long)(P - 1) * NRED;
     const unsigned gx = (unsigned)((nl + SEGT - 1) / SEGT), gw = (unsigned)((nl + WAVE - 1) / WAVE);
-    auto lines = [&](auto fin) {
-        constexpr bool FIN = decltype(fin)::value;
-        if constexpr (DIR == 0)
-            hipLaunchKernelGGL((k_seg_lines_x<NS, FIN>), dim3(gw, (unsigned)P), dim3(WAVE), 0, c->stream, f, u, fx,
-                               fxx, c->ccd_z, tu, td, G, ext, P, cs);
-        else
-            hipLaunchKernelGGL((k_seg_lines<DIR, NS, FIN>), dim3(gx, (unsigned)P), dim3(SEGT), 0, c->stream, f, u,
-                               fx, fxx, c->ccd_z, tu, td, G, ext, P, cs);
-    };
     if (P > 1) {
-        lines(std::false_type{});
+        if constexpr (DIR == 0)
+            hipLaunchKernelGGL(k_seg_ends_x<NS>, dim3(gw, (unsigned)P), dim3(WAVE), 0, c->stream, f, c->ccd_z,
+                               tu + end_off, td + end_off, G, ext, P);
+        else
+            hipLaunchKernelGGL((k_seg_ends<DIR, NS>), dim3(gx, (unsigned)P), dim3(SEGT), 0, c->stream, f, c->ccd_z,
+                               tu + end_off, td + end_off, G, ext, P);
         if (P - 1 <= SEG_MAXI)
             hipLaunchKernelGGL(k_seg_reduce_r<SEG_MAXI>, dim3(gx, NS), dim3(SEGT), 0, c->stream, c->ccd_z,
-                               tu + (long)N * NPACK, td + (long)N * NPACK, nl, P);
+                               tu + red_off, td + red_off, nl, P);
         else
-            hipLaunchKernelGGL(k_seg_reduce, dim3(gx, NS), dim3(SEGT), 0, c->stream, c->ccd_z, tu + (long)N * NPACK,
-                               td + (long)N * NPACK, nl, P);
+            hipLaunchKernelGGL(k_seg_reduce, dim3(gx, NS), dim3(SEGT), 0, c->stream, c->ccd_z, tu + red_off,
+                               td + red_off, nl, P);
     }
-    lines(std::true_type{});
+    if constexpr (DIR == 0)
+        hipLaunchKernelGGL(k_seg_lines_x<NS>, dim3(gw, (unsigned)P), dim3(WAVE), 0, c->stream, f, u, fx, fxx,
+                           c->ccd_z, tu, td, G, ext, P, cs);
+    else
+        hipLaunchKernelGGL((k_seg_lines<DIR, NS>), dim3(gx, (unsigned)P), dim3(SEGT), 0, c->stream, f, u, fx, fxx,
+                           c->ccd_z, tu, td, G, ext, P, cs);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
 
+// CCD of f along direction dir: set 0 the newOpenMP lines through the ghosts
+// (n + 6 points), set 1 the MPLS_RisingBubble interior lines (n points)
 int ccd_lines_set(fcfd_ctx *c, int set, int dir, const double *f, double *fx, double *fxx) {
     const Geom &G = c->G;
     const int ext = set ? 0 : G.g;

@@ -495,25 +495,39 @@ __global__ void __launch_bounds__(FUSED_THREADS) k_smooth_fused(const float *__r
     const int t = threadIdx.x;
     const int i0 = bx_ * TX - H, j0 = by_ * TY - H, k0 = DIM == 3 ? bz_ * TZ - H : bz_;   // 2D: the slice
     const float hx = (float)ih.x, hy = (float)ih.y, hz = (float)ih.z;
-    for (int q = t; q < NE; q += FUSED_THREADS) {
+    // staging: every load of the block's NIT rounds is issued before the
+    // first is waited for (a loop with a branch around its loads costs one
+    // memory latency per round); positions outside the ghosted array load a
+    // clamped in-array address and stage 0
+    constexpr int NIT = (NE + FUSED_THREADS - 1) / FUSED_THREADS;
+    float ev[NIT], fv[NIT], b1[NIT], b2[NIT], b3[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int q = min(t + it * FUSED_THREADS, NE - 1);
         const int qx = q % EX, qy = (q / EX) % EY, qz = q / SZ;
         const int i = i0 + qx, j = j0 + qy, k = k0 + qz;
         const bool inarr = i >= -1 && i <= L.nx && j >= -1 && j <= L.ny && (DIM == 2 || (k >= -1 && k <= L.nz));
-        float ev = 0.f, fv = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;
-        if (inarr) {
-            const long id = L.id(i, j, k);
-            ev = e[id];
-            if (prol) ev += P.at(i, j, k);
-            fv = f[id];
-            b1 = bx[id];
-            b2 = by[id];
-            if (DIM == 3) b3 = bz[id];
+        const int ic = clampi(i, -1, L.nx), jc = clampi(j, -1, L.ny), kc = DIM == 3 ? clampi(k, -1, L.nz) : k;
+        const long id = L.id(ic, jc, kc);
+        float e_ = e[id];
+        if (prol) e_ += P.at(ic, jc, kc);
+        const float f_ = f[id], x_ = bx[id], y_ = by[id], z_ = DIM == 3 ? bz[id] : 0.f;
+        ev[it] = inarr ? e_ : 0.f;
+        fv[it] = inarr ? f_ : 0.f;
+        b1[it] = inarr ? x_ : 0.f;
+        b2[it] = inarr ? y_ : 0.f;
+        b3[it] = inarr ? z_ : 0.f;
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int q = t + it * FUSED_THREADS;
+        if (q < NE) {
+            se[q] = ev[it];
+            sf[q] = fv[it];
+            sbx[q] = b1[it];
+            sby[q] = b2[it];
+            if (DIM == 3) sbz[q] = b3[it];
         }
-        se[q] = ev;
-        sf[q] = fv;
-        sbx[q] = b1;
-        sby[q] = b2;
-        if (DIM == 3) sbz[q] = b3;
     }
     __syncthreads();
     // half-sweep h: cells of colour (color0 + h) & 1 within distance
@@ -1597,15 +1611,16 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
             // per launch (one 123 KB block per CU) while the tiles fit in one
             // round on the device, else one sweep per launch (two blocks per CU)
             const long n0c = (long)c->mg[0].nx * c->mg[0].ny * c->mg[0].nz;
-            const int fused = c->pr.ppe_fused;   // 0 auto, 1 / 2 sweeps per launch, 3 off
-            if (fused != 3 && lv >= 1 && M.e32b && sweeps >= 2 && 32L * M.nx * M.ny * M.nz <= n0c && !dt &&
-                !zinit) {
+            // 0 auto, 1 / 2 sweeps per launch, 3 off, 4 auto on every level >= 1
+            const int fused = c->pr.ppe_fused;
+            const bool small = fused == 4 || 32L * M.nx * M.ny * M.nz <= n0c;
+            if (fused != 3 && lv >= 1 && M.e32b && sweeps >= 2 && small && !dt && !zinit) {
                 using FT = FusedTile<DIM>;
                 const dim3 gf((M.nx + FT::TX - 1) / FT::TX, (M.ny + FT::TY - 1) / FT::TY,
                               DIM == 3 ? (M.nz + FT::TZ - 1) / FT::TZ : M.nz);
                 const int c0 = reverse ? 1 : 0;
                 int n2, n1;
-                if (fused == 2 || (fused == 0 && (long)gf.x * gf.y * gf.z <= c->ncu)) {
+                if (fused == 2 || ((fused == 0 || fused == 4) && (long)gf.x * gf.y * gf.z <= c->ncu)) {
                     n2 = sweeps / 2;
                     n1 = sweeps % 2;
                     if ((n2 + n1) % 2) { n2 -= 1; n1 += 2; }

@@ -144,7 +144,7 @@ class Case:
     ppe_nu1: int = 2            # RB-GS sweeps before / after the coarse correction
     ppe_nu2: int = 2
     ppe_nu_coarse: int = 6      # sweeps on levels of <= 1/32 of the finest cells
-    ppe_fused: int = 0          # coarse FP32 sweeps: 0 auto, 1 / 2 per launch, 3 colour launches
+    ppe_fused: int = 0          # coarse FP32 sweeps: 0 auto, 1 / 2 per launch, 3 colour launches, 4 auto from level 1
     ppe_rho_half: int = 0       # 1: FP32 level-0 smoother reads the density in FP16 (3D: level-1 coefficients too)
     imass: float = 0.0
     name: str = "case"

@@ -72,7 +72,8 @@ typedef struct fcfd_params {
     int32_t ppe_fused;           /* FP32 coarse-level sweeps: 0 auto (two per
                                     launch while the tiles fit the device in
                                     one round, else one), 1 one per launch,
-                                    2 two per launch, 3 one colour per launch */
+                                    2 two per launch, 3 one colour per launch,
+                                    4 as 0 on every level below the finest    */
     int32_t ppe_rho_half;        /* 1: the FP32 level-0 smoother and restriction
                                     read the density in FP16, and in 3D those of
                                     level 1 its face coefficients (mixed mode) */

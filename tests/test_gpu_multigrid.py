@@ -48,15 +48,17 @@ def test_fused_coarse_smoothing_matches_fp64_cycle():
     assert np.abs(a - b).max() <= 1e-8 * np.abs(a).max()
 
 
-def test_fused_coarse_smoothing_plain_cycles():
+@pytest.mark.parametrize("fused", [0, 4])
+def test_fused_coarse_smoothing_plain_cycles(fused):
     """Defect correction (no Krylov): every cycle's output is used as is, so a
-    wrong coarse-level sweep shows as a stalled or slower iteration."""
+    wrong coarse-level sweep shows as a stalled or slower iteration.  fused =
+    4 also runs the second level (64x64x256) through the fused tiles."""
     from fortrancfd_amd.types import PPE_FP64, PPE_MIXED
     case, s = _system()
     zeros = np.zeros(case.size)
     cyc = {}
     for name, prec in (("fp64", PPE_FP64), ("mixed", PPE_MIXED)):
-        s.set_params(ppe_krylov=0, ppe_inner=1)
+        s.set_params(ppe_krylov=0, ppe_inner=1, ppe_fused=fused)
         s.push("p", zeros)
         c, rel = s.ppe_solve_rhs(1e-10, prec)
         print(name, c, rel)
