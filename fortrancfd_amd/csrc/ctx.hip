@@ -380,6 +380,8 @@ extern "C" int fcfd_destroy(fcfd_ctx *c) {
         if (p) hipFree(p);
     for (auto p : c->adj32)
         if (p) hipFree(p);
+    for (auto e : c->cg_ev)
+        if (e) hipEventDestroy(e);
     if (c->red.host) hipHostFree(c->red.host);
     if (c->diag_host) hipHostFree(c->diag_host);
     if (c->cases_dev) hipFree(c->cases_dev);

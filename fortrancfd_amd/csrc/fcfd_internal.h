@@ -138,7 +138,9 @@ struct fcfd_ctx {
     float *kzo32;
     // captured V-cycles: [kind][inner cycles 1..4]; kind 0 fp64 on p, 1 fp64 on
     // z with the CG dots, 2 fp32 on e32, 3 fp32 on e32 with the CG dots
-    hipGraphExec_t vgraph[4][5];
+    hipGraphExec_t vgraph[2][4][5];                      // [host status slot][kind][inner]
+    int *mg_skip;              // device flag: every case's flexible-CG solve has ended
+    hipEvent_t cg_ev[2];       // end of the flexible-CG iteration of each host status slot
     std::vector<std::pair<int, int>> vgraph_prof[4][5];  // (region, event) pairs inside each graph
     int capture_kind = -1, capture_inner = 0;            // graph being captured (or -1)
     double prof_gms[fcfd::NREG] = {};                    // timings folded in from graph replays

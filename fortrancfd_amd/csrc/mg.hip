@@ -37,24 +37,35 @@ using namespace fcfd;
 
 namespace fcfd {
 
-// layout of one level: interior (nx,ny,nz), ghost width g (gz = 0 in 2D)
+// layout of one level: interior (nx,ny,nz), ghost width g (gz = 0 in 2D).
+// skip: while a flexible-CG iteration is queued ahead of the host's reading
+// of the previous one's status, its kernels read this device flag first and
+// return at once when the solve has already ended (see ppe_solve_t)
 struct Lay {
     int nx, ny, nz, g, gz;
     long sx, sxy;
+    const int *skip = nullptr;
     __host__ __device__ long id(int i, int j, int k) const {
         return (long)(i + g) + (long)(j + g) * sx + (long)(k + gz) * sxy;
     }
+    __device__ bool skipped() const { return skip && *skip != 0; }
 };
+
+// the skip flag given to the layouts made on this host thread (set while a
+// pipelined flexible-CG iteration is launched or captured)
+static thread_local const int *t_skip = nullptr;
 
 static Lay lay_of(const Geom &G) {
     Lay L;
     L.nx = G.nx; L.ny = G.ny; L.nz = G.nz; L.g = G.g; L.gz = G.gz; L.sx = G.sx; L.sxy = G.sxy;
+    L.skip = t_skip;
     return L;
 }
 
 static Lay lay_of(const MgLevel &M, int dim) {
     Lay L;
     L.nx = M.nx; L.ny = M.ny; L.nz = M.nz; L.g = 1; L.gz = dim == 3 ? 1 : 0; L.sx = M.sx; L.sxy = M.sxy;
+    L.skip = t_skip;
     return L;
 }
 
@@ -248,6 +259,7 @@ __global__ void __launch_bounds__(256) k_smooth(T *__restrict__ e, const T *__re
                                                 const CT *__restrict__ bx, const CT *__restrict__ by,
                                                 const CT *__restrict__ bz, Lay L, Inv2 ih, int color, Prol<T> P,
                                                 Dots<T> D) {
+    if (L.skipped()) return;
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int j = by_ * blockDim.y + threadIdx.y, k = bz_;
@@ -337,6 +349,7 @@ template <int DIM, int MODE, bool DOTS, typename RT>
 __global__ void __launch_bounds__(256) k_smooth_split(float *__restrict__ e, const float *__restrict__ f,
                                                       const RT *__restrict__ rho, Split S, Inv2 ih, int color,
                                                       Prol<float> P, Dots<float> D, Lay L) {
+    if (L.skipped()) return;
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int m = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
@@ -424,6 +437,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict_split(const float *__res
                                                               const RT *__restrict__ rho, Split S, Inv2 ih,
                                                               float *__restrict__ fc, float *__restrict__ ec, Lay C,
                                                               int cx, int cy, int cz) {
+    if (C.skipped()) return;
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int I = bx_ * blockDim.x + threadIdx.x, J = by_ * blockDim.y + threadIdx.y, K = bz_;
@@ -486,6 +500,7 @@ __global__ void __launch_bounds__(FUSED_THREADS) k_smooth_fused(const float *__r
                                                                 const float *__restrict__ by,
                                                                 const float *__restrict__ bz, Lay L, Inv2 ih,
                                                                 int color0, Prol<float> P, int prol) {
+    if (L.skipped()) return;
     constexpr int H = 2 * NS, TX = FusedTile<DIM>::TX, TY = FusedTile<DIM>::TY, TZ = FusedTile<DIM>::TZ;
     constexpr int EX = TX + 2 * H, EY = TY + 2 * H, EZ = DIM == 3 ? TZ + 2 * H : 1, NE = EX * EY * EZ;
     constexpr int SY = EX, SZ = EX * EY;
@@ -580,6 +595,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const T *__restrict__ e,
                                                         const CT *__restrict__ by, const CT *__restrict__ bz, Lay L,
                                                         Inv2 ih, T *__restrict__ fc, T *__restrict__ ec, Lay C,
                                                         int cx, int cy, int cz) {
+    if (L.skipped()) return;
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int I = bx_ * blockDim.x + threadIdx.x, J = by_ * blockDim.y + threadIdx.y, K = bz_;
@@ -613,6 +629,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const T *__restrict__ e,
 template <typename T>
 __global__ void __launch_bounds__(256) k_prolong(T *__restrict__ e, const T *__restrict__ ec, Lay L, Lay C, int cx,
                                                  int cy, int cz) {
+    if (L.skipped()) return;
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
@@ -703,7 +720,8 @@ __global__ void k_to_f16(const double *__restrict__ a, _Float16 *__restrict__ b,
 // contiguous chunk b of slot s into partial2[(q ns + s) 256 + b]; stage 2
 // (one block per case) sums the 256 chunk sums per slot.
 __global__ void __launch_bounds__(256) k_sum_stage1(const double *__restrict__ partial, long nb, long chunk,
-                                                    double *__restrict__ partial2) {
+                                                    double *__restrict__ partial2, const int *__restrict__ skip) {
+    if (skip && *skip) return;
     __shared__ double sh[4];
     const int s = blockIdx.y, q = blockIdx.z;
     const long ntot = nb * gridDim.z;
@@ -722,9 +740,14 @@ __global__ void __launch_bounds__(256) k_sum_stage1(const double *__restrict__ p
 // ended while others of the stack go on: its alpha and beta are 0 from then
 // on, so the x += alpha p still pending from its last iteration is the only
 // update it receives, as at the end of a single solve.
+// The stopping state of each case's solve follows (k_cg_start, k_cg_check):
+// ||f||, the relative residual, its best value, iterations, iterations
+// since the best, a non-finite flag (the host reads SC_DONE .. SC_NAN), and
+// the solve's tolerance and iteration limit.
 enum {
     SC_RZ = 0, SC_RZO = 1, SC_PQ = 2, SC_RR = 3, SC_MEAN = 4, SC_RZ_OLD = 5, SC_BETA = 6, SC_ALPHA = 7,
-    SC_FIRST = 8, SC_BREAK = 9, SC_DONE = 10, SC_NHOST = 11
+    SC_FIRST = 8, SC_BREAK = 9, SC_DONE = 10, SC_FN = 11, SC_R = 12, SC_BEST = 13, SC_K = 14, SC_SINCE = 15,
+    SC_NAN = 16, SC_NST = SC_NAN - SC_DONE + 1, SC_TOL = 17, SC_MAXC = 18
 };
 constexpr int SCS = 64;
 enum { SUM_STORE = 0, SUM_CG_BETA = 1, SUM_CG_ALPHA = 2 };
@@ -736,7 +759,9 @@ enum { SUM_STORE = 0, SUM_CG_BETA = 1, SUM_CG_ALPHA = 2 };
 // r.z < 0 and p.Ap < 0 until round-off; otherwise alpha = 0 and the
 // breakdown flag stops the iteration.
 __global__ void __launch_bounds__(256) k_sum_stage2(const double *__restrict__ partial2, int ns,
-                                                    double *__restrict__ out, int op, double *__restrict__ sc) {
+                                                    double *__restrict__ out, int op, double *__restrict__ sc,
+                                                    const int *__restrict__ skip) {
+    if (skip && *skip) return;
     __shared__ double sh[4];
     const int q = blockIdx.x;   // case
     partial2 += (long)q * ns * 256;
@@ -773,6 +798,76 @@ __global__ void k_cg_init(double *sc) {
 
 // freeze case q of a batched solve (see SC_DONE)
 __global__ void k_cg_done(double *sc, int q) { sc[(long)q * SCS + SC_DONE] = 1.0; }
+
+// *skip = every case's solve has ended (one block of CG_THREADS threads)
+constexpr int CG_THREADS = 256;
+__device__ __forceinline__ void cg_all_done(const double *sc, int nq, int *skip) {
+    int done = 1;
+    for (int q = threadIdx.x; q < nq; q += CG_THREADS) done &= sc[(long)q * SCS + SC_DONE] != 0.0;
+    const int all = __syncthreads_and(done);
+    if (threadIdx.x == 0) *skip = all ? 1 : 0;
+}
+
+// Stopping state of the flexible-CG solve of every case from the initial
+// sums ||r0||^2 (slot SC_RZ) and ||f||^2 (slot SC_RZO): the relative
+// residual r = ||r0|| / ||f|| (||f|| = 1 for f = 0); a case ends at once
+// when r <= tol.  Thread t: cases t, t + CG_THREADS, ...; the skip flag is
+// set when all have ended.
+__global__ void __launch_bounds__(CG_THREADS) k_cg_start(double *sc, int nq, double tol, int maxc, int *skip) {
+    for (int q = threadIdx.x; q < nq; q += CG_THREADS) {
+        double *s = sc + (long)q * SCS;
+        const double fn = s[SC_RZO] > 0.0 ? sqrt(s[SC_RZO]) : 1.0, r = sqrt(s[SC_RZ]) / fn;
+        s[SC_TOL] = tol;
+        s[SC_MAXC] = (double)maxc;
+        s[SC_FN] = fn;
+        s[SC_R] = r;
+        s[SC_BEST] = r;
+        s[SC_K] = 0.0;
+        s[SC_SINCE] = 0.0;
+        s[SC_NAN] = isfinite(r) ? 0.0 : 1.0;
+        if (!(r > tol && 0 < maxc) || !isfinite(r)) s[SC_DONE] = 1.0;
+    }
+    cg_all_done(sc, nq, skip);
+}
+
+// After an iteration: a case that has not ended counts it, forms its
+// relative residual sqrt(||r||^2) / ||f||, and ends on a breakdown, on
+// r <= tol, after maxc iterations (both as k_cg_start stored them), or when r has not fallen below 0.9 of
+// its best for 4 iterations (the FP64 round-off floor); an ended case is
+// frozen (SC_DONE) for the iterations still queued.  A non-finite r ends
+// the case and is flagged.
+__global__ void __launch_bounds__(CG_THREADS) k_cg_check(double *sc, int nq, int *skip) {
+    if (*skip) return;
+    for (int q = threadIdx.x; q < nq; q += CG_THREADS) {
+        double *s = sc + (long)q * SCS;
+        const double tol = s[SC_TOL], maxc = s[SC_MAXC];
+        if (s[SC_DONE] == 0.0) {
+            if (s[SC_BREAK] != 0.0) {
+                s[SC_DONE] = 1.0;
+            } else {
+                const double k = s[SC_K] + 1.0, r = sqrt(s[SC_RR]) / s[SC_FN];
+                s[SC_K] = k;
+                s[SC_R] = r;
+                bool done = false;
+                if (!isfinite(r)) {
+                    s[SC_NAN] = 1.0;
+                    done = true;
+                } else {
+                    if (r < 0.9 * s[SC_BEST]) {
+                        s[SC_BEST] = r;
+                        s[SC_SINCE] = 0.0;
+                    } else {
+                        s[SC_SINCE] += 1.0;
+                        if (s[SC_SINCE] >= 4.0) done = true;
+                    }
+                    if (!(r > tol && k < maxc)) done = true;
+                }
+                if (done) s[SC_DONE] = 1.0;
+            }
+        }
+    }
+    cg_all_done(sc, nq, skip);
+}
 
 // sum of an interior field into block partials
 __global__ void __launch_bounds__(256) k_sum(const double *__restrict__ a, Lay L, double *__restrict__ partial) {
@@ -830,6 +925,7 @@ __global__ void __launch_bounds__(1024) k_coarse_cg(T *__restrict__ eout, const 
                                                     const double *__restrict__ by, const double *__restrict__ bz,
                                                     double *__restrict__ ws, Lay L, Inv2 ih, int maxit, double rtol) {
     (void)bxT; (void)byT; (void)bzT;
+    if (L.skipped()) return;
     __shared__ double red[1024];
     const int nt = blockDim.x, t = threadIdx.x;
     const long n = (long)L.nx * L.ny * L.nz;
@@ -1076,6 +1172,7 @@ __global__ void __launch_bounds__(CINV_THREADS) k_coarse_inv(const double *__res
 template <typename T>
 __global__ void __launch_bounds__(CDENSE) k_coarse_dense(T *__restrict__ eout, const T *__restrict__ fin,
                                                          const double *__restrict__ minv, Lay L, int dim) {
+    if (L.skipped()) return;
     constexpr int NW = CDENSE / WAVE;
     __shared__ double b[CDENSE];
     __shared__ double ws[NW];
@@ -1147,6 +1244,7 @@ struct TailArgs {
 
 template <typename T, int DIM>
 __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
+    if (A.lv[0].L.skipped()) return;
     __shared__ double red[TAIL_THREADS / WAVE];
     __shared__ double bvec[CDENSE];
     const int t = threadIdx.x;
@@ -1403,6 +1501,8 @@ int mg_free(fcfd_ctx *c) {
     if (c->rho32) hipFree(c->rho32);
     if (c->rho16) hipFree(c->rho16);
     if (c->mg_scalars) hipFree(c->mg_scalars);
+    if (c->mg_skip) hipFree(c->mg_skip);
+    c->mg_skip = nullptr;
     void *kv[] = {c->kr, c->kp, c->kq, c->kz, c->kzo, c->kzo32};
     for (void *p : kv)
         if (p) hipFree(p);
@@ -1418,8 +1518,10 @@ int mg_free(fcfd_ctx *c) {
 int mg_drop_graphs(fcfd_ctx *c) {
     for (int a = 0; a < 4; ++a)
         for (int b = 0; b < 5; ++b) {
-            if (c->vgraph[a][b]) hipGraphExecDestroy(c->vgraph[a][b]);
-            c->vgraph[a][b] = nullptr;
+            for (int s = 0; s < 2; ++s) {
+                if (c->vgraph[s][a][b]) hipGraphExecDestroy(c->vgraph[s][a][b]);
+                c->vgraph[s][a][b] = nullptr;
+            }
             c->vgraph_prof[a][b].clear();
         }
     return FCFD_OK;
@@ -1508,6 +1610,9 @@ static int mg_build(fcfd_ctx *c) {
         return FCFD_ERR_ARG;
     }
     FCFD_TRY(alloc0(&c->mg_scalars, SCS * c->nb + c->nb * std::max(6 * ncoarse, ncoarse * ncoarse)));
+    FCFD_TRY(alloc0(&c->mg_skip, 1));
+    for (auto &e : c->cg_ev)
+        if (!e) FCFD_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return FCFD_OK;
 }
 
@@ -1913,6 +2018,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 5))
                                                  double *__restrict__ x, const double *__restrict__ rho, Lay L,
                                                  Split S, Inv2 ih, const double *__restrict__ sc,
                                                  double *__restrict__ partial) {
+    if (L.skipped()) return;
     __shared__ double sh[4];
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
@@ -1969,6 +2075,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 5))
 __global__ void __launch_bounds__(256) k_fcg_update(const double *__restrict__ q, double *__restrict__ r, Lay L,
                                                     const double *__restrict__ sc, float *__restrict__ r32, Split S,
                                                     double *__restrict__ partial, int dim) {
+    if (L.skipped()) return;
     __shared__ double sh[4];
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
@@ -2003,9 +2110,9 @@ __global__ void __launch_bounds__(256) k_fcg_finish(double *__restrict__ x, cons
 static int sum_partials(fcfd_ctx *c, long nb, int ns, double *out, int op = SUM_STORE) {
     const long nbq = nb / c->nb, chunk = (nbq + 255) / 256;
     hipLaunchKernelGGL(k_sum_stage1, dim3(256, ns, c->nb), dim3(256), 0, c->stream, c->red.partial, nbq, chunk,
-                       c->red.partial2);
+                       c->red.partial2, t_skip);
     hipLaunchKernelGGL(k_sum_stage2, dim3(c->nb), dim3(256), 0, c->stream, c->red.partial2, ns, out, op,
-                       c->mg_scalars);
+                       c->mg_scalars, t_skip);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
@@ -2035,10 +2142,10 @@ static int field_mean(fcfd_ctx *c, const double *a) {
 // every scalar the body needs lives in device memory.  With the profiler on,
 // the body is launched directly (events inside a graph cannot be timed).
 template <typename Body>
-static int run_graph(fcfd_ctx *c, int kind, int inner, Body body) {
+static int run_graph(fcfd_ctx *c, int kind, int inner, Body body, int slot = 0) {
     static const bool no_graph = getenv("FCFD_NO_GRAPH") != nullptr;
     if (no_graph || c->prof.on) return body();
-    hipGraphExec_t &ex = c->vgraph[kind][inner];
+    hipGraphExec_t &ex = c->vgraph[slot][kind][inner];
     if (!ex) {
         hipGraph_t graph;
         c->capture_kind = kind;
@@ -2072,9 +2179,10 @@ static int vcycles(fcfd_ctx *c, int kind, int inner) {
 // One flexible-CG iteration, all on the device: z = V-cycle(s)(r) (kind 1:
 // FP64 z, kind 3: FP32 z) with r.z and r.zold folded into the last sweep,
 // beta; p = z + beta p, q = A z + beta q, z_old = z, p.q; alpha; x += alpha p,
-// r -= alpha q, the next cycle's rhs; ||r||^2; then the scalars to the host.
+// r -= alpha q, the next cycle's rhs; ||r||^2; the stopping test of every
+// case (k_cg_check); then the stopping state to host buffer `slot`.
 template <int DIM>
-static int fcg_iteration(fcfd_ctx *c, int kind, int inner) {
+static int fcg_iteration(fcfd_ctx *c, int kind, int inner, int slot) {
     const Geom &G = c->G;
     dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
     const long nb = (long)g.x * g.y * g.z;
@@ -2115,17 +2223,20 @@ static int fcg_iteration(fcfd_ctx *c, int kind, int inner) {
         FCFD_CHECK(hipGetLastError());
     }
     FCFD_TRY(sum_partials(c, nb, 1, sc + SC_RR, SUM_STORE));
-    // the first SC_NHOST scalars of every case, packed
-    FCFD_TRY(gather_to_host(c, c->red.host, sc, SCS, SC_NHOST, c->nb));
+    hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(CG_THREADS), 0, c->stream, sc, c->nb, c->mg_skip);
+    FCFD_CHECK(hipGetLastError());
+    // SC_DONE .. SC_NAN of every case, packed
+    FCFD_TRY(gather_to_host(c, c->red.host + (long)slot * SC_NST * c->nb, sc + SC_DONE, SCS, SC_NST, c->nb));
     return FCFD_OK;
 }
 
 // The solve runs the loop of a single system for every case of the context
 // at once (one case unless it is a batched 2D stack).  Each case keeps its
-// own stopping state; once a case's loop would have ended it is frozen on
-// the device (k_cg_done; the mixed defect correction skips its update), so
-// its result is that of its own solve.  FP64 plain V-cycles cannot freeze a
-// case and go on improving finished ones until the last case stops.
+// own stopping state (on the device for flexible CG: k_cg_start,
+// k_cg_check); once a case's loop would have ended it is frozen on the
+// device (SC_DONE; the mixed defect correction skips its update), so its
+// result is that of its own solve.  FP64 plain V-cycles cannot freeze a case
+// and go on improving finished ones until the last case stops.
 template <int DIM>
 static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel) {
     const Geom &G = c->G;
@@ -2182,46 +2293,95 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     };
     if (verbose) fprintf(stderr, "fcfd mg[%s%s]: it 0 rel %.3e\n", mixed ? "mixed" : "fp64", krylov ? "+fcg" : "", rmax());
     hipLaunchKernelGGL(k_cg_init, dim3(nq), dim3(1), 0, c->stream, c->mg_scalars);
-    for (int q = 0; q < nq; ++q)
-        if (cs[q].done) hipLaunchKernelGGL(k_cg_done, dim3(1), dim3(1), 0, c->stream, c->mg_scalars, q);
+    if (krylov)
+        hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(CG_THREADS), 0, c->stream, c->mg_scalars, nq, rel_tol, maxc,
+                           c->mg_skip);
+    else
+        for (int q = 0; q < nq; ++q)
+            if (cs[q].done) hipLaunchKernelGGL(k_cg_done, dim3(1), dim3(1), 0, c->stream, c->mg_scalars, q);
     FCFD_CHECK(hipGetLastError());
     int it = 0;
-    while (active > 0) {
-        if (!krylov) {
-            // defect correction (mixed) or plain V-cycles on p (FP64)
-            FCFD_TRY(run_graph(c, kind, inner, [&]() { return vcycles<DIM>(c, kind, inner); }));
-            if (mixed) {
-                hipLaunchKernelGGL(k_add_f32, g, blk, 0, c->stream, p, M0.e32, L0, S0, c->mg_scalars, DIM);
-                FCFD_CHECK(hipGetLastError());
+    if (krylov && active > 0) {
+        // Flexible CG, one graph per iteration, pipelined: while the host
+        // reads iteration it's stopping state (k_cg_check), iteration it + 1
+        // is already queued, so the device does not wait on the host between
+        // iterations.  An iteration queued after the solve has ended finds
+        // the skip flag set and all its kernels return at once, but its
+        // launches still cost time: the host queues ahead only while the
+        // residual's last rate of decrease predicts that the iteration in
+        // flight will not end every case.  Two graphs alternate, each
+        // writing its own host status buffer.
+        auto launch = [&](int slot) -> int {
+            t_skip = c->mg_skip;
+            const int rc = run_graph(
+                c, kind, inner, [&]() { return fcg_iteration<DIM>(c, kind, inner, slot); }, slot);
+            t_skip = nullptr;
+            FCFD_TRY(rc);
+            FCFD_CHECK(hipEventRecord(c->cg_ev[slot], c->stream));
+            return FCFD_OK;
+        };
+        std::vector<double> rprev(nq);
+        for (int q = 0; q < nq; ++q) rprev[q] = cs[q].r;
+        FCFD_TRY(launch(0));
+        int queued = 1;
+        bool ahead = true;
+        for (;;) {
+            if (ahead && queued == it + 1 && queued < maxc) {
+                FCFD_TRY(launch(queued & 1));
+                ++queued;
             }
-            {
-                ProfScope ps_(c, R_RESID0);
-                hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, f, rho, L0, ih, nullptr,
-                                   mixed ? M0.f32 : nullptr, S0, c->red.partial, 0);
-                FCFD_CHECK(hipGetLastError());
+            const int slot = it & 1;
+            FCFD_CHECK(hipEventSynchronize(c->cg_ev[slot]));
+            FCFD_TRY(prof_graph_accum(c, kind, inner));
+            ++it;
+            const double *h = c->red.host + (long)slot * SC_NST * nq;
+            bool nan = false;
+            active = 0;
+            ahead = false;
+            for (int q = 0; q < nq; ++q) {
+                const double *hq = h + (long)q * SC_NST;
+                Case_ &a = cs[q];
+                a.done = hq[0] != 0.0;
+                a.k = (int)hq[SC_K - SC_DONE];
+                a.r = hq[SC_R - SC_DONE];
+                a.since = (int)hq[SC_SINCE - SC_DONE];
+                nan = nan || hq[SC_NAN - SC_DONE] != 0.0;
+                active += !a.done;
+                // residual after the next iteration at the last rate
+                if (!a.done && !(a.r * (a.r / rprev[q]) <= rel_tol)) ahead = true;
+                rprev[q] = a.r;
             }
-            FCFD_TRY(sums_to_host(c, nb, 1, s2.data()));
-            FCFD_TRY(prof_graph_accum(c, kind, inner));
-        } else {
-            // flexible CG preconditioned by the V-cycle(s), one graph per iteration
-            FCFD_TRY(run_graph(c, kind, inner, [&]() { return fcg_iteration<DIM>(c, kind, inner); }));
-            FCFD_CHECK(hipStreamSynchronize(c->stream));
-            FCFD_TRY(prof_graph_accum(c, kind, inner));
+            if (verbose) fprintf(stderr, "fcfd mg[%s+fcg]: it %d rel %.3e\n", mixed ? "mixed" : "fp64", it, rmax());
+            if (nan) {
+                FCFD_CHECK(hipStreamSynchronize(c->stream));
+                return FCFD_ERR_NAN;
+            }
+            if (active == 0) break;
+            if (queued == it) {   // nothing in flight: the next iteration is due now
+                FCFD_TRY(launch(queued & 1));
+                ++queued;
+            }
         }
+    }
+    while (!krylov && active > 0) {
+        // defect correction (mixed) or plain V-cycles on p (FP64)
+        FCFD_TRY(run_graph(c, kind, inner, [&]() { return vcycles<DIM>(c, kind, inner); }));
+        if (mixed) {
+            hipLaunchKernelGGL(k_add_f32, g, blk, 0, c->stream, p, M0.e32, L0, S0, c->mg_scalars, DIM);
+            FCFD_CHECK(hipGetLastError());
+        }
+        {
+            ProfScope ps_(c, R_RESID0);
+            hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, f, rho, L0, ih, nullptr,
+                               mixed ? M0.f32 : nullptr, S0, c->red.partial, 0);
+            FCFD_CHECK(hipGetLastError());
+        }
+        FCFD_TRY(sums_to_host(c, nb, 1, s2.data()));
+        FCFD_TRY(prof_graph_accum(c, kind, inner));
         ++it;
         for (int q = 0; q < nq; ++q) {
             Case_ &a = cs[q];
             if (a.done) continue;
-            if (krylov) {
-                const double *h = c->red.host + (long)q * SC_NHOST;
-                if (h[SC_BREAK] != 0.0) {
-                    if (verbose) fprintf(stderr, "fcfd mg: case %d breakdown rz %.3e pq %.3e\n", q, h[SC_RZ_OLD],
-                                         h[SC_PQ]);
-                    a.done = true;
-                    continue;
-                }
-                s2[q] = h[SC_RR];
-            }
             ++a.k;
             a.r = sqrt(s2[q]) / a.fn;
             if (!std::isfinite(a.r)) return FCFD_ERR_NAN;
@@ -2232,7 +2392,7 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
         }
         active = 0;
         for (const Case_ &a : cs) active += !a.done;
-        if (verbose) fprintf(stderr, "fcfd mg[%s%s]: it %d rel %.3e\n", mixed ? "mixed" : "fp64", krylov ? "+fcg" : "", it, rmax());
+        if (verbose) fprintf(stderr, "fcfd mg[%s]: it %d rel %.3e\n", mixed ? "mixed" : "fp64", it, rmax());
         if (active > 0)
             for (int q = 0; q < nq; ++q)
                 if (cs[q].done) hipLaunchKernelGGL(k_cg_done, dim3(1), dim3(1), 0, c->stream, c->mg_scalars, q);
