@@ -1478,6 +1478,14 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
 
 // ------------------------------------------------------------ host side
 
+// 256-thread block for rows of w threads: 64 x 4, or 32 x 8 where that
+// leaves fewer lanes idle past the row end (140 cells: 160 instead of 192
+// threads per row; the 70-pair colour rows: 96 instead of 128)
+static dim3 mg_blk(int w) {
+    const long p64 = (w + 63) / 64 * 64L, p32 = (w + 31) / 32 * 32L;
+    return p32 < p64 ? dim3(32, 8) : dim3(64, 4);
+}
+
 static dim3 grd3(int nx, int ny, int nz, dim3 blk) {
     return dim3((nx + blk.x - 1) / blk.x, (ny + blk.y - 1) / blk.y, nz);
 }
@@ -1683,7 +1691,6 @@ template <typename T, int DIM>
 static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1, int nu2,
                   const Dots<T> *dots = nullptr, bool zero_init = false) {
     const Geom &G = c->G;
-    dim3 blk(64, 4);
     const int nl = (int)c->mg.size();
     if (dots && nu2 < 1) return FCFD_ERR_ARG;        // the dots come from the last post-smoothing sweep
     // FP32 cycle: level 0 may read the density in FP16 (the split copy rho16)
@@ -1707,6 +1714,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         Inv2 ih = inv2(M.hx, M.hy, M.hz);
         T *e; const T *f, *bx, *by, *bz;
         arrays(lv, e, f, bx, by, bz);
+        const dim3 blk = mg_blk((M.nx + 1) / 2);   // a thread per cell pair of a row
         dim3 g((unsigned)(((M.nx + 1) / 2 + blk.x - 1) / blk.x), (M.ny + blk.y - 1) / blk.y, M.nz);
         const Prol<T> P0 = prol ? *prol : Prol<T>{nullptr, Lay{}, 0, 0, 0};
         const Dots<T> D0 = dt ? *dt : Dots<T>{nullptr, nullptr, nullptr};
@@ -1857,6 +1865,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         arrays(lv, e, f, bx, by, bz);
         T *ec; const T *fc_, *a, *b, *cc;
         arrays(lv + 1, ec, fc_, a, b, cc);
+        const dim3 blk = mg_blk(C.nx);
         dim3 g = grd3(C.nx, C.ny, C.nz, blk);
         ProfScope ps_(c, lv == 0 ? R_RESTRICT0 : R_RESTRICTC);
         const int mz = DIM == 3 ? M.cz : 1;
@@ -1957,6 +1966,7 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
         } else {
             if (lv == 0 && sizeof(T) == 4) return FCFD_ERR_ARG;   // the split level 0 prolongs inside a sweep
             ProfScope ps_(c, lv == 0 ? R_PROLONG0 : R_PROLONGC);
+            const dim3 blk = mg_blk(M.nx);
             hipLaunchKernelGGL((k_prolong<T>), grd3(M.nx, M.ny, M.nz, blk), blk, 0, c->stream, e, (const T *)ec, L,
                                LC, M.cx, M.cy, DIM == 3 ? M.cz : 1);
             FCFD_CHECK(hipGetLastError());
@@ -2130,7 +2140,7 @@ static int sums_to_host(fcfd_ctx *c, long nb, int ns, double *out) {
 // mean of an interior FP64 field into mg_scalars[q SCS + SC_MEAN] per case (device)
 static int field_mean(fcfd_ctx *c, const double *a) {
     const Geom &G = c->G;
-    dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
+    dim3 blk = mg_blk(G.nx), g = grd3(G.nx, G.ny, G.nz, blk);
     long nb = (long)g.x * g.y * g.z;
     hipLaunchKernelGGL(k_sum, g, blk, 0, c->stream, a, lay_of(G), c->red.partial);
     FCFD_TRY(sum_partials(c, nb, 1, c->mg_scalars + SC_MEAN));
@@ -2184,9 +2194,11 @@ static int vcycles(fcfd_ctx *c, int kind, int inner) {
 template <int DIM>
 static int fcg_iteration(fcfd_ctx *c, int kind, int inner, int slot) {
     const Geom &G = c->G;
-    dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
+    dim3 blk = mg_blk(G.nx), g = grd3(G.nx, G.ny, G.nz, blk);
     const long nb = (long)g.x * g.y * g.z;
-    const long nb_s = (long)(((G.nx + 1) / 2 + blk.x - 1) / blk.x) * ((G.ny + blk.y - 1) / blk.y) * G.nz;
+    // blocks of the level-0 half-sweep that folds in the CG dots (vcycle's smooth)
+    const dim3 bs = mg_blk((G.nx + 1) / 2);
+    const long nb_s = (long)(((G.nx + 1) / 2 + bs.x - 1) / bs.x) * ((G.ny + bs.y - 1) / bs.y) * G.nz;
     const dim3 gdir(g.x, g.y, DIM == 3 ? (G.nz + FCG_KC - 1) / FCG_KC : G.nz);
     const Lay L0 = lay_of(G);
     const Split S0 = split_of(G);
@@ -2240,7 +2252,7 @@ static int fcg_iteration(fcfd_ctx *c, int kind, int inner, int slot) {
 template <int DIM>
 static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel) {
     const Geom &G = c->G;
-    dim3 blk(64, 4), g = grd3(G.nx, G.ny, G.nz, blk);
+    dim3 blk = mg_blk(G.nx), g = grd3(G.nx, G.ny, G.nz, blk);
     const long nb = (long)g.x * g.y * g.z;
     const Lay L0 = lay_of(G);
     const Split S0 = split_of(G);
