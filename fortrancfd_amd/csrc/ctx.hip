@@ -254,7 +254,9 @@ static int create_impl(const fcfd_grid *grid, const fcfd_params *params, int nca
     c->gd = *grid;
     c->pr = *params;
     c->nb = ncases > 0 ? ncases : 1;
-    for (auto &t : c->tile) { t[0] = 64; t[1] = 4; t[2] = 0; }
+    // 64 x 4 tiles; in 3D each XCD walks the planes of 8-row slabs, so the
+    // z-neighbour planes the stencils re-read are still in its L2
+    for (auto &t : c->tile) { t[0] = 64; t[1] = 4; t[2] = grid->dim == 3 ? 8 : 0; }
     if (ncases > 0) c->cases.assign(cases, cases + ncases);
     guard_fp16(c);
     Geom &G = c->G;
