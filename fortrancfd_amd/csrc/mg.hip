@@ -1988,7 +1988,11 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
                                                    double *__restrict__ partial, int with_f) {
     __shared__ double sh[4];
     int bx_, by_, bz_;
-    xcd_block(bx_, by_, bz_);
+    // 3D: each XCD walks the planes of 8-row slabs, so the k-1 and k+1 planes
+    // of p and rho are still in its L2 (whole planes per XCD re-fetched them:
+    // 1.67x the algorithmic bytes at 256^3)
+    if (DIM == 3) xcd_block_slab(bx_, by_, bz_, 8);
+    else xcd_block(bx_, by_, bz_);
     const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
     double rr = 0.0, ff = 0.0;
     if (i < L.nx && j < L.ny) {
