@@ -180,60 +180,58 @@ __global__ void k_m3_redis_a(const double *__restrict__ T, const double *__restr
 // RIS_MASS's 27-point constraint (a, b zero-gradient beyond the walls, as
 // bc3d leaves them) and one RK stage of LS_REDIS (LS_REDISTANCE.F90:68-146)
 // on T in place: s = (SGN - C) |grad| - SGN, C = delta (51 a + sum a) /
-// (51 b + sum b).  A thread walks M3_KC planes of one column and carries the
-// 3 x 3 plane sums of planes k-1, k, k+1, so each plane's nine values of a
-// and b are loaded once instead of three times.  The columns (i, j) are
-// numbered in one dimension (x fastest), blockIdx.y picks the planes.
-constexpr int M3_KC = 16, M3_BT = 256;
+// (51 b + sum b).  C enters only where delta != 0, the interface band of
+// |T| <= INTERFACE_WIDTH (well under 1 % of the cells of a bubble problem);
+// elsewhere C = 0 and the cell needs none of the 54 values of a and b, so a
+// thread per cell reads them only in the band (3 x 3 plane sums of planes
+// k-1, k, k+1 in the order of the reference's loop nest).
+constexpr int M3_BT = 256;
 __global__ void __launch_bounds__(M3_BT) k_m3_redis_b(double *__restrict__ T, const double *__restrict__ S,
                                                      const double *__restrict__ Gr, const double *__restrict__ D,
                                                      const double *__restrict__ A, const double *__restrict__ B,
                                                      double *__restrict__ s0, double *__restrict__ s1, Geom G,
                                                      double rdt, int stage) {
-    int bx_, by_, bz_;
-    xcd_block(bx_, by_, bz_);
-    const unsigned t = (unsigned)bx_ * M3_BT + threadIdx.x;
-    if (t >= (unsigned)G.nx * (unsigned)G.ny) return;
-    const int j = (int)(t / (unsigned)G.nx), i = (int)(t - (unsigned)j * (unsigned)G.nx);
-    const int k0 = by_ * M3_KC, k1 = min(k0 + M3_KC, G.nz);
-    const int im = clampi(i - 1, 0, G.nx - 1), ip = clampi(i + 1, 0, G.nx - 1);
-    const int jm = clampi(j - 1, 0, G.ny - 1), jp = clampi(j + 1, 0, G.ny - 1);
-    // 3 x 3 sums of a and b over plane k (clamped), rows j-1, j, j+1
-    auto plane = [&](int k, double &pa, double &pb) {
-        const int kc = clampi(k, 0, G.nz - 1);
-        pa = 0.0;
-        pb = 0.0;
-        const int js[3] = {jm, j, jp};
+    int i, j, k;
+    if (!cell_of_flat(G, i, j, k)) return;
+    const long id = G.id(i, j, k);
+    const double d = D[id], sg = S[id], g = Gr[id];
+    double c = 0.0;
+    if (d != 0.0) {
+        const int im = clampi(i - 1, 0, G.nx - 1), ip = clampi(i + 1, 0, G.nx - 1);
+        const int jm = clampi(j - 1, 0, G.ny - 1), jp = clampi(j + 1, 0, G.ny - 1);
+        // 3 x 3 sums of a and b over plane kk (clamped), rows j-1, j, j+1
+        auto plane = [&](int kk, double &pa, double &pb) {
+            const int kc = clampi(kk, 0, G.nz - 1);
+            pa = 0.0;
+            pb = 0.0;
+            const int js[3] = {jm, j, jp};
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            const long row = G.id(0, js[r], kc);
-            pa += A[row + im];
-            pa += A[row + i];
-            pa += A[row + ip];
-            pb += B[row + im];
-            pb += B[row + i];
-            pb += B[row + ip];
-        }
-    };
-    double am, bm, ac, bc, ap, bp;
-    plane(k0 - 1, am, bm);
-    plane(k0, ac, bc);
-    for (int k = k0; k < k1; ++k) {
+            for (int r = 0; r < 3; ++r) {
+                const long row = G.id(0, js[r], kc);
+                pa += A[row + im];
+                pa += A[row + i];
+                pa += A[row + ip];
+                pb += B[row + im];
+                pb += B[row + i];
+                pb += B[row + ip];
+            }
+        };
+        double am, bm, ac, bc, ap, bp;
+        plane(k - 1, am, bm);
+        plane(k, ac, bc);
         plane(k + 1, ap, bp);
-        const long id = G.id(i, j, k);
         const double suma = 51.0 * A[id] + (am + ac + ap), sumb = 51.0 * B[id] + (bm + bc + bp);
-        const double c = (sumb != 0.0 ? suma / sumb : 0.0) * D[id];
-        const double s = (S[id] - c) * Gr[id] - S[id];
-        if (stage == 0) {
-            s0[id] = s;
-            T[id] = T[id] - rdt * s;
-        } else if (stage == 1) {
-            s1[id] = s;
-            T[id] = T[id] - rdt / 4.0 * (-3.0 * s0[id] + s);
-        } else {
-            T[id] = T[id] - rdt / 12.0 * (-s0[id] - s1[id] + 8.0 * s);
-        }
-        am = ac; bm = bc; ac = ap; bc = bp;
+        c = (sumb != 0.0 ? suma / sumb : 0.0) * d;
+    }
+    const double s = (sg - c) * g - sg;
+    if (stage == 0) {
+        s0[id] = s;
+        T[id] = T[id] - rdt * s;
+    } else if (stage == 1) {
+        s1[id] = s;
+        T[id] = T[id] - rdt / 4.0 * (-3.0 * s0[id] + s);
+    } else {
+        T[id] = T[id] - rdt / 12.0 * (-s0[id] - s1[id] + 8.0 * s);
     }
 }
 
@@ -556,7 +554,6 @@ static int m3_redistance(fcfd_ctx *c, double *T) {
     hipLaunchKernelGGL(k_m3_sign, grd, blk_i(), 0, c->stream, T, F[FCFD_SIGN], G, c->pr.ls_wid);
     FCFD_CHECK(hipGetLastError());
     const dim3 ga = grid_flat(G, M3_BT);
-    const dim3 gb((unsigned)(((long)G.nx * G.ny + M3_BT - 1) / M3_BT), (unsigned)((G.nz + M3_KC - 1) / M3_KC));
     const int n = m3_redis_steps(c);
     for (int it = 0; it < n; ++it)
         for (int s = 0; s < 3; ++s) {
@@ -568,7 +565,7 @@ static int m3_redistance(fcfd_ctx *c, double *T) {
             }
             {
                 ProfScope ps_(c, R_REDIS_B);
-                hipLaunchKernelGGL(k_m3_redis_b, gb, dim3(M3_BT), 0, c->stream, T, F[FCFD_SIGN], F[FCFD_GRAD],
+                hipLaunchKernelGGL(k_m3_redis_b, ga, dim3(M3_BT), 0, c->stream, T, F[FCFD_SIGN], F[FCFD_GRAD],
                                    F[FCFD_DELTA], c->s2, c->s3, c->l1, c->l2, G, c->pr.rdt, s);
                 FCFD_CHECK(hipGetLastError());
             }
