@@ -766,12 +766,13 @@ __global__ void __launch_bounds__(256) k_redis_b(double *__restrict__ phi, const
     }
 }
 
-// Pass B in 3D, RB_KC planes per thread: the 3x3x3 sums of s2 and s3 are
-// carried as 3x3 plane sums of planes k-1, k, k+1 (clamped), so a cell
-// loads 9 values of each instead of 27.  Same update as k_redis_b; the sums
-// associate differently (plane by plane), a change of rounding only.  The
-// stage and btn are template arguments so that every load of a plane is
-// issued at the top of its iteration, before any of them is consumed.
+// Pass B in 3D, RB_KC planes per thread.  The constraint term is
+// (a / b) delta, and delta = 0 off the interface band, so only band cells
+// (a thin shell, a few % of the cells) read the 3x3x3 sums of s2 and s3,
+// as 3x3 plane sums of planes k-1, k, k+1 (clamped).  Same update as
+// k_redis_b; the sums associate differently (plane by plane), a change of
+// rounding only.  The stage and btn are template arguments so that every
+// load of a plane is issued at the top of its iteration.
 constexpr int RB_KC = 16;
 template <int STAGE, bool BTN>
 __global__ void __launch_bounds__(256) k_redis_b_km(double *__restrict__ phi, const double *__restrict__ S0,
@@ -808,30 +809,25 @@ __global__ void __launch_bounds__(256) k_redis_b_km(double *__restrict__ phi, co
                 b += t3[q];
             }
         };
-        double am = 0.0, bm = 0.0, ac = 0.0, bc = 0.0;
-        if (BTN) {
-            plane(G.id(i, j, max(k0 - 1, 0)), am, bm);
-            plane(G.id(i, j, k0), ac, bc);
-        }
         long id = G.id(i, j, k0);
         for (int k = k0; k < k1; ++k, id += G.sxy) {
             // every load of the plane first
             const double p0 = phi[id], S = S0[id], gr = grad[id];
             const double l1o = STAGE >= 1 ? l1[id] : 0.0, l2o = STAGE == 2 ? l2[id] : 0.0;
             const double pst = STAGE == 2 ? phis[id] : 0.0;
-            double c2 = 0.0, c3 = 0.0, ap = ac, bp = bc;
-            if (BTN) {
-                c2 = s2[id];
-                c3 = s3[id];
-                if (k + 1 <= G.nz - 1) plane(id + G.sxy, ap, bp);
-            }
             double hh, dd;
             lsfun(p0, w, hh, dd);
+            // the constraint enters as (a / b) delta: only cells of the
+            // interface band (delta != 0) read the 27-point sums
             double s1 = 0.0;
-            if (BTN) {
-                const double a = 51.0 * c2 + am + ac + ap, b = 51.0 * c3 + bm + bc + bp;
+            if (BTN && dd != 0.0) {
+                double am, bm, ac, bc, ap = 0.0, bp = 0.0;
+                plane(id - (k > 0 ? G.sxy : 0), am, bm);
+                plane(id, ac, bc);
+                if (k + 1 <= G.nz - 1) plane(id + G.sxy, ap, bp);
+                else { ap = ac; bp = bc; }
+                const double a = 51.0 * s2[id] + am + ac + ap, b = 51.0 * s3[id] + bm + bc + bp;
                 if (fabs(b) > 1.0e-12) s1 = a / b * dd;
-                am = ac; bm = bc; ac = ap; bc = bp;
             }
             const double l = (S - s1) * gr - S;
             double src;
