@@ -2025,7 +2025,7 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
 // (8 waves per SIMD, <= 64 VGPRs) the compiler issues the neighbour loads of
 // a plane one at a time, each followed by a wait; allowing 5 waves per SIMD
 // lets it issue them together (about 7 % faster at 256^3).
-constexpr int FCG_KC = 16;
+constexpr int FCG_KC = 8;
 template <typename T, int DIM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 5))) k_fcg_dir(const T *__restrict__ z, T *__restrict__ zo,
                                                  double *__restrict__ p, double *__restrict__ q,
