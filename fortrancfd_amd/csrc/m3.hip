@@ -47,21 +47,12 @@ __device__ __forceinline__ double m3_heav(double x, double w, double &hp) {
     return 0.5 * (1.0 + x / w + 1.0 / kPi * s);
 }
 
-__device__ __forceinline__ bool cell_of_block(const Geom &G, int &i, int &j, int &k) {
-    int bx_, by_, bz_;
-    xcd_block(bx_, by_, bz_);
-    i = bx_ * blockDim.x + threadIdx.x;
-    j = by_ * blockDim.y + threadIdx.y;
-    k = bz_;
-    return i < G.nx && j < G.ny;
-}
-
 // HEAVY_F / AMURHO over the interior: H (zero below EPS), delta and, when R
 // is given, rho = rho_12 H + (1 - H), mu = mu_12 H + (1 - H)
 __global__ void k_m3_props(const double *__restrict__ phi, double *__restrict__ H, double *__restrict__ D,
                            double *__restrict__ R, double *__restrict__ M, Geom G, double w, double r12, double m12) {
     int i, j, k;
-    if (!cell_of_block(G, i, j, k)) return;
+    if (!cell_of_flat(G, i, j, k)) return;
     const long id = G.id(i, j, k);
     double hp;
     double h = m3_heav(phi[id], w, hp);
@@ -78,7 +69,7 @@ __global__ void k_m3_props(const double *__restrict__ phi, double *__restrict__ 
 __global__ void k_m3_cell_vel(const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
                               double *__restrict__ UH, double *__restrict__ VH, double *__restrict__ WH, Geom G) {
     int i, j, k;
-    if (!cell_of_block(G, i, j, k)) return;
+    if (!cell_of_flat(G, i, j, k)) return;
     const long id = G.id(i, j, k);
     UH[id] = 0.5 * (U[id] + U[id - 1]);
     VH[id] = 0.5 * (V[id] + V[id - G.sx]);
@@ -93,7 +84,7 @@ __global__ void k_m3_adv_stage(double *__restrict__ phi, const double *__restric
                                const double *__restrict__ fy, const double *__restrict__ fz, double *__restrict__ s0,
                                double *__restrict__ s1, Geom G, double dt, int stage) {
     int i, j, k;
-    if (!cell_of_block(G, i, j, k)) return;
+    if (!cell_of_flat(G, i, j, k)) return;
     const long id = G.id(i, j, k);
     const double s = -(fx[id] * uh[id] + fy[id] * vh[id] + fz[id] * wh[id]);
     if (stage == 0) {
@@ -110,7 +101,7 @@ __global__ void k_m3_adv_stage(double *__restrict__ phi, const double *__restric
 // sgnf (LS_REDISTANCE.F90:360-376)
 __global__ void k_m3_sign(const double *__restrict__ phi, double *__restrict__ S, Geom G, double w) {
     int i, j, k;
-    if (!cell_of_block(G, i, j, k)) return;
+    if (!cell_of_flat(G, i, j, k)) return;
     const long id = G.id(i, j, k);
     double hp;
     const double h = m3_heav(phi[id], w, hp);
@@ -300,10 +291,10 @@ __global__ void k_m3_sums(const double *__restrict__ H, const double *__restrict
 __global__ void k_m3_mpls_update(double *__restrict__ phi, const double *__restrict__ D, const double *__restrict__ Gr,
                                  const double *__restrict__ sums, double *__restrict__ lam_out, Geom G, double im,
                                  double dt) {
-    int i, j, k;
-    const bool in = cell_of_block(G, i, j, k);
+    int i = -1, j = -1, k = -1;
+    const bool in = cell_of_flat(G, i, j, k);
     const double lam = (im - sums[1]) / (dt * sums[2]);
-    if (i == 0 && j == 0 && k == 0) *lam_out = lam;
+    if (in && i == 0 && j == 0 && k == 0) *lam_out = lam;
     if (!in) return;
     const long id = G.id(i, j, k);
     phi[id] = phi[id] + lam * D[id] * Gr[id] * dt;
@@ -313,7 +304,7 @@ __global__ void k_m3_mpls_update(double *__restrict__ phi, const double *__restr
 __global__ void k_m3_merge(const double *__restrict__ p1, const double *__restrict__ p2, double *__restrict__ phi,
                            Geom G) {
     int i, j, k;
-    if (!cell_of_block(G, i, j, k)) return;
+    if (!cell_of_flat(G, i, j, k)) return;
     const long id = G.id(i, j, k);
     phi[id] = fmax(p1[id], p2[id]);
 }
@@ -324,7 +315,7 @@ __global__ void k_m3_curv(const double *__restrict__ X, const double *__restrict
                           const double *__restrict__ XY, const double *__restrict__ XZ, const double *__restrict__ YZ,
                           double *__restrict__ C, Geom G) {
     int i, j, k;
-    if (!cell_of_block(G, i, j, k)) return;
+    if (!cell_of_flat(G, i, j, k)) return;
     const long id = G.id(i, j, k);
     const double fx = X[id], fy = Y[id], fz = Z[id];
     C[id] = (XX[id] * (sq(fy) + sq(fz)) + YY[id] * (sq(fx) + sq(fz)) + ZZ[id] * (sq(fx) + sq(fy)) -
@@ -362,7 +353,7 @@ __global__ void k_m3_predict(M3Vel V, M3Src Sr, const double *__restrict__ mo, c
                              const double *__restrict__ nx_, const double *__restrict__ ny_,
                              const double *__restrict__ nz_, Geom G, M3Phys ph) {
     int i, j, k;
-    if (!cell_of_block(G, i, j, k)) return;
+    if (!cell_of_flat(G, i, j, k)) return;
     const long id = G.id(i, j, k), sx = G.sx, sxy = G.sxy;
     const double dx = G.dx, dy = G.dy, dz = G.dz;
     const double re2x = ph.re * (dx * dx), re2y = ph.re * (dy * dy), re2z = ph.re * (dz * dz);
@@ -417,7 +408,7 @@ __global__ void k_m3_rhs(const double *__restrict__ uo, const double *__restrict
                          const double *__restrict__ u1, const double *__restrict__ v1, const double *__restrict__ w1,
                          double *__restrict__ rhs, Geom G, double dt) {
     int i, j, k;
-    if (!cell_of_block(G, i, j, k)) return;
+    if (!cell_of_flat(G, i, j, k)) return;
     const long id = G.id(i, j, k), sx = G.sx, sxy = G.sxy;
     const double ue = i < G.nx - 1 ? u1[id] : 0.0, uw = i > 0 ? u1[id - 1] : 0.0;
     const double vn = j < G.ny - 1 ? v1[id] : 0.0, vs = j > 0 ? v1[id - sx] : 0.0;
@@ -469,12 +460,12 @@ __global__ void k_m3_momentum(double *__restrict__ u, double *__restrict__ v, do
 
 // ------------------------------------------------------------ host side
 
-static dim3 blk_i() { return dim3(64, 4); }
 
 // the fields only the MPLS_RisingBubble step holds, allocated on first use
 // (by an M3 entry point or a push / pull of one of them)
 int m3_fields(fcfd_ctx *c) {
-    if (c->G.dim != 3) return FCFD_ERR_ARG;
+    // the kernels number the interior cells in 32 bits (grid_flat)
+    if (c->G.dim != 3 || !flat_fits(c->G)) return FCFD_ERR_ARG;
     for (int f : {FCFD_PHI1, FCFD_PHI2, FCFD_RHO_OLD, FCFD_MU_OLD})
         if (!c->f[f]) {
             FCFD_CHECK(hipMalloc((void **)&c->f[f], sizeof(double) * c->G.ntot));
@@ -490,7 +481,7 @@ static int m3_props(fcfd_ctx *c, const double *phi, int with_rho) {
     double **F = c->f;
     {
         ProfScope ps_(c, R_MISC);
-        hipLaunchKernelGGL(k_m3_props, grid_interior(G, blk_i()), blk_i(), 0, c->stream, phi, F[FCFD_HEAVY],
+        hipLaunchKernelGGL(k_m3_props, grid_flat(G, M3_BT), dim3(M3_BT), 0, c->stream, phi, F[FCFD_HEAVY],
                            F[FCFD_DELTA], with_rho ? F[FCFD_RHO] : nullptr, with_rho ? F[FCFD_MU] : nullptr, G,
                            c->pr.ls_wid, c->pr.rho_12, c->pr.mu_12);
         FCFD_CHECK(hipGetLastError());
@@ -510,7 +501,7 @@ static int m3_ls_rk3(fcfd_ctx *c) {
     double **F = c->f;
     {
         ProfScope ps_(c, R_MISC);
-        hipLaunchKernelGGL(k_m3_cell_vel, grid_interior(G, blk_i()), blk_i(), 0, c->stream, F[FCFD_U], F[FCFD_V],
+        hipLaunchKernelGGL(k_m3_cell_vel, grid_flat(G, M3_BT), dim3(M3_BT), 0, c->stream, F[FCFD_U], F[FCFD_V],
                            F[FCFD_W], F[FCFD_NU], F[FCFD_NV], F[FCFD_NW], G);
         FCFD_CHECK(hipGetLastError());
     }
@@ -523,7 +514,7 @@ static int m3_ls_rk3(fcfd_ctx *c) {
         for (int dir = 0; dir < 3; ++dir) FCFD_TRY(uccd_lines_set(c, 1, dir, F[FCFD_PHI], vel[dir], d[dir], nullptr));
         {
             ProfScope ps_(c, R_LS_ADV);
-            hipLaunchKernelGGL(k_m3_adv_stage, grid_interior(G, blk_i()), blk_i(), 0, c->stream, F[FCFD_PHI], vel[0],
+            hipLaunchKernelGGL(k_m3_adv_stage, grid_flat(G, M3_BT), dim3(M3_BT), 0, c->stream, F[FCFD_PHI], vel[0],
                                vel[1], vel[2], d[0], d[1], d[2], c->l1, c->l2, G, c->pr.dt, s);
             FCFD_CHECK(hipGetLastError());
         }
@@ -549,9 +540,8 @@ static int m3_redis_steps(const fcfd_ctx *c) {
 static int m3_redistance(fcfd_ctx *c, double *T) {
     const Geom &G = c->G;
     double **F = c->f;
-    dim3 grd = grid_interior(G, blk_i());
     if (!flat_fits(G)) return FCFD_ERR_ARG;
-    hipLaunchKernelGGL(k_m3_sign, grd, blk_i(), 0, c->stream, T, F[FCFD_SIGN], G, c->pr.ls_wid);
+    hipLaunchKernelGGL(k_m3_sign, grid_flat(G, M3_BT), dim3(M3_BT), 0, c->stream, T, F[FCFD_SIGN], G, c->pr.ls_wid);
     FCFD_CHECK(hipGetLastError());
     const dim3 ga = grid_flat(G, M3_BT);
     const int n = m3_redis_steps(c);
@@ -601,7 +591,7 @@ static int m3_mpls(fcfd_ctx *c, double im) {
         FCFD_TRY(ccd_lines_set(c, 1, 2, F[FCFD_PHI], F[FCFD_NZ], c->cz));
         FCFD_TRY(m3_sums(c, 1));
         ProfScope ps_(c, R_MPLS_UPD);
-        hipLaunchKernelGGL(k_m3_mpls_update, grid_interior(G, blk_i()), blk_i(), 0, c->stream, F[FCFD_PHI],
+        hipLaunchKernelGGL(k_m3_mpls_update, grid_flat(G, M3_BT), dim3(M3_BT), 0, c->stream, F[FCFD_PHI],
                            F[FCFD_DELTA], F[FCFD_GRAD], c->red.result, c->red.result + RES_LAMBDA, G, im, c->pr.dt);
         FCFD_CHECK(hipGetLastError());
     }
@@ -622,7 +612,7 @@ static int m3_ls_ccd(fcfd_ctx *c) {
     FCFD_TRY(ccd_lines_set(c, 1, 2, F[FCFD_NY], F[FCFD_NYZ], c->cx));   // FY_Z
     {
         ProfScope ps_(c, R_CURV);
-        hipLaunchKernelGGL(k_m3_curv, grid_interior(G, blk_i()), blk_i(), 0, c->stream, F[FCFD_NX], F[FCFD_NY],
+        hipLaunchKernelGGL(k_m3_curv, grid_flat(G, M3_BT), dim3(M3_BT), 0, c->stream, F[FCFD_NX], F[FCFD_NY],
                            F[FCFD_NZ], F[FCFD_NXX], F[FCFD_NYY], F[FCFD_NZZ], F[FCFD_NXY], F[FCFD_NXZ], F[FCFD_NYZ],
                            F[FCFD_CURV], G);
         FCFD_CHECK(hipGetLastError());
@@ -643,7 +633,7 @@ static int m3_solve_ls(fcfd_ctx *c) {
         FCFD_TRY(m3_mpls(c, im[b]));
         FCFD_TRY(copy_field(c, F[fb[b]], F[FCFD_PHI]));
     }
-    hipLaunchKernelGGL(k_m3_merge, grid_interior(G, blk_i()), blk_i(), 0, c->stream, F[FCFD_PHI1], F[FCFD_PHI2],
+    hipLaunchKernelGGL(k_m3_merge, grid_flat(G, M3_BT), dim3(M3_BT), 0, c->stream, F[FCFD_PHI1], F[FCFD_PHI2],
                        F[FCFD_PHI], G);
     FCFD_CHECK(hipGetLastError());
     FCFD_TRY(bc_scalar(c, F[FCFD_PHI]));
@@ -667,7 +657,7 @@ static int m3_predict(fcfd_ctx *c) {
             c->un, c->vn, c->wn};
     M3Phys ph{c->pr.re, c->pr.we, c->pr.fr * c->pr.fr, c->pr.ls_wid, c->pr.btn_sf, c->pr.btn_g};
     ProfScope ps_(c, R_PRED);
-    hipLaunchKernelGGL(k_m3_predict, grid_interior(G, blk_i()), blk_i(), 0, c->stream, V, S, F[FCFD_MU_OLD],
+    hipLaunchKernelGGL(k_m3_predict, grid_flat(G, M3_BT), dim3(M3_BT), 0, c->stream, V, S, F[FCFD_MU_OLD],
                        F[FCFD_RHO_OLD], F[FCFD_PHI], F[FCFD_CURV], F[FCFD_NX], F[FCFD_NY], F[FCFD_NZ], G, ph);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
@@ -691,7 +681,7 @@ static int m3_solve_ns(fcfd_ctx *c) {
         if (!c->m3.rec_mass) return FCFD_OK;
         {
             ProfScope ps_(c, R_PPE_SRC);
-            hipLaunchKernelGGL(k_m3_rhs, grid_interior(G, blk_i()), blk_i(), 0, c->stream, F[FCFD_U_OLD],
+            hipLaunchKernelGGL(k_m3_rhs, grid_flat(G, M3_BT), dim3(M3_BT), 0, c->stream, F[FCFD_U_OLD],
                                F[FCFD_V_OLD], F[FCFD_W_OLD], c->un, c->vn, c->wn, F[FCFD_PPE_SRC], G, c->pr.dt);
             FCFD_CHECK(hipGetLastError());
         }

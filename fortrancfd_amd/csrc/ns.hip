@@ -157,7 +157,10 @@ __global__ void __launch_bounds__(256) k_predictor(
     if (i >= G.nx || j >= G.ny) return;
     const long id = G.id(i, j, k);
     const long sx = G.sx, sxy = G.sxy;
-    const double dx = G.dx, dy = G.dy, dz = G.dz;
+    // the one-sided differences scaled by 1/h: three divisions per thread
+    // instead of 6 per component (an FP64 division is ~10 instructions); the
+    // products differ from the quotients by at most an ulp
+    const double idx = 1.0 / G.dx, idy = 1.0 / G.dy, idz = 1.0 / G.dz;
     // advecting velocities (u, v, w) at each MAC face
     double adv[3][3];
     adv[0][0] = Ut[id];
@@ -185,15 +188,15 @@ __global__ void __launch_bounds__(256) k_predictor(
     for (int c = 0; c < DIM; ++c) {
         const double *g = F[c];
         const double u = adv[c][0], v = adv[c][1];
-        double xp = 0.5 * (-g[id + 2] + 4.0 * g[id + 1] - 3.0 * g[id]) / dx;
-        double xm = 0.5 * (g[id - 2] - 4.0 * g[id - 1] + 3.0 * g[id]) / dx;
-        double yp = 0.5 * (-g[id + 2 * sx] + 4.0 * g[id + sx] - 3.0 * g[id]) / dy;
-        double ym = 0.5 * (g[id - 2 * sx] - 4.0 * g[id - sx] + 3.0 * g[id]) / dy;
+        double xp = 0.5 * (-g[id + 2] + 4.0 * g[id + 1] - 3.0 * g[id]) * idx;
+        double xm = 0.5 * (g[id - 2] - 4.0 * g[id - 1] + 3.0 * g[id]) * idx;
+        double yp = 0.5 * (-g[id + 2 * sx] + 4.0 * g[id + sx] - 3.0 * g[id]) * idy;
+        double ym = 0.5 * (g[id - 2 * sx] - 4.0 * g[id - sx] + 3.0 * g[id]) * idy;
         double a = -((u + fabs(u)) * xm + (u - fabs(u)) * xp) / 2.0 - ((v + fabs(v)) * ym + (v - fabs(v)) * yp) / 2.0;
         if (DIM == 3) {
             const double w = adv[c][2];
-            double zp = 0.5 * (-g[id + 2 * sxy] + 4.0 * g[id + sxy] - 3.0 * g[id]) / dz;
-            double zm = 0.5 * (g[id - 2 * sxy] - 4.0 * g[id - sxy] + 3.0 * g[id]) / dz;
+            double zp = 0.5 * (-g[id + 2 * sxy] + 4.0 * g[id + sxy] - 3.0 * g[id]) * idz;
+            double zm = 0.5 * (g[id - 2 * sxy] - 4.0 * g[id - sxy] + 3.0 * g[id]) * idz;
             a = a - ((w + fabs(w)) * zm + (w - fabs(w)) * zp) / 2.0;
         }
         double s = a + D[c][id];
