@@ -356,7 +356,8 @@ int ccd_setup(fcfd_ctx *c) {
             FCFD_CHECK(hipMalloc((void **)&dst, sizeof(double) * pk.size()));
             FCFD_CHECK(hipMemcpy(dst, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
             // the partitioned solve numbers lines and segment points in 32 bits
-            if (nl[d] >= (1L << 31) || seg_count(N) > 65535) continue;
+            // (x lines: one launch dimension counts the groups of 64 lines)
+            if (nl[d] >= (1L << 31) || seg_count(N) > 65535 || (d == 0 && (nl[d] + 63) / 64 > 65535)) continue;
             ccd_seg_tables(kind, N, h[d], pk);
             FCFD_CHECK(hipMalloc((void **)&seg, sizeof(double) * pk.size()));
             FCFD_CHECK(hipMemcpy(seg, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
@@ -866,6 +867,18 @@ __global__ void __launch_bounds__(SEGT) k_seg_lines(const double *__restrict__ f
 // own row from registers.
 constexpr int SEGX_TS = SEG + 3;
 
+// (line group, segment) of an x-line block.  The launch is (P, groups): the
+// segments of a line group are consecutive logical blocks, and xcd_block
+// deals consecutive logical blocks to one XCD, so the cache lines that two
+// neighbouring segments of a row share (a row's 18-value span is not 128-B
+// aligned) are fetched into that XCD's L2 once.
+__device__ __forceinline__ void segx_block(long &t0, int &s) {
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    s = bx;
+    t0 = (long)by * WAVE;
+}
+
 // the block's rows (x lines t0 ..) and f(a - 1 .. a + n) of each, clipped to
 // the line, into the tile; fv[k] = f(a - 1 + k) of this lane's row
 __device__ __forceinline__ void segx_stage(const double *__restrict__ f, double *tile, long *rowb, const Geom &G,
@@ -910,9 +923,11 @@ __global__ void __launch_bounds__(WAVE) k_seg_ends_x(const double *__restrict__ 
     __shared__ long rowb[WAVE];
     const long nl = (long)G.ny * G.nz;
     const int N = G.nx + 2 * ext;
-    const long t0 = (long)blockIdx.x * WAVE, t = t0 + threadIdx.x;
+    long t0;
+    int s;
+    segx_block(t0, s);
+    const long t = t0 + threadIdx.x;
     const int rows = (int)min((long)WAVE, nl - t0);
-    const int s = blockIdx.y;
     const int a = seg_start(s, N, P), n = seg_start(s + 1, N, P) - a;
     double fv[SEG + 2];
     segx_stage(f, tile, rowb, G, ext, nl, t0, rows, a, n, N, fv);
@@ -930,9 +945,11 @@ __global__ void __launch_bounds__(WAVE) k_seg_lines_x(const double *__restrict__
     const long nl = (long)G.ny * G.nz;
     const int N = G.nx + 2 * ext;
     const int lane = threadIdx.x;
-    const long t0 = (long)blockIdx.x * WAVE, t = t0 + lane;
+    long t0;
+    int s;
+    segx_block(t0, s);
+    const long t = t0 + lane;
     const int rows = (int)min((long)WAVE, nl - t0);
-    const int s = blockIdx.y;
     const int a = seg_start(s, N, P), n = seg_start(s + 1, N, P) - a;
     double fv[SEG + 2];
     segx_stage(f, tile, rowb, G, ext, nl, t0, rows, a, n, N, fv);
@@ -1110,7 +1127,7 @@ static int seg_solve(fcfd_ctx *c, int ext, int k0, int k1, const double *f, cons
     const unsigned gx = (unsigned)((nl + SEGT - 1) / SEGT), gw = (unsigned)((nl + WAVE - 1) / WAVE);
     if (P > 1) {
         if constexpr (DIR == 0)
-            hipLaunchKernelGGL(k_seg_ends_x<NS>, dim3(gw, (unsigned)P), dim3(WAVE), 0, c->stream, f, c->ccd_z,
+            hipLaunchKernelGGL(k_seg_ends_x<NS>, dim3((unsigned)P, gw), dim3(WAVE), 0, c->stream, f, c->ccd_z,
                                tu + end_off, td + end_off, G, ext, P);
         else
             hipLaunchKernelGGL((k_seg_ends<DIR, NS>), dim3(gx, (unsigned)P), dim3(SEGT), 0, c->stream, f, c->ccd_z,
@@ -1123,7 +1140,7 @@ static int seg_solve(fcfd_ctx *c, int ext, int k0, int k1, const double *f, cons
                                td + red_off, nl, P);
     }
     if constexpr (DIR == 0)
-        hipLaunchKernelGGL(k_seg_lines_x<NS>, dim3(gw, (unsigned)P), dim3(WAVE), 0, c->stream, f, u, fx, fxx,
+        hipLaunchKernelGGL(k_seg_lines_x<NS>, dim3((unsigned)P, gw), dim3(WAVE), 0, c->stream, f, u, fx, fxx,
                            c->ccd_z, tu, td, G, ext, P, cs);
     else
         hipLaunchKernelGGL((k_seg_lines<DIR, NS>), dim3(gx, (unsigned)P), dim3(SEGT), 0, c->stream, f, u, fx, fxx,
