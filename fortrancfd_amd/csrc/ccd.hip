@@ -356,8 +356,9 @@ int ccd_setup(fcfd_ctx *c) {
             FCFD_CHECK(hipMalloc((void **)&dst, sizeof(double) * pk.size()));
             FCFD_CHECK(hipMemcpy(dst, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
             // the partitioned solve numbers lines and segment points in 32 bits
-            // (x lines: one launch dimension counts the groups of 64 lines)
-            if (nl[d] >= (1L << 31) || seg_count(N) > 65535 || (d == 0 && (nl[d] + 63) / 64 > 65535)) continue;
+            // (one launch dimension counts the groups of 64 x lines or 256 y / z lines)
+            if (nl[d] >= (1L << 31) || seg_count(N) > 65535 || (nl[d] + (d == 0 ? 63 : 255)) / (d == 0 ? 64 : 256) > 65535)
+                continue;
             ccd_seg_tables(kind, N, h[d], pk);
             FCFD_CHECK(hipMalloc((void **)&seg, sizeof(double) * pk.size()));
             FCFD_CHECK(hipMemcpy(seg, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
@@ -676,6 +677,16 @@ __global__ void __launch_bounds__(CCDX_LINES) k_ccd_lines_x(const double *__rest
 // segment, so every coefficient read is uniform over the wave.
 constexpr int SEGT = 256;
 
+// (segment, first line) of a y / z block: as for x lines (segx_block) the
+// launch is (P, line groups), so the P segments of a group, whose f reads
+// overlap by one point at each end, run consecutively on one XCD
+__device__ __forceinline__ void segyz_block(int &s, long &t) {
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    s = bx;
+    t = (long)by * SEGT + threadIdx.x;
+}
+
 // line geometry of the y (DIR 1) or z (DIR 2) lines
 template <int DIR>
 __device__ __forceinline__ void seg_line_geom(const Geom &G, int ext, long &nl, int &N, long &st, double &dx) {
@@ -713,9 +724,11 @@ __global__ void __launch_bounds__(SEGT) k_seg_ends(const double *__restrict__ f,
     int N;
     double dx;
     seg_line_geom<DIR>(G, ext, nl, N, st, dx);
-    const long t = (long)blockIdx.x * SEGT + threadIdx.x;
+    int s;
+    long t;
+    segyz_block(s, t);
     if (t >= nl) return;
-    const int s = blockIdx.y, a = seg_start(s, N, P);
+    const int a = seg_start(s, N, P);
     const int lb = (int)((unsigned)t / (unsigned)G.nx), la = (int)((unsigned)t - (unsigned)lb * (unsigned)G.nx);
     const long line = DIR == 1 ? G.id(la, -ext, lb) : G.id(la, lb, -ext);
     // all loads unconditional (indices clamped to the line), issued together;
@@ -816,9 +829,10 @@ __global__ void __launch_bounds__(SEGT) k_seg_lines(const double *__restrict__ f
     int N;
     double dx;
     seg_line_geom<DIR>(G, ext, nl, N, st, dx);
-    const long t = (long)blockIdx.x * SEGT + threadIdx.x;
+    int s;
+    long t;
+    segyz_block(s, t);
     if (t >= nl) return;
-    const int s = blockIdx.y;
     const int a = seg_start(s, N, P), n = seg_start(s + 1, N, P) - a;
     const int lb = (int)((unsigned)t / (unsigned)G.nx), la = (int)((unsigned)t - (unsigned)lb * (unsigned)G.nx);
     const long line = DIR == 1 ? G.id(la, -ext, lb) : G.id(la, lb, -ext);
@@ -1130,7 +1144,7 @@ static int seg_solve(fcfd_ctx *c, int ext, int k0, int k1, const double *f, cons
             hipLaunchKernelGGL(k_seg_ends_x<NS>, dim3((unsigned)P, gw), dim3(WAVE), 0, c->stream, f, c->ccd_z,
                                tu + end_off, td + end_off, G, ext, P);
         else
-            hipLaunchKernelGGL((k_seg_ends<DIR, NS>), dim3(gx, (unsigned)P), dim3(SEGT), 0, c->stream, f, c->ccd_z,
+            hipLaunchKernelGGL((k_seg_ends<DIR, NS>), dim3((unsigned)P, gx), dim3(SEGT), 0, c->stream, f, c->ccd_z,
                                tu + end_off, td + end_off, G, ext, P);
         if (P - 1 <= SEG_MAXI)
             hipLaunchKernelGGL(k_seg_reduce_r<SEG_MAXI>, dim3(gx, NS), dim3(SEGT), 0, c->stream, c->ccd_z,
@@ -1143,7 +1157,7 @@ static int seg_solve(fcfd_ctx *c, int ext, int k0, int k1, const double *f, cons
         hipLaunchKernelGGL(k_seg_lines_x<NS>, dim3((unsigned)P, gw), dim3(WAVE), 0, c->stream, f, u, fx, fxx,
                            c->ccd_z, tu, td, G, ext, P, cs);
     else
-        hipLaunchKernelGGL((k_seg_lines<DIR, NS>), dim3(gx, (unsigned)P), dim3(SEGT), 0, c->stream, f, u, fx, fxx,
+        hipLaunchKernelGGL((k_seg_lines<DIR, NS>), dim3((unsigned)P, gx), dim3(SEGT), 0, c->stream, f, u, fx, fxx,
                            c->ccd_z, tu, td, G, ext, P, cs);
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;

@@ -228,7 +228,31 @@ void orc_get_diag(const orc_state *s, fcfd_diag *d) {
 #define ORC_PRAGMA(x) _Pragma(#x)
 /* OpenMP versions over the interior (used by the converged pressure solve) */
 #define PFOR_INT(s) ORC_PRAGMA(omp parallel for collapse(2)) FOR_INT(s)
-#define PFOR_SUM(s, v) ORC_PRAGMA(omp parallel for collapse(2) reduction(+ : v)) FOR_INT(s)
+/* Deterministic parallel sum over the interior: every thread adds its static
+ * share of the (k, j) rows in loop order into a private v, and the thread
+ * partials are added in thread order after the loop.  An OpenMP reduction
+ * combines them in arrival order, which made the converged pressure differ
+ * between runs at the 1e-11 level; this sum is the same in every run with
+ * a given thread count, and with one thread it is the sequential sum.
+ * Use: PSUM_BEGIN(s, v) { body adding to v } PSUM_END(v) */
+#define ORC_MAXT 256
+static int orc_nthreads(void) {
+    const int n = omp_get_max_threads();
+    return n < ORC_MAXT ? n : ORC_MAXT;
+}
+#define PSUM_BEGIN(s, v)                                                        \
+    {                                                                           \
+        double v##_p_[ORC_MAXT];                                                \
+        int v##_n_ = 1;                                                         \
+        ORC_PRAGMA(omp parallel num_threads(orc_nthreads())) {                  \
+            double v = 0.0;                                                     \
+            ORC_PRAGMA(omp for collapse(2) schedule(static)) FOR_INT(s)
+#define PSUM_END(v)                                                             \
+            v##_p_[omp_get_thread_num()] = v;                                   \
+            if (omp_get_thread_num() == 0) v##_n_ = omp_get_num_threads();       \
+        }                                                                       \
+        for (int t_ = 0; t_ < v##_n_; ++t_) v += v##_p_[t_];                   \
+    }
 #define FOR_ALL(s)                                           \
     for (int k = -(s)->gz; k < (s)->nz + (s)->gz; ++k)       \
         for (int j = -(s)->g; j < (s)->ny + (s)->g; ++j)     \
@@ -400,15 +424,30 @@ int orc_ls_mv(orc_state *s) {
     double *H = s->heavy.lv[L_NOW];
     const double r12 = s->pr.rho_12;
     double mass = 0.0, vol = 0.0;
-#pragma omp parallel for collapse(2) reduction(+ : mass, vol)
-    for (int k = 0; k < s->nz; ++k)
-        for (int j = 0; j < s->ny; ++j)
-            for (int i = 0; i < s->nx; ++i) {
-                double h = H[ID(s, i, j, k)];
-                double rho = h + r12 * (1.0 - h);
-                mass = mass + rho * h * dv;
-                vol = vol + h * dv;
-            }
+    {   /* deterministic parallel sums, as PSUM_BEGIN / PSUM_END */
+        double mp[ORC_MAXT], vp[ORC_MAXT];
+        int nt = 1;
+#pragma omp parallel num_threads(orc_nthreads())
+        {
+            double m = 0.0, v = 0.0;
+#pragma omp for collapse(2) schedule(static)
+            for (int k = 0; k < s->nz; ++k)
+                for (int j = 0; j < s->ny; ++j)
+                    for (int i = 0; i < s->nx; ++i) {
+                        double h = H[ID(s, i, j, k)];
+                        double rho = h + r12 * (1.0 - h);
+                        m = m + rho * h * dv;
+                        v = v + h * dv;
+                    }
+            mp[omp_get_thread_num()] = m;
+            vp[omp_get_thread_num()] = v;
+            if (omp_get_thread_num() == 0) nt = omp_get_num_threads();
+        }
+        for (int t = 0; t < nt; ++t) {
+            mass = mass + mp[t];
+            vol = vol + vp[t];
+        }
+    }
     s->mass = mass;
     s->vol = vol;
     return FCFD_OK;
@@ -1071,10 +1110,7 @@ int orc_mass_preserving_level_set(orc_state *s) {
         orc_ls_mv(s);
         orc_surface_norms(s);
         double lam = 0.0;
-#pragma omp parallel for collapse(2) reduction(+ : lam)
-        for (int k = 0; k < s->nz; ++k)
-            for (int j = 0; j < s->ny; ++j)
-                for (int i = 0; i < s->nx; ++i) {
+        PSUM_BEGIN(s, lam) {
                     long id = ID(s, i, j, k);
                     double plam;
                     if (s->dim == 3) {
@@ -1087,7 +1123,7 @@ int orc_mass_preserving_level_set(orc_state *s) {
                         plam = plam * (2.0 * (1.0 - r12) * H[id] + r12) * s->dx * s->dy;
                     }
                     lam = lam + plam;
-                }
+        } PSUM_END(lam)
         lam = (s->imass - s->mass) / lam;
         s->mpls_lam = lam;
         FOR_INT(s) {
@@ -1416,42 +1452,42 @@ static int ppe_pcg(orc_state *s, double rel_tol) {
     double *r = falloc(n), *z = falloc(n), *q = falloc(n), *Aq = falloc(n), *f = falloc(n);
     const double ncell = (double)s->nx * (double)s->ny * (double)s->nz;
     double mean = 0.0;
-    PFOR_SUM(s, mean) mean += s->coe_src[ID(s, i, j, k)];
+    PSUM_BEGIN(s, mean) mean += s->coe_src[ID(s, i, j, k)]; PSUM_END(mean)
     mean /= ncell;
     double fn = 0.0;
-    PFOR_SUM(s, fn) { long id = ID(s, i, j, k); f[id] = -(s->coe_src[id] - mean); fn += f[id] * f[id]; }
+    PSUM_BEGIN(s, fn) { long id = ID(s, i, j, k); f[id] = -(s->coe_src[id] - mean); fn += f[id] * f[id]; } PSUM_END(fn)
     fn = sqrt(fn);
     orc_ppe_apply(s, P, Aq);
     double rn = 0.0, rz = 0.0, zm = 0.0;
-    PFOR_SUM(s, rn) {
+    PSUM_BEGIN(s, rn) {
         long id = ID(s, i, j, k);
         r[id] = f[id] + Aq[id];
         rn += r[id] * r[id];
-    }
+    } PSUM_END(rn)
     int it = 0, rc = FCFD_OK;
     if (fn == 0.0) fn = 1.0;
     if (sqrt(rn) / fn > rel_tol) {
-        PFOR_SUM(s, zm) { long id = ID(s, i, j, k); z[id] = r[id] / (-s->coe_c[id]); zm += z[id]; }
+        PSUM_BEGIN(s, zm) { long id = ID(s, i, j, k); z[id] = r[id] / (-s->coe_c[id]); zm += z[id]; } PSUM_END(zm)
         zm /= ncell;
-        PFOR_SUM(s, rz) { long id = ID(s, i, j, k); z[id] -= zm; q[id] = z[id]; rz += r[id] * z[id]; }
+        PSUM_BEGIN(s, rz) { long id = ID(s, i, j, k); z[id] -= zm; q[id] = z[id]; rz += r[id] * z[id]; } PSUM_END(rz)
         for (it = 1; it <= 200000; ++it) {
             orc_ppe_apply(s, q, Aq);
             double qAq = 0.0;
-            PFOR_SUM(s, qAq) { long id = ID(s, i, j, k); Aq[id] = -Aq[id]; qAq += q[id] * Aq[id]; }
+            PSUM_BEGIN(s, qAq) { long id = ID(s, i, j, k); Aq[id] = -Aq[id]; qAq += q[id] * Aq[id]; } PSUM_END(qAq)
             double alpha = rz / qAq;
             rn = 0.0;
-            PFOR_SUM(s, rn) {
+            PSUM_BEGIN(s, rn) {
                 long id = ID(s, i, j, k);
                 P[id] += alpha * q[id];
                 r[id] -= alpha * Aq[id];
                 rn += r[id] * r[id];
-            }
+            } PSUM_END(rn)
             if (sqrt(rn) / fn <= rel_tol) break;
             double rz2 = 0.0;
             zm = 0.0;
-            PFOR_SUM(s, zm) { long id = ID(s, i, j, k); z[id] = r[id] / (-s->coe_c[id]); zm += z[id]; }
+            PSUM_BEGIN(s, zm) { long id = ID(s, i, j, k); z[id] = r[id] / (-s->coe_c[id]); zm += z[id]; } PSUM_END(zm)
             zm /= ncell;
-            PFOR_SUM(s, rz2) { long id = ID(s, i, j, k); z[id] -= zm; rz2 += r[id] * z[id]; }
+            PSUM_BEGIN(s, rz2) { long id = ID(s, i, j, k); z[id] -= zm; rz2 += r[id] * z[id]; } PSUM_END(rz2)
             double beta = rz2 / rz;
             rz = rz2;
             PFOR_INT(s) { long id = ID(s, i, j, k); q[id] = z[id] + beta * q[id]; }
@@ -1459,7 +1495,7 @@ static int ppe_pcg(orc_state *s, double rel_tol) {
         if (it > 200000) rc = FCFD_ERR_NOT_CONVERGED;
     }
     double pm = 0.0;
-    PFOR_SUM(s, pm) pm += P[ID(s, i, j, k)];
+    PSUM_BEGIN(s, pm) pm += P[ID(s, i, j, k)]; PSUM_END(pm)
     pm /= ncell;
     PFOR_INT(s) P[ID(s, i, j, k)] -= pm;
     bc(s, P);
