@@ -54,7 +54,9 @@ def region_bytes(case, mixed: bool) -> dict:
         "ccd_lines": 8 * N * 3,                # one direction per launch: R{f} W{fx,fxx}
         "curv": 8 * N * 10,
         "diff_source": 8 * N * 11,
-        "predictor": 8 * N * 18,
+        # R{u,v,w (Picard iterate), u,v,w old, D x3, S_old x3} W{u* x3, S x3},
+        # S stored by the last of the step's picard launches only
+        "predictor": 8 * N * (15 + 3 / (case.picard_iters if case.picard_iters > 0 else 4)),
         "ppe_src": 8 * N * 4,
         "projection": 8 * N * 8,
         "node_vel": 8 * N * 6,

@@ -48,6 +48,21 @@ struct Geom {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// d[m] <- d[clamp(m, lo, hi)] for m = 0 .. W-1 (0 <= lo <= hi < W) by
+// selects, without indexing registers at run time: the clamped differences
+// of a line near a wall taken from the unclamped ones of its load window
+template <int W>
+__device__ __forceinline__ void clamp_window(double (&d)[W], int lo, int hi) {
+    double vlo = d[0], vhi = d[0];
+#pragma unroll
+    for (int r = 1; r < W; ++r) {
+        vlo = lo == r ? d[r] : vlo;
+        vhi = hi == r ? d[r] : vhi;
+    }
+#pragma unroll
+    for (int m = 0; m < W; ++m) d[m] = m < lo ? vlo : (m > hi ? vhi : d[m]);
+}
+
 // One level of the pressure multigrid hierarchy (see mg.hip).
 struct MgLevel {
     int nx, ny, nz;          // cells
@@ -317,7 +332,7 @@ int redis(fcfd_ctx *c, int btn, int *iters);
 int mpls(fcfd_ctx *c);
 int ls_mv(fcfd_ctx *c, double *mass, double *vol);
 int ns_diff_source(fcfd_ctx *c);
-int ns_predictor(fcfd_ctx *c, const double *ut, const double *vt, const double *wt);
+int ns_predictor(fcfd_ctx *c, const double *ut, const double *vt, const double *wt, int write_src = 1);
 int ppe_src(fcfd_ctx *c);
 int projection(fcfd_ctx *c);
 int node_vel(fcfd_ctx *c);

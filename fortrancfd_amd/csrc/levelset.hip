@@ -673,16 +673,26 @@ __global__ void __launch_bounds__(256) k_redis_a(const double *__restrict__ phi,
     const long st[3] = {1, G.sx, G.sxy};
     const double ih[3] = {1.0 / G.dx, 1.0 / G.dy, 1.0 / G.dz};
     double sel[3];
+    // phi at pos-4 .. pos+3 of the three lines, all loads before any branch
+    // (pos-4 is clamped to the outermost ghost layer; that value is never
+    // selected below)
+    double gw[3][8];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        gw[d][0] = phi[id + (long)(max(pos[d] - 4, -3) - pos[d]) * st[d]];
+#pragma unroll
+        for (int r = 1; r < 8; ++r) gw[d][r] = phi[id + (long)(r - 4) * st[d]];
+    }
     const bool pos_sign = S0[id] >= 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        double nr[7];  // D-phi at pos-3 .. pos+3 (ghost entries copy the boundary one)
+        // D-phi at pos-3 .. pos+3, ghost entries copying the boundary one:
+        // nr[m] = D(clamp(pos - 3 + m, 0, n - 1)) = dw[clamp(m, 3 - pos, n + 2 - pos)]
+        // with dw[m] = D(pos - 3 + m) the window's neighbour differences
+        double nr[7];
 #pragma unroll
-        for (int o = -3; o <= 3; ++o) {
-            int q = clampi(pos[d] + o, 0, n[d] - 1);
-            long a = id + (long)(q - pos[d]) * st[d];
-            nr[o + 3] = (phi[a] - phi[a - st[d]]) * ih[d];
-        }
+        for (int m = 0; m < 7; ++m) nr[m] = (gw[d][m + 1] - gw[d][m]) * ih[d];
+        if (pos[d] < 3 || pos[d] > n[d] - 4) clamp_window(nr, 3 - pos[d], n[d] + 2 - pos[d]);
         const double EPS = 1.0e-10;
         double fm = weno_left(nr[1], nr[2], nr[3], nr[4], nr[5], EPS);   // tdata%s2
         double fp = weno_right(nr[2], nr[3], nr[4], nr[5], nr[6], EPS);  // tdata%s1

@@ -117,25 +117,23 @@ __device__ __forceinline__ double sq(double x) { return x * x; }
 
 // the one-sided WENO values of the backward differences along a line:
 // F(m) = (T(m) - T(m-1)) / h for interior m, F of a ghost index = F of the
-// nearest interior index (FV_DERI's bc3d); n the line's interior length
-__device__ __forceinline__ void m3_line_weno(const double *__restrict__ T, long id, long st, int i, int n, double ih,
-                                             double &up, double &um) {
-    double f[6];   // F(i-2) .. F(i+3); the differences times 1 / h
-    if (i >= 2 && i <= n - 4) {
-        // no index clamped: the six differences share the seven values T(i-3 .. i+3)
-        double g[7];
+// nearest interior index (FV_DERI's bc3d); n the line's interior length.
+// The seven values T(i-3 .. i+3) lie inside the 3 ghost layers for every
+// i, and every clamped difference is one of their six neighbour
+// differences: F(clamp(i-2+q)) = d[clamp(q, 2-i, n+1-i)].  So the loads
+// take no branch, the caller issues the three lines of a cell together
+// (m3_line_load), and only the cells within two of a wall remap their
+// differences.
+__device__ __forceinline__ void m3_line_load(const double *__restrict__ T, long id, long st, double (&g)[7]) {
 #pragma unroll
-        for (int r = 0; r < 7; ++r) g[r] = T[id + (long)(r - 3) * st];
+    for (int r = 0; r < 7; ++r) g[r] = T[id + (long)(r - 3) * st];
+}
+
+__device__ __forceinline__ void m3_line_weno(const double (&g)[7], int i, int n, double ih, double &up, double &um) {
+    double f[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) f[q] = (g[q + 1] - g[q]) * ih;
-    } else {
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const int m = clampi(i - 2 + q, 0, n - 1);
-            const long o = id + (long)(m - i) * st;
-            f[q] = (T[o] - T[o - st]) * ih;
-        }
-    }
+    for (int q = 0; q < 6; ++q) f[q] = (g[q + 1] - g[q]) * ih;
+    if (i < 2 || i > n - 4) clamp_window(f, 2 - i, n + 1 - i);
     um = weno_left(f[0], f[1], f[2], f[3], f[4], kEps);
     up = weno_right(f[1], f[2], f[3], f[4], f[5], kEps);
 }
@@ -143,16 +141,21 @@ __device__ __forceinline__ void m3_line_weno(const double *__restrict__ T, long 
 // GRADFI + the pointwise part of RIS_MASS (LS_REDISTANCE.F90:148-238):
 // |grad T| by Godunov on the WENO values, delta of T, and the constraint
 // terms a = SGN delta (|grad| - 1), b = |grad| delta^2
-__global__ void k_m3_redis_a(const double *__restrict__ T, const double *__restrict__ S, double *__restrict__ Gr,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4)))
+k_m3_redis_a(const double *__restrict__ T, const double *__restrict__ S, double *__restrict__ Gr,
                              double *__restrict__ D, double *__restrict__ A, double *__restrict__ B, Geom G, double w) {
     int i, j, k;
     if (!cell_of_flat(G, i, j, k)) return;
     const long id = G.id(i, j, k);
-    double up, um, vp, vm, wp, wm;
-    m3_line_weno(T, id, 1, i, G.nx, 1.0 / G.dx, up, um);
-    m3_line_weno(T, id, G.sx, j, G.ny, 1.0 / G.dy, vp, vm);
-    m3_line_weno(T, id, G.sxy, k, G.nz, 1.0 / G.dz, wp, wm);
+    double gx[7], gy[7], gz[7];
+    m3_line_load(T, id, 1, gx);
+    m3_line_load(T, id, G.sx, gy);
+    m3_line_load(T, id, G.sxy, gz);
     const double sg = S[id];
+    double up, um, vp, vm, wp, wm;
+    m3_line_weno(gx, i, G.nx, 1.0 / G.dx, up, um);
+    m3_line_weno(gy, j, G.ny, 1.0 / G.dy, vp, vm);
+    m3_line_weno(gz, k, G.nz, 1.0 / G.dz, wp, wm);
     double g;
     if (sg >= 0.0)
         g = sqrt(sq(fmax(-fmin(up, 0.0), fmax(um, 0.0))) + sq(fmax(-fmin(vp, 0.0), fmax(vm, 0.0))) +
@@ -161,7 +164,7 @@ __global__ void k_m3_redis_a(const double *__restrict__ T, const double *__restr
         g = sqrt(sq(fmax(fmax(up, 0.0), -fmin(um, 0.0))) + sq(fmax(fmax(vp, 0.0), -fmin(vm, 0.0))) +
                  sq(fmax(fmax(wp, 0.0), -fmin(wm, 0.0))));
     double hp;
-    m3_heav(T[id], w, hp);
+    m3_heav(gx[3], w, hp);
     Gr[id] = g;
     D[id] = hp;
     A[id] = (sg * hp) * (g - 1.0);
@@ -348,10 +351,16 @@ __device__ __forceinline__ double upwind2(double a, const double *__restrict__ o
                       am * (-o[id + 2 * d] + 4.0 * o[id + d] - 3.0 * o[id]));
 }
 
-__global__ void k_m3_predict(M3Vel V, M3Src Sr, const double *__restrict__ mo, const double *__restrict__ ro,
-                             const double *__restrict__ phi, const double *__restrict__ K,
-                             const double *__restrict__ nx_, const double *__restrict__ ny_,
-                             const double *__restrict__ nz_, Geom G, M3Phys ph) {
+// The smoothed delta of the three faces is formed first (its band test is a
+// branch, and loads after a branch cannot be issued before it); the
+// component loop then has no branches, so its loads issue together.  The
+// outermost face of each direction (not updated: its source keeps the old
+// value) is formed too and not selected; its stencil stays inside the
+// ghost layers.
+__global__ void __launch_bounds__(M3_BT) __attribute__((amdgpu_waves_per_eu(1, 4)))
+k_m3_predict(M3Vel V, M3Src Sr, const double *__restrict__ mo, const double *__restrict__ ro,
+             const double *__restrict__ phi, const double *__restrict__ K, const double *__restrict__ nx_,
+             const double *__restrict__ ny_, const double *__restrict__ nz_, Geom G, M3Phys ph) {
     int i, j, k;
     if (!cell_of_flat(G, i, j, k)) return;
     const long id = G.id(i, j, k), sx = G.sx, sxy = G.sxy;
@@ -364,40 +373,43 @@ __global__ void k_m3_predict(M3Vel V, M3Src Sr, const double *__restrict__ mo, c
     const double *srco[3] = {Sr.uso, Sr.vso, Sr.wso};
     double *u1[3] = {Sr.u1, Sr.v1, Sr.w1};
     const bool inner[3] = {i < G.nx - 1, j < G.ny - 1, k < G.nz - 1};
+    double detf[3], pf[3];
+    const double pc = phi[id];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pf[c] = phi[id + off[c]];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m3_heav(0.5 * (pc + pf[c]), ph.wid, detf[c]);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        double s = src[c][id];
-        if (inner[c]) {
-            const long d = off[c];
-            double a[3];   // advecting velocity at this face, per direction
-            if (c == 0) {
-                a[0] = V.u[id];
-                a[1] = 0.25 * (V.v[id + 1] + V.v[id] + V.v[id + 1 - sx] + V.v[id - sx]);
-                a[2] = 0.25 * (V.w[id + 1] + V.w[id] + V.w[id + 1 - sxy] + V.w[id - sxy]);
-            } else if (c == 1) {
-                a[0] = 0.25 * (V.u[id + sx] + V.u[id] + V.u[id - 1 + sx] + V.u[id - 1]);
-                a[1] = V.v[id];
-                a[2] = 0.25 * (V.w[id + sx] + V.w[id] + V.w[id + sx - sxy] + V.w[id - sxy]);
-            } else {
-                a[0] = 0.25 * (V.u[id + sxy] + V.u[id] + V.u[id - 1 + sxy] + V.u[id - 1]);
-                a[1] = 0.25 * (V.v[id + sxy] + V.v[id] + V.v[id - sx + sxy] + V.v[id - sx]);
-                a[2] = V.w[id];
-            }
-            const double *q = old[c];
-            const double cm = mo[id] + mo[id + d], cr = ro[id] + ro[id + d], cc = cm / cr;
-            double detf;
-            m3_heav(0.5 * (phi[id] + phi[id + d]), ph.wid, detf);
-            const double curvk = 0.5 * (K[id] + K[id + d]), fin = 0.5 * (nrm[c][id] + nrm[c][id + d]);
-            const double a1 = upwind2(a[0], q, id, 1, dx), a2 = upwind2(a[1], q, id, sx, dy),
-                         a3 = upwind2(a[2], q, id, sxy, dz);
-            const double a4 = 1.0 / re2x * cc * (q[id + 1] - 2.0 * q[id] + q[id - 1]);
-            const double a5 = 1.0 / re2y * cc * (q[id + sx] - 2.0 * q[id] + q[id - sx]);
-            const double a6 = 1.0 / re2z * cc * (q[id + sxy] - 2.0 * q[id] + q[id - sxy]);
-            const double a7 = ph.st_force != 0.0 ? 2.0 / (cr * ph.we) * curvk * detf * fin : 0.0;
-            s = a1 + a2 + a3 - a4 - a5 - a6 + a7;
-            if (c == 2 && ph.g_force != 0.0) s = s + 1.0 / ph.fr2;
-            src[c][id] = s;
+        const double s_old = src[c][id];
+        const long d = off[c];
+        double a[3];   // advecting velocity at this face, per direction
+        if (c == 0) {
+            a[0] = V.u[id];
+            a[1] = 0.25 * (V.v[id + 1] + V.v[id] + V.v[id + 1 - sx] + V.v[id - sx]);
+            a[2] = 0.25 * (V.w[id + 1] + V.w[id] + V.w[id + 1 - sxy] + V.w[id - sxy]);
+        } else if (c == 1) {
+            a[0] = 0.25 * (V.u[id + sx] + V.u[id] + V.u[id - 1 + sx] + V.u[id - 1]);
+            a[1] = V.v[id];
+            a[2] = 0.25 * (V.w[id + sx] + V.w[id] + V.w[id + sx - sxy] + V.w[id - sxy]);
+        } else {
+            a[0] = 0.25 * (V.u[id + sxy] + V.u[id] + V.u[id - 1 + sxy] + V.u[id - 1]);
+            a[1] = 0.25 * (V.v[id + sxy] + V.v[id] + V.v[id - sx + sxy] + V.v[id - sx]);
+            a[2] = V.w[id];
         }
+        const double *q = old[c];
+        const double cm = mo[id] + mo[id + d], cr = ro[id] + ro[id + d], cc = cm / cr;
+        const double curvk = 0.5 * (K[id] + K[id + d]), fin = 0.5 * (nrm[c][id] + nrm[c][id + d]);
+        const double a1 = upwind2(a[0], q, id, 1, dx), a2 = upwind2(a[1], q, id, sx, dy),
+                     a3 = upwind2(a[2], q, id, sxy, dz);
+        const double a4 = 1.0 / re2x * cc * (q[id + 1] - 2.0 * q[id] + q[id - 1]);
+        const double a5 = 1.0 / re2y * cc * (q[id + sx] - 2.0 * q[id] + q[id - sx]);
+        const double a6 = 1.0 / re2z * cc * (q[id + sxy] - 2.0 * q[id] + q[id - sxy]);
+        const double a7 = ph.st_force != 0.0 ? 2.0 / (cr * ph.we) * curvk * detf[c] * fin : 0.0;
+        double s = a1 + a2 + a3 - a4 - a5 - a6 + a7;
+        if (c == 2 && ph.g_force != 0.0) s = s + 1.0 / ph.fr2;
+        s = inner[c] ? s : s_old;
+        if (inner[c]) src[c][id] = s;
         u1[c][id] = 1.5 * s - 0.5 * srco[c][id];
     }
 }

@@ -155,7 +155,9 @@ extern "C" int fcfd_ns_solver(fcfd_ctx *c) {
     const double tol = c->pr.ppe_tol > 0.0 ? c->pr.ppe_tol : 1e-10;
     int total = 0, rc = FCFD_OK;
     for (int it = 0; it < np; ++it) {
-        FCFD_TRY(ns_predictor(c, c->f[FCFD_U], c->f[FCFD_V], c->f[FCFD_W]));
+        // S is stored by the last Picard iteration only (the earlier ones'
+        // S would be overwritten unread)
+        FCFD_TRY(ns_predictor(c, c->f[FCFD_U], c->f[FCFD_V], c->f[FCFD_W], it == np - 1));
         FCFD_TRY(ppe_src(c));
         int cyc = 0;
         int r = ppe_solve(c, tol, c->pr.ppe_precision, &cyc, nullptr);
