@@ -289,6 +289,23 @@ __device__ __forceinline__ bool cell_of_flat(const Geom &G, int &i, int &j, int 
     return true;
 }
 
+// cell (i, j, k) of flattened interior index t (i fastest), by 32-bit
+// division where t fits (a 64-bit one is several times longer)
+__device__ __forceinline__ void cell_of_index(long t, const Geom &G, int &i, int &j, int &k) {
+    if (t < 4294967296L) {
+        const unsigned tt = (unsigned)t, nx = (unsigned)G.nx, ny = (unsigned)G.ny;
+        const unsigned r = tt / nx, kk = r / ny;
+        i = (int)(tt - r * nx);
+        j = (int)(r - kk * ny);
+        k = (int)kk;
+    } else {
+        i = (int)(t % G.nx);
+        const long r = t / G.nx;
+        j = (int)(r % G.ny);
+        k = (int)(r / G.ny);
+    }
+}
+
 inline bool flat_fits(const Geom &G) { return (double)G.nx * G.ny * G.nz < 4294967296.0 - 1024.0; }
 
 inline dim3 grid_flat(const Geom &G, int blk) {

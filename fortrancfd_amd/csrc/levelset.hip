@@ -80,12 +80,6 @@ int ls_props(fcfd_ctx *c, const double *phi, double *H, double *D, double *S, do
 constexpr int RBLK = 1024;
 constexpr int RTHR = 256;
 
-__device__ __forceinline__ void cell_of(long t, const Geom &G, int &i, int &j, int &k) {
-    i = (int)(t % G.nx);
-    long r = t / G.nx;
-    j = (int)(r % G.ny);
-    k = (int)(r / G.ny);
-}
 
 template <int NS>
 __device__ __forceinline__ void block_sum_store(double (&acc)[NS], double *partial, int nblocks) {
@@ -140,7 +134,7 @@ __global__ void k_mass_sums(const double *__restrict__ phi, const double *__rest
     const double dv = DIM == 3 ? G.dx * G.dy * G.dz : G.dx * G.dy;
     for (long t = (long)blockIdx.x * RTHR + threadIdx.x; t < n; t += (long)RBLK * RTHR) {
         int i, j, k;
-        cell_of(t, G, i, j, k);
+        cell_of_index(t, G, i, j, k);
         if (DIM == 2) k = q;
         long id = G.id(i, j, k);
         double h, d;

@@ -273,9 +273,8 @@ __global__ void k_m3_sums(const double *__restrict__ H, const double *__restrict
     double acc[3] = {0.0, 0.0, 0.0};
     const long n = (long)G.nx * G.ny * G.nz;
     for (long t = (long)blockIdx.x * RT + threadIdx.x; t < n; t += (long)RB * RT) {
-        const int i = (int)(t % G.nx);
-        const long r = t / G.nx;
-        const int j = (int)(r % G.ny), k = (int)(r / G.ny);
+        int i, j, k;
+        cell_of_index(t, G, i, j, k);
         const long id = G.id(i, j, k);
         const double h = H[id];
         acc[0] += h;
@@ -439,30 +438,35 @@ __global__ void k_m3_momentum(double *__restrict__ u, double *__restrict__ v, do
                               const double *__restrict__ v1, const double *__restrict__ w1,
                               const double *__restrict__ p, const double *__restrict__ rho, double *__restrict__ partial,
                               Geom G, double dt) {
+    // every load of a cell before the face tests (the stencils of the
+    // outermost faces, which are not updated, stay inside the ghost layers)
     double acc[3] = {0.0, 0.0, 0.0};
     const long n = (long)G.nx * G.ny * G.nz;
     for (long t = (long)blockIdx.x * RT + threadIdx.x; t < n; t += (long)RB * RT) {
-        const int i = (int)(t % G.nx);
-        const long r = t / G.nx;
-        const int j = (int)(r % G.ny), k = (int)(r / G.ny);
+        int i, j, k;
+        cell_of_index(t, G, i, j, k);
         const long id = G.id(i, j, k);
+        const double pc = p[id], rc = rho[id];
+        const double pe = p[id + 1], pn = p[id + G.sx], pt = p[id + G.sxy];
+        const double re = rho[id + 1], rn = rho[id + G.sx], rt = rho[id + G.sxy];
+        const double uc = u[id], vc = v[id], wc = w[id];
+        double au = -u1[id] - 2.0 * (pe - pc) / (G.dx * (rc + re));
+        au = uo[id] + au * dt;
+        double av = -v1[id] - 2.0 * (pn - pc) / (G.dy * (rc + rn));
+        av = vo[id] + av * dt;
+        double aw = -w1[id] - 2.0 * (pt - pc) / (G.dz * (rc + rt));
+        aw = wo[id] + aw * dt;
         if (i < G.nx - 1) {
-            double a = -u1[id] - 2.0 * (p[id + 1] - p[id]) / (G.dx * (rho[id] + rho[id + 1]));
-            a = uo[id] + a * dt;
-            acc[0] += sq(a - u[id]);
-            u[id] = a;
+            acc[0] += sq(au - uc);
+            u[id] = au;
         }
         if (j < G.ny - 1) {
-            double a = -v1[id] - 2.0 * (p[id + G.sx] - p[id]) / (G.dy * (rho[id] + rho[id + G.sx]));
-            a = vo[id] + a * dt;
-            acc[1] += sq(a - v[id]);
-            v[id] = a;
+            acc[1] += sq(av - vc);
+            v[id] = av;
         }
         if (k < G.nz - 1) {
-            double a = -w1[id] - 2.0 * (p[id + G.sxy] - p[id]) / (G.dz * (rho[id] + rho[id + G.sxy]));
-            a = wo[id] + a * dt;
-            acc[2] += sq(a - w[id]);
-            w[id] = a;
+            acc[2] += sq(aw - wc);
+            w[id] = aw;
         }
     }
     m3_block_store<3>(acc, partial);

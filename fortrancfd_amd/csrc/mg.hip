@@ -869,14 +869,29 @@ __global__ void __launch_bounds__(CG_THREADS) k_cg_check(double *sc, int nq, int
     cg_all_done(sc, nq, skip);
 }
 
-// sum of an interior field into block partials
+// sum of an interior field into block partials; in 3D a block sums SUM_KC
+// planes (each thread its column's, loads issued together, the planes past
+// the top clamped and not added), in 2D its slice (a case of a batched stack)
+constexpr int SUM_KC = 8;
+template <int DIM>
 __global__ void __launch_bounds__(256) k_sum(const double *__restrict__ a, Lay L, double *__restrict__ partial) {
     __shared__ double sh[4];
     int bx_, by_, bz_;
     xcd_block(bx_, by_, bz_);
-    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y, k = bz_;
+    const int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y;
     double v = 0.0;
-    if (i < L.nx && j < L.ny) v = a[L.id(i, j, k)];
+    if (i < L.nx && j < L.ny) {
+        if (DIM == 3) {
+            const int k0 = bz_ * SUM_KC;
+            double x[SUM_KC];
+#pragma unroll
+            for (int q = 0; q < SUM_KC; ++q) x[q] = a[L.id(i, j, min(k0 + q, L.nz - 1))];
+#pragma unroll
+            for (int q = 0; q < SUM_KC; ++q) v += k0 + q < L.nz ? x[q] : 0.0;
+        } else {
+            v = a[L.id(i, j, bz_)];
+        }
+    }
     v = block_sum256(v, sh);
     if (threadIdx.x + threadIdx.y == 0) partial[bx_ + (long)gridDim.x * (by_ + (long)gridDim.y * bz_)] = v;
 }
@@ -2145,8 +2160,10 @@ static int sums_to_host(fcfd_ctx *c, long nb, int ns, double *out) {
 static int field_mean(fcfd_ctx *c, const double *a) {
     const Geom &G = c->G;
     dim3 blk = mg_blk(G.nx), g = grd3(G.nx, G.ny, G.nz, blk);
+    if (G.dim == 3) g.z = (G.nz + SUM_KC - 1) / SUM_KC;
     long nb = (long)g.x * g.y * g.z;
-    hipLaunchKernelGGL(k_sum, g, blk, 0, c->stream, a, lay_of(G), c->red.partial);
+    if (G.dim == 3) hipLaunchKernelGGL(k_sum<3>, g, blk, 0, c->stream, a, lay_of(G), c->red.partial);
+    else hipLaunchKernelGGL(k_sum<2>, g, blk, 0, c->stream, a, lay_of(G), c->red.partial);
     FCFD_TRY(sum_partials(c, nb, 1, c->mg_scalars + SC_MEAN));
     return FCFD_OK;
 }

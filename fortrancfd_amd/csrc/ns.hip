@@ -262,18 +262,19 @@ __global__ void k_node_vel(const double *__restrict__ U, const double *__restric
     if (DIM == 3) NW[id] = 0.5 * (W[id - G.sxy] + W[id]);
 }
 
-// max and sum of |div u| (ns_check_convergence_div :70-88): block partials
+// max and sum of |div u| (ns_check_convergence_div :70-88): block partials.
+// A grid-stride loop over the flattened interior with 32-bit index
+// arithmetic (the host keeps the cell count plus one stride below 2^32).
 template <int DIM>
-__global__ void k_div(const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
-                      double *__restrict__ partial, Geom G, int nblocks) {
+__global__ void __launch_bounds__(256) k_div(const double *__restrict__ U, const double *__restrict__ V,
+                                             const double *__restrict__ W, double *__restrict__ partial, Geom G,
+                                             int nblocks) {
     __shared__ double smax[256], ssum[256];
     double mx = 0.0, sm = 0.0;
-    const long n = (long)G.nx * G.ny * G.nz;
-    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)nblocks * 256) {
-        int i = (int)(t % G.nx);
-        long r = t / G.nx;
-        int j = (int)(r % G.ny), k = (int)(r / G.ny);
-        long id = G.id(i, j, k);
+    const unsigned nx = (unsigned)G.nx, nxy = nx * (unsigned)G.ny, n = nxy * (unsigned)G.nz;
+    for (unsigned t = blockIdx.x * 256u + threadIdx.x; t < n; t += (unsigned)nblocks * 256u) {
+        const unsigned k = t / nxy, r = t - k * nxy, j = r / nx, i = r - j * nx;
+        const long id = G.id((int)i, (int)j, (int)k);
         double d = (U[id] - U[id - 1]) / G.dx + (V[id] - V[id - G.sx]) / G.dy;
         if (DIM == 3) d = d + (W[id] - W[id - G.sxy]) / G.dz;
         mx = fmax(mx, fabs(d));
@@ -456,7 +457,10 @@ __global__ void k_div_final(const double *__restrict__ partial, int nblocks, dou
 int div_check(fcfd_ctx *c, int to_host) {
     const Geom &G = c->G;
     double **F = c->f;
-    const int nb = 256;
+    // 8 blocks per CU: with one block per CU the loop's dependent loads left
+    // the launch latency-bound (222 us at 256^3)
+    const int nb = 2048;
+    if ((double)G.nx * G.ny * G.nz + (double)nb * 256 >= 4294967296.0) return FCFD_ERR_ARG;
     if (G.dim == 3)
         hipLaunchKernelGGL(k_div<3>, dim3(nb), dim3(256), 0, c->stream, F[FCFD_U], F[FCFD_V], F[FCFD_W],
                            c->red.partial, G, nb);
