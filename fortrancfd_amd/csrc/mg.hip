@@ -2301,7 +2301,6 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
         FCFD_CHECK(hipGetLastError());
     }
     std::vector<double> s2(2 * nq);
-    FCFD_TRY(sums_to_host(c, nb, 2, s2.data()));
     struct Case_ {
         double fn, r, best;
         int k, since;
@@ -2309,22 +2308,32 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     };
     std::vector<Case_> cs(nq);
     int active = 0;
-    for (int q = 0; q < nq; ++q) {
-        Case_ &a = cs[q];
-        a.fn = s2[2 * q + 1] > 0.0 ? sqrt(s2[2 * q + 1]) : 1.0;
-        a.r = sqrt(s2[2 * q]) / a.fn;
-        a.best = a.r;
-        a.k = a.since = 0;
-        if (!std::isfinite(a.r)) return FCFD_ERR_NAN;
-        a.done = !(a.r > rel_tol && a.k < maxc);
-        active += !a.done;
+    if (krylov) {
+        // ||r0||^2 and ||f||^2 stay on the device: k_cg_start forms each
+        // case's stopping state there, and the host learns it with the first
+        // iteration's status (no host round trip before the first cycle)
+        FCFD_TRY(sum_partials(c, nb, 2, c->mg_scalars));
+        for (Case_ &a : cs) a = Case_{1.0, 0.0, 0.0, 0, 0, false};
+        active = nq;
+    } else {
+        FCFD_TRY(sums_to_host(c, nb, 2, s2.data()));
+        for (int q = 0; q < nq; ++q) {
+            Case_ &a = cs[q];
+            a.fn = s2[2 * q + 1] > 0.0 ? sqrt(s2[2 * q + 1]) : 1.0;
+            a.r = sqrt(s2[2 * q]) / a.fn;
+            a.best = a.r;
+            a.k = a.since = 0;
+            if (!std::isfinite(a.r)) return FCFD_ERR_NAN;
+            a.done = !(a.r > rel_tol && a.k < maxc);
+            active += !a.done;
+        }
     }
     auto rmax = [&]() {
         double m = 0.0;
         for (const Case_ &a : cs) m = std::max(m, a.r);
         return m;
     };
-    if (verbose) fprintf(stderr, "fcfd mg[%s%s]: it 0 rel %.3e\n", mixed ? "mixed" : "fp64", krylov ? "+fcg" : "", rmax());
+    if (verbose && !krylov) fprintf(stderr, "fcfd mg[%s]: it 0 rel %.3e\n", mixed ? "mixed" : "fp64", rmax());
     hipLaunchKernelGGL(k_cg_init, dim3(nq), dim3(1), 0, c->stream, c->mg_scalars);
     if (krylov)
         hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(CG_THREADS), 0, c->stream, c->mg_scalars, nq, rel_tol, maxc,
@@ -2353,8 +2362,9 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
             FCFD_CHECK(hipEventRecord(c->cg_ev[slot], c->stream));
             return FCFD_OK;
         };
-        std::vector<double> rprev(nq);
-        for (int q = 0; q < nq; ++q) rprev[q] = cs[q].r;
+        // the residual before the first status read is unknown here: 0 makes
+        // the first prediction queue ahead
+        std::vector<double> rprev(nq, 0.0);
         FCFD_TRY(launch(0));
         int queued = 1;
         bool ahead = true;
@@ -2430,14 +2440,15 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
             for (int q = 0; q < nq; ++q)
                 if (cs[q].done) hipLaunchKernelGGL(k_cg_done, dim3(1), dim3(1), 0, c->stream, c->mg_scalars, q);
     }
-    if (krylov && it > 0) {
-        hipLaunchKernelGGL(k_fcg_finish, g, blk, 0, c->stream, p, c->kp, L0, c->mg_scalars, DIM);
-        FCFD_CHECK(hipGetLastError());
-    }
     int rc = FCFD_OK, kmax = 0;
     for (const Case_ &a : cs) {
         kmax = std::max(kmax, a.k);
         if (a.r > rel_tol && !(a.r <= 10.0 * rel_tol && a.since >= 4)) rc = FCFD_ERR_NOT_CONVERGED;
+    }
+    // (no iteration ran when every case had converged at the start)
+    if (krylov && kmax > 0) {
+        hipLaunchKernelGGL(k_fcg_finish, g, blk, 0, c->stream, p, c->kp, L0, c->mg_scalars, DIM);
+        FCFD_CHECK(hipGetLastError());
     }
     const double r = rmax();
     // mean(p) = 0 per case as the reference's iteration enforces, then ghost copy
