@@ -742,12 +742,13 @@ __global__ void __launch_bounds__(256) k_sum_stage1(const double *__restrict__ p
 // update it receives, as at the end of a single solve.
 // The stopping state of each case's solve follows (k_cg_start, k_cg_check):
 // ||f||, the relative residual, its best value, iterations, iterations
-// since the best, a non-finite flag (the host reads SC_DONE .. SC_NAN), and
-// the solve's tolerance and iteration limit.
+// since the best, a non-finite flag, the starting relative residual (the
+// host reads SC_DONE .. SC_R0), and the solve's tolerance and iteration
+// limit.
 enum {
     SC_RZ = 0, SC_RZO = 1, SC_PQ = 2, SC_RR = 3, SC_MEAN = 4, SC_RZ_OLD = 5, SC_BETA = 6, SC_ALPHA = 7,
     SC_FIRST = 8, SC_BREAK = 9, SC_DONE = 10, SC_FN = 11, SC_R = 12, SC_BEST = 13, SC_K = 14, SC_SINCE = 15,
-    SC_NAN = 16, SC_NST = SC_NAN - SC_DONE + 1, SC_TOL = 17, SC_MAXC = 18
+    SC_NAN = 16, SC_R0 = 17, SC_NST = SC_R0 - SC_DONE + 1, SC_TOL = 18, SC_MAXC = 19
 };
 constexpr int SCS = 64;
 enum { SUM_STORE = 0, SUM_CG_BETA = 1, SUM_CG_ALPHA = 2 };
@@ -821,6 +822,7 @@ __global__ void __launch_bounds__(CG_THREADS) k_cg_start(double *sc, int nq, dou
         s[SC_MAXC] = (double)maxc;
         s[SC_FN] = fn;
         s[SC_R] = r;
+        s[SC_R0] = r;
         s[SC_BEST] = r;
         s[SC_K] = 0.0;
         s[SC_SINCE] = 0.0;
@@ -2362,12 +2364,15 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
             FCFD_CHECK(hipEventRecord(c->cg_ev[slot], c->stream));
             return FCFD_OK;
         };
-        // the residual before the first status read is unknown here: 0 makes
-        // the first prediction queue ahead
+        // the starting residual arrives with the first status (SC_R0)
         std::vector<double> rprev(nq, 0.0);
+        // With the region profiler on, every launch is timed: an iteration
+        // queued ahead and then skipped would enter the region averages as a
+        // near-empty launch, so the profiled solve does not queue ahead.
+        const bool pipelined = !c->prof.on;
         FCFD_TRY(launch(0));
         int queued = 1;
-        bool ahead = true;
+        bool ahead = pipelined;
         for (;;) {
             if (ahead && queued == it + 1 && queued < maxc) {
                 FCFD_TRY(launch(queued & 1));
@@ -2389,9 +2394,10 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
                 a.r = hq[SC_R - SC_DONE];
                 a.since = (int)hq[SC_SINCE - SC_DONE];
                 nan = nan || hq[SC_NAN - SC_DONE] != 0.0;
+                if (it == 1) rprev[q] = hq[SC_R0 - SC_DONE];
                 active += !a.done;
                 // residual after the next iteration at the last rate
-                if (!a.done && !(a.r * (a.r / rprev[q]) <= rel_tol)) ahead = true;
+                if (!a.done && !(a.r * (a.r / rprev[q]) <= rel_tol)) ahead = pipelined;
                 rprev[q] = a.r;
             }
             if (verbose) fprintf(stderr, "fcfd mg[%s+fcg]: it %d rel %.3e\n", mixed ? "mixed" : "fp64", it, rmax());
