@@ -2272,6 +2272,9 @@ static int fcg_iteration(fcfd_ctx *c, int kind, int inner, int slot) {
 // device (SC_DONE; the mixed defect correction skips its update), so its
 // result is that of its own solve.  FP64 plain V-cycles cannot freeze a case
 // and go on improving finished ones until the last case stops.
+// the host queues the next flexible-CG iteration ahead while the predicted
+// residual after the iteration in flight exceeds AHEAD_MARGIN x tolerance
+constexpr double AHEAD_MARGIN = 3.0;
 template <int DIM>
 static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel) {
     const Geom &G = c->G;
@@ -2396,8 +2399,10 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
                 nan = nan || hq[SC_NAN - SC_DONE] != 0.0;
                 if (it == 1) rprev[q] = hq[SC_R0 - SC_DONE];
                 active += !a.done;
-                // residual after the next iteration at the last rate
-                if (!a.done && !(a.r * (a.r / rprev[q]) <= rel_tol)) ahead = pipelined;
+                // residual after the next iteration at the last rate, with a
+                // margin: an iteration queued in vain costs ~0.4 ms of
+                // launches at 256^3, one not queued in time a host round trip
+                if (!a.done && !(a.r * (a.r / rprev[q]) <= AHEAD_MARGIN * rel_tol)) ahead = pipelined;
                 rprev[q] = a.r;
             }
             if (verbose) fprintf(stderr, "fcfd mg[%s+fcg]: it %d rel %.3e\n", mixed ? "mixed" : "fp64", it, rmax());
