@@ -786,6 +786,56 @@ __global__ void __launch_bounds__(256) k_sum_stage2(const double *__restrict__ p
     }
 }
 
+// The same sums in one launch: a block of SUM1_T threads per case q sums
+// each slot's nbq partials (four independent chains per thread, so their
+// loads overlap; then the waves in order), then the scalar step op as in
+// k_sum_stage2.  Two dependent launches of ~4.4 us become one.
+constexpr int SUM1_T = 1024;
+__global__ void __launch_bounds__(SUM1_T) k_sum_one(const double *__restrict__ partial, long nbq, int ns,
+                                                   double *__restrict__ out, int op, double *__restrict__ sc,
+                                                   const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double sh[SUM1_T / WAVE];
+    const int q = blockIdx.x;   // case
+    const long ntot = nbq * gridDim.x;
+    out += (long)q * SCS;
+    sc += (long)q * SCS;
+    for (int s = 0; s < ns; ++s) {
+        const double *pp = partial + s * ntot + q * nbq;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        long b = threadIdx.x;
+        for (; b + 3 * SUM1_T < nbq; b += 4 * SUM1_T) {
+            a0 += pp[b];
+            a1 += pp[b + SUM1_T];
+            a2 += pp[b + 2 * SUM1_T];
+            a3 += pp[b + 3 * SUM1_T];
+        }
+        for (; b < nbq; b += SUM1_T) a0 += pp[b];
+        double a = (a0 + a1) + (a2 + a3);
+        for (int o = WAVE / 2; o > 0; o >>= 1) a += __shfl_xor(a, o, WAVE);
+        if ((threadIdx.x & (WAVE - 1)) == 0) sh[threadIdx.x / WAVE] = a;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < SUM1_T / WAVE; ++w) t += sh[w];
+            out[s] = t;
+        }
+        __syncthreads();
+    }
+    const bool done = sc[SC_DONE] != 0.0;
+    if (threadIdx.x == 0 && op == SUM_CG_BETA) {
+        const double rz = out[0], rzo = out[1];
+        sc[SC_BETA] = (sc[SC_FIRST] != 0.0 || done) ? 0.0 : (rz - rzo) / sc[SC_RZ_OLD];
+        sc[SC_RZ_OLD] = rz;
+        sc[SC_FIRST] = 0.0;
+    } else if (threadIdx.x == 0 && op == SUM_CG_ALPHA) {
+        const double pq = out[0], rz = sc[SC_RZ_OLD];
+        const bool ok = pq < 0.0 && rz < 0.0 && !done;
+        sc[SC_ALPHA] = ok ? rz / pq : 0.0;
+        if (!ok && !done) sc[SC_BREAK] = 1.0;
+    }
+}
+
 // start of a flexible-CG solve: first iteration, no breakdown (case blockIdx.x)
 __global__ void k_cg_init(double *sc) {
     sc += (long)blockIdx.x * SCS;
@@ -2138,12 +2188,24 @@ __global__ void __launch_bounds__(256) k_fcg_finish(double *__restrict__ x, cons
 // out[q SCS + s] = sums of the ns slots of the block partials of a level-0
 // launch of nb blocks, per case q of the context, then the scalar step `op`
 // (SUM_*) on each case's scalars
+// Up to SUM1_MAX partials per case one block reduces them in one launch;
+// beyond, the 256-block first stage is faster than one block's reads (at
+// 256^3 a single block over the 65536 partials of a plane-per-block launch
+// made the step 1 % slower, while the batched 2D stacks, 128-2048 partials
+// per case, ran 7 % faster in one launch).
+constexpr long SUM1_MAX = 16384;
 static int sum_partials(fcfd_ctx *c, long nb, int ns, double *out, int op = SUM_STORE) {
-    const long nbq = nb / c->nb, chunk = (nbq + 255) / 256;
-    hipLaunchKernelGGL(k_sum_stage1, dim3(256, ns, c->nb), dim3(256), 0, c->stream, c->red.partial, nbq, chunk,
-                       c->red.partial2, t_skip);
-    hipLaunchKernelGGL(k_sum_stage2, dim3(c->nb), dim3(256), 0, c->stream, c->red.partial2, ns, out, op,
-                       c->mg_scalars, t_skip);
+    const long nbq = nb / c->nb;
+    if (nbq <= SUM1_MAX) {
+        hipLaunchKernelGGL(k_sum_one, dim3(c->nb), dim3(SUM1_T), 0, c->stream, c->red.partial, nbq, ns, out, op,
+                           c->mg_scalars, t_skip);
+    } else {
+        const long chunk = (nbq + 255) / 256;
+        hipLaunchKernelGGL(k_sum_stage1, dim3(256, ns, c->nb), dim3(256), 0, c->stream, c->red.partial, nbq, chunk,
+                           c->red.partial2, t_skip);
+        hipLaunchKernelGGL(k_sum_stage2, dim3(c->nb), dim3(256), 0, c->stream, c->red.partial2, ns, out, op,
+                           c->mg_scalars, t_skip);
+    }
     FCFD_CHECK(hipGetLastError());
     return FCFD_OK;
 }
