@@ -1287,8 +1287,16 @@ __global__ void __launch_bounds__(CDENSE) k_coarse_dense(T *__restrict__ eout, c
 // in the registers of the thread that owns the cell; a thread owns whole
 // coarse cells with all their children, so the restriction needs no
 // exchange.  FP64 would need twice the registers (spills) and keeps TAIL_MAX.
+//
+// The levels of at most TAIL_MAX cells (one cell per thread) keep their
+// error in LDS for the whole cycle (ghosted boxes of TAIL_ELDS elements in
+// all, ghosts 0): the owner thread holds the cell's six face coefficients
+// and rhs in registers, loaded from global memory once per down and once per
+// up phase, so a half-sweep waits on LDS instead of on dependent L2 loads.
+// Each level's error is written back at the end of its up phase.
 constexpr int TAIL_MAX = 1024, TAIL_THREADS = 1024, TAIL_LEVELS = 8;
 constexpr int TAIL_TOP = 8192, TAIL_LDS = 12288, TAIL_SLOTS = TAIL_TOP / TAIL_THREADS;
+constexpr int TAIL_ELDS = 4096;
 
 template <typename T>
 struct TailLevel {
@@ -1307,6 +1315,8 @@ struct TailArgs {
     int n;                 // levels in the tail; the last is solved densely
     int top;               // lv[0] is an LDS/register top level (FP32 only)
     const double *minv;    // dense inverse of the last level (k_coarse_inv)
+    int lds;               // levels from lv[top] on keep their error in LDS
+    int eoff[TAIL_LEVELS]; // offset of each such level's ghosted box
 };
 
 template <typename T, int DIM>
@@ -1349,6 +1359,51 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
         return s;
     };
     const int nl = A.n;
+    // ---- LDS-resident levels (A.lds): error in a ghosted box per level
+    __shared__ T el_[TAIL_ELDS];
+    __shared__ T rt_[TAIL_MAX];
+    auto lbox = [](const Lay &L, int i, int j, int k) {   // k: 0 in 2D
+        return (i + 1) + (L.nx + 2) * ((j + 1) + (DIM == 3 ? (L.ny + 2) * (k + 1) : 0));
+    };
+    auto boxn = [](const Lay &L) { return (L.nx + 2) * (L.ny + 2) * (DIM == 3 ? L.nz + 2 : 1); };
+    // the owner's face coefficients (E W N S T B, as stencil() forms them) and rhs
+    auto load_regs = [&](const TailLevel<T> &V, T (&cf)[6], T &fv) {
+        const Lay &L = V.L;
+        if (t < ncells(L)) {
+            int i, j, k;
+            cell_coords(L, t, i, j, k);
+            const long id = L.id(i, j, k);
+            cf[0] = (T)V.bx[id] * (T)V.ih.x;
+            cf[1] = (T)V.bx[id - 1] * (T)V.ih.x;
+            cf[2] = (T)V.by[id] * (T)V.ih.y;
+            cf[3] = (T)V.by[id - L.sx] * (T)V.ih.y;
+            cf[4] = DIM == 3 ? (T)V.bz[id] * (T)V.ih.z : (T)0;
+            cf[5] = DIM == 3 ? (T)V.bz[id - L.sxy] * (T)V.ih.z : (T)0;
+            fv = V.f[id];
+        }
+    };
+    // one colour of Gauss-Seidel on an LDS-resident level, stencil()'s order
+    auto gs_lds = [&](const TailLevel<T> &V, T *el, const T (&cf)[6], T fv, int color) {
+        const Lay &L = V.L;
+        if (t < ncells(L)) {
+            int i, j, k;
+            cell_coords(L, t, i, j, k);
+            const int kl = DIM == 3 ? k : 0;
+            if (((i + j + kl) & 1) == color) {
+                const int li = lbox(L, i, j, kl), bsx = L.nx + 2, bsxy = bsx * (L.ny + 2);
+                const T ep = el[li];
+                const T vE = el[li + 1], vW = el[li - 1], vN = el[li + bsx], vS = el[li - bsx];
+                T sc = cf[0] + cf[1] + cf[2] + cf[3];
+                T sce = cf[0] * (vE - ep) + cf[1] * (vW - ep) + cf[2] * (vN - ep) + cf[3] * (vS - ep);
+                if (DIM == 3) {
+                    sc += cf[4] + cf[5];
+                    sce += cf[4] * (el[li + bsxy] - ep) + cf[5] * (el[li - bsxy] - ep);
+                }
+                el[li] = sc > (T)0 ? ep + gs_step<T>(sce - fv, sc) : ep;
+            }
+        }
+        __syncthreads();
+    };
     // ---- top level in LDS + registers (FP32 only; A.top is never set for FP64).
     // Fully coarsened (2x2x2, or 2x2 in 2D): thread t owns coarse cell(s)
     // cc = t + TAIL_THREADS * g and slot sl = g * NCH + ch holds child ch =
@@ -1460,7 +1515,58 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
         }
     }
     // down: smooth, restrict the residual, clear the coarse error
-    for (int a = a0; a < nl - 1; ++a) {
+    for (int a = a0; a < nl - 1 && A.lds; ++a) {
+        const TailLevel<T> &F = A.lv[a];
+        const TailLevel<T> &C = A.lv[a + 1];
+        const Lay &L = F.L;
+        T *el = el_ + A.eoff[a];
+        T cf[6], fv = (T)0;
+        for (int q = t; q < boxn(L); q += TAIL_THREADS) el[q] = (T)0;
+        __syncthreads();
+        load_regs(F, cf, fv);
+        const int n = ncells(L);
+        if (a == a0 && t < n) {   // the first level's error as it was left (later ones start at 0)
+            int i, j, k;
+            cell_coords(L, t, i, j, k);
+            el[lbox(L, i, j, DIM == 3 ? k : 0)] = F.e[L.id(i, j, k)];
+        }
+        __syncthreads();
+        for (int s = 0; s < F.nu1; ++s) { gs_lds(F, el, cf, fv, 0); gs_lds(F, el, cf, fv, 1); }
+        // residual of each cell by its owner in apply_A's order, then the
+        // coarse sums over the children in k_resid_restrict's order
+        if (t < n) {
+            int i, j, k;
+            cell_coords(L, t, i, j, k);
+            const int li = lbox(L, i, j, DIM == 3 ? k : 0), bsx = L.nx + 2, bsxy = bsx * (L.ny + 2);
+            const T ep = el[li];
+            T acc = cf[0] * (el[li + 1] - ep) + cf[1] * (el[li - 1] - ep) + cf[2] * (el[li + bsx] - ep) +
+                    cf[3] * (el[li - bsx] - ep);
+            if (DIM == 3) acc += cf[4] * (el[li + bsxy] - ep) + cf[5] * (el[li - bsxy] - ep);
+            rt_[t] = fv - acc;
+        }
+        __syncthreads();
+        const int nc = ncells(C.L);
+        if (t < nc) {
+            int I, J, K;
+            cell_coords(C.L, t, I, J, K);
+            const int KL = DIM == 3 ? K : 0;
+            T acc = 0;
+            int nch = 0;
+            for (int kk = 0; kk < F.cz; ++kk)
+                for (int jj = 0; jj < F.cy; ++jj)
+                    for (int ii = 0; ii < F.cx; ++ii) {
+                        const int i = I * F.cx + ii, j = J * F.cy + jj, k = KL * F.cz + kk;
+                        if (i >= L.nx || j >= L.ny || (DIM == 3 && k >= L.nz)) continue;
+                        acc += rt_[i + L.nx * (j + L.ny * k)];
+                        ++nch;
+                    }
+            const long cid = C.L.id(I, J, K);
+            C.f[cid] = acc / (T)nch;
+            C.e[cid] = (T)0;
+        }
+        __syncthreads();
+    }
+    for (int a = a0; a < nl - 1 && !A.lds; ++a) {
         const TailLevel<T> &F = A.lv[a];
         const TailLevel<T> &C = A.lv[a + 1];
         for (int s = 0; s < F.nu1; ++s) { gs(F, 0); gs(F, 1); }
@@ -1505,10 +1611,40 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
             for (int cc = 0; cc < n; ++cc) x += minv[cc * n + t] * bvec[cc];
         const double xm = block_sum(t < n ? x : 0.0) / (double)n;
         if (t < n) V.e[V.L.id(i, j, k)] = (T)(x - xm);
+        if (A.lds) {
+            T *el = el_ + A.eoff[nl - 1];
+            for (int q = t; q < boxn(V.L); q += TAIL_THREADS) el[q] = (T)0;
+            __syncthreads();
+            if (t < n) el[lbox(V.L, i, j, DIM == 3 ? k : 0)] = (T)(x - xm);
+        }
         __syncthreads();
     }
-    // up: prolong (piecewise constant), smooth
-    for (int a = nl - 2; a >= a0; --a) {
+    // up: prolong (piecewise constant), smooth; LDS levels write their error back
+    for (int a = nl - 2; a >= a0 && A.lds; --a) {
+        const TailLevel<T> &F = A.lv[a];
+        const TailLevel<T> &C = A.lv[a + 1];
+        const Lay &L = F.L;
+        T *el = el_ + A.eoff[a];
+        const T *elc = el_ + A.eoff[a + 1];
+        T cf[6], fv = (T)0;
+        load_regs(F, cf, fv);
+        const int n = ncells(L);
+        if (t < n) {
+            int i, j, k;
+            cell_coords(L, t, i, j, k);
+            const int kl = DIM == 3 ? k : 0;
+            el[lbox(L, i, j, kl)] += elc[lbox(C.L, i / F.cx, j / F.cy, kl / F.cz)];
+        }
+        __syncthreads();
+        for (int s = 0; s < F.nu2; ++s) { gs_lds(F, el, cf, fv, 1); gs_lds(F, el, cf, fv, 0); }
+        if (t < n) {
+            int i, j, k;
+            cell_coords(L, t, i, j, k);
+            F.e[L.id(i, j, k)] = el[lbox(L, i, j, DIM == 3 ? k : 0)];
+        }
+        __syncthreads();
+    }
+    for (int a = nl - 2; a >= a0 && !A.lds; --a) {
         const TailLevel<T> &F = A.lv[a];
         const TailLevel<T> &C = A.lv[a + 1];
         const int n = ncells(F.L);
@@ -1996,6 +2132,19 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
             V.cx = M.cx; V.cy = M.cy; V.cz = DIM == 3 ? M.cz : 1;
             V.nu1 = sweeps(lv, nu1);
             V.nu2 = sweeps(lv, nu2);
+        }
+        // the levels below the top keep their error in LDS when their
+        // ghosted boxes fit
+        {
+            int off = 0;
+            bool fit = true;
+            for (int q = A.top; q < A.n; ++q) {
+                const Lay &L = A.lv[q].L;
+                A.eoff[q] = off;
+                off += (L.nx + 2) * (L.ny + 2) * (DIM == 3 ? L.nz + 2 : 1);
+                fit = fit && (long)L.nx * L.ny * (DIM == 3 ? L.nz : 1) <= TAIL_MAX;
+            }
+            A.lds = fit && off <= TAIL_ELDS ? 1 : 0;
         }
         ProfScope ps_(c, R_COARSE);
         hipLaunchKernelGGL((k_mg_tail<T, DIM>), dim3(DIM == 2 ? c->nb : 1), dim3(TAIL_THREADS), 0, c->stream, A);
