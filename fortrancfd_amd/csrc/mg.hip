@@ -1288,13 +1288,13 @@ __global__ void __launch_bounds__(CDENSE) k_coarse_dense(T *__restrict__ eout, c
 // coarse cells with all their children, so the restriction needs no
 // exchange.  FP64 would need twice the registers (spills) and keeps TAIL_MAX.
 //
-// The levels of at most TAIL_MAX cells (one cell per thread) keep their
+// The levels of at most TAIL_MAX cells (TAIL_CPT cells per thread) keep their
 // error in LDS for the whole cycle (ghosted boxes of TAIL_ELDS elements in
 // all, ghosts 0): the owner thread holds the cell's six face coefficients
 // and rhs in registers, loaded from global memory once per down and once per
 // up phase, so a half-sweep waits on LDS instead of on dependent L2 loads.
 // Each level's error is written back at the end of its up phase.
-constexpr int TAIL_MAX = 1024, TAIL_THREADS = 1024, TAIL_LEVELS = 8;
+constexpr int TAIL_MAX = 2048, TAIL_THREADS = 1024, TAIL_LEVELS = 8, TAIL_CPT = TAIL_MAX / TAIL_THREADS;
 constexpr int TAIL_TOP = 8192, TAIL_LDS = 12288, TAIL_SLOTS = TAIL_TOP / TAIL_THREADS;
 constexpr int TAIL_ELDS = 4096;
 
@@ -1366,41 +1366,49 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
         return (i + 1) + (L.nx + 2) * ((j + 1) + (DIM == 3 ? (L.ny + 2) * (k + 1) : 0));
     };
     auto boxn = [](const Lay &L) { return (L.nx + 2) * (L.ny + 2) * (DIM == 3 ? L.nz + 2 : 1); };
-    // the owner's face coefficients (E W N S T B, as stencil() forms them) and rhs
-    auto load_regs = [&](const TailLevel<T> &V, T (&cf)[6], T &fv) {
+    // the owner's face coefficients (E W N S T B, as stencil() forms them) and
+    // rhs of its cells c = t + u TAIL_THREADS
+    auto load_regs = [&](const TailLevel<T> &V, T (&cf)[TAIL_CPT][6], T (&fv)[TAIL_CPT]) {
         const Lay &L = V.L;
-        if (t < ncells(L)) {
-            int i, j, k;
-            cell_coords(L, t, i, j, k);
-            const long id = L.id(i, j, k);
-            cf[0] = (T)V.bx[id] * (T)V.ih.x;
-            cf[1] = (T)V.bx[id - 1] * (T)V.ih.x;
-            cf[2] = (T)V.by[id] * (T)V.ih.y;
-            cf[3] = (T)V.by[id - L.sx] * (T)V.ih.y;
-            cf[4] = DIM == 3 ? (T)V.bz[id] * (T)V.ih.z : (T)0;
-            cf[5] = DIM == 3 ? (T)V.bz[id - L.sxy] * (T)V.ih.z : (T)0;
-            fv = V.f[id];
+#pragma unroll
+        for (int u = 0; u < TAIL_CPT; ++u) {
+            const int c = t + u * TAIL_THREADS;
+            if (c < ncells(L)) {
+                int i, j, k;
+                cell_coords(L, c, i, j, k);
+                const long id = L.id(i, j, k);
+                cf[u][0] = (T)V.bx[id] * (T)V.ih.x;
+                cf[u][1] = (T)V.bx[id - 1] * (T)V.ih.x;
+                cf[u][2] = (T)V.by[id] * (T)V.ih.y;
+                cf[u][3] = (T)V.by[id - L.sx] * (T)V.ih.y;
+                cf[u][4] = DIM == 3 ? (T)V.bz[id] * (T)V.ih.z : (T)0;
+                cf[u][5] = DIM == 3 ? (T)V.bz[id - L.sxy] * (T)V.ih.z : (T)0;
+                fv[u] = V.f[id];
+            }
         }
     };
     // one colour of Gauss-Seidel on an LDS-resident level, stencil()'s order
-    auto gs_lds = [&](const TailLevel<T> &V, T *el, const T (&cf)[6], T fv, int color) {
+    auto gs_lds = [&](const TailLevel<T> &V, T *el, const T (&cf)[TAIL_CPT][6], const T (&fv)[TAIL_CPT],
+                      int color) {
         const Lay &L = V.L;
-        if (t < ncells(L)) {
+#pragma unroll
+        for (int u = 0; u < TAIL_CPT; ++u) {
+            const int c = t + u * TAIL_THREADS;
+            if (c >= ncells(L)) continue;
             int i, j, k;
-            cell_coords(L, t, i, j, k);
+            cell_coords(L, c, i, j, k);
             const int kl = DIM == 3 ? k : 0;
-            if (((i + j + kl) & 1) == color) {
-                const int li = lbox(L, i, j, kl), bsx = L.nx + 2, bsxy = bsx * (L.ny + 2);
-                const T ep = el[li];
-                const T vE = el[li + 1], vW = el[li - 1], vN = el[li + bsx], vS = el[li - bsx];
-                T sc = cf[0] + cf[1] + cf[2] + cf[3];
-                T sce = cf[0] * (vE - ep) + cf[1] * (vW - ep) + cf[2] * (vN - ep) + cf[3] * (vS - ep);
-                if (DIM == 3) {
-                    sc += cf[4] + cf[5];
-                    sce += cf[4] * (el[li + bsxy] - ep) + cf[5] * (el[li - bsxy] - ep);
-                }
-                el[li] = sc > (T)0 ? ep + gs_step<T>(sce - fv, sc) : ep;
+            if (((i + j + kl) & 1) != color) continue;
+            const int li = lbox(L, i, j, kl), bsx = L.nx + 2, bsxy = bsx * (L.ny + 2);
+            const T ep = el[li];
+            const T vE = el[li + 1], vW = el[li - 1], vN = el[li + bsx], vS = el[li - bsx];
+            T sc = cf[u][0] + cf[u][1] + cf[u][2] + cf[u][3];
+            T sce = cf[u][0] * (vE - ep) + cf[u][1] * (vW - ep) + cf[u][2] * (vN - ep) + cf[u][3] * (vS - ep);
+            if (DIM == 3) {
+                sc += cf[u][4] + cf[u][5];
+                sce += cf[u][4] * (el[li + bsxy] - ep) + cf[u][5] * (el[li - bsxy] - ep);
             }
+            el[li] = sc > (T)0 ? ep + gs_step<T>(sce - fv[u], sc) : ep;
         }
         __syncthreads();
     };
@@ -1520,35 +1528,39 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
         const TailLevel<T> &C = A.lv[a + 1];
         const Lay &L = F.L;
         T *el = el_ + A.eoff[a];
-        T cf[6], fv = (T)0;
+        T cf[TAIL_CPT][6], fv[TAIL_CPT];
         for (int q = t; q < boxn(L); q += TAIL_THREADS) el[q] = (T)0;
         __syncthreads();
         load_regs(F, cf, fv);
         const int n = ncells(L);
-        if (a == a0 && t < n) {   // the first level's error as it was left (later ones start at 0)
-            int i, j, k;
-            cell_coords(L, t, i, j, k);
-            el[lbox(L, i, j, DIM == 3 ? k : 0)] = F.e[L.id(i, j, k)];
-        }
+        if (a == a0)   // the first level's error as it was left (later ones start at 0)
+            for (int c = t; c < n; c += TAIL_THREADS) {
+                int i, j, k;
+                cell_coords(L, c, i, j, k);
+                el[lbox(L, i, j, DIM == 3 ? k : 0)] = F.e[L.id(i, j, k)];
+            }
         __syncthreads();
         for (int s = 0; s < F.nu1; ++s) { gs_lds(F, el, cf, fv, 0); gs_lds(F, el, cf, fv, 1); }
         // residual of each cell by its owner in apply_A's order, then the
         // coarse sums over the children in k_resid_restrict's order
-        if (t < n) {
+#pragma unroll
+        for (int u = 0; u < TAIL_CPT; ++u) {
+            const int c = t + u * TAIL_THREADS;
+            if (c >= n) continue;
             int i, j, k;
-            cell_coords(L, t, i, j, k);
+            cell_coords(L, c, i, j, k);
             const int li = lbox(L, i, j, DIM == 3 ? k : 0), bsx = L.nx + 2, bsxy = bsx * (L.ny + 2);
             const T ep = el[li];
-            T acc = cf[0] * (el[li + 1] - ep) + cf[1] * (el[li - 1] - ep) + cf[2] * (el[li + bsx] - ep) +
-                    cf[3] * (el[li - bsx] - ep);
-            if (DIM == 3) acc += cf[4] * (el[li + bsxy] - ep) + cf[5] * (el[li - bsxy] - ep);
-            rt_[t] = fv - acc;
+            T acc = cf[u][0] * (el[li + 1] - ep) + cf[u][1] * (el[li - 1] - ep) + cf[u][2] * (el[li + bsx] - ep) +
+                    cf[u][3] * (el[li - bsx] - ep);
+            if (DIM == 3) acc += cf[u][4] * (el[li + bsxy] - ep) + cf[u][5] * (el[li - bsxy] - ep);
+            rt_[c] = fv[u] - acc;
         }
         __syncthreads();
         const int nc = ncells(C.L);
-        if (t < nc) {
+        for (int cc = t; cc < nc; cc += TAIL_THREADS) {
             int I, J, K;
-            cell_coords(C.L, t, I, J, K);
+            cell_coords(C.L, cc, I, J, K);
             const int KL = DIM == 3 ? K : 0;
             T acc = 0;
             int nch = 0;
@@ -1626,20 +1638,20 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(TailArgs<T> A) {
         const Lay &L = F.L;
         T *el = el_ + A.eoff[a];
         const T *elc = el_ + A.eoff[a + 1];
-        T cf[6], fv = (T)0;
+        T cf[TAIL_CPT][6], fv[TAIL_CPT];
         load_regs(F, cf, fv);
         const int n = ncells(L);
-        if (t < n) {
+        for (int c = t; c < n; c += TAIL_THREADS) {
             int i, j, k;
-            cell_coords(L, t, i, j, k);
+            cell_coords(L, c, i, j, k);
             const int kl = DIM == 3 ? k : 0;
             el[lbox(L, i, j, kl)] += elc[lbox(C.L, i / F.cx, j / F.cy, kl / F.cz)];
         }
         __syncthreads();
         for (int s = 0; s < F.nu2; ++s) { gs_lds(F, el, cf, fv, 1); gs_lds(F, el, cf, fv, 0); }
-        if (t < n) {
+        for (int c = t; c < n; c += TAIL_THREADS) {
             int i, j, k;
-            cell_coords(L, t, i, j, k);
+            cell_coords(L, c, i, j, k);
             F.e[L.id(i, j, k)] = el[lbox(L, i, j, DIM == 3 ? k : 0)];
         }
         __syncthreads();
