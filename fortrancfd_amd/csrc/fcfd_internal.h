@@ -358,7 +358,7 @@ int mg_setup(fcfd_ctx *c);
 int mg_free(fcfd_ctx *c);
 int mg_drop_graphs(fcfd_ctx *c);
 int prof_graph_accum(fcfd_ctx *c, int kind, int inner);
-int ppe_solve(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel);
+int ppe_solve(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel, int normalize = 1);
 int copy_field(fcfd_ctx *c, double *dst, const double *src);
 
 }  // namespace fcfd

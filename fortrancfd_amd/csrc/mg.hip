@@ -2208,12 +2208,15 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
 // FP64 residual r = f - A p of level 0 with ||r||^2 and (with_f) ||f||^2
 // folded into block partials; optional outputs: r64 (Krylov residual), r32
 // (rhs of the FP32 cycle).  The cycles' error arrays need no clearing: the
-// first pre-smoothing half-sweeps take a zero initial guess.
+// first pre-smoothing half-sweeps take a zero initial guess.  With mean
+// (per case, times mscale) the rhs is f - mean, formed here as k_center_copy
+// forms it, so the flexible CG needs no centred copy of the source.
 template <int DIM>
 __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p, const double *__restrict__ f,
                                                    const double *__restrict__ rho, Lay L, Inv2 ih,
                                                    double *__restrict__ r64, float *__restrict__ r32, Split S,
-                                                   double *__restrict__ partial, int with_f) {
+                                                   double *__restrict__ partial, int with_f,
+                                                   const double *__restrict__ mean, double mscale) {
     __shared__ double sh[4];
     int bx_, by_, bz_;
     // 3D: each XCD walks the planes of 8-row slabs, so the k-1 and k+1 planes
@@ -2225,9 +2228,10 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
     double rr = 0.0, ff = 0.0;
     if (i < L.nx && j < L.ny) {
         long id = L.id(i, j, k);
-        double r = f[id] - apply_A<double, DIM, true, double>(p, rho, nullptr, nullptr, nullptr, L, ih, i, j, k, id);
+        const double fv = mean ? f[id] - mean[(DIM == 2 ? (long)k : 0L) * SCS] * mscale : f[id];
+        double r = fv - apply_A<double, DIM, true, double>(p, rho, nullptr, nullptr, nullptr, L, ih, i, j, k, id);
         rr = r * r;
-        if (with_f) ff = f[id] * f[id];
+        if (with_f) ff = fv * fv;
         if (r64) r64[id] = r;
         if (r32) r32[S.at(i, j, k)] = (float)r;
     }
@@ -2499,7 +2503,7 @@ static int fcg_iteration(fcfd_ctx *c, int kind, int inner, int slot) {
 // residual after the iteration in flight exceeds AHEAD_MARGIN x tolerance
 constexpr double AHEAD_MARGIN = 3.0;
 template <int DIM>
-static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel) {
+static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel, int normalize) {
     const Geom &G = c->G;
     dim3 blk = mg_blk(G.nx), g = grd3(G.nx, G.ny, G.nz, blk);
     const long nb = (long)g.x * g.y * g.z;
@@ -2516,16 +2520,26 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     const int kind = mixed ? (krylov ? 3 : 2) : (krylov ? 1 : 0);
     const int maxc = c->pr.ppe_max_cycles > 0 ? c->pr.ppe_max_cycles : 200;
     static const bool verbose = getenv("FCFD_MG_VERBOSE") != nullptr;
-    // compatibility of the Neumann problem: f = src - mean(src), per case
+    // compatibility of the Neumann problem: f = src - mean(src), per case;
+    // the flexible CG reads f once (r0), so its residual kernel centres the
+    // source itself, while plain V-cycles read the centred copy every cycle
     FCFD_TRY(field_mean(c, c->f[FCFD_PPE_SRC]));
-    hipLaunchKernelGGL(k_center_copy, g, blk, 0, c->stream, c->f[FCFD_PPE_SRC], f, L0, c->mg_scalars + SC_MEAN,
-                       1.0 / ncell, DIM);
-    FCFD_CHECK(hipGetLastError());
+    if (!krylov) {
+        hipLaunchKernelGGL(k_center_copy, g, blk, 0, c->stream, c->f[FCFD_PPE_SRC], f, L0, c->mg_scalars + SC_MEAN,
+                           1.0 / ncell, DIM);
+        FCFD_CHECK(hipGetLastError());
+    }
     // r0 = f - A p
     {
         ProfScope ps_(c, R_RESID0);
-        hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, f, rho, L0, ih, krylov ? c->kr : nullptr,
-                           mixed ? M0.f32 : nullptr, S0, c->red.partial, 1);
+        if (krylov)
+            hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, (const double *)c->f[FCFD_PPE_SRC], rho,
+                               L0, ih, c->kr, mixed ? M0.f32 : nullptr, S0, c->red.partial, 1,
+                               (const double *)c->mg_scalars + SC_MEAN, 1.0 / ncell);
+        else
+            hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, (const double *)f, rho, L0, ih,
+                               (double *)nullptr, mixed ? M0.f32 : nullptr, S0, c->red.partial, 1,
+                               (const double *)nullptr, 0.0);
         FCFD_CHECK(hipGetLastError());
     }
     std::vector<double> s2(2 * nq);
@@ -2649,8 +2663,9 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
         }
         {
             ProfScope ps_(c, R_RESID0);
-            hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, f, rho, L0, ih, nullptr,
-                               mixed ? M0.f32 : nullptr, S0, c->red.partial, 0);
+            hipLaunchKernelGGL(k_resid_out<DIM>, g, blk, 0, c->stream, p, (const double *)f, rho, L0, ih,
+                               (double *)nullptr, mixed ? M0.f32 : nullptr, S0, c->red.partial, 0,
+                               (const double *)nullptr, 0.0);
             FCFD_CHECK(hipGetLastError());
         }
         FCFD_TRY(sums_to_host(c, nb, 1, s2.data()));
@@ -2685,10 +2700,14 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
         FCFD_CHECK(hipGetLastError());
     }
     const double r = rmax();
-    // mean(p) = 0 per case as the reference's iteration enforces, then ghost copy
-    FCFD_TRY(field_mean(c, p));
-    hipLaunchKernelGGL(k_sub_scalar, g, blk, 0, c->stream, p, L0, c->mg_scalars + SC_MEAN, 1.0 / ncell, DIM);
-    FCFD_CHECK(hipGetLastError());
+    // mean(p) = 0 per case as the reference's iteration enforces (skipped on
+    // request: a Picard solve whose p only enters gradients and the next
+    // solve's start, A p being blind to a constant), then ghost copy
+    if (normalize) {
+        FCFD_TRY(field_mean(c, p));
+        hipLaunchKernelGGL(k_sub_scalar, g, blk, 0, c->stream, p, L0, c->mg_scalars + SC_MEAN, 1.0 / ncell, DIM);
+        FCFD_CHECK(hipGetLastError());
+    }
     FCFD_TRY(bc_scalar(c, p));
     if (cycles) *cycles = kmax;
     if (rel) *rel = r;
@@ -2697,10 +2716,10 @@ static int ppe_solve_t(fcfd_ctx *c, double rel_tol, int precision, int *cycles, 
     return rc;
 }
 
-int ppe_solve(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel) {
+int ppe_solve(fcfd_ctx *c, double rel_tol, int precision, int *cycles, double *rel, int normalize) {
     if (!c->mg_ready) FCFD_TRY(mg_setup(c));
-    if (c->G.dim == 3) return ppe_solve_t<3>(c, rel_tol, precision, cycles, rel);
-    return ppe_solve_t<2>(c, rel_tol, precision, cycles, rel);
+    if (c->G.dim == 3) return ppe_solve_t<3>(c, rel_tol, precision, cycles, rel, normalize);
+    return ppe_solve_t<2>(c, rel_tol, precision, cycles, rel, normalize);
 }
 
 }  // namespace fcfd

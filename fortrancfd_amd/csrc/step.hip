@@ -160,7 +160,9 @@ extern "C" int fcfd_ns_solver(fcfd_ctx *c) {
         FCFD_TRY(ns_predictor(c, c->f[FCFD_U], c->f[FCFD_V], c->f[FCFD_W], it == np - 1));
         FCFD_TRY(ppe_src(c));
         int cyc = 0;
-        int r = ppe_solve(c, tol, c->pr.ppe_precision, &cyc, nullptr);
+        // mean(p) = 0 after the last Picard solve (the earlier ones' p enters
+        // only gradients and the next solve's start)
+        int r = ppe_solve(c, tol, c->pr.ppe_precision, &cyc, nullptr, it == np - 1);
         if (r != FCFD_OK && r != FCFD_ERR_NOT_CONVERGED) return r;
         if (r != FCFD_OK) rc = r;
         total += cyc;
