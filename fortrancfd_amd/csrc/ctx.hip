@@ -22,33 +22,27 @@ namespace fcfd {
 // cell; the ghost shell is enumerated as z slabs, then y slabs, then x slabs.
 template <int DIM, typename T>
 __global__ void k_bc_scalar(T *__restrict__ d, Geom G, long nzslab, long nyslab, long total) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const int g = G.g, X = G.nx + 2 * g, Y = G.ny + 2 * g;
+    // 32-bit index arithmetic: the host keeps the ghost count below 2^32
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)total) return;
+    const unsigned g = G.g, X = G.nx + 2 * g, Y = G.ny + 2 * g, nzs = (unsigned)nzslab, nys = (unsigned)nyslab;
     int i, j, k;
-    if (DIM == 3 && t < nzslab) {                    // k in ghost layers, all i, j
-        long per = (long)X * Y;
-        int layer = (int)(t / per);
-        long r = t % per;
-        j = (int)(r / X) - g;
-        i = (int)(r % X) - g;
-        k = layer < G.gz ? layer - G.gz : G.nz + (layer - G.gz);
-    } else if (t < nzslab + nyslab) {                // j in ghost layers, interior k, all i
-        long u = t - nzslab;
-        long per = (long)X * 2 * g;
-        k = (int)(u / per);
-        long r = u % per;
-        int layer = (int)(r / X);
-        i = (int)(r % X) - g;
-        j = layer < g ? layer - g : G.ny + (layer - g);
+    if (DIM == 3 && t < nzs) {                       // k in ghost layers, all i, j
+        const unsigned per = X * Y, layer = t / per, r = t - layer * per, jr = r / X;
+        j = (int)jr - (int)g;
+        i = (int)(r - jr * X) - (int)g;
+        k = (int)layer < G.gz ? (int)layer - G.gz : G.nz + ((int)layer - G.gz);
+    } else if (t < nzs + nys) {                      // j in ghost layers, interior k, all i
+        const unsigned u = t - nzs, per = X * 2 * g, kk = u / per, r = u - kk * per, layer = r / X;
+        k = (int)kk;
+        i = (int)(r - layer * X) - (int)g;
+        j = layer < g ? (int)layer - (int)g : G.ny + ((int)layer - (int)g);
     } else {                                         // i in ghost layers, interior j, k
-        long u = t - nzslab - nyslab;
-        long per = (long)G.ny * 2 * g;
-        k = (int)(u / per);
-        long r = u % per;
-        j = (int)(r / (2 * g));
-        int layer = (int)(r % (2 * g));
-        i = layer < g ? layer - g : G.nx + (layer - g);
+        const unsigned u = t - nzs - nys, per = (unsigned)G.ny * 2 * g, kk = u / per, r = u - kk * per;
+        const unsigned jr = r / (2 * g), layer = r - jr * (2 * g);
+        k = (int)kk;
+        j = (int)jr;
+        i = layer < g ? (int)layer - (int)g : G.nx + ((int)layer - (int)g);
     }
     // 2D: k is the case of a batched stack (0 otherwise), which has no z ghosts
     int ci = clampi(i, 0, G.nx - 1), cj = clampi(j, 0, G.ny - 1), ck = DIM == 3 ? clampi(k, 0, G.nz - 1) : k;
@@ -66,6 +60,7 @@ static int bc_scalar_t(fcfd_ctx *c, T *d) {
     long total = nzslab + nyslab + nxslab;
     int blk = 256;
     long nb = (total + blk - 1) / blk;
+    if (nb * blk >= 4294967296L) return FCFD_ERR_ARG;   // the kernel numbers ghosts in 32 bits
     ProfScope ps_(c, R_BC);
     if (G.dim == 3)
         hipLaunchKernelGGL((k_bc_scalar<3, T>), dim3((unsigned)nb), dim3(blk), 0, c->stream, d, G, nzslab, nyslab, total);
@@ -111,15 +106,25 @@ __global__ void k_velbc(double *__restrict__ u, double *__restrict__ v, double *
     if (a >= na || bb >= nb) return;
     double *comp[3] = {u, v, w};
     const int n = AX == 0 ? G.nx : (AX == 1 ? G.ny : G.nz);
+    // per side: every source value first, then the ghost stores, then the
+    // wall zero of the normal component (the ghosts alias the arrays they
+    // are copied from, so in load-store order each load would wait for the
+    // previous store); the layers ii of a lane are at most 3 (g = 3)
+    constexpr int NL = AX == 0 ? 1 : 3;
     for (int side = 0; side < 2; ++side) {
-        int bt = side == 0 ? b.lo : b.hi;
+        const int bt = side == 0 ? b.lo : b.hi;
         if (bt != 1 && bt != 2) continue;
-        double sg = bt == 1 ? -1.0 : 1.0;
+        const double sg = bt == 1 ? -1.0 : 1.0;
+        double val[DIM][NL];
+        long dsti[DIM][NL];
+#pragma unroll
         for (int cpt = 0; cpt < DIM; ++cpt) {
-            double *f = comp[cpt];
-            bool normal = (cpt == AX) && !b.nodal;   // face-normal component
-            for (int ii = AX == 0 ? lay : 1; ii <= (AX == 0 ? lay : g); ++ii) {
-                int dst = side == 0 ? -ii : n - 1 + ii;
+            const double *f = comp[cpt];
+            const bool normal = (cpt == AX) && !b.nodal;   // face-normal component
+#pragma unroll
+            for (int q = 0; q < NL; ++q) {
+                const int ii = AX == 0 ? lay : min(1 + q, g);   // (layers past g repeat the last one)
+                const int dst = side == 0 ? -ii : n - 1 + ii;
                 int src;
                 if (side == 0) src = normal ? ii - 2 : ii - 1;
                 else src = normal ? n - 1 - ii : n - ii;
@@ -127,13 +132,22 @@ __global__ void k_velbc(double *__restrict__ u, double *__restrict__ v, double *
                 if (AX == 0) { ld = G.id(dst, a - g, bb - G.gz); ls = G.id(src, a - g, bb - G.gz); }
                 else if (AX == 1) { ld = G.id(a - g, dst, bb - G.gz); ls = G.id(a - g, src, bb - G.gz); }
                 else { ld = G.id(a - g, bb - g, dst); ls = G.id(a - g, bb - g, src); }
-                f[ld] = sg * f[ls];
+                val[cpt][q] = f[ls];
+                dsti[cpt][q] = ld;
             }
+        }
+#pragma unroll
+        for (int cpt = 0; cpt < DIM; ++cpt)
+#pragma unroll
+            for (int q = 0; q < NL; ++q) comp[cpt][dsti[cpt][q]] = sg * val[cpt][q];
+#pragma unroll
+        for (int cpt = 0; cpt < DIM; ++cpt) {
+            const bool normal = (cpt == AX) && !b.nodal;
             if (normal && (AX != 0 || lay == 1)) {
-                int wall = side == 0 ? -1 : n - 1;
-                long lw = AX == 0 ? G.id(wall, a - g, bb - G.gz)
-                                  : (AX == 1 ? G.id(a - g, wall, bb - G.gz) : G.id(a - g, bb - g, wall));
-                f[lw] = 0.0;
+                const int wall = side == 0 ? -1 : n - 1;
+                const long lw = AX == 0 ? G.id(wall, a - g, bb - G.gz)
+                                        : (AX == 1 ? G.id(a - g, wall, bb - G.gz) : G.id(a - g, bb - g, wall));
+                comp[cpt][lw] = 0.0;
             }
         }
     }
