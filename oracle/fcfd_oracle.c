@@ -17,7 +17,10 @@
  * single-threaded run agrees with the compiled reference to round-off.
  * Reference x**2.0d0 is written x*x.
  *
- * Parity pinning of each function is listed in DESIGN.md (oracle section).
+ * Each function is pinned against the compiled reference bit for bit, single
+ * threaded (tests/test_oracle_vs_reference.py, tests/golden/ref_*.npz); the
+ * exact pressure solve against SciPy's sparse LU and the reference relaxation's
+ * fixed point (tests/test_oracle_ppe.py, tests/test_oracle_golden.py).
  */
 #include "fcfd_oracle.h"
 #include "band.h"

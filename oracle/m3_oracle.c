@@ -23,7 +23,9 @@
  *     here they run J = 1..NODE_Y.  Parity runs use NODE_Y = NODE_Z.
  *   - the pressure solve can be the reference's SOR (PRESS_POISSON) or the
  *     converged solve of the same operator (band Cholesky, band.h).
- * Parity pinning of each function is listed in DESIGN.md (oracle section).
+ * Each function is pinned against the compiled reference bit for bit
+ * (tests/test_m3_oracle.py, tests/golden/m3_ref.npz); the converged pressure
+ * solve as the fixed point of the reference's SOR.
  */
 #include <math.h>
 #include <stdlib.h>
