@@ -43,7 +43,8 @@ for d in 2D 3D; do
     echo "build_ref: built $out/$exe"
 done
 # the drop-in: the newOpenMP 2D main loop through the fcfd_c binding module
-# (the block printed in INTEGRATION.md) against libfcfd.so
+# (the fortran block that starts `module fcfd_c`, extracted below from the
+# repository's integration notes) against libfcfd.so
 lib="$here/../fortrancfd_amd/libfcfd.so"
 if [ -f "$lib" ]; then
     b="$out/build_2D"

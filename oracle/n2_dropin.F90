@@ -1,9 +1,10 @@
 ! n2_dropin.F90 — the newOpenMP 2D main loop with its hot path in libfcfd.
 !
-! TEST INFRASTRUCTURE: the drop-in of the integration notes, end to end.  It
-! is built against the reference's own data modules (`Two phase flows/
-! newOpenMP/2D/lib`, compiled where they lie by build_ref.sh) and the fcfd_c
-! binding module printed in INTEGRATION.md, and linked with libfcfd.so.  The
+! TEST INFRASTRUCTURE: the Fortran drop-in, end to end.  It is built against
+! the reference's own data modules (`Two phase flows/newOpenMP/2D/lib`,
+! compiled where they lie by build_ref.sh) and the fcfd_c binding module
+! (the ISO_C_BINDING interfaces of include/fcfd.h), and linked with
+! libfcfd.so.  The
 ! loop is the reference's (`.../2D/src/main.f90:13-46`) with the switch and
 ! the hot-path subroutines answered by shims of the reference's names
 ! (interface_solver, ns_solver, ls_mv) that call the library; the fields stay

@@ -1,5 +1,6 @@
-"""The Fortran binding module printed in INTEGRATION.md compiles with AMD
-flang, links against libfcfd.so and agrees with the C ABI: derived-type sizes
+"""The Fortran binding module (fcfd_c, the ISO_C_BINDING interfaces of
+include/fcfd.h, kept as a fortran block of the repository's integration notes)
+compiles with AMD flang, links against libfcfd.so and agrees with the C ABI: derived-type sizes
 equal the C struct sizes, the field-id parameters equal the header enum, and
 a GPU-free entry point (fcfd_prof_nregions) is callable.  No GPU is used."""
 from __future__ import annotations
