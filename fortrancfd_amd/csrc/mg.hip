@@ -2256,8 +2256,11 @@ __global__ void __launch_bounds__(256) k_resid_out(const double *__restrict__ p,
 // before the next plane needs them again.  At the default occupancy target
 // (8 waves per SIMD, <= 64 VGPRs) the compiler issues the neighbour loads of
 // a plane one at a time, each followed by a wait; allowing 5 waves per SIMD
-// lets it issue them together (about 7 % faster at 256^3).
-constexpr int FCG_KC = 8;
+// lets it issue them together (about 7 % faster at 256^3).  Four planes
+// per thread: twice the blocks of eight, which hides more latency than the
+// two boundary planes per column re-read cost (256^3 step 0.25 %, M3 0.6 %
+// faster than eight).
+constexpr int FCG_KC = 4;
 template <typename T, int DIM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 5))) k_fcg_dir(const T *__restrict__ z, T *__restrict__ zo,
                                                  double *__restrict__ p, double *__restrict__ q,
