@@ -357,6 +357,20 @@ __global__ void __launch_bounds__(256) k_smooth_split(float *__restrict__ e, con
     const long id = (long)(m + 1) + S.row(j, k);
     const long cc = color ? S.half : 0, oc = S.half - cc;   // this colour, the other colour
     const bool upd = i < S.nx && j < S.ny;
+    // the dots' operands of this cell and of the pair's other cell, loaded
+    // before the update's store: the other colour's e is not written by this
+    // half-sweep, but the compiler cannot tell the two halves of e apart and
+    // would issue these loads one round trip each after the store
+    const int ip = 2 * m + 1 - s;   // the pair's other cell: final from the previous half-sweep
+    const bool oth = ip < S.nx && j < S.ny;
+    float zo_c = 0.f, f_o = 0.f, e_o = 0.f, zo_o = 0.f;
+    if (DOTS) {   // unconditional, at a valid slot for threads past the grid
+        const long idd = (upd || oth) ? id : S.row(0, 0) + 1;
+        zo_c = D.zo[cc + idd];
+        f_o = f[oc + idd];
+        e_o = e[oc + idd];
+        zo_o = D.zo[oc + idd];
+    }
     float enew = 0.f;
     if (upd) {
         const RT *ro = rho + oc;
@@ -389,13 +403,12 @@ __global__ void __launch_bounds__(256) k_smooth_split(float *__restrict__ e, con
         if (upd) {
             const double rv = (double)f[cc + id];
             rz = rv * (double)enew;
-            rzo = rv * (double)D.zo[cc + id];
+            rzo = rv * (double)zo_c;
         }
-        const int ip = 2 * m + 1 - s;   // the pair's other cell: final from the previous half-sweep
-        if (ip < S.nx && j < S.ny) {
-            const double rv = (double)f[oc + id];
-            rz += rv * (double)e[oc + id];
-            rzo += rv * (double)D.zo[oc + id];
+        if (oth) {
+            const double rv = (double)f_o;
+            rz += rv * (double)e_o;
+            rzo += rv * (double)zo_o;
         }
         rz = block_sum256(rz, sh);
         rzo = block_sum256(rzo, sh);
