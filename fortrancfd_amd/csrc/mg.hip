@@ -506,13 +506,16 @@ struct FusedTile {
     static constexpr int TX = DIM == 3 ? 16 : 32, TY = DIM == 3 ? 8 : 16, TZ = DIM == 3 ? 8 : 1;
 };
 constexpr int FUSED_THREADS = 1024;
-template <int DIM, int NS>
+// PROL: the first launch after a prolongation adds the coarse correction as
+// it stages e (a template argument: a run-time flag put a branch between the
+// staging rounds' loads, one memory round trip each)
+template <int DIM, int NS, bool PROL>
 __global__ void __launch_bounds__(FUSED_THREADS) k_smooth_fused(const float *__restrict__ e, float *__restrict__ eo,
                                                                 const float *__restrict__ f,
                                                                 const float *__restrict__ bx,
                                                                 const float *__restrict__ by,
                                                                 const float *__restrict__ bz, Lay L, Inv2 ih,
-                                                                int color0, Prol<float> P, int prol) {
+                                                                int color0, Prol<float> P) {
     if (L.skipped()) return;
     constexpr int H = 2 * NS, TX = FusedTile<DIM>::TX, TY = FusedTile<DIM>::TY, TZ = FusedTile<DIM>::TZ;
     constexpr int EX = TX + 2 * H, EY = TY + 2 * H, EZ = DIM == 3 ? TZ + 2 * H : 1, NE = EX * EY * EZ;
@@ -538,7 +541,7 @@ __global__ void __launch_bounds__(FUSED_THREADS) k_smooth_fused(const float *__r
         const int ic = clampi(i, -1, L.nx), jc = clampi(j, -1, L.ny), kc = DIM == 3 ? clampi(k, -1, L.nz) : k;
         const long id = L.id(ic, jc, kc);
         float e_ = e[id];
-        if (prol) e_ += P.at(ic, jc, kc);
+        if (PROL) e_ += P.at(ic, jc, kc);
         const float f_ = f[id], x_ = bx[id], y_ = by[id], z_ = DIM == 3 ? bz[id] : 0.f;
         ev[it] = inarr ? e_ : 0.f;
         fv[it] = inarr ? f_ : 0.f;
@@ -1975,12 +1978,17 @@ static int vcycle(fcfd_ctx *c, int l, T *e0, const T *f0, const T *rho0, int nu1
                     const int pm = prol && q == 0;
                     const float *ei = bufs[q & 1];
                     float *eo = bufs[(q + 1) & 1];
-                    if (q < n2)
-                        hipLaunchKernelGGL((k_smooth_fused<DIM, 2>), gf, dim3(FUSED_THREADS), 0, c->stream, ei, eo, f,
-                                           bx, by, bz, L, ih, c0, P0, pm);
-                    else
-                        hipLaunchKernelGGL((k_smooth_fused<DIM, 1>), gf, dim3(FUSED_THREADS), 0, c->stream, ei, eo, f,
-                                           bx, by, bz, L, ih, c0, P0, pm);
+#define FCFD_FUSED(NSW, PR)                                                                                    \
+    hipLaunchKernelGGL((k_smooth_fused<DIM, NSW, PR>), gf, dim3(FUSED_THREADS), 0, c->stream, ei, eo, f, bx, by, bz, \
+                       L, ih, c0, P0)
+                    if (q < n2) {
+                        if (pm) FCFD_FUSED(2, true);
+                        else FCFD_FUSED(2, false);
+                    } else {
+                        if (pm) FCFD_FUSED(1, true);
+                        else FCFD_FUSED(1, false);
+                    }
+#undef FCFD_FUSED
                 }
                 FCFD_CHECK(hipGetLastError());
                 return FCFD_OK;
